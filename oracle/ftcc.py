@@ -1,0 +1,704 @@
+"""numpy FP64 restatement of kelvin's finite-temperature CCSD hot path.
+
+TEST INFRASTRUCTURE ONLY: this is the oracle.  The product package
+``kelvin_amd`` never imports it; tests, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use it as the checker.
+
+Every function cites the reference file:line it restates (paths relative to
+the reference checkout).  The CCSD residual itself lives in the third-party
+package cqcpy (``cc_equations._Stanton``, unversioned, HEAD of
+github.com/awhite862/cqcpy per ``.github/workflows/clone_deps.sh:2``), which is
+absent here; ``stanton_rhs`` restates the published Stanton-Gauss equations
+(J. F. Stanton, J. Gauss, J. D. Watts, R. J. Bartlett, J. Chem. Phys. 94, 4334
+(1991); spin-orbital form of Crawford & Schaefer, Rev. Comp. Chem. 14, 33
+(2000)) in kelvin's block conventions.
+
+How each piece is pinned (tests/test_oracle_*.py):
+  * quadrature rules: exactness of Simpson on cubics, closed forms, the
+    finite-difference identity for the ``d_*`` beta-derivative rules.
+  * Fermi factors / GP0 / dGP0: closed forms and E0 = beta*sum(dGP0)+mu*N0+G0.
+  * UEG integrals: antisymmetry, pair-exchange symmetry, momentum conservation,
+    and the end-to-end goldens below.
+  * integrators: the constant-source closed form T(tau) = Tbar (1-e^{-tau D})/D
+    (up to Simpson error) and agreement of the literal O(ng^2) and scan forms.
+  * ``stanton_rhs``: brute-force Fock-space projection
+    <Phi_mu| e^{-T} H e^{T} |Phi_0> on random non-Hermitian Hamiltonians
+    (oracle/fock_space.py), then end to end.
+  * end to end: the reference's golden UEG grand potentials
+    (kelvin/tests/test_ft_ccsd.py:23 with :119-133, Omega_cc=-0.010750238811
+    at 1e-9) and the per-iteration energies printed in
+    examples/ueg_ft_cc_compare.out:10-26 and examples/ueg_ft_ccsd_ESN.out:9-25.
+"""
+import logging
+import numpy
+
+einsum = numpy.einsum
+
+
+def _es(spec, *ops):
+    return numpy.einsum(spec, *ops, optimize=True)
+
+
+# ---------------------------------------------------------------------------
+# Fermi-Dirac factors and free-fermion grand potential (cqcpy ft_utils; call
+# sites kelvin/ccsd.py:627-629,637-638, kelvin/cc_utils.py:571-574).  Closed
+# forms, written so that tiny occupations keep their relative precision.
+# ---------------------------------------------------------------------------
+def ff(beta, en, mu):
+    """Occupied Fermi factor 1/(1+exp(beta(e-mu)))."""
+    x = beta * (numpy.asarray(en, dtype=float) - mu)
+    out = numpy.empty_like(x)
+    p = x > 0
+    ex = numpy.exp(-x[p])
+    out[p] = ex / (1.0 + ex)
+    out[~p] = 1.0 / (1.0 + numpy.exp(x[~p]))
+    return out
+
+
+def ffv(beta, en, mu):
+    """Virtual Fermi factor 1 - ff = 1/(1+exp(-beta(e-mu)))."""
+    return ff(beta, -(numpy.asarray(en, dtype=float) - mu), 0.0)
+
+
+def GP0(beta, en, mu):
+    """Per-orbital free-fermion grand potential -(1/beta) ln(1+e^{-beta(e-mu)})."""
+    x = beta * (numpy.asarray(en, dtype=float) - mu)
+    # ln(1+e^{-x}) = max(-x,0) + log1p(e^{-|x|})
+    return -(numpy.maximum(-x, 0.0) + numpy.log1p(numpy.exp(-numpy.abs(x)))) / beta
+
+
+def dGP0(beta, en, mu):
+    """d GP0 / d beta per orbital (used by kelvin/ccsd.py:177-180)."""
+    e = numpy.asarray(en, dtype=float)
+    return ((e - mu) * ff(beta, e, mu) - GP0(beta, e, mu)) / beta
+
+
+def D1(ev, eo):
+    """D1[a,i] = e_a - e_i (cqcpy utils.D1, call site kelvin/ccsd.py:633)."""
+    return ev[:, None] - eo[None, :]
+
+
+def D2(ev, eo):
+    """D2[a,b,i,j] = e_a + e_b - e_i - e_j (cqcpy utils.D2, kelvin/ccsd.py:634)."""
+    return (ev[:, None, None, None] + ev[None, :, None, None]
+            - eo[None, None, :, None] - eo[None, None, None, :])
+
+
+# ---------------------------------------------------------------------------
+# Quadrature rules: kelvin/quadrature.py:17-256
+# ---------------------------------------------------------------------------
+def get_G(ng, delta):
+    """kelvin/quadrature.py:32-41 (Simpson weights for int_0^{t_y})."""
+    G = numpy.zeros((ng, ng))
+    if ng > 1:
+        G[1, 0] = G[1, 1] = 0.5 * delta
+    for y in range(2, ng):
+        G[y] = G[y - 2]
+        G[y, y - 2] += delta / 3.0
+        G[y, y - 1] += 4.0 * delta / 3.0
+        G[y, y] += delta / 3.0
+    return G
+
+
+def get_gint(ng, delta):
+    """kelvin/quadrature.py:44-60 (Simpson weights for int_0^beta)."""
+    g = numpy.zeros(ng)
+    if ng % 2 == 0:
+        g[0] += 0.5 * delta
+        g[1] += 0.5 * delta
+        for y in range(3, ng, 2):
+            g[y - 2] += delta / 3.0
+            g[y - 1] += 4.0 * delta / 3.0
+            g[y] += delta / 3.0
+    else:
+        for y in range(2, ng, 2):
+            g[y - 2] += delta / 3.0
+            g[y - 1] += 4.0 * delta / 3.0
+            g[y] += delta / 3.0
+    return g
+
+
+def get_G_midpoint(ng, delta):
+    """kelvin/quadrature.py:17-23."""
+    G = numpy.zeros((ng, ng))
+    G[0, 0] = delta
+    for y in range(1, ng):
+        G[y] = G[y - 1]
+        G[y, y] += delta
+    return G
+
+
+def get_gL(ng, delta):
+    """kelvin/quadrature.py:63-67."""
+    g = numpy.zeros(ng)
+    g[:ng - 1] = delta
+    return g
+
+
+def get_GL(ng, delta):
+    """kelvin/quadrature.py:70-75."""
+    G = numpy.zeros((ng, ng))
+    for i in range(ng):
+        for j in range(i - 1):
+            G[i, j] = delta
+    return G
+
+
+def _mapped(ng, beta, delta, si, jac):
+    g = beta * get_gint(ng, delta) * jac
+    G = get_G(ng, delta)
+    for i in range(ng):
+        G[i] = beta * G[i] * jac
+    return g, G
+
+
+def ft_quad(ng, beta, quad):
+    """kelvin/quadrature.py:215-235 with the rules at :78-212."""
+    if quad == 'lin':                                   # :102-107
+        delta = beta / (ng - 1.0)
+        ti = numpy.arange(ng) * delta
+        return ti, get_gint(ng, delta), get_G(ng, delta)
+    if quad == 'ln':                                    # :119-129
+        e = numpy.exp(1)
+        delta = (e - 1.0) / (ng - 1.0)
+        si = numpy.arange(ng) * delta + 1.0
+        g, G = _mapped(ng, beta, delta, si, 1.0 / si)
+        return beta * numpy.log(si), g, G
+    if quad == 'sin':                                   # :144-153
+        delta = numpy.pi / (ng - 1.0)
+        si = numpy.arange(ng) * delta - numpy.pi / 2
+        g, G = _mapped(ng, beta, delta, si, numpy.cos(si) / 2.0)
+        return beta * (numpy.sin(si) + 1.0) / 2.0, g, G
+    if quad == 'exp':                                   # :166-176
+        delta = numpy.log(2.0) / (ng - 1.0)
+        si = numpy.arange(ng) * delta
+        g, G = _mapped(ng, beta, delta, si, numpy.exp(si))
+        return beta * (numpy.exp(si) - 1.0), g, G
+    if quad in ('quad', 'cub', 'quar'):                 # :190-199
+        p = {'quad': 2, 'cub': 3, 'quar': 4}[quad]
+        delta = 1.0 / (ng - 1.0)
+        si = numpy.arange(ng) * delta
+        jac = (numpy.power(si, p - 1) * float(p) + 1.0) / 2.0
+        g, G = _mapped(ng, beta, delta, si, jac)
+        return beta * (numpy.power(si, p) + si) / 2.0, g, G
+    if quad == 'mid':                                   # :94-99
+        delta = beta / ng
+        ti = numpy.arange(ng) * delta + delta / 2
+        return ti, numpy.full(ng, delta), get_G_midpoint(ng, delta)
+    if quad == 'L':                                     # :78-83
+        delta = beta / (ng - 1.0)
+        ti = numpy.arange(ng) * delta
+        return ti, get_gL(ng, delta), get_GL(ng, delta)
+    raise Exception("Unrecognized quadrature rule: {}".format(quad))
+
+
+def d_ft_quad(ng, beta, quad):
+    """kelvin/quadrature.py:238-256: d/dbeta of (g, G) at fixed mapped grid."""
+    if quad == 'lin':                                   # :110-116
+        delta = beta / (ng - 1.0)
+        return get_gint(ng, delta / beta), get_G(ng, delta / beta)
+    if quad == 'L':                                     # :86-91
+        delta = beta / (ng - 1.0)
+        return get_gL(ng, delta / beta), get_GL(ng, delta / beta)
+    if quad in ('ln', 'sin', 'exp', 'quad', 'cub', 'quar'):
+        # the mapped rules are linear in beta: d/dbeta = value at beta=1
+        ti, g, G = ft_quad(ng, 1.0, quad)
+        return g, G
+    raise Exception("Unrecognized quadrature rule: {}".format(quad))
+
+
+# ---------------------------------------------------------------------------
+# exp(-tau D) integrators: kelvin/quadrature.py:292-317 (literal O(ng^2) form)
+# ---------------------------------------------------------------------------
+def int_tbar1(ng, t1bar, ti, D1, G):
+    """kelvin/quadrature.py:292-303."""
+    t1_out = numpy.zeros(t1bar.shape, dtype=t1bar.dtype)
+    dt = numpy.zeros(ng)
+    for y in range(ng):
+        for i in range(y):
+            dt[i] = ti[i] - ti[y]
+        gtemp = numpy.exp(dt[:, None, None] * D1[None, :, :])
+        t1_out[y] = einsum('x,xai->ai', G[y], gtemp * t1bar)
+    return t1_out
+
+
+def int_tbar2(ng, t2bar, ti, D2, G):
+    """kelvin/quadrature.py:306-317."""
+    t2_out = numpy.zeros(t2bar.shape, dtype=t2bar.dtype)
+    dt = numpy.zeros(ng)
+    for y in range(ng):
+        for i in range(y):
+            dt[i] = ti[i] - ti[y]
+        gtemp = numpy.exp(dt[:, None, None, None, None] * D2[None])
+        t2_out[y] = einsum('x,xabij->abij', G[y], gtemp * t2bar)
+    return t2_out
+
+
+def int_tbar_scan(ng, tbar, ti, D, G):
+    """Same result as int_tbar1/2, one slice at a time (memory O(ng*size)).
+
+    Mirrors the reference's dt convention exactly: row y only ever sets
+    dt[x] for x < y, so for x >= y the propagator is exp(0) = 1 and any
+    weight G[y,x] on or above the diagonal multiplies Tbar[x] unpropagated
+    (kelvin/quadrature.py:296-299).
+    """
+    out = numpy.zeros_like(tbar)
+    dt = numpy.zeros(ng)
+    for y in range(ng):
+        for i in range(y):
+            dt[i] = ti[i] - ti[y]
+        acc = numpy.zeros_like(tbar[0])
+        for x in range(ng):
+            if G[y, x] != 0.0:
+                acc += G[y, x] * numpy.exp(dt[x] * D) * tbar[x]
+        out[y] = acc
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Fermi-weighted integrals: kelvin/cc_utils.py:569-602 and :825-858
+# ---------------------------------------------------------------------------
+class Blocks(dict):
+    """Attribute bag standing in for cqcpy one_e_blocks / two_e_blocks."""
+    __getattr__ = dict.__getitem__
+
+
+def _weigh(eri, idx, w):
+    (p, q, r, s), (wp, wq, wr, ws) = idx, w
+    return (eri[numpy.ix_(p, q, r, s)] * wp[:, None, None, None]
+            * wq[None, :, None, None] * wr[None, None, :, None]
+            * ws[None, None, None, :])
+
+
+def ft_integrals(fock, eri, en, beta, mu):
+    """kelvin/cc_utils.py:569-602 (fock = sys.g_fock_tot(), eri = g_aint_tot())."""
+    n = en.shape[0]
+    iall = numpy.arange(n)
+    return ft_active_integrals(fock, eri, en, ff(beta, en, mu),
+                               ffv(beta, en, mu), iall, iall)
+
+
+def ft_active_integrals(fock, eri, en, focc, fvir, iocc, ivir):
+    """kelvin/cc_utils.py:825-858 (focc/fvir already restricted to iocc/ivir)."""
+    fmo = fock - numpy.diag(en)                                  # :577-578
+    so = numpy.sqrt(numpy.asarray(focc, dtype=float))
+    sv = numpy.sqrt(numpy.asarray(fvir, dtype=float))
+    o, v = numpy.asarray(iocc), numpy.asarray(ivir)
+    F = Blocks(
+        oo=fmo[numpy.ix_(o, o)] * so[:, None] * so[None, :],
+        ov=fmo[numpy.ix_(o, v)] * so[:, None] * sv[None, :],
+        vo=fmo[numpy.ix_(v, o)] * sv[:, None] * so[None, :],
+        vv=fmo[numpy.ix_(v, v)] * sv[:, None] * sv[None, :])
+    W = {'o': (o, so), 'v': (v, sv)}
+    I = Blocks()
+    for name in ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo',
+                 'oovv', 'vooo', 'ooov', 'oooo'):
+        idx = tuple(W[c][0] for c in name)
+        w = tuple(W[c][1] for c in name)
+        I[name] = _weigh(eri, idx, w)
+    return F, I
+
+
+# ---------------------------------------------------------------------------
+# Stanton-Gauss CCSD right-hand side (cqcpy cc_equations._Stanton, called at
+# kelvin/ft_cc_equations.py:105-107 with fac=-1).  All indices run over the
+# occupied (m,n,i,j) or virtual (a,b,e,f) lists, which at finite T are both
+# the full orbital set.  Layouts: t1[a,i], t2[a,b,i,j].
+#
+# Integral map (antisymmetry <pq||rs> = -<qp||rs> = -<pq||sr> = <qp||sr>):
+#   <ma||fe> = -vovv[a,m,f,e]   <mb||ej> = -vovo[b,m,e,j]   <na||if> = vovo[a,n,f,i]
+#   <mn||ej> = -ooov[m,n,j,e]   <nm||ei> = -ooov[n,m,i,e]   <mb||ij> = -vooo[b,m,i,j]
+# ---------------------------------------------------------------------------
+def stanton_rhs(F, I, t1, t2):
+    """Return (rhs1, rhs2) = (f_ai + R1(T), <ab||ij> + R2(T)).
+
+    kelvin's residual is Tbar1 = -rhs1, Tbar2 = -rhs2
+    (kelvin/ft_cc_equations.py:101-107: T1new = -F.vo, then _Stanton with
+    fac=-1 adds -R1).  The Fock blocks are used in full (no (1-delta) mask):
+    the caller has already removed diag(en) (kelvin/cc_utils.py:577-578).
+    """
+    es = _es
+    vovv, ooov, oovv = I.vovv, I.ooov, I.oovv
+    tt = es('ai,bj->abij', t1, t1)
+    tt = tt - tt.transpose(1, 0, 2, 3)
+    tau = t2 + tt
+    taut = t2 + 0.5 * tt
+
+    # one-particle intermediates
+    Fme = F.ov + es('fn,mnef->me', t1, oovv)
+    Fae = (F.vv - 0.5 * es('me,am->ae', F.ov, t1)
+           - es('fm,amfe->ae', t1, vovv)
+           - 0.5 * es('afmn,mnef->ae', taut, oovv))
+    Fmi = (F.oo + 0.5 * es('ei,me->mi', t1, F.ov)
+           + es('en,mnie->mi', t1, ooov)
+           + 0.5 * es('efin,mnef->mi', taut, oovv))
+
+    # two-particle intermediates
+    tmp = es('ej,mnie->mnij', t1, ooov)
+    Wmnij = (I.oooo + tmp - tmp.transpose(0, 1, 3, 2)
+             + 0.25 * es('efij,mnef->mnij', tau, oovv))
+    tmp = es('bm,amef->abef', t1, vovv)
+    Wabef = (I.vvvv - tmp + tmp.transpose(1, 0, 2, 3)
+             + 0.25 * es('abmn,mnef->abef', tau, oovv))
+    ovvo = -I.vovo.transpose(1, 0, 2, 3)             # <mb||ej> as [m,b,e,j]
+    Wmbej = (ovvo - es('fj,bmef->mbej', t1, vovv)
+             + es('bn,mnje->mbej', t1, ooov)
+             - 0.5 * es('fbjn,mnef->mbej', t2, oovv)
+             - es('fj,bn,mnef->mbej', t1, t1, oovv))
+
+    # singles
+    r1 = (F.vo + es('ei,ae->ai', t1, Fae) - es('am,mi->ai', t1, Fmi)
+          + es('aeim,me->ai', t2, Fme)
+          - es('fn,anfi->ai', t1, I.vovo)
+          + 0.5 * es('efim,amef->ai', t2, vovv)
+          + 0.5 * es('aemn,nmie->ai', t2, ooov))
+
+    # doubles
+    r2 = I.vvoo.copy()
+    Fbe = Fae - 0.5 * es('bm,me->be', t1, Fme)
+    tmp = es('aeij,be->abij', t2, Fbe)
+    r2 += tmp - tmp.transpose(1, 0, 2, 3)
+    Fmj = Fmi + 0.5 * es('ej,me->mj', t1, Fme)
+    tmp = es('abim,mj->abij', t2, Fmj)
+    r2 -= tmp - tmp.transpose(0, 1, 3, 2)
+    r2 += 0.5 * es('abmn,mnij->abij', tau, Wmnij)
+    r2 += 0.5 * es('efij,abef->abij', tau, Wabef)
+    tmp = (es('aeim,mbej->abij', t2, Wmbej)
+           - es('ei,am,mbej->abij', t1, t1, ovvo))
+    r2 += (tmp - tmp.transpose(1, 0, 2, 3) - tmp.transpose(0, 1, 3, 2)
+           + tmp.transpose(1, 0, 3, 2))
+    tmp = es('ei,abej->abij', t1, I.vvvo)
+    r2 += tmp - tmp.transpose(0, 1, 3, 2)
+    tmp = es('am,bmij->abij', t1, I.vooo)
+    r2 += tmp - tmp.transpose(1, 0, 2, 3)
+    return r1, r2
+
+
+def ccsd_stanton(F, I, T1old, T2old, D1, D2, ti, ng, G):
+    """kelvin/ft_cc_equations.py:96-113."""
+    T1new = numpy.empty_like(T1old)
+    T2new = numpy.empty_like(T2old)
+    for y in range(ng):
+        r1, r2 = stanton_rhs(F, I, T1old[y], T2old[y])
+        T1new[y] = -r1
+        T2new[y] = -r2
+    return int_tbar1(ng, T1new, ti, D1, G), int_tbar2(ng, T2new, ti, D2, G)
+
+
+def residual_tbar(F, I, T1, T2):
+    """Pre-integration residual at every slice: (Tbar1, Tbar2)."""
+    ng = T1.shape[0]
+    T1b = numpy.empty_like(T1)
+    T2b = numpy.empty_like(T2)
+    for y in range(ng):
+        r1, r2 = stanton_rhs(F, I, T1[y], T2[y])
+        T1b[y], T2b[y] = -r1, -r2
+    return T1b, T2b
+
+
+# ---------------------------------------------------------------------------
+# Grand potential: kelvin/ft_cc_energy.py:7-32
+# ---------------------------------------------------------------------------
+def ft_cc_energy(T1, T2, f, eri, g, beta, Qterm=True):
+    t2_temp = 0.25 * T2
+    if Qterm:
+        t2_temp = t2_temp + 0.5 * einsum('yai,ybj->yabij', T1, T1)
+    t1_n = einsum('y,yai->ai', g, T1)
+    t2_n = einsum('y,yabij->abij', g, t2_temp)
+    Es1 = einsum('ai,ia->', t1_n, f)
+    Es2 = einsum('abij,ijab->', t2_n, eri)
+    return (Es1 + Es2) / beta
+
+
+# ---------------------------------------------------------------------------
+# Damped fixed point: kelvin/cc_utils.py:111-173
+# ---------------------------------------------------------------------------
+def ft_cc_iter(T1old, T2old, F, I, D1, D2, g, G, beta, ng, ti, conv,
+               history=None):
+    """Return (E, T1, T2, converged, niter); history collects (E, res)."""
+    ethresh, tthresh = conv["econv"], conv["tconv"]
+    max_iter, alpha = conv["max_iter"], conv["damp"]
+    converged = False
+    i = 0
+    Eold = 888888888.888888888
+    nl1 = numpy.linalg.norm(T1old) + 0.1
+    nl2 = numpy.linalg.norm(T2old) + 0.1
+    while i < max_iter and not converged:
+        T1, T2 = ccsd_stanton(F, I, T1old, T2old, D1, D2, ti, ng, G)
+        res1 = numpy.linalg.norm(T1 - T1old) / nl1
+        res2 = numpy.linalg.norm(T2 - T2old) / nl2
+        T1old = alpha * T1old + (1.0 - alpha) * T1
+        T2old = alpha * T2old + (1.0 - alpha) * T2
+        nl1 = numpy.linalg.norm(T1old) + 0.1
+        nl2 = numpy.linalg.norm(T2old) + 0.1
+        E = ft_cc_energy(T1old, T2old, F.ov, I.oovv, g, beta)
+        if history is not None:
+            history.append((E, res1 + res2))
+        logging.info(' %2d  %.10f   %.4E' % (i + 1, E, res1 + res2))
+        i += 1
+        if numpy.abs(E - Eold) < ethresh and res1 + res2 < tthresh:
+            converged = True
+        Eold = E
+    return Eold, T1old, T2old, converged, i
+
+
+def ft_ccsd(sys, T, mu, ngrid=10, damp=0.0, max_iter=40, econv=1e-8,
+            tconv=None, quad='lin', athresh=0.0, history=None):
+    """kelvin/ccsd.py:43-76 (construction) + :595-712 (_ft_ccsd, 'g' path).
+
+    Returns a dict with Omega (total), Omega_cc, G0, G1, mp2, T1, T2, niter.
+    """
+    tconv = tconv if tconv is not None else 1000.0 * econv       # ccsd.py:54
+    beta = 1.0 / T
+    ti, g, G = ft_quad(ngrid, beta, quad)
+    ng = ngrid
+    en = sys.g_energies_tot()
+    fo = ff(beta, en, mu)
+    fv = ffv(beta, en, mu)
+    n = en.shape[0]
+    G0 = GP0(beta, en, mu).sum() + sys.const_energy()            # :636-638
+    G1 = sys.get_mp1()                                            # :639
+    fock, eri = sys.g_fock_tot(), sys.g_aint_tot()
+    if athresh > 0.0:                                             # :642-660
+        iocc = [i for i, x in enumerate(fo) if x > athresh]
+        ivir = [i for i, x in enumerate(fv) if x > athresh]
+        F, I = ft_active_integrals(fock, eri, en, fo[iocc], fv[ivir],
+                                   iocc, ivir)
+        d1, d2 = D1(en, en), D2(en, en)
+        d1 = d1[numpy.ix_(ivir, iocc)]
+        d2 = d2[numpy.ix_(ivir, ivir, iocc, iocc)]
+    else:
+        F, I = ft_integrals(fock, eri, en, beta, mu)
+        d1, d2 = D1(en, en), D2(en, en)
+    nv, no = F.vo.shape
+    # MP2-level guess: kelvin/ccsd.py:680-688
+    T1old = -numpy.broadcast_to(F.vo, (ng, nv, no)).copy()
+    T2old = -numpy.broadcast_to(I.vvoo, (ng, nv, nv, no, no)).copy()
+    T1old = int_tbar1(ng, T1old, ti, d1, G)
+    T2old = int_tbar2(ng, T2old, ti, d2, G)
+    E2 = ft_cc_energy(T1old, T2old, F.ov, I.oovv, g, beta, Qterm=False)
+    conv = {"econv": econv, "tconv": tconv, "max_iter": max_iter,
+            "damp": damp}
+    Ecc, T1, T2, converged, niter = ft_cc_iter(
+        T1old, T2old, F, I, d1, d2, g, G, beta, ng, ti, conv, history)
+    return dict(Omega=Ecc + G0 + G1, Omega_cc=Ecc, G0=G0, G1=G1, mp2=E2,
+                T1=T1, T2=T2, niter=niter, converged=converged, n=n)
+
+
+# ---------------------------------------------------------------------------
+# Systems (the reference's integral providers).
+# ---------------------------------------------------------------------------
+class UEGBasis:
+    """kelvin/ueg_utils.py:21-54 (literal loops, same stable ordering)."""
+
+    def __init__(self, L, cutoff, norb=None):
+        self.L = L
+        kmax = numpy.sqrt(2 * cutoff)
+        imax = int(numpy.ceil(L * kmax / (2 * numpy.pi)) + 0.1)
+        basis = []
+        for i in range(-imax, imax):
+            for j in range(-imax, imax):
+                for k in range(-imax, imax):
+                    k2 = 4.0 * numpy.pi * numpy.pi / (L * L) * (i*i + j*j + k*k)
+                    if k2 < 2.0 * cutoff:
+                        basis.append((i, j, k))
+        basis.sort(key=lambda x: x[0]*x[0] + x[1]*x[1] + x[2]*x[2])
+        if norb is not None:
+            basis = basis[:norb]
+        self.basis = basis
+        c = 2.0 * numpy.pi / L
+        self.ks = [numpy.array((c*x[0], c*x[1], c*x[2])) for x in basis]
+        self.Es = [0.5 * float(k @ k) for k in self.ks]
+
+    def get_nbsf(self):
+        return len(self.basis)
+
+    def build_g2e_matrix(self, anti=True):
+        """kelvin/ueg_utils.py:109-138, iterating over (p,q,r) and solving
+        momentum conservation for s instead of testing all s."""
+        nb = self.get_nbsf()
+        n = 2 * nb
+        lookup = {tuple(x): i for i, x in enumerate(self.basis)}
+        aaa = 4.0 * numpy.pi / self.L ** 3
+        c2 = (2.0 * numpy.pi / self.L) ** 2
+        V = numpy.zeros((n, n, n, n))
+        for p in range(n):
+            xp, sp = self.basis[p % nb], p // nb
+            for q in range(n):
+                xq, sq = self.basis[q % nb], q // nb
+                for r in range(n):
+                    xr, sr = self.basis[r % nb], r // nb
+                    if p == r or sp != sr:
+                        continue
+                    xs = tuple(xp[d] + xq[d] - xr[d] for d in range(3))
+                    s0 = lookup.get(xs)
+                    if s0 is None:
+                        continue
+                    s = s0 + nb * sq
+                    if q == s:
+                        continue
+                    dk2 = c2 * sum((xp[d] - xr[d]) ** 2 for d in range(3))
+                    V[p, q, r, s] = aaa / dk2
+        if anti:
+            return V - V.transpose(0, 1, 3, 2)
+        return V
+
+
+class UEGSystem:
+    """kelvin/ueg_system.py:31-65 and the generalised ('g') methods used on
+    the hot path: g_energies_tot :266-267, g_fock_tot :400-419,
+    g_aint_tot :650-651, get_mp1 :126-131, const_energy :86-90."""
+
+    def __init__(self, T, L, Emax, mu, norb=None):
+        self.T, self.L, self.mu = T, L, mu
+        self.basis = UEGBasis(L, Emax, norb=norb)
+        beta = 1.0 / T if T > 0.0 else 1.0e20
+        self.N = ff(beta, self.g_energies_tot(), mu).sum()
+        self.den = self.N / L ** 3
+        self.rs = (3 / (4.0 * numpy.pi * self.den)) ** (1.0 / 3.0)
+        self.Ef = 0.5 * (3.0 * numpy.pi ** 2 * self.den) ** (2.0 / 3.0)
+        self.Tf = T / self.Ef
+        self._eri = None
+
+    def verify(self, T, mu):
+        return T == self.T and mu == self.mu
+
+    def has_u(self):
+        return False
+
+    def const_energy(self):
+        return 0.0
+
+    def g_energies_tot(self):
+        e = numpy.asarray(self.basis.Es)
+        return numpy.hstack((e, e))
+
+    def g_aint_tot(self):
+        if self._eri is None:
+            self._eri = self.basis.build_g2e_matrix()
+        return self._eri
+
+    def g_fock_tot(self):
+        d = self.g_energies_tot()
+        fo = ff(1.0 / self.T, d, self.mu)
+        return numpy.diag(d) + einsum('prqr,r->pq', self.g_aint_tot(), fo)
+
+    def get_mp1(self):
+        fo = ff(1.0 / self.T, self.g_energies_tot(), self.mu)
+        return 0.5 * einsum('ijij,i,j->', self.g_aint_tot(), fo, fo)
+
+
+class DFModelSystem:
+    """Molecule-like model with density-fitted ERIs.
+
+    norb spatial orbitals, energies ~U[-1,1] (sorted) drawn first, then
+    B[Q,p,q] (Q=2*norb) with upper triangle ~N(0, sigma^2), mirrored.
+    (pq|rs) = sum_Q B_Qpq B_Qrs; spin-blocked (alpha block, then beta) and
+    antisymmetrised <pq||rs> = (pr|qs) d_sp,sr d_sq,ss - (ps|qr) d_sp,ss d_sq,sr.
+    The Fock matrix is diag(energies), so every Fermi-weighted Fock block is
+    zero; E1 = 1/2 sum_ij <ij||ij> fo_i fo_j (the UEG-'g' convention).
+    """
+
+    def __init__(self, T, mu, norb=8, seed=0, sigma=0.03):
+        rng = numpy.random.default_rng(seed)
+        self.T, self.mu = T, mu
+        e = numpy.sort(rng.uniform(-1.0, 1.0, norb))
+        B = numpy.zeros((2 * norb, norb, norb))
+        iu = numpy.triu_indices(norb)
+        for Q in range(2 * norb):
+            B[Q][iu] = rng.normal(0.0, sigma, len(iu[0]))
+            B[Q] = B[Q] + numpy.triu(B[Q], 1).T
+        self.e_spatial, self.B = e, B
+        chem = einsum('Qpq,Qrs->pqrs', B, B)
+        n = 2 * norb
+        V = numpy.zeros((n, n, n, n))
+        for s1 in range(2):
+            for s2 in range(2):
+                o1, o2 = s1 * norb, s2 * norb
+                # <pq|rs> = (pr|qs) with p,r spin s1 and q,s spin s2
+                V[o1:o1+norb, o2:o2+norb, o1:o1+norb, o2:o2+norb] = \
+                    chem.transpose(0, 2, 1, 3)
+        self._eri = V - V.transpose(0, 1, 3, 2)
+        self._en = numpy.hstack((e, e))
+
+    def verify(self, T, mu):
+        return T == self.T and mu == self.mu
+
+    def has_u(self):
+        return False
+
+    def const_energy(self):
+        return 0.0
+
+    def g_energies_tot(self):
+        return self._en.copy()
+
+    def g_fock_tot(self):
+        return numpy.diag(self._en)
+
+    def g_aint_tot(self):
+        return self._eri
+
+    def get_mp1(self):
+        fo = ff(1.0 / self.T, self._en, self.mu)
+        return 0.5 * einsum('ijij,i,j->', self._eri, fo, fo)
+
+
+class GenericSystem:
+    """Explicit (h, <pq||rs>, reference energies) system for brute-force
+    checks.  Fock = h + sum_r <pr||qr> fo_r; E1 = <H - H0>_0 =
+    sum_p (h_pp - e_p) fo_p + 1/2 sum_pq <pq||pq> fo_p fo_q (the content of
+    kelvin/scf_system.py:54-68 with ft_mp.mp1, kelvin/ft_mp.py:22-29)."""
+
+    def __init__(self, T, mu, h, eri, en):
+        self.T, self.mu, self.h, self._eri, self._en = T, mu, h, eri, en
+
+    def verify(self, T, mu):
+        return T == self.T and mu == self.mu
+
+    def has_u(self):
+        return False
+
+    def const_energy(self):
+        return 0.0
+
+    def g_energies_tot(self):
+        return self._en.copy()
+
+    def _fo(self):
+        return ff(1.0 / self.T, self._en, self.mu)
+
+    def g_fock_tot(self):
+        return self.h + einsum('prqr,r->pq', self._eri, self._fo())
+
+    def g_aint_tot(self):
+        return self._eri
+
+    def get_mp1(self):
+        fo = self._fo()
+        return (numpy.dot(numpy.diag(self.h) - self._en, fo)
+                + 0.5 * einsum('ijij,i,j->', self._eri, fo, fo))
+
+
+def ueg_params(nk, rs, N, t_over_tf):
+    """Box length, temperature and Fermi energy for a UEG with N electrons at
+    Wigner-Seitz radius rs (L from kelvin/tests/test_ueg.py:8-10, E_F from
+    kelvin/ueg_system.py:57-62)."""
+    L = rs * (4.0 * numpy.pi * N / 3.0) ** (1.0 / 3.0)
+    Ef = 0.5 * (3.0 * numpy.pi ** 2 * N / L ** 3) ** (2.0 / 3.0)
+    return L, t_over_tf * Ef, Ef
+
+
+def solve_mu(en, beta, N, lo=None, hi=None, tol=1e-14):
+    """Bisection for sum_p ff(beta, e_p, mu) = N (non-interacting)."""
+    lo = en.min() - 50.0 / beta if lo is None else lo
+    hi = en.max() + 50.0 / beta if hi is None else hi
+    for _ in range(400):
+        mid = 0.5 * (lo + hi)
+        if ff(beta, en, mid).sum() > N:
+            hi = mid
+        else:
+            lo = mid
+        if hi - lo < tol:
+            break
+    return 0.5 * (lo + hi)

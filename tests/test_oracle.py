@@ -1,0 +1,234 @@
+"""Pins the CPU oracle (oracle/ftcc.py) before anything is compared with it.
+
+Each check is chosen so that a plausible slip (a dropped term, a wrong sign,
+a transposed index, a wrong weight) fails one of them:
+  * the residual against a brute-force Fock-space projection of
+    e^{-T} H e^{T} (random, non-Hermitian Hamiltonians, o != v);
+  * the reference's golden grand potentials and per-iteration energies;
+  * FT-CCSD = FT-FCI for two spin orbitals (kelvin/tests/test_ft_ccsd.py:26-47);
+  * closed forms for the quadrature, Fermi factors and integrators.
+"""
+import json
+import os
+
+import numpy
+import pytest
+
+from oracle import ftcc
+from oracle import fock_space as fs
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_goldens.json")))
+
+
+def _blocks(f, eri, no):
+    n = f.shape[0]
+    sl = {'o': slice(0, no), 'v': slice(no, n)}
+    F = ftcc.Blocks(oo=f[sl['o'], sl['o']], ov=f[sl['o'], sl['v']],
+                    vo=f[sl['v'], sl['o']], vv=f[sl['v'], sl['v']])
+    I = ftcc.Blocks({k: eri[sl[k[0]], sl[k[1]], sl[k[2]], sl[k[3]]]
+                     for k in ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo',
+                               'oovv', 'vooo', 'ooov', 'oooo')})
+    return F, I
+
+
+def random_amplitudes(nv, no, rng, scale=0.2):
+    t1 = rng.normal(size=(nv, no)) * scale
+    t2 = rng.normal(size=(nv, nv, no, no)) * scale
+    t2 = t2 - t2.transpose(1, 0, 2, 3)
+    t2 = t2 - t2.transpose(0, 1, 3, 2)
+    return t1, t2
+
+
+@pytest.mark.parametrize("no,nv,herm", [(2, 3, True), (3, 3, False),
+                                        (2, 4, False), (3, 2, False)])
+def test_stanton_matches_bruteforce_projection(no, nv, herm):
+    rng = numpy.random.default_rng(10 * no + nv)
+    n = no + nv
+    f = rng.normal(size=(n, n)) * 0.3
+    if herm:
+        f = f + f.T
+    eri = fs.random_antisym(n, rng, hermitian=herm) * 0.1
+    t1, t2 = random_amplitudes(nv, no, rng)
+    F, I = _blocks(f, eri, no)
+    r1, r2 = ftcc.stanton_rhs(F, I, t1, t2)
+    b1, b2 = fs.cc_projection(f, eri, t1, t2, no)
+    assert numpy.abs(r1 - b1).max() < 1e-13
+    assert numpy.abs(r2 - b2).max() < 1e-13
+
+
+def test_planted_mistake_is_caught():
+    """A sign flip in one Wmbej term must break the brute-force agreement."""
+    rng = numpy.random.default_rng(7)
+    no, nv = 2, 3
+    n = no + nv
+    f = rng.normal(size=(n, n)) * 0.3
+    eri = fs.random_antisym(n, rng, hermitian=False) * 0.1
+    t1, t2 = random_amplitudes(nv, no, rng)
+    F, I = _blocks(f, eri, no)
+    I_bad = ftcc.Blocks(I)
+    I_bad['vovo'] = I.vovo * 1.001
+    r1, r2 = ftcc.stanton_rhs(F, I_bad, t1, t2)
+    b1, b2 = fs.cc_projection(f, eri, t1, t2, no)
+    assert numpy.abs(r2 - b2).max() > 1e-6
+
+
+def _ueg(g):
+    s = g["system"]
+    L = 2 * numpy.pi if s["L"] == "2*pi" else s["L"]
+    return ftcc.UEGSystem(s["T"], L, s["Emax"], s["mu"], norb=s["norb"]), s
+
+
+def test_golden_G1_omega_cc():
+    g = GOLD["G1"]
+    ueg, s = _ueg(g)
+    r = ftcc.ft_ccsd(ueg, s["T"], s["mu"], **g["ccsd"])
+    assert abs(r["Omega_cc"] - g["Omega_cc"]) < g["tol"]
+
+
+def test_golden_G4_iterations():
+    g = GOLD["G4"]
+    ueg, s = _ueg(g)
+    assert abs(ueg.N - g["N0"]) < 1e-10
+    hist = []
+    r = ftcc.ft_ccsd(ueg, s["T"], s["mu"], history=hist, **g["ccsd"])
+    assert abs(r["mp2"] - g["mp2"]) < 6e-11
+    for (E, _), ref in zip(hist, g["iter_energies"]):
+        assert abs(E - ref) < 6e-11
+    assert abs(r["Omega"] - g["Omega"]) < 6e-11
+    assert abs(r["Omega_cc"] - g["Omega_cc"]) < 6e-12
+
+
+def test_golden_G3_first_iterations():
+    g = GOLD["G3"]
+    ueg, s = _ueg(g)
+    assert abs(ueg.N - g["N0"]) < 1e-10
+    hist = []
+    cc = dict(g["ccsd"], max_iter=2)
+    r = ftcc.ft_ccsd(ueg, s["T"], s["mu"], history=hist, **cc)
+    assert abs(r["mp2"] - g["mp2"]) < 6e-11
+    for (E, _), ref in zip(hist, g["iter_energies"][:2]):
+        assert abs(E - ref) < 6e-11
+
+
+def test_ft_fci_exact_two_spin_orbitals():
+    eps, U = -0.3, 0.8
+    h = numpy.diag([eps, eps])
+    eri = numpy.zeros((2, 2, 2, 2))
+    eri[0, 1, 0, 1] = eri[1, 0, 1, 0] = U
+    eri[0, 1, 1, 0] = eri[1, 0, 0, 1] = -U
+    T, mu = 1.0, 0.1
+    sysm = ftcc.GenericSystem(T, mu, h, eri, numpy.array([eps, eps]) + 0.2)
+    r = ftcc.ft_ccsd(sysm, T, mu, ngrid=200, max_iter=60, econv=1e-12)
+    assert abs(r["Omega"] - fs.ft_fci_omega(h, eri, 1.0 / T, mu)) < 1e-9
+
+
+# --- quadrature ----------------------------------------------------------
+@pytest.mark.parametrize("ng", [9, 10, 17])
+def test_simpson_lin_exactness(ng):
+    beta = 2.3
+    ti, g, G = ftcc.ft_quad(ng, beta, 'lin')
+    assert ti[0] == 0.0 and abs(ti[-1] - beta) < 1e-14
+    deg = 3 if ng % 2 == 1 else 1
+    for k in range(deg + 1):
+        assert abs(g @ ti ** k - beta ** (k + 1) / (k + 1)) < 1e-12
+    for y in range(1, ng):
+        degy = 3 if y % 2 == 0 else 1
+        for k in range(degy + 1):
+            assert abs(G[y] @ ti ** k - ti[y] ** (k + 1) / (k + 1)) < 1e-12
+    assert numpy.all(G[0] == 0.0)
+    assert numpy.all(numpy.triu(G, 1) == 0.0)
+
+
+@pytest.mark.parametrize("quad", ['lin', 'ln', 'sin', 'exp', 'quad', 'cub',
+                                  'quar', 'L'])
+def test_beta_derivative_rules(quad):
+    ng, beta, h = 11, 1.7, 1e-5
+    gd, Gd = ftcc.d_ft_quad(ng, beta, quad)
+    _, gp, Gp = ftcc.ft_quad(ng, beta + h, quad)
+    _, gm, Gm = ftcc.ft_quad(ng, beta - h, quad)
+    assert numpy.abs((gp - gm) / (2 * h) - gd).max() < 1e-8
+    assert numpy.abs((Gp - Gm) / (2 * h) - Gd).max() < 1e-8
+
+
+def test_mapped_rules_integrate():
+    beta = 1.9
+    for quad in ('ln', 'sin', 'exp', 'quad', 'cub', 'quar', 'mid'):
+        ti, g, G = ftcc.ft_quad(41, beta, quad)
+        assert abs(g.sum() - beta) < 1e-5
+        assert abs(g @ ti - beta ** 2 / 2) < 1e-4
+    with pytest.raises(Exception):
+        ftcc.ft_quad(10, 1.0, 'xyz')
+
+
+# --- Fermi factors ---------------------------------------------------------
+def test_fermi_closed_forms():
+    e = numpy.linspace(-3, 3, 41)
+    beta, mu = 37.0, 0.2
+    fo, fv = ftcc.ff(beta, e, mu), ftcc.ffv(beta, e, mu)
+    assert numpy.abs(fo + fv - 1.0).max() < 1e-15
+    x = beta * (e - mu)
+    ok = numpy.abs(x) < 30
+    assert numpy.allclose(fo[ok], 1.0 / (1.0 + numpy.exp(x[ok])), rtol=1e-14)
+    assert numpy.allclose(ftcc.GP0(beta, e, mu)[ok],
+                          -numpy.log(1 + numpy.exp(-x[ok])) / beta, rtol=1e-13)
+    tiny = ftcc.ff(500.0, numpy.array([1.0]), 0.0)[0]
+    assert abs(tiny / numpy.exp(-500.0) - 1.0) < 1e-12
+    tinyv = ftcc.ffv(500.0, numpy.array([-1.0]), 0.0)[0]
+    assert abs(tinyv / numpy.exp(-500.0) - 1.0) < 1e-12
+    h = 1e-6
+    dnum = (ftcc.GP0(beta + h, e, mu) - ftcc.GP0(beta - h, e, mu)) / (2 * h)
+    assert numpy.abs(dnum - ftcc.dGP0(beta, e, mu)).max() < 1e-8
+    E0 = beta * ftcc.dGP0(beta, e, mu).sum() + mu * fo.sum() + ftcc.GP0(beta, e, mu).sum()
+    assert abs(E0 - (e * fo).sum()) < 1e-12
+
+
+# --- integrators -----------------------------------------------------------
+def test_integrator_constant_source_closed_form():
+    """int_0^tau c e^{-(tau-s)D} ds = c (1-e^{-tau D})/D; the error must fall
+    as delta^3 (the trapezoid first panel of odd rows dominates)."""
+    rng = numpy.random.default_rng(2)
+    beta = 3.0
+    e = rng.uniform(-1, 1, 3)
+    d1 = ftcc.D1(e, e)
+    d1[numpy.abs(d1) < 1e-3] = 0.3
+    c = rng.normal(size=d1.shape)
+    errs = []
+    for ng in (41, 81):
+        ti, g, G = ftcc.ft_quad(ng, beta, 'lin')
+        tb = numpy.broadcast_to(c, (ng,) + c.shape).copy()
+        t = ftcc.int_tbar1(ng, tb, ti, d1, G)
+        exact = c[None] * (1 - numpy.exp(-ti[:, None, None] * d1[None])) / d1[None]
+        errs.append(numpy.abs(t - exact).max() / numpy.abs(exact).max())
+    assert errs[0] < 1e-4
+    assert 6.0 < errs[0] / errs[1] < 10.0
+
+
+def test_integrator_literal_equals_scan():
+    rng = numpy.random.default_rng(3)
+    ng, n = 7, 3
+    e = rng.uniform(-1, 1, n)
+    d2 = ftcc.D2(e, e)
+    tb = rng.normal(size=(ng, n, n, n, n))
+    for quad in ('lin', 'mid', 'L', 'sin'):
+        ti, g, G = ftcc.ft_quad(ng, 2.0, quad)
+        a = ftcc.int_tbar2(ng, tb, ti, d2, G)
+        b = ftcc.int_tbar_scan(ng, tb, ti, d2, G)
+        assert numpy.abs(a - b).max() < 1e-13
+
+
+# --- systems ----------------------------------------------------------------
+def test_ueg_integral_symmetries():
+    ueg = ftcc.UEGSystem(0.1, 3.8855, 7.4, 2.0, norb=19)
+    I = ueg.g_aint_tot()
+    assert numpy.abs(I + I.transpose(1, 0, 2, 3)).max() < 1e-14
+    assert numpy.abs(I + I.transpose(0, 1, 3, 2)).max() < 1e-14
+    assert numpy.abs(I - I.transpose(2, 3, 0, 1)).max() < 1e-14
+
+
+def test_df_model_system_shapes():
+    s = ftcc.DFModelSystem(0.1, 0.0, norb=8, seed=0)
+    eri = s.g_aint_tot()
+    assert eri.shape == (16,) * 4
+    assert numpy.abs(eri + eri.transpose(1, 0, 2, 3)).max() < 1e-15
+    assert numpy.abs(eri - eri.transpose(2, 3, 0, 1)).max() < 1e-15
