@@ -1,0 +1,157 @@
+/*
+ * ftcc.h -- C ABI of libftcc.so, the MI355X (gfx950) finite-temperature CCSD
+ * amplitude engine.
+ *
+ * Every pointer argument named *_dev is a device pointer (HBM, e.g. owned by a
+ * torch tensor); *_host pointers are host memory.  Amplitudes use the
+ * reference's layouts, C order, float64:
+ *     T1[y][a][i]          shape (ng, nv, no)
+ *     T2[y][a][b][i][j]    shape (ng, nv, nv, no, no)
+ * All calls are ordered on the stream given to ftcc_create; calls that return
+ * host scalars synchronise that stream.  A context is not thread safe: use
+ * one context per stream.  Every function returns 0 on success and a nonzero
+ * FTCC_E* code on failure, with a message available from ftcc_last_error.
+ *
+ * Citations name the reference function each entry point replaces (paths are
+ * relative to the kelvin checkout).
+ */
+#ifndef FTCC_H
+#define FTCC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ftcc_ctx ftcc_ctx;
+
+/* contraction precision of the O(n^6)/O(n^5) products; everything else
+ * (Fermi factors, propagators, quadrature, damping, norms, energies) is FP64 */
+enum {
+  FTCC_PREC_FP64 = 0,   /* FP64 operands, FP64 MFMA                          */
+  FTCC_PREC_FP32 = 1,   /* operands rounded to FP32, FP32 MFMA and accum.     */
+  FTCC_PREC_BF16X3 = 2  /* x = hi+lo in bf16; hi*hi+hi*lo+lo*hi, FP32 accum.  */
+};
+
+enum {
+  FTCC_OK = 0,
+  FTCC_EINVAL = 1,      /* bad argument (size, pointer, grid)                 */
+  FTCC_ESTATE = 2,      /* call order: integrals / grid not set               */
+  FTCC_EHIP = 3,        /* HIP runtime error (launch, copy)                   */
+  FTCC_ENOMEM = 4       /* device allocation failed                           */
+};
+
+/* Create a context for no occupied-role and nv virtual-role orbitals (at
+ * finite temperature with no active-space cut, no = nv = n) and ng grid
+ * points.  stream is a hipStream_t (NULL = default stream).  Replaces the
+ * set-up half of kelvin/ccsd.py:_ft_ccsd (ccsd.py:595-675). */
+int ftcc_create(ftcc_ctx **ctx, int no, int nv, int ng, int precision,
+                void *stream);
+int ftcc_destroy(ftcc_ctx *ctx);
+const char *ftcc_last_error(const ftcc_ctx *ctx);
+int ftcc_set_precision(ftcc_ctx *ctx, int precision);
+
+/* Fermi-weighted integrals (kelvin/cc_utils.py:569-602 ft_integrals and
+ * :825-858 ft_active_integrals).  en_dev[n], fock_dev[n*n] (= g_fock_tot,
+ * diagonal included), eri_dev[n^4] (= g_aint_tot, antisymmetrised).
+ * iocc_host[no] / ivir_host[nv] list the orbitals kept in each role (NULL =
+ * all n).  Fermi factors and sqrt weights are formed on the device in FP64;
+ * the denominators e_a-e_i, e_a+e_b-e_i-e_j (kelvin/ccsd.py:633-634) are
+ * formed on the fly from the role-ordered energies.  If omega0 is not NULL
+ * it receives sum_p GP0(e_p) over all n orbitals (kelvin/ft_mp.py:12-14 over
+ * cqcpy ft_utils.GP0, kelvin/ccsd.py:637-638). */
+int ftcc_set_system(ftcc_ctx *ctx, int n, const double *en_dev,
+                    const double *fock_dev, const double *eri_dev,
+                    double beta, double mu, const int *iocc_host,
+                    const int *ivir_host, double *omega0);
+
+/* Arbitrary blocks in the reference's layouts (cqcpy one_e_blocks /
+ * two_e_blocks as passed to kelvin/ft_cc_equations.py:96 ccsd_stanton):
+ *   F: oo[no,no] ov[no,nv] vo[nv,no] vv[nv,nv]
+ *   I: vvvv vvvo vovv vvoo vovo oovv vooo ooov oooo
+ * (pointer order exactly as listed; device pointers, copied). */
+int ftcc_set_blocks(ftcc_ctx *ctx, const double *const F_dev[4],
+                    const double *const I_dev[9]);
+
+/* Explicit denominators D1_dev[nv,no], D2_dev[nv,nv,no,no] (the arguments
+ * D1, D2 of kelvin/ft_cc_equations.py:96), copied. */
+int ftcc_set_denominators(ftcc_ctx *ctx, const double *D1_dev,
+                          const double *D2_dev);
+
+/* Copy one stored block into dst_dev: which = 0..3 for F oo ov vo vv,
+ * 4..12 for I vvvv vvvo vovv vvoo vovo oovv vooo ooov oooo. */
+int ftcc_get_block(ftcc_ctx *ctx, int which, double *dst_dev);
+
+/* Quadrature abscissae and Volterra weights (kelvin/quadrature.py ft_quad
+ * output): ti_host[ng], G_host[ng*ng] row-major; ng may differ from the
+ * value given to ftcc_create (amplitude arguments then have ng slices).
+ * Any G is accepted (compute_ESN passes Gd and G[i,j]*(t_j-t_i)/beta,
+ * kelvin/ccsd.py:1118-1137). */
+int ftcc_set_grid(ftcc_ctx *ctx, int ng, const double *ti_host,
+                  const double *G_host);
+
+/* Tbar = -F.vo, -I.vvoo at every grid point (the source of the initial
+ * guess, kelvin/ccsd.py:680-686). */
+int ftcc_driving_terms(ftcc_ctx *ctx, double *T1bar_dev, double *T2bar_dev);
+
+/* Tbar = -F.vo - R1(T), -I.vvoo - R2(T) at every grid point: the loop of
+ * kelvin/ft_cc_equations.py:101-107 (cqcpy cc_equations._Stanton, fac=-1).
+ * Bit y of skip_mask (y < 64) promises T(tau_y) == 0 exactly; such slices
+ * get the driving term alone. */
+int ftcc_residual(ftcc_ctx *ctx, const double *T1_dev, const double *T2_dev,
+                  double *T1bar_dev, double *T2bar_dev, uint64_t skip_mask);
+
+/* T(tau_y) = sum_x G[y,x] exp(-(tau_y - tau_x) D) Tbar(tau_x), in place
+ * (kelvin/quadrature.py:292-317 int_tbar1 / int_tbar2).  Either pointer may
+ * be NULL to integrate only the other. */
+int ftcc_integrate(ftcc_ctx *ctx, double *T1_dev, double *T2_dev);
+
+/* (1/beta) sum_y g_y [ T1.F.ov + (T2/4 + q*T1 T1/2).I.oovv ]
+ * (kelvin/ft_cc_energy.py:7-32; g_host[ng]; qterm = Qterm). */
+int ftcc_energy(ftcc_ctx *ctx, const double *T1_dev, const double *T2_dev,
+                const double *g_host, double beta, int qterm,
+                double *energy);
+
+/* sum of squares of n doubles (numpy.linalg.norm(x)**2 of
+ * kelvin/cc_utils.py:139-140,145-151). */
+int ftcc_sumsq(ftcc_ctx *ctx, const double *x_dev, int64_t n, double *out);
+
+/* One body of the damped fixed point (kelvin/cc_utils.py:141-155):
+ *   (T1,T2) = integrate(residual(T1old, T2old))
+ *   d1 = |T1 - T1old|^2, d2 = |T2 - T2old|^2
+ *   Told <- damp*Told + (1-damp)*T      (in place in T1old_dev / T2old_dev)
+ *   n1 = |T1old|^2, n2 = |T2old|^2, energy = ft_cc_energy(T1old, T2old, g)
+ * T1bar_dev / T2bar_dev are caller-provided scratch of amplitude size.
+ * out5 receives {d1, d2, n1, n2, energy}. */
+int ftcc_step(ftcc_ctx *ctx, double *T1old_dev, double *T2old_dev,
+              double *T1bar_dev, double *T2bar_dev, const double *g_host,
+              double beta, double damp, uint64_t skip_mask, double out5[5]);
+
+/* Device bytes held by the context (integrals + workspace). */
+int ftcc_workspace_bytes(const ftcc_ctx *ctx, int64_t *bytes);
+
+/* Kernel timing with HIP events on the context's stream (no counterpart in
+ * the reference; used by the benchmark).  While enabled, every launch of an
+ * O(n^6) product of the residual is bracketed by events.  ftcc_profile_read
+ * synchronises the stream and returns, since the last read, the number of
+ * such launches, their summed device time in ms and their summed
+ * algorithmic flop count (2*M*N*K per product and grid point). */
+int ftcc_profile_enable(ftcc_ctx *ctx, int on);
+int ftcc_profile_read(ftcc_ctx *ctx, int64_t *launches, double *ms,
+                      double *flops);
+
+/* Batched strided GEMM of the residual engine, exposed for testing:
+ * C[b] = alpha * A[b] . B[b] + beta * C[b] with element strides
+ * A(m,k) = A + b*sAb + m*sAm + k*sAk, likewise B(k,n), C(m,n);
+ * float64 operands in memory, contraction precision FTCC_PREC_*. */
+int ftcc_gemm(ftcc_ctx *ctx, int M, int N, int K, int batch, double alpha,
+              const double *A, int64_t sAm, int64_t sAk, int64_t sAb,
+              const double *B, int64_t sBk, int64_t sBn, int64_t sBb,
+              double beta, double *C, int64_t sCm, int64_t sCn, int64_t sCb,
+              int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FTCC_H */

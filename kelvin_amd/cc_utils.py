@@ -1,0 +1,126 @@
+"""Fermi-weighted integrals and the damped amplitude iteration.
+
+Mirrors kelvin/cc_utils.py: ``ft_integrals`` (:569-602),
+``ft_active_integrals`` (:825-858), ``form_new_ampl`` (:17-49) and
+``ft_cc_iter`` (:111-173).  The weighted blocks are built on the GPU and kept
+there; each iteration is one ``ftcc_step`` (residual at every grid point,
+exp(-tau D) integration, damping, norms and energy fused on the device);
+the host only keeps the loop's scalars.
+"""
+import logging
+import math
+import time
+
+import numpy
+import torch
+
+from . import ft_cc_equations
+from .engine import FTEngine
+from .ov_blocks import EngineBlocks, EngineDenominators, engine_eri, engine_fock
+
+
+def _engine_for_system(sys, en, beta, mu, iocc, ivir, ng, precision):
+    n = len(en)
+    no = n if iocc is None else len(iocc)
+    nv = n if ivir is None else len(ivir)
+    eng = FTEngine(no, nv, ng, precision=precision)
+    eng.omega0 = eng.set_system(en, sys.g_fock_tot(), sys.g_aint_tot(), beta,
+                                mu, iocc=iocc, ivir=ivir)
+    return eng
+
+
+def ft_integrals(sys, en, beta, mu, ng=1, precision="fp64"):
+    """Return (F, I): handles on the GPU-resident sqrt(f)-weighted Fock and
+    ERI blocks over all n orbitals (kelvin/cc_utils.py:569-602)."""
+    eng = _engine_for_system(sys, en, beta, mu, None, None, ng, precision)
+    return engine_fock(eng), engine_eri(eng)
+
+
+def ft_active_integrals(sys, en, focc, fvir, iocc, ivir, beta=None, mu=None,
+                        ng=1, precision="fp64"):
+    """Blocks restricted to the orbitals with f > athresh
+    (kelvin/cc_utils.py:825-858).  The device recomputes the Fermi factors
+    from (beta, mu); focc/fvir are accepted for signature compatibility."""
+    if beta is None or mu is None:
+        T, mu = getattr(sys, "T"), getattr(sys, "mu")
+        beta = 1.0 / T
+    eng = _engine_for_system(sys, en, beta, mu, list(iocc), list(ivir), ng,
+                             precision)
+    return engine_fock(eng), engine_eri(eng)
+
+
+def form_new_ampl(method, F, I, T1old, T2old, D1, D2, ti, ng, G):
+    """New (integrated) amplitudes (kelvin/cc_utils.py:17-49)."""
+    if method == "CCSD":
+        return ft_cc_equations.ccsd_stanton(F, I, T1old, T2old, D1, D2, ti,
+                                            ng, G)
+    if method in ("CCD", "LCCSD", "LCCD"):
+        raise NotImplementedError(
+            "method %s is not built for the GPU path" % method)
+    raise Exception("Unrecognized method keyword")
+
+
+def zero_slices(engine, T1, T2, G):
+    """Bit mask of grid points whose amplitudes are exactly zero and stay so
+    (G[y,:] == 0): their residual is the driving term alone."""
+    mask = 0
+    Gm = numpy.asarray(G)
+    for y in range(min(T1.shape[0], 64)):
+        if numpy.all(Gm[y] == 0.0):
+            if engine.sumsq(T1[y]) == 0.0 and engine.sumsq(T2[y]) == 0.0:
+                mask |= 1 << y
+    return mask
+
+
+def ft_cc_iter(method, T1old, T2old, F, I, D1, D2, g, G, beta, ng, ti,
+               iprint, conv_options, history=None):
+    """Damped fixed-point iteration (kelvin/cc_utils.py:111-173).
+
+    T1old/T2old are device tensors updated in place; returns
+    (Eold, T1old, T2old).  ``history`` (optional list) receives
+    (E, res) per iteration.
+    """
+    if method != "CCSD":
+        return _other_method(method)
+    eng = ft_cc_equations.engine_of(F, I, D1, D2, ng)
+    eng.set_grid(ti, G)
+    tbeg = time.time()
+    converged = False
+    ethresh = conv_options["econv"]
+    tthresh = conv_options["tconv"]
+    max_iter = conv_options["max_iter"]
+    alpha = conv_options["damp"]
+    T1b = torch.empty_like(T1old)
+    T2b = torch.empty_like(T2old)
+    skip = zero_slices(eng, T1old, T2old, G)
+    i = 0
+    Eold = 888888888.888888888
+    nl1 = math.sqrt(eng.sumsq(T1old)) + 0.1
+    nl2 = math.sqrt(eng.sumsq(T2old)) + 0.1
+    while i < max_iter and not converged:
+        d1, d2, n1, n2, E = eng.step(T1old, T2old, T1b, T2b, g, beta, alpha,
+                                     skip)
+        res1 = math.sqrt(d1) / nl1
+        res2 = math.sqrt(d2) / nl2
+        nl1 = math.sqrt(n1) + 0.1
+        nl2 = math.sqrt(n2) + 0.1
+        logging.info(' %2d  %.10f   %.4E' % (i + 1, E, res1 + res2))
+        if history is not None:
+            history.append((E, res1 + res2))
+        i = i + 1
+        if abs(E - Eold) < ethresh and res1 + res2 < tthresh:
+            converged = True
+        Eold = E
+    if not converged:
+        logging.warning("{} did not converge!".format(method))
+    tend = time.time()
+    logging.info("Total {} time: {:.4f} s".format(method, (tend - tbeg)))
+    del T1b, T2b
+    return Eold, T1old, T2old
+
+
+def _other_method(method):
+    if method in ("CCD", "LCCSD", "LCCD"):
+        raise NotImplementedError(
+            "method %s is not built for the GPU path" % method)
+    raise Exception("Unrecognized method keyword")

@@ -1,0 +1,931 @@
+// C ABI of libftcc: context, Fermi-weighted integrals, the Stanton-Gauss
+// residual at every grid point, integration, damping and energies.
+//
+// Residual schedule for a chunk of nb grid points (all indices over the
+// occupied-role list o and virtual-role list v; O = no, V = nv).  Six
+// O(n^6) products run on the matrix cores:
+//   G1  Wmnij += 1/4 oovv[mn,ef] . tau[ef,ij]
+//   G2  Wabef += 1/4 tau[ab,mn] . oovv[mn,ef]
+//   G3  R2    += 1/2 tau[ab,mn] . Wmnij[mn,ij]
+//   G4  R2    += 1/2 Wabef[ab,ef] . tau[ef,ij]
+//   G5  Wr[em,bj] -= oovv[m,n,e,f](as [em,fn]) . A[fn,bj],
+//                    A[f,n,b,j] = 1/2 t_jn^fb + t_j^f t_n^b
+//   G6  Z[ai,bj]   = t2[a,e,i,m](as [ai,em]) . Wr[em,bj]
+// ("ring" tensors are stored [p,k,q,l] for <pq..kl> so G5/G6 are plain
+// products), and the O(n^5) terms are batched GEMMs on the same engine.
+// Equations: Stanton, Gauss, Watts, Bartlett, J. Chem. Phys. 94, 4334 (1991)
+// in kelvin's block conventions (kelvin/ft_cc_equations.py:96-113).
+#include "ftcc_internal.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <algorithm>
+#include <cstring>
+
+using namespace ftcc;
+
+struct ftcc_ctx {
+  int no = 0, nv = 0, ng = 0, prec = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+
+  // integrals in the reference layouts
+  double *F[4] = {nullptr, nullptr, nullptr, nullptr};   // oo ov vo vv
+  double *I[9] = {};  // vvvv vvvo vovv vvoo vovo oovv vooo ooov oooo
+  // derived layouts
+  double *oovv_ring = nullptr;  // [e,m,f,n] = oovv[m,n,e,f]
+  double *oovv_fmne = nullptr;  // [f,m,n,e] = oovv[m,n,e,f]
+  double *ovvo_ring = nullptr;  // [e,m,b,j] = <mb||ej> = -vovo[b,m,e,j]
+  double *ooov_p1 = nullptr;    // [m,i,n,e] = ooov[m,n,i,e]
+  double *ooov_emni = nullptr;  // [e,m,n,i] = ooov[n,m,i,e]
+  double *oovvT = nullptr;      // [a,b,i,j] = oovv[i,j,a,b]
+  double *fovT = nullptr;       // [a,i] = F.ov[i,a]
+  double *eo = nullptr, *ev = nullptr;
+  double *D1 = nullptr, *D2 = nullptr;   // explicit denominators (optional)
+  bool have_D = false;
+  bool have_ints = false;
+  bool have_energies = false;
+  Grid grid;
+
+  // per-slot workspace (nb slots)
+  int nb = 1;
+  double *tau = nullptr, *taut, *t2r, *aring, *t2p, *tautp, *wmnij, *wabef,
+      *wr, *z, *u, *s1, *s2, *q1, *q2;
+  double *t1T, *Fme, *FmeT, *Fae, *Fmi, *Fbe, *Fmj, *ytmp;
+  Scratch scr;
+  double *partials = nullptr;   // 3*MAX blocks
+  double *red = nullptr;        // device scalars
+  double *host_red = nullptr;   // pinned
+
+  // event timing of the O(n^6) products
+  struct Pending {
+    hipEvent_t e0, e1;
+    double flops;
+  };
+  bool prof_on = false;
+  std::vector<Pending> prof_pending;
+  std::vector<hipEvent_t> ev_pool;
+  double prof_ms = 0.0, prof_flops = 0.0;
+  int64_t prof_launches = 0;
+};
+
+namespace {
+
+constexpr int MAX_PARTIAL_BLOCKS = 8192;
+
+double *dalloc(ftcc_ctx *c, int64_t n) {
+  if (n <= 0) n = 1;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, sizeof(double) * (size_t)n);
+  if (e != hipSuccess)
+    throw Error(FTCC_ENOMEM, "hipMalloc of " + std::to_string(8 * n) +
+                                 " bytes failed: " + hipGetErrorString(e));
+  c->allocs.push_back(p);
+  c->bytes += 8 * n;
+  return (double *)p;
+}
+
+template <typename Fn>
+int guarded(ftcc_ctx *c, Fn fn) {
+  if (!c) return FTCC_EINVAL;
+  try {
+    fn();
+    c->err.clear();
+    return FTCC_OK;
+  } catch (const Error &e) {
+    c->err = e.msg;
+    return e.code;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return FTCC_EINVAL;
+  }
+}
+
+int64_t p4(int a, int b, int c, int d) {
+  return (int64_t)a * b * c * d;
+}
+
+// Output strides for writing a dense (d0,d1,d2,d3) source into the axis
+// order `perm` (perm[k] = source axis placed at output position k).
+void derive(const double *src, double *dst, const int d[4], const int perm[4],
+            double alpha, hipStream_t s) {
+  int od[4];
+  for (int k = 0; k < 4; ++k) od[k] = d[perm[k]];
+  int64_t ostride[4];
+  ostride[3] = 1;
+  for (int k = 2; k >= 0; --k) ostride[k] = ostride[k + 1] * od[k + 1];
+  int64_t sstr[4];
+  for (int k = 0; k < 4; ++k) sstr[perm[k]] = ostride[k];
+  permute4(src, dst, d[0], d[1], d[2], d[3], sstr[0], sstr[1], sstr[2],
+           sstr[3], alpha, 0.0, 1, 0, 0, s);
+}
+
+void build_derived(ftcc_ctx *c) {
+  const int O = c->no, V = c->nv;
+  hipStream_t s = c->stream;
+  const int doovv[4] = {O, O, V, V};
+  const int pr[4] = {2, 0, 3, 1};
+  derive(c->I[5], c->oovv_ring, doovv, pr, 1.0, s);
+  const int pf[4] = {3, 0, 1, 2};
+  derive(c->I[5], c->oovv_fmne, doovv, pf, 1.0, s);
+  const int pt[4] = {2, 3, 0, 1};
+  derive(c->I[5], c->oovvT, doovv, pt, 1.0, s);
+  const int dvovo[4] = {V, O, V, O};
+  const int pv[4] = {2, 1, 0, 3};
+  derive(c->I[4], c->ovvo_ring, dvovo, pv, -1.0, s);
+  const int dooov[4] = {O, O, O, V};
+  const int p1[4] = {0, 2, 1, 3};
+  derive(c->I[7], c->ooov_p1, dooov, p1, 1.0, s);
+  const int pe[4] = {3, 1, 0, 2};
+  derive(c->I[7], c->ooov_emni, dooov, pe, 1.0, s);
+  transpose2(c->F[1], c->fovT, O, V, 1, s);
+  c->have_ints = true;
+}
+
+// GEMM helper: dense row-major operands with per-slot batch strides.
+Gemm mk(int M, int N, int K, double alpha, double beta, const double *A,
+        int64_t sAm, int64_t sAk, int64_t sA1, const double *B, int64_t sBk,
+        int64_t sBn, int64_t sB1, double *C, int64_t sCm, int64_t sCn,
+        int64_t sC1, int nb1) {
+  Gemm g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = A;
+  g.sAm = sAm;
+  g.sAk = sAk;
+  g.sA1 = sA1;
+  g.B = B;
+  g.sBk = sBk;
+  g.sBn = sBn;
+  g.sB1 = sB1;
+  g.C = C;
+  g.sCm = sCm;
+  g.sCn = sCn;
+  g.sC1 = sC1;
+  g.nb1 = nb1;
+  return g;
+}
+
+void ensure_partial(ftcc_ctx *c, const Gemm &g) {
+  const size_t need = gemm_partial_elems(g);
+  if (need > c->scr.partial_elems) {
+    // grow: allocation happens outside the kernel stream order, so drain it
+    FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (c->scr.partial) {
+      auto it = std::find(c->allocs.begin(), c->allocs.end(),
+                          (void *)c->scr.partial);
+      if (it != c->allocs.end()) c->allocs.erase(it);
+      c->bytes -= 8 * (int64_t)c->scr.partial_elems;
+      FTCC_CHECK_HIP(hipFree(c->scr.partial));
+    }
+    c->scr.partial = dalloc(c, (int64_t)need);
+    c->scr.partial_elems = need;
+  }
+}
+
+hipEvent_t take_event(ftcc_ctx *c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  FTCC_CHECK_HIP(hipEventCreate(&e));
+  return e;
+}
+
+// Accumulate finished event pairs (call only after the stream drained).
+void flush_profile(ftcc_ctx *c) {
+  for (auto &p : c->prof_pending) {
+    float ms = 0.f;
+    FTCC_CHECK_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
+    c->prof_ms += ms;
+    c->prof_flops += p.flops;
+    c->prof_launches += 1;
+    c->ev_pool.push_back(p.e0);
+    c->ev_pool.push_back(p.e1);
+  }
+  c->prof_pending.clear();
+}
+
+void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false) {
+  ensure_partial(c, g);
+  if (big && c->prof_on) {
+    ftcc_ctx::Pending p;
+    p.e0 = take_event(c);
+    p.e1 = take_event(c);
+    p.flops = 2.0 * g.M * (double)g.N * g.K * g.nb1 * g.nb2;
+    FTCC_CHECK_HIP(hipEventRecord(p.e0, c->stream));
+    gemm_run(g, prec, c->scr, c->stream);
+    FTCC_CHECK_HIP(hipEventRecord(p.e1, c->stream));
+    c->prof_pending.push_back(p);
+  } else {
+    gemm_run(g, prec, c->scr, c->stream);
+  }
+}
+
+void ensure_integrals(ftcc_ctx *c) {
+  if (c->F[0]) return;
+  const int O = c->no, V = c->nv;
+  const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
+  const int64_t S2 = VV * OO;
+  c->F[0] = dalloc(c, OO);
+  c->F[1] = dalloc(c, VO);
+  c->F[2] = dalloc(c, VO);
+  c->F[3] = dalloc(c, VV);
+  const int64_t isz[9] = {VV * VV, VV * VO, VO * VV, S2, S2,
+                          S2,      V * OO * O, OO * VO, OO * OO};
+  for (int k = 0; k < 9; ++k) c->I[k] = dalloc(c, isz[k]);
+  c->oovv_ring = dalloc(c, S2);
+  c->oovv_fmne = dalloc(c, S2);
+  c->oovvT = dalloc(c, S2);
+  c->ovvo_ring = dalloc(c, S2);
+  c->ooov_p1 = dalloc(c, OO * VO);
+  c->ooov_emni = dalloc(c, OO * VO);
+  c->fovT = dalloc(c, VO);
+  c->eo = dalloc(c, O);
+  c->ev = dalloc(c, V);
+}
+
+// Residual workspace: as many grid points per chunk as fit the budget
+// (FTCC_WORKSPACE_GB, default 24 GiB) and are needed to fill the GPU.
+void ensure_workspace(ftcc_ctx *c) {
+  if (c->tau) return;
+  const int O = c->no, V = c->nv, ng = c->ng;
+  const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
+  const int64_t S2 = VV * OO;
+  const int64_t q1sz = std::max<int64_t>(std::max<int64_t>(OO * OO, VV * VV), S2);
+  const int64_t sm = std::max<int64_t>(VV, std::max<int64_t>(VO, OO));
+  const int64_t per_slot =
+      8 * (11 * S2 + OO * OO + VV * VV + OO * VO + q1sz + 8 * sm);
+  double budget_gb = 24.0;
+  if (const char *env = getenv("FTCC_WORKSPACE_GB")) budget_gb = atof(env);
+  const int64_t budget = (int64_t)(budget_gb * (1LL << 30));
+  const int64_t want = std::max<int64_t>(1, (1LL << 24) / std::max<int64_t>(1, S2));
+  const int nb = (int)std::min<int64_t>(
+      ng, std::max<int64_t>(1, std::min<int64_t>(want, budget / per_slot)));
+  c->nb = nb;
+  c->tau = dalloc(c, S2 * nb);
+  c->taut = dalloc(c, S2 * nb);
+  c->t2r = dalloc(c, S2 * nb);
+  c->aring = dalloc(c, S2 * nb);
+  c->t2p = dalloc(c, S2 * nb);
+  c->tautp = dalloc(c, S2 * nb);
+  c->wr = dalloc(c, S2 * nb);
+  c->z = dalloc(c, S2 * nb);
+  c->s1 = dalloc(c, S2 * nb);
+  c->s2 = dalloc(c, S2 * nb);
+  c->wmnij = dalloc(c, OO * OO * nb);
+  c->wabef = dalloc(c, VV * VV * nb);
+  c->u = dalloc(c, OO * VO * nb);
+  c->q1 = dalloc(c, q1sz * nb);
+  c->q2 = dalloc(c, S2 * nb);
+  c->t1T = dalloc(c, VO * nb);
+  c->Fme = dalloc(c, VO * nb);
+  c->FmeT = dalloc(c, VO * nb);
+  c->Fae = dalloc(c, VV * nb);
+  c->Fmi = dalloc(c, OO * nb);
+  c->Fbe = dalloc(c, VV * nb);
+  c->Fmj = dalloc(c, OO * nb);
+  c->ytmp = dalloc(c, sm * nb);
+}
+
+// Residual for `nb` consecutive slices starting at t1/t2 (slice strides
+// V*O and V*V*O*O); writes t1bar/t2bar.
+void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
+                    double *t1bar, double *t2bar, int nb) {
+  const int O = c->no, V = c->nv;
+  const int P = c->prec;           // contraction precision of n^5/n^6 terms
+  const int P64 = FTCC_PREC_FP64;  // small terms
+  hipStream_t s = c->stream;
+  const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
+  const int64_t S2 = VV * OO;           // doubles slice
+  const int64_t O4 = OO * OO, V4 = VV * VV;
+  const double *Foo = c->F[0], *Fov = c->F[1], *Fvo = c->F[2], *Fvv = c->F[3];
+  const double *vvvv = c->I[0], *vvvo = c->I[1], *vovv = c->I[2],
+               *vvoo = c->I[3], *oovv = c->I[5], *vooo = c->I[6],
+               *ooov = c->I[7], *oooo = c->I[8];
+
+  // amplitude-derived operands
+  PrepOut po{c->tau, c->taut, c->t2r, c->aring, c->t2p, c->tautp};
+  prep_amplitudes(t1, t2, V, O, nb, po, s);
+  transpose2(t1, c->t1T, V, O, nb, s);   // t1T[m,f] = t1[f,m]
+
+  // Fme[m,e] = Fov[m,e] + sum_{fn} <mn||ef> t_n^f
+  gemv_rows(c->oovv_ring, VO, 0, t1, VO, c->FmeT, VO, (int)VO, (int)VO, nb,
+            1.0, 0.0, s);
+  transpose2(c->FmeT, c->Fme, V, O, nb, s);
+  bcast(c->Fme, Fov, VO, nb, 1.0, 1.0, s);
+  transpose2(c->Fme, c->FmeT, O, V, nb, s);
+
+  // Fae[a,e] = Fvv - 1/2 t1.Fov + sum t_m^f <ma||fe> - 1/2 taut.oovv
+  bcast(c->Fae, Fvv, VV, nb, 1.0, 0.0, s);
+  run(c, mk(V, V, O, -0.5, 1.0, t1, O, 1, VO, Fov, V, 1, 0, c->Fae, V, 1, VV,
+            nb), P64);
+  for (int y = 0; y < nb; ++y)
+    gemv_cols(vovv, V, VO * V, c->t1T + y * VO, 0, c->Fae + y * VV, V,
+              (int)VO, V, V, -1.0, 1.0, c->scr.partial, c->scr.partial_elems,
+              s);
+  run(c, mk(V, V, (int)(V * OO), -0.5, 1.0, c->taut, V * OO, 1, S2,
+            c->oovv_fmne, V, 1, 0, c->Fae, V, 1, VV, nb), P);
+
+  // Fmi[m,i] = Foo + 1/2 Fov.t1 + sum t_n^e <mn||ie> + 1/2 oovv.tautp
+  bcast(c->Fmi, Foo, OO, nb, 1.0, 0.0, s);
+  run(c, mk(O, O, V, 0.5, 1.0, Fov, V, 1, 0, t1, O, 1, VO, c->Fmi, O, 1, OO,
+            nb), P64);
+  gemv_rows(c->ooov_p1, VO, 0, c->t1T, VO, c->Fmi, OO, (int)OO, (int)VO, nb,
+            1.0, 1.0, s);
+  run(c, mk(O, O, (int)(O * VV), 0.5, 1.0, oovv, O * VV, 1, 0, c->tautp, O, 1,
+            S2, c->Fmi, O, 1, OO, nb), P);
+
+  // Fbe = Fae - 1/2 t1.Fme ; Fmj = Fmi + 1/2 Fme.t1
+  axpby(c->Fbe, c->Fae, VV * nb, 1.0, 0.0, s);
+  run(c, mk(V, V, O, -0.5, 1.0, t1, O, 1, VO, c->Fme, V, 1, VO, c->Fbe, V, 1,
+            VV, nb), P64);
+  axpby(c->Fmj, c->Fmi, OO * nb, 1.0, 0.0, s);
+  run(c, mk(O, O, V, 0.5, 1.0, c->Fme, V, 1, VO, t1, O, 1, VO, c->Fmj, O, 1,
+            OO, nb), P64);
+
+  // Wmnij = oooo + P(ij) t_j^e <mn||ie> + 1/4 oovv.tau          (G1)
+  bcast(c->wmnij, oooo, O4, nb, 1.0, 0.0, s);
+  run(c, mk((int)(OO * O), O, V, 1.0, 0.0, ooov, V, 1, 0, t1, O, 1, VO, c->q1,
+            O, 1, O4, nb), P);
+  antisym_add(c->wmnij, c->q1, O, O, O, O, true, 1.0, nb, s);
+  run(c, mk((int)OO, (int)OO, (int)VV, 0.25, 1.0, oovv, VV, 1, 0, c->tau, OO,
+            1, S2, c->wmnij, OO, 1, O4, nb), P, true);
+
+  // Wabef = vvvv - P(ab) t_m^b <am||ef> + 1/4 tau.oovv          (G2)
+  bcast(c->wabef, vvvv, V4, nb, 1.0, 0.0, s);
+  {
+    Gemm g = mk(V, (int)VV, O, 1.0, 0.0, t1, O, 1, VO, vovv, VV, 1, 0, c->q1,
+                VV, 1, V4, nb);
+    g.nb2 = V;
+    g.sA2 = 0;
+    g.sB2 = O * VV;
+    g.sC2 = V * VV;
+    run(c, g, P);
+  }
+  antisym_add(c->wabef, c->q1, V, V, V, V, false, -1.0, nb, s);
+  run(c, mk((int)VV, (int)VV, (int)OO, 0.25, 1.0, c->tau, OO, 1, S2, oovv, VV,
+            1, 0, c->wabef, VV, 1, V4, nb), P, true);
+
+  // Wr[e,m,b,j] = <mb||ej> + t_j^f <mb||ef> - t_n^b <mn||ej> - A.<mn||ef>
+  run(c, mk((int)(V * VO), O, V, 1.0, 0.0, vovv, V, 1, 0, t1, O, 1, VO, c->q1,
+            O, 1, S2, nb), P);
+  {
+    Gemm g = mk(V, (int)VO, O, 1.0, 0.0, t1, O, 1, VO, ooov, VO, 1, 0, c->q2,
+                VO, 1, S2, nb);
+    g.nb2 = O;
+    g.sA2 = 0;
+    g.sB2 = O * VO;
+    g.sC2 = V * VO;
+    run(c, g, P);
+  }
+  ring_assemble(c->wr, c->ovvo_ring, c->q1, c->q2, V, O, nb, s);
+  run(c, mk((int)VO, (int)VO, (int)VO, -1.0, 1.0, c->oovv_ring, VO, 1, 0,
+            c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), P, true);             // G5
+
+  // Z[ai,bj] = t2r.Wr - t_i^e t_m^a <mb||ej>                    (G6)
+  run(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr, VO,
+            1, S2, c->z, VO, 1, S2, nb), P, true);
+  run(c, mk(O, (int)(O * VO), V, 1.0, 0.0, t1, 1, O, VO, c->ovvo_ring, O * VO,
+            1, 0, c->u, O * VO, 1, OO * VO, nb), P);
+  {
+    Gemm g = mk(V, (int)VO, O, -1.0, 1.0, t1, O, 1, VO, c->u, VO, 1, OO * VO,
+                c->z, O * VO, 1, S2, nb);
+    g.nb2 = O;
+    g.sA2 = 0;
+    g.sB2 = O * VO;
+    g.sC2 = VO;
+    run(c, g, P);
+  }
+
+  // doubles accumulator in t2bar: vvoo + 1/2 tau.Wmnij + 1/2 Wabef.tau
+  bcast(t2bar, vvoo, S2, nb, 1.0, 0.0, s);
+  run(c, mk((int)VV, (int)OO, (int)OO, 0.5, 1.0, c->tau, OO, 1, S2, c->wmnij,
+            OO, 1, O4, t2bar, OO, 1, S2, nb), P, true);                        // G3
+  run(c, mk((int)VV, (int)OO, (int)VV, 0.5, 1.0, c->wabef, VV, 1, V4, c->tau,
+            OO, 1, S2, t2bar, OO, 1, S2, nb), P, true);                        // G4
+
+  // S1 (antisymmetrised in ab): Fbe[b,e] t2[a,e,i,j] + t1[a,m] vooo[b,m,i,j]
+  {
+    Gemm g = mk(V, (int)OO, V, 1.0, 0.0, c->Fbe, V, 1, VV, t2, OO, 1, S2,
+                c->s1, OO, 1, S2, nb);
+    g.nb2 = V;
+    g.sA2 = 0;
+    g.sB2 = V * OO;
+    g.sC2 = V * OO;
+    run(c, g, P);
+  }
+  {
+    Gemm g = mk(V, (int)OO, O, 1.0, 1.0, t1, O, 1, VO, vooo, OO, 1, 0, c->s1,
+                V * OO, 1, S2, nb);
+    g.nb2 = V;
+    g.sA2 = 0;
+    g.sB2 = O * OO;
+    g.sC2 = OO;
+    run(c, g, P);
+  }
+  // S2 (antisymmetrised in ij): -t2[a,b,i,m] Fmj[m,j] + t1[e,i] vvvo[a,b,e,j]
+  run(c, mk((int)(VV * O), O, O, -1.0, 0.0, t2, O, 1, S2, c->Fmj, O, 1, OO,
+            c->s2, O, 1, S2, nb), P);
+  {
+    Gemm g = mk(O, O, V, 1.0, 1.0, t1, 1, O, VO, vvvo, O, 1, 0, c->s2, O, 1,
+                S2, nb);
+    g.nb2 = (int)VV;
+    g.sA2 = 0;
+    g.sB2 = VO;
+    g.sC2 = OO;
+    run(c, g, P);
+  }
+  r2_final(t2bar, c->s1, c->s2, c->z, V, O, nb, s);
+
+  // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
+  //                    + 1/2 vovv.t2p + 1/2 t2.ooov_emni)
+  bcast(t1bar, Fvo, VO, nb, 1.0, 0.0, s);
+  run(c, mk(V, O, V, 1.0, 1.0, c->Fae, V, 1, VV, t1, O, 1, VO, t1bar, O, 1, VO,
+            nb), P64);
+  run(c, mk(V, O, O, -1.0, 1.0, t1, O, 1, VO, c->Fmi, O, 1, OO, t1bar, O, 1,
+            VO, nb), P64);
+  gemv_rows(c->t2r, VO, S2, c->FmeT, VO, t1bar, VO, (int)VO, (int)VO, nb, 1.0,
+            1.0, s);
+  for (int y = 0; y < nb; ++y)
+    gemv_cols(c->ovvo_ring, VO, 0, t1 + y * VO, 0, t1bar + y * VO, 0, (int)VO,
+              (int)VO, 1, 1.0, 1.0, c->scr.partial, c->scr.partial_elems, s);
+  run(c, mk(V, O, (int)(O * VV), 0.5, 1.0, vovv, O * VV, 1, 0, c->t2p, O, 1,
+            S2, t1bar, O, 1, VO, nb), P);
+  run(c, mk(V, O, (int)(V * OO), 0.5, 1.0, t2, V * OO, 1, S2, c->ooov_emni, O,
+            1, 0, t1bar, O, 1, VO, nb), P);
+  axpby(t1bar, t1bar, VO * nb, -1.0, 0.0, s);
+}
+
+void check_grid_D(ftcc_ctx *c) {
+  require(c->have_energies || c->have_D,
+          "denominators not set (ftcc_set_system/ftcc_set_denominators)",
+          FTCC_ESTATE);
+  require(c->grid.ng == c->ng, "grid not set (ftcc_set_grid)", FTCC_ESTATE);
+}
+
+void check_ready(ftcc_ctx *c) {
+  require(c->have_ints, "integrals not set (ftcc_set_system/ftcc_set_blocks)",
+          FTCC_ESTATE);
+  require(c->have_energies || c->have_D,
+          "denominators not set (ftcc_set_system/ftcc_set_denominators)",
+          FTCC_ESTATE);
+  require(c->grid.ng == c->ng, "grid not set (ftcc_set_grid)", FTCC_ESTATE);
+}
+
+void upload_g(ftcc_ctx *c, const double *g) {
+  FTCC_CHECK_HIP(hipMemcpyAsync(c->grid.d_g, g, 8 * c->ng,
+                                hipMemcpyHostToDevice, c->stream));
+}
+
+void residual_all(ftcc_ctx *c, const double *T1, const double *T2,
+                  double *T1b, double *T2b, uint64_t skip) {
+  ensure_workspace(c);
+  const int O = c->no, V = c->nv;
+  const int64_t VO = (int64_t)V * O, S2 = (int64_t)V * V * O * O;
+  int y = 0;
+  while (y < c->ng) {
+    if (y < 64 && (skip >> y) & 1) {
+      bcast(T1b + y * VO, c->F[2], VO, 1, -1.0, 0.0, c->stream);
+      bcast(T2b + y * S2, c->I[3], S2, 1, -1.0, 0.0, c->stream);
+      ++y;
+      continue;
+    }
+    int nb = 0;
+    while (y + nb < c->ng && nb < c->nb &&
+           !((y + nb) < 64 && (skip >> (y + nb)) & 1))
+      ++nb;
+    residual_chunk(c, T1 + y * VO, T2 + y * S2, T1b + y * VO, T2b + y * S2,
+                   nb);
+    y += nb;
+  }
+}
+
+UpdateArgs upd(ftcc_ctx *c) {
+  UpdateArgs a;
+  a.ng = c->ng;
+  a.nv = c->nv;
+  a.no = c->no;
+  a.ti = c->grid.d_ti;
+  a.G = c->grid.d_G;
+  a.g = c->grid.d_g;
+  a.eo = c->eo;
+  a.ev = c->ev;
+  a.D = nullptr;
+  a.tbar = nullptr;
+  a.told = nullptr;
+  a.damp = 0.0;
+  a.t1new = nullptr;
+  a.eint = nullptr;
+  a.partials = c->partials;
+  a.src = nullptr;
+  return a;
+}
+
+void read_scalars(ftcc_ctx *c, int n) {
+  FTCC_CHECK_HIP(hipMemcpyAsync(c->host_red, c->red, sizeof(double) * n,
+                                hipMemcpyDeviceToHost, c->stream));
+  FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+  flush_profile(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ftcc_create(ftcc_ctx **out, int no, int nv, int ng, int precision,
+                void *stream) {
+  if (!out) return FTCC_EINVAL;
+  *out = nullptr;
+  ftcc_ctx *c = new ftcc_ctx();
+  int rc = guarded(c, [&] {
+    require(no > 0 && nv > 0 && ng > 0, "no, nv and ng must be positive");
+    require(precision >= 0 && precision <= 2, "unknown precision");
+    c->no = no;
+    c->nv = nv;
+    c->ng = ng;
+    c->prec = precision;
+    c->stream = (hipStream_t)stream;
+    const int64_t VO = (int64_t)nv * no;
+    // integrals
+    // split-K / GEMV partial sums: start with a modest buffer, grown on use
+    c->scr.partial_elems = (size_t)std::max<int64_t>(1 << 20, 64 * VO);
+    c->scr.partial = dalloc(c, (int64_t)c->scr.partial_elems);
+    c->partials = dalloc(c, 3 * MAX_PARTIAL_BLOCKS);
+    c->red = dalloc(c, 16);
+    FTCC_CHECK_HIP(hipHostMalloc((void **)&c->host_red, 16 * sizeof(double),
+                                 hipHostMallocDefault));
+  });
+  if (rc != FTCC_OK) {
+    fprintf(stderr, "ftcc_create: %s\n", c->err.c_str());
+    ftcc_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return FTCC_OK;
+}
+
+int ftcc_destroy(ftcc_ctx *c) {
+  if (!c) return FTCC_OK;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &p : c->prof_pending) {
+    (void)hipEventDestroy(p.e0);
+    (void)hipEventDestroy(p.e1);
+  }
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->grid.d_ti) (void)hipFree(c->grid.d_ti);
+  if (c->grid.d_g) (void)hipFree(c->grid.d_g);
+  if (c->grid.d_G) (void)hipFree(c->grid.d_G);
+  if (c->host_red) (void)hipHostFree(c->host_red);
+  delete c;
+  return FTCC_OK;
+}
+
+const char *ftcc_last_error(const ftcc_ctx *c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+int ftcc_set_precision(ftcc_ctx *c, int precision) {
+  return guarded(c, [&] {
+    require(precision >= 0 && precision <= 2, "unknown precision");
+    c->prec = precision;
+  });
+}
+
+int ftcc_set_system(ftcc_ctx *c, int n, const double *en, const double *fock,
+                    const double *eri, double beta, double mu,
+                    const int *iocc, const int *ivir, double *omega0) {
+  return guarded(c, [&] {
+    require(n > 0 && en && fock && eri, "bad system arguments");
+    require(beta > 0.0 && std::isfinite(beta), "beta must be positive");
+    const int O = c->no, V = c->nv;
+    std::vector<int> io(O), iv(V);
+    for (int k = 0; k < O; ++k) {
+      io[k] = iocc ? iocc[k] : k;
+      require(io[k] >= 0 && io[k] < n, "occupied index out of range");
+    }
+    for (int k = 0; k < V; ++k) {
+      iv[k] = ivir ? ivir[k] : k;
+      require(iv[k] >= 0 && iv[k] < n, "virtual index out of range");
+    }
+    hipStream_t s = c->stream;
+    ensure_integrals(c);
+    int *d_io = nullptr, *d_iv = nullptr;
+    double *fo = dalloc(c, 5 * (int64_t)n);   // fo fv sfo sfv gp0
+    double *wo = dalloc(c, O), *wv = dalloc(c, V);
+    FTCC_CHECK_HIP(hipMalloc(&d_io, sizeof(int) * O));
+    FTCC_CHECK_HIP(hipMalloc(&d_iv, sizeof(int) * V));
+    c->allocs.push_back(d_io);
+    c->allocs.push_back(d_iv);
+    FTCC_CHECK_HIP(hipMemcpyAsync(d_io, io.data(), sizeof(int) * O,
+                                  hipMemcpyHostToDevice, s));
+    FTCC_CHECK_HIP(hipMemcpyAsync(d_iv, iv.data(), sizeof(int) * V,
+                                  hipMemcpyHostToDevice, s));
+    double *fv = fo + n, *sfo = fo + 2 * n, *sfv = fo + 3 * n,
+           *gp0 = fo + 4 * n;
+    fermi_weights(en, n, beta, mu, fo, fv, sfo, sfv, gp0, s);
+    gather_vec(sfo, d_io, O, wo, s);
+    gather_vec(sfv, d_iv, V, wv, s);
+    gather_vec(en, d_io, O, c->eo, s);
+    gather_vec(en, d_iv, V, c->ev, s);
+    // Fock blocks oo ov vo vv
+    fock_block(fock, en, n, d_io, d_io, wo, wo, O, O, c->F[0], s);
+    fock_block(fock, en, n, d_io, d_iv, wo, wv, O, V, c->F[1], s);
+    fock_block(fock, en, n, d_iv, d_io, wv, wo, V, O, c->F[2], s);
+    fock_block(fock, en, n, d_iv, d_iv, wv, wv, V, V, c->F[3], s);
+    const char *names[9] = {"vvvv", "vvvo", "vovv", "vvoo", "vovo",
+                            "oovv", "vooo", "ooov", "oooo"};
+    for (int k = 0; k < 9; ++k) {
+      const int *p[4];
+      const double *w[4];
+      int d[4];
+      for (int a = 0; a < 4; ++a) {
+        const bool occ = names[k][a] == 'o';
+        p[a] = occ ? d_io : d_iv;
+        w[a] = occ ? wo : wv;
+        d[a] = occ ? O : V;
+      }
+      gather_weighted4(eri, n, p[0], p[1], p[2], p[3], w[0], w[1], w[2], w[3],
+                       d[0], d[1], d[2], d[3], c->I[k], s);
+    }
+    c->have_D = false;
+    c->have_energies = true;
+    build_derived(c);
+    if (omega0) {
+      const int nbk = vsum(gp0, n, c->partials, s);
+      reduce_partials(c->partials, nbk, c->red, s);
+      read_scalars(c, 1);
+      *omega0 = c->host_red[0];
+    } else {
+      FTCC_CHECK_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
+int ftcc_set_blocks(ftcc_ctx *c, const double *const F[4],
+                    const double *const I[9]) {
+  return guarded(c, [&] {
+    require(F && I, "null block pointer");
+    const int O = c->no, V = c->nv;
+    const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
+    const int64_t S2 = VV * OO;
+    const int64_t fsz[4] = {OO, VO, VO, VV};
+    const int64_t isz[9] = {VV * VV, VV * VO, VO * VV, S2, S2,
+                            S2,      V * OO * O, OO * VO, OO * OO};
+    hipStream_t s = c->stream;
+    ensure_integrals(c);
+    for (int k = 0; k < 4; ++k) {
+      require(F[k] != nullptr, "null F block");
+      FTCC_CHECK_HIP(hipMemcpyAsync(c->F[k], F[k], 8 * fsz[k],
+                                    hipMemcpyDeviceToDevice, s));
+    }
+    for (int k = 0; k < 9; ++k) {
+      require(I[k] != nullptr, "null I block");
+      FTCC_CHECK_HIP(hipMemcpyAsync(c->I[k], I[k], 8 * isz[k],
+                                    hipMemcpyDeviceToDevice, s));
+    }
+    build_derived(c);
+    FTCC_CHECK_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int ftcc_set_denominators(ftcc_ctx *c, const double *D1, const double *D2) {
+  return guarded(c, [&] {
+    require(D1 && D2, "null denominator pointer");
+    const int O = c->no, V = c->nv;
+    const int64_t VO = (int64_t)V * O, S2 = VO * VO;
+    if (!c->D1) c->D1 = dalloc(c, VO);
+    if (!c->D2) c->D2 = dalloc(c, S2);
+    FTCC_CHECK_HIP(hipMemcpyAsync(c->D1, D1, 8 * VO, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    FTCC_CHECK_HIP(hipMemcpyAsync(c->D2, D2, 8 * S2, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    c->have_D = true;
+    FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int ftcc_get_block(ftcc_ctx *c, int which, double *dst) {
+  return guarded(c, [&] {
+    require(c->have_ints, "integrals not set", FTCC_ESTATE);
+    require(which >= 0 && which < 13 && dst, "bad block request");
+    const int O = c->no, V = c->nv;
+    const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
+    const int64_t S2 = VV * OO;
+    const int64_t sz[13] = {OO, VO, VO, VV, VV * VV, VV * VO, VO * VV, S2, S2,
+                            S2, V * OO * O, OO * VO, OO * OO};
+    const double *src = which < 4 ? c->F[which] : c->I[which - 4];
+    FTCC_CHECK_HIP(hipMemcpyAsync(dst, src, 8 * sz[which],
+                                  hipMemcpyDeviceToDevice, c->stream));
+    FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int ftcc_driving_terms(ftcc_ctx *c, double *T1b, double *T2b) {
+  return guarded(c, [&] {
+    require(c->have_ints, "integrals not set", FTCC_ESTATE);
+    require(T1b && T2b, "null pointer");
+    const int64_t VO = (int64_t)c->nv * c->no;
+    bcast(T1b, c->F[2], VO, c->ng, -1.0, 0.0, c->stream);
+    bcast(T2b, c->I[3], VO * VO, c->ng, -1.0, 0.0, c->stream);
+  });
+}
+
+int ftcc_set_grid(ftcc_ctx *c, int ng, const double *ti, const double *G) {
+  return guarded(c, [&] {
+    require(ti && G, "null grid pointer");
+    require(ng > 0, "ng must be positive");
+    Grid &gr = c->grid;
+    if (gr.d_ti && ng != gr.ng) {
+      FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+      FTCC_CHECK_HIP(hipFree(gr.d_ti));
+      FTCC_CHECK_HIP(hipFree(gr.d_g));
+      FTCC_CHECK_HIP(hipFree(gr.d_G));
+      gr.d_ti = gr.d_g = gr.d_G = nullptr;
+    }
+    c->ng = ng;
+    gr.ng = ng;
+    gr.ti.assign(ti, ti + ng);
+    gr.G.assign(G, G + (size_t)ng * ng);
+    if (!gr.d_ti) {
+      FTCC_CHECK_HIP(hipMalloc(&gr.d_ti, 8 * ng));
+      FTCC_CHECK_HIP(hipMalloc(&gr.d_g, 8 * ng));
+      FTCC_CHECK_HIP(hipMalloc(&gr.d_G, 8 * (size_t)ng * ng));
+    }
+    FTCC_CHECK_HIP(hipMemcpyAsync(gr.d_ti, ti, 8 * ng, hipMemcpyHostToDevice,
+                                  c->stream));
+    FTCC_CHECK_HIP(hipMemcpyAsync(gr.d_G, G, 8 * (size_t)ng * ng,
+                                  hipMemcpyHostToDevice, c->stream));
+    FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int ftcc_residual(ftcc_ctx *c, const double *T1, const double *T2,
+                  double *T1b, double *T2b, uint64_t skip) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(T1 && T2 && T1b && T2b, "null amplitude pointer");
+    residual_all(c, T1, T2, T1b, T2b, skip);
+  });
+}
+
+int ftcc_integrate(ftcc_ctx *c, double *T1, double *T2) {
+  return guarded(c, [&] {
+    check_grid_D(c);
+    UpdateArgs a = upd(c);
+    const int64_t VO = (int64_t)c->nv * c->no;
+    // large grids: integrate from a transient copy of Tbar
+    double *copy = nullptr;
+    if (update_needs_copy(c->ng)) {
+      const int64_t elems = (int64_t)c->ng * (T2 ? VO * VO : VO);
+      FTCC_CHECK_HIP(hipMalloc(&copy, 8 * elems));
+    }
+    auto integrate_one = [&](double *T, int64_t per, bool doubles) {
+      a.tbar = T;
+      a.src = nullptr;
+      if (copy) {
+        FTCC_CHECK_HIP(hipMemcpyAsync(copy, T, 8 * per * c->ng,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        a.src = copy;
+      }
+      if (doubles) {
+        a.D = c->have_D ? c->D2 : nullptr;
+        update_t2(a, 0, c->stream);
+      } else {
+        a.D = c->have_D ? c->D1 : nullptr;
+        update_t1(a, 0, c->stream);
+      }
+    };
+    try {
+      if (T1) integrate_one(T1, VO, false);
+      if (T2) integrate_one(T2, VO * VO, true);
+    } catch (...) {
+      if (copy) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(copy);
+      }
+      throw;
+    }
+    if (copy) {
+      FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+      FTCC_CHECK_HIP(hipFree(copy));
+    }
+  });
+}
+
+int ftcc_energy(ftcc_ctx *c, const double *T1, const double *T2,
+                const double *g, double beta, int qterm, double *energy) {
+  return guarded(c, [&] {
+    require(c->have_ints, "integrals not set", FTCC_ESTATE);
+    require(c->grid.d_g != nullptr, "grid not set (ftcc_set_grid)",
+            FTCC_ESTATE);
+    require(T1 && T2 && g && energy, "null pointer");
+    require(beta > 0.0, "beta must be positive");
+    hipStream_t s = c->stream;
+    upload_g(c, g);
+    int nb = energy_t1(T1, c->grid.d_g, c->fovT, c->ng, c->nv, c->no,
+                       c->partials, s);
+    reduce_partials(c->partials, nb, c->red, s);
+    nb = energy_t2(T1, T2, c->grid.d_g, c->oovvT, c->ng, c->nv, c->no, qterm,
+                   c->partials, s);
+    reduce_partials(c->partials, nb, c->red + 3, s);
+    read_scalars(c, 6);
+    *energy = (c->host_red[2] + c->host_red[5]) / beta;
+  });
+}
+
+int ftcc_sumsq(ftcc_ctx *c, const double *x, int64_t n, double *out) {
+  return guarded(c, [&] {
+    require(x && out && n >= 0, "bad sumsq arguments");
+    const int nb = sumsq(x, n, c->partials, c->stream);
+    reduce_partials(c->partials, nb, c->red, c->stream);
+    read_scalars(c, 1);
+    *out = c->host_red[0];
+  });
+}
+
+int ftcc_step(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
+              double *T2b, const double *g, double beta, double damp,
+              uint64_t skip, double out5[5]) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(T1old && T2old && T1b && T2b && g && out5, "null pointer");
+    require(beta > 0.0, "beta must be positive");
+    hipStream_t s = c->stream;
+    upload_g(c, g);
+    residual_all(c, T1old, T2old, T1b, T2b, skip);
+    UpdateArgs a = upd(c);
+    a.damp = damp;
+    // singles first: the doubles energy needs the new T1
+    a.tbar = T1b;
+    a.told = T1old;
+    a.D = c->have_D ? c->D1 : nullptr;
+    a.eint = c->fovT;
+    int nb = update_t1(a, 1, s);
+    reduce_partials(c->partials, nb, c->red, s);
+    a.tbar = T2b;
+    a.told = T2old;
+    a.D = c->have_D ? c->D2 : nullptr;
+    a.eint = c->oovvT;
+    a.t1new = T1old;
+    nb = update_t2(a, 1, s);
+    reduce_partials(c->partials, nb, c->red + 3, s);
+    read_scalars(c, 6);
+    out5[0] = c->host_red[0];
+    out5[1] = c->host_red[3];
+    out5[2] = c->host_red[1];
+    out5[3] = c->host_red[4];
+    out5[4] = (c->host_red[2] + c->host_red[5]) / beta;
+  });
+}
+
+int ftcc_workspace_bytes(const ftcc_ctx *c, int64_t *bytes) {
+  if (!c || !bytes) return FTCC_EINVAL;
+  *bytes = c->bytes;
+  return FTCC_OK;
+}
+
+int ftcc_profile_enable(ftcc_ctx *c, int on) {
+  return guarded(c, [&] { c->prof_on = (on != 0); });
+}
+
+int ftcc_profile_read(ftcc_ctx *c, int64_t *launches, double *ms,
+                      double *flops) {
+  return guarded(c, [&] {
+    require(launches && ms && flops, "null pointer");
+    FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+    flush_profile(c);
+    *launches = c->prof_launches;
+    *ms = c->prof_ms;
+    *flops = c->prof_flops;
+    c->prof_launches = 0;
+    c->prof_ms = 0.0;
+    c->prof_flops = 0.0;
+  });
+}
+
+int ftcc_gemm(ftcc_ctx *c, int M, int N, int K, int batch, double alpha,
+              const double *A, int64_t sAm, int64_t sAk, int64_t sAb,
+              const double *B, int64_t sBk, int64_t sBn, int64_t sBb,
+              double beta, double *C, int64_t sCm, int64_t sCn, int64_t sCb,
+              int precision) {
+  return guarded(c, [&] {
+    require(M >= 0 && N >= 0 && K >= 0 && batch >= 1, "bad GEMM sizes");
+    require(A && B && C, "null GEMM operand");
+    Gemm g = mk(M, N, K, alpha, beta, A, sAm, sAk, sAb, B, sBk, sBn, sBb, C,
+                sCm, sCn, sCb, batch);
+    run(c, g, precision);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// Internal declarations shared by the libftcc translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/ftcc.h"
+
+#define FTCC_CHECK_HIP(expr)                                                  \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      throw ftcc::Error(FTCC_EHIP, std::string(#expr) + ": " +                \
+                                      hipGetErrorString(e_));                 \
+    }                                                                         \
+  } while (0)
+
+namespace ftcc {
+
+struct Error {
+  int code;
+  std::string msg;
+  Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+inline void require(bool ok, const std::string &msg, int code = FTCC_EINVAL) {
+  if (!ok) throw Error(code, msg);
+}
+
+// ---------------------------------------------------------------------------
+// Batched strided GEMM:  C = alpha * A.B + beta * C  (all float64 in memory)
+//   A(m,k) at A + b1*sA1 + b2*sA2 + m*sAm + k*sAk   (b1 < nb1, b2 < nb2)
+//   B(k,n) at B + b1*sB1 + b2*sB2 + k*sBk + n*sBn
+//   C(m,n) at C + b1*sC1 + b2*sC2 + m*sCm + n*sCn
+// If reduce_b2 is set, the nb2 products are summed into one C per b1
+// (sC2 is ignored): used for contractions whose K index is split over a
+// stride break.
+// ---------------------------------------------------------------------------
+struct Gemm {
+  int M = 0, N = 0, K = 0;
+  int nb1 = 1, nb2 = 1;
+  bool reduce_b2 = false;
+  double alpha = 1.0, beta = 0.0;
+  const double *A = nullptr;
+  int64_t sAm = 0, sAk = 0, sA1 = 0, sA2 = 0;
+  const double *B = nullptr;
+  int64_t sBk = 0, sBn = 0, sB1 = 0, sB2 = 0;
+  double *C = nullptr;
+  int64_t sCm = 0, sCn = 0, sC1 = 0, sC2 = 0;
+};
+
+// Scratch the GEMM engine may use for split-K partial sums and low-precision
+// operand copies.  Grown on demand by the context (hipMalloc outside any
+// capture).
+struct Scratch {
+  double *partial = nullptr;
+  size_t partial_elems = 0;
+  void *lowp = nullptr;
+  size_t lowp_bytes = 0;
+};
+
+void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s);
+size_t gemm_partial_elems(const Gemm &g);
+
+// ---------------------------------------------------------------------------
+// Elementwise / permutation / reduction kernels (tensor_ops.hip)
+// ---------------------------------------------------------------------------
+// out[i0*o0 + i1*o1 + i2*o2 + i3*o3] = alpha*in[i0,i1,i2,i3] + beta*out[...]
+// with `in` dense of shape (d0,d1,d2,d3); batched over nb with strides
+// in_b / out_b.
+void permute4(const double *in, double *out, int d0, int d1, int d2, int d3,
+              int64_t o0, int64_t o1, int64_t o2, int64_t o3, double alpha,
+              double beta, int nb, int64_t in_b, int64_t out_b, hipStream_t s);
+
+// out[i0,i1,i2,i3] (dense) = eri[p0[i0],p1[i1],p2[i2],p3[i3]] *
+//                            w0[i0]*w1[i1]*w2[i2]*w3[i3]
+void gather_weighted4(const double *eri, int n, const int *p0, const int *p1,
+                      const int *p2, const int *p3, const double *w0,
+                      const double *w1, const double *w2, const double *w3,
+                      int d0, int d1, int d2, int d3, double *out,
+                      hipStream_t s);
+
+// y[b][r] = alpha * sum_k M[b][r*ldm + k] * x[b][k] + beta*y[b][r]
+void gemv_rows(const double *M, int64_t ldm, int64_t sMb, const double *x,
+               int64_t sxb, double *y, int64_t syb, int rows, int K, int nb,
+               double alpha, double beta, hipStream_t s);
+// y[b][c] = alpha * sum_r x[b][r] * M[b][r*ldm + c] + beta*y[b][c]
+void gemv_cols(const double *M, int64_t ldm, int64_t sMb, const double *x,
+               int64_t sxb, double *y, int64_t syb, int rows, int cols,
+               int nb, double alpha, double beta, double *partial,
+               size_t partial_elems, hipStream_t s);
+
+// y = a*x + b*y over n elements
+void axpby(double *y, const double *x, int64_t n, double a, double b,
+           hipStream_t s);
+void fill(double *y, int64_t n, double v, hipStream_t s);
+// y[k][i] = a*x[i] + b*y[k][i] for k < nb (x shared by all batches)
+void bcast(double *y, const double *x, int64_t n, int nb, double a, double b,
+           hipStream_t s);
+
+// Fermi factors, sqrt weights and GP0 on the device (FP64)
+void fermi_weights(const double *en, int n, double beta, double mu, double *fo,
+                   double *fv, double *sfo, double *sfv, double *gp0,
+                   hipStream_t s);
+void gather_vec(const double *src, const int *idx, int n, double *dst,
+                hipStream_t s);
+// F block: out[i,j] = (fock[p[i],q[j]] - (p[i]==q[j])*en[p[i]]) * wi[i]*wj[j]
+void fock_block(const double *fock, const double *en, int n, const int *p,
+                const int *q, const double *wi, const double *wj, int di,
+                int dj, double *out, hipStream_t s);
+void denominators(const double *ev, const double *eo, int nv, int no,
+                  double *D1, double *D2, hipStream_t s);
+
+// Per-slice preparation of the amplitude-derived operands (see ftcc_api.hip)
+struct PrepOut {
+  double *tau, *taut, *t2r, *aring, *t2p, *tautp;
+};
+void prep_amplitudes(const double *t1, const double *t2, int nv, int no,
+                     int nb, const PrepOut &o, hipStream_t s);
+// W[x0,x1,x2,x3] += sgn*(Q[x0,x1,x2,x3] - Q[x0,x1,x3,x2])   (swap_last)
+// W[x0,x1,x2,x3] += sgn*(Q[x0,x1,x2,x3] - Q[x1,x0,x2,x3])   (!swap_last)
+void antisym_add(double *W, const double *Q, int d0, int d1, int d2, int d3,
+                 bool swap_last, double sgn, int nb, hipStream_t s);
+// Wr[e,m,b,j] = ovvo_ring[e,m,b,j] - Q1[b,m,e,j] + Q2[m,b,j,e]
+void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
+                   const double *Q2, int nv, int no, int nb, hipStream_t s);
+// T2bar[a,b,i,j] = -( acc + S1 - S1^(ab) + S2 - S2^(ij) + PP(Z ring) )
+void r2_final(double *t2bar, const double *S1, const double *S2,
+              const double *Z, int nv, int no, int nb, hipStream_t s);
+// x[n][c] (nb batches of n*c) -> out[c][n] transposed copy for small mats
+void transpose2(const double *in, double *out, int r, int c, int nb,
+                hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Integration, damping, norms and energy (update.hip)
+// ---------------------------------------------------------------------------
+struct Grid {
+  int ng = 0;
+  std::vector<double> ti, G;
+  double *d_ti = nullptr, *d_g = nullptr, *d_G = nullptr;
+};
+
+// mode 0: integrate in place (Tbar -> T).  mode 1: integrate Tbar, then damp
+// into Told with norms and energy partials.
+struct UpdateArgs {
+  int ng, nv, no;
+  const double *ti, *G, *g;     // device
+  const double *eo, *ev;        // device energies for D (if D == nullptr)
+  const double *D;              // explicit D1 or D2 (device) or nullptr
+  double *tbar;                 // residual in; integrated out in mode 0
+  const double *src;            // if set, Tbar is read from here (grids too
+                                // large for LDS staging integrate in place
+                                // only from a copy)
+  double *told;                 // damped iterate (mode 1)
+  double damp;
+  const double *t1new;          // mode 1 T2 pass: updated T1 (ng,nv,no)
+  const double *eint;           // energy integral in amplitude layout
+                                // (F.ov^T for T1, oovv^T for T2)
+  double *partials;             // per-block {d, n, e} triples
+};
+int update_t1(const UpdateArgs &a, int mode, hipStream_t s);   // returns #blocks
+int update_t2(const UpdateArgs &a, int mode, hipStream_t s);
+// true when the grid is too large to stage Tbar in LDS (in-place
+// integration then needs a source copy in UpdateArgs::src)
+bool update_needs_copy(int ng);
+// out[k] = sum over blocks of partials[3*b+k], k=0..2 (fixed order)
+void reduce_partials(const double *partials, int nblocks, double *out3,
+                     hipStream_t s);
+// energy of given amplitudes: per-block partials of
+// sum_y g_y [T1.F.ov^T] and sum_y g_y [(T2/4 + q T1T1/2).oovv^T]
+int energy_t2(const double *T1, const double *T2, const double *g,
+              const double *oovvT, int ng, int nv, int no, int qterm,
+              double *partials, hipStream_t s);
+int energy_t1(const double *T1, const double *g, const double *fovT, int ng,
+              int nv, int no, double *partials, hipStream_t s);
+int sumsq(const double *x, int64_t n, double *partials, hipStream_t s);
+int vsum(const double *x, int64_t n, double *partials, hipStream_t s);
+
+}  // namespace ftcc

@@ -1,0 +1,546 @@
+// Batched strided GEMM on the CDNA4 matrix cores.
+//
+// One workgroup = 4 wave64s computes a 128x128 tile of C; each wave owns a
+// 64x64 quadrant as 4x4 MFMA 16x16 blocks.  K advances in slabs of 16 staged
+// through LDS (double buffered, one barrier per slab) with the next slab
+// prefetched into registers while the current one feeds the MFMAs.
+//
+// LDS holds both operands "K-inner": As[m][k], Bs[n][k], rows padded so the
+// 16 lanes that share a k-group hit distinct banks.  Inside a slab the
+// physical k index used by lane group g (= lane/16) in MFMA step s is 4g+s, so
+// each lane fetches its four k values for the four MFMA steps with one
+// 128-bit (FP32) or two 128-bit (FP64) LDS reads; A and B use the same map, so
+// the contraction is unchanged.
+//
+// Global loads are scalar and coalesced along whichever operand dimension is
+// contiguous in memory (template flags), so every stride pattern is legal and
+// the common layouts (A k-contiguous or m-contiguous, B n-contiguous or
+// k-contiguous) stream whole cache lines.
+//
+// Precisions: FP64 (v_mfma_f64_16x16x4_f64), FP32 (v_mfma_f32_16x16x4_f32,
+// FP64 operands rounded to FP32 on load) and split-BF16 "bf16x3" (each FP32
+// value x = hi + lo with hi = bf16(x), lo = bf16(x - hi); the product keeps
+// hi*hi + hi*lo + lo*hi, accumulated in FP32 by v_mfma_f32_16x16x32_bf16).
+#include "ftcc_internal.h"
+
+#include <algorithm>
+
+namespace ftcc {
+namespace {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef double d2_t __attribute__((ext_vector_type(2)));
+
+constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+
+struct GemmDev {
+  int M, N, K, nb1, nb2, splits, kchunk, tilesM, tilesN;
+  int reduce_b2;
+  double alpha, beta;
+  const double *A;
+  int64_t sAm, sAk, sA1, sA2;
+  const double *B;
+  int64_t sBk, sBn, sB1, sB2;
+  double *C;
+  int64_t sCm, sCn, sC1, sC2;
+  double *partial;   // [R][nbC][M][N] when splits*R > 1
+  int nbC;           // number of distinct C batches (nb1 or nb1*nb2)
+};
+
+template <typename T> struct Mma;
+template <> struct Mma<double> {
+  using acc_t = d4_t;
+  static constexpr int LDK = BK + 2;   // 36-dword rows: conflict-free reads
+  // output element v of lane group lg sits in row lg + 4v of the 16x16 block
+  __device__ static int crow(int lg, int v) { return lg + 4 * v; }
+  __device__ static acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mma<float> {
+  using acc_t = f4_t;
+  static constexpr int LDK = BK + 4;   // 20-dword rows: conflict-free reads
+  // output element v of lane group lg sits in row 4*lg + v of the block
+  __device__ static int crow(int lg, int v) { return 4 * lg + v; }
+  __device__ static acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// Decode the flat block index into (tile n, tile m, split, b2, b1).
+struct BlockPos {
+  int tn, tm, ks, b1, b2;
+};
+__device__ inline BlockPos decode(const GemmDev &p) {
+  int64_t bid = blockIdx.x;
+  BlockPos q;
+  q.tn = (int)(bid % p.tilesN);
+  bid /= p.tilesN;
+  q.tm = (int)(bid % p.tilesM);
+  bid /= p.tilesM;
+  q.ks = (int)(bid % p.splits);
+  bid /= p.splits;
+  q.b2 = (int)(bid % p.nb2);
+  q.b1 = (int)(bid / p.nb2);
+  return q;
+}
+
+template <typename T, bool A_MFAST, bool B_NFAST>
+__device__ inline void load_slab(const GemmDev &p, const double *A,
+                                 const double *B, int m0, int n0, int k0,
+                                 int kend, T ra[8], T rb[8]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    int m, k;
+    if (A_MFAST) {
+      m = m0 + (t & 127);
+      k = k0 + (t >> 7) + 2 * r;
+    } else {
+      m = m0 + (t >> 4) + 16 * r;
+      k = k0 + (t & 15);
+    }
+    double v = 0.0;
+    if (m < p.M && k < kend) v = A[(int64_t)m * p.sAm + (int64_t)k * p.sAk];
+    ra[r] = (T)v;
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    int n, k;
+    if (B_NFAST) {
+      n = n0 + (t & 127);
+      k = k0 + (t >> 7) + 2 * r;
+    } else {
+      n = n0 + (t >> 4) + 16 * r;
+      k = k0 + (t & 15);
+    }
+    double v = 0.0;
+    if (n < p.N && k < kend) v = B[(int64_t)k * p.sBk + (int64_t)n * p.sBn];
+    rb[r] = (T)v;
+  }
+}
+
+template <typename T, bool A_MFAST, bool B_NFAST>
+__device__ inline void store_slab(T *As, T *Bs, const T ra[8],
+                                  const T rb[8]) {
+  constexpr int LDK = Mma<T>::LDK;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    if (A_MFAST)
+      As[(t & 127) * LDK + (t >> 7) + 2 * r] = ra[r];
+    else
+      As[((t >> 4) + 16 * r) * LDK + (t & 15)] = ra[r];
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    if (B_NFAST)
+      Bs[(t & 127) * LDK + (t >> 7) + 2 * r] = rb[r];
+    else
+      Bs[((t >> 4) + 16 * r) * LDK + (t & 15)] = rb[r];
+  }
+}
+
+template <typename T>
+__device__ inline void load4(const T *src, T out[4]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) out[s] = src[s];
+}
+
+template <typename T, bool A_MFAST, bool B_NFAST>
+__global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
+  using acc_t = typename Mma<T>::acc_t;
+  constexpr int LDK = Mma<T>::LDK;
+  __shared__ T As[2][BM * LDK];
+  __shared__ T Bs[2][BN * LDK];
+
+  const BlockPos q = decode(p);
+  const double *A = p.A + q.b1 * p.sA1 + q.b2 * p.sA2;
+  const double *B = p.B + q.b1 * p.sB1 + q.b2 * p.sB2;
+  const int m0 = q.tm * BM, n0 = q.tn * BN;
+  const int kbeg = q.ks * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
+  const int nslab = (kend - kbeg + BK - 1) / BK;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int li = lane & 15, lg = lane >> 4;
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  T ra[8], rb[8];
+  if (nslab > 0) {
+    load_slab<T, A_MFAST, B_NFAST>(p, A, B, m0, n0, kbeg, kend, ra, rb);
+    store_slab<T, A_MFAST, B_NFAST>(As[0], Bs[0], ra, rb);
+  }
+  __syncthreads();
+  for (int s = 0; s < nslab; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nslab)
+      load_slab<T, A_MFAST, B_NFAST>(p, A, B, m0, n0, kbeg + (s + 1) * BK,
+                                     kend, ra, rb);
+    T a[4][4], b[4][4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+      load4<T>(&As[buf][(wm * 64 + rt * 16 + li) * LDK + 4 * lg], a[rt]);
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+      load4<T>(&Bs[buf][(wn * 64 + ct * 16 + li) * LDK + 4 * lg], b[ct]);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+          acc[rt][ct] = Mma<T>::op(a[rt][kk], b[ct][kk], acc[rt][ct]);
+    if (s + 1 < nslab)
+      store_slab<T, A_MFAST, B_NFAST>(As[buf ^ 1], Bs[buf ^ 1], ra, rb);
+    __syncthreads();
+  }
+
+  // epilogue: lane holds column li of each 16x16 block, rows Mma<T>::crow
+  const bool direct = (p.partial == nullptr);
+  double *C = p.C + q.b1 * p.sC1 + q.b2 * p.sC2;
+  const int cb = p.reduce_b2 ? q.b1 : q.b1 * p.nb2 + q.b2;
+  const int rsel = p.reduce_b2 ? q.b2 * p.splits + q.ks : q.ks;
+  double *P = direct ? nullptr
+                     : p.partial + ((int64_t)rsel * p.nbC + cb) *
+                                       (int64_t)p.M * p.N;
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+      const int col = n0 + wn * 64 + ct * 16 + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + wm * 64 + rt * 16 + Mma<T>::crow(lg, v);
+        if (row < p.M && col < p.N) {
+          const double x = (double)acc[rt][ct][v];
+          if (direct) {
+            double *c = C + (int64_t)row * p.sCm + (int64_t)col * p.sCn;
+            *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
+          } else {
+            P[(int64_t)row * p.N + col] = x;
+          }
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bf16x3: operands split on load into (hi, lo) bf16 pairs.  MFMA
+// v_mfma_f32_16x16x32_bf16: lane l supplies A[i=l%16][k = 8*(l/16) + 0..7]
+// and B[k = 8*(l/16) + 0..7][j=l%16]; C lane layout as the 16x16x4 ops.
+// Slab depth 32; LDS rows hold 32 hi then 32 lo values (K-inner).
+// ---------------------------------------------------------------------------
+constexpr int BK3 = 32;
+constexpr int LDK3 = 2 * BK3 + 8;   // bf16 elements per LDS row (hi|lo|pad)
+
+__device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+template <bool A_MFAST, bool B_NFAST>
+__device__ inline void load_slab3(const GemmDev &p, const double *A,
+                                  const double *B, int m0, int n0, int k0,
+                                  int kend, float ra[16], float rb[16]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    int m, k;
+    if (A_MFAST) {
+      m = m0 + (t & 127);
+      k = k0 + (t >> 7) + 2 * r;
+    } else {
+      m = m0 + (t >> 5) + 8 * r;
+      k = k0 + (t & 31);
+    }
+    double v = 0.0;
+    if (m < p.M && k < kend) v = A[(int64_t)m * p.sAm + (int64_t)k * p.sAk];
+    ra[r] = (float)v;
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    int n, k;
+    if (B_NFAST) {
+      n = n0 + (t & 127);
+      k = k0 + (t >> 7) + 2 * r;
+    } else {
+      n = n0 + (t >> 5) + 8 * r;
+      k = k0 + (t & 31);
+    }
+    double v = 0.0;
+    if (n < p.N && k < kend) v = B[(int64_t)k * p.sBk + (int64_t)n * p.sBn];
+    rb[r] = (float)v;
+  }
+}
+
+template <bool A_MFAST, bool B_NFAST>
+__device__ inline void store_slab3(__bf16 *As, __bf16 *Bs, const float ra[16],
+                                   const float rb[16]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    int row, k;
+    if (A_MFAST) {
+      row = t & 127;
+      k = (t >> 7) + 2 * r;
+    } else {
+      row = (t >> 5) + 8 * r;
+      k = t & 31;
+    }
+    __bf16 hi, lo;
+    split_bf16(ra[r], hi, lo);
+    As[row * LDK3 + k] = hi;
+    As[row * LDK3 + BK3 + k] = lo;
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    int row, k;
+    if (B_NFAST) {
+      row = t & 127;
+      k = (t >> 7) + 2 * r;
+    } else {
+      row = (t >> 5) + 8 * r;
+      k = t & 31;
+    }
+    __bf16 hi, lo;
+    split_bf16(rb[r], hi, lo);
+    Bs[row * LDK3 + k] = hi;
+    Bs[row * LDK3 + BK3 + k] = lo;
+  }
+}
+
+template <bool A_MFAST, bool B_NFAST>
+__global__ __launch_bounds__(NT, 2) void gemm_bf16x3_kernel(GemmDev p) {
+  __shared__ __bf16 As[2][BM * LDK3];
+  __shared__ __bf16 Bs[2][BN * LDK3];
+  const BlockPos q = decode(p);
+  const double *A = p.A + q.b1 * p.sA1 + q.b2 * p.sA2;
+  const double *B = p.B + q.b1 * p.sB1 + q.b2 * p.sB2;
+  const int m0 = q.tm * BM, n0 = q.tn * BN;
+  const int kbeg = q.ks * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
+  const int nslab = (kend - kbeg + BK3 - 1) / BK3;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int li = lane & 15, lg = lane >> 4;
+
+  f4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0, 0, 0, 0};
+
+  float ra[16], rb[16];
+  if (nslab > 0) {
+    load_slab3<A_MFAST, B_NFAST>(p, A, B, m0, n0, kbeg, kend, ra, rb);
+    store_slab3<A_MFAST, B_NFAST>(As[0], Bs[0], ra, rb);
+  }
+  __syncthreads();
+  for (int s = 0; s < nslab; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nslab)
+      load_slab3<A_MFAST, B_NFAST>(p, A, B, m0, n0, kbeg + (s + 1) * BK3,
+                                   kend, ra, rb);
+    bf8_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const __bf16 *src = &As[buf][(wm * 64 + rt * 16 + li) * LDK3 + 8 * lg];
+      ah[rt] = *(const bf8_t *)src;
+      al[rt] = *(const bf8_t *)(src + BK3);
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+      const __bf16 *src = &Bs[buf][(wn * 64 + ct * 16 + li) * LDK3 + 8 * lg];
+      bh[ct] = *(const bf8_t *)src;
+      bl[ct] = *(const bf8_t *)(src + BK3);
+    }
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            al[rt], bh[ct], acc[rt][ct], 0, 0, 0);
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            ah[rt], bl[ct], acc[rt][ct], 0, 0, 0);
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            ah[rt], bh[ct], acc[rt][ct], 0, 0, 0);
+      }
+    if (s + 1 < nslab)
+      store_slab3<A_MFAST, B_NFAST>(As[buf ^ 1], Bs[buf ^ 1], ra, rb);
+    __syncthreads();
+  }
+
+  const bool direct = (p.partial == nullptr);
+  double *C = p.C + q.b1 * p.sC1 + q.b2 * p.sC2;
+  const int cb = p.reduce_b2 ? q.b1 : q.b1 * p.nb2 + q.b2;
+  const int rsel = p.reduce_b2 ? q.b2 * p.splits + q.ks : q.ks;
+  double *P = direct ? nullptr
+                     : p.partial + ((int64_t)rsel * p.nbC + cb) *
+                                       (int64_t)p.M * p.N;
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+      const int col = n0 + wn * 64 + ct * 16 + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + wm * 64 + rt * 16 + 4 * lg + v;
+        if (row < p.M && col < p.N) {
+          const double x = (double)acc[rt][ct][v];
+          if (direct) {
+            double *c = C + (int64_t)row * p.sCm + (int64_t)col * p.sCn;
+            *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
+          } else {
+            P[(int64_t)row * p.N + col] = x;
+          }
+        }
+      }
+    }
+}
+
+// C[b][m,n] = alpha * sum_r partial[r][b][m][n] + beta*C  (fixed order)
+__global__ void reduce_partial_kernel(const double *partial, int R, int nbC,
+                                      int M, int N, int nb2, int reduce_b2,
+                                      double alpha, double beta, double *C,
+                                      int64_t sCm, int64_t sCn, int64_t sC1,
+                                      int64_t sC2) {
+  const int64_t MN = (int64_t)M * N;
+  const int64_t total = MN * nbC;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cb = idx / MN;
+    const int64_t mn = idx % MN;
+    const int m = (int)(mn / N), n = (int)(mn % N);
+    double s = 0.0;
+    for (int r = 0; r < R; ++r) s += partial[((int64_t)r * nbC + cb) * MN + mn];
+    int64_t b1, b2;
+    if (reduce_b2) {
+      b1 = cb;
+      b2 = 0;
+    } else {
+      b1 = cb / nb2;
+      b2 = cb % nb2;
+    }
+    double *c = C + b1 * sC1 + b2 * sC2 + (int64_t)m * sCm + (int64_t)n * sCn;
+    *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
+  }
+}
+
+int choose_splits(const Gemm &g, int tiles, int bk) {
+  // enough workgroups to cover 256 CUs twice over; keep >= 4 slabs per split
+  const int64_t blocks = (int64_t)tiles * g.nb1 * g.nb2;
+  if (blocks >= 512) return 1;
+  const int maxs = std::max(1, g.K / (4 * bk));
+  int want = (int)((512 + blocks - 1) / blocks);
+  return std::max(1, std::min(want, std::min(maxs, 256)));
+}
+
+template <typename K3>
+void launch(K3 kernel, const GemmDev &d, hipStream_t s) {
+  const int64_t nblocks =
+      (int64_t)d.tilesM * d.tilesN * d.splits * d.nb1 * d.nb2;
+  hipLaunchKernelGGL(kernel, dim3((unsigned)nblocks), dim3(NT), 0, s, d);
+  FTCC_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+size_t gemm_partial_elems(const Gemm &g) {
+  const int tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+  const int bk = BK;  // conservative: smallest slab gives the most splits
+  const int splits = choose_splits(g, tiles, bk);
+  const int R = splits * (g.reduce_b2 ? g.nb2 : 1);
+  if (R <= 1) return 0;
+  const int nbC = g.reduce_b2 ? g.nb1 : g.nb1 * g.nb2;
+  return (size_t)R * nbC * g.M * g.N;
+}
+
+void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || g.nb1 <= 0 || g.nb2 <= 0) return;
+  require(precision >= 0 && precision <= 2, "unknown precision");
+  const int bk = (precision == FTCC_PREC_BF16X3) ? BK3 : BK;
+  GemmDev d;
+  d.M = g.M;
+  d.N = g.N;
+  d.K = g.K;
+  d.nb1 = g.nb1;
+  d.nb2 = g.nb2;
+  d.reduce_b2 = g.reduce_b2 ? 1 : 0;
+  d.alpha = g.alpha;
+  d.beta = g.beta;
+  d.A = g.A;
+  d.sAm = g.sAm;
+  d.sAk = g.sAk;
+  d.sA1 = g.sA1;
+  d.sA2 = g.sA2;
+  d.B = g.B;
+  d.sBk = g.sBk;
+  d.sBn = g.sBn;
+  d.sB1 = g.sB1;
+  d.sB2 = g.sB2;
+  d.C = g.C;
+  d.sCm = g.sCm;
+  d.sCn = g.sCn;
+  d.sC1 = g.sC1;
+  d.sC2 = g.sC2;
+  d.tilesM = (g.M + BM - 1) / BM;
+  d.tilesN = (g.N + BN - 1) / BN;
+  if (g.K <= 0) {
+    // C = beta*C only
+    d.splits = 1;
+    d.kchunk = 0;
+  } else {
+    d.splits = choose_splits(g, d.tilesM * d.tilesN, BK);
+    // round the chunk to whole slabs of the active precision
+    int kc = (g.K + d.splits - 1) / d.splits;
+    kc = ((kc + bk - 1) / bk) * bk;
+    d.kchunk = kc;
+    d.splits = (g.K + kc - 1) / kc;
+  }
+  const int R = d.splits * (g.reduce_b2 ? g.nb2 : 1);
+  d.nbC = g.reduce_b2 ? g.nb1 : g.nb1 * g.nb2;
+  d.partial = nullptr;
+  if (R > 1) {
+    const size_t need = (size_t)R * d.nbC * g.M * g.N;
+    require(need <= scr.partial_elems, "GEMM split-K scratch too small",
+            FTCC_ENOMEM);
+    d.partial = scr.partial;
+  }
+  const bool amf = (g.sAm == 1 && g.sAk != 1);
+  const bool bnf = (g.sBn == 1 && g.sBk != 1);
+  if (precision == FTCC_PREC_FP64) {
+    if (amf && bnf) launch(gemm_mfma_kernel<double, true, true>, d, s);
+    else if (amf) launch(gemm_mfma_kernel<double, true, false>, d, s);
+    else if (bnf) launch(gemm_mfma_kernel<double, false, true>, d, s);
+    else launch(gemm_mfma_kernel<double, false, false>, d, s);
+  } else if (precision == FTCC_PREC_FP32) {
+    if (amf && bnf) launch(gemm_mfma_kernel<float, true, true>, d, s);
+    else if (amf) launch(gemm_mfma_kernel<float, true, false>, d, s);
+    else if (bnf) launch(gemm_mfma_kernel<float, false, true>, d, s);
+    else launch(gemm_mfma_kernel<float, false, false>, d, s);
+  } else {
+    if (amf && bnf) launch(gemm_bf16x3_kernel<true, true>, d, s);
+    else if (amf) launch(gemm_bf16x3_kernel<true, false>, d, s);
+    else if (bnf) launch(gemm_bf16x3_kernel<false, true>, d, s);
+    else launch(gemm_bf16x3_kernel<false, false>, d, s);
+  }
+  if (R > 1) {
+    const int64_t total = (int64_t)d.nbC * g.M * g.N;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(reduce_partial_kernel, dim3(blocks), dim3(256), 0, s,
+                       d.partial, R, d.nbC, g.M, g.N, g.nb2, d.reduce_b2,
+                       g.alpha, g.beta, g.C, g.sCm, g.sCn, g.sC1, g.sC2);
+    FTCC_CHECK_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace ftcc

@@ -1,0 +1,458 @@
+// HBM-bound helper kernels of the residual: permutations, Fermi weights,
+// GEMVs and the antisymmetrising assembly of the doubles residual.
+// All are grid-stride loops over the element range with 64-bit indexing.
+#include "ftcc_internal.h"
+
+#include <algorithm>
+#include <math.h>
+
+namespace ftcc {
+namespace {
+
+constexpr int TPB = 256;
+
+inline int blocks_for(int64_t n, int cap = 1 << 20) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + TPB - 1) / TPB, cap));
+}
+
+#define GRID_STRIDE(i, n)                                                   \
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); \
+       i += (int64_t)gridDim.x * blockDim.x)
+
+__global__ void permute4_kernel(const double *in, double *out, int d0, int d1,
+                                int d2, int d3, int64_t o0, int64_t o1,
+                                int64_t o2, int64_t o3, double alpha,
+                                double beta, int nb, int64_t in_b,
+                                int64_t out_b) {
+  const int64_t per = (int64_t)d0 * d1 * d2 * d3;
+  GRID_STRIDE(idx, per * nb) {
+    const int64_t b = idx / per;
+    int64_t r = idx % per;
+    const int i3 = (int)(r % d3);
+    r /= d3;
+    const int i2 = (int)(r % d2);
+    r /= d2;
+    const int i1 = (int)(r % d1);
+    const int i0 = (int)(r / d1);
+    double *o = out + b * out_b + i0 * o0 + i1 * o1 + i2 * o2 + i3 * o3;
+    const double v = alpha * in[b * in_b + (idx % per)];
+    *o = (beta == 0.0) ? v : v + beta * *o;
+  }
+}
+
+__global__ void gather_weighted4_kernel(const double *eri, int n,
+                                        const int *p0, const int *p1,
+                                        const int *p2, const int *p3,
+                                        const double *w0, const double *w1,
+                                        const double *w2, const double *w3,
+                                        int d0, int d1, int d2, int d3,
+                                        double *out) {
+  const int64_t total = (int64_t)d0 * d1 * d2 * d3;
+  GRID_STRIDE(idx, total) {
+    int64_t r = idx;
+    const int i3 = (int)(r % d3);
+    r /= d3;
+    const int i2 = (int)(r % d2);
+    r /= d2;
+    const int i1 = (int)(r % d1);
+    const int i0 = (int)(r / d1);
+    const int64_t src =
+        (((int64_t)p0[i0] * n + p1[i1]) * n + p2[i2]) * n + p3[i3];
+    out[idx] = eri[src] * w0[i0] * w1[i1] * w2[i2] * w3[i3];
+  }
+}
+
+// one wave64 per row; deterministic shuffle reduction
+__global__ void gemv_rows_kernel(const double *M, int64_t ldm, int64_t sMb,
+                                 const double *x, int64_t sxb, double *y,
+                                 int64_t syb, int rows, int K, int nb,
+                                 double alpha, double beta) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t w = wave; w < (int64_t)rows * nb; w += nwaves) {
+    const int64_t b = w / rows, r = w % rows;
+    const double *row = M + b * sMb + r * ldm;
+    const double *xb = x + b * sxb;
+    double s = 0.0;
+    for (int k = lane; k < K; k += 64) s += row[k] * xb[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) {
+      double *o = y + b * syb + r;
+      *o = (beta == 0.0) ? alpha * s : alpha * s + beta * *o;
+    }
+  }
+}
+
+// partial[chunk][b][c] = sum_{r in chunk} x[b][r] * M[b][r*ldm + c]
+__global__ void gemv_cols_partial_kernel(const double *M, int64_t ldm,
+                                         int64_t sMb, const double *x,
+                                         int64_t sxb, int rows, int cols,
+                                         int nb, int rchunk, double *partial) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int chunk = blockIdx.y;
+  const int b = blockIdx.z;
+  if (c >= cols) return;
+  const int r0 = chunk * rchunk, r1 = min(rows, r0 + rchunk);
+  const double *Mb = M + b * sMb;
+  const double *xb = x + b * sxb;
+  double s = 0.0;
+  for (int r = r0; r < r1; ++r) s += xb[r] * Mb[(int64_t)r * ldm + c];
+  partial[((int64_t)chunk * nb + b) * cols + c] = s;
+}
+
+__global__ void gemv_cols_reduce_kernel(const double *partial, int nchunk,
+                                        int cols, int nb, double *y,
+                                        int64_t syb, double alpha,
+                                        double beta) {
+  GRID_STRIDE(idx, (int64_t)cols * nb) {
+    const int64_t b = idx / cols, c = idx % cols;
+    double s = 0.0;
+    for (int k = 0; k < nchunk; ++k) s += partial[((int64_t)k * nb + b) * cols + c];
+    double *o = y + b * syb + c;
+    *o = (beta == 0.0) ? alpha * s : alpha * s + beta * *o;
+  }
+}
+
+__global__ void axpby_kernel(double *y, const double *x, int64_t n, double a,
+                             double b) {
+  GRID_STRIDE(i, n) y[i] = (b == 0.0) ? a * x[i] : a * x[i] + b * y[i];
+}
+
+__global__ void fill_kernel(double *y, int64_t n, double v) {
+  GRID_STRIDE(i, n) y[i] = v;
+}
+
+__global__ void bcast_kernel(double *y, const double *x, int64_t n, int nb,
+                             double a, double b) {
+  GRID_STRIDE(i, n * nb) {
+    const double v = a * x[i % n];
+    y[i] = (b == 0.0) ? v : v + b * y[i];
+  }
+}
+
+// fo = 1/(1+e^{x}), fv = 1/(1+e^{-x}), x = beta(e-mu), each evaluated with
+// the exponential of a non-positive argument; GP0 = -(max(-x,0) +
+// log1p(e^{-|x|}))/beta.
+__global__ void fermi_kernel(const double *en, int n, double beta, double mu,
+                             double *fo, double *fv, double *sfo, double *sfv,
+                             double *gp0) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double x = beta * (en[p] - mu);
+  const double ex = exp(-fabs(x));
+  double o, v;
+  if (x > 0) {
+    o = ex / (1.0 + ex);
+    v = 1.0 / (1.0 + ex);
+  } else {
+    o = 1.0 / (1.0 + ex);
+    v = ex / (1.0 + ex);
+  }
+  fo[p] = o;
+  fv[p] = v;
+  sfo[p] = sqrt(o);
+  sfv[p] = sqrt(v);
+  gp0[p] = -(fmax(-x, 0.0) + log1p(ex)) / beta;
+}
+
+__global__ void gather_vec_kernel(const double *src, const int *idx, int n,
+                                  double *dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ void fock_block_kernel(const double *fock, const double *en, int n,
+                                  const int *p, const int *q,
+                                  const double *wi, const double *wj, int di,
+                                  int dj, double *out) {
+  GRID_STRIDE(idx, (int64_t)di * dj) {
+    const int i = (int)(idx / dj), j = (int)(idx % dj);
+    const int pi = p[i], qj = q[j];
+    double f = fock[(int64_t)pi * n + qj];
+    if (pi == qj) f -= en[pi];
+    out[idx] = f * wi[i] * wj[j];
+  }
+}
+
+__global__ void denominators_kernel(const double *ev, const double *eo, int nv,
+                                    int no, double *D1, double *D2) {
+  const int64_t total = (int64_t)nv * nv * no * no;
+  GRID_STRIDE(idx, total) {
+    int64_t r = idx;
+    const int j = (int)(r % no);
+    r /= no;
+    const int i = (int)(r % no);
+    r /= no;
+    const int b = (int)(r % nv);
+    const int a = (int)(r / nv);
+    D2[idx] = ev[a] + ev[b] - eo[i] - eo[j];
+    if (b == 0 && j == 0) D1[(int64_t)a * no + i] = ev[a] - eo[i];
+  }
+}
+
+// One thread per t2 element (a,b,i,j) of each slice:
+//   tau   [a,b,i,j] = t2 + t1[a,i]t1[b,j] - t1[b,i]t1[a,j]
+//   taut  [a,b,i,j] = t2 + (t1[a,i]t1[b,j] - t1[b,i]t1[a,j])/2
+//   t2r   [a,i,b,j] = t2[a,b,i,j]                    (ring layout)
+//   aring [a,i,b,j] = -t2[a,b,i,j]/2 + t1[a,j]t1[b,i]
+//   t2p   [j,a,b,i] = t2[a,b,i,j]
+//   tautp [j,a,b,i] = taut[a,b,i,j]
+__global__ void prep_kernel(const double *t1s, const double *t2s, int nv,
+                            int no, int nb, double *tau, double *taut,
+                            double *t2r, double *aring, double *t2p,
+                            double *tautp) {
+  const int64_t per = (int64_t)nv * nv * no * no;
+  const int64_t per1 = (int64_t)nv * no;
+  GRID_STRIDE(idx, per * nb) {
+    const int64_t y = idx / per;
+    const int64_t e = idx % per;
+    int64_t r = e;
+    const int j = (int)(r % no);
+    r /= no;
+    const int i = (int)(r % no);
+    r /= no;
+    const int b = (int)(r % nv);
+    const int a = (int)(r / nv);
+    const double *t1 = t1s + y * per1;
+    const double v = t2s[y * per + e];
+    const double x = t1[a * no + i] * t1[b * no + j] -
+                     t1[b * no + i] * t1[a * no + j];
+    const double tt = v + 0.5 * x;
+    tau[y * per + e] = v + x;
+    taut[y * per + e] = tt;
+    const int64_t ring = (((int64_t)a * no + i) * nv + b) * no + j;
+    t2r[y * per + ring] = v;
+    aring[y * per + ring] = -0.5 * v + t1[a * no + j] * t1[b * no + i];
+    const int64_t pj = (((int64_t)j * nv + a) * nv + b) * no + i;
+    t2p[y * per + pj] = v;
+    tautp[y * per + pj] = tt;
+  }
+}
+
+__global__ void antisym_add_kernel(double *W, const double *Q, int d0, int d1,
+                                   int d2, int d3, int swap_last, double sgn,
+                                   int nb) {
+  const int64_t per = (int64_t)d0 * d1 * d2 * d3;
+  GRID_STRIDE(idx, per * nb) {
+    const int64_t y = idx / per;
+    int64_t r = idx % per;
+    const int x3 = (int)(r % d3);
+    r /= d3;
+    const int x2 = (int)(r % d2);
+    r /= d2;
+    const int x1 = (int)(r % d1);
+    const int x0 = (int)(r / d1);
+    int64_t other;
+    if (swap_last)
+      other = (((int64_t)x0 * d1 + x1) * d2 + x3) * d3 + x2;
+    else
+      other = (((int64_t)x1 * d1 + x0) * d2 + x2) * d3 + x3;
+    W[idx] += sgn * (Q[idx] - Q[y * per + other]);
+  }
+}
+
+// Wr[e,m,b,j] = ovvo_ring[e,m,b,j] - Q1[b,m,e,j] + Q2[m,b,j,e]
+__global__ void ring_assemble_kernel(double *Wr, const double *ovvo_ring,
+                                     const double *Q1, const double *Q2,
+                                     int nv, int no, int nb) {
+  const int64_t per = (int64_t)nv * no * nv * no;
+  GRID_STRIDE(idx, per * nb) {
+    const int64_t y = idx / per;
+    const int64_t w = idx % per;
+    int64_t r = w;
+    const int j = (int)(r % no);
+    r /= no;
+    const int b = (int)(r % nv);
+    r /= nv;
+    const int m = (int)(r % no);
+    const int e = (int)(r / no);
+    const int64_t q1 = (((int64_t)b * no + m) * nv + e) * no + j;
+    const int64_t q2 = (((int64_t)m * nv + b) * no + j) * nv + e;
+    Wr[idx] = ovvo_ring[w] - Q1[y * per + q1] + Q2[y * per + q2];
+  }
+}
+
+// t2bar[a,b,i,j] = -( acc[a,b,i,j]            (already in t2bar)
+//                     + S1[a,b,i,j] - S1[b,a,i,j]
+//                     + S2[a,b,i,j] - S2[a,b,j,i]
+//                     + Z[a,i,b,j] - Z[b,i,a,j] - Z[a,j,b,i] + Z[b,j,a,i] )
+__global__ void r2_final_kernel(double *t2bar, const double *S1,
+                                const double *S2, const double *Z, int nv,
+                                int no, int nb) {
+  const int64_t per = (int64_t)nv * nv * no * no;
+  GRID_STRIDE(idx, per * nb) {
+    const int64_t y = idx / per;
+    int64_t r = idx % per;
+    const int j = (int)(r % no);
+    r /= no;
+    const int i = (int)(r % no);
+    r /= no;
+    const int b = (int)(r % nv);
+    const int a = (int)(r / nv);
+    const double *s1 = S1 + y * per;
+    const double *s2 = S2 + y * per;
+    const double *z = Z + y * per;
+    auto nat = [&](int p, int q, int k, int l) {
+      return (((int64_t)p * nv + q) * no + k) * no + l;
+    };
+    auto ring = [&](int p, int k, int q, int l) {
+      return (((int64_t)p * no + k) * nv + q) * no + l;
+    };
+    double v = t2bar[idx];
+    v += s1[nat(a, b, i, j)] - s1[nat(b, a, i, j)];
+    v += s2[nat(a, b, i, j)] - s2[nat(a, b, j, i)];
+    v += z[ring(a, i, b, j)] - z[ring(b, i, a, j)] - z[ring(a, j, b, i)] +
+         z[ring(b, j, a, i)];
+    t2bar[idx] = -v;
+  }
+}
+
+__global__ void transpose2_kernel(const double *in, double *out, int r, int c,
+                                  int nb) {
+  const int64_t per = (int64_t)r * c;
+  GRID_STRIDE(idx, per * nb) {
+    const int64_t b = idx / per;
+    const int64_t e = idx % per;
+    const int i = (int)(e / c), j = (int)(e % c);
+    out[b * per + (int64_t)j * r + i] = in[idx];
+  }
+}
+
+}  // namespace
+
+#define LAUNCH(kernel, n, ...)                                            \
+  do {                                                                    \
+    hipLaunchKernelGGL(kernel, dim3(blocks_for(n)), dim3(TPB), 0, s,      \
+                       __VA_ARGS__);                                      \
+    FTCC_CHECK_HIP(hipGetLastError());                                    \
+  } while (0)
+
+void permute4(const double *in, double *out, int d0, int d1, int d2, int d3,
+              int64_t o0, int64_t o1, int64_t o2, int64_t o3, double alpha,
+              double beta, int nb, int64_t in_b, int64_t out_b,
+              hipStream_t s) {
+  const int64_t n = (int64_t)d0 * d1 * d2 * d3 * nb;
+  if (n == 0) return;
+  LAUNCH(permute4_kernel, n, in, out, d0, d1, d2, d3, o0, o1, o2, o3, alpha,
+         beta, nb, in_b, out_b);
+}
+
+void gather_weighted4(const double *eri, int n, const int *p0, const int *p1,
+                      const int *p2, const int *p3, const double *w0,
+                      const double *w1, const double *w2, const double *w3,
+                      int d0, int d1, int d2, int d3, double *out,
+                      hipStream_t s) {
+  const int64_t total = (int64_t)d0 * d1 * d2 * d3;
+  if (total == 0) return;
+  LAUNCH(gather_weighted4_kernel, total, eri, n, p0, p1, p2, p3, w0, w1, w2,
+         w3, d0, d1, d2, d3, out);
+}
+
+void gemv_rows(const double *M, int64_t ldm, int64_t sMb, const double *x,
+               int64_t sxb, double *y, int64_t syb, int rows, int K, int nb,
+               double alpha, double beta, hipStream_t s) {
+  const int64_t waves = (int64_t)rows * nb;
+  if (waves == 0) return;
+  const int blocks = (int)std::min<int64_t>((waves + 3) / 4, 1 << 16);
+  hipLaunchKernelGGL(gemv_rows_kernel, dim3(blocks), dim3(TPB), 0, s, M, ldm,
+                     sMb, x, sxb, y, syb, rows, K, nb, alpha, beta);
+  FTCC_CHECK_HIP(hipGetLastError());
+}
+
+void gemv_cols(const double *M, int64_t ldm, int64_t sMb, const double *x,
+               int64_t sxb, double *y, int64_t syb, int rows, int cols,
+               int nb, double alpha, double beta, double *partial,
+               size_t partial_elems, hipStream_t s) {
+  if (rows == 0 || cols == 0 || nb == 0) return;
+  const int cblocks = (cols + TPB - 1) / TPB;
+  // aim for ~2048 workgroups in total, at least 64 rows per chunk
+  int nchunk = (int)std::max<int64_t>(
+      1, std::min<int64_t>(rows / 64, 2048 / std::max<int64_t>(1, (int64_t)cblocks * nb)));
+  nchunk = std::max(1, std::min(nchunk, 1024));
+  while ((size_t)nchunk * nb * cols > partial_elems && nchunk > 1) nchunk /= 2;
+  require((size_t)nchunk * nb * cols <= partial_elems,
+          "gemv scratch too small", FTCC_ENOMEM);
+  const int rchunk = (rows + nchunk - 1) / nchunk;
+  nchunk = (rows + rchunk - 1) / rchunk;
+  hipLaunchKernelGGL(gemv_cols_partial_kernel, dim3(cblocks, nchunk, nb),
+                     dim3(TPB), 0, s, M, ldm, sMb, x, sxb, rows, cols, nb,
+                     rchunk, partial);
+  FTCC_CHECK_HIP(hipGetLastError());
+  LAUNCH(gemv_cols_reduce_kernel, (int64_t)cols * nb, partial, nchunk, cols,
+         nb, y, syb, alpha, beta);
+}
+
+void axpby(double *y, const double *x, int64_t n, double a, double b,
+           hipStream_t s) {
+  if (n == 0) return;
+  LAUNCH(axpby_kernel, n, y, x, n, a, b);
+}
+
+void fill(double *y, int64_t n, double v, hipStream_t s) {
+  if (n == 0) return;
+  LAUNCH(fill_kernel, n, y, n, v);
+}
+
+void bcast(double *y, const double *x, int64_t n, int nb, double a, double b,
+           hipStream_t s) {
+  if (n * nb == 0) return;
+  LAUNCH(bcast_kernel, n * nb, y, x, n, nb, a, b);
+}
+
+void fermi_weights(const double *en, int n, double beta, double mu, double *fo,
+                   double *fv, double *sfo, double *sfv, double *gp0,
+                   hipStream_t s) {
+  LAUNCH(fermi_kernel, n, en, n, beta, mu, fo, fv, sfo, sfv, gp0);
+}
+
+void gather_vec(const double *src, const int *idx, int n, double *dst,
+                hipStream_t s) {
+  if (n == 0) return;
+  LAUNCH(gather_vec_kernel, n, src, idx, n, dst);
+}
+
+void fock_block(const double *fock, const double *en, int n, const int *p,
+                const int *q, const double *wi, const double *wj, int di,
+                int dj, double *out, hipStream_t s) {
+  if ((int64_t)di * dj == 0) return;
+  LAUNCH(fock_block_kernel, (int64_t)di * dj, fock, en, n, p, q, wi, wj, di,
+         dj, out);
+}
+
+void denominators(const double *ev, const double *eo, int nv, int no,
+                  double *D1, double *D2, hipStream_t s) {
+  LAUNCH(denominators_kernel, (int64_t)nv * nv * no * no, ev, eo, nv, no, D1,
+         D2);
+}
+
+void prep_amplitudes(const double *t1, const double *t2, int nv, int no,
+                     int nb, const PrepOut &o, hipStream_t s) {
+  LAUNCH(prep_kernel, (int64_t)nv * nv * no * no * nb, t1, t2, nv, no, nb,
+         o.tau, o.taut, o.t2r, o.aring, o.t2p, o.tautp);
+}
+
+void antisym_add(double *W, const double *Q, int d0, int d1, int d2, int d3,
+                 bool swap_last, double sgn, int nb, hipStream_t s) {
+  LAUNCH(antisym_add_kernel, (int64_t)d0 * d1 * d2 * d3 * nb, W, Q, d0, d1,
+         d2, d3, swap_last ? 1 : 0, sgn, nb);
+}
+
+void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
+                   const double *Q2, int nv, int no, int nb, hipStream_t s) {
+  LAUNCH(ring_assemble_kernel, (int64_t)nv * no * nv * no * nb, Wr,
+         ovvo_ring, Q1, Q2, nv, no, nb);
+}
+
+void r2_final(double *t2bar, const double *S1, const double *S2,
+              const double *Z, int nv, int no, int nb, hipStream_t s) {
+  LAUNCH(r2_final_kernel, (int64_t)nv * nv * no * no * nb, t2bar, S1, S2, Z,
+         nv, no, nb);
+}
+
+void transpose2(const double *in, double *out, int r, int c, int nb,
+                hipStream_t s) {
+  LAUNCH(transpose2_kernel, (int64_t)r * c * nb, in, out, r, c, nb);
+}
+
+}  // namespace ftcc

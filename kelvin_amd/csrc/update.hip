@@ -1,0 +1,288 @@
+// exp(-tau D) integration, damping, norms and grand-potential partials.
+//
+// One thread owns one amplitude element across all ng grid points: it stages
+// Tbar[0..ng) in its own LDS column, forms
+//     T(tau_y) = sum_x G[y,x] * exp(dt_yx * D) * Tbar(tau_x),
+//     dt_yx = tau_x - tau_y for x < y and 0 otherwise
+// (the convention of kelvin/quadrature.py:292-317), and in the fused mode
+// damps it into the stored iterate while accumulating |T - Told|^2,
+// |Told_new|^2 and the energy contraction (kelvin/cc_utils.py:145-155,
+// kelvin/ft_cc_energy.py:7-32).  Everything is FP64.  Per-block partial sums
+// are reduced by a single fixed-order pass, so results are deterministic.
+#include "ftcc_internal.h"
+
+#include <algorithm>
+#include <math.h>
+
+namespace ftcc {
+namespace {
+
+constexpr int UT = 128;          // threads per block of the update kernels
+constexpr int MAX_LDS_NG = 64;   // grids up to this size stage Tbar in LDS
+constexpr int MAX_BLOCKS = 8192;
+
+__device__ inline void block_reduce3(double a, double b, double c,
+                                     double *out) {
+  __shared__ double red[3][UT / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+    c += __shfl_xor(c, off, 64);
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+    red[2][w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int i = 0; i < UT / 64; ++i) {
+      s0 += red[0][i];
+      s1 += red[1][i];
+      s2 += red[2][i];
+    }
+    out[3 * blockIdx.x + 0] = s0;
+    out[3 * blockIdx.x + 1] = s1;
+    out[3 * blockIdx.x + 2] = s2;
+  }
+}
+
+// Element decode: T2 element (a,b,i,j) or T1 element (a,i) (b=j=-1).
+struct Elem {
+  int a, b, i, j;
+};
+
+template <bool DOUBLES>
+__device__ inline Elem decode(int64_t e, int nv, int no) {
+  Elem r;
+  if (DOUBLES) {
+    r.j = (int)(e % no);
+    e /= no;
+    r.i = (int)(e % no);
+    e /= no;
+    r.b = (int)(e % nv);
+    r.a = (int)(e / nv);
+  } else {
+    r.i = (int)(e % no);
+    r.a = (int)(e / no);
+    r.b = r.j = -1;
+  }
+  return r;
+}
+
+template <bool DOUBLES, bool STAGE_LDS>
+__global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
+  extern __shared__ double col[];   // [ng][UT] when STAGE_LDS
+  const int ng = p.ng, nv = p.nv, no = p.no;
+  const int64_t per = DOUBLES ? (int64_t)nv * nv * no * no : (int64_t)nv * no;
+  const int64_t per1 = (int64_t)nv * no;
+  double dsum = 0.0, nsum = 0.0, esum = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
+       e += (int64_t)gridDim.x * UT) {
+    const Elem q = decode<DOUBLES>(e, nv, no);
+    double d;
+    if (p.D)
+      d = p.D[e];
+    else if (DOUBLES)
+      d = p.ev[q.a] + p.ev[q.b] - p.eo[q.i] - p.eo[q.j];
+    else
+      d = p.ev[q.a] - p.eo[q.i];
+    if (STAGE_LDS)
+      for (int x = 0; x < ng; ++x) col[x * UT + threadIdx.x] = p.tbar[x * per + e];
+    double eacc = 0.0;
+    for (int y = 0; y < ng; ++y) {
+      double tn = 0.0;
+      const double *Gy = p.G + (int64_t)y * ng;
+      const double ty = p.ti[y];
+      for (int x = 0; x < ng; ++x) {
+        const double gyx = Gy[x];
+        if (gyx == 0.0) continue;
+        const double tb = STAGE_LDS ? col[x * UT + threadIdx.x]
+                                    : (p.src ? p.src : p.tbar)[x * per + e];
+        const double w = (x < y) ? exp((p.ti[x] - ty) * d) : 1.0;
+        tn += gyx * (w * tb);
+      }
+      if (mode == 0) {
+        p.tbar[y * per + e] = tn;
+      } else {
+        const double to = p.told[y * per + e];
+        const double df = tn - to;
+        dsum += df * df;
+        const double tnew = p.damp * to + (1.0 - p.damp) * tn;
+        p.told[y * per + e] = tnew;
+        nsum += tnew * tnew;
+        if (DOUBLES) {
+          const double *t1 = p.t1new + y * per1;
+          const double qq = t1[q.a * no + q.i] * t1[q.b * no + q.j];
+          eacc += p.g[y] * (0.25 * tnew + 0.5 * qq);
+        } else {
+          eacc += p.g[y] * tnew;
+        }
+      }
+    }
+    if (mode == 1) esum += eacc * p.eint[e];
+  }
+  if (mode == 1) block_reduce3(dsum, nsum, esum, p.partials);
+}
+
+template <bool DOUBLES>
+int launch_update(const UpdateArgs &a, int mode, hipStream_t s) {
+  const int64_t per = DOUBLES ? (int64_t)a.nv * a.nv * a.no * a.no
+                              : (int64_t)a.nv * a.no;
+  const int blocks =
+      (int)std::max<int64_t>(1, std::min<int64_t>((per + UT - 1) / UT, MAX_BLOCKS));
+  if (a.ng <= MAX_LDS_NG) {
+    const size_t lds = sizeof(double) * a.ng * UT;
+    hipLaunchKernelGGL((update_kernel<DOUBLES, true>), dim3(blocks), dim3(UT),
+                       lds, s, a, mode);
+  } else {
+    hipLaunchKernelGGL((update_kernel<DOUBLES, false>), dim3(blocks),
+                       dim3(UT), 0, s, a, mode);
+  }
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
+__global__ void reduce_partials_kernel(const double *partials, int nblocks,
+                                       double *out3) {
+  __shared__ double red[3][256];
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) {
+    s0 += partials[3 * b];
+    s1 += partials[3 * b + 1];
+    s2 += partials[3 * b + 2];
+  }
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  red[2][threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+      red[2][threadIdx.x] += red[2][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out3[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// partials: {0, 0, energy} so that the same reducer applies
+__global__ __launch_bounds__(UT) void energy_t2_kernel(
+    const double *T1, const double *T2, const double *g, const double *oovvT,
+    int ng, int nv, int no, int qterm, double *partials) {
+  const int64_t per = (int64_t)nv * nv * no * no;
+  const int64_t per1 = (int64_t)nv * no;
+  double esum = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
+       e += (int64_t)gridDim.x * UT) {
+    const Elem q = decode<true>(e, nv, no);
+    double acc = 0.0;
+    for (int y = 0; y < ng; ++y) {
+      double v = 0.25 * T2[y * per + e];
+      if (qterm) {
+        const double *t1 = T1 + y * per1;
+        v += 0.5 * t1[q.a * no + q.i] * t1[q.b * no + q.j];
+      }
+      acc += g[y] * v;
+    }
+    esum += acc * oovvT[e];
+  }
+  block_reduce3(0.0, 0.0, esum, partials);
+}
+
+__global__ __launch_bounds__(UT) void energy_t1_kernel(const double *T1,
+                                                       const double *g,
+                                                       const double *fovT,
+                                                       int ng, int nv, int no,
+                                                       double *partials) {
+  const int64_t per = (int64_t)nv * no;
+  double esum = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
+       e += (int64_t)gridDim.x * UT) {
+    double acc = 0.0;
+    for (int y = 0; y < ng; ++y) acc += g[y] * T1[y * per + e];
+    esum += acc * fovT[e];
+  }
+  block_reduce3(0.0, 0.0, esum, partials);
+}
+
+__global__ __launch_bounds__(UT) void sumsq_kernel(const double *x, int64_t n,
+                                                   double *partials) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)UT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * UT)
+    s += x[i] * x[i];
+  block_reduce3(s, 0.0, 0.0, partials);
+}
+
+__global__ __launch_bounds__(UT) void vsum_kernel(const double *x, int64_t n,
+                                                  double *partials) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)UT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * UT)
+    s += x[i];
+  block_reduce3(s, 0.0, 0.0, partials);
+}
+
+int nblocks_for(int64_t n) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + UT - 1) / UT, MAX_BLOCKS));
+}
+
+}  // namespace
+
+bool update_needs_copy(int ng) { return ng > MAX_LDS_NG; }
+
+int update_t1(const UpdateArgs &a, int mode, hipStream_t s) {
+  return launch_update<false>(a, mode, s);
+}
+
+int update_t2(const UpdateArgs &a, int mode, hipStream_t s) {
+  return launch_update<true>(a, mode, s);
+}
+
+void reduce_partials(const double *partials, int nblocks, double *out3,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s,
+                     partials, nblocks, out3);
+  FTCC_CHECK_HIP(hipGetLastError());
+}
+
+int energy_t2(const double *T1, const double *T2, const double *g,
+              const double *oovvT, int ng, int nv, int no, int qterm,
+              double *partials, hipStream_t s) {
+  const int blocks = nblocks_for((int64_t)nv * nv * no * no);
+  hipLaunchKernelGGL(energy_t2_kernel, dim3(blocks), dim3(UT), 0, s, T1, T2,
+                     g, oovvT, ng, nv, no, qterm, partials);
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
+int energy_t1(const double *T1, const double *g, const double *fovT, int ng,
+              int nv, int no, double *partials, hipStream_t s) {
+  const int blocks = nblocks_for((int64_t)nv * no);
+  hipLaunchKernelGGL(energy_t1_kernel, dim3(blocks), dim3(UT), 0, s, T1, g,
+                     fovT, ng, nv, no, partials);
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
+int sumsq(const double *x, int64_t n, double *partials, hipStream_t s) {
+  const int blocks = nblocks_for(n);
+  hipLaunchKernelGGL(sumsq_kernel, dim3(blocks), dim3(UT), 0, s, x, n,
+                     partials);
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
+int vsum(const double *x, int64_t n, double *partials, hipStream_t s) {
+  const int blocks = nblocks_for(n);
+  hipLaunchKernelGGL(vsum_kernel, dim3(blocks), dim3(UT), 0, s, x, n,
+                     partials);
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
+}  // namespace ftcc

@@ -1,0 +1,267 @@
+"""FTEngine: one libftcc context bound to one GPU and torch's current stream.
+
+Device memory for amplitudes is owned by torch tensors (float64, C order,
+the reference's layouts T1[y,a,i], T2[y,a,b,i,j]); integrals and residual
+workspace are owned by the library context.  Every method calls through the
+C ABI of include/ftcc.h; nothing here computes on the host.
+"""
+import ctypes
+
+import numpy
+import torch
+
+from . import _lib
+from ._lib import FTCCError, PREC
+
+BLOCK_INDEX = {"oo": 0, "ov": 1, "vo": 2, "vv": 3, "vvvv": 4, "vvvo": 5,
+               "vovv": 6, "vvoo": 7, "vovo": 8, "oovv": 9, "vooo": 10,
+               "ooov": 11, "oooo": 12}
+TWO_E = ("vvvv", "vvvo", "vovv", "vvoo", "vovo", "oovv", "vooo", "ooov",
+         "oooo")
+ONE_E = ("oo", "ov", "vo", "vv")
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _dbuf(x):
+    a = numpy.ascontiguousarray(numpy.asarray(x, dtype=numpy.float64))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_device():
+    if not torch.cuda.is_available():
+        raise FTCCError("kelvin_amd needs a ROCm GPU (torch.cuda is not "
+                        "available); there is no CPU path")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def block_shape(name, no, nv):
+    return tuple(no if ch == 'o' else nv for ch in name)
+
+
+class FTEngine:
+    """Finite-temperature CCSD amplitude engine for (no, nv, ng)."""
+
+    def __init__(self, no, nv, ng, precision="fp64", device=None):
+        if precision not in PREC:
+            raise ValueError("precision must be one of %s" % sorted(PREC))
+        self.no, self.nv, self.ng = int(no), int(nv), int(ng)
+        self.precision = precision
+        self.device = torch.device(device) if device is not None \
+            else default_device()
+        self._lib = _lib.load()
+        self._h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            rc = self._lib.ftcc_create(ctypes.byref(self._h), self.no,
+                                       self.nv, self.ng, PREC[precision],
+                                       ctypes.c_void_p(stream))
+        if rc != 0:
+            raise FTCCError("ftcc_create failed (code %d)" % rc)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.ftcc_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        with torch.cuda.device(self.device):
+            rc = getattr(self._lib, name)(self._h, *args)
+        _lib.check(rc, self._h, name)
+
+    def _check(self, t, shape, name):
+        if not isinstance(t, torch.Tensor):
+            raise TypeError("%s must be a torch tensor on %s" % (name, self.device))
+        if t.device != self.device or t.dtype != torch.float64:
+            raise ValueError("%s must be float64 on %s" % (name, self.device))
+        if tuple(t.shape) != tuple(shape) or not t.is_contiguous():
+            raise ValueError("%s must be contiguous with shape %s, got %s"
+                             % (name, tuple(shape), tuple(t.shape)))
+
+    def t1_shape(self):
+        return (self.ng, self.nv, self.no)
+
+    def t2_shape(self):
+        return (self.ng, self.nv, self.nv, self.no, self.no)
+
+    def empty_amplitudes(self):
+        z = dict(dtype=torch.float64, device=self.device)
+        return torch.empty(self.t1_shape(), **z), torch.empty(self.t2_shape(), **z)
+
+    def to_device(self, x):
+        if isinstance(x, torch.Tensor):
+            return x.to(device=self.device, dtype=torch.float64).contiguous()
+        return torch.as_tensor(numpy.ascontiguousarray(x, dtype=numpy.float64),
+                               device=self.device)
+
+    # -- set-up --------------------------------------------------------------
+    def set_precision(self, precision):
+        self._call("ftcc_set_precision", PREC[precision])
+        self.precision = precision
+
+    def set_system(self, en, fock, eri, beta, mu, iocc=None, ivir=None):
+        """Fermi-weighted integrals from (en, g_fock_tot, g_aint_tot);
+        returns sum_p GP0(e_p)."""
+        en = self.to_device(en)
+        n = en.shape[0]
+        fock = self.to_device(fock)
+        eri = self.to_device(eri)
+        self._check(fock, (n, n), "fock")
+        self._check(eri, (n, n, n, n), "eri")
+        io = numpy.arange(n, dtype=numpy.int32) if iocc is None \
+            else numpy.asarray(iocc, dtype=numpy.int32)
+        iv = numpy.arange(n, dtype=numpy.int32) if ivir is None \
+            else numpy.asarray(ivir, dtype=numpy.int32)
+        if io.size != self.no or iv.size != self.nv:
+            raise ValueError("active lists do not match (no, nv)")
+        om = ctypes.c_double()
+        ip = ctypes.POINTER(ctypes.c_int)
+        self._call("ftcc_set_system", n, _vp(en), _vp(fock), _vp(eri),
+                   float(beta), float(mu), io.ctypes.data_as(ip),
+                   iv.ctypes.data_as(ip), ctypes.byref(om))
+        return om.value
+
+    def set_blocks(self, F, I):
+        """Arbitrary one_e_blocks / two_e_blocks (numpy or torch)."""
+        no, nv = self.no, self.nv
+        fs = []
+        for name in ONE_E:
+            t = self.to_device(getattr(F, name))
+            self._check(t, block_shape(name, no, nv), "F." + name)
+            fs.append(t)
+        ts = []
+        for name in TWO_E:
+            t = self.to_device(getattr(I, name))
+            self._check(t, block_shape(name, no, nv), "I." + name)
+            ts.append(t)
+        fp = (ctypes.c_void_p * 4)(*[t.data_ptr() for t in fs])
+        ip = (ctypes.c_void_p * 9)(*[t.data_ptr() for t in ts])
+        self._call("ftcc_set_blocks", fp, ip)
+
+    def set_denominators(self, D1, D2):
+        D1 = self.to_device(D1)
+        D2 = self.to_device(D2)
+        self._check(D1, (self.nv, self.no), "D1")
+        self._check(D2, (self.nv, self.nv, self.no, self.no), "D2")
+        self._call("ftcc_set_denominators", _vp(D1), _vp(D2))
+
+    def set_grid(self, ti, G):
+        """Set (ti, G); the grid size may change (amplitudes then have
+        len(ti) slices)."""
+        ti_a, ti_p = _dbuf(ti)
+        G_a, G_p = _dbuf(G)
+        ng = ti_a.shape[0]
+        if ti_a.ndim != 1 or G_a.shape != (ng, ng):
+            raise ValueError("grid shapes must be (ng,) and (ng, ng)")
+        self._call("ftcc_set_grid", ng, ti_p, G_p)
+        self.ng = ng
+        self._grid = (ti_a.copy(), G_a.copy())
+
+    def get_block(self, name):
+        shape = block_shape(name, self.no, self.nv)
+        out = torch.empty(shape, dtype=torch.float64, device=self.device)
+        self._call("ftcc_get_block", BLOCK_INDEX[name], _vp(out))
+        return out
+
+    # -- hot path --------------------------------------------------------------
+    def driving_terms(self, T1b, T2b):
+        self._check(T1b, self.t1_shape(), "T1bar")
+        self._check(T2b, self.t2_shape(), "T2bar")
+        self._call("ftcc_driving_terms", _vp(T1b), _vp(T2b))
+
+    def residual(self, T1, T2, T1b, T2b, skip_mask=0):
+        for t, s, nm in ((T1, self.t1_shape(), "T1"), (T2, self.t2_shape(), "T2"),
+                         (T1b, self.t1_shape(), "T1bar"),
+                         (T2b, self.t2_shape(), "T2bar")):
+            self._check(t, s, nm)
+        self._call("ftcc_residual", _vp(T1), _vp(T2), _vp(T1b), _vp(T2b),
+                   ctypes.c_uint64(skip_mask))
+
+    def integrate(self, T1=None, T2=None):
+        if T1 is not None:
+            self._check(T1, self.t1_shape(), "T1bar")
+        if T2 is not None:
+            self._check(T2, self.t2_shape(), "T2bar")
+        self._call("ftcc_integrate",
+                   _vp(T1) if T1 is not None else ctypes.c_void_p(),
+                   _vp(T2) if T2 is not None else ctypes.c_void_p())
+
+    def energy(self, T1, T2, g, beta, qterm=True):
+        self._check(T1, self.t1_shape(), "T1")
+        self._check(T2, self.t2_shape(), "T2")
+        g_a, g_p = _dbuf(g)
+        out = ctypes.c_double()
+        self._call("ftcc_energy", _vp(T1), _vp(T2), g_p, float(beta),
+                   int(bool(qterm)), ctypes.byref(out))
+        return out.value
+
+    def sumsq(self, x):
+        if x.device != self.device or x.dtype != torch.float64 \
+                or not x.is_contiguous():
+            raise ValueError("sumsq needs a contiguous float64 device tensor")
+        out = ctypes.c_double()
+        self._call("ftcc_sumsq", _vp(x), ctypes.c_int64(x.numel()),
+                   ctypes.byref(out))
+        return out.value
+
+    def step(self, T1old, T2old, T1b, T2b, g, beta, damp, skip_mask=0):
+        """One damped fixed-point body; returns (d1, d2, n1, n2, E)."""
+        for t, s, nm in ((T1old, self.t1_shape(), "T1old"),
+                         (T2old, self.t2_shape(), "T2old"),
+                         (T1b, self.t1_shape(), "T1bar"),
+                         (T2b, self.t2_shape(), "T2bar")):
+            self._check(t, s, nm)
+        g_a, g_p = _dbuf(g)
+        out = (ctypes.c_double * 5)()
+        self._call("ftcc_step", _vp(T1old), _vp(T2old), _vp(T1b), _vp(T2b),
+                   g_p, float(beta), float(damp), ctypes.c_uint64(skip_mask),
+                   out)
+        return tuple(out)
+
+    def profile(self, on=True):
+        """Bracket every O(n^6) product launch with HIP events."""
+        self._call("ftcc_profile_enable", int(bool(on)))
+
+    def profile_read(self):
+        """(launches, device ms, algorithmic flops) since the last read."""
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        fl = ctypes.c_double()
+        self._call("ftcc_profile_read", ctypes.byref(n), ctypes.byref(ms),
+                   ctypes.byref(fl))
+        return n.value, ms.value, fl.value
+
+    def workspace_bytes(self):
+        b = ctypes.c_int64()
+        self._call("ftcc_workspace_bytes", ctypes.byref(b))
+        return b.value
+
+    def gemm(self, A, B, C, alpha=1.0, beta=0.0, precision=None):
+        """C[b] = alpha A[b] @ B[b] + beta C[b] for 2-D or 3-D (batched)
+        tensors with arbitrary strides (testing hook for the MFMA engine)."""
+        def st(t):
+            if t.dim() == 2:
+                return 1, 0, t.stride(0), t.stride(1)
+            return t.shape[0], t.stride(0), t.stride(1), t.stride(2)
+        for t in (A, B, C):
+            if t.device != self.device or t.dtype != torch.float64:
+                raise ValueError("gemm operands must be float64 on device")
+        nb, sAb, sAm, sAk = st(A)
+        _, sBb, sBk, sBn = st(B)
+        _, sCb, sCm, sCn = st(C)
+        M, K = A.shape[-2], A.shape[-1]
+        N = B.shape[-1]
+        prec = PREC[precision or self.precision]
+        self._call("ftcc_gemm", M, N, K, nb, float(alpha), _vp(A), sAm, sAk,
+                   sAb, _vp(B), sBk, sBn, sBb, float(beta), _vp(C), sCm, sCn,
+                   sCb, prec)
+        return C

@@ -162,7 +162,10 @@ def ft_quad(ng, beta, quad):
         e = numpy.exp(1)
         delta = (e - 1.0) / (ng - 1.0)
         si = numpy.arange(ng) * delta + 1.0
-        g, G = _mapped(ng, beta, delta, si, 1.0 / si)
+        g = beta * get_gint(ng, delta) / si
+        G = get_G(ng, delta)
+        for i in range(ng):
+            G[i] = beta * G[i] / si
         return beta * numpy.log(si), g, G
     if quad == 'sin':                                   # :144-153
         delta = numpy.pi / (ng - 1.0)

@@ -1,0 +1,106 @@
+"""CPU checks of the boundary and the host logic (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy
+import pytest
+
+from oracle import ftcc as oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "ftcc.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(ftcc_\w+)\(",
+                                 text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from kelvin_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libftcc.so not built (python -m kelvin_amd.build)")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    # the ctypes table binds exactly the declared entry points
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_engine_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd._lib import FTCCError
+    with pytest.raises(FTCCError):
+        FTEngine(2, 2, 3)
+
+
+@pytest.mark.parametrize("quad", ['lin', 'ln', 'sin', 'exp', 'quad', 'cub',
+                                  'quar', 'mid', 'L'])
+def test_host_grids_match_oracle(quad):
+    from kelvin_amd import quadrature
+    for ng in (8, 9, 24):
+        a = quadrature.ft_quad(ng, 3.1, quad)
+        b = oracle.ft_quad(ng, 3.1, quad)
+        for x, y in zip(a, b):
+            assert numpy.array_equal(x, y)
+        if quad != 'mid':
+            for x, y in zip(quadrature.d_ft_quad(ng, 3.1, quad),
+                            oracle.d_ft_quad(ng, 3.1, quad)):
+                assert numpy.array_equal(x, y)
+    with pytest.raises(Exception):
+        quadrature.ft_quad(5, 1.0, 'xyz')
+    with pytest.raises(Exception):
+        quadrature.d_ft_quad(5, 1.0, 'mid')
+
+
+@pytest.mark.parametrize("norb,L,Emax", [(7, 2 * numpy.pi, 1.2),
+                                         (19, 2 * numpy.pi, 1.2),
+                                         (19, 1.942, 30.0)])
+def test_ueg_provider_matches_oracle(norb, L, Emax):
+    from kelvin_amd.ueg_system import UEGSystem
+    T, mu = 0.1, 0.1
+    a = UEGSystem(T, L, Emax, mu=mu, norb=norb, orbtype='g')
+    b = oracle.UEGSystem(T, L, Emax, mu, norb=norb)
+    assert a.basis.basis == b.basis.basis
+    assert numpy.array_equal(a.g_energies_tot(), b.g_energies_tot())
+    assert numpy.abs(a.g_aint_tot() - b.g_aint_tot()).max() == 0.0
+    assert numpy.abs(a.g_fock_tot() - b.g_fock_tot()).max() < 1e-15
+    assert abs(a.get_mp1() - b.get_mp1()) < 1e-15
+    assert abs(a.N - b.N) < 1e-14
+
+
+def test_ueg_config_shells_and_mu():
+    from kelvin_amd.ueg_system import ueg_from_config, ueg_parameters
+    L, T, Ef = ueg_parameters(4.0, 14, 0.1)
+    assert abs(L - 15.540520) < 1e-6
+    assert abs(Ef - 0.115099) < 1e-6
+    s = ueg_from_config(19, 4.0, N=14, t_over_tf=0.1)
+    assert len(s.g_energies_tot()) == 38
+    assert abs(s.N - 14.0) < 1e-9
+    assert abs(s.mu - 0.118495) < 1e-5
+    s = ueg_from_config(33, 2.0, N=14, t_over_tf=0.1)
+    assert abs(s.mu - 0.473966) < 1e-5 and abs(s.L - 7.770260) < 1e-6
+
+
+def test_df_model_matches_oracle():
+    from kelvin_amd.model_systems import DFModelSystem
+    a = DFModelSystem(0.1, 0.0, norb=8, seed=0)
+    b = oracle.DFModelSystem(0.1, 0.0, norb=8, seed=0)
+    assert numpy.array_equal(a.g_aint_tot(), b.g_aint_tot())
+    assert numpy.array_equal(a.g_energies_tot(), b.g_energies_tot())
+    assert abs(a.get_mp1() - b.get_mp1()) < 1e-15
+
+
+def test_bench_refuses_multi_gpu():
+    import subprocess
+    import sys
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"),
+                        "--gpus", "2"], capture_output=True, text=True)
+    assert p.returncode != 0
+    assert "one GPU" in (p.stdout + p.stderr)

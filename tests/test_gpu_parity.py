@@ -1,0 +1,333 @@
+"""GPU parity of libftcc against the CPU oracle and the reference goldens.
+
+Bar: FP64 results agree with the oracle to 1e-12 relative per element
+(residual, integration) and to 1e-10 Ha in grand potentials; end-to-end
+runs reproduce the reference's golden Omega_cc / per-iteration energies.
+"""
+import json
+import os
+
+import numpy
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from oracle import ftcc as oracle  # noqa: E402
+from oracle import fock_space as fs  # noqa: E402
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_goldens.json")))
+
+
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return numpy.abs(a - b).max() / max(numpy.abs(b).max(), 1e-300)
+
+
+def random_problem(no, nv, rng, herm=False):
+    n = no + nv
+    f = rng.normal(size=(n, n)) * 0.3
+    if herm:
+        f = f + f.T
+    eri = fs.random_antisym(n, rng, hermitian=herm) * 0.1
+    sl = {'o': slice(0, no), 'v': slice(no, n)}
+    F = oracle.Blocks(oo=f[sl['o'], sl['o']], ov=f[sl['o'], sl['v']],
+                      vo=f[sl['v'], sl['o']], vv=f[sl['v'], sl['v']])
+    I = oracle.Blocks({k: numpy.ascontiguousarray(
+        eri[sl[k[0]], sl[k[1]], sl[k[2]], sl[k[3]]])
+        for k in ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo', 'oovv', 'vooo',
+                  'ooov', 'oooo')})
+    return F, I
+
+
+def random_T(ng, nv, no, rng, scale=0.2):
+    t1 = rng.normal(size=(ng, nv, no)) * scale
+    t2 = rng.normal(size=(ng, nv, nv, no, no)) * scale
+    t2 = t2 - t2.transpose(0, 2, 1, 3, 4)
+    t2 = t2 - t2.transpose(0, 1, 2, 4, 3)
+    return t1, t2
+
+
+# ---------------------------------------------------------------- GEMM engine
+@pytest.mark.parametrize("M,N,K,batch", [(16, 16, 4, 1), (129, 70, 33, 1),
+                                         (200, 130, 300, 3), (7, 5, 20000, 1),
+                                         (256, 256, 256, 2)])
+@pytest.mark.parametrize("layout", ["nn", "tn", "nt", "tt"])
+def test_gemm_fp64_matches_torch(M, N, K, batch, layout):
+    from kelvin_amd.engine import FTEngine
+    d = dev()
+    g = torch.Generator(device="cpu").manual_seed(M * 7 + N + K)
+    A = torch.randn(batch, M, K, generator=g, dtype=torch.float64)
+    B = torch.randn(batch, K, N, generator=g, dtype=torch.float64)
+    C0 = torch.randn(batch, M, N, generator=g, dtype=torch.float64)
+    Ad = A.to(d) if layout[0] == 'n' else A.transpose(1, 2).contiguous().to(d).transpose(1, 2)
+    Bd = B.to(d) if layout[1] == 'n' else B.transpose(1, 2).contiguous().to(d).transpose(1, 2)
+    C = C0.to(d).clone()
+    eng = FTEngine(2, 2, 1)
+    eng.gemm(Ad, Bd, C, alpha=0.75, beta=-0.5, precision="fp64")
+    ref = 0.75 * torch.bmm(A, B) - 0.5 * C0
+    assert rel(to_np(C), ref.numpy()) < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16x3", 1e-4)])
+def test_gemm_mixed_precision(prec, tol):
+    """Per-product rounding: 2^-24 (FP32) and ~2^-16 (two-term bf16 split,
+    lo*lo dropped); the max error of N(0,1) inputs over 78k entries is a few
+    times that, scaled by sqrt(K)."""
+    from kelvin_amd.engine import FTEngine
+    d = dev()
+    g = torch.Generator(device="cpu").manual_seed(5)
+    A = torch.randn(300, 1000, generator=g, dtype=torch.float64)
+    B = torch.randn(1000, 260, generator=g, dtype=torch.float64)
+    C = torch.zeros(300, 260, dtype=torch.float64, device=d)
+    eng = FTEngine(2, 2, 1)
+    eng.gemm(A.to(d), B.to(d), C, precision=prec)
+    ref = (A @ B).numpy()
+    # error relative to the natural scale sqrt(K)*|a||b|
+    err = numpy.abs(to_np(C) - ref).max() / numpy.sqrt(1000.0)
+    assert err < tol, err
+
+
+# ---------------------------------------------------------------- residual
+@pytest.mark.parametrize("no,nv,ng", [(3, 4, 2), (5, 5, 3), (6, 3, 1),
+                                      (9, 11, 2)])
+def test_residual_matches_oracle_random_blocks(no, nv, ng):
+    from kelvin_amd import ft_cc_equations
+    from kelvin_amd.ov_blocks import one_e_blocks, two_e_blocks
+    dev()
+    rng = numpy.random.default_rng(100 * no + nv)
+    F, I = random_problem(no, nv, rng)
+    t1, t2 = random_T(ng, nv, no, rng)
+    ref1, ref2 = oracle.residual_tbar(F, I, t1, t2)
+    D1 = rng.normal(size=(nv, no))
+    D2 = rng.normal(size=(nv, nv, no, no))
+    Fb = one_e_blocks(**{k: F[k] for k in ('oo', 'ov', 'vo', 'vv')})
+    Ib = two_e_blocks(**{k: I[k] for k in I})
+    b1, b2 = ft_cc_equations.ccsd_residual(Fb, Ib, t1, t2, D1=D1, D2=D2,
+                                           ti=numpy.zeros(ng),
+                                           G=numpy.zeros((ng, ng)))
+    assert rel(to_np(b1), ref1) < 1e-12
+    assert rel(to_np(b2), ref2) < 1e-12
+
+
+def test_ccsd_stanton_matches_oracle_ft_blocks():
+    """ccsd_stanton (residual + integration) on Fermi-weighted UEG blocks."""
+    from kelvin_amd import cc_utils, ft_cc_equations, quadrature
+    from kelvin_amd.ueg_system import UEGSystem
+    dev()
+    T, mu = 0.1, 0.1
+    sysg = UEGSystem(T, 2 * numpy.pi, 1.2, mu=mu, norb=7, orbtype='g')
+    osys = oracle.UEGSystem(T, 2 * numpy.pi, 1.2, mu, norb=7)
+    beta, ng = 1.0 / T, 6
+    ti, g, G = quadrature.ft_quad(ng, beta, 'lin')
+    en = sysg.g_energies_tot()
+    F, I = cc_utils.ft_integrals(sysg, en, beta, mu, ng=ng)
+    Fo, Io = oracle.ft_integrals(osys.g_fock_tot(), osys.g_aint_tot(), en,
+                                 beta, mu)
+    for k in ('oo', 'ov', 'vo', 'vv'):
+        assert rel(to_np(getattr(F, k)), Fo[k]) < 1e-14
+    for k in Io:
+        assert numpy.abs(to_np(getattr(I, k)) - Io[k]).max() < 1e-16 + 1e-14 * numpy.abs(Io[k]).max()
+    n = en.shape[0]
+    rng = numpy.random.default_rng(3)
+    t1, t2 = random_T(ng, n, n, rng, scale=0.05)
+    d1, d2 = oracle.D1(en, en), oracle.D2(en, en)
+    r1, r2 = oracle.ccsd_stanton(Fo, Io, t1, t2, d1, d2, ti, ng, G)
+    from kelvin_amd.ov_blocks import EngineDenominators
+    a1, a2 = ft_cc_equations.ccsd_stanton(
+        F, I, t1, t2, EngineDenominators(F.engine, 1),
+        EngineDenominators(F.engine, 2), ti, ng, G)
+    assert rel(to_np(a1), r1) < 1e-12
+    assert rel(to_np(a2), r2) < 1e-12
+
+
+# ---------------------------------------------------------------- integrators
+@pytest.mark.parametrize("quad", ['lin', 'mid', 'L', 'sin', 'exp'])
+def test_integrators_match_oracle(quad):
+    from kelvin_amd import quadrature
+    dev()
+    rng = numpy.random.default_rng(11)
+    ng, nv, no = 9, 4, 3
+    ti, g, G = quadrature.ft_quad(ng, 4.0, quad)
+    ev, eo = rng.uniform(-1, 1, nv), rng.uniform(-1, 1, no)
+    D1, D2 = oracle.D1(ev, eo), oracle.D2(ev, eo)
+    t1, t2 = random_T(ng, nv, no, rng)
+    a1 = quadrature.int_tbar1(ng, t1, ti, D1, G)
+    a2 = quadrature.int_tbar2(ng, t2, ti, D2, G)
+    assert rel(to_np(a1), oracle.int_tbar1(ng, t1, ti, D1, G)) < 1e-13
+    assert rel(to_np(a2), oracle.int_tbar2(ng, t2, ti, D2, G)) < 1e-13
+
+
+def test_integrator_large_grid_global_path():
+    """ng above the LDS staging limit uses the global-memory path."""
+    from kelvin_amd import quadrature
+    dev()
+    rng = numpy.random.default_rng(12)
+    ng, nv, no = 80, 2, 2
+    ti, g, G = quadrature.ft_quad(ng, 3.0, 'lin')
+    ev, eo = rng.uniform(-1, 1, nv), rng.uniform(-1, 1, no)
+    t1, t2 = random_T(ng, nv, no, rng)
+    D2 = oracle.D2(ev, eo)
+    a2 = quadrature.int_tbar2(ng, t2, ti, D2, G)
+    assert rel(to_np(a2), oracle.int_tbar2(ng, t2, ti, D2, G)) < 1e-13
+
+
+def test_energy_matches_oracle():
+    from kelvin_amd.ft_cc_energy import ft_cc_energy
+    dev()
+    rng = numpy.random.default_rng(13)
+    ng, nv, no = 5, 4, 3
+    t1, t2 = random_T(ng, nv, no, rng)
+    f = rng.normal(size=(no, nv))
+    eri = rng.normal(size=(no, no, nv, nv))
+    g = rng.uniform(0.1, 1.0, ng)
+    for q in (True, False):
+        ref = oracle.ft_cc_energy(t1, t2, f, eri, g, 2.5, Qterm=q)
+        got = ft_cc_energy(t1, t2, f, eri, g, 2.5, Qterm=q)
+        assert abs(got - ref) < 1e-13 * max(1.0, abs(ref))
+
+
+# ---------------------------------------------------------------- end to end
+def _ueg_gold(key, orbtype='g'):
+    from kelvin_amd.ueg_system import UEGSystem
+    s = GOLD[key]["system"]
+    L = 2 * numpy.pi if s["L"] == "2*pi" else s["L"]
+    return UEGSystem(s["T"], L, s["Emax"], mu=s["mu"], norb=s["norb"],
+                     orbtype=orbtype), s
+
+
+def test_golden_G1_ueg_omega_cc():
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    g = GOLD["G1"]
+    for orbtype in ('g', 'u'):
+        sysg, s = _ueg_gold("G1", orbtype)
+        cc = ccsd(sysg, T=s["T"], mu=s["mu"], iprint=0, **g["ccsd"])
+        Etot, Ecc = cc.run()
+        assert abs(Ecc - g["Omega_cc"]) < g["tol"]
+
+
+def test_golden_G4_iterations_and_oracle():
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    g = GOLD["G4"]
+    sysg, s = _ueg_gold("G4")
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], **g["ccsd"])
+    Etot, Ecc = cc.run()
+    assert abs(cc.mp2 - g["mp2"]) < 6e-11
+    for (E, _), ref in zip(cc.history, g["iter_energies"]):
+        assert abs(E - ref) < 6e-11
+    assert abs(Etot - g["Omega"]) < 6e-11
+    osys = oracle.UEGSystem(s["T"], 2 * numpy.pi, s["Emax"], s["mu"],
+                            norb=s["norb"])
+    r = oracle.ft_ccsd(osys, s["T"], s["mu"], **g["ccsd"])
+    assert abs(Ecc - r["Omega_cc"]) < 1e-12
+    assert abs(Etot - r["Omega"]) < 1e-10
+    assert rel(to_np(cc.T2), r["T2"]) < 1e-9
+
+
+@pytest.mark.parametrize("key", ["G3", "G5"])
+def test_golden_n38_runs(key):
+    """n = 38 spin orbitals, ng = 10: every printed iteration energy."""
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    g = GOLD[key]
+    sysg, s = _ueg_gold(key)
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], **g["ccsd"])
+    Etot, Ecc = cc.run()
+    assert abs(cc.mp2 - g["mp2"]) < 6e-11
+    for (E, _), ref in zip(cc.history, g["iter_energies"]):
+        assert abs(E - ref) < 6e-11
+    assert abs(Ecc - g["Omega_cc"]) < 2e-11
+    assert abs(Etot - g["Omega"]) < 6e-10
+
+
+def test_golden_G6_n66():
+    """n = 66 spin orbitals (33 k-points), the reference's largest run."""
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    g = GOLD["G6"]
+    sysg, s = _ueg_gold("G6")
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], **g["ccsd"])
+    Etot, Ecc = cc.run()
+    assert abs(cc.mp2 - g["mp2"]) < 6e-11
+    for (E, _), ref in zip(cc.history, g["iter_energies"]):
+        assert abs(E - ref) < 6e-11
+    assert abs(Ecc - g["Omega_cc"]) < 2e-11
+    assert abs(Etot - g["Omega"]) < 6e-10
+
+
+def test_df_model_c1_matches_oracle():
+    """The molecule-like n = 16, ng = 10, damp 0.5 case."""
+    from kelvin_amd.ccsd import ccsd
+    from kelvin_amd.model_systems import DFModelSystem
+    dev()
+    T, mu = 0.1, 0.0
+    cc = ccsd(DFModelSystem(T, mu, norb=8, seed=0), T=T, mu=mu, ngrid=10,
+              damp=0.5, econv=1e-8)
+    Etot, Ecc = cc.run()
+    r = oracle.ft_ccsd(oracle.DFModelSystem(T, mu, norb=8, seed=0), T, mu,
+                       ngrid=10, damp=0.5, econv=1e-8)
+    assert cc.history and len(cc.history) == r["niter"]
+    assert abs(Ecc - r["Omega_cc"]) < 1e-12
+    assert abs(Etot - r["Omega"]) < 1e-10
+
+
+def test_active_space_matches_oracle():
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    sysg, s = _ueg_gold("G1")
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], ngrid=8, damp=0.2, max_iter=50,
+              athresh=0.05)
+    Etot, Ecc = cc.run()
+    osys = oracle.UEGSystem(s["T"], 2 * numpy.pi, s["Emax"], s["mu"],
+                            norb=s["norb"])
+    r = oracle.ft_ccsd(osys, s["T"], s["mu"], ngrid=8, damp=0.2, max_iter=50,
+                       athresh=0.05)
+    assert tuple(cc.T1.shape) == (8, 14, 2)   # 12 occupied-role orbitals cut
+    assert abs(Ecc - r["Omega_cc"]) < 1e-12
+
+
+def test_fci_exact_two_spin_orbitals():
+    from kelvin_amd.ccsd import ccsd
+    from kelvin_amd.model_systems import ArraySystem
+    dev()
+    eps, U = -0.3, 0.8
+    h = numpy.diag([eps, eps])
+    eri = numpy.zeros((2, 2, 2, 2))
+    eri[0, 1, 0, 1] = eri[1, 0, 1, 0] = U
+    eri[0, 1, 1, 0] = eri[1, 0, 0, 1] = -U
+    T, mu = 1.0, 0.1
+    sysm = ArraySystem(T, mu, h, eri, numpy.array([eps, eps]) + 0.2)
+    cc = ccsd(sysm, T=T, mu=mu, ngrid=200, max_iter=60, econv=1e-12)
+    Etot, Ecc = cc.run()
+    assert abs(Etot - fs.ft_fci_omega(h, eri, 1.0 / T, mu)) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_refined_matches_fp64(prec):
+    """Mixed-precision contractions, then FP64 refinement: the converged
+    grand potential equals the tightly converged FP64 one (bar 1e-7 Ha;
+    checked at 1e-9)."""
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    sysg, s = _ueg_gold("G3")
+    kw = dict(ngrid=10, damp=0.2, max_iter=80, econv=1e-11)
+    ref = ccsd(sysg, T=s["T"], mu=s["mu"], **kw)
+    Eref, Ecc_ref = ref.run()
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], precision=prec, **kw)
+    Etot, Ecc = cc.run()
+    assert abs(Ecc - Ecc_ref) < 1e-9
+    assert abs(Etot - Eref) < 1e-9
+    assert abs(Ecc_ref - GOLD["G3"]["Omega_cc"]) < 1e-8
