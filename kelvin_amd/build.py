@@ -12,7 +12,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libftcc.so")
 ARCH = os.environ.get("FTCC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["ftcc_api.hip", "gemm.hip", "tensor_ops.hip", "update.hip"]
+SOURCES = ["ftcc_api.hip", "gemm.hip", "sbgemm.hip", "tensor_ops.hip",
+           "update.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

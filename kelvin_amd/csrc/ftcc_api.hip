@@ -66,6 +66,11 @@ struct ftcc_ctx {
     hipEvent_t e0, e1;
     double flops;
   };
+  // split-bf16 operands of the six O(n^6) products (bf16x3 mode)
+  SplitMat sp_tauA, sp_tauB, sp_t2rA, sp_aringB, sp_wmnijB, sp_wabefA, sp_wrB;
+  SplitMat sp_oovvA, sp_oovvB, sp_ringA;
+  bool split_fixed_valid = false;
+
   bool prof_on = false;
   std::vector<Pending> prof_pending;
   std::vector<hipEvent_t> ev_pool;
@@ -144,6 +149,7 @@ void build_derived(ftcc_ctx *c) {
   derive(c->I[7], c->ooov_emni, dooov, pe, 1.0, s);
   transpose2(c->F[1], c->fovT, O, V, 1, s);
   c->have_ints = true;
+  c->split_fixed_valid = false;
 }
 
 // GEMM helper: dense row-major operands with per-slot batch strides.
@@ -215,19 +221,81 @@ void flush_profile(ftcc_ctx *c) {
   c->prof_pending.clear();
 }
 
-void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false) {
-  ensure_partial(c, g);
+// Run fn() bracketed by events when profiling is on (O(n^6) products only).
+template <typename Fn>
+void timed(ftcc_ctx *c, bool big, double flops, Fn fn) {
   if (big && c->prof_on) {
     ftcc_ctx::Pending p;
     p.e0 = take_event(c);
     p.e1 = take_event(c);
-    p.flops = 2.0 * g.M * (double)g.N * g.K * g.nb1 * g.nb2;
+    p.flops = flops;
     FTCC_CHECK_HIP(hipEventRecord(p.e0, c->stream));
-    gemm_run(g, prec, c->scr, c->stream);
+    fn();
     FTCC_CHECK_HIP(hipEventRecord(p.e1, c->stream));
     c->prof_pending.push_back(p);
   } else {
-    gemm_run(g, prec, c->scr, c->stream);
+    fn();
+  }
+}
+
+double gemm_flops(const Gemm &g) {
+  return 2.0 * g.M * (double)g.N * g.K * g.nb1 * g.nb2;
+}
+
+void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false) {
+  ensure_partial(c, g);
+  timed(c, big, gemm_flops(g),
+        [&] { gemm_run(g, prec, c->scr, c->stream); });
+}
+
+// One of the six O(n^6) products: split-bf16 operands in bf16x3 mode
+// (A as [M][K], B as [N][K]), the generic engine otherwise.
+void big(ftcc_ctx *c, const Gemm &g, int prec, const SplitMat &A,
+         const SplitMat &B) {
+  if (prec != FTCC_PREC_BF16X3) {
+    run(c, g, prec, true);
+    return;
+  }
+  timed(c, true, gemm_flops(g), [&] {
+    sbgemm(g.M, g.N, g.K, g.nb1, g.alpha, A, B, g.beta, g.C, g.sCm, g.sCn,
+           g.sC1, c->stream);
+  });
+}
+
+// Split-bf16 operand buffers (allocated on first use of bf16x3).
+SplitMat alloc_split(ftcc_ctx *c, int rows, int K, int nb) {
+  SplitMat m;
+  m.ld = split_ld(K);
+  m.bstride = (int64_t)rows * m.ld;
+  const int64_t elems = m.bstride * nb;
+  // dalloc counts doubles: bf16 pairs need elems/2 doubles per array
+  m.hi = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
+  m.lo = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
+  return m;
+}
+
+void ensure_split(ftcc_ctx *c) {
+  const int O = c->no, V = c->nv, nb = c->nb;
+  const int OO = O * O, VV = V * V, VO = V * O;
+  if (!c->sp_tauA.hi) {
+    c->sp_tauA = alloc_split(c, VV, OO, nb);
+    c->sp_tauB = alloc_split(c, OO, VV, nb);
+    c->sp_t2rA = alloc_split(c, VO, VO, nb);
+    c->sp_aringB = alloc_split(c, VO, VO, nb);
+    c->sp_wmnijB = alloc_split(c, OO, OO, nb);
+    c->sp_wabefA = alloc_split(c, VV, VV, nb);
+    c->sp_wrB = alloc_split(c, VO, VO, nb);
+    c->sp_oovvA = alloc_split(c, OO, VV, 1);
+    c->sp_oovvB = alloc_split(c, VV, OO, 1);
+    c->sp_ringA = alloc_split(c, VO, VO, 1);
+    c->sp_oovvA.bstride = c->sp_oovvB.bstride = c->sp_ringA.bstride = 0;
+  }
+  if (!c->split_fixed_valid) {
+    hipStream_t s = c->stream;
+    split_convert(c->I[5], VV, 1, 0, OO, VV, c->sp_oovvA, 1, s);
+    split_convert(c->I[5], 1, VV, 0, VV, OO, c->sp_oovvB, 1, s);
+    split_convert(c->oovv_ring, VO, 1, 0, VO, VO, c->sp_ringA, 1, s);
+    c->split_fixed_valid = true;
   }
 }
 
@@ -317,6 +385,14 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   PrepOut po{c->tau, c->taut, c->t2r, c->aring, c->t2p, c->tautp};
   prep_amplitudes(t1, t2, V, O, nb, po, s);
   transpose2(t1, c->t1T, V, O, nb, s);   // t1T[m,f] = t1[f,m]
+  if (P == FTCC_PREC_BF16X3) {
+    // split operands: A as [M][K], B as [N][K] (K contiguous)
+    ensure_split(c);
+    split_convert(c->tau, OO, 1, S2, (int)VV, (int)OO, c->sp_tauA, nb, s);
+    split_convert(c->tau, 1, OO, S2, (int)OO, (int)VV, c->sp_tauB, nb, s);
+    split_convert(c->t2r, VO, 1, S2, (int)VO, (int)VO, c->sp_t2rA, nb, s);
+    split_convert(c->aring, 1, VO, S2, (int)VO, (int)VO, c->sp_aringB, nb, s);
+  }
 
   // Fme[m,e] = Fov[m,e] + sum_{fn} <mn||ef> t_n^f
   gemv_rows(c->oovv_ring, VO, 0, t1, VO, c->FmeT, VO, (int)VO, (int)VO, nb,
@@ -358,8 +434,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   run(c, mk((int)(OO * O), O, V, 1.0, 0.0, ooov, V, 1, 0, t1, O, 1, VO, c->q1,
             O, 1, O4, nb), P);
   antisym_add(c->wmnij, c->q1, O, O, O, O, true, 1.0, nb, s);
-  run(c, mk((int)OO, (int)OO, (int)VV, 0.25, 1.0, oovv, VV, 1, 0, c->tau, OO,
-            1, S2, c->wmnij, OO, 1, O4, nb), P, true);
+  big(c, mk((int)OO, (int)OO, (int)VV, 0.25, 1.0, oovv, VV, 1, 0, c->tau, OO,
+            1, S2, c->wmnij, OO, 1, O4, nb), P, c->sp_oovvA, c->sp_tauB);
 
   // Wabef = vvvv - P(ab) t_m^b <am||ef> + 1/4 tau.oovv          (G2)
   bcast(c->wabef, vvvv, V4, nb, 1.0, 0.0, s);
@@ -373,8 +449,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P);
   }
   antisym_add(c->wabef, c->q1, V, V, V, V, false, -1.0, nb, s);
-  run(c, mk((int)VV, (int)VV, (int)OO, 0.25, 1.0, c->tau, OO, 1, S2, oovv, VV,
-            1, 0, c->wabef, VV, 1, V4, nb), P, true);
+  big(c, mk((int)VV, (int)VV, (int)OO, 0.25, 1.0, c->tau, OO, 1, S2, oovv, VV,
+            1, 0, c->wabef, VV, 1, V4, nb), P, c->sp_tauA, c->sp_oovvB);
 
   // Wr[e,m,b,j] = <mb||ej> + t_j^f <mb||ef> - t_n^b <mn||ej> - A.<mn||ef>
   run(c, mk((int)(V * VO), O, V, 1.0, 0.0, vovv, V, 1, 0, t1, O, 1, VO, c->q1,
@@ -389,12 +465,15 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P);
   }
   ring_assemble(c->wr, c->ovvo_ring, c->q1, c->q2, V, O, nb, s);
-  run(c, mk((int)VO, (int)VO, (int)VO, -1.0, 1.0, c->oovv_ring, VO, 1, 0,
-            c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), P, true);             // G5
+  big(c, mk((int)VO, (int)VO, (int)VO, -1.0, 1.0, c->oovv_ring, VO, 1, 0,
+            c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), P, c->sp_ringA,
+      c->sp_aringB);                                                     // G5
 
   // Z[ai,bj] = t2r.Wr - t_i^e t_m^a <mb||ej>                    (G6)
-  run(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr, VO,
-            1, S2, c->z, VO, 1, S2, nb), P, true);
+  if (P == FTCC_PREC_BF16X3)
+    split_convert(c->wr, 1, VO, S2, (int)VO, (int)VO, c->sp_wrB, nb, s);
+  big(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr, VO,
+            1, S2, c->z, VO, 1, S2, nb), P, c->sp_t2rA, c->sp_wrB);
   run(c, mk(O, (int)(O * VO), V, 1.0, 0.0, t1, 1, O, VO, c->ovvo_ring, O * VO,
             1, 0, c->u, O * VO, 1, OO * VO, nb), P);
   {
@@ -409,10 +488,16 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
 
   // doubles accumulator in t2bar: vvoo + 1/2 tau.Wmnij + 1/2 Wabef.tau
   bcast(t2bar, vvoo, S2, nb, 1.0, 0.0, s);
-  run(c, mk((int)VV, (int)OO, (int)OO, 0.5, 1.0, c->tau, OO, 1, S2, c->wmnij,
-            OO, 1, O4, t2bar, OO, 1, S2, nb), P, true);                        // G3
-  run(c, mk((int)VV, (int)OO, (int)VV, 0.5, 1.0, c->wabef, VV, 1, V4, c->tau,
-            OO, 1, S2, t2bar, OO, 1, S2, nb), P, true);                        // G4
+  if (P == FTCC_PREC_BF16X3) {
+    split_convert(c->wmnij, 1, OO, O4, (int)OO, (int)OO, c->sp_wmnijB, nb, s);
+    split_convert(c->wabef, VV, 1, V4, (int)VV, (int)VV, c->sp_wabefA, nb, s);
+  }
+  big(c, mk((int)VV, (int)OO, (int)OO, 0.5, 1.0, c->tau, OO, 1, S2, c->wmnij,
+            OO, 1, O4, t2bar, OO, 1, S2, nb), P, c->sp_tauA,
+      c->sp_wmnijB);                                                     // G3
+  big(c, mk((int)VV, (int)OO, (int)VV, 0.5, 1.0, c->wabef, VV, 1, V4, c->tau,
+            OO, 1, S2, t2bar, OO, 1, S2, nb), P, c->sp_wabefA,
+      c->sp_tauB);                                                       // G4
 
   // S1 (antisymmetrised in ab): Fbe[b,e] t2[a,e,i,j] + t1[a,m] vooo[b,m,i,j]
   {

@@ -63,6 +63,23 @@ struct Scratch {
 void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s);
 size_t gemm_partial_elems(const Gemm &g);
 
+// Split-BF16 operand: x = hi + lo, two bf16 arrays [rows][ld] (K contiguous,
+// zero-padded to ld), batch stride in elements.
+struct SplitMat {
+  __bf16 *hi = nullptr, *lo = nullptr;
+  int64_t ld = 0;
+  int64_t bstride = 0;
+};
+// padded leading dimension for a K-long operand row
+int64_t split_ld(int K);
+// d[b][r][c] = src[b*sb + r*sr + c*sc] split into (hi, lo), r < R, c < C
+void split_convert(const double *src, int64_t sr, int64_t sc, int64_t sb,
+                   int R, int C, const SplitMat &d, int nb, hipStream_t s);
+// C[b] = alpha * A[b] . B[b]^T + beta * C[b] with A as [M][K], B as [N][K]
+void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
+            const SplitMat &B, double beta, double *C, int64_t sCm,
+            int64_t sCn, int64_t sCb, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // Elementwise / permutation / reduction kernels (tensor_ops.hip)
 // ---------------------------------------------------------------------------

@@ -1,0 +1,293 @@
+// Split-BF16 GEMM ("bf16x3") for the O(n^6) products, and the kernels that
+// produce its operands.
+//
+// Operands are stored in HBM already split: x = hi + lo with hi = bf16(x),
+// lo = bf16(x - hi), as two bf16 arrays with K contiguous (A as [M][ld],
+// B transposed as [N][ld]); K is zero-padded to a multiple of 32 so the
+// inner loop needs no bounds checks along K.  C = alpha * A.B + beta * C is
+// accumulated in FP32 by v_mfma_f32_32x32x16_bf16 from hi*hi + hi*lo + lo*hi
+// and written as FP64.
+//
+// Tiling: a 512-thread workgroup (8 wave64s) owns a 256x256 tile of C, each
+// wave a 128x64 quadrant (4x2 MFMA 32x32 blocks, 128 accumulator VGPRs).  K
+// advances in slabs of 32 through double-buffered LDS; each LDS row holds the
+// 32 hi values, then the 32 lo values, then 8 bf16 of padding (144 B rows:
+// the 8 lanes of one 128 B LDS cycle land on distinct banks).  The next slab
+// is prefetched into registers with 16-byte loads while the current one feeds
+// the MFMAs.  Workgroup ids are remapped so that each XCD works on a compact
+// group of tiles (shared A/B panels stay in that XCD's L2).
+#include "ftcc_internal.h"
+
+#include <algorithm>
+
+namespace ftcc {
+namespace {
+
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef float f16_t __attribute__((ext_vector_type(16)));
+typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 256, BN = 256, BK = 32, NT = 512, LDR = 72;
+constexpr int GROUP_M = 8;
+constexpr size_t LDS_BYTES = 2 * 2 * (size_t)BM * LDR * sizeof(__bf16);
+
+struct SbDev {
+  int M, N, Kp, nb, tilesM, tilesN;
+  const __bf16 *Ahi, *Alo;
+  int64_t lda, sAb;
+  const __bf16 *Bhi, *Blo;
+  int64_t ldb, sBb;
+  double *C;
+  int64_t sCm, sCn, sCb;
+  double alpha, beta;
+};
+
+__device__ inline void tile_of(const SbDev &p, int &bt, int &tm, int &tn) {
+  const int nblk = gridDim.x, b = blockIdx.x;
+  const int q = nblk / 8, r = nblk % 8, x = b % 8, j = b / 8;
+  const int lid = (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+  const int per = p.tilesM * p.tilesN;
+  bt = lid / per;
+  const int t = lid % per;
+  const int span = GROUP_M * p.tilesN;
+  const int first_m = (t / span) * GROUP_M;
+  const int gsz = min(p.tilesM - first_m, GROUP_M);
+  tm = first_m + (t % span) % gsz;
+  tn = (t % span) / gsz;
+}
+
+// 2048 16-byte chunks per operand slab: (array, row, quarter) per chunk
+__device__ inline void load_operand(const __bf16 *hi, const __bf16 *lo,
+                                    int64_t ld, int row0, int rows, int k0,
+                                    u4_t reg[4]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int c = t + NT * r;
+    const int arr = c >> 10, row = (c >> 2) & 255, kq = c & 3;
+    const __bf16 *src = (arr ? lo : hi) + (int64_t)(row0 + row) * ld + k0 +
+                        8 * kq;
+    if (row0 + row < rows)
+      reg[r] = *(const u4_t *)src;
+    else
+      reg[r] = u4_t{0u, 0u, 0u, 0u};
+  }
+}
+
+__device__ inline void store_operand(__bf16 *lds, const u4_t reg[4]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int c = t + NT * r;
+    const int arr = c >> 10, row = (c >> 2) & 255, kq = c & 3;
+    *(u4_t *)(lds + row * LDR + arr * BK + 8 * kq) = reg[r];
+  }
+}
+
+__global__ __launch_bounds__(NT, 1) void sbgemm_kernel(SbDev p) {
+  extern __shared__ __bf16 lds[];   // [2 stages][A|B][BM*LDR]
+  int bt, tm, tn;
+  tile_of(p, bt, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const __bf16 *Ahi = p.Ahi + bt * p.sAb, *Alo = p.Alo + bt * p.sAb;
+  const __bf16 *Bhi = p.Bhi + bt * p.sBb, *Blo = p.Blo + bt * p.sBb;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 2, wn = wave & 3;
+  const int li = lane & 31, lg = lane >> 5;
+  const int nslab = p.Kp / BK;
+
+  f16_t acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+
+  u4_t ra[4], rb[4];
+  constexpr int STAGE = 2 * BM * LDR;   // bf16 per stage (A then B)
+  load_operand(Ahi, Alo, p.lda, m0, p.M, 0, ra);
+  load_operand(Bhi, Blo, p.ldb, n0, p.N, 0, rb);
+  store_operand(lds, ra);
+  store_operand(lds + BM * LDR, rb);
+  __syncthreads();
+  for (int s = 0; s < nslab; ++s) {
+    const __bf16 *As = lds + (s & 1) * STAGE;
+    const __bf16 *Bs = As + BM * LDR;
+    if (s + 1 < nslab) {
+      load_operand(Ahi, Alo, p.lda, m0, p.M, (s + 1) * BK, ra);
+      load_operand(Bhi, Blo, p.ldb, n0, p.N, (s + 1) * BK, rb);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf8_t ah[4], al[4], bh[2], bl[2];
+      const int koff = ks * 16 + 8 * lg;
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        const __bf16 *a = As + (wm * 128 + rt * 32 + li) * LDR + koff;
+        ah[rt] = *(const bf8_t *)a;
+        al[rt] = *(const bf8_t *)(a + BK);
+      }
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        const __bf16 *b = Bs + (wn * 64 + ct * 32 + li) * LDR + koff;
+        bh[ct] = *(const bf8_t *)b;
+        bl[ct] = *(const bf8_t *)(b + BK);
+      }
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+              al[rt], bh[ct], acc[rt][ct], 0, 0, 0);
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+              ah[rt], bl[ct], acc[rt][ct], 0, 0, 0);
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+              ah[rt], bh[ct], acc[rt][ct], 0, 0, 0);
+        }
+    }
+    if (s + 1 < nslab) {
+      __bf16 *nxt = lds + ((s + 1) & 1) * STAGE;
+      store_operand(nxt, ra);
+      store_operand(nxt + BM * LDR, rb);
+    }
+    __syncthreads();
+  }
+
+  // lane holds column li, rows 8*(v/4) + 4*lg + v%4 of each 32x32 block
+  double *C = p.C + bt * p.sCb;
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      const int col = n0 + wn * 64 + ct * 32 + li;
+      if (col >= p.N) continue;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int row = m0 + wm * 128 + rt * 32 + 8 * (v >> 2) + 4 * lg + (v & 3);
+        if (row < p.M) {
+          double *c = C + (int64_t)row * p.sCm + (int64_t)col * p.sCn;
+          const double x = (double)acc[rt][ct][v];
+          *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
+        }
+      }
+    }
+}
+
+__device__ inline void split1(double v, __bf16 &hi, __bf16 &lo) {
+  const float f = (float)v;
+  hi = (__bf16)f;
+  lo = (__bf16)(f - (float)hi);
+}
+
+// dst[b][r][c] = src[b*sb + r*sr + c*sc] for c < C, 0 for C <= c < ld;
+// threads run along c (use when sc is the small stride).
+__global__ void split_rows_kernel(const double *src, int64_t sr, int64_t sc,
+                                  int64_t sb, int R, int C, int64_t ld,
+                                  int64_t db, __bf16 *hi, __bf16 *lo, int nb) {
+  const int64_t per = (int64_t)R * ld;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+       idx < per * nb; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = idx / per, e = idx % per;
+    const int r = (int)(e / ld);
+    const int c = (int)(e % ld);
+    const double v = (c < C) ? src[b * sb + (int64_t)r * sr + (int64_t)c * sc] : 0.0;
+    __bf16 h, l;
+    split1(v, h, l);
+    hi[b * db + e] = h;
+    lo[b * db + e] = l;
+  }
+}
+
+// Same map through a 64x64 LDS tile: reads run along r (sr is the small
+// stride), writes along c.
+__global__ void split_transpose_kernel(const double *src, int64_t sr,
+                                       int64_t sc, int64_t sb, int R, int C,
+                                       int64_t ld, int64_t db, __bf16 *hi,
+                                       __bf16 *lo) {
+  __shared__ double tile[64][65];
+  const int b = blockIdx.z;
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 256 threads
+  for (int i = ty; i < 64; i += 4) {
+    const int r = r0 + tx, c = c0 + i;
+    tile[i][tx] = (r < R && c < C)
+                      ? src[b * sb + (int64_t)r * sr + (int64_t)c * sc]
+                      : 0.0;
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += 4) {
+    const int r = r0 + i, c = c0 + tx;
+    if (r < R && c < ld) {
+      __bf16 h, l;
+      split1(tile[tx][i], h, l);
+      const int64_t o = b * db + (int64_t)r * ld + c;
+      hi[o] = h;
+      lo[o] = l;
+    }
+  }
+}
+
+}  // namespace
+
+int64_t split_ld(int K) { return ((int64_t)K + 63) / 64 * 64; }
+
+void split_convert(const double *src, int64_t sr, int64_t sc, int64_t sb,
+                   int R, int C, const SplitMat &d, int nb, hipStream_t s) {
+  require(d.ld >= C && d.ld % 32 == 0, "split operand leading dimension");
+  if ((int64_t)R * C == 0) return;
+  if (sr == 1 && sc != 1) {
+    const int gx = (int)((d.ld + 63) / 64), gy = (R + 63) / 64;
+    hipLaunchKernelGGL(split_transpose_kernel, dim3(gx, gy, nb), dim3(256), 0,
+                       s, src, sr, sc, sb, R, C, d.ld, d.bstride, d.hi, d.lo);
+  } else {
+    const int64_t total = (int64_t)R * d.ld * nb;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(split_rows_kernel, dim3(blocks), dim3(256), 0, s, src,
+                       sr, sc, sb, R, C, d.ld, d.bstride, d.hi, d.lo, nb);
+  }
+  FTCC_CHECK_HIP(hipGetLastError());
+}
+
+void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
+            const SplitMat &B, double beta, double *C, int64_t sCm,
+            int64_t sCn, int64_t sCb, hipStream_t s) {
+  if (M <= 0 || N <= 0 || nb <= 0) return;
+  require(A.ld % 32 == 0 && B.ld % 32 == 0 && A.ld >= K && B.ld >= K,
+          "split operands need K padded to 32");
+  static bool attr_set = false;
+  if (!attr_set) {
+    FTCC_CHECK_HIP(hipFuncSetAttribute(
+        (const void *)sbgemm_kernel,
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES));
+    attr_set = true;
+  }
+  SbDev p;
+  p.M = M;
+  p.N = N;
+  p.Kp = (int)((K + BK - 1) / BK * BK);
+  p.nb = nb;
+  p.tilesM = (M + BM - 1) / BM;
+  p.tilesN = (N + BN - 1) / BN;
+  p.Ahi = A.hi;
+  p.Alo = A.lo;
+  p.lda = A.ld;
+  p.sAb = A.bstride;
+  p.Bhi = B.hi;
+  p.Blo = B.lo;
+  p.ldb = B.ld;
+  p.sBb = B.bstride;
+  p.C = C;
+  p.sCm = sCm;
+  p.sCn = sCn;
+  p.sCb = sCb;
+  p.alpha = alpha;
+  p.beta = beta;
+  const int64_t nblk = (int64_t)p.tilesM * p.tilesN * nb;
+  require(nblk < (1LL << 31), "too many GEMM tiles");
+  hipLaunchKernelGGL(sbgemm_kernel, dim3((unsigned)nblk), dim3(NT), LDS_BYTES,
+                     s, p);
+  FTCC_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace ftcc

@@ -121,6 +121,32 @@ def test_residual_matches_oracle_random_blocks(no, nv, ng):
     assert rel(to_np(b2), ref2) < 1e-12
 
 
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 2e-4)])
+@pytest.mark.parametrize("no,nv,ng", [(9, 11, 2), (17, 17, 1)])
+def test_residual_mixed_precision(prec, tol, no, nv, ng):
+    """Mixed-precision contractions (split-bf16 GEMM engine for the six
+    O(n^6) products) against the FP64 oracle, relative to max |Tbar|."""
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd.ov_blocks import one_e_blocks, two_e_blocks
+    d = dev()
+    rng = numpy.random.default_rng(7 * no + nv)
+    F, I = random_problem(no, nv, rng)
+    t1, t2 = random_T(ng, nv, no, rng)
+    ref1, ref2 = oracle.residual_tbar(F, I, t1, t2)
+    eng = FTEngine(no, nv, ng, precision=prec)
+    eng.set_blocks(one_e_blocks(**{k: F[k] for k in ('oo', 'ov', 'vo', 'vv')}),
+                   two_e_blocks(**{k: I[k] for k in I}))
+    eng.set_denominators(numpy.ones((nv, no)), numpy.ones((nv, nv, no, no)))
+    eng.set_grid(numpy.zeros(ng), numpy.zeros((ng, ng)))
+    T1 = torch.as_tensor(t1, device=d)
+    T2 = torch.as_tensor(t2, device=d)
+    b1, b2 = torch.empty_like(T1), torch.empty_like(T2)
+    eng.residual(T1, T2, b1, b2)
+    assert rel(to_np(b1), ref1) < tol
+    assert rel(to_np(b2), ref2) < tol
+    assert rel(to_np(b2), ref2) > 1e-12     # really ran in low precision
+
+
 def test_ccsd_stanton_matches_oracle_ft_blocks():
     """ccsd_stanton (residual + integration) on Fermi-weighted UEG blocks."""
     from kelvin_amd import cc_utils, ft_cc_equations, quadrature
