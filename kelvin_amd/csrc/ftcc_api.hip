@@ -372,6 +372,9 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   const int O = c->no, V = c->nv;
   const int P = c->prec;           // contraction precision of n^5/n^6 terms
   const int P64 = FTCC_PREC_FP64;  // small terms
+  // the O(n^5) contractions stream n^4 operands once (HBM-bound), so they
+  // run in FP64 whatever the precision of the O(n^6) products
+  const int P5 = FTCC_PREC_FP64;
   hipStream_t s = c->stream;
   const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
   const int64_t S2 = VV * OO;           // doubles slice
@@ -410,7 +413,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
               (int)VO, V, V, -1.0, 1.0, c->scr.partial, c->scr.partial_elems,
               s);
   run(c, mk(V, V, (int)(V * OO), -0.5, 1.0, c->taut, V * OO, 1, S2,
-            c->oovv_fmne, V, 1, 0, c->Fae, V, 1, VV, nb), P);
+            c->oovv_fmne, V, 1, 0, c->Fae, V, 1, VV, nb), P5);
 
   // Fmi[m,i] = Foo + 1/2 Fov.t1 + sum t_n^e <mn||ie> + 1/2 oovv.tautp
   bcast(c->Fmi, Foo, OO, nb, 1.0, 0.0, s);
@@ -419,7 +422,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   gemv_rows(c->ooov_p1, VO, 0, c->t1T, VO, c->Fmi, OO, (int)OO, (int)VO, nb,
             1.0, 1.0, s);
   run(c, mk(O, O, (int)(O * VV), 0.5, 1.0, oovv, O * VV, 1, 0, c->tautp, O, 1,
-            S2, c->Fmi, O, 1, OO, nb), P);
+            S2, c->Fmi, O, 1, OO, nb), P5);
 
   // Fbe = Fae - 1/2 t1.Fme ; Fmj = Fmi + 1/2 Fme.t1
   axpby(c->Fbe, c->Fae, VV * nb, 1.0, 0.0, s);
@@ -432,7 +435,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // Wmnij = oooo + P(ij) t_j^e <mn||ie> + 1/4 oovv.tau          (G1)
   bcast(c->wmnij, oooo, O4, nb, 1.0, 0.0, s);
   run(c, mk((int)(OO * O), O, V, 1.0, 0.0, ooov, V, 1, 0, t1, O, 1, VO, c->q1,
-            O, 1, O4, nb), P);
+            O, 1, O4, nb), P5);
   antisym_add(c->wmnij, c->q1, O, O, O, O, true, 1.0, nb, s);
   big(c, mk((int)OO, (int)OO, (int)VV, 0.25, 1.0, oovv, VV, 1, 0, c->tau, OO,
             1, S2, c->wmnij, OO, 1, O4, nb), P, c->sp_oovvA, c->sp_tauB);
@@ -446,7 +449,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sA2 = 0;
     g.sB2 = O * VV;
     g.sC2 = V * VV;
-    run(c, g, P);
+    run(c, g, P5);
   }
   antisym_add(c->wabef, c->q1, V, V, V, V, false, -1.0, nb, s);
   big(c, mk((int)VV, (int)VV, (int)OO, 0.25, 1.0, c->tau, OO, 1, S2, oovv, VV,
@@ -454,7 +457,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
 
   // Wr[e,m,b,j] = <mb||ej> + t_j^f <mb||ef> - t_n^b <mn||ej> - A.<mn||ef>
   run(c, mk((int)(V * VO), O, V, 1.0, 0.0, vovv, V, 1, 0, t1, O, 1, VO, c->q1,
-            O, 1, S2, nb), P);
+            O, 1, S2, nb), P5);
   {
     Gemm g = mk(V, (int)VO, O, 1.0, 0.0, t1, O, 1, VO, ooov, VO, 1, 0, c->q2,
                 VO, 1, S2, nb);
@@ -462,7 +465,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sA2 = 0;
     g.sB2 = O * VO;
     g.sC2 = V * VO;
-    run(c, g, P);
+    run(c, g, P5);
   }
   ring_assemble(c->wr, c->ovvo_ring, c->q1, c->q2, V, O, nb, s);
   big(c, mk((int)VO, (int)VO, (int)VO, -1.0, 1.0, c->oovv_ring, VO, 1, 0,
@@ -475,7 +478,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   big(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr, VO,
             1, S2, c->z, VO, 1, S2, nb), P, c->sp_t2rA, c->sp_wrB);
   run(c, mk(O, (int)(O * VO), V, 1.0, 0.0, t1, 1, O, VO, c->ovvo_ring, O * VO,
-            1, 0, c->u, O * VO, 1, OO * VO, nb), P);
+            1, 0, c->u, O * VO, 1, OO * VO, nb), P5);
   {
     Gemm g = mk(V, (int)VO, O, -1.0, 1.0, t1, O, 1, VO, c->u, VO, 1, OO * VO,
                 c->z, O * VO, 1, S2, nb);
@@ -483,7 +486,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sA2 = 0;
     g.sB2 = O * VO;
     g.sC2 = VO;
-    run(c, g, P);
+    run(c, g, P5);
   }
 
   // doubles accumulator in t2bar: vvoo + 1/2 tau.Wmnij + 1/2 Wabef.tau
@@ -507,7 +510,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sA2 = 0;
     g.sB2 = V * OO;
     g.sC2 = V * OO;
-    run(c, g, P);
+    run(c, g, P5);
   }
   {
     Gemm g = mk(V, (int)OO, O, 1.0, 1.0, t1, O, 1, VO, vooo, OO, 1, 0, c->s1,
@@ -516,11 +519,11 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sA2 = 0;
     g.sB2 = O * OO;
     g.sC2 = OO;
-    run(c, g, P);
+    run(c, g, P5);
   }
   // S2 (antisymmetrised in ij): -t2[a,b,i,m] Fmj[m,j] + t1[e,i] vvvo[a,b,e,j]
   run(c, mk((int)(VV * O), O, O, -1.0, 0.0, t2, O, 1, S2, c->Fmj, O, 1, OO,
-            c->s2, O, 1, S2, nb), P);
+            c->s2, O, 1, S2, nb), P5);
   {
     Gemm g = mk(O, O, V, 1.0, 1.0, t1, 1, O, VO, vvvo, O, 1, 0, c->s2, O, 1,
                 S2, nb);
@@ -528,7 +531,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sA2 = 0;
     g.sB2 = VO;
     g.sC2 = OO;
-    run(c, g, P);
+    run(c, g, P5);
   }
   r2_final(t2bar, c->s1, c->s2, c->z, V, O, nb, s);
 
@@ -545,9 +548,9 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     gemv_cols(c->ovvo_ring, VO, 0, t1 + y * VO, 0, t1bar + y * VO, 0, (int)VO,
               (int)VO, 1, 1.0, 1.0, c->scr.partial, c->scr.partial_elems, s);
   run(c, mk(V, O, (int)(O * VV), 0.5, 1.0, vovv, O * VV, 1, 0, c->t2p, O, 1,
-            S2, t1bar, O, 1, VO, nb), P);
+            S2, t1bar, O, 1, VO, nb), P5);
   run(c, mk(V, O, (int)(V * OO), 0.5, 1.0, t2, V * OO, 1, S2, c->ooov_emni, O,
-            1, 0, t1bar, O, 1, VO, nb), P);
+            1, 0, t1bar, O, 1, VO, nb), P5);
   axpby(t1bar, t1bar, VO * nb, -1.0, 0.0, s);
 }
 
@@ -613,6 +616,8 @@ UpdateArgs upd(ftcc_ctx *c) {
   a.eint = nullptr;
   a.partials = c->partials;
   a.src = nullptr;
+  a.uniform = c->grid.uniform ? 1 : 0;
+  a.delta = c->grid.delta;
   return a;
 }
 
@@ -842,6 +847,16 @@ int ftcc_set_grid(ftcc_ctx *c, int ng, const double *ti, const double *G) {
     gr.ng = ng;
     gr.ti.assign(ti, ti + ng);
     gr.G.assign(G, G + (size_t)ng * ng);
+    // equally spaced abscissae ('lin', 'mid', 'L' rules): the propagator
+    // between points x < y is exp(-delta d)^(y-x)
+    gr.uniform = ng > 1;
+    gr.delta = ng > 1 ? ti[1] - ti[0] : 0.0;
+    double tmax = 0.0;
+    for (int k = 0; k < ng; ++k) tmax = std::max(tmax, std::fabs(ti[k]));
+    for (int k = 0; k < ng && gr.uniform; ++k)
+      if (std::fabs(ti[k] - ti[0] - k * gr.delta) > 8e-16 * (tmax + 1e-300) * (k + 1))
+        gr.uniform = false;
+    if (!(gr.delta > 0.0)) gr.uniform = false;
     if (!gr.d_ti) {
       FTCC_CHECK_HIP(hipMalloc(&gr.d_ti, 8 * ng));
       FTCC_CHECK_HIP(hipMalloc(&gr.d_g, 8 * ng));

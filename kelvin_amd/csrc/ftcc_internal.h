@@ -155,6 +155,8 @@ void transpose2(const double *in, double *out, int r, int c, int nb,
 struct Grid {
   int ng = 0;
   std::vector<double> ti, G;
+  bool uniform = false;         // tau_x - tau_y == (x - y) * delta to ~1 ulp
+  double delta = 0.0;
   double *d_ti = nullptr, *d_g = nullptr, *d_G = nullptr;
 };
 
@@ -171,6 +173,8 @@ struct UpdateArgs {
                                 // only from a copy)
   double *told;                 // damped iterate (mode 1)
   double damp;
+  int uniform;                  // grid equally spaced: tau_x - tau_y =
+  double delta;                 //   (x - y) * delta
   const double *t1new;          // mode 1 T2 pass: updated T1 (ng,nv,no)
   const double *eint;           // energy integral in amplitude layout
                                 // (F.ov^T for T1, oovv^T for T2)

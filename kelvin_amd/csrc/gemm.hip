@@ -436,12 +436,14 @@ __global__ void reduce_partial_kernel(const double *partial, int R, int nbC,
 }
 
 int choose_splits(const Gemm &g, int tiles, int bk) {
-  // enough workgroups to cover 256 CUs twice over; keep >= 4 slabs per split
+  // Few output tiles over a long K (the O(n^5) contractions onto n x n
+  // matrices) stream their operands from HBM once: split K until ~4
+  // workgroups per CU keep enough loads in flight; >= 8 slabs per split.
   const int64_t blocks = (int64_t)tiles * g.nb1 * g.nb2;
-  if (blocks >= 512) return 1;
-  const int maxs = std::max(1, g.K / (4 * bk));
-  int want = (int)((512 + blocks - 1) / blocks);
-  return std::max(1, std::min(want, std::min(maxs, 256)));
+  if (blocks >= 1024) return 1;
+  const int maxs = std::max(1, g.K / (8 * bk));
+  int want = (int)((1024 + blocks - 1) / blocks);
+  return std::max(1, std::min(want, std::min(maxs, 2048)));
 }
 
 template <typename K3>

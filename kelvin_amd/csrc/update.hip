@@ -72,12 +72,26 @@ __device__ inline Elem decode(int64_t e, int nv, int no) {
   return r;
 }
 
+// Weight of Tbar(tau_x) in T(tau_y): exp((tau_x - tau_y) d) for x < y.  On
+// an equally spaced grid (p.uniform, spacing p.delta) it is r^(y-x) with
+// r = exp(-delta d), formed as a running product from x = y-1 downwards, so
+// each element costs one exp instead of ng(ng-1)/2.
 template <bool DOUBLES, bool STAGE_LDS>
 __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
-  extern __shared__ double col[];   // [ng][UT] when STAGE_LDS
+  extern __shared__ double lds[];   // G[ng*ng], ti[ng], then [ng][UT] columns
   const int ng = p.ng, nv = p.nv, no = p.no;
+  const double *Gs = p.G, *tis = p.ti;
+  double *col = lds + ng * ng + ng;
+  if (STAGE_LDS) {
+    for (int k = threadIdx.x; k < ng * ng; k += UT) lds[k] = p.G[k];
+    for (int k = threadIdx.x; k < ng; k += UT) lds[ng * ng + k] = p.ti[k];
+    __syncthreads();
+    Gs = lds;
+    tis = lds + ng * ng;
+  }
   const int64_t per = DOUBLES ? (int64_t)nv * nv * no * no : (int64_t)nv * no;
   const int64_t per1 = (int64_t)nv * no;
+  const double *src = p.src ? p.src : p.tbar;
   double dsum = 0.0, nsum = 0.0, esum = 0.0;
   for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
        e += (int64_t)gridDim.x * UT) {
@@ -90,19 +104,29 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
     else
       d = p.ev[q.a] - p.eo[q.i];
     if (STAGE_LDS)
-      for (int x = 0; x < ng; ++x) col[x * UT + threadIdx.x] = p.tbar[x * per + e];
+      for (int x = 0; x < ng; ++x) col[x * UT + threadIdx.x] = src[x * per + e];
+    const double r = p.uniform ? exp(-p.delta * d) : 0.0;
     double eacc = 0.0;
     for (int y = 0; y < ng; ++y) {
       double tn = 0.0;
-      const double *Gy = p.G + (int64_t)y * ng;
-      const double ty = p.ti[y];
-      for (int x = 0; x < ng; ++x) {
+      const double *Gy = Gs + y * ng;
+      const double ty = tis[y];
+      double w = 1.0;
+      for (int x = y - 1; x >= 0; --x) {
+        if (p.uniform) w *= r;
         const double gyx = Gy[x];
         if (gyx == 0.0) continue;
         const double tb = STAGE_LDS ? col[x * UT + threadIdx.x]
-                                    : (p.src ? p.src : p.tbar)[x * per + e];
-        const double w = (x < y) ? exp((p.ti[x] - ty) * d) : 1.0;
-        tn += gyx * (w * tb);
+                                    : src[x * per + e];
+        const double wx = p.uniform ? w : exp((tis[x] - ty) * d);
+        tn += gyx * (wx * tb);
+      }
+      for (int x = y; x < ng; ++x) {
+        const double gyx = Gy[x];
+        if (gyx == 0.0) continue;
+        const double tb = STAGE_LDS ? col[x * UT + threadIdx.x]
+                                    : src[x * per + e];
+        tn += gyx * tb;
       }
       if (mode == 0) {
         p.tbar[y * per + e] = tn;
@@ -134,7 +158,16 @@ int launch_update(const UpdateArgs &a, int mode, hipStream_t s) {
   const int blocks =
       (int)std::max<int64_t>(1, std::min<int64_t>((per + UT - 1) / UT, MAX_BLOCKS));
   if (a.ng <= MAX_LDS_NG) {
-    const size_t lds = sizeof(double) * a.ng * UT;
+    const size_t lds = sizeof(double) * (a.ng * a.ng + a.ng + a.ng * UT);
+    static bool attr = false;
+    if (!attr) {
+      FTCC_CHECK_HIP(hipFuncSetAttribute(
+          (const void *)update_kernel<DOUBLES, true>,
+          hipFuncAttributeMaxDynamicSharedMemorySize,
+          (int)(sizeof(double) * (MAX_LDS_NG * MAX_LDS_NG + MAX_LDS_NG +
+                                  MAX_LDS_NG * UT))));
+      attr = true;
+    }
     hipLaunchKernelGGL((update_kernel<DOUBLES, true>), dim3(blocks), dim3(UT),
                        lds, s, a, mode);
   } else {
