@@ -43,6 +43,8 @@ struct ftcc_ctx {
   double *ooov_p1 = nullptr;    // [m,i,n,e] = ooov[m,n,i,e]
   double *ooov_emni = nullptr;  // [e,m,n,i] = ooov[n,m,i,e]
   double *oovvT = nullptr;      // [a,b,i,j] = oovv[i,j,a,b]
+  double *vovv_embf = nullptr;  // [e,m,b,f] = vovv[b,m,e,f]
+  double *ooov_emnj = nullptr;  // [e,m,n,j] = ooov[m,n,j,e]
   double *fovT = nullptr;       // [a,i] = F.ov[i,a]
   double *eo = nullptr, *ev = nullptr;
   double *D1 = nullptr, *D2 = nullptr;   // explicit denominators (optional)
@@ -54,7 +56,7 @@ struct ftcc_ctx {
   // per-slot workspace (nb slots)
   int nb = 1;
   double *tau = nullptr, *taut, *t2r, *aring, *t2p, *tautp, *wmnij, *wabef,
-      *wr, *z, *u, *s1, *s2, *q1, *q2;
+      *wr, *z, *u, *s1, *s2;
   double *t1T, *Fme, *FmeT, *Fae, *Fmi, *Fbe, *Fmj, *ytmp;
   Scratch scr;
   double *partials = nullptr;   // 3*MAX blocks
@@ -147,6 +149,11 @@ void build_derived(ftcc_ctx *c) {
   derive(c->I[7], c->ooov_p1, dooov, p1, 1.0, s);
   const int pe[4] = {3, 1, 0, 2};
   derive(c->I[7], c->ooov_emni, dooov, pe, 1.0, s);
+  const int pj[4] = {3, 0, 1, 2};
+  derive(c->I[7], c->ooov_emnj, dooov, pj, 1.0, s);
+  const int dvovv[4] = {V, O, V, V};
+  const int pb[4] = {2, 1, 0, 3};
+  derive(c->I[2], c->vovv_embf, dvovv, pb, 1.0, s);
   transpose2(c->F[1], c->fovT, O, V, 1, s);
   c->have_ints = true;
   c->split_fixed_valid = false;
@@ -317,6 +324,8 @@ void ensure_integrals(ftcc_ctx *c) {
   c->ovvo_ring = dalloc(c, S2);
   c->ooov_p1 = dalloc(c, OO * VO);
   c->ooov_emni = dalloc(c, OO * VO);
+  c->ooov_emnj = dalloc(c, OO * VO);
+  c->vovv_embf = dalloc(c, VO * VV);
   c->fovT = dalloc(c, VO);
   c->eo = dalloc(c, O);
   c->ev = dalloc(c, V);
@@ -329,10 +338,10 @@ void ensure_workspace(ftcc_ctx *c) {
   const int O = c->no, V = c->nv, ng = c->ng;
   const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
   const int64_t S2 = VV * OO;
-  const int64_t q1sz = std::max<int64_t>(std::max<int64_t>(OO * OO, VV * VV), S2);
   const int64_t sm = std::max<int64_t>(VV, std::max<int64_t>(VO, OO));
   const int64_t per_slot =
-      8 * (11 * S2 + OO * OO + VV * VV + OO * VO + q1sz + 8 * sm);
+      8 * (10 * S2 + OO * OO + VV * VV + OO * VO + 8 * sm) +
+      4 * 7 * (S2 + 64 * std::max<int64_t>(VV, std::max<int64_t>(VO, OO)));
   double budget_gb = 24.0;
   if (const char *env = getenv("FTCC_WORKSPACE_GB")) budget_gb = atof(env);
   const int64_t budget = (int64_t)(budget_gb * (1LL << 30));
@@ -353,8 +362,6 @@ void ensure_workspace(ftcc_ctx *c) {
   c->wmnij = dalloc(c, OO * OO * nb);
   c->wabef = dalloc(c, VV * VV * nb);
   c->u = dalloc(c, OO * VO * nb);
-  c->q1 = dalloc(c, q1sz * nb);
-  c->q2 = dalloc(c, S2 * nb);
   c->t1T = dalloc(c, VO * nb);
   c->Fme = dalloc(c, VO * nb);
   c->FmeT = dalloc(c, VO * nb);
@@ -434,40 +441,57 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
 
   // Wmnij = oooo + P(ij) t_j^e <mn||ie> + 1/4 oovv.tau          (G1)
   bcast(c->wmnij, oooo, O4, nb, 1.0, 0.0, s);
-  run(c, mk((int)(OO * O), O, V, 1.0, 0.0, ooov, V, 1, 0, t1, O, 1, VO, c->q1,
-            O, 1, O4, nb), P5);
-  antisym_add(c->wmnij, c->q1, O, O, O, O, true, 1.0, nb, s);
+  // + sum_e <mn||ie> t_j^e into [(m,n,i), j]
+  run(c, mk((int)(OO * O), O, V, 1.0, 1.0, ooov, V, 1, 0, t1, O, 1, VO,
+            c->wmnij, O, 1, O4, nb), P5);
+  {
+    // - sum_e t_i^e <mn||je>: per (m,n), C[i][j] -= t1^T[i,e] ooov[m,n,j,e]
+    Gemm g = mk(O, O, V, -1.0, 1.0, t1, 1, O, VO, ooov, 1, V, 0, c->wmnij, O,
+                1, O4, nb);
+    g.nb2 = (int)OO;
+    g.sA2 = 0;
+    g.sB2 = O * V;
+    g.sC2 = OO;
+    run(c, g, P5);
+  }
   big(c, mk((int)OO, (int)OO, (int)VV, 0.25, 1.0, oovv, VV, 1, 0, c->tau, OO,
             1, S2, c->wmnij, OO, 1, O4, nb), P, c->sp_oovvA, c->sp_tauB);
 
   // Wabef = vvvv - P(ab) t_m^b <am||ef> + 1/4 tau.oovv          (G2)
   bcast(c->wabef, vvvv, V4, nb, 1.0, 0.0, s);
   {
-    Gemm g = mk(V, (int)VV, O, 1.0, 0.0, t1, O, 1, VO, vovv, VV, 1, 0, c->q1,
-                VV, 1, V4, nb);
+    // - sum_m t_m^b <am||ef>: per a, C[b][ef] -= t1[b,m] vovv[a,m,ef]
+    Gemm g = mk(V, (int)VV, O, -1.0, 1.0, t1, O, 1, VO, vovv, VV, 1, 0,
+                c->wabef, VV, 1, V4, nb);
     g.nb2 = V;
     g.sA2 = 0;
     g.sB2 = O * VV;
     g.sC2 = V * VV;
     run(c, g, P5);
+    // + sum_m t_m^a <bm||ef>: per b, rows a of W[a,b,ef]
+    g.alpha = 1.0;
+    g.sCm = V * VV;
+    g.sC2 = VV;
+    run(c, g, P5);
   }
-  antisym_add(c->wabef, c->q1, V, V, V, V, false, -1.0, nb, s);
   big(c, mk((int)VV, (int)VV, (int)OO, 0.25, 1.0, c->tau, OO, 1, S2, oovv, VV,
             1, 0, c->wabef, VV, 1, V4, nb), P, c->sp_tauA, c->sp_oovvB);
 
   // Wr[e,m,b,j] = <mb||ej> + t_j^f <mb||ef> - t_n^b <mn||ej> - A.<mn||ef>
-  run(c, mk((int)(V * VO), O, V, 1.0, 0.0, vovv, V, 1, 0, t1, O, 1, VO, c->q1,
-            O, 1, S2, nb), P5);
+  bcast(c->wr, c->ovvo_ring, S2, nb, 1.0, 0.0, s);
+  // t_j^f <mb||ef> = -sum_f vovv[b,m,e,f] t1[f,j]: rows (e,m,b) of Wr
+  run(c, mk((int)(V * VO), O, V, -1.0, 1.0, c->vovv_embf, V, 1, 0, t1, O, 1,
+            VO, c->wr, O, 1, S2, nb), P5);
   {
-    Gemm g = mk(V, (int)VO, O, 1.0, 0.0, t1, O, 1, VO, ooov, VO, 1, 0, c->q2,
-                VO, 1, S2, nb);
-    g.nb2 = O;
+    // -t_n^b <mn||ej> = sum_n t1[b,n] ooov[m,n,j,e]: per (e,m), C[b][j]
+    Gemm g = mk(V, O, O, 1.0, 1.0, t1, O, 1, VO, c->ooov_emnj, O, 1, 0, c->wr,
+                O, 1, S2, nb);
+    g.nb2 = (int)VO;
     g.sA2 = 0;
-    g.sB2 = O * VO;
-    g.sC2 = V * VO;
+    g.sB2 = OO;
+    g.sC2 = VO;
     run(c, g, P5);
   }
-  ring_assemble(c->wr, c->ovvo_ring, c->q1, c->q2, V, O, nb, s);
   big(c, mk((int)VO, (int)VO, (int)VO, -1.0, 1.0, c->oovv_ring, VO, 1, 0,
             c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), P, c->sp_ringA,
       c->sp_aringB);                                                     // G5

@@ -224,9 +224,6 @@ __global__ void prep_kernel(const double *t1s, const double *t2s, int nv,
     const int64_t ring = (((int64_t)a * no + i) * nv + b) * no + j;
     t2r[y * per + ring] = v;
     aring[y * per + ring] = -0.5 * v + t1[a * no + j] * t1[b * no + i];
-    const int64_t pj = (((int64_t)j * nv + a) * nv + b) * no + i;
-    t2p[y * per + pj] = v;
-    tautp[y * per + pj] = tt;
   }
 }
 
@@ -305,6 +302,74 @@ __global__ void r2_final_kernel(double *t2bar, const double *S1,
     v += z[ring(a, i, b, j)] - z[ring(b, i, a, j)] - z[ring(a, j, b, i)] +
          z[ring(b, j, a, i)];
     t2bar[idx] = -v;
+  }
+}
+
+// Same assembly, one 32x32 (i,j) tile of one (a,b) pair per workgroup: the
+// three operands read transposed (S2[a,b,j,i], Z[a,j,b,i], Z[b,j,a,i]) are
+// loaded along i into LDS so every global access is coalesced.
+__global__ __launch_bounds__(256) void r2_final_tiled_kernel(
+    double *t2bar, const double *S1, const double *S2, const double *Z,
+    int nv, int no) {
+  __shared__ double tS2[32][33], tZ1[32][33], tZ2[32][33];
+  const int tiles = (no + 31) / 32;
+  int64_t bid = blockIdx.x;
+  const int tj = (int)(bid % tiles);
+  bid /= tiles;
+  const int ti = (int)(bid % tiles);
+  bid /= tiles;
+  const int b = (int)(bid % nv);
+  bid /= nv;
+  const int a = (int)(bid % nv);
+  const int64_t y = bid / nv;
+  const int64_t per = (int64_t)nv * nv * no * no;
+  const double *s1 = S1 + y * per, *s2 = S2 + y * per, *z = Z + y * per;
+  double *out = t2bar + y * per;
+  const int i0 = ti * 32, j0 = tj * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    const bool ok = i < no && j < no;
+    tS2[r][tx] = ok ? s2[(((int64_t)a * nv + b) * no + j) * no + i] : 0.0;
+    tZ1[r][tx] = ok ? z[(((int64_t)a * no + j) * nv + b) * no + i] : 0.0;
+    tZ2[r][tx] = ok ? z[(((int64_t)b * no + j) * nv + a) * no + i] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + r, j = j0 + tx;
+    if (i >= no || j >= no) continue;
+    const int64_t nat = (((int64_t)a * nv + b) * no + i) * no + j;
+    double v = out[nat];
+    v += s1[nat] - s1[(((int64_t)b * nv + a) * no + i) * no + j];
+    v += s2[nat] - tS2[tx][r];
+    v += z[(((int64_t)a * no + i) * nv + b) * no + j] -
+         z[(((int64_t)b * no + i) * nv + a) * no + j] - tZ1[tx][r] +
+         tZ2[tx][r];
+    out[nat] = -v;
+  }
+}
+
+// out[b][c][r] = in[b][r][c] for an R x C matrix per batch, 32x32 tiles
+__global__ __launch_bounds__(256) void transpose_tiled_kernel(
+    const double *in, double *out, int64_t R, int C) {
+  __shared__ double t[32][33];
+  const int64_t tilesC = (C + 31) / 32;
+  const int64_t r0 = (blockIdx.x / tilesC) * 32;
+  const int c0 = (int)(blockIdx.x % tilesC) * 32;
+  const int64_t b = blockIdx.y;
+  const double *src = in + b * R * C;
+  double *dst = out + b * R * C;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t r = r0 + k;
+    const int c = c0 + tx;
+    t[k][tx] = (r < R && c < C) ? src[r * C + c] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + k;
+    const int64_t r = r0 + tx;
+    if (r < R && c < C) dst[(int64_t)c * R + r] = t[tx][k];
   }
 }
 
@@ -430,6 +495,16 @@ void prep_amplitudes(const double *t1, const double *t2, int nv, int no,
                      int nb, const PrepOut &o, hipStream_t s) {
   LAUNCH(prep_kernel, (int64_t)nv * nv * no * no * nb, t1, t2, nv, no, nb,
          o.tau, o.taut, o.t2r, o.aring, o.t2p, o.tautp);
+  // t2p / tautp: transpose of the [(a,b,i)][j] matrix of each slice
+  const int64_t R = (int64_t)nv * nv * no;
+  const int64_t tiles = ((R + 31) / 32) * ((no + 31) / 32);
+  require(tiles < (1LL << 31) && nb < 65536, "transpose grid too large");
+  hipLaunchKernelGGL(transpose_tiled_kernel, dim3((unsigned)tiles, nb),
+                     dim3(256), 0, s, t2, o.t2p, R, no);
+  FTCC_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(transpose_tiled_kernel, dim3((unsigned)tiles, nb),
+                     dim3(256), 0, s, o.taut, o.tautp, R, no);
+  FTCC_CHECK_HIP(hipGetLastError());
 }
 
 void antisym_add(double *W, const double *Q, int d0, int d1, int d2, int d3,
@@ -446,8 +521,12 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
 
 void r2_final(double *t2bar, const double *S1, const double *S2,
               const double *Z, int nv, int no, int nb, hipStream_t s) {
-  LAUNCH(r2_final_kernel, (int64_t)nv * nv * no * no * nb, t2bar, S1, S2, Z,
-         nv, no, nb);
+  const int tiles = (no + 31) / 32;
+  const int64_t blocks = (int64_t)nb * nv * nv * tiles * tiles;
+  require(blocks < (1LL << 31), "r2 assembly grid too large");
+  hipLaunchKernelGGL(r2_final_tiled_kernel, dim3((unsigned)blocks), dim3(256),
+                     0, s, t2bar, S1, S2, Z, nv, no);
+  FTCC_CHECK_HIP(hipGetLastError());
 }
 
 void transpose2(const double *in, double *out, int r, int c, int nb,

@@ -18,7 +18,7 @@ namespace ftcc {
 namespace {
 
 constexpr int UT = 128;          // threads per block of the update kernels
-constexpr int MAX_LDS_NG = 64;   // grids up to this size stage Tbar in LDS
+constexpr int MAX_LDS_NG = 48;   // grids up to this size stage Tbar, Told in LDS
 constexpr int MAX_BLOCKS = 8192;
 
 __device__ inline void block_reduce3(double a, double b, double c,
@@ -78,10 +78,11 @@ __device__ inline Elem decode(int64_t e, int nv, int no) {
 // each element costs one exp instead of ng(ng-1)/2.
 template <bool DOUBLES, bool STAGE_LDS>
 __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
-  extern __shared__ double lds[];   // G[ng*ng], ti[ng], then [ng][UT] columns
+  extern __shared__ double lds[];   // G[ng*ng], ti[ng], [ng][UT] Tbar, [ng][UT] Told
   const int ng = p.ng, nv = p.nv, no = p.no;
   const double *Gs = p.G, *tis = p.ti;
   double *col = lds + ng * ng + ng;
+  double *ocol = col + ng * UT;
   if (STAGE_LDS) {
     for (int k = threadIdx.x; k < ng * ng; k += UT) lds[k] = p.G[k];
     for (int k = threadIdx.x; k < ng; k += UT) lds[ng * ng + k] = p.ti[k];
@@ -103,8 +104,13 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
       d = p.ev[q.a] + p.ev[q.b] - p.eo[q.i] - p.eo[q.j];
     else
       d = p.ev[q.a] - p.eo[q.i];
-    if (STAGE_LDS)
+    if (STAGE_LDS) {
+      // issue every load of this element before any arithmetic
       for (int x = 0; x < ng; ++x) col[x * UT + threadIdx.x] = src[x * per + e];
+      if (mode == 1)
+        for (int x = 0; x < ng; ++x)
+          ocol[x * UT + threadIdx.x] = p.told[x * per + e];
+    }
     const double r = p.uniform ? exp(-p.delta * d) : 0.0;
     double eacc = 0.0;
     for (int y = 0; y < ng; ++y) {
@@ -131,7 +137,7 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
       if (mode == 0) {
         p.tbar[y * per + e] = tn;
       } else {
-        const double to = p.told[y * per + e];
+        const double to = STAGE_LDS ? ocol[y * UT + threadIdx.x] : p.told[y * per + e];
         const double df = tn - to;
         dsum += df * df;
         const double tnew = p.damp * to + (1.0 - p.damp) * tn;
@@ -158,14 +164,14 @@ int launch_update(const UpdateArgs &a, int mode, hipStream_t s) {
   const int blocks =
       (int)std::max<int64_t>(1, std::min<int64_t>((per + UT - 1) / UT, MAX_BLOCKS));
   if (a.ng <= MAX_LDS_NG) {
-    const size_t lds = sizeof(double) * (a.ng * a.ng + a.ng + a.ng * UT);
+    const size_t lds = sizeof(double) * (a.ng * a.ng + a.ng + 2 * a.ng * UT);
     static bool attr = false;
     if (!attr) {
       FTCC_CHECK_HIP(hipFuncSetAttribute(
           (const void *)update_kernel<DOUBLES, true>,
           hipFuncAttributeMaxDynamicSharedMemorySize,
           (int)(sizeof(double) * (MAX_LDS_NG * MAX_LDS_NG + MAX_LDS_NG +
-                                  MAX_LDS_NG * UT))));
+                                  2 * MAX_LDS_NG * UT))));
       attr = true;
     }
     hipLaunchKernelGGL((update_kernel<DOUBLES, true>), dim3(blocks), dim3(UT),
