@@ -157,12 +157,100 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
   if (mode == 1) block_reduce3(dsum, nsum, esum, p.partials);
 }
 
+// Register-resident form for equally spaced grids of up to 32 points.  The
+// grid size is a compile-time constant, so Tbar[0..NG) lives in registers
+// (all NG loads of an element are issued before its arithmetic), G and the
+// abscissae are uniform across the block and are read through scalar loads
+// with constant offsets, and the propagator r^(y-x) is the same running
+// product from x = y-1 downwards as in update_kernel.  No LDS is used, so
+// occupancy is set by registers alone.
+template <bool DOUBLES, int NG>
+__global__ __launch_bounds__(UT) void update_reg_kernel(UpdateArgs p,
+                                                        int mode) {
+  const int nv = p.nv, no = p.no;
+  const int64_t per = DOUBLES ? (int64_t)nv * nv * no * no : (int64_t)nv * no;
+  const int64_t per1 = (int64_t)nv * no;
+  const double *__restrict__ G = p.G;
+  const double *__restrict__ src = p.src ? p.src : p.tbar;
+  double dsum = 0.0, nsum = 0.0, esum = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
+       e += (int64_t)gridDim.x * UT) {
+    const Elem q = decode<DOUBLES>(e, nv, no);
+    double d;
+    if (p.D)
+      d = p.D[e];
+    else if (DOUBLES)
+      d = p.ev[q.a] + p.ev[q.b] - p.eo[q.i] - p.eo[q.j];
+    else
+      d = p.ev[q.a] - p.eo[q.i];
+    double tb[NG];
+#pragma unroll
+    for (int x = 0; x < NG; ++x) tb[x] = src[x * per + e];
+    const double r = exp(-p.delta * d);
+    double eacc = 0.0;
+#pragma unroll
+    for (int y = 0; y < NG; ++y) {
+      double tn = 0.0;
+      double w = 1.0;
+#pragma unroll
+      for (int x = y - 1; x >= 0; --x) {
+        w *= r;
+        tn += G[y * NG + x] * (w * tb[x]);
+      }
+#pragma unroll
+      for (int x = y; x < NG; ++x) tn += G[y * NG + x] * tb[x];
+      if (mode == 0) {
+        p.tbar[y * per + e] = tn;
+      } else {
+        const double to = p.told[y * per + e];
+        const double df = tn - to;
+        dsum += df * df;
+        const double tnew = p.damp * to + (1.0 - p.damp) * tn;
+        p.told[y * per + e] = tnew;
+        nsum += tnew * tnew;
+        if (DOUBLES) {
+          const double *t1 = p.t1new + y * per1;
+          const double qq = t1[q.a * no + q.i] * t1[q.b * no + q.j];
+          eacc += p.g[y] * (0.25 * tnew + 0.5 * qq);
+        } else {
+          eacc += p.g[y] * tnew;
+        }
+      }
+    }
+    if (mode == 1) esum += eacc * p.eint[e];
+  }
+  if (mode == 1) block_reduce3(dsum, nsum, esum, p.partials);
+}
+
+template <bool DOUBLES, int NG>
+void launch_reg(const UpdateArgs &a, int mode, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL((update_reg_kernel<DOUBLES, NG>), dim3(blocks), dim3(UT),
+                     0, s, a, mode);
+}
+
+template <bool DOUBLES, int NG = 1>
+bool dispatch_reg(const UpdateArgs &a, int mode, int blocks, hipStream_t s) {
+  if constexpr (NG > 32) {
+    return false;
+  } else {
+    if (a.ng == NG) {
+      launch_reg<DOUBLES, NG>(a, mode, blocks, s);
+      return true;
+    }
+    return dispatch_reg<DOUBLES, NG + 1>(a, mode, blocks, s);
+  }
+}
+
 template <bool DOUBLES>
 int launch_update(const UpdateArgs &a, int mode, hipStream_t s) {
   const int64_t per = DOUBLES ? (int64_t)a.nv * a.nv * a.no * a.no
                               : (int64_t)a.nv * a.no;
   const int blocks =
       (int)std::max<int64_t>(1, std::min<int64_t>((per + UT - 1) / UT, MAX_BLOCKS));
+  if (a.uniform && dispatch_reg<DOUBLES>(a, mode, blocks, s)) {
+    FTCC_CHECK_HIP(hipGetLastError());
+    return blocks;
+  }
   if (a.ng <= MAX_LDS_NG) {
     const size_t lds = sizeof(double) * (a.ng * a.ng + a.ng + 2 * a.ng * UT);
     static bool attr = false;
