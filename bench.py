@@ -49,7 +49,31 @@ def parse():
     ap.add_argument("--cpu-budget-s", type=float, default=90.0,
                     help="predicted CPU sample time above which the sample "
                          "is taken at a smaller n and extrapolated")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="before the warmup, copy a buffer of this many MiB "
+                         "once on the device: the known-size copy kernel that "
+                         "hardware byte counters are scaled by")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sbgemm_traffic_ueg_n114.txt")
+
+
+def measured_traffic(precision, n, ng):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    FETCH_SIZE/WRITE_SIZE profile of this workload (a line
+    'traffic_bytes_per_launch <precision> <n> <ng> <bytes>'), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                w = line.split()
+                if (len(w) == 5 and w[0] == "traffic_bytes_per_launch"
+                        and w[1] == precision and int(w[2]) == n
+                        and int(w[3]) == ng):
+                    return float(w[4])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_threads():
@@ -158,6 +182,14 @@ def main():
     skip = zero_slices(eng, T1, T2, G)
     log("device set-up done in %.1f s (context %.1f GB)"
         % (time.time() - t0, eng.workspace_bytes() / 1e9))
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        src = torch.ones(nb, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy of %d MiB done" % args.copy_calibration_mib)
+        del src, dst
 
     for w in range(args.warmup):
         t = time.time()
@@ -208,10 +240,11 @@ def main():
     peak = PEAK_TFLOPS[args.precision]
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-        "unit": "TFLOP/s", "frac": round(achieved / peak, 5), "traffic": None,
-        "kernel": "gemm_%s (the six O(n^6) products of the residual)"
-                  % ("bf16x3_kernel" if args.precision == "bf16x3"
-                     else "mfma_kernel<%s>" % ("double" if args.precision == "fp64" else "float")),
+        "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
+        "traffic": measured_traffic(args.precision, n, ng),
+        "kernel": "%s (the six O(n^6) products of the residual)"
+                  % ("sbgemm_kernel" if args.precision == "bf16x3"
+                     else "gemm_mfma_kernel<%s>" % ("double" if args.precision == "fp64" else "float")),
         "launch_flops": launch_flops, "avg_launch_ms": round(avg_ms, 4),
         "launches": launches,
         "mfma_ops_per_product": MFMA_PER_PRODUCT[args.precision],

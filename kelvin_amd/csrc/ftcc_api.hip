@@ -642,6 +642,8 @@ UpdateArgs upd(ftcc_ctx *c) {
   a.src = nullptr;
   a.uniform = c->grid.uniform ? 1 : 0;
   a.delta = c->grid.delta;
+  a.rec = c->grid.rec;
+  a.rc = c->grid.d_rc;
   return a;
 }
 
@@ -700,6 +702,7 @@ int ftcc_destroy(ftcc_ctx *c) {
   if (c->grid.d_ti) (void)hipFree(c->grid.d_ti);
   if (c->grid.d_g) (void)hipFree(c->grid.d_g);
   if (c->grid.d_G) (void)hipFree(c->grid.d_G);
+  if (c->grid.d_rc) (void)hipFree(c->grid.d_rc);
   if (c->host_red) (void)hipHostFree(c->host_red);
   delete c;
   return FTCC_OK;
@@ -865,7 +868,8 @@ int ftcc_set_grid(ftcc_ctx *c, int ng, const double *ti, const double *G) {
       FTCC_CHECK_HIP(hipFree(gr.d_ti));
       FTCC_CHECK_HIP(hipFree(gr.d_g));
       FTCC_CHECK_HIP(hipFree(gr.d_G));
-      gr.d_ti = gr.d_g = gr.d_G = nullptr;
+      FTCC_CHECK_HIP(hipFree(gr.d_rc));
+      gr.d_ti = gr.d_g = gr.d_G = gr.d_rc = nullptr;
     }
     c->ng = ng;
     gr.ng = ng;
@@ -881,11 +885,39 @@ int ftcc_set_grid(ftcc_ctx *c, int ng, const double *ti, const double *G) {
       if (std::fabs(ti[k] - ti[0] - k * gr.delta) > 8e-16 * (tmax + 1e-300) * (k + 1))
         gr.uniform = false;
     if (!(gr.delta > 0.0)) gr.uniform = false;
+    // Simpson ('lin'), midpoint and 'L' weight matrices: zero above the
+    // diagonal, each row equal to the row s above it left of the last three
+    // columns; integration then runs as a recurrence of step s.  The
+    // comparisons are exact: the rules build each row by copying another.
+    auto above = [&](int y, int s, int x) {
+      return y - s >= 0 ? G[(size_t)(y - s) * ng + x] : 0.0;
+    };
+    auto fits = [&](int s) {
+      for (int y = 0; y < ng; ++y)
+        for (int x = 0; x < ng; ++x) {
+          const double v = G[(size_t)y * ng + x];
+          if (x > y && v != 0.0) return false;
+          if (x < y - 2 && v != above(y, s, x)) return false;
+        }
+      return true;
+    };
+    gr.rec = 0;
+    if (gr.uniform && ng <= MAX_REC_NG) gr.rec = fits(1) ? 1 : fits(2) ? 2 : 0;
+    std::vector<double> rc(3 * (size_t)ng, 0.0);
+    for (int y = 0; y < ng && gr.rec; ++y)
+      for (int k = 0; k < 3; ++k) {
+        const int x = y - 2 + k;
+        if (x >= 0)
+          rc[(size_t)k * ng + y] = G[(size_t)y * ng + x] - above(y, gr.rec, x);
+      }
     if (!gr.d_ti) {
       FTCC_CHECK_HIP(hipMalloc(&gr.d_ti, 8 * ng));
       FTCC_CHECK_HIP(hipMalloc(&gr.d_g, 8 * ng));
       FTCC_CHECK_HIP(hipMalloc(&gr.d_G, 8 * (size_t)ng * ng));
+      FTCC_CHECK_HIP(hipMalloc(&gr.d_rc, 8 * 3 * (size_t)ng));
     }
+    FTCC_CHECK_HIP(hipMemcpyAsync(gr.d_rc, rc.data(), 8 * 3 * (size_t)ng,
+                                  hipMemcpyHostToDevice, c->stream));
     FTCC_CHECK_HIP(hipMemcpyAsync(gr.d_ti, ti, 8 * ng, hipMemcpyHostToDevice,
                                   c->stream));
     FTCC_CHECK_HIP(hipMemcpyAsync(gr.d_G, G, 8 * (size_t)ng * ng,
@@ -910,7 +942,7 @@ int ftcc_integrate(ftcc_ctx *c, double *T1, double *T2) {
     const int64_t VO = (int64_t)c->nv * c->no;
     // large grids: integrate from a transient copy of Tbar
     double *copy = nullptr;
-    if (update_needs_copy(c->ng)) {
+    if (update_needs_copy(a)) {
       const int64_t elems = (int64_t)c->ng * (T2 ? VO * VO : VO);
       FTCC_CHECK_HIP(hipMalloc(&copy, 8 * elems));
     }

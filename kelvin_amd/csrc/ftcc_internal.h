@@ -157,8 +157,14 @@ struct Grid {
   std::vector<double> ti, G;
   bool uniform = false;         // tau_x - tau_y == (x - y) * delta to ~1 ulp
   double delta = 0.0;
-  double *d_ti = nullptr, *d_g = nullptr, *d_G = nullptr;
+  // step s of the integration recurrence (0: none): uniform, G zero above
+  // the diagonal and G[y][x] == G[y-s][x] for x < y-2; coefficients d_rc
+  // (a[ng], b[ng], c[ng])
+  int rec = 0;
+  double *d_ti = nullptr, *d_g = nullptr, *d_G = nullptr, *d_rc = nullptr;
 };
+// Largest grid integrated by the recurrence (its coefficients sit in LDS).
+constexpr int MAX_REC_NG = 1024;
 
 // mode 0: integrate in place (Tbar -> T).  mode 1: integrate Tbar, then damp
 // into Told with norms and energy partials.
@@ -175,6 +181,8 @@ struct UpdateArgs {
   double damp;
   int uniform;                  // grid equally spaced: tau_x - tau_y =
   double delta;                 //   (x - y) * delta
+  int rec;                      // step (1, 2) of the integration recurrence
+  const double *rc;             //   (0: none), coefficients a, b, c [ng]
   const double *t1new;          // mode 1 T2 pass: updated T1 (ng,nv,no)
   const double *eint;           // energy integral in amplitude layout
                                 // (F.ov^T for T1, oovv^T for T2)
@@ -182,9 +190,9 @@ struct UpdateArgs {
 };
 int update_t1(const UpdateArgs &a, int mode, hipStream_t s);   // returns #blocks
 int update_t2(const UpdateArgs &a, int mode, hipStream_t s);
-// true when the grid is too large to stage Tbar in LDS (in-place
-// integration then needs a source copy in UpdateArgs::src)
-bool update_needs_copy(int ng);
+// true when in-place integration needs a source copy in UpdateArgs::src
+// (the general kernel on a grid too large to stage Tbar in LDS)
+bool update_needs_copy(const UpdateArgs &a);
 // out[k] = sum over blocks of partials[3*b+k], k=0..2 (fixed order)
 void reduce_partials(const double *partials, int nblocks, double *out3,
                      hipStream_t s);

@@ -157,21 +157,34 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
   if (mode == 1) block_reduce3(dsum, nsum, esum, p.partials);
 }
 
-// Register-resident form for equally spaced grids of up to 32 points.  The
-// grid size is a compile-time constant, so Tbar[0..NG) lives in registers
-// (all NG loads of an element are issued before its arithmetic), G and the
-// abscissae are uniform across the block and are read through scalar loads
-// with constant offsets, and the propagator r^(y-x) is the same running
-// product from x = y-1 downwards as in update_kernel.  No LDS is used, so
-// occupancy is set by registers alone.
-template <bool DOUBLES, int NG>
-__global__ __launch_bounds__(UT) void update_reg_kernel(UpdateArgs p,
+// Recurrence form.  On an equally spaced grid whose weight matrix is zero
+// above the diagonal and whose rows repeat the row s above them left of the
+// last three columns (G[y][x] == G[y-s][x] for x < y-2; kelvin's Simpson
+// rule builds G that way with s = 2, kelvin/quadrature.py:32-41, the
+// midpoint and 'L' rules with s = 1), the causal sum obeys
+//     T_y = r^s T_{y-s} + a_y r^2 Tbar_{y-2} + b_y r Tbar_{y-1} + c_y Tbar_y
+// with r = exp(-delta d) and (a_y, b_y, c_y) = G[y][y-2..y] - G[y-s][y-2..y]
+// (rows above the first taken as zero).  Each element then costs a few
+// flops per grid point, and the kernel is bound by its HBM streams alone:
+// Tbar in (and T out in mode 0), Told in and out in mode 1.  Grid points are
+// taken in batches of RCH so that every load of a batch is in flight
+// together.  In-place integration is safe: slice y is read before it is
+// written and earlier slices are carried in registers.
+constexpr int RCH = 8;
+
+template <bool DOUBLES>
+__global__ __launch_bounds__(UT) void update_rec_kernel(UpdateArgs p,
                                                         int mode) {
-  const int nv = p.nv, no = p.no;
+  extern __shared__ double co[];   // a[ng], b[ng], c[ng], g[ng]
+  const int ng = p.ng, nv = p.nv, no = p.no;
+  for (int k = threadIdx.x; k < 3 * ng; k += UT) co[k] = p.rc[k];
+  if (mode == 1)
+    for (int k = threadIdx.x; k < ng; k += UT) co[3 * ng + k] = p.g[k];
+  __syncthreads();
+  const double *ca = co, *cb = co + ng, *cc = co + 2 * ng, *cg = co + 3 * ng;
   const int64_t per = DOUBLES ? (int64_t)nv * nv * no * no : (int64_t)nv * no;
   const int64_t per1 = (int64_t)nv * no;
-  const double *__restrict__ G = p.G;
-  const double *__restrict__ src = p.src ? p.src : p.tbar;
+  const double *src = p.src ? p.src : p.tbar;
   double dsum = 0.0, nsum = 0.0, esum = 0.0;
   for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
        e += (int64_t)gridDim.x * UT) {
@@ -183,37 +196,46 @@ __global__ __launch_bounds__(UT) void update_reg_kernel(UpdateArgs p,
       d = p.ev[q.a] + p.ev[q.b] - p.eo[q.i] - p.eo[q.j];
     else
       d = p.ev[q.a] - p.eo[q.i];
-    double tb[NG];
+    const double r = exp(-p.delta * d), r2 = r * r;
+    const double rs = p.rec == 1 ? r : r2;
+    double tbm1 = 0.0, tbm2 = 0.0, tm1 = 0.0, tm2 = 0.0, eacc = 0.0;
+    for (int y0 = 0; y0 < ng; y0 += RCH) {
+      double tb[RCH], to[RCH];
 #pragma unroll
-    for (int x = 0; x < NG; ++x) tb[x] = src[x * per + e];
-    const double r = exp(-p.delta * d);
-    double eacc = 0.0;
+      for (int k = 0; k < RCH; ++k)
+        if (y0 + k < ng) tb[k] = src[(y0 + k) * per + e];
+      if (mode == 1) {
 #pragma unroll
-    for (int y = 0; y < NG; ++y) {
-      double tn = 0.0;
-      double w = 1.0;
-#pragma unroll
-      for (int x = y - 1; x >= 0; --x) {
-        w *= r;
-        tn += G[y * NG + x] * (w * tb[x]);
+        for (int k = 0; k < RCH; ++k)
+          if (y0 + k < ng) to[k] = p.told[(y0 + k) * per + e];
       }
 #pragma unroll
-      for (int x = y; x < NG; ++x) tn += G[y * NG + x] * tb[x];
-      if (mode == 0) {
-        p.tbar[y * per + e] = tn;
-      } else {
-        const double to = p.told[y * per + e];
-        const double df = tn - to;
-        dsum += df * df;
-        const double tnew = p.damp * to + (1.0 - p.damp) * tn;
-        p.told[y * per + e] = tnew;
-        nsum += tnew * tnew;
-        if (DOUBLES) {
-          const double *t1 = p.t1new + y * per1;
-          const double qq = t1[q.a * no + q.i] * t1[q.b * no + q.j];
-          eacc += p.g[y] * (0.25 * tnew + 0.5 * qq);
-        } else {
-          eacc += p.g[y] * tnew;
+      for (int k = 0; k < RCH; ++k) {
+        const int y = y0 + k;
+        if (y < ng) {
+          const double tn = rs * (p.rec == 1 ? tm1 : tm2) +
+                            ca[y] * (r2 * tbm2) + cb[y] * (r * tbm1) +
+                            cc[y] * tb[k];
+          tm2 = tm1;
+          tm1 = tn;
+          tbm2 = tbm1;
+          tbm1 = tb[k];
+          if (mode == 0) {
+            p.tbar[y * per + e] = tn;
+          } else {
+            const double df = tn - to[k];
+            dsum += df * df;
+            const double tnew = p.damp * to[k] + (1.0 - p.damp) * tn;
+            p.told[y * per + e] = tnew;
+            nsum += tnew * tnew;
+            if (DOUBLES) {
+              const double *t1 = p.t1new + y * per1;
+              const double qq = t1[q.a * no + q.i] * t1[q.b * no + q.j];
+              eacc += cg[y] * (0.25 * tnew + 0.5 * qq);
+            } else {
+              eacc += cg[y] * tnew;
+            }
+          }
         }
       }
     }
@@ -222,32 +244,15 @@ __global__ __launch_bounds__(UT) void update_reg_kernel(UpdateArgs p,
   if (mode == 1) block_reduce3(dsum, nsum, esum, p.partials);
 }
 
-template <bool DOUBLES, int NG>
-void launch_reg(const UpdateArgs &a, int mode, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL((update_reg_kernel<DOUBLES, NG>), dim3(blocks), dim3(UT),
-                     0, s, a, mode);
-}
-
-template <bool DOUBLES, int NG = 1>
-bool dispatch_reg(const UpdateArgs &a, int mode, int blocks, hipStream_t s) {
-  if constexpr (NG > 32) {
-    return false;
-  } else {
-    if (a.ng == NG) {
-      launch_reg<DOUBLES, NG>(a, mode, blocks, s);
-      return true;
-    }
-    return dispatch_reg<DOUBLES, NG + 1>(a, mode, blocks, s);
-  }
-}
-
 template <bool DOUBLES>
 int launch_update(const UpdateArgs &a, int mode, hipStream_t s) {
   const int64_t per = DOUBLES ? (int64_t)a.nv * a.nv * a.no * a.no
                               : (int64_t)a.nv * a.no;
   const int blocks =
       (int)std::max<int64_t>(1, std::min<int64_t>((per + UT - 1) / UT, MAX_BLOCKS));
-  if (a.uniform && dispatch_reg<DOUBLES>(a, mode, blocks, s)) {
+  if (a.rec) {
+    hipLaunchKernelGGL((update_rec_kernel<DOUBLES>), dim3(blocks), dim3(UT),
+                       sizeof(double) * 4 * a.ng, s, a, mode);
     FTCC_CHECK_HIP(hipGetLastError());
     return blocks;
   }
@@ -360,7 +365,9 @@ int nblocks_for(int64_t n) {
 
 }  // namespace
 
-bool update_needs_copy(int ng) { return ng > MAX_LDS_NG; }
+bool update_needs_copy(const UpdateArgs &a) {
+  return !a.rec && a.ng > MAX_LDS_NG;
+}
 
 int update_t1(const UpdateArgs &a, int mode, hipStream_t s) {
   return launch_update<false>(a, mode, s);

@@ -195,13 +195,36 @@ def test_integrators_match_oracle(quad):
     assert rel(to_np(a2), oracle.int_tbar2(ng, t2, ti, D2, G)) < 1e-13
 
 
-def test_integrator_large_grid_global_path():
-    """ng above the LDS staging limit uses the global-memory path."""
+def test_integrator_recurrence_and_general_paths():
+    """A Simpson G is integrated by the two-step recurrence; the same G with
+    one entry moved by an ulp fails the structure test and takes the general
+    kernel.  Both agree with the oracle's literal sum."""
+    from kelvin_amd import quadrature
+    dev()
+    rng = numpy.random.default_rng(14)
+    ng, nv, no = 24, 5, 3
+    ti, g, G = quadrature.ft_quad(ng, 20.0, 'lin')
+    Gp = G.copy()
+    Gp[9, 2] = numpy.nextafter(Gp[9, 2], 1.0)
+    ev, eo = rng.uniform(-1, 1, nv), rng.uniform(-1, 1, no)
+    D1, D2 = oracle.D1(ev, eo), oracle.D2(ev, eo)
+    t1, t2 = random_T(ng, nv, no, rng)
+    for GG in (G, Gp):
+        a1 = quadrature.int_tbar1(ng, t1, ti, D1, GG)
+        a2 = quadrature.int_tbar2(ng, t2, ti, D2, GG)
+        assert rel(to_np(a1), oracle.int_tbar1(ng, t1, ti, D1, GG)) < 1e-13
+        assert rel(to_np(a2), oracle.int_tbar2(ng, t2, ti, D2, GG)) < 1e-13
+
+
+@pytest.mark.parametrize("quad", ['lin', 'sin'])
+def test_integrator_large_grid(quad):
+    """ng = 80: 'lin' runs the recurrence in place; 'sin' (unequal spacing)
+    exceeds the LDS staging limit and integrates from a source copy."""
     from kelvin_amd import quadrature
     dev()
     rng = numpy.random.default_rng(12)
     ng, nv, no = 80, 2, 2
-    ti, g, G = quadrature.ft_quad(ng, 3.0, 'lin')
+    ti, g, G = quadrature.ft_quad(ng, 3.0, quad)
     ev, eo = rng.uniform(-1, 1, nv), rng.uniform(-1, 1, no)
     t1, t2 = random_T(ng, nv, no, rng)
     D2 = oracle.D2(ev, eo)

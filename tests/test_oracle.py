@@ -217,6 +217,38 @@ def test_integrator_literal_equals_scan():
         assert numpy.abs(a - b).max() < 1e-13
 
 
+@pytest.mark.parametrize("quad,step", [('lin', 2), ('mid', 1), ('L', 1)])
+def test_recurrence_equals_literal_integrator(quad, step):
+    """The device integrates equally spaced grids whose G rows repeat the row
+    `step` above them left of the last three columns (Simpson: 2; midpoint
+    and 'L': 1) by T_y = r^s T_{y-s} + sum_{x=y-2..y} (G[y,x] - G[y-s,x])
+    r^(y-x) Tbar_x.  Restated here in numpy, it must equal the literal
+    causal sum, including for D < 0 (r > 1)."""
+    rng = numpy.random.default_rng(5)
+    ng, n = 24, 3
+    e = rng.uniform(-1.5, 1.5, n)
+    d2 = ftcc.D2(e, e)
+    tb = rng.normal(size=(ng, n, n, n, n))
+    ti, g, G = ftcc.ft_quad(ng, 8.0, quad)
+    Gs = numpy.vstack([numpy.zeros((step, ng)), G])   # Gs[y + step] = G[y]
+    for y in range(ng):                               # the structure relied on
+        assert numpy.array_equal(G[y, :max(0, y - 2)], Gs[y, :max(0, y - 2)])
+    assert not numpy.triu(G, 1).any()
+    delta = ti[1] - ti[0]
+    r = numpy.exp(-delta * d2)
+    out = numpy.zeros_like(tb)
+    for y in range(ng):
+        t = r ** step * out[y - step] if y >= step else numpy.zeros_like(d2)
+        for x in range(max(0, y - 2), y + 1):
+            t = t + (G[y, x] - Gs[y, x]) * r ** (y - x) * tb[x]
+        out[y] = t
+    ref = ftcc.int_tbar2(ng, tb, ti, d2, G)
+    assert numpy.abs(out - ref).max() < 1e-12 * numpy.abs(ref).max()
+    if step == 2:          # the Simpson rows do not repeat the row just above
+        assert not all(numpy.array_equal(G[y, :y - 2], G[y - 1, :y - 2])
+                       for y in range(3, ng))
+
+
 # --- systems ----------------------------------------------------------------
 def test_ueg_integral_symmetries():
     ueg = ftcc.UEGSystem(0.1, 3.8855, 7.4, 2.0, norb=19)
