@@ -1080,7 +1080,33 @@ int ftcc_gemm(ftcc_ctx *c, int M, int N, int K, int batch, double alpha,
     require(A && B && C, "null GEMM operand");
     Gemm g = mk(M, N, K, alpha, beta, A, sAm, sAk, sAb, B, sBk, sBn, sBb, C,
                 sCm, sCn, sCb, batch);
-    run(c, g, precision);
+    if (precision != FTCC_PREC_BF16X3 || (int64_t)M * N * K == 0) {
+      run(c, g, precision);
+      return;
+    }
+    // bf16x3: split both operands into transient K-contiguous buffers and
+    // run the split-bf16 kernel of the O(n^6) products
+    SplitMat As, Bs;
+    As.ld = Bs.ld = split_ld(K);
+    As.bstride = (int64_t)M * As.ld;
+    Bs.bstride = (int64_t)N * Bs.ld;
+    const size_t abytes = 2 * (size_t)As.bstride * batch;
+    const size_t bbytes = 2 * (size_t)Bs.bstride * batch;
+    char *buf = nullptr;
+    FTCC_CHECK_HIP(hipMalloc(&buf, 2 * (abytes + bbytes)));
+    As.hi = (__bf16 *)buf;
+    As.lo = (__bf16 *)(buf + abytes);
+    Bs.hi = (__bf16 *)(buf + 2 * abytes);
+    Bs.lo = (__bf16 *)(buf + 2 * abytes + bbytes);
+    split_convert(A, sAm, sAk, sAb, M, K, As, batch, c->stream);
+    split_convert(B, sBn, sBk, sBb, N, K, Bs, batch, c->stream);
+    timed(c, true, gemm_flops(g), [&] {
+      sbgemm(M, N, K, batch, alpha, As, Bs, beta, C, sCm, sCn, sCb,
+             c->stream);
+    });
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    FTCC_CHECK_HIP(e);
   });
 }
 

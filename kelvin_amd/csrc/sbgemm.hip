@@ -5,17 +5,22 @@
 // lo = bf16(x - hi), as two bf16 arrays with K contiguous (A as [M][ld],
 // B transposed as [N][ld]); K is zero-padded to a multiple of 32 so the
 // inner loop needs no bounds checks along K.  C = alpha * A.B + beta * C is
-// accumulated in FP32 by v_mfma_f32_32x32x16_bf16 from hi*hi + hi*lo + lo*hi
-// and written as FP64.
+// accumulated in FP32 by v_mfma_f32_32x32x16_bf16 from lo*hi + hi*lo + hi*hi
+// (in that order for every accumulator) and written as FP64.
 //
-// Tiling: a 512-thread workgroup (8 wave64s) owns a 256x256 tile of C, each
-// wave a 128x64 quadrant (4x2 MFMA 32x32 blocks, 128 accumulator VGPRs).  K
-// advances in slabs of 32 through double-buffered LDS; each LDS row holds the
-// 32 hi values, then the 32 lo values, then 8 bf16 of padding (144 B rows:
-// the 8 lanes of one 128 B LDS cycle land on distinct banks).  The next slab
-// is prefetched into registers with 16-byte loads while the current one feeds
-// the MFMAs.  Workgroup ids are remapped so that each XCD works on a compact
-// group of tiles (shared A/B panels stay in that XCD's L2).
+// Tiling: a 512-thread workgroup (8 wave64s, two per SIMD) owns a 256x256
+// tile of C, each wave a 128x64 quadrant (4x2 MFMA 32x32 blocks, 128 FP32
+// accumulators per lane); per 16-deep K step a wave reads its A and B
+// fragments (hi and lo) from LDS and issues the three MFMAs of each block in
+// turn.  K advances in slabs of 32 through double-buffered LDS; each LDS row
+// holds the 32 hi values, then the 32 lo values, then 8 bf16 of padding
+// (144 B rows: 16 lanes reading one row each land on distinct banks).  The
+// next slab is prefetched into registers with 16-byte loads while the
+// current one feeds the MFMAs.  Workgroup ids are remapped so that each XCD
+// works on a compact group of tiles (shared A/B panels stay in that XCD's
+// L2).  Measured alternatives -- 4 waves of 128x128 with accumulators in
+// AGPRs, with and without a software pipeline that double-buffers the
+// fragments, and product-major MFMA order -- were no faster on gfx950.
 #include "ftcc_internal.h"
 
 #include <algorithm>
@@ -27,7 +32,7 @@ typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 typedef float f16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 256, BN = 256, BK = 32, NT = 512, LDR = 72;
+constexpr int BM = 256, BN = 256, BK = 32, LDR = 72;
 constexpr int GROUP_M = 8;
 constexpr size_t LDS_BYTES = 2 * 2 * (size_t)BM * LDR * sizeof(__bf16);
 
@@ -56,13 +61,14 @@ __device__ inline void tile_of(const SbDev &p, int &bt, int &tm, int &tn) {
   tn = (t % span) / gsz;
 }
 
-// 2048 16-byte chunks per operand slab: (array, row, quarter) per chunk
+// 2048 16-byte chunks per 256-row operand slab: (array, row, quarter) each
+template <int NT>
 __device__ inline void load_operand(const __bf16 *hi, const __bf16 *lo,
                                     int64_t ld, int row0, int rows, int k0,
-                                    u4_t reg[4]) {
+                                    u4_t reg[2048 / NT]) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < 2048 / NT; ++r) {
     const int c = t + NT * r;
     const int arr = c >> 10, row = (c >> 2) & 255, kq = c & 3;
     const __bf16 *src = (arr ? lo : hi) + (int64_t)(row0 + row) * ld + k0 +
@@ -74,17 +80,19 @@ __device__ inline void load_operand(const __bf16 *hi, const __bf16 *lo,
   }
 }
 
-__device__ inline void store_operand(__bf16 *lds, const u4_t reg[4]) {
+template <int NT>
+__device__ inline void store_operand(__bf16 *lds, const u4_t reg[2048 / NT]) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < 2048 / NT; ++r) {
     const int c = t + NT * r;
     const int arr = c >> 10, row = (c >> 2) & 255, kq = c & 3;
     *(u4_t *)(lds + row * LDR + arr * BK + 8 * kq) = reg[r];
   }
 }
 
-__global__ __launch_bounds__(NT, 1) void sbgemm_kernel(SbDev p) {
+__global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
+  constexpr int WN = 4, NT = 512, RT = 4, CT = 2;
   extern __shared__ __bf16 lds[];   // [2 stages][A|B][BM*LDR]
   int bt, tm, tn;
   tile_of(p, bt, tm, tn);
@@ -92,52 +100,52 @@ __global__ __launch_bounds__(NT, 1) void sbgemm_kernel(SbDev p) {
   const __bf16 *Ahi = p.Ahi + bt * p.sAb, *Alo = p.Alo + bt * p.sAb;
   const __bf16 *Bhi = p.Bhi + bt * p.sBb, *Blo = p.Blo + bt * p.sBb;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 2, wn = wave & 3;
+  const int wm = wave / WN, wn = wave % WN;
   const int li = lane & 31, lg = lane >> 5;
   const int nslab = p.Kp / BK;
 
-  f16_t acc[4][2];
+  f16_t acc[RT][CT];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RT; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < CT; ++j)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
 
-  u4_t ra[4], rb[4];
+  u4_t ra[2048 / NT], rb[2048 / NT];
   constexpr int STAGE = 2 * BM * LDR;   // bf16 per stage (A then B)
-  load_operand(Ahi, Alo, p.lda, m0, p.M, 0, ra);
-  load_operand(Bhi, Blo, p.ldb, n0, p.N, 0, rb);
-  store_operand(lds, ra);
-  store_operand(lds + BM * LDR, rb);
+  load_operand<NT>(Ahi, Alo, p.lda, m0, p.M, 0, ra);
+  load_operand<NT>(Bhi, Blo, p.ldb, n0, p.N, 0, rb);
+  store_operand<NT>(lds, ra);
+  store_operand<NT>(lds + BM * LDR, rb);
   __syncthreads();
   for (int s = 0; s < nslab; ++s) {
     const __bf16 *As = lds + (s & 1) * STAGE;
     const __bf16 *Bs = As + BM * LDR;
     if (s + 1 < nslab) {
-      load_operand(Ahi, Alo, p.lda, m0, p.M, (s + 1) * BK, ra);
-      load_operand(Bhi, Blo, p.ldb, n0, p.N, (s + 1) * BK, rb);
+      load_operand<NT>(Ahi, Alo, p.lda, m0, p.M, (s + 1) * BK, ra);
+      load_operand<NT>(Bhi, Blo, p.ldb, n0, p.N, (s + 1) * BK, rb);
     }
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
-      bf8_t ah[4], al[4], bh[2], bl[2];
+      bf8_t ah[RT], al[RT], bh[CT], bl[CT];
       const int koff = ks * 16 + 8 * lg;
 #pragma unroll
-      for (int rt = 0; rt < 4; ++rt) {
+      for (int rt = 0; rt < RT; ++rt) {
         const __bf16 *a = As + (wm * 128 + rt * 32 + li) * LDR + koff;
         ah[rt] = *(const bf8_t *)a;
         al[rt] = *(const bf8_t *)(a + BK);
       }
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
-        const __bf16 *b = Bs + (wn * 64 + ct * 32 + li) * LDR + koff;
+      for (int ct = 0; ct < CT; ++ct) {
+        const __bf16 *b = Bs + (wn * (32 * CT) + ct * 32 + li) * LDR + koff;
         bh[ct] = *(const bf8_t *)b;
         bl[ct] = *(const bf8_t *)(b + BK);
       }
 #pragma unroll
-      for (int rt = 0; rt < 4; ++rt)
+      for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
+        for (int ct = 0; ct < CT; ++ct) {
           acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
               al[rt], bh[ct], acc[rt][ct], 0, 0, 0);
           acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
@@ -148,8 +156,8 @@ __global__ __launch_bounds__(NT, 1) void sbgemm_kernel(SbDev p) {
     }
     if (s + 1 < nslab) {
       __bf16 *nxt = lds + ((s + 1) & 1) * STAGE;
-      store_operand(nxt, ra);
-      store_operand(nxt + BM * LDR, rb);
+      store_operand<NT>(nxt, ra);
+      store_operand<NT>(nxt + BM * LDR, rb);
     }
     __syncthreads();
   }
@@ -157,10 +165,10 @@ __global__ __launch_bounds__(NT, 1) void sbgemm_kernel(SbDev p) {
   // lane holds column li, rows 8*(v/4) + 4*lg + v%4 of each 32x32 block
   double *C = p.C + bt * p.sCb;
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
+  for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct) {
-      const int col = n0 + wn * 64 + ct * 32 + li;
+    for (int ct = 0; ct < CT; ++ct) {
+      const int col = n0 + wn * (32 * CT) + ct * 32 + li;
       if (col >= p.N) continue;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
@@ -285,8 +293,8 @@ void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
   p.beta = beta;
   const int64_t nblk = (int64_t)p.tilesM * p.tilesN * nb;
   require(nblk < (1LL << 31), "too many GEMM tiles");
-  hipLaunchKernelGGL(sbgemm_kernel, dim3((unsigned)nblk), dim3(NT), LDS_BYTES,
-                     s, p);
+  hipLaunchKernelGGL(sbgemm_kernel, dim3((unsigned)nblk), dim3(512),
+                     LDS_BYTES, s, p);
   FTCC_CHECK_HIP(hipGetLastError());
 }
 
