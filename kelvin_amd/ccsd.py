@@ -13,6 +13,7 @@ Extensions over the reference (all optional):
   refine      with a mixed precision, continue in FP64 after convergence
               until the FP64 iteration converges (default True).
 """
+import copy
 import logging
 import time
 
@@ -89,9 +90,118 @@ class ccsd(object):
                 "rt_iter='point' is not built for the GPU path")
         return self._ft_ccsd(T1in=T1, T2in=T2)
 
-    def compute_ESN(self, L1=None, L2=None, gderiv=True):
-        raise NotImplementedError(
-            "E/S/N (lambda equations) are not built for the GPU path yet")
+    def compute_ESN(self, L1=None, L2=None, gderiv=True, delta=1e-4,
+                    richardson=False, fd_econv=1e-12):
+        """Energy, entropy and particle number (kelvin/ccsd.py:128-214).
+
+        kelvin differentiates the grand potential of the discretised problem
+        analytically (lambda equations, response densities and the grid
+        derivative, kelvin/ccsd.py:167-214).  Here the same derivatives
+        N = -dOmega/dmu and S = -dOmega/dT are taken by central differences
+        of full GPU solves at (T, mu +- delta) and (T +- delta, mu), each
+        warm-started from the converged amplitudes and converged to
+        ``fd_econv``, with the grid rescaled with beta as in the reference;
+        the truncation error is O(delta^2) (O(delta^4) with
+        ``richardson=True``, which adds four solves at 2*delta).  The zero
+        order parts are the closed forms of kelvin/ccsd.py:175-180;
+        E = Omega + T S + mu N for every part.  L1/L2 are accepted for
+        signature compatibility and unused.  The system must derive its
+        (T, mu)-dependent quantities from its ``T`` and ``mu`` attributes
+        (true of the systems in this package).
+        """
+        if not gderiv:
+            raise NotImplementedError(
+                "gderiv=False (approximate grid derivative) is not built")
+        if not self.finite_T:
+            logging.warning(
+                "Computing thermodynamic quantities at zero temperature")
+            raise NotImplementedError(
+                "zero-temperature CCSD is not part of the GPU path")
+        if self.T1 is None:
+            self.run()
+        T, mu, beta = self.T, self.mu, self.beta
+        en = numpy.asarray(self.sys.g_energies_tot(), dtype=numpy.float64)
+
+        # zero order (closed forms)
+        fo = ft_utils.ff(beta, en, mu)
+        N0 = fo.sum()
+        E0 = beta * ft_utils.dGP0(beta, en, mu).sum() + mu * N0 + self.G0
+
+        def parts(Ts, mus):
+            """(E1, Omega_cc) at a neighbouring (T, mu)."""
+            sysx = self._system_at(Ts, mus)
+            cc = ccsd(sysx, T=Ts, mu=mus, iprint=self.iprint,
+                      singles=self.singles, econv=min(self.econv, fd_econv),
+                      tconv=self.tconv, max_iter=max(self.max_iter, 200),
+                      damp=self.damp, ngrid=self.ngrid, athresh=self.athresh,
+                      quad=self.quad, rt_iter=self.rt_iter,
+                      precision=self.precision, refine=True)
+            warm = self._warm_start(cc)
+            cc.run(T1=warm[0], T2=warm[1])
+            out = (cc.G1, cc.Gcc)
+            del cc
+            return out
+
+        def derivs(h):
+            e1p, ccp = parts(T, mu + h)
+            e1m, ccm = parts(T, mu - h)
+            e1f, ccf = parts(T + h, mu)
+            e1b, ccb = parts(T - h, mu)
+            return (numpy.array([-(e1p - e1m), -(ccp - ccm)]) / (2 * h),
+                    numpy.array([-(e1f - e1b), -(ccf - ccb)]) / (2 * h))
+
+        # the base point itself at the finite-difference convergence
+        e1c, gcc = parts(T, mu)
+        n_h, s_h = derivs(delta)
+        if richardson:
+            n_2h, s_2h = derivs(2 * delta)
+            n_h = (4.0 * n_h - n_2h) / 3.0
+            s_h = (4.0 * s_h - s_2h) / 3.0
+        N1, Ncc = n_h
+        S1, Scc = s_h
+        E1 = e1c + T * S1 + mu * N1
+        Ecc = gcc + T * Scc + mu * Ncc
+
+        self.N0, self.N1, self.Ncc = N0, N1, Ncc
+        self.N = N0 + N1 + Ncc
+        self.E0, self.E1, self.Ecc = E0, E1, Ecc
+        self.E = E0 + E1 + Ecc
+        Gtot = self.G0 + e1c + gcc
+        self.S = -beta * (Gtot - self.E + mu * self.N)
+        self.S0 = -beta * (self.G0 - self.E0 + mu * self.N0)
+        self.S1 = -beta * (e1c - self.E1 + mu * self.N1)
+        self.Scc = self.S - self.S0 - self.S1
+        return self.E, self.S, self.N
+
+    def _system_at(self, T, mu):
+        """The system at another (T, mu): a shallow copy whose temperature
+        and chemical potential are replaced."""
+        s = copy.copy(self.sys)
+        s.T = T
+        s.mu = mu
+        if hasattr(s, "beta"):
+            s.beta = 1.0 / T
+        if not s.verify(T, mu):
+            raise Exception("system cannot be moved to T=%r, mu=%r" % (T, mu))
+        return s
+
+    def _warm_start(self, other):
+        """Converged amplitudes as the starting point of a neighbouring
+        solve, when the active space has the same size."""
+        if self.T1 is None:
+            return None, None
+        en = numpy.asarray(other.sys.g_energies_tot(), dtype=numpy.float64)
+        n = en.shape[0]
+        if other.athresh > 0.0:
+            fo = ft_utils.ff(other.beta, en, other.mu)
+            fv = ft_utils.ffv(other.beta, en, other.mu)
+            shape = (int((fv > other.athresh).sum()),
+                     int((fo > other.athresh).sum()))
+        else:
+            shape = (n, n)
+        if tuple(self.T1.shape[1:]) != shape:
+            return None, None
+        return self.T1, self.T2
 
     def _ft_ccsd(self, T1in=None, T2in=None):
         ng = self.ngrid

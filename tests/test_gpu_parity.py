@@ -380,3 +380,27 @@ def test_mixed_precision_refined_matches_fp64(prec):
     assert abs(Ecc - Ecc_ref) < 1e-9
     assert abs(Etot - Eref) < 1e-9
     assert abs(Ecc_ref - GOLD["G3"]["Omega_cc"]) < 1e-8
+
+
+# ------------------------------------------------------------ E, S, N
+@pytest.mark.parametrize("key,precision", [("G4", "fp64"), ("G3", "fp64"),
+                                           ("G3", "bf16x3"), ("G5", "fp64")])
+def test_golden_ESN(key, precision):
+    """compute_ESN against the reference's lambda-equation E, S, N
+    (kelvin/ccsd.py:128-214 values printed in the goldens' sources).  The
+    finite-difference truncation error at delta = 1e-4 is ~5e-8 (measured
+    with the oracle on G4); mixed precision refines every solve in FP64."""
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    g = GOLD[key]
+    sysg, s = _ueg_gold(key)
+    kw = dict(g["ccsd"])
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], precision=precision, **kw)
+    cc.run()
+    E, S, N = cc.compute_ESN()
+    tol = 1e-6 if key != "G5" else 2e-6
+    assert abs(E - g["E"]) < tol
+    assert abs(S - g["S"]) < tol
+    assert abs(N - g["N"]) < tol
+    assert abs(cc.N0 + cc.N1 + cc.Ncc - N) < 1e-12
+    assert abs(cc.E0 + cc.E1 + cc.Ecc - E) < 1e-12
