@@ -180,8 +180,10 @@ def main():
     eng.driving_terms(T1, T2)
     eng.integrate(T1, T2)
     skip = zero_slices(eng, T1, T2, G)
-    log("device set-up done in %.1f s (context %.1f GB)"
-        % (time.time() - t0, eng.workspace_bytes() / 1e9))
+    spin_blocked = eng.spin_blocked()
+    log("device set-up done in %.1f s (context %.1f GB, spin-blocked "
+        "products: %s)" % (time.time() - t0, eng.workspace_bytes() / 1e9,
+                           spin_blocked))
     if args.copy_calibration_mib > 0:
         nb = args.copy_calibration_mib << 20
         src = torch.ones(nb, dtype=torch.uint8, device=dev)
@@ -270,10 +272,11 @@ def main():
                                % (args.nk, n, args.rs, args.t_over_tf,
                                   args.nelec, ng),
                    "n": n, "ng": ng, "precision": args.precision,
-                   "damp": args.damp},
+                   "damp": args.damp, "spin_blocked": spin_blocked},
         "fp64": fp64,
         "omega_step_delta_vs_fp64": d_omega,
         "textbook_tflops": algo * value / 1e12,
+        "product_flops_per_iter": gemm_flops / args.steps,
         "omega0_plus_e1": omega0 + e1,
         "energies": energies,
         "roofline": roofline,

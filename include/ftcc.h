@@ -74,6 +74,22 @@ int ftcc_set_system(ftcc_ctx *ctx, int n, const double *en_dev,
 int ftcc_set_blocks(ftcc_ctx *ctx, const double *const F_dev[4],
                     const double *const I_dev[9]);
 
+/* Spin blocking of the six O(n^6) products (no counterpart in the
+ * reference, whose einsums are dense; the results are the same).  Labels
+ * (0 or 1, host arrays) of the occupied-role and virtual-role orbitals: if
+ * every Fock and integral block is exactly zero wherever spin is not
+ * conserved, each product runs as its three nonzero spin blocks (10/64 of
+ * the dense work); otherwise FTCC_EINVAL and the products stay dense.  Both
+ * NULL switches blocking off.  ftcc_set_system tries the labels "first
+ * half alpha, second half beta" of the n orbitals by itself (environment
+ * FTCC_SPIN=0 prevents it); ftcc_set_blocks switches blocking off.
+ * Amplitudes are assumed to conserve spin like the integrals (every
+ * iterate started from the driving terms does). */
+int ftcc_set_spin(ftcc_ctx *ctx, const int *lab_o_host,
+                  const int *lab_v_host);
+/* *on = 1 when the products run spin-blocked. */
+int ftcc_get_spin(ftcc_ctx *ctx, int *on);
+
 /* Explicit denominators D1_dev[nv,no], D2_dev[nv,nv,no,no] (the arguments
  * D1, D2 of kelvin/ft_cc_equations.py:96), copied. */
 int ftcc_set_denominators(ftcc_ctx *ctx, const double *D1_dev,

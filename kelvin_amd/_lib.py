@@ -45,6 +45,9 @@ SIGNATURES = {
     "ftcc_step": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _c_double_p,
                                  ctypes.c_double, ctypes.c_double,
                                  ctypes.c_uint64, _c_double_p]),
+    "ftcc_set_spin": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int)]),
+    "ftcc_get_spin": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "ftcc_workspace_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_i64)]),
     "ftcc_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ftcc_profile_read": (ctypes.c_int, [_vp, ctypes.POINTER(_i64),

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 
 using namespace ftcc;
 
@@ -31,6 +32,7 @@ struct ftcc_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<void *> allocs;
+  std::map<void *, int64_t> alloc_doubles;   // sizes of dalloc's buffers
   int64_t bytes = 0;
 
   // integrals in the reference layouts
@@ -72,6 +74,18 @@ struct ftcc_ctx {
   SplitMat sp_tauA, sp_tauB, sp_t2rA, sp_aringB, sp_wmnijB, sp_wabefA, sp_wrB;
   SplitMat sp_oovvA, sp_oovvB, sp_ringA;
   bool split_fixed_valid = false;
+  int split_layout = -1;        // spin state the split buffers are sized for
+
+  // Spin blocking of the O(n^6) products: when every integral block
+  // conserves spin under the orbital labels, the pair indices of the ten
+  // product operands are reordered by spin class and each product runs as
+  // three dense block products (10/64 of the dense work).
+  int spin = 0;
+  int *d_lab_o = nullptr, *d_lab_v = nullptr;
+  PairSpace ps_vv, ps_oo, ps_vo;   // (a,b), (i,j) and ring (a,i) pairs
+  double *b64[10] = {};            // FP64 blocked operands (fp64/fp32 modes)
+  int b64_layout = -1;
+  bool b64_fixed_valid = false;
 
   bool prof_on = false;
   std::vector<Pending> prof_pending;
@@ -92,8 +106,22 @@ double *dalloc(ftcc_ctx *c, int64_t n) {
     throw Error(FTCC_ENOMEM, "hipMalloc of " + std::to_string(8 * n) +
                                  " bytes failed: " + hipGetErrorString(e));
   c->allocs.push_back(p);
+  c->alloc_doubles[p] = n;
   c->bytes += 8 * n;
   return (double *)p;
+}
+
+// release one allocation of dalloc
+void dfree(ftcc_ctx *c, void *p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  auto sz = c->alloc_doubles.find(p);
+  if (sz != c->alloc_doubles.end()) {
+    c->bytes -= 8 * sz->second;
+    c->alloc_doubles.erase(sz);
+  }
+  FTCC_CHECK_HIP(hipFree(p));
 }
 
 template <typename Fn>
@@ -157,6 +185,7 @@ void build_derived(ftcc_ctx *c) {
   transpose2(c->F[1], c->fovT, O, V, 1, s);
   c->have_ints = true;
   c->split_fixed_valid = false;
+  c->b64_fixed_valid = false;
 }
 
 // GEMM helper: dense row-major operands with per-slot batch strides.
@@ -269,41 +298,308 @@ void big(ftcc_ctx *c, const Gemm &g, int prec, const SplitMat &A,
   });
 }
 
-// Split-bf16 operand buffers (allocated on first use of bf16x3).
-SplitMat alloc_split(ftcc_ctx *c, int rows, int K, int nb) {
-  SplitMat m;
-  m.ld = split_ld(K);
-  m.bstride = (int64_t)rows * m.ld;
-  const int64_t elems = m.bstride * nb;
-  // dalloc counts doubles: bf16 pairs need elems/2 doubles per array
-  m.hi = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
-  m.lo = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
-  return m;
+// ---------------------------------------------------------------------------
+// Operands of the six O(n^6) products.  Each is a matrix over two pair
+// spaces (rows, K); A operands are stored [M][K], B operands transposed as
+// [N][K].  Pair spaces: 0 = (a,b) virtual pairs, 1 = (i,j) occupied pairs,
+// 2 = ring (a,i) pairs.  Ids 7..9 depend on the integrals only.
+enum Op {
+  OP_TAU_A, OP_TAU_B, OP_T2R_A, OP_ARING_B, OP_WMNIJ_B, OP_WABEF_A, OP_WR_B,
+  OP_OOVV_A, OP_OOVV_B, OP_RING_A, NOPS
+};
+constexpr int OP_ROWS[NOPS] = {0, 1, 2, 2, 1, 0, 2, 1, 0, 2};
+constexpr int OP_K[NOPS] = {1, 0, 2, 2, 1, 0, 2, 0, 1, 2};
+constexpr bool OP_FIXED[NOPS] = {false, false, false, false, false,
+                                 false, false, true,  true,  true};
+
+const PairSpace &space(const ftcc_ctx *c, int sp) {
+  return sp == 0 ? c->ps_vv : sp == 1 ? c->ps_oo : c->ps_vo;
 }
 
-void ensure_split(ftcc_ctx *c) {
-  const int O = c->no, V = c->nv, nb = c->nb;
-  const int OO = O * O, VV = V * V, VO = V * O;
-  if (!c->sp_tauA.hi) {
-    c->sp_tauA = alloc_split(c, VV, OO, nb);
-    c->sp_tauB = alloc_split(c, OO, VV, nb);
-    c->sp_t2rA = alloc_split(c, VO, VO, nb);
-    c->sp_aringB = alloc_split(c, VO, VO, nb);
-    c->sp_wmnijB = alloc_split(c, OO, OO, nb);
-    c->sp_wabefA = alloc_split(c, VV, VV, nb);
-    c->sp_wrB = alloc_split(c, VO, VO, nb);
-    c->sp_oovvA = alloc_split(c, OO, VV, 1);
-    c->sp_oovvB = alloc_split(c, VV, OO, 1);
-    c->sp_ringA = alloc_split(c, VO, VO, 1);
-    c->sp_oovvA.bstride = c->sp_oovvB.bstride = c->sp_ringA.bstride = 0;
+// length of a pair space in the current layout
+int space_len(const ftcc_ctx *c, int sp) {
+  if (c->spin) return space(c, sp).total;
+  const int O = c->no, V = c->nv;
+  return sp == 0 ? V * V : sp == 1 ? O * O : V * O;
+}
+
+SplitMat &split_of(ftcc_ctx *c, int id) {
+  SplitMat *t[NOPS] = {&c->sp_tauA,   &c->sp_tauB,   &c->sp_t2rA,
+                       &c->sp_aringB, &c->sp_wmnijB, &c->sp_wabefA,
+                       &c->sp_wrB,    &c->sp_oovvA,  &c->sp_oovvB,
+                       &c->sp_ringA};
+  return *t[id];
+}
+
+// elements of one array of a split operand (hi or lo), all slots
+int64_t split_elems(const ftcc_ctx *c, int id) {
+  const int64_t rows = space_len(c, OP_ROWS[id]);
+  return rows * split_ld(space_len(c, OP_K[id])) * (OP_FIXED[id] ? 1 : c->nb);
+}
+
+// Split-bf16 operand buffers for the current layout (allocated on first use
+// of bf16x3, reallocated when the spin layout changes).
+void ensure_split_buffers(ftcc_ctx *c) {
+  if (c->split_layout == c->spin) return;
+  for (int id = 0; id < NOPS; ++id) {
+    SplitMat &m = split_of(c, id);
+    dfree(c, m.hi);
+    dfree(c, m.lo);
+    m = SplitMat();
   }
-  if (!c->split_fixed_valid) {
-    hipStream_t s = c->stream;
-    split_convert(c->I[5], VV, 1, 0, OO, VV, c->sp_oovvA, 1, s);
-    split_convert(c->I[5], 1, VV, 0, VV, OO, c->sp_oovvB, 1, s);
-    split_convert(c->oovv_ring, VO, 1, 0, VO, VO, c->sp_ringA, 1, s);
+  for (int id = 0; id < NOPS; ++id) {
+    SplitMat &m = split_of(c, id);
+    m.ld = split_ld(space_len(c, OP_K[id]));
+    const int64_t rows = space_len(c, OP_ROWS[id]);
+    m.bstride = OP_FIXED[id] ? 0 : rows * m.ld;
+    const int64_t elems = split_elems(c, id);
+    // dalloc counts doubles: four bf16 per double
+    m.hi = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
+    m.lo = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
+  }
+  c->split_layout = c->spin;
+  c->split_fixed_valid = false;
+}
+
+// FP64 spin-blocked operand buffers (FP64 and FP32 modes)
+void ensure_b64_buffers(ftcc_ctx *c) {
+  if (c->b64_layout == c->spin && c->b64[0]) return;
+  for (int id = 0; id < NOPS; ++id) {
+    dfree(c, c->b64[id]);
+    c->b64[id] = nullptr;
+  }
+  for (int id = 0; id < NOPS; ++id)
+    c->b64[id] = dalloc(c, (int64_t)space_len(c, OP_ROWS[id]) *
+                               space_len(c, OP_K[id]) *
+                               (OP_FIXED[id] ? 1 : c->nb));
+  c->b64_layout = c->spin;
+  c->b64_fixed_valid = false;
+}
+
+// Write operand `id` of nb slots from its FP64 source: src[b*sb + row*sr +
+// k*sc] with row, k flat pair indices of the dense layout.
+void put_operand(ftcc_ctx *c, int id, const double *src, int64_t sr,
+                 int64_t sc, int64_t sb, int nb) {
+  hipStream_t s = c->stream;
+  if (c->prec == FTCC_PREC_BF16X3) {
+    SplitMat &m = split_of(c, id);
+    if (c->spin) {
+      const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, OP_K[id]);
+      split_gather(src, sr, sc, sb, R.d_map, R.total, K.d_map, m, nb, s);
+    } else {
+      split_convert(src, sr, sc, sb, space_len(c, OP_ROWS[id]),
+                    space_len(c, OP_K[id]), m, nb, s);
+    }
+  } else if (c->spin) {
+    const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, OP_K[id]);
+    gather64(src, sr, sc, sb, R.d_map, R.total, K.d_map, K.total, c->b64[id],
+             (int64_t)R.total * K.total, nb, s);
+  }
+}
+
+// operands that depend on the integrals only
+void ensure_fixed_operands(ftcc_ctx *c) {
+  const int O = c->no, V = c->nv;
+  const int64_t VV = (int64_t)V * V, VO = (int64_t)V * O;
+  if (c->prec == FTCC_PREC_BF16X3) {
+    ensure_split_buffers(c);
+    if (c->split_fixed_valid) return;
     c->split_fixed_valid = true;
+  } else if (c->spin) {
+    ensure_b64_buffers(c);
+    if (c->b64_fixed_valid) return;
+    c->b64_fixed_valid = true;
+  } else {
+    return;
   }
+  put_operand(c, OP_OOVV_A, c->I[5], VV, 1, 0, 1);   // [mn][ef]
+  put_operand(c, OP_OOVV_B, c->I[5], 1, VV, 0, 1);   // [ef][mn]
+  put_operand(c, OP_RING_A, c->oovv_ring, VO, 1, 0, 1);
+}
+
+// Pairs of (lx, ly) grouped by class 2*lx + ly in `order`, each class padded
+// to a multiple of 256 (the product tile).
+PairSpace build_pairs(ftcc_ctx *c, const std::vector<int> &lx,
+                      const std::vector<int> &ly, const int order[4]) {
+  PairSpace ps;
+  std::vector<int> map;
+  const int ny = (int)ly.size();
+  for (int q = 0; q < 4; ++q) {
+    const int cls = order[q], sx = cls >> 1, sy = cls & 1;
+    ps.start[cls] = (int)map.size();
+    int cnt = 0;
+    for (int x = 0; x < (int)lx.size(); ++x) {
+      if (lx[x] != sx) continue;
+      for (int y = 0; y < ny; ++y)
+        if (ly[y] == sy) {
+          map.push_back(x * ny + y);
+          ++cnt;
+        }
+    }
+    while (map.size() % 256) map.push_back(-1);
+    ps.count[cls] = cnt;
+    ps.len[cls] = (int)map.size() - ps.start[cls];
+  }
+  ps.total = (int)map.size();
+  ps.d_map = (int *)dalloc(c, ((int64_t)map.size() + 1) / 2 + 1);
+  if (!map.empty())
+    FTCC_CHECK_HIP(hipMemcpyAsync(ps.d_map, map.data(),
+                                  sizeof(int) * map.size(),
+                                  hipMemcpyHostToDevice, c->stream));
+  FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return ps;
+}
+
+// Class orders: pairs of equal total spin are adjacent in the ladder spaces
+// ((a,b), (i,j): 00 | 01 10 | 11); pairs of equal spin flip in the ring
+// space ((a,i): 00 11 | 01 | 10).
+constexpr int LADDER_ORDER[4] = {0, 1, 2, 3};
+constexpr int RING_ORDER[4] = {0, 3, 1, 2};
+
+struct Range {
+  int start = 0, len = 0, count = 0;
+};
+struct Blk {
+  Range r, k, c;
+};
+
+// classes c1 (and c2, adjacent after c1) of a pair space
+Range group(const PairSpace &p, int c1, int c2 = -1) {
+  Range g;
+  g.start = p.start[c1];
+  g.len = p.len[c1];
+  g.count = p.count[c1];
+  if (c2 >= 0) {
+    g.len += p.len[c2];
+    g.count += p.count[c2];
+  }
+  return g;
+}
+
+// The three nonzero blocks of a product C[R,Cs] = A[R,K] B[K,Cs].
+//   ladder products (G1-G4): every operand conserves total spin of its pairs;
+//   G5: Wr[em,bj] -= oovv_ring[em,fn] aring[fn,bj]: ring pairs of A and B
+//       flip spin with opposite signs, so C keeps it;
+//   G6: Z[ai,bj] = t2r[ai,em] Wr[em,bj]: A flips sign, B keeps it.
+enum BlockKind { LADDER, RING_G5, RING_G6 };
+void block_list(int kind, const PairSpace &R, const PairSpace &K,
+                const PairSpace &C, Blk out[3]) {
+  if (kind == LADDER) {
+    out[0] = {group(R, 0), group(K, 0), group(C, 0)};
+    out[1] = {group(R, 1, 2), group(K, 1, 2), group(C, 1, 2)};
+    out[2] = {group(R, 3), group(K, 3), group(C, 3)};
+    return;
+  }
+  // ring classes: 0 and 3 keep spin, 1 lowers it (a alpha, i beta), 2 raises
+  out[0] = {group(R, 0, 3), group(K, 0, 3), group(C, 0, 3)};
+  if (kind == RING_G5) {
+    out[1] = {group(R, 1), group(K, 2), group(C, 1)};
+    out[2] = {group(R, 2), group(K, 1), group(C, 2)};
+  } else {
+    out[1] = {group(R, 1), group(K, 2), group(C, 2)};
+    out[2] = {group(R, 2), group(K, 1), group(C, 1)};
+  }
+}
+
+// One O(n^6) product C = alpha A.B + beta C over g.nb1 slots, g describing
+// it in the dense layout and ia/ib naming its operands: dense, or as the
+// three nonzero spin blocks written straight into C through the pair maps.
+void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
+  const int P = c->prec;
+  if (!c->spin) {
+    big(c, g, P, split_of(c, ia), split_of(c, ib));
+    return;
+  }
+  const PairSpace &R = space(c, OP_ROWS[ia]), &K = space(c, OP_K[ia]);
+  const PairSpace &Cs = space(c, OP_ROWS[ib]);
+  Blk blk[3];
+  block_list(kind, R, K, Cs, blk);
+  for (const Blk &b : blk) {
+    // an empty K range leaves C as it is: the callers either accumulate
+    // (beta = 1) or have cleared C beforehand
+    if (!b.r.count || !b.k.count || !b.c.count) continue;
+    const double flops = 2.0 * b.r.count * (double)b.c.count * b.k.count * g.nb1;
+    const int *rmap = R.d_map + b.r.start, *cmap = Cs.d_map + b.c.start;
+    if (P == FTCC_PREC_BF16X3) {
+      SplitMat A = split_of(c, ia), B = split_of(c, ib);
+      const int64_t ao = (int64_t)b.r.start * A.ld + b.k.start;
+      const int64_t bo = (int64_t)b.c.start * B.ld + b.k.start;
+      A.hi += ao;
+      A.lo += ao;
+      B.hi += bo;
+      B.lo += bo;
+      timed(c, true, flops, [&] {
+        sbgemm(b.r.len, b.c.len, b.k.len, g.nb1, g.alpha, A, B, g.beta, g.C,
+               g.sCm, g.sCn, g.sC1, c->stream, rmap, cmap);
+      });
+    } else {
+      const int64_t ld = K.total;
+      Gemm gb = mk(b.r.len, b.c.len, b.k.len, g.alpha, g.beta,
+                   c->b64[ia] + (int64_t)b.r.start * ld + b.k.start, ld, 1,
+                   OP_FIXED[ia] ? 0 : (int64_t)R.total * ld,
+                   c->b64[ib] + (int64_t)b.c.start * ld + b.k.start, 1, ld,
+                   OP_FIXED[ib] ? 0 : (int64_t)Cs.total * ld, g.C, g.sCm,
+                   g.sCn, g.sC1, g.nb1);
+      gb.rmap = rmap;
+      gb.cmap = cmap;
+      ensure_partial(c, gb);
+      timed(c, true, flops,
+            [&] { gemm_run(gb, P, c->scr, c->stream); });
+    }
+  }
+}
+
+void read_scalars(ftcc_ctx *c, int n);
+
+// Spin labels (0/1) of the occupied and virtual index lists: blocking is
+// switched on when every Fock and integral block conserves spin under them.
+bool try_spin(ftcc_ctx *c, const std::vector<int> &lo,
+              const std::vector<int> &lv) {
+  const int O = c->no, V = c->nv;
+  hipStream_t s = c->stream;
+  c->spin = 0;
+  if (!c->d_lab_o) {
+    c->d_lab_o = (int *)dalloc(c, (O + 1) / 2 + 1);
+    c->d_lab_v = (int *)dalloc(c, (V + 1) / 2 + 1);
+  }
+  FTCC_CHECK_HIP(hipMemcpyAsync(c->d_lab_o, lo.data(), sizeof(int) * O,
+                                hipMemcpyHostToDevice, s));
+  FTCC_CHECK_HIP(hipMemcpyAsync(c->d_lab_v, lv.data(), sizeof(int) * V,
+                                hipMemcpyHostToDevice, s));
+  auto lab = [&](char x) { return x == 'o' ? c->d_lab_o : c->d_lab_v; };
+  auto dim = [&](char x) { return x == 'o' ? O : V; };
+  double bad = 0.0;
+  const char *fn[4] = {"oo", "ov", "vo", "vv"};
+  for (int k = 0; k < 4; ++k) {
+    // f[p][q] as a (P,1,Q,1) block: label(p) must equal label(q)
+    const int nbk = spin_violations(c->F[k], dim(fn[k][0]), 1, dim(fn[k][1]),
+                                    1, 1, lab(fn[k][0]), nullptr,
+                                    lab(fn[k][1]), nullptr, c->partials, s);
+    reduce_partials(c->partials, nbk, c->red, s);
+    read_scalars(c, 3);
+    bad += c->host_red[2];
+  }
+  const char *in[9] = {"vvvv", "vvvo", "vovv", "vvoo", "vovo",
+                       "oovv", "vooo", "ooov", "oooo"};
+  for (int k = 0; k < 9 && bad == 0.0; ++k) {
+    const char *b = in[k];
+    const int nbk = spin_violations(c->I[k], dim(b[0]), dim(b[1]), dim(b[2]),
+                                    dim(b[3]), 1, lab(b[0]), lab(b[1]),
+                                    lab(b[2]), lab(b[3]), c->partials, s);
+    reduce_partials(c->partials, nbk, c->red, s);
+    read_scalars(c, 3);
+    bad += c->host_red[2];
+  }
+  if (bad != 0.0) return false;
+  for (PairSpace *ps : {&c->ps_vv, &c->ps_oo, &c->ps_vo}) {
+    dfree(c, ps->d_map);
+    *ps = PairSpace();
+  }
+  c->ps_vv = build_pairs(c, lv, lv, LADDER_ORDER);
+  c->ps_oo = build_pairs(c, lo, lo, LADDER_ORDER);
+  c->ps_vo = build_pairs(c, lv, lo, RING_ORDER);
+  c->spin = 1;
+  return true;
 }
 
 void ensure_integrals(ftcc_ctx *c) {
@@ -395,14 +691,13 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   PrepOut po{c->tau, c->taut, c->t2r, c->aring, c->t2p, c->tautp};
   prep_amplitudes(t1, t2, V, O, nb, po, s);
   transpose2(t1, c->t1T, V, O, nb, s);   // t1T[m,f] = t1[f,m]
-  if (P == FTCC_PREC_BF16X3) {
-    // split operands: A as [M][K], B as [N][K] (K contiguous)
-    ensure_split(c);
-    split_convert(c->tau, OO, 1, S2, (int)VV, (int)OO, c->sp_tauA, nb, s);
-    split_convert(c->tau, 1, OO, S2, (int)OO, (int)VV, c->sp_tauB, nb, s);
-    split_convert(c->t2r, VO, 1, S2, (int)VO, (int)VO, c->sp_t2rA, nb, s);
-    split_convert(c->aring, 1, VO, S2, (int)VO, (int)VO, c->sp_aringB, nb, s);
-  }
+  // product operands (split into bf16 pairs, and/or reordered by spin
+  // class): A as [M][K], B as [N][K] (K contiguous)
+  ensure_fixed_operands(c);
+  put_operand(c, OP_TAU_A, c->tau, OO, 1, S2, nb);
+  put_operand(c, OP_TAU_B, c->tau, 1, OO, S2, nb);
+  put_operand(c, OP_T2R_A, c->t2r, VO, 1, S2, nb);
+  put_operand(c, OP_ARING_B, c->aring, 1, VO, S2, nb);
 
   // Fme[m,e] = Fov[m,e] + sum_{fn} <mn||ef> t_n^f
   gemv_rows(c->oovv_ring, VO, 0, t1, VO, c->FmeT, VO, (int)VO, (int)VO, nb,
@@ -454,8 +749,9 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sC2 = OO;
     run(c, g, P5);
   }
-  big(c, mk((int)OO, (int)OO, (int)VV, 0.25, 1.0, oovv, VV, 1, 0, c->tau, OO,
-            1, S2, c->wmnij, OO, 1, O4, nb), P, c->sp_oovvA, c->sp_tauB);
+  product(c, mk((int)OO, (int)OO, (int)VV, 0.25, 1.0, oovv, VV, 1, 0, c->tau,
+                OO, 1, S2, c->wmnij, OO, 1, O4, nb), OP_OOVV_A, OP_TAU_B,
+          LADDER);
 
   // Wabef = vvvv - P(ab) t_m^b <am||ef> + 1/4 tau.oovv          (G2)
   bcast(c->wabef, vvvv, V4, nb, 1.0, 0.0, s);
@@ -474,8 +770,9 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sC2 = VV;
     run(c, g, P5);
   }
-  big(c, mk((int)VV, (int)VV, (int)OO, 0.25, 1.0, c->tau, OO, 1, S2, oovv, VV,
-            1, 0, c->wabef, VV, 1, V4, nb), P, c->sp_tauA, c->sp_oovvB);
+  product(c, mk((int)VV, (int)VV, (int)OO, 0.25, 1.0, c->tau, OO, 1, S2, oovv,
+                VV, 1, 0, c->wabef, VV, 1, V4, nb), OP_TAU_A, OP_OOVV_B,
+          LADDER);
 
   // Wr[e,m,b,j] = <mb||ej> + t_j^f <mb||ef> - t_n^b <mn||ej> - A.<mn||ef>
   bcast(c->wr, c->ovvo_ring, S2, nb, 1.0, 0.0, s);
@@ -492,15 +789,16 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sC2 = VO;
     run(c, g, P5);
   }
-  big(c, mk((int)VO, (int)VO, (int)VO, -1.0, 1.0, c->oovv_ring, VO, 1, 0,
-            c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), P, c->sp_ringA,
-      c->sp_aringB);                                                     // G5
+  product(c, mk((int)VO, (int)VO, (int)VO, -1.0, 1.0, c->oovv_ring, VO, 1, 0,
+                c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), OP_RING_A,
+          OP_ARING_B, RING_G5);                                          // G5
 
   // Z[ai,bj] = t2r.Wr - t_i^e t_m^a <mb||ej>                    (G6)
-  if (P == FTCC_PREC_BF16X3)
-    split_convert(c->wr, 1, VO, S2, (int)VO, (int)VO, c->sp_wrB, nb, s);
-  big(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr, VO,
-            1, S2, c->z, VO, 1, S2, nb), P, c->sp_t2rA, c->sp_wrB);
+  put_operand(c, OP_WR_B, c->wr, 1, VO, S2, nb);
+  // spin blocks leave the spin-forbidden part of Z unwritten: clear it
+  if (c->spin) fill(c->z, S2 * nb, 0.0, s);
+  product(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr,
+                VO, 1, S2, c->z, VO, 1, S2, nb), OP_T2R_A, OP_WR_B, RING_G6);
   run(c, mk(O, (int)(O * VO), V, 1.0, 0.0, t1, 1, O, VO, c->ovvo_ring, O * VO,
             1, 0, c->u, O * VO, 1, OO * VO, nb), P5);
   {
@@ -515,16 +813,14 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
 
   // doubles accumulator in t2bar: vvoo + 1/2 tau.Wmnij + 1/2 Wabef.tau
   bcast(t2bar, vvoo, S2, nb, 1.0, 0.0, s);
-  if (P == FTCC_PREC_BF16X3) {
-    split_convert(c->wmnij, 1, OO, O4, (int)OO, (int)OO, c->sp_wmnijB, nb, s);
-    split_convert(c->wabef, VV, 1, V4, (int)VV, (int)VV, c->sp_wabefA, nb, s);
-  }
-  big(c, mk((int)VV, (int)OO, (int)OO, 0.5, 1.0, c->tau, OO, 1, S2, c->wmnij,
-            OO, 1, O4, t2bar, OO, 1, S2, nb), P, c->sp_tauA,
-      c->sp_wmnijB);                                                     // G3
-  big(c, mk((int)VV, (int)OO, (int)VV, 0.5, 1.0, c->wabef, VV, 1, V4, c->tau,
-            OO, 1, S2, t2bar, OO, 1, S2, nb), P, c->sp_wabefA,
-      c->sp_tauB);                                                       // G4
+  put_operand(c, OP_WMNIJ_B, c->wmnij, 1, OO, O4, nb);
+  put_operand(c, OP_WABEF_A, c->wabef, VV, 1, V4, nb);
+  product(c, mk((int)VV, (int)OO, (int)OO, 0.5, 1.0, c->tau, OO, 1, S2,
+                c->wmnij, OO, 1, O4, t2bar, OO, 1, S2, nb), OP_TAU_A,
+          OP_WMNIJ_B, LADDER);                                           // G3
+  product(c, mk((int)VV, (int)OO, (int)VV, 0.5, 1.0, c->wabef, VV, 1, V4,
+                c->tau, OO, 1, S2, t2bar, OO, 1, S2, nb), OP_WABEF_A,
+          OP_TAU_B, LADDER);                                             // G4
 
   // S1 (antisymmetrised in ab): Fbe[b,e] t2[a,e,i,j] + t1[a,m] vooo[b,m,i,j]
   {
@@ -599,9 +895,38 @@ void upload_g(ftcc_ctx *c, const double *g) {
                                 hipMemcpyHostToDevice, c->stream));
 }
 
+// true when all ng slices of T1 [ng][V][O] and T2 [ng][V][V][O][O] conserve
+// spin under the labels (the blocked products rely on it)
+bool amplitudes_conserve_spin(ftcc_ctx *c, const double *T1,
+                              const double *T2) {
+  const int O = c->no, V = c->nv;
+  hipStream_t s = c->stream;
+  double bad = 0.0;
+  int nbk = spin_violations(T1, V, 1, O, 1, c->ng, c->d_lab_v, nullptr,
+                            c->d_lab_o, nullptr, c->partials, s);
+  reduce_partials(c->partials, nbk, c->red, s);
+  read_scalars(c, 3);
+  bad += c->host_red[2];
+  nbk = spin_violations(T2, V, V, O, O, c->ng, c->d_lab_v, c->d_lab_v,
+                        c->d_lab_o, c->d_lab_o, c->partials, s);
+  reduce_partials(c->partials, nbk, c->red, s);
+  read_scalars(c, 3);
+  bad += c->host_red[2];
+  return bad == 0.0;
+}
+
 void residual_all(ftcc_ctx *c, const double *T1, const double *T2,
                   double *T1b, double *T2b, uint64_t skip) {
   ensure_workspace(c);
+  // spin blocks are exact only for spin-conserving amplitudes (every CC
+  // iterate is); anything else runs dense
+  const int spin_saved = c->spin;
+  if (c->spin && !amplitudes_conserve_spin(c, T1, T2)) c->spin = 0;
+  struct Restore {
+    ftcc_ctx *c;
+    int v;
+    ~Restore() { c->spin = v; }
+  } restore{c, spin_saved};
   const int O = c->no, V = c->nv;
   const int64_t VO = (int64_t)V * O, S2 = (int64_t)V * V * O * O;
   int y = 0;
@@ -778,6 +1103,16 @@ int ftcc_set_system(ftcc_ctx *c, int n, const double *en, const double *fock,
     c->have_D = false;
     c->have_energies = true;
     build_derived(c);
+    // spin-orbital systems list the alpha orbitals first, then the beta
+    // ones: try those labels (FTCC_SPIN=0 keeps every product dense)
+    c->spin = 0;
+    const char *env = getenv("FTCC_SPIN");
+    if (n % 2 == 0 && !(env && atoi(env) == 0)) {
+      std::vector<int> lo(O), lv(V);
+      for (int k = 0; k < O; ++k) lo[k] = io[k] >= n / 2;
+      for (int k = 0; k < V; ++k) lv[k] = iv[k] >= n / 2;
+      try_spin(c, lo, lv);
+    }
     if (omega0) {
       const int nbk = vsum(gp0, n, c->partials, s);
       reduce_partials(c->partials, nbk, c->red, s);
@@ -812,7 +1147,32 @@ int ftcc_set_blocks(ftcc_ctx *c, const double *const F[4],
                                     hipMemcpyDeviceToDevice, s));
     }
     build_derived(c);
+    c->spin = 0;   // labels unknown: ftcc_set_spin may switch blocking on
     FTCC_CHECK_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int ftcc_set_spin(ftcc_ctx *c, const int *lab_o, const int *lab_v) {
+  return guarded(c, [&] {
+    require(c->have_ints, "integrals not set (ftcc_set_system/ftcc_set_blocks)",
+            FTCC_ESTATE);
+    if (!lab_o && !lab_v) {
+      c->spin = 0;
+      return;
+    }
+    require(lab_o && lab_v, "give both label arrays or neither");
+    std::vector<int> lo(lab_o, lab_o + c->no), lv(lab_v, lab_v + c->nv);
+    for (int x : lo) require(x == 0 || x == 1, "spin labels must be 0 or 1");
+    for (int x : lv) require(x == 0 || x == 1, "spin labels must be 0 or 1");
+    require(try_spin(c, lo, lv),
+            "the integral blocks do not conserve spin under these labels");
+  });
+}
+
+int ftcc_get_spin(ftcc_ctx *c, int *on) {
+  return guarded(c, [&] {
+    require(on != nullptr, "null output pointer");
+    *on = c->spin;
   });
 }
 

@@ -48,6 +48,9 @@ struct Gemm {
   int64_t sBk = 0, sBn = 0, sB1 = 0, sB2 = 0;
   double *C = nullptr;
   int64_t sCm = 0, sCn = 0, sC1 = 0, sC2 = 0;
+  // optional scatter of the output: row m of the product is written to row
+  // rmap[m] of C (column n to cmap[n]); -1 drops it
+  const int *rmap = nullptr, *cmap = nullptr;
 };
 
 // Scratch the GEMM engine may use for split-K partial sums and low-precision
@@ -70,15 +73,40 @@ struct SplitMat {
   int64_t ld = 0;
   int64_t bstride = 0;
 };
+// Spin-class ordering of a pair space (x, y), x from a set with spin labels
+// lx, y from a set with labels ly: pairs are grouped by class
+// c = 2*lx[x] + ly[y] in a chosen class order, lexicographic inside a class,
+// and each class is padded to a multiple of 256.  map[r] is the flat pair
+// index x*ny + y of blocked position r (-1 for padding).
+struct PairSpace {
+  int total = 0;
+  int start[4] = {0, 0, 0, 0};   // per class id: blocked offset,
+  int len[4] = {0, 0, 0, 0};     //   padded length,
+  int count[4] = {0, 0, 0, 0};   //   number of pairs
+  int *d_map = nullptr;          // device, length total
+};
+
 // padded leading dimension for a K-long operand row
 int64_t split_ld(int K);
 // d[b][r][c] = src[b*sb + r*sr + c*sc] split into (hi, lo), r < R, c < C
 void split_convert(const double *src, int64_t sr, int64_t sc, int64_t sb,
                    int R, int C, const SplitMat &d, int nb, hipStream_t s);
-// C[b] = alpha * A[b] . B[b]^T + beta * C[b] with A as [M][K], B as [N][K]
+// d[b][r][c] = src[b*sb + rmap[r]*sr + cmap[c]*sc] split into (hi, lo) for
+// r < R, c < d.ld; entries whose map is -1 are zero
+void split_gather(const double *src, int64_t sr, int64_t sc, int64_t sb,
+                  const int *rmap, int R, const int *cmap, const SplitMat &d,
+                  int nb, hipStream_t s);
+// the same gather into FP64: dst[b*db + r*ld + c], c < ld
+void gather64(const double *src, int64_t sr, int64_t sc, int64_t sb,
+              const int *rmap, int R, const int *cmap, int ld, double *dst,
+              int64_t db, int nb, hipStream_t s);
+// C[b] = alpha * A[b] . B[b]^T + beta * C[b] with A as [M][K], B as [N][K];
+// output row m goes to row rmap[m] of C and column n to cmap[n] when the
+// maps are given (-1 drops the element)
 void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
             const SplitMat &B, double beta, double *C, int64_t sCm,
-            int64_t sCn, int64_t sCb, hipStream_t s);
+            int64_t sCn, int64_t sCb, hipStream_t s,
+            const int *rmap = nullptr, const int *cmap = nullptr);
 
 // ---------------------------------------------------------------------------
 // Elementwise / permutation / reduction kernels (tensor_ops.hip)
@@ -205,5 +233,11 @@ int energy_t1(const double *T1, const double *g, const double *fovT, int ng,
               int nv, int no, double *partials, hipStream_t s);
 int sumsq(const double *x, int64_t n, double *partials, hipStream_t s);
 int vsum(const double *x, int64_t n, double *partials, hipStream_t s);
+// count (third slot of the partial triples) of the nonzero entries of nb
+// consecutive (d0,d1,d2,d3) blocks with l0[i0] + l1[i1] != l2[i2] + l3[i3];
+// null labels read as 0; returns the number of partial triples
+int spin_violations(const double *a, int d0, int d1, int d2, int d3, int nb,
+                    const int *l0, const int *l1, const int *l2, const int *l3,
+                    double *partials, hipStream_t s);
 
 }  // namespace ftcc

@@ -17,10 +17,12 @@
 // the common layouts (A k-contiguous or m-contiguous, B n-contiguous or
 // k-contiguous) stream whole cache lines.
 //
-// Precisions: FP64 (v_mfma_f64_16x16x4_f64), FP32 (v_mfma_f32_16x16x4_f32,
-// FP64 operands rounded to FP32 on load) and split-BF16 "bf16x3" (each FP32
-// value x = hi + lo with hi = bf16(x), lo = bf16(x - hi); the product keeps
-// hi*hi + hi*lo + lo*hi, accumulated in FP32 by v_mfma_f32_16x16x32_bf16).
+// Precisions: FP64 (v_mfma_f64_16x16x4_f64) and FP32 (v_mfma_f32_16x16x4_f32,
+// FP64 operands rounded to FP32 on load).  Split-BF16 products run in
+// sbgemm.hip on operands split ahead of time.
+//
+// The output may be scattered through row and column maps (spin-blocked
+// products write their blocks straight into the caller's layout).
 #include "ftcc_internal.h"
 
 #include <algorithm>
@@ -30,7 +32,6 @@ namespace {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
-typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 typedef double d2_t __attribute__((ext_vector_type(2)));
 
 constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
@@ -47,6 +48,7 @@ struct GemmDev {
   int64_t sCm, sCn, sC1, sC2;
   double *partial;   // [R][nbC][M][N] when splits*R > 1
   int nbC;           // number of distinct C batches (nb1 or nb1*nb2)
+  const int *rmap, *cmap;   // output scatter (nullptr: identity)
 };
 
 template <typename T> struct Mma;
@@ -223,181 +225,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
         if (row < p.M && col < p.N) {
           const double x = (double)acc[rt][ct][v];
           if (direct) {
-            double *c = C + (int64_t)row * p.sCm + (int64_t)col * p.sCn;
-            *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
-          } else {
-            P[(int64_t)row * p.N + col] = x;
-          }
-        }
-      }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// bf16x3: operands split on load into (hi, lo) bf16 pairs.  MFMA
-// v_mfma_f32_16x16x32_bf16: lane l supplies A[i=l%16][k = 8*(l/16) + 0..7]
-// and B[k = 8*(l/16) + 0..7][j=l%16]; C lane layout as the 16x16x4 ops.
-// Slab depth 32; LDS rows hold 32 hi then 32 lo values (K-inner).
-// ---------------------------------------------------------------------------
-constexpr int BK3 = 32;
-constexpr int LDK3 = 2 * BK3 + 8;   // bf16 elements per LDS row (hi|lo|pad)
-
-__device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-  hi = (__bf16)x;
-  lo = (__bf16)(x - (float)hi);
-}
-
-template <bool A_MFAST, bool B_NFAST>
-__device__ inline void load_slab3(const GemmDev &p, const double *A,
-                                  const double *B, int m0, int n0, int k0,
-                                  int kend, float ra[16], float rb[16]) {
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    int m, k;
-    if (A_MFAST) {
-      m = m0 + (t & 127);
-      k = k0 + (t >> 7) + 2 * r;
-    } else {
-      m = m0 + (t >> 5) + 8 * r;
-      k = k0 + (t & 31);
-    }
-    double v = 0.0;
-    if (m < p.M && k < kend) v = A[(int64_t)m * p.sAm + (int64_t)k * p.sAk];
-    ra[r] = (float)v;
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    int n, k;
-    if (B_NFAST) {
-      n = n0 + (t & 127);
-      k = k0 + (t >> 7) + 2 * r;
-    } else {
-      n = n0 + (t >> 5) + 8 * r;
-      k = k0 + (t & 31);
-    }
-    double v = 0.0;
-    if (n < p.N && k < kend) v = B[(int64_t)k * p.sBk + (int64_t)n * p.sBn];
-    rb[r] = (float)v;
-  }
-}
-
-template <bool A_MFAST, bool B_NFAST>
-__device__ inline void store_slab3(__bf16 *As, __bf16 *Bs, const float ra[16],
-                                   const float rb[16]) {
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    int row, k;
-    if (A_MFAST) {
-      row = t & 127;
-      k = (t >> 7) + 2 * r;
-    } else {
-      row = (t >> 5) + 8 * r;
-      k = t & 31;
-    }
-    __bf16 hi, lo;
-    split_bf16(ra[r], hi, lo);
-    As[row * LDK3 + k] = hi;
-    As[row * LDK3 + BK3 + k] = lo;
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    int row, k;
-    if (B_NFAST) {
-      row = t & 127;
-      k = (t >> 7) + 2 * r;
-    } else {
-      row = (t >> 5) + 8 * r;
-      k = t & 31;
-    }
-    __bf16 hi, lo;
-    split_bf16(rb[r], hi, lo);
-    Bs[row * LDK3 + k] = hi;
-    Bs[row * LDK3 + BK3 + k] = lo;
-  }
-}
-
-template <bool A_MFAST, bool B_NFAST>
-__global__ __launch_bounds__(NT, 2) void gemm_bf16x3_kernel(GemmDev p) {
-  __shared__ __bf16 As[2][BM * LDK3];
-  __shared__ __bf16 Bs[2][BN * LDK3];
-  const BlockPos q = decode(p);
-  const double *A = p.A + q.b1 * p.sA1 + q.b2 * p.sA2;
-  const double *B = p.B + q.b1 * p.sB1 + q.b2 * p.sB2;
-  const int m0 = q.tm * BM, n0 = q.tn * BN;
-  const int kbeg = q.ks * p.kchunk;
-  const int kend = min(p.K, kbeg + p.kchunk);
-  const int nslab = (kend - kbeg + BK3 - 1) / BK3;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int li = lane & 15, lg = lane >> 4;
-
-  f4_t acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0, 0, 0, 0};
-
-  float ra[16], rb[16];
-  if (nslab > 0) {
-    load_slab3<A_MFAST, B_NFAST>(p, A, B, m0, n0, kbeg, kend, ra, rb);
-    store_slab3<A_MFAST, B_NFAST>(As[0], Bs[0], ra, rb);
-  }
-  __syncthreads();
-  for (int s = 0; s < nslab; ++s) {
-    const int buf = s & 1;
-    if (s + 1 < nslab)
-      load_slab3<A_MFAST, B_NFAST>(p, A, B, m0, n0, kbeg + (s + 1) * BK3,
-                                   kend, ra, rb);
-    bf8_t ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt) {
-      const __bf16 *src = &As[buf][(wm * 64 + rt * 16 + li) * LDK3 + 8 * lg];
-      ah[rt] = *(const bf8_t *)src;
-      al[rt] = *(const bf8_t *)(src + BK3);
-    }
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
-      const __bf16 *src = &Bs[buf][(wn * 64 + ct * 16 + li) * LDK3 + 8 * lg];
-      bh[ct] = *(const bf8_t *)src;
-      bl[ct] = *(const bf8_t *)(src + BK3);
-    }
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-      for (int ct = 0; ct < 4; ++ct) {
-        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-            al[rt], bh[ct], acc[rt][ct], 0, 0, 0);
-        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-            ah[rt], bl[ct], acc[rt][ct], 0, 0, 0);
-        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-            ah[rt], bh[ct], acc[rt][ct], 0, 0, 0);
-      }
-    if (s + 1 < nslab)
-      store_slab3<A_MFAST, B_NFAST>(As[buf ^ 1], Bs[buf ^ 1], ra, rb);
-    __syncthreads();
-  }
-
-  const bool direct = (p.partial == nullptr);
-  double *C = p.C + q.b1 * p.sC1 + q.b2 * p.sC2;
-  const int cb = p.reduce_b2 ? q.b1 : q.b1 * p.nb2 + q.b2;
-  const int rsel = p.reduce_b2 ? q.b2 * p.splits + q.ks : q.ks;
-  double *P = direct ? nullptr
-                     : p.partial + ((int64_t)rsel * p.nbC + cb) *
-                                       (int64_t)p.M * p.N;
-#pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
-      const int col = n0 + wn * 64 + ct * 16 + li;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm * 64 + rt * 16 + 4 * lg + v;
-        if (row < p.M && col < p.N) {
-          const double x = (double)acc[rt][ct][v];
-          if (direct) {
-            double *c = C + (int64_t)row * p.sCm + (int64_t)col * p.sCn;
+            const int rr = p.rmap ? p.rmap[row] : row;
+            const int cc = p.cmap ? p.cmap[col] : col;
+            if (rr < 0 || cc < 0) continue;
+            double *c = C + (int64_t)rr * p.sCm + (int64_t)cc * p.sCn;
             *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
           } else {
             P[(int64_t)row * p.N + col] = x;
@@ -412,14 +243,17 @@ __global__ void reduce_partial_kernel(const double *partial, int R, int nbC,
                                       int M, int N, int nb2, int reduce_b2,
                                       double alpha, double beta, double *C,
                                       int64_t sCm, int64_t sCn, int64_t sC1,
-                                      int64_t sC2) {
+                                      int64_t sC2, const int *rmap,
+                                      const int *cmap) {
   const int64_t MN = (int64_t)M * N;
   const int64_t total = MN * nbC;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
        idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t cb = idx / MN;
     const int64_t mn = idx % MN;
-    const int m = (int)(mn / N), n = (int)(mn % N);
+    const int m0 = (int)(mn / N), n0 = (int)(mn % N);
+    const int m = rmap ? rmap[m0] : m0, n = cmap ? cmap[n0] : n0;
+    if (m < 0 || n < 0) continue;
     double s = 0.0;
     for (int r = 0; r < R; ++r) s += partial[((int64_t)r * nbC + cb) * MN + mn];
     int64_t b1, b2;
@@ -468,8 +302,9 @@ size_t gemm_partial_elems(const Gemm &g) {
 
 void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.nb1 <= 0 || g.nb2 <= 0) return;
-  require(precision >= 0 && precision <= 2, "unknown precision");
-  const int bk = (precision == FTCC_PREC_BF16X3) ? BK3 : BK;
+  require(precision == FTCC_PREC_FP64 || precision == FTCC_PREC_FP32,
+          "the generic GEMM runs FP64 or FP32 (bf16x3 uses sbgemm)");
+  const int bk = BK;
   GemmDev d;
   d.M = g.M;
   d.N = g.N;
@@ -511,6 +346,8 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   const int R = d.splits * (g.reduce_b2 ? g.nb2 : 1);
   d.nbC = g.reduce_b2 ? g.nb1 : g.nb1 * g.nb2;
   d.partial = nullptr;
+  d.rmap = g.rmap;
+  d.cmap = g.cmap;
   if (R > 1) {
     const size_t need = (size_t)R * d.nbC * g.M * g.N;
     require(need <= scr.partial_elems, "GEMM split-K scratch too small",
@@ -529,18 +366,14 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
     else if (amf) launch(gemm_mfma_kernel<float, true, false>, d, s);
     else if (bnf) launch(gemm_mfma_kernel<float, false, true>, d, s);
     else launch(gemm_mfma_kernel<float, false, false>, d, s);
-  } else {
-    if (amf && bnf) launch(gemm_bf16x3_kernel<true, true>, d, s);
-    else if (amf) launch(gemm_bf16x3_kernel<true, false>, d, s);
-    else if (bnf) launch(gemm_bf16x3_kernel<false, true>, d, s);
-    else launch(gemm_bf16x3_kernel<false, false>, d, s);
   }
   if (R > 1) {
     const int64_t total = (int64_t)d.nbC * g.M * g.N;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
     hipLaunchKernelGGL(reduce_partial_kernel, dim3(blocks), dim3(256), 0, s,
                        d.partial, R, d.nbC, g.M, g.N, g.nb2, d.reduce_b2,
-                       g.alpha, g.beta, g.C, g.sCm, g.sCn, g.sC1, g.sC2);
+                       g.alpha, g.beta, g.C, g.sCm, g.sCn, g.sC1, g.sC2,
+                       g.rmap, g.cmap);
     FTCC_CHECK_HIP(hipGetLastError());
   }
 }

@@ -45,6 +45,7 @@ struct SbDev {
   double *C;
   int64_t sCm, sCn, sCb;
   double alpha, beta;
+  const int *rmap, *cmap;   // output scatter (nullptr: identity; -1 drops)
 };
 
 __device__ inline void tile_of(const SbDev &p, int &bt, int &tm, int &tn) {
@@ -170,11 +171,15 @@ __global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
     for (int ct = 0; ct < CT; ++ct) {
       const int col = n0 + wn * (32 * CT) + ct * 32 + li;
       if (col >= p.N) continue;
+      const int cc = p.cmap ? p.cmap[col] : col;
+      if (cc < 0) continue;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int row = m0 + wm * 128 + rt * 32 + 8 * (v >> 2) + 4 * lg + (v & 3);
         if (row < p.M) {
-          double *c = C + (int64_t)row * p.sCm + (int64_t)col * p.sCn;
+          const int rr = p.rmap ? p.rmap[row] : row;
+          if (rr < 0) continue;
+          double *c = C + (int64_t)rr * p.sCm + (int64_t)cc * p.sCn;
           const double x = (double)acc[rt][ct][v];
           *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
         }
@@ -188,52 +193,94 @@ __device__ inline void split1(double v, __bf16 &hi, __bf16 &lo) {
   lo = (__bf16)(f - (float)hi);
 }
 
-// dst[b][r][c] = src[b*sb + r*sr + c*sc] for c < C, 0 for C <= c < ld;
-// threads run along c (use when sc is the small stride).
-__global__ void split_rows_kernel(const double *src, int64_t sr, int64_t sc,
-                                  int64_t sb, int R, int C, int64_t ld,
-                                  int64_t db, __bf16 *hi, __bf16 *lo, int nb) {
+// Destination writers of the gather kernels: split into (hi, lo) bf16, or
+// plain FP64 (the spin-blocked operands of the FP64 and FP32 modes).
+struct SplitOut {
+  __bf16 *hi, *lo;
+  __device__ void operator()(int64_t o, double v) const {
+    __bf16 h, l;
+    split1(v, h, l);
+    hi[o] = h;
+    lo[o] = l;
+  }
+};
+struct F64Out {
+  double *p;
+  __device__ void operator()(int64_t o, double v) const { p[o] = v; }
+};
+
+// source row / column of destination index i: map[i] (-1: zero) or i itself
+// below the bound n
+__device__ inline int src_index(const int *map, int i, int n) {
+  return map ? map[i] : (i < n ? i : -1);
+}
+
+// dst[b][r][c] = src[b*sb + R(r)*sr + C(c)*sc] for r < R, c < ld, zero where
+// a map gives -1 (or, without maps, for c >= C); threads run along c (use
+// when sc is the small stride).
+template <typename Out>
+__global__ void gather_rows_kernel(const double *src, int64_t sr, int64_t sc,
+                                   int64_t sb, const int *rmap, int R,
+                                   const int *cmap, int C, int64_t ld,
+                                   int64_t db, Out out, int nb) {
   const int64_t per = (int64_t)R * ld;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
        idx < per * nb; idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = idx / per, e = idx % per;
     const int r = (int)(e / ld);
     const int c = (int)(e % ld);
-    const double v = (c < C) ? src[b * sb + (int64_t)r * sr + (int64_t)c * sc] : 0.0;
-    __bf16 h, l;
-    split1(v, h, l);
-    hi[b * db + e] = h;
-    lo[b * db + e] = l;
+    const int rr = src_index(rmap, r, R), cc = src_index(cmap, c, C);
+    const double v = (rr >= 0 && cc >= 0)
+                         ? src[b * sb + (int64_t)rr * sr + (int64_t)cc * sc]
+                         : 0.0;
+    out(b * db + e, v);
   }
 }
 
 // Same map through a 64x64 LDS tile: reads run along r (sr is the small
 // stride), writes along c.
-__global__ void split_transpose_kernel(const double *src, int64_t sr,
-                                       int64_t sc, int64_t sb, int R, int C,
-                                       int64_t ld, int64_t db, __bf16 *hi,
-                                       __bf16 *lo) {
+template <typename Out>
+__global__ void gather_transpose_kernel(const double *src, int64_t sr,
+                                        int64_t sc, int64_t sb,
+                                        const int *rmap, int R,
+                                        const int *cmap, int C, int64_t ld,
+                                        int64_t db, Out out) {
   __shared__ double tile[64][65];
   const int b = blockIdx.z;
   const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 256 threads
   for (int i = ty; i < 64; i += 4) {
     const int r = r0 + tx, c = c0 + i;
-    tile[i][tx] = (r < R && c < C)
-                      ? src[b * sb + (int64_t)r * sr + (int64_t)c * sc]
+    const int rr = r < R ? src_index(rmap, r, R) : -1;
+    const int cc = c < ld ? src_index(cmap, c, C) : -1;
+    tile[i][tx] = (rr >= 0 && cc >= 0)
+                      ? src[b * sb + (int64_t)rr * sr + (int64_t)cc * sc]
                       : 0.0;
   }
   __syncthreads();
   for (int i = ty; i < 64; i += 4) {
     const int r = r0 + i, c = c0 + tx;
-    if (r < R && c < ld) {
-      __bf16 h, l;
-      split1(tile[tx][i], h, l);
-      const int64_t o = b * db + (int64_t)r * ld + c;
-      hi[o] = h;
-      lo[o] = l;
-    }
+    if (r < R && c < ld) out(b * db + (int64_t)r * ld + c, tile[tx][i]);
   }
+}
+
+template <typename Out>
+void gather_launch(const double *src, int64_t sr, int64_t sc, int64_t sb,
+                   const int *rmap, int R, const int *cmap, int C, int64_t ld,
+                   int64_t db, Out out, int nb, hipStream_t s) {
+  if ((int64_t)R * ld == 0 || nb <= 0) return;
+  if (sr == 1 && sc != 1) {
+    const int gx = (int)((ld + 63) / 64), gy = (R + 63) / 64;
+    hipLaunchKernelGGL(gather_transpose_kernel<Out>, dim3(gx, gy, nb),
+                       dim3(256), 0, s, src, sr, sc, sb, rmap, R, cmap, C, ld,
+                       db, out);
+  } else {
+    const int64_t total = (int64_t)R * ld * nb;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(gather_rows_kernel<Out>, dim3(blocks), dim3(256), 0, s,
+                       src, sr, sc, sb, rmap, R, cmap, C, ld, db, out, nb);
+  }
+  FTCC_CHECK_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -244,22 +291,29 @@ void split_convert(const double *src, int64_t sr, int64_t sc, int64_t sb,
                    int R, int C, const SplitMat &d, int nb, hipStream_t s) {
   require(d.ld >= C && d.ld % 32 == 0, "split operand leading dimension");
   if ((int64_t)R * C == 0) return;
-  if (sr == 1 && sc != 1) {
-    const int gx = (int)((d.ld + 63) / 64), gy = (R + 63) / 64;
-    hipLaunchKernelGGL(split_transpose_kernel, dim3(gx, gy, nb), dim3(256), 0,
-                       s, src, sr, sc, sb, R, C, d.ld, d.bstride, d.hi, d.lo);
-  } else {
-    const int64_t total = (int64_t)R * d.ld * nb;
-    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
-    hipLaunchKernelGGL(split_rows_kernel, dim3(blocks), dim3(256), 0, s, src,
-                       sr, sc, sb, R, C, d.ld, d.bstride, d.hi, d.lo, nb);
-  }
-  FTCC_CHECK_HIP(hipGetLastError());
+  gather_launch(src, sr, sc, sb, nullptr, R, nullptr, C, d.ld, d.bstride,
+                SplitOut{d.hi, d.lo}, nb, s);
+}
+
+void split_gather(const double *src, int64_t sr, int64_t sc, int64_t sb,
+                  const int *rmap, int R, const int *cmap, const SplitMat &d,
+                  int nb, hipStream_t s) {
+  require(d.ld % 32 == 0, "split operand leading dimension");
+  gather_launch(src, sr, sc, sb, rmap, R, cmap, (int)d.ld, d.ld, d.bstride,
+                SplitOut{d.hi, d.lo}, nb, s);
+}
+
+void gather64(const double *src, int64_t sr, int64_t sc, int64_t sb,
+              const int *rmap, int R, const int *cmap, int ld, double *dst,
+              int64_t db, int nb, hipStream_t s) {
+  gather_launch(src, sr, sc, sb, rmap, R, cmap, ld, ld, db, F64Out{dst}, nb,
+                s);
 }
 
 void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
             const SplitMat &B, double beta, double *C, int64_t sCm,
-            int64_t sCn, int64_t sCb, hipStream_t s) {
+            int64_t sCn, int64_t sCb, hipStream_t s, const int *rmap,
+            const int *cmap) {
   if (M <= 0 || N <= 0 || nb <= 0) return;
   require(A.ld % 32 == 0 && B.ld % 32 == 0 && A.ld >= K && B.ld >= K,
           "split operands need K padded to 32");
@@ -291,6 +345,8 @@ void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
   p.sCb = sCb;
   p.alpha = alpha;
   p.beta = beta;
+  p.rmap = rmap;
+  p.cmap = cmap;
   const int64_t nblk = (int64_t)p.tilesM * p.tilesN * nb;
   require(nblk < (1LL << 31), "too many GEMM tiles");
   hipLaunchKernelGGL(sbgemm_kernel, dim3((unsigned)nblk), dim3(512),

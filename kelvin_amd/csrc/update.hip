@@ -359,6 +359,31 @@ __global__ __launch_bounds__(UT) void vsum_kernel(const double *x, int64_t n,
   block_reduce3(s, 0.0, 0.0, partials);
 }
 
+// Entries of nb dense (d0,d1,d2,d3) blocks that are nonzero although
+// l0[i0] + l1[i1] != l2[i2] + l3[i3] (a null label array reads as 0): the
+// spin-conservation test of integral blocks and amplitudes.
+__global__ __launch_bounds__(UT) void spin_violations_kernel(
+    const double *a, int d0, int d1, int d2, int d3, int nb, const int *l0,
+    const int *l1, const int *l2, const int *l3, double *partials) {
+  const int64_t per = (int64_t)d0 * d1 * d2 * d3;
+  const int64_t n = per * nb;
+  double cnt = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * UT) {
+    int64_t r = e % per;
+    const int i3 = (int)(r % d3);
+    r /= d3;
+    const int i2 = (int)(r % d2);
+    r /= d2;
+    const int i1 = (int)(r % d1);
+    const int i0 = (int)(r / d1);
+    const int sx = (l0 ? l0[i0] : 0) + (l1 ? l1[i1] : 0);
+    const int sy = (l2 ? l2[i2] : 0) + (l3 ? l3[i3] : 0);
+    if (sx != sy && a[e] != 0.0) cnt += 1.0;
+  }
+  block_reduce3(0.0, 0.0, cnt, partials);
+}
+
 int nblocks_for(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + UT - 1) / UT, MAX_BLOCKS));
 }
@@ -415,6 +440,16 @@ int vsum(const double *x, int64_t n, double *partials, hipStream_t s) {
   const int blocks = nblocks_for(n);
   hipLaunchKernelGGL(vsum_kernel, dim3(blocks), dim3(UT), 0, s, x, n,
                      partials);
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
+int spin_violations(const double *a, int d0, int d1, int d2, int d3, int nb,
+                    const int *l0, const int *l1, const int *l2, const int *l3,
+                    double *partials, hipStream_t s) {
+  const int blocks = nblocks_for((int64_t)d0 * d1 * d2 * d3 * nb);
+  hipLaunchKernelGGL(spin_violations_kernel, dim3(blocks), dim3(UT), 0, s, a,
+                     d0, d1, d2, d3, nb, l0, l1, l2, l3, partials);
   FTCC_CHECK_HIP(hipGetLastError());
   return blocks;
 }

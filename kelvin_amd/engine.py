@@ -147,6 +147,27 @@ class FTEngine:
         ip = (ctypes.c_void_p * 9)(*[t.data_ptr() for t in ts])
         self._call("ftcc_set_blocks", fp, ip)
 
+    def set_spin(self, lab_o=None, lab_v=None):
+        """Spin labels (0/1) of the occupied and virtual lists for blocked
+        O(n^6) products; None for both switches blocking off.  Raises
+        FTCCError if the integrals do not conserve spin under the labels."""
+        if lab_o is None and lab_v is None:
+            self._call("ftcc_set_spin", None, None)
+            return
+        lo = numpy.ascontiguousarray(lab_o, dtype=numpy.int32)
+        lv = numpy.ascontiguousarray(lab_v, dtype=numpy.int32)
+        if lo.shape != (self.no,) or lv.shape != (self.nv,):
+            raise ValueError("spin labels need shapes (no,) and (nv,)")
+        ip = ctypes.POINTER(ctypes.c_int)
+        self._call("ftcc_set_spin", lo.ctypes.data_as(ip),
+                   lv.ctypes.data_as(ip))
+
+    def spin_blocked(self):
+        """True when the O(n^6) products run as spin blocks."""
+        on = ctypes.c_int()
+        self._call("ftcc_get_spin", ctypes.byref(on))
+        return bool(on.value)
+
     def set_denominators(self, D1, D2):
         D1 = self.to_device(D1)
         D2 = self.to_device(D2)

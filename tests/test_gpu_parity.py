@@ -404,3 +404,122 @@ def test_golden_ESN(key, precision):
     assert abs(N - g["N"]) < tol
     assert abs(cc.N0 + cc.N1 + cc.Ncc - N) < 1e-12
     assert abs(cc.E0 + cc.E1 + cc.Ecc - E) < 1e-12
+
+
+# ------------------------------------------------------------ spin blocking
+def _spin_problem(no, nv, ng, lab, rng, scale=0.2):
+    """Random Fock/ERI blocks and amplitudes that conserve spin under the
+    orbital labels `lab` (occupied first): entries with unequal spin sums
+    are zeroed, which keeps the antisymmetry."""
+    n = no + nv
+    f = rng.normal(size=(n, n)) * 0.3
+    f = f * (lab[:, None] == lab[None, :])
+    eri = fs.random_antisym(n, rng, hermitian=False) * 0.1
+    s = lab
+    eri = eri * ((s[:, None, None, None] + s[None, :, None, None]) ==
+                 (s[None, None, :, None] + s[None, None, None, :]))
+    sl = {'o': slice(0, no), 'v': slice(no, n)}
+    F = oracle.Blocks(oo=f[sl['o'], sl['o']], ov=f[sl['o'], sl['v']],
+                      vo=f[sl['v'], sl['o']], vv=f[sl['v'], sl['v']])
+    I = oracle.Blocks({k: numpy.ascontiguousarray(
+        eri[sl[k[0]], sl[k[1]], sl[k[2]], sl[k[3]]])
+        for k in ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo', 'oovv', 'vooo',
+                  'ooov', 'oooo')})
+    lo, lv = lab[:no], lab[no:]
+    t1, t2 = random_T(ng, nv, no, rng, scale)
+    t1 = t1 * (lv[:, None] == lo[None, :])
+    t2 = t2 * ((lv[:, None, None, None] + lv[None, :, None, None]) ==
+               (lo[None, None, :, None] + lo[None, None, None, :]))
+    return F, I, t1, t2, lo, lv
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5),
+                                      ("bf16x3", 2e-4)])
+@pytest.mark.parametrize("no,nv,labels", [
+    (5, 6, "alt"),        # unequal classes in every pair space
+    (3, 7, "occ-alpha"),  # no beta occupied orbital: empty classes
+    (17, 17, "half")])    # classes above one 256 tile
+def test_spin_blocked_residual(prec, tol, no, nv, labels):
+    """Spin-blocked O(n^6) products (three blocks each, written through the
+    pair maps) against the oracle's dense residual, and equal to the dense
+    engine path to rounding."""
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd.ov_blocks import one_e_blocks, two_e_blocks
+    d = dev()
+    rng = numpy.random.default_rng(31 * no + nv)
+    n = no + nv
+    if labels == "alt":
+        lab = numpy.arange(n) % 2
+    elif labels == "occ-alpha":
+        lab = numpy.concatenate([numpy.zeros(no, int), numpy.arange(nv) % 2])
+    else:
+        lab = numpy.concatenate([numpy.arange(no) >= no // 2,
+                                 numpy.arange(nv) >= nv // 2]).astype(int)
+    ng = 2
+    F, I, t1, t2, lo, lv = _spin_problem(no, nv, ng, lab, rng)
+    ref1, ref2 = oracle.residual_tbar(F, I, t1, t2)
+    out = {}
+    for blocked in (True, False):
+        eng = FTEngine(no, nv, ng, precision=prec)
+        eng.set_blocks(one_e_blocks(**{k: F[k] for k in ('oo', 'ov', 'vo', 'vv')}),
+                       two_e_blocks(**{k: I[k] for k in I}))
+        eng.set_denominators(numpy.ones((nv, no)), numpy.ones((nv, nv, no, no)))
+        eng.set_grid(numpy.zeros(ng), numpy.zeros((ng, ng)))
+        if blocked:
+            eng.set_spin(lo, lv)
+            assert eng.spin_blocked()
+        else:
+            assert not eng.spin_blocked()
+        T1 = torch.as_tensor(t1, device=d)
+        T2 = torch.as_tensor(t2, device=d)
+        b1, b2 = torch.empty_like(T1), torch.empty_like(T2)
+        eng.residual(T1, T2, b1, b2)
+        out[blocked] = (to_np(b1), to_np(b2))
+        assert rel(out[blocked][0], ref1) < tol
+        assert rel(out[blocked][1], ref2) < tol
+    # blocked and dense differ only by summation order (and rounding of the
+    # split operands, which is the same per element)
+    assert rel(out[True][1], out[False][1]) < max(tol, 1e-12)
+
+
+def test_spin_labels_validated():
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd._lib import FTCCError
+    from kelvin_amd.ov_blocks import one_e_blocks, two_e_blocks
+    dev()
+    rng = numpy.random.default_rng(2)
+    no, nv = 3, 4
+    lab = numpy.arange(no + nv) % 2
+    F, I, _, _, lo, lv = _spin_problem(no, nv, 1, lab, rng)
+    eng = FTEngine(no, nv, 1)
+    eng.set_blocks(one_e_blocks(**{k: F[k] for k in ('oo', 'ov', 'vo', 'vv')}),
+                   two_e_blocks(**{k: I[k] for k in I}))
+    with pytest.raises(FTCCError):
+        eng.set_spin(1 - lo, lv)          # wrong labels: integrals violate
+    assert not eng.spin_blocked()
+    eng.set_spin(lo, lv)
+    assert eng.spin_blocked()
+    eng.set_spin(None, None)
+    assert not eng.spin_blocked()
+
+
+def test_spin_detected_for_ueg_and_dense_for_generic():
+    """UEG spin orbitals (alpha block, then beta) are detected; a system
+    whose integrals mix spins keeps dense products."""
+    from kelvin_amd.ccsd import ccsd
+    from kelvin_amd.model_systems import ArraySystem
+    dev()
+    sysg, s = _ueg_gold("G1")
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], ngrid=4, max_iter=2)
+    cc.run()
+    assert cc.engine.spin_blocked()
+    rng = numpy.random.default_rng(9)
+    n = 6
+    h = rng.normal(size=(n, n)) * 0.2
+    h = h + h.T
+    eri = fs.random_antisym(n, rng, hermitian=True) * 0.1
+    en = numpy.sort(rng.uniform(-1, 1, n))
+    sysa = ArraySystem(0.5, 0.0, h, eri, en)
+    cc = ccsd(sysa, T=0.5, mu=0.0, ngrid=4, max_iter=2)
+    cc.run()
+    assert not cc.engine.spin_blocked()
