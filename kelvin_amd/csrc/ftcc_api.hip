@@ -514,38 +514,56 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
   const PairSpace &Cs = space(c, OP_ROWS[ib]);
   Blk blk[3];
   block_list(kind, R, K, Cs, blk);
+  if (P == FTCC_PREC_BF16X3) {
+    // the three blocks write disjoint parts of C: one grouped launch, the
+    // largest block first (its tiles are the longest)
+    SbProblem pb[3];
+    int np = 0;
+    double flops = 0.0;
+    for (const Blk &b : blk) {
+      // an empty K range leaves C as it is: the callers either accumulate
+      // (beta = 1) or have cleared C beforehand
+      if (!b.r.count || !b.k.count || !b.c.count) continue;
+      SbProblem &q = pb[np++];
+      q.M = b.r.len;
+      q.N = b.c.len;
+      q.K = b.k.len;
+      q.A = split_of(c, ia);
+      q.B = split_of(c, ib);
+      const int64_t ao = (int64_t)b.r.start * q.A.ld + b.k.start;
+      const int64_t bo = (int64_t)b.c.start * q.B.ld + b.k.start;
+      q.A.hi += ao;
+      q.A.lo += ao;
+      q.B.hi += bo;
+      q.B.lo += bo;
+      q.rmap = R.d_map + b.r.start;
+      q.cmap = Cs.d_map + b.c.start;
+      flops += 2.0 * b.r.count * (double)b.c.count * b.k.count * g.nb1;
+    }
+    std::sort(pb, pb + np, [](const SbProblem &x, const SbProblem &y) {
+      return (double)x.M * x.N * x.K > (double)y.M * y.N * y.K;
+    });
+    if (np == 0) return;
+    timed(c, true, flops, [&] {
+      sbgemm_grouped(np, pb, g.nb1, g.alpha, g.beta, g.C, g.sCm, g.sCn,
+                     g.sC1, c->stream);
+    });
+    return;
+  }
   for (const Blk &b : blk) {
-    // an empty K range leaves C as it is: the callers either accumulate
-    // (beta = 1) or have cleared C beforehand
     if (!b.r.count || !b.k.count || !b.c.count) continue;
     const double flops = 2.0 * b.r.count * (double)b.c.count * b.k.count * g.nb1;
-    const int *rmap = R.d_map + b.r.start, *cmap = Cs.d_map + b.c.start;
-    if (P == FTCC_PREC_BF16X3) {
-      SplitMat A = split_of(c, ia), B = split_of(c, ib);
-      const int64_t ao = (int64_t)b.r.start * A.ld + b.k.start;
-      const int64_t bo = (int64_t)b.c.start * B.ld + b.k.start;
-      A.hi += ao;
-      A.lo += ao;
-      B.hi += bo;
-      B.lo += bo;
-      timed(c, true, flops, [&] {
-        sbgemm(b.r.len, b.c.len, b.k.len, g.nb1, g.alpha, A, B, g.beta, g.C,
-               g.sCm, g.sCn, g.sC1, c->stream, rmap, cmap);
-      });
-    } else {
-      const int64_t ld = K.total;
-      Gemm gb = mk(b.r.len, b.c.len, b.k.len, g.alpha, g.beta,
-                   c->b64[ia] + (int64_t)b.r.start * ld + b.k.start, ld, 1,
-                   OP_FIXED[ia] ? 0 : (int64_t)R.total * ld,
-                   c->b64[ib] + (int64_t)b.c.start * ld + b.k.start, 1, ld,
-                   OP_FIXED[ib] ? 0 : (int64_t)Cs.total * ld, g.C, g.sCm,
-                   g.sCn, g.sC1, g.nb1);
-      gb.rmap = rmap;
-      gb.cmap = cmap;
-      ensure_partial(c, gb);
-      timed(c, true, flops,
-            [&] { gemm_run(gb, P, c->scr, c->stream); });
-    }
+    const int64_t ld = K.total;
+    Gemm gb = mk(b.r.len, b.c.len, b.k.len, g.alpha, g.beta,
+                 c->b64[ia] + (int64_t)b.r.start * ld + b.k.start, ld, 1,
+                 OP_FIXED[ia] ? 0 : (int64_t)R.total * ld,
+                 c->b64[ib] + (int64_t)b.c.start * ld + b.k.start, 1, ld,
+                 OP_FIXED[ib] ? 0 : (int64_t)Cs.total * ld, g.C, g.sCm,
+                 g.sCn, g.sC1, g.nb1);
+    gb.rmap = R.d_map + b.r.start;
+    gb.cmap = Cs.d_map + b.c.start;
+    ensure_partial(c, gb);
+    timed(c, true, flops, [&] { gemm_run(gb, P, c->scr, c->stream); });
   }
 }
 

@@ -107,6 +107,16 @@ void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
             const SplitMat &B, double beta, double *C, int64_t sCm,
             int64_t sCn, int64_t sCb, hipStream_t s,
             const int *rmap = nullptr, const int *cmap = nullptr);
+// Up to three products sharing C, alpha, beta and the batch in one launch
+// (their outputs must be disjoint); tiles are issued in the order given.
+struct SbProblem {
+  int M, N, K;
+  SplitMat A, B;
+  const int *rmap, *cmap;
+};
+void sbgemm_grouped(int nprob, const SbProblem *prob, int nb, double alpha,
+                    double beta, double *C, int64_t sCm, int64_t sCn,
+                    int64_t sCb, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Elementwise / permutation / reduction kernels (tensor_ops.hip)

@@ -36,30 +36,47 @@ constexpr int BM = 256, BN = 256, BK = 32, LDR = 72;
 constexpr int GROUP_M = 8;
 constexpr size_t LDS_BYTES = 2 * 2 * (size_t)BM * LDR * sizeof(__bf16);
 
-struct SbDev {
-  int M, N, Kp, nb, tilesM, tilesN;
+// One product of a grouped launch (the spin blocks of one O(n^6) product
+// share C, alpha, beta and the batch; up to MAXP problems per launch).
+constexpr int MAXP = 3;
+struct SbProb {
+  int M, N, Kp, tilesM, tilesN;
+  int tile0;                 // first linear tile of this problem
   const __bf16 *Ahi, *Alo;
   int64_t lda, sAb;
   const __bf16 *Bhi, *Blo;
   int64_t ldb, sBb;
+  const int *rmap, *cmap;    // output scatter (nullptr: identity; -1 drops)
+};
+struct SbDev {
+  int nprob, nb;
+  SbProb pr[MAXP];
   double *C;
   int64_t sCm, sCn, sCb;
   double alpha, beta;
-  const int *rmap, *cmap;   // output scatter (nullptr: identity; -1 drops)
 };
 
-__device__ inline void tile_of(const SbDev &p, int &bt, int &tm, int &tn) {
+// Workgroup -> (problem, batch, tile m, tile n).  Consecutive workgroup ids
+// go to different XCDs, so ids are first regrouped per XCD (each XCD then
+// takes a contiguous run of linear tiles); inside a problem, tiles run in
+// GROUP_M-row bands so that an XCD's concurrent tiles share A/B panels.
+__device__ inline SbProb tile_of(const SbDev &p, int &bt, int &tm, int &tn) {
   const int nblk = gridDim.x, b = blockIdx.x;
   const int q = nblk / 8, r = nblk % 8, x = b % 8, j = b / 8;
   const int lid = (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
-  const int per = p.tilesM * p.tilesN;
-  bt = lid / per;
-  const int t = lid % per;
-  const int span = GROUP_M * p.tilesN;
+  SbProb pr = p.pr[0];
+  if (p.nprob > 1 && lid >= p.pr[1].tile0) pr = p.pr[1];
+  if (p.nprob > 2 && lid >= p.pr[2].tile0) pr = p.pr[2];
+  const int local = lid - pr.tile0;
+  const int per = pr.tilesM * pr.tilesN;
+  bt = local / per;
+  const int t = local % per;
+  const int span = GROUP_M * pr.tilesN;
   const int first_m = (t / span) * GROUP_M;
-  const int gsz = min(p.tilesM - first_m, GROUP_M);
+  const int gsz = min(pr.tilesM - first_m, GROUP_M);
   tm = first_m + (t % span) % gsz;
   tn = (t % span) / gsz;
+  return pr;
 }
 
 // 2048 16-byte chunks per 256-row operand slab: (array, row, quarter) each
@@ -96,14 +113,14 @@ __global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
   constexpr int WN = 4, NT = 512, RT = 4, CT = 2;
   extern __shared__ __bf16 lds[];   // [2 stages][A|B][BM*LDR]
   int bt, tm, tn;
-  tile_of(p, bt, tm, tn);
+  const SbProb pr = tile_of(p, bt, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
-  const __bf16 *Ahi = p.Ahi + bt * p.sAb, *Alo = p.Alo + bt * p.sAb;
-  const __bf16 *Bhi = p.Bhi + bt * p.sBb, *Blo = p.Blo + bt * p.sBb;
+  const __bf16 *Ahi = pr.Ahi + bt * pr.sAb, *Alo = pr.Alo + bt * pr.sAb;
+  const __bf16 *Bhi = pr.Bhi + bt * pr.sBb, *Blo = pr.Blo + bt * pr.sBb;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave / WN, wn = wave % WN;
   const int li = lane & 31, lg = lane >> 5;
-  const int nslab = p.Kp / BK;
+  const int nslab = pr.Kp / BK;
 
   f16_t acc[RT][CT];
 #pragma unroll
@@ -115,8 +132,8 @@ __global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
 
   u4_t ra[2048 / NT], rb[2048 / NT];
   constexpr int STAGE = 2 * BM * LDR;   // bf16 per stage (A then B)
-  load_operand<NT>(Ahi, Alo, p.lda, m0, p.M, 0, ra);
-  load_operand<NT>(Bhi, Blo, p.ldb, n0, p.N, 0, rb);
+  load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, 0, ra);
+  load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, 0, rb);
   store_operand<NT>(lds, ra);
   store_operand<NT>(lds + BM * LDR, rb);
   __syncthreads();
@@ -124,8 +141,8 @@ __global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
     const __bf16 *As = lds + (s & 1) * STAGE;
     const __bf16 *Bs = As + BM * LDR;
     if (s + 1 < nslab) {
-      load_operand<NT>(Ahi, Alo, p.lda, m0, p.M, (s + 1) * BK, ra);
-      load_operand<NT>(Bhi, Blo, p.ldb, n0, p.N, (s + 1) * BK, rb);
+      load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, (s + 1) * BK, ra);
+      load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, (s + 1) * BK, rb);
     }
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
@@ -170,14 +187,14 @@ __global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) {
       const int col = n0 + wn * (32 * CT) + ct * 32 + li;
-      if (col >= p.N) continue;
-      const int cc = p.cmap ? p.cmap[col] : col;
+      if (col >= pr.N) continue;
+      const int cc = pr.cmap ? pr.cmap[col] : col;
       if (cc < 0) continue;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int row = m0 + wm * 128 + rt * 32 + 8 * (v >> 2) + 4 * lg + (v & 3);
-        if (row < p.M) {
-          const int rr = p.rmap ? p.rmap[row] : row;
+        if (row < pr.M) {
+          const int rr = pr.rmap ? pr.rmap[row] : row;
           if (rr < 0) continue;
           double *c = C + (int64_t)rr * p.sCm + (int64_t)cc * p.sCn;
           const double x = (double)acc[rt][ct][v];
@@ -310,13 +327,10 @@ void gather64(const double *src, int64_t sr, int64_t sc, int64_t sb,
                 s);
 }
 
-void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
-            const SplitMat &B, double beta, double *C, int64_t sCm,
-            int64_t sCn, int64_t sCb, hipStream_t s, const int *rmap,
-            const int *cmap) {
-  if (M <= 0 || N <= 0 || nb <= 0) return;
-  require(A.ld % 32 == 0 && B.ld % 32 == 0 && A.ld >= K && B.ld >= K,
-          "split operands need K padded to 32");
+void sbgemm_grouped(int nprob, const SbProblem *prob, int nb, double alpha,
+                    double beta, double *C, int64_t sCm, int64_t sCn,
+                    int64_t sCb, hipStream_t s) {
+  require(nprob >= 1 && nprob <= MAXP, "1 to 3 problems per grouped launch");
   static bool attr_set = false;
   if (!attr_set) {
     FTCC_CHECK_HIP(hipFuncSetAttribute(
@@ -325,33 +339,54 @@ void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
     attr_set = true;
   }
   SbDev p;
-  p.M = M;
-  p.N = N;
-  p.Kp = (int)((K + BK - 1) / BK * BK);
   p.nb = nb;
-  p.tilesM = (M + BM - 1) / BM;
-  p.tilesN = (N + BN - 1) / BN;
-  p.Ahi = A.hi;
-  p.Alo = A.lo;
-  p.lda = A.ld;
-  p.sAb = A.bstride;
-  p.Bhi = B.hi;
-  p.Blo = B.lo;
-  p.ldb = B.ld;
-  p.sBb = B.bstride;
   p.C = C;
   p.sCm = sCm;
   p.sCn = sCn;
   p.sCb = sCb;
   p.alpha = alpha;
   p.beta = beta;
-  p.rmap = rmap;
-  p.cmap = cmap;
-  const int64_t nblk = (int64_t)p.tilesM * p.tilesN * nb;
-  require(nblk < (1LL << 31), "too many GEMM tiles");
-  hipLaunchKernelGGL(sbgemm_kernel, dim3((unsigned)nblk), dim3(512),
+  p.nprob = 0;
+  int64_t tiles = 0;
+  for (int k = 0; k < nprob; ++k) {
+    const SbProblem &q = prob[k];
+    if (q.M <= 0 || q.N <= 0) continue;
+    require(q.A.ld % 32 == 0 && q.B.ld % 32 == 0 && q.A.ld >= q.K &&
+                q.B.ld >= q.K,
+            "split operands need K padded to 32");
+    SbProb &d = p.pr[p.nprob++];
+    d.M = q.M;
+    d.N = q.N;
+    d.Kp = (int)((q.K + BK - 1) / BK * BK);
+    d.tilesM = (q.M + BM - 1) / BM;
+    d.tilesN = (q.N + BN - 1) / BN;
+    d.tile0 = (int)tiles;
+    d.Ahi = q.A.hi;
+    d.Alo = q.A.lo;
+    d.lda = q.A.ld;
+    d.sAb = q.A.bstride;
+    d.Bhi = q.B.hi;
+    d.Blo = q.B.lo;
+    d.ldb = q.B.ld;
+    d.sBb = q.B.bstride;
+    d.rmap = q.rmap;
+    d.cmap = q.cmap;
+    tiles += (int64_t)d.tilesM * d.tilesN * nb;
+  }
+  if (p.nprob == 0 || nb <= 0) return;
+  require(tiles < (1LL << 31), "too many GEMM tiles");
+  hipLaunchKernelGGL(sbgemm_kernel, dim3((unsigned)tiles), dim3(512),
                      LDS_BYTES, s, p);
   FTCC_CHECK_HIP(hipGetLastError());
+}
+
+void sbgemm(int M, int N, int K, int nb, double alpha, const SplitMat &A,
+            const SplitMat &B, double beta, double *C, int64_t sCm,
+            int64_t sCn, int64_t sCb, hipStream_t s, const int *rmap,
+            const int *cmap) {
+  if (M <= 0 || N <= 0 || nb <= 0) return;
+  SbProblem q{M, N, K, A, B, rmap, cmap};
+  sbgemm_grouped(1, &q, nb, alpha, beta, C, sCm, sCn, sCb, s);
 }
 
 }  // namespace ftcc
