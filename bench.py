@@ -59,18 +59,19 @@ def parse():
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sbgemm_traffic_ueg_n114.txt")
 
 
-def measured_traffic(precision, n, ng):
+def measured_traffic(precision, n, ng, layout):
     """HBM bytes per launch of the dominant kernel, from the committed
     FETCH_SIZE/WRITE_SIZE profile of this workload (a line
-    'traffic_bytes_per_launch <precision> <n> <ng> <bytes>'), or None."""
+    'traffic_bytes_per_launch <precision> <n> <ng> <layout> <bytes>',
+    layout 'spin' or 'dense'), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 w = line.split()
-                if (len(w) == 5 and w[0] == "traffic_bytes_per_launch"
+                if (len(w) == 6 and w[0] == "traffic_bytes_per_launch"
                         and w[1] == precision and int(w[2]) == n
-                        and int(w[3]) == ng):
-                    return float(w[4])
+                        and int(w[3]) == ng and w[4] == layout):
+                    return float(w[5])
     except OSError:
         pass
     return None
@@ -243,7 +244,8 @@ def main():
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
         "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
-        "traffic": measured_traffic(args.precision, n, ng),
+        "traffic": measured_traffic(args.precision, n, ng,
+                                    "spin" if spin_blocked else "dense"),
         "kernel": "%s (the six O(n^6) products of the residual)"
                   % ("sbgemm_kernel" if args.precision == "bf16x3"
                      else "gemm_mfma_kernel<%s>" % ("double" if args.precision == "fp64" else "float")),
