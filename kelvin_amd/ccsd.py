@@ -150,6 +150,9 @@ class ccsd(object):
             return (numpy.array([-(e1p - e1m), -(ccp - ccm)]) / (2 * h),
                     numpy.array([-(e1f - e1b), -(ccf - ccb)]) / (2 * h))
 
+        # the neighbouring solves each hold a full engine: release this one
+        # (its converged amplitudes stay for the warm starts)
+        self.engine = None
         # the base point itself at the finite-difference convergence
         e1c, gcc = parts(T, mu)
         n_h, s_h = derivs(delta)

@@ -656,10 +656,12 @@ void ensure_workspace(ftcc_ctx *c) {
   const int64_t per_slot =
       8 * (10 * S2 + OO * OO + VV * VV + OO * VO + 8 * sm) +
       4 * 7 * (S2 + 64 * std::max<int64_t>(VV, std::max<int64_t>(VO, OO)));
-  double budget_gb = 24.0;
+  double budget_gb = 48.0;
   if (const char *env = getenv("FTCC_WORKSPACE_GB")) budget_gb = atof(env);
   const int64_t budget = (int64_t)(budget_gb * (1LL << 30));
-  const int64_t want = std::max<int64_t>(1, (1LL << 24) / std::max<int64_t>(1, S2));
+  // about 2^24 doubles per chunk, but at least two grid points when the
+  // budget allows (more tiles per product launch at large n)
+  const int64_t want = std::max<int64_t>(2, (1LL << 24) / std::max<int64_t>(1, S2));
   const int nb = (int)std::min<int64_t>(
       ng, std::max<int64_t>(1, std::min<int64_t>(want, budget / per_slot)));
   c->nb = nb;
