@@ -144,6 +144,21 @@ int ftcc_step(ftcc_ctx *ctx, double *T1old_dev, double *T2old_dev,
               double *T1bar_dev, double *T2bar_dev, const double *g_host,
               double beta, double damp, uint64_t skip_mask, double out5[5]);
 
+/* Pointwise schedule, rt_iter='point' (kelvin/cc_utils.py:176-242,
+ * quadrature.py:348-369).  Amplitudes T1/T2 and residuals T1bar/T2bar
+ * have ng slices.  ftcc_point_begin prepares grid point ig: ig = 0 stores
+ * the driving terms -F.vo, -I.vvoo as Tbar[0]; ig = 1 stores them as
+ * Tbar[1] and sets T[1] to their single-point integral; ig >= 2 sets T[ig]
+ * by linear extrapolation from T[ig-1] and T[ig-2]. */
+int ftcc_point_begin(ftcc_ctx *ctx, int ig, double *T1_dev, double *T2_dev,
+                     double *T1bar_dev, double *T2bar_dev);
+/* One iteration at point ig >= 1: Tbar[ig] = residual(T[ig]);
+ * T = sum_x G[ig,x] exp((t_x - t_ig) D) Tbar[x]; out2 = {|T1 - T1[ig]|^2,
+ * |T2 - T2[ig]|^2}; T[ig] <- damp*T[ig] + (1-damp)*T. */
+int ftcc_point_step(ftcc_ctx *ctx, int ig, double *T1_dev, double *T2_dev,
+                    double *T1bar_dev, double *T2bar_dev, double damp,
+                    double out2[2]);
+
 /* Device bytes held by the context (integrals + workspace). */
 int ftcc_workspace_bytes(const ftcc_ctx *ctx, int64_t *bytes);
 

@@ -45,6 +45,10 @@ SIGNATURES = {
     "ftcc_step": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _c_double_p,
                                  ctypes.c_double, ctypes.c_double,
                                  ctypes.c_uint64, _c_double_p]),
+    "ftcc_point_begin": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp,
+                                        _vp]),
+    "ftcc_point_step": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                       ctypes.c_double, _c_double_p]),
     "ftcc_set_spin": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int)]),
     "ftcc_get_spin": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),

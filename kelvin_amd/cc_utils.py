@@ -119,6 +119,43 @@ def ft_cc_iter(method, T1old, T2old, F, I, D1, D2, g, G, beta, ng, ti,
     return Eold, T1old, T2old
 
 
+def ft_cc_iter_extrap(method, F, I, D1, D2, g, G, beta, ng, ti,
+                      iprint, conv_options):
+    """Pointwise schedule (kelvin/cc_utils.py:176-242): converge the grid
+    points one at a time, each started from the integrated driving terms
+    (point 1) or a linear extrapolation of the previous two points, with
+    damping and the RMS-change test res1 + res2 < tconv.  Returns device
+    tensors (T1, T2)."""
+    if method != "CCSD":
+        return _other_method(method)
+    eng = ft_cc_equations.engine_of(F, I, D1, D2, ng)
+    eng.set_grid(ti, G)
+    thresh = conv_options["tconv"]
+    max_iter = conv_options["max_iter"]
+    alpha = conv_options["damp"]
+    T1, T2 = eng.empty_amplitudes()
+    T1.zero_()
+    T2.zero_()
+    T1b = torch.zeros_like(T1)
+    T2b = torch.zeros_like(T2)
+    nl1 = math.sqrt(float(eng.nv * eng.no))
+    nl2 = math.sqrt(float(eng.nv * eng.nv * eng.no * eng.no))
+    for ig in range(ng):
+        eng.point_begin(ig, T1, T2, T1b, T2b)
+        if ig == 0:
+            continue
+        logging.info("Time point {}".format(ig))
+        i, converged = 0, False
+        while i < max_iter and not converged:
+            d1, d2 = eng.point_step(ig, T1, T2, T1b, T2b, alpha)
+            res = math.sqrt(d1) / nl1 + math.sqrt(d2) / nl2
+            logging.info(' %2d  %.4E' % (i + 1, res))
+            i += 1
+            converged = res < thresh
+    del T1b, T2b
+    return T1, T2
+
+
 def _other_method(method):
     if method in ("CCD", "LCCSD", "LCCD"):
         raise NotImplementedError(

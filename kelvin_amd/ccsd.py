@@ -85,9 +85,6 @@ class ccsd(object):
                      % ft_utils.HtoK(self.T))
         if not self.singles:
             raise NotImplementedError("CCD is not built for the GPU path")
-        if self.rt_iter[0] != 'a' and T2 is None:
-            raise NotImplementedError(
-                "rt_iter='point' is not built for the GPU path")
         return self._ft_ccsd(T1in=T1, T2in=T2)
 
     def compute_ESN(self, L1=None, L2=None, gderiv=True, delta=1e-4,
@@ -241,6 +238,31 @@ class ccsd(object):
         F, I = engine_fock(eng), engine_eri(eng)
         D1 = EngineDenominators(eng, 1)
         D2 = EngineDenominators(eng, 2)
+        conv = {"econv": self.econv, "tconv": self.tconv,
+                "max_iter": self.max_iter, "damp": self.damp}
+        if self.rt_iter[0] != 'a' and T2in is None:
+            # pointwise schedule (kelvin/ccsd.py:697-700)
+            self.history = []
+            T1, T2 = cc_utils.ft_cc_iter_extrap(
+                "CCSD", F, I, D1, D2, g, G, self.beta_max, ng, ti,
+                self.iprint, conv)
+            if prec0 != "fp64" and self.refine:
+                logging.info("FP64 refinement after %s iterations" % prec0)
+                eng.set_precision("fp64")
+                conv_r = dict(conv)
+                if self.refine_max_iter is not None:
+                    conv_r["max_iter"] = self.refine_max_iter
+                Eccn, T1, T2 = cc_utils.ft_cc_iter(
+                    "CCSD", T1, T2, F, I, D1, D2, g, G, self.beta_max, ng,
+                    ti, self.iprint, conv_r, history=self.history)
+            Eccn = eng.energy(T1, T2, g, self.beta_max, qterm=True)
+            self.T1, self.T2 = T1, T2
+            self.G0, self.G1, self.Gcc = E0, E1, Eccn
+            self.Gtot = E0 + E1 + Eccn
+            return (Eccn + E01, Eccn)
+        if self.rt_iter[0] != 'a':
+            logging.warning("Converngece scheme ({}) is being ignored."
+                            .format(self.rt_iter))
         if T1in is not None and T2in is not None:
             T1old = eng.to_device(T1in).clone()
             T2old = eng.to_device(T2in).clone()
@@ -251,8 +273,6 @@ class ccsd(object):
         E2 = eng.energy(T1old, T2old, g, self.beta_max, qterm=False)
         logging.info('MP2 Energy: {:.10f}'.format(E2))
         self.mp2 = E2
-        conv = {"econv": self.econv, "tconv": self.tconv,
-                "max_iter": self.max_iter, "damp": self.damp}
         self.history = []
         Eccn, T1, T2 = cc_utils.ft_cc_iter(
             "CCSD", T1old, T2old, F, I, D1, D2, g, G, self.beta_max, ng, ti,

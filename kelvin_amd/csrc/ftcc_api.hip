@@ -918,16 +918,16 @@ void upload_g(ftcc_ctx *c, const double *g) {
 // true when all ng slices of T1 [ng][V][O] and T2 [ng][V][V][O][O] conserve
 // spin under the labels (the blocked products rely on it)
 bool amplitudes_conserve_spin(ftcc_ctx *c, const double *T1,
-                              const double *T2) {
+                              const double *T2, int nslices) {
   const int O = c->no, V = c->nv;
   hipStream_t s = c->stream;
   double bad = 0.0;
-  int nbk = spin_violations(T1, V, 1, O, 1, c->ng, c->d_lab_v, nullptr,
+  int nbk = spin_violations(T1, V, 1, O, 1, nslices, c->d_lab_v, nullptr,
                             c->d_lab_o, nullptr, c->partials, s);
   reduce_partials(c->partials, nbk, c->red, s);
   read_scalars(c, 3);
   bad += c->host_red[2];
-  nbk = spin_violations(T2, V, V, O, O, c->ng, c->d_lab_v, c->d_lab_v,
+  nbk = spin_violations(T2, V, V, O, O, nslices, c->d_lab_v, c->d_lab_v,
                         c->d_lab_o, c->d_lab_o, c->partials, s);
   reduce_partials(c->partials, nbk, c->red, s);
   read_scalars(c, 3);
@@ -941,7 +941,7 @@ void residual_all(ftcc_ctx *c, const double *T1, const double *T2,
   // spin blocks are exact only for spin-conserving amplitudes (every CC
   // iterate is); anything else runs dense
   const int spin_saved = c->spin;
-  if (c->spin && !amplitudes_conserve_spin(c, T1, T2)) c->spin = 0;
+  if (c->spin && !amplitudes_conserve_spin(c, T1, T2, c->ng)) c->spin = 0;
   struct Restore {
     ftcc_ctx *c;
     int v;
@@ -1356,6 +1356,81 @@ int ftcc_integrate(ftcc_ctx *c, double *T1, double *T2) {
       FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
       FTCC_CHECK_HIP(hipFree(copy));
     }
+  });
+}
+
+int ftcc_point_begin(ftcc_ctx *c, int ig, double *T1, double *T2,
+                     double *T1b, double *T2b) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(T1 && T2 && T1b && T2b, "null amplitude pointer");
+    require(ig >= 0 && ig < c->ng, "grid point out of range");
+    const int64_t VO = (int64_t)c->nv * c->no, S2 = VO * VO;
+    hipStream_t s = c->stream;
+    if (ig <= 1) {
+      // Tbar[ig] = driving terms; T[1] = their single-point integral
+      bcast(T1b + ig * VO, c->F[2], VO, 1, -1.0, 0.0, s);
+      bcast(T2b + ig * S2, c->I[3], S2, 1, -1.0, 0.0, s);
+      if (ig == 1) {
+        UpdateArgs a = upd(c);
+        a.tbar = T1b;
+        a.told = T1 + VO;
+        a.D = c->have_D ? c->D1 : nullptr;
+        integrate_point(a, false, 1, 0, s);
+        a.tbar = T2b;
+        a.told = T2 + S2;
+        a.D = c->have_D ? c->D2 : nullptr;
+        integrate_point(a, true, 1, 0, s);
+      }
+    } else {
+      // linear extrapolation from the two previous points
+      const double *ti = c->grid.ti.data();
+      const double f = (ti[ig] - ti[ig - 1]) / (ti[ig - 2] - ti[ig - 1]);
+      axpby(T1 + ig * VO, T1 + (ig - 1) * VO, VO, 1.0 - f, 0.0, s);
+      axpby(T1 + ig * VO, T1 + (ig - 2) * VO, VO, f, 1.0, s);
+      axpby(T2 + ig * S2, T2 + (ig - 1) * S2, S2, 1.0 - f, 0.0, s);
+      axpby(T2 + ig * S2, T2 + (ig - 2) * S2, S2, f, 1.0, s);
+    }
+    FTCC_CHECK_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int ftcc_point_step(ftcc_ctx *c, int ig, double *T1, double *T2, double *T1b,
+                    double *T2b, double damp, double out2[2]) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(T1 && T2 && T1b && T2b && out2, "null pointer");
+    require(ig >= 1 && ig < c->ng, "grid point out of range");
+    ensure_workspace(c);
+    const int64_t VO = (int64_t)c->nv * c->no, S2 = VO * VO;
+    hipStream_t s = c->stream;
+    // residual at point ig alone (spin blocks if its amplitudes allow)
+    const int spin_saved = c->spin;
+    if (c->spin &&
+        !amplitudes_conserve_spin(c, T1 + ig * VO, T2 + ig * S2, 1))
+      c->spin = 0;
+    struct Restore {
+      ftcc_ctx *c;
+      int v;
+      ~Restore() { c->spin = v; }
+    } restore{c, spin_saved};
+    residual_chunk(c, T1 + ig * VO, T2 + ig * S2, T1b + ig * VO,
+                   T2b + ig * S2, 1);
+    UpdateArgs a = upd(c);
+    a.damp = damp;
+    a.tbar = T1b;
+    a.told = T1 + ig * VO;
+    a.D = c->have_D ? c->D1 : nullptr;
+    int nbk = integrate_point(a, false, ig, 1, s);
+    reduce_partials(c->partials, nbk, c->red, s);
+    a.tbar = T2b;
+    a.told = T2 + ig * S2;
+    a.D = c->have_D ? c->D2 : nullptr;
+    nbk = integrate_point(a, true, ig, 1, s);
+    reduce_partials(c->partials, nbk, c->red + 3, s);
+    read_scalars(c, 6);
+    out2[0] = c->host_red[0];
+    out2[1] = c->host_red[3];
   });
 }
 

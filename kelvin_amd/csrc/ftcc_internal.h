@@ -226,6 +226,11 @@ struct UpdateArgs {
                                 // (F.ov^T for T1, oovv^T for T2)
   double *partials;             // per-block {d, n, e} triples
 };
+// One grid point y of the integral from the ng slices of a.tbar into the
+// single slice a.told: mode 0 stores it, mode 1 damps into it and leaves
+// |T - told|^2 in the first partial of each block (returns #blocks)
+int integrate_point(const UpdateArgs &a, bool doubles, int y, int mode,
+                    hipStream_t s);
 int update_t1(const UpdateArgs &a, int mode, hipStream_t s);   // returns #blocks
 int update_t2(const UpdateArgs &a, int mode, hipStream_t s);
 // true when in-place integration needs a source copy in UpdateArgs::src

@@ -277,6 +277,60 @@ int launch_update(const UpdateArgs &a, int mode, hipStream_t s) {
   return blocks;
 }
 
+// Single grid point y of the causal integral (the pointwise schedule):
+//   T(tau_y) = sum_x G[y,x] exp((tau_x - tau_y) d) Tbar(tau_x)  (x < y;
+//   weight 1 for x >= y), reading the ng slices of Tbar.
+// mode 0 stores T into told (the slice of point y); mode 1 accumulates
+// |T - told|^2 into the first partial and damps told = damp*told +
+// (1-damp)*T (kelvin/cc_utils.py:226-233).
+template <bool DOUBLES>
+__global__ __launch_bounds__(UT) void point_kernel(UpdateArgs p, int y,
+                                                   int mode) {
+  extern __shared__ double grow[];   // G[y][0..ng), ti[0..ng)
+  const int ng = p.ng, nv = p.nv, no = p.no;
+  for (int k = threadIdx.x; k < ng; k += UT) {
+    grow[k] = p.G[(int64_t)y * ng + k];
+    grow[ng + k] = p.ti[k];
+  }
+  __syncthreads();
+  const double *gy = grow, *tis = grow + ng;
+  const int64_t per = DOUBLES ? (int64_t)nv * nv * no * no : (int64_t)nv * no;
+  double dsum = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
+       e += (int64_t)gridDim.x * UT) {
+    const Elem q = decode<DOUBLES>(e, nv, no);
+    double d;
+    if (p.D)
+      d = p.D[e];
+    else if (DOUBLES)
+      d = p.ev[q.a] + p.ev[q.b] - p.eo[q.i] - p.eo[q.j];
+    else
+      d = p.ev[q.a] - p.eo[q.i];
+    const double r = p.uniform ? exp(-p.delta * d) : 0.0;
+    double t = 0.0, w = 1.0;
+    for (int x = y - 1; x >= 0; --x) {
+      if (p.uniform) w *= r;
+      const double gx = gy[x];
+      if (gx == 0.0) continue;
+      const double wx = p.uniform ? w : exp((tis[x] - tis[y]) * d);
+      t += gx * (wx * p.tbar[x * per + e]);
+    }
+    for (int x = y; x < ng; ++x) {
+      const double gx = gy[x];
+      if (gx != 0.0) t += gx * p.tbar[x * per + e];
+    }
+    if (mode == 0) {
+      p.told[e] = t;
+    } else {
+      const double to = p.told[e];
+      const double df = t - to;
+      dsum += df * df;
+      p.told[e] = p.damp * to + (1.0 - p.damp) * t;
+    }
+  }
+  if (mode == 1) block_reduce3(dsum, 0.0, 0.0, p.partials);
+}
+
 __global__ void reduce_partials_kernel(const double *partials, int nblocks,
                                        double *out3) {
   __shared__ double red[3][256];
@@ -450,6 +504,23 @@ int spin_violations(const double *a, int d0, int d1, int d2, int d3, int nb,
   const int blocks = nblocks_for((int64_t)d0 * d1 * d2 * d3 * nb);
   hipLaunchKernelGGL(spin_violations_kernel, dim3(blocks), dim3(UT), 0, s, a,
                      d0, d1, d2, d3, nb, l0, l1, l2, l3, partials);
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
+int integrate_point(const UpdateArgs &a, bool doubles, int y, int mode,
+                    hipStream_t s) {
+  const int64_t per = doubles ? (int64_t)a.nv * a.nv * a.no * a.no
+                              : (int64_t)a.nv * a.no;
+  const int blocks =
+      (int)std::max<int64_t>(1, std::min<int64_t>((per + UT - 1) / UT, MAX_BLOCKS));
+  const size_t lds = sizeof(double) * 2 * a.ng;
+  if (doubles)
+    hipLaunchKernelGGL(point_kernel<true>, dim3(blocks), dim3(UT), lds, s, a,
+                       y, mode);
+  else
+    hipLaunchKernelGGL(point_kernel<false>, dim3(blocks), dim3(UT), lds, s,
+                       a, y, mode);
   FTCC_CHECK_HIP(hipGetLastError());
   return blocks;
 }

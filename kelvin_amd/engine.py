@@ -248,6 +248,28 @@ class FTEngine:
                    out)
         return tuple(out)
 
+    def _check_amps(self, T1, T2, T1b, T2b):
+        for t, s, nm in ((T1, self.t1_shape(), "T1"),
+                         (T2, self.t2_shape(), "T2"),
+                         (T1b, self.t1_shape(), "T1bar"),
+                         (T2b, self.t2_shape(), "T2bar")):
+            self._check(t, s, nm)
+
+    def point_begin(self, ig, T1, T2, T1b, T2b):
+        """Prepare grid point ig of the pointwise schedule (driving terms,
+        first-point integral or linear extrapolation)."""
+        self._check_amps(T1, T2, T1b, T2b)
+        self._call("ftcc_point_begin", int(ig), _vp(T1), _vp(T2), _vp(T1b),
+                   _vp(T2b))
+
+    def point_step(self, ig, T1, T2, T1b, T2b, damp):
+        """One iteration at grid point ig; returns (|dT1|^2, |dT2|^2)."""
+        self._check_amps(T1, T2, T1b, T2b)
+        out = (ctypes.c_double * 2)()
+        self._call("ftcc_point_step", int(ig), _vp(T1), _vp(T2), _vp(T1b),
+                   _vp(T2b), float(damp), out)
+        return out[0], out[1]
+
     def profile(self, on=True):
         """Bracket every O(n^6) product launch with HIP events."""
         self._call("ftcc_profile_enable", int(bool(on)))

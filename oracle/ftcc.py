@@ -445,9 +445,66 @@ def ft_cc_iter(T1old, T2old, F, I, D1, D2, g, G, beta, ng, ti, conv,
     return Eold, T1old, T2old, converged, i
 
 
+def int_tbar_single(ng, ig, tbar, ti, D, G):
+    """kelvin/quadrature.py:348-369 (int_tbar1_single / int_tbar2_single):
+    the causal integral at point ig alone, dt = t_x - t_ig for x < ig."""
+    dt = numpy.zeros(ng)
+    for x in range(ig):
+        dt[x] = ti[x] - ti[ig]
+    shp = (ng,) + (1,) * D.ndim
+    gtemp = numpy.exp(dt.reshape(shp) * D[None])
+    return numpy.einsum('x,x...->...', G[ig], gtemp * tbar)
+
+
+def ft_cc_iter_extrap(F, I, D1, D2, g, G, beta, ng, ti, conv):
+    """kelvin/cc_utils.py:176-242: converge the grid points one at a time.
+    Point 0 is never iterated; point 1 starts from the integrated driving
+    terms, later points from a linear extrapolation of the previous two;
+    each point iterates residual -> single-point integral -> damping until
+    the RMS change of T1 plus that of T2 is below tconv."""
+    thresh, max_iter, alpha = conv["tconv"], conv["max_iter"], conv["damp"]
+    nv, no = F.vo.shape
+    t1bar = numpy.zeros((ng, nv, no))
+    t2bar = numpy.zeros((ng, nv, nv, no, no))
+    T1new = numpy.zeros((ng, nv, no))
+    T2new = numpy.zeros((ng, nv, nv, no, no))
+    for ig in range(ng):
+        if ig == 0:
+            t1bar[0] = -F.vo
+            t2bar[0] = -I.vvoo
+            continue
+        elif ig == 1:
+            t1bar[ig] = -F.vo
+            t2bar[ig] = -I.vvoo
+            T1new[ig] = int_tbar_single(ng, ig, t1bar, ti, D1, G)
+            T2new[ig] = int_tbar_single(ng, ig, t2bar, ti, D2, G)
+        else:
+            c = (ti[ig] - ti[ig - 1]) / (ti[ig - 2] - ti[ig - 1])
+            T1new[ig] = T1new[ig - 1] + (T1new[ig - 2] - T1new[ig - 1]) * c
+            T2new[ig] = T2new[ig - 1] + (T2new[ig - 2] - T2new[ig - 1]) * c
+        nl1 = numpy.sqrt(float(T1new[ig].size))
+        nl2 = numpy.sqrt(float(T2new[ig].size))
+        i, converged = 0, False
+        while i < max_iter and not converged:
+            r1, r2 = stanton_rhs(F, I, T1new[ig], T2new[ig])
+            t1bar[ig] = -r1            # stanton_rhs includes f_ai, <ab||ij>
+            t2bar[ig] = -r2
+            T1 = int_tbar_single(ng, ig, t1bar, ti, D1, G)
+            T2 = int_tbar_single(ng, ig, t2bar, ti, D2, G)
+            res1 = numpy.linalg.norm(T1 - T1new[ig]) / nl1
+            res2 = numpy.linalg.norm(T2 - T2new[ig]) / nl2
+            T1new[ig] = alpha * T1new[ig] + (1.0 - alpha) * T1
+            T2new[ig] = alpha * T2new[ig] + (1.0 - alpha) * T2
+            i += 1
+            converged = res1 + res2 < thresh
+    return T1new, T2new
+
+
 def ft_ccsd(sys, T, mu, ngrid=10, damp=0.0, max_iter=40, econv=1e-8,
-            tconv=None, quad='lin', athresh=0.0, history=None):
-    """kelvin/ccsd.py:43-76 (construction) + :595-712 (_ft_ccsd, 'g' path).
+            tconv=None, quad='lin', athresh=0.0, history=None,
+            rt_iter="all"):
+    """kelvin/ccsd.py:43-76 (construction) + :595-712 (_ft_ccsd, 'g' path;
+    rt_iter='point' takes the pointwise schedule of :697-700).
 
     Returns a dict with Omega (total), Omega_cc, G0, G1, mp2, T1, T2, niter.
     """
@@ -474,6 +531,13 @@ def ft_ccsd(sys, T, mu, ngrid=10, damp=0.0, max_iter=40, econv=1e-8,
         F, I = ft_integrals(fock, eri, en, beta, mu)
         d1, d2 = D1(en, en), D2(en, en)
     nv, no = F.vo.shape
+    if rt_iter[0] != 'a':                                         # :697-700
+        conv = {"econv": econv, "tconv": tconv, "max_iter": max_iter,
+                "damp": damp}
+        T1, T2 = ft_cc_iter_extrap(F, I, d1, d2, g, G, beta, ng, ti, conv)
+        Ecc = ft_cc_energy(T1, T2, F.ov, I.oovv, g, beta)
+        return dict(Omega=Ecc + G0 + G1, Omega_cc=Ecc, G0=G0, G1=G1,
+                    mp2=None, T1=T1, T2=T2, niter=None, converged=None, n=n)
     # MP2-level guess: kelvin/ccsd.py:680-688
     T1old = -numpy.broadcast_to(F.vo, (ng, nv, no)).copy()
     T2old = -numpy.broadcast_to(I.vvoo, (ng, nv, nv, no, no)).copy()

@@ -523,3 +523,27 @@ def test_spin_detected_for_ueg_and_dense_for_generic():
     cc = ccsd(sysa, T=0.5, mu=0.0, ngrid=4, max_iter=2)
     cc.run()
     assert not cc.engine.spin_blocked()
+
+
+# ------------------------------------------------------------ rt_iter='point'
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_pointwise_schedule(precision):
+    """rt_iter='point' (kelvin/cc_utils.py:176-242) on the G1 system: equal
+    to the oracle's pointwise restatement, and to the all-at-once result
+    within 1e-8 (kelvin/tests/test_ft_ccsd.py:167-184)."""
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    sysg, s = _ueg_gold("G1")
+    kw = dict(ngrid=10, damp=0.2, max_iter=50, tconv=1e-8)
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], rt_iter="point",
+              precision=precision, **kw)
+    Etot, Ecc = cc.run()
+    osys = oracle.UEGSystem(s["T"], 2 * numpy.pi, s["Emax"], s["mu"],
+                            norb=s["norb"])
+    ref = oracle.ft_ccsd(osys, s["T"], s["mu"], rt_iter='point', **kw)
+    tol = 1e-10 if precision == "fp64" else 1e-8
+    assert abs(Ecc - ref["Omega_cc"]) < tol
+    assert abs(Etot - ref["Omega"]) < tol
+    cc_all = ccsd(sysg, T=s["T"], mu=s["mu"], **kw)
+    Etot_all, Ecc_all = cc_all.run()
+    assert abs(Ecc - Ecc_all) < 1e-8

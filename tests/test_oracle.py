@@ -264,3 +264,14 @@ def test_df_model_system_shapes():
     assert eri.shape == (16,) * 4
     assert numpy.abs(eri + eri.transpose(1, 0, 2, 3)).max() < 1e-15
     assert numpy.abs(eri - eri.transpose(2, 3, 0, 1)).max() < 1e-15
+
+
+def test_pointwise_schedule_reaches_the_same_fixed_point():
+    """kelvin/tests/test_ft_ccsd.py:167-184: rt_iter='point' and the
+    all-at-once iteration give the same Omega_cc within 1e-8."""
+    u = ftcc.UEGSystem(0.1, 2 * numpy.pi, 1.2, 0.1, norb=7)
+    kw = dict(ngrid=10, damp=0.2, max_iter=50, tconv=1e-8)
+    r1 = ftcc.ft_ccsd(u, 0.1, 0.1, **kw)
+    r2 = ftcc.ft_ccsd(u, 0.1, 0.1, rt_iter='point', **kw)
+    assert abs(r1["Omega_cc"] - r2["Omega_cc"]) < 1e-8
+    assert abs(r2["Omega_cc"] - GOLD["G1"]["Omega_cc"]) < 1e-8
