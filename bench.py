@@ -165,17 +165,15 @@ def main():
     en = sysg.g_energies_tot()
     n = en.shape[0]
     beta, mu = 1.0 / sysg.T, sysg.mu
-    eri = sysg.g_aint_tot()
-    fock = sysg.g_fock_tot()
-    e1 = sysg.get_mp1()
-    log("UEG nk=%d n=%d beta=%.4f mu=%.6f built in %.1f s"
-        % (args.nk, n, beta, mu, time.time() - t0))
     ng = args.ng
     ti, g, G = quadrature.ft_quad(ng, beta, 'lin')
     eng = FTEngine(n, n, ng, precision=args.precision)
+    fock, eri, e1 = sysg.g_device_integrals(eng)     # built in HBM
+    log("UEG nk=%d n=%d beta=%.4f mu=%.6f, integrals built on the device "
+        "in %.1f s" % (args.nk, n, beta, mu, time.time() - t0))
     omega0 = eng.set_system(en, fock, eri, beta, mu)
     eng.set_grid(ti, G)
-    del eri
+    del eri, fock
     T1, T2 = eng.empty_amplitudes()
     T1b, T2b = eng.empty_amplitudes()
     eng.driving_terms(T1, T2)

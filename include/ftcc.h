@@ -83,12 +83,30 @@ int ftcc_set_blocks(ftcc_ctx *ctx, const double *const F_dev[4],
  * NULL switches blocking off.  ftcc_set_system tries the labels "first
  * half alpha, second half beta" of the n orbitals by itself (environment
  * FTCC_SPIN=0 prevents it); ftcc_set_blocks switches blocking off.
- * Amplitudes are assumed to conserve spin like the integrals (every
- * iterate started from the driving terms does). */
+ * The blocks are exact for amplitudes that conserve spin too (every
+ * iterate started from the driving terms does): each residual call counts
+ * the amplitude entries that do not and runs dense if there are any. */
 int ftcc_set_spin(ftcc_ctx *ctx, const int *lab_o_host,
                   const int *lab_v_host);
 /* *on = 1 when the products run spin-blocked. */
 int ftcc_get_spin(ftcc_ctx *ctx, int *on);
+
+/* Uniform-electron-gas integrals built on the device (the provider of
+ * kelvin/ueg_utils.py:83-138 and kelvin/ueg_system.py:400-419): nk plane
+ * waves with integer vectors ivec_host[nk][3] and wave vectors
+ * kvec_host[nk][3] = 2 pi/L * ivec in a box of side L.  Writes the
+ * antisymmetrised spin-orbital <pq||rs> (n = 2 nk; alpha orbitals, then
+ * beta) into eri_dev[n^4], bitwise equal to the host construction. */
+int ftcc_ueg_eri(ftcc_ctx *ctx, int nk, const int *ivec_host,
+                 const double *kvec_host, double L, double *eri_dev);
+
+/* Mean-field quantities of a system whose integrals are on the device:
+ * fock_dev[n,n] = diag(en) + sum_r <pr||qr> fo[r] (kelvin/ueg_system.py:
+ * 400-419) and *e1 = 1/2 sum_ij <ij||ij> fo_i fo_j (kelvin/ueg_system.py:
+ * 126-131).  Either output may be NULL. */
+int ftcc_mean_field(ftcc_ctx *ctx, int n, const double *en_dev,
+                    const double *eri_dev, const double *fo_dev,
+                    double *fock_dev, double *e1);
 
 /* Explicit denominators D1_dev[nv,no], D2_dev[nv,nv,no,no] (the arguments
  * D1, D2 of kelvin/ft_cc_equations.py:96), copied. */

@@ -49,6 +49,11 @@ SIGNATURES = {
                                         _vp]),
     "ftcc_point_step": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
                                        ctypes.c_double, _c_double_p]),
+    "ftcc_ueg_eri": (ctypes.c_int, [_vp, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int), _c_double_p,
+                                    ctypes.c_double, _vp]),
+    "ftcc_mean_field": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                       _c_double_p]),
     "ftcc_set_spin": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int)]),
     "ftcc_get_spin": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),

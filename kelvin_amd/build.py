@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libftcc.so")
 ARCH = os.environ.get("FTCC_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["ftcc_api.hip", "gemm.hip", "sbgemm.hip", "tensor_ops.hip",
-           "update.hip"]
+           "update.hip", "ueg.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -227,10 +227,15 @@ class ccsd(object):
         eng = FTEngine(no, nv, ng, precision=prec0)
         self.engine = eng
         t0 = time.time()
-        G0 = eng.set_system(en, self.sys.g_fock_tot(), self.sys.g_aint_tot(),
-                            beta, mu, iocc=iocc, ivir=ivir)
+        if hasattr(self.sys, "g_device_integrals"):
+            # integrals built in HBM (the UEG provider)
+            fock, eri, E1 = self.sys.g_device_integrals(eng)
+        else:
+            fock, eri = self.sys.g_fock_tot(), self.sys.g_aint_tot()
+            E1 = self.sys.get_mp1()
+        G0 = eng.set_system(en, fock, eri, beta, mu, iocc=iocc, ivir=ivir)
+        del fock, eri
         E0 = G0 + En
-        E1 = self.sys.get_mp1()
         E01 = E0 + E1
         eng.set_grid(ti, G)
         logging.info("integrals on device: %.3f s (%.2f GB held)"

@@ -1434,6 +1434,45 @@ int ftcc_point_step(ftcc_ctx *c, int ig, double *T1, double *T2, double *T1b,
   });
 }
 
+int ftcc_ueg_eri(ftcc_ctx *c, int nk, const int *ivec, const double *kvec,
+                 double L, double *eri) {
+  return guarded(c, [&] {
+    require(nk > 0 && ivec && kvec && eri && L > 0.0, "bad UEG arguments");
+    int *dX = nullptr;
+    double *dk = nullptr;
+    FTCC_CHECK_HIP(hipMalloc(&dX, sizeof(int) * 3 * nk));
+    FTCC_CHECK_HIP(hipMalloc(&dk, sizeof(double) * 3 * nk));
+    hipStream_t s = c->stream;
+    FTCC_CHECK_HIP(hipMemcpyAsync(dX, ivec, sizeof(int) * 3 * nk,
+                                  hipMemcpyHostToDevice, s));
+    FTCC_CHECK_HIP(hipMemcpyAsync(dk, kvec, sizeof(double) * 3 * nk,
+                                  hipMemcpyHostToDevice, s));
+    const double aaa = 4.0 * M_PI / (L * L * L);
+    ueg_eri(nk, dX, dk, aaa, eri, s);
+    const hipError_t e = hipStreamSynchronize(s);
+    (void)hipFree(dX);
+    (void)hipFree(dk);
+    FTCC_CHECK_HIP(e);
+  });
+}
+
+int ftcc_mean_field(ftcc_ctx *c, int n, const double *en, const double *eri,
+                    const double *fo, double *fock, double *e1) {
+  return guarded(c, [&] {
+    require(n > 0 && en && eri && fo, "bad mean-field arguments");
+    hipStream_t s = c->stream;
+    if (fock) mean_field_fock(n, en, eri, fo, fock, s);
+    if (e1) {
+      const int nbk = mp1_partials(n, eri, fo, c->partials, s);
+      reduce_partials(c->partials, nbk, c->red, s);
+      read_scalars(c, 1);
+      *e1 = 0.5 * c->host_red[0];
+    } else {
+      FTCC_CHECK_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
 int ftcc_energy(ftcc_ctx *c, const double *T1, const double *T2,
                 const double *g, double beta, int qterm, double *energy) {
   return guarded(c, [&] {

@@ -119,6 +119,20 @@ void sbgemm_grouped(int nprob, const SbProblem *prob, int nb, double alpha,
                     int64_t sCb, hipStream_t s);
 
 // ---------------------------------------------------------------------------
+// System integrals (ueg.hip)
+// ---------------------------------------------------------------------------
+// antisymmetrised spin-orbital UEG integrals, n = 2*nk (alpha, then beta);
+// X_dev: integer wave vectors [nk][3], k_dev: wave vectors [nk][3]
+void ueg_eri(int nk, const int *X_dev, const double *k_dev, double aaa,
+             double *eri, hipStream_t s);
+// fock = diag(en) + sum_r <pr||qr> fo[r]
+void mean_field_fock(int n, const double *en, const double *eri,
+                     const double *fo, double *fock, hipStream_t s);
+// partial triples whose first slots sum to sum_ij <ij||ij> fo_i fo_j
+int mp1_partials(int n, const double *eri, const double *fo, double *partials,
+                 hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // Elementwise / permutation / reduction kernels (tensor_ops.hip)
 // ---------------------------------------------------------------------------
 // out[i0*o0 + i1*o1 + i2*o2 + i3*o3] = alpha*in[i0,i1,i2,i3] + beta*out[...]

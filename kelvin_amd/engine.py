@@ -168,6 +168,35 @@ class FTEngine:
         self._call("ftcc_get_spin", ctypes.byref(on))
         return bool(on.value)
 
+    def ueg_eri(self, ivec, kvec, L):
+        """Antisymmetrised spin-orbital UEG integrals (alpha, then beta) of
+        the plane waves ivec[nk,3] (integers), kvec[nk,3] built in HBM."""
+        iv = numpy.ascontiguousarray(ivec, dtype=numpy.int32)
+        kv = numpy.ascontiguousarray(kvec, dtype=numpy.float64)
+        nk = iv.shape[0]
+        if iv.shape != (nk, 3) or kv.shape != (nk, 3):
+            raise ValueError("ivec and kvec need shape (nk, 3)")
+        n = 2 * nk
+        eri = torch.empty((n, n, n, n), dtype=torch.float64, device=self.device)
+        self._call("ftcc_ueg_eri", nk,
+                   iv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                   kv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                   float(L), _vp(eri))
+        return eri
+
+    def mean_field(self, en, eri, fo):
+        """(fock, E1): diag(en) + sum_r <pr||qr> fo_r and
+        1/2 sum_ij <ij||ij> fo_i fo_j from device integrals."""
+        en = self.to_device(en)
+        fo = self.to_device(fo)
+        n = en.shape[0]
+        self._check(eri, (n, n, n, n), "eri")
+        fock = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        e1 = ctypes.c_double()
+        self._call("ftcc_mean_field", n, _vp(en), _vp(eri), _vp(fo),
+                   _vp(fock), ctypes.byref(e1))
+        return fock, e1.value
+
     def set_denominators(self, D1, D2):
         D1 = self.to_device(D1)
         D2 = self.to_device(D2)

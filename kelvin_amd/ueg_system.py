@@ -174,6 +174,18 @@ class UEGSystem(System):
         JK = numpy.einsum('prqr,r->pq', self.g_aint_tot(), fo)
         return numpy.diag(d) + JK
 
+    def g_device_integrals(self, engine):
+        """(fock, eri, E1) built on the GPU by the engine's library: eri in
+        HBM, bitwise equal to g_aint_tot(); fock and E1 its mean field at
+        (T, mu) as g_fock_tot() and get_mp1() define them (equal to their
+        host values up to the order of the sums)."""
+        b = self.basis
+        eri = engine.ueg_eri(b._ivec, numpy.asarray(b.ks), self.L)
+        d = self.g_energies_tot()
+        fo = self._fo() if self.T > 0.0 else (d < self.mu).astype(float)
+        fock, e1 = engine.mean_field(d, eri, fo)
+        return fock, eri, e1
+
     def get_mp1(self):
         """1/2 sum_ij <ij||ij> fo_i fo_j (kelvin/ueg_system.py:126-131)."""
         if self.T > 0:

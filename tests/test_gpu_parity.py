@@ -579,3 +579,20 @@ def test_ueg66_sweep_mixed_refined(rs, t_over_tf):
     mix = ccsd(sysg, T=sysg.T, mu=sysg.mu, precision="bf16x3", **kw)
     Om, Occ = mix.run()
     assert abs(Om - Om64) < 1e-7, (Om, Om64)
+
+
+# ------------------------------------------------------- UEG integrals in HBM
+@pytest.mark.parametrize("norb", [7, 19])
+def test_ueg_integrals_built_on_device(norb):
+    """ftcc_ueg_eri reproduces the host construction bit for bit; the mean
+    field (Fock, E1) equals g_fock_tot / get_mp1 to rounding."""
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd.ueg_system import UEGSystem
+    dev()
+    sysg = UEGSystem(0.1, 2 * numpy.pi, 1.2, mu=0.1, norb=norb, orbtype='g')
+    n = sysg.g_energies_tot().shape[0]
+    eng = FTEngine(n, n, 2)
+    fock, eri, e1 = sysg.g_device_integrals(eng)
+    assert numpy.array_equal(to_np(eri), sysg.g_aint_tot())
+    assert numpy.abs(to_np(fock) - sysg.g_fock_tot()).max() < 1e-14
+    assert abs(e1 - sysg.get_mp1()) < 1e-14
