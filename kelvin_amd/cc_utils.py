@@ -156,6 +156,90 @@ def ft_cc_iter_extrap(method, F, I, D1, D2, g, G, beta, ng, ti,
     return T1, T2
 
 
+def uft_integrals(sys, ea, eb, beta, mu):
+    """Unrestricted Fermi-weighted blocks (kelvin/cc_utils.py:696-777):
+    (Fa, Fb, Ia, Ib, Iabab) as device tensors.  The spin-orbital Fock matrix
+    and antisymmetrised integrals are assembled from sys.u_fock_tot() and
+    sys.u_aint_tot(), weighted on the device like the spin-orbital path, and
+    the alpha/beta blocks are taken back out."""
+    from . import unrestricted
+    ea = numpy.asarray(ea, dtype=numpy.float64)
+    eb = numpy.asarray(eb, dtype=numpy.float64)
+    na, nb = ea.shape[0], eb.shape[0]
+    n = na + nb
+    fa, fb = sys.u_fock_tot()
+    eA, eB, eAB = sys.u_aint_tot()
+    fock = numpy.zeros((n, n))
+    fock[:na, :na] = fa
+    fock[na:, na:] = fb
+    eri = numpy.zeros((n, n, n, n))
+    a, b = slice(0, na), slice(na, n)
+    eri[a, a, a, a] = eA
+    eri[b, b, b, b] = eB
+    eri[a, b, a, b] = eAB
+    eri[a, b, b, a] = -eAB.transpose(0, 1, 3, 2)
+    eri[b, a, a, b] = -eAB.transpose(1, 0, 2, 3)
+    eri[b, a, b, a] = eAB.transpose(1, 0, 3, 2)
+    eng = FTEngine(n, n, 1)
+    eng.set_system(numpy.concatenate([ea, eb]), fock, eri, beta, mu)
+    del eri
+    F, I = engine_fock(eng), engine_eri(eng)
+    lay = unrestricted.SpinLayout(na, nb, na, nb)
+    return unrestricted.unrestricted_blocks(lay, F, I)
+
+
+def ft_ucc_iter(method, T1aold, T1bold, T2aaold, T2abold, T2bbold, Fa, Fb,
+                Ia, Ib, Iabab, D1a, D1b, D2aa, D2ab, D2bb, g, G, beta, ng, ti,
+                iprint, conv_options):
+    """Unrestricted damped fixed point (kelvin/cc_utils.py:245-317), with
+    the reference's spin-block norms: nl1 = |T1a| + 0.1 + |T1b|,
+    nl2 = |T2aa| + 0.1 + |T2ab| + |T2bb|, res = sum of block changes over
+    those.  Returns (E, (T1a, T1b), (T2aa, T2ab, T2bb))."""
+    from . import unrestricted
+    if method != "CCSD":
+        return _other_method(method)
+    eng, lay = ft_cc_equations.unrestricted_engine(
+        Fa, Fb, Ia, Ib, Iabab, D1a, D1b, D2aa, D2ab, D2bb, ng)
+    eng.set_grid(ti, G)
+    T1, T2 = unrestricted.spin_orbital_amplitudes(
+        lay, T1aold, T1bold, T2aaold, T2abold, T2bbold, eng.device)
+    T1n = torch.empty_like(T1)
+    T2n = torch.empty_like(T2)
+    skip = zero_slices(eng, T1, T2, G)
+    ethresh = conv_options["econv"]
+    tthresh = conv_options["tconv"]
+    max_iter = conv_options["max_iter"]
+    alpha = conv_options["damp"]
+
+    def nrm(x):
+        return math.sqrt(eng.sumsq(x.contiguous()))
+
+    tbeg = time.time()
+    converged, i, Eold = False, 0, 888888888.888888888
+    while i < max_iter and not converged:
+        eng.residual(T1, T2, T1n, T2n, skip)
+        eng.integrate(T1n, T2n)
+        o1, o2 = unrestricted.unrestricted_amplitudes(lay, T1, T2)
+        n1, n2 = unrestricted.unrestricted_amplitudes(lay, T1n, T2n)
+        nl1 = nrm(o1[0]) + 0.1 + nrm(o1[1])
+        nl2 = nrm(o2[0]) + 0.1 + nrm(o2[1]) + nrm(o2[2])
+        res1 = sum(nrm(x - y) for x, y in zip(n1, o1)) / nl1
+        res2 = sum(nrm(x - y) for x, y in zip(n2, o2)) / nl2
+        T1.mul_(alpha).add_(T1n, alpha=1.0 - alpha)
+        T2.mul_(alpha).add_(T2n, alpha=1.0 - alpha)
+        E = eng.energy(T1, T2, g, beta, qterm=True)
+        logging.info(' %2d  %.10f   %.4E' % (i + 1, E, res1 + res2))
+        i += 1
+        if abs(E - Eold) < ethresh and res1 + res2 < tthresh:
+            converged = True
+        Eold = E
+    if not converged:
+        logging.warning("{} did not converge!".format(method))
+    logging.info("Total {} time: {:.4f} s".format(method, time.time() - tbeg))
+    T1u, T2u = unrestricted.unrestricted_amplitudes(lay, T1, T2)
+    return Eold, T1u, T2u
+
+
 def _other_method(method):
     if method in ("CCD", "LCCSD", "LCCD"):
         raise NotImplementedError(

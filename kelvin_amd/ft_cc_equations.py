@@ -7,6 +7,7 @@ the exp(-tau D) integration of kelvin/quadrature.py:292-317.
 """
 import torch
 
+from . import unrestricted
 from .engine import FTEngine
 from .ov_blocks import EngineBlocks, EngineDenominators
 
@@ -62,3 +63,35 @@ def ccsd_residual(F, I, T1, T2, D1=None, D2=None, ti=None, G=None):
     T2b = torch.empty_like(T2)
     eng.residual(T1, T2, T1b, T2b)
     return T1b, T2b
+
+
+def unrestricted_engine(Fa, Fb, Ia, Ib, Iabab, D1a, D1b, D2aa, D2ab, D2bb, ng):
+    """An engine holding the spin-orbital assembly of alpha/beta blocks, with
+    spin-blocked products (labels: alpha orbitals first in each role)."""
+    lay = unrestricted.layout_of(Fa, Fb)
+    eng = FTEngine(lay.size('o'), lay.size('v'), ng)
+    F, I = unrestricted.spin_orbital_blocks(lay, Fa, Fb, Ia, Ib, Iabab,
+                                            eng.device)
+    eng.set_blocks(F, I)
+    del F, I
+    D1, D2 = unrestricted.spin_orbital_denominators(lay, D1a, D1b, D2aa, D2ab,
+                                                    D2bb, eng.device)
+    eng.set_denominators(D1, D2)
+    eng.set_spin(lay.labels('o'), lay.labels('v'))
+    return eng, lay
+
+
+def uccsd_stanton(Fa, Fb, Ia, Ib, Iabab, T1aold, T1bold, T2aaold, T2abold,
+                  T2bbold, D1a, D1b, D2aa, D2ab, D2bb, ti, ng, G):
+    """New unrestricted amplitudes ((T1a, T1b), (T2aa, T2ab, T2bb)) as device
+    tensors (kelvin/ft_cc_equations.py:130-164)."""
+    eng, lay = unrestricted_engine(Fa, Fb, Ia, Ib, Iabab, D1a, D1b, D2aa,
+                                   D2ab, D2bb, ng)
+    eng.set_grid(ti, G)
+    T1, T2 = unrestricted.spin_orbital_amplitudes(
+        lay, T1aold, T1bold, T2aaold, T2abold, T2bbold, eng.device)
+    T1b = torch.empty_like(T1)
+    T2b = torch.empty_like(T2)
+    eng.residual(T1, T2, T1b, T2b)
+    eng.integrate(T1b, T2b)
+    return unrestricted.unrestricted_amplitudes(lay, T1b, T2b)

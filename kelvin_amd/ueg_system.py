@@ -174,6 +174,29 @@ class UEGSystem(System):
         JK = numpy.einsum('prqr,r->pq', self.g_aint_tot(), fo)
         return numpy.diag(d) + JK
 
+    def u_energies_tot(self):
+        """(ea, eb): the plane-wave energies for each spin."""
+        e = numpy.asarray(self.basis.Es)
+        return e, e.copy()
+
+    def u_fock_tot(self):
+        """(fa, fb): the spin blocks of g_fock_tot (the Fock matrix is spin
+        diagonal)."""
+        f = self.g_fock_tot()
+        nk = self.basis.get_nbsf()
+        return f[:nk, :nk].copy(), f[nk:, nk:].copy()
+
+    def u_aint_tot(self):
+        """(Va, Vb, Vabab): antisymmetrised same-spin integrals and the
+        alpha-beta <pq|rs> (whose exchange part vanishes by spin), the spin
+        blocks of g_aint_tot (kelvin/ueg_system.py:266-267 builds the same
+        from the spatial integrals)."""
+        V = self.g_aint_tot()
+        nk = self.basis.get_nbsf()
+        a, b = slice(0, nk), slice(nk, 2 * nk)
+        return (V[a, a, a, a].copy(), V[b, b, b, b].copy(),
+                V[a, b, a, b].copy())
+
     def g_device_integrals(self, engine):
         """(fock, eri, E1) built on the GPU by the engine's library: eri in
         HBM, bitwise equal to g_aint_tot(); fock and E1 its mean field at

@@ -104,3 +104,54 @@ def test_bench_refuses_multi_gpu():
                         "--gpus", "2"], capture_output=True, text=True)
     assert p.returncode != 0
     assert "one GPU" in (p.stdout + p.stderr)
+
+
+def test_unrestricted_assembly_round_trip():
+    """Alpha/beta blocks taken out of a spin-conserving antisymmetric
+    spin-orbital problem (unequal alpha/beta counts) and assembled again give
+    back every spin-orbital block and amplitude exactly (CPU tensors)."""
+    import numpy
+    import torch
+    from kelvin_amd import unrestricted
+    from oracle import fock_space as fs
+    rng = numpy.random.default_rng(4)
+    lay = unrestricted.SpinLayout(noa=2, nob=3, nva=3, nvb=2)
+    no, nv = lay.size('o'), lay.size('v')
+    n = no + nv
+    # spin of each orbital: occupied list first, then virtual list
+    spin = numpy.array(lay.labels('o') + lay.labels('v'))
+    eri = fs.random_antisym(n, rng, hermitian=False)
+    keep = ((spin[:, None, None, None] + spin[None, :, None, None]) ==
+            (spin[None, None, :, None] + spin[None, None, None, :]))
+    eri = eri * keep
+    f = rng.normal(size=(n, n)) * (spin[:, None] == spin[None, :])
+    role = {'o': slice(0, no), 'v': slice(no, n)}
+    Fg = type("B", (), {})()
+    for k in ("oo", "ov", "vo", "vv"):
+        setattr(Fg, k, torch.as_tensor(f[role[k[0]], role[k[1]]]))
+    Ig = type("B", (), {})()
+    for k in ("vvvv", "vvvo", "vovv", "vvoo", "vovo", "oovv", "vooo", "ooov",
+              "oooo"):
+        setattr(Ig, k, torch.as_tensor(
+            eri[role[k[0]], role[k[1]], role[k[2]], role[k[3]]]))
+    Fa, Fb, Ia, Ib, Iab = unrestricted.unrestricted_blocks(lay, Fg, Ig)
+    F2, I2 = unrestricted.spin_orbital_blocks(lay, Fa, Fb, Ia, Ib, Iab, "cpu")
+    for k in ("oo", "ov", "vo", "vv"):
+        assert torch.equal(getattr(F2, k), getattr(Fg, k))
+    for k in ("vvvv", "vvvo", "vovv", "vvoo", "vovo", "oovv", "vooo", "ooov",
+              "oooo"):
+        assert torch.equal(getattr(I2, k), getattr(Ig, k)), k
+    # amplitudes: antisymmetric, spin-conserving, round trip
+    ng = 2
+    lo, lv = numpy.array(lay.labels('o')), numpy.array(lay.labels('v'))
+    t2 = rng.normal(size=(ng, nv, nv, no, no))
+    t2 = t2 - t2.transpose(0, 2, 1, 3, 4)
+    t2 = t2 - t2.transpose(0, 1, 2, 4, 3)
+    t2 = t2 * ((lv[:, None, None, None] + lv[None, :, None, None]) ==
+               (lo[None, None, :, None] + lo[None, None, None, :]))
+    t1 = rng.normal(size=(ng, nv, no)) * (lv[:, None] == lo[None, :])
+    T1u, T2u = unrestricted.unrestricted_amplitudes(
+        lay, torch.as_tensor(t1), torch.as_tensor(t2))
+    T1g, T2g = unrestricted.spin_orbital_amplitudes(lay, *T1u, *T2u, "cpu")
+    assert numpy.array_equal(T1g.numpy(), t1)
+    assert numpy.array_equal(T2g.numpy(), t2)
