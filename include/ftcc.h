@@ -88,7 +88,11 @@ int ftcc_set_blocks(ftcc_ctx *ctx, const double *const F_dev[4],
  * the amplitude entries that do not and runs dense if there are any. */
 int ftcc_set_spin(ftcc_ctx *ctx, const int *lab_o_host,
                   const int *lab_v_host);
-/* *on = 1 when the products run spin-blocked. */
+/* *on = 0 when the products run dense, 1 when the O(n^6) products run
+ * spin-blocked, 2 when in addition the O(n^5) contractions run over the
+ * spin-allowed blocks only (labels form one run of 0s then one of 1s in the
+ * occupied and in the virtual list; FTCC_SPIN5=0 in the environment at
+ * ftcc_set_system/ftcc_set_spin time keeps them dense). */
 int ftcc_get_spin(ftcc_ctx *ctx, int *on);
 
 /* Uniform-electron-gas integrals built on the device (the provider of

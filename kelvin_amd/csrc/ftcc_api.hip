@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <set>
+#include <vector>
 
 using namespace ftcc;
 
@@ -82,6 +84,12 @@ struct ftcc_ctx {
   // three dense block products (10/64 of the dense work).
   int spin = 0;
   int *d_lab_o = nullptr, *d_lab_v = nullptr;
+  // O(n^5) contractions restricted to spin-allowed blocks: needs each label
+  // list to be one run of 0s then one run of 1s (spin5 = 1); starts/counts
+  // of the two runs in the occupied and virtual index lists
+  int spin5 = 0;
+  int o_start[2] = {0, 0}, o_count[2] = {0, 0};
+  int v_start[2] = {0, 0}, v_count[2] = {0, 0};
   PairSpace ps_vv, ps_oo, ps_vo;   // (a,b), (i,j) and ring (a,i) pairs
   double *b64[10] = {};            // FP64 blocked operands (fp64/fp32 modes)
   int b64_layout = -1;
@@ -282,6 +290,153 @@ void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false) {
   ensure_partial(c, g);
   timed(c, big, gemm_flops(g),
         [&] { gemm_run(g, prec, c->scr, c->stream); });
+}
+
+// One index of a spin-restricted contraction: its letter, its orbital role
+// ('o' or 'v') and its stride in A, B and C (C's strides also address D).
+struct SIdx {
+  char l, role;
+  int64_t sa, sb, sc;
+};
+
+// Spin conservation rule of an operand, "mn|ie": the spins of the letters
+// left of '|' sum to those right of it ("e|j": a spin-diagonal matrix).
+bool rule_holds(const char *rule, const std::map<char, int> &sp) {
+  int left = 0, right = 0;
+  bool r = false;
+  for (const char *q = rule; *q; ++q) {
+    if (*q == '|') {
+      r = true;
+      continue;
+    }
+    (r ? right : left) += sp.at(*q);
+  }
+  return left == right;
+}
+
+// C[M,N] = alpha * A[M,K] B[K,N] + beta * (D or C) over the spin-allowed
+// blocks only: M and N run over up to three orbital indices each, K over
+// one, and every assignment of spins to the letters under which the rules of
+// A, B and C all hold is one launch over the product of the letters' spin
+// runs (the other blocks of A and B are zero, those of C are not written).
+// `base` carries the pointers, alpha/beta, D, the slice count and the slice
+// strides; the rest is filled here.
+void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
+              const std::vector<SIdx> &Mi, const std::vector<SIdx> &Ni,
+              const SIdx &Ki, const char *ruleA, const char *ruleB,
+              const char *ruleC) {
+  std::vector<char> letters;
+  auto add = [&](char l) {
+    if (std::find(letters.begin(), letters.end(), l) == letters.end())
+      letters.push_back(l);
+  };
+  for (const SIdx &x : Mi) add(x.l);
+  for (const SIdx &x : Ni) add(x.l);
+  add(Ki.l);
+  require(Mi.size() >= 1 && Mi.size() <= 3 && Ni.size() >= 1 &&
+              Ni.size() <= 3 && letters.size() == Mi.size() + Ni.size() + 1,
+          "spin-restricted GEMM: bad index lists");
+  auto start = [&](const SIdx &x, int sp) {
+    return x.role == 'o' ? c->o_start[sp] : c->v_start[sp];
+  };
+  auto count = [&](const SIdx &x, int sp) {
+    return x.role == 'o' ? c->o_count[sp] : c->v_count[sp];
+  };
+  const bool writes_all = base.beta == 0.0 || base.D != nullptr;
+  std::set<std::vector<int>> c_blocks;   // C spin blocks that were written
+  const int L = (int)letters.size();
+  for (int mask = 0; mask < (1 << L); ++mask) {
+    std::map<char, int> sp;
+    for (int k = 0; k < L; ++k) sp[letters[k]] = (mask >> k) & 1;
+    if (!rule_holds(ruleA, sp) || !rule_holds(ruleB, sp) ||
+        !rule_holds(ruleC, sp))
+      continue;
+    Gemm g = base;
+    int64_t oa = 0, ob = 0, oc = 0;
+    int64_t M = 1, N = 1;
+    std::vector<int> cm, cn;
+    for (const SIdx &x : Mi) {
+      const int s0 = sp[x.l];
+      oa += start(x, s0) * x.sa;
+      oc += start(x, s0) * x.sc;
+      cm.push_back(count(x, s0));
+      M *= count(x, s0);
+    }
+    for (const SIdx &x : Ni) {
+      const int s0 = sp[x.l];
+      ob += start(x, s0) * x.sb;
+      oc += start(x, s0) * x.sc;
+      cn.push_back(count(x, s0));
+      N *= count(x, s0);
+    }
+    const int sk = sp[Ki.l];
+    oa += start(Ki, sk) * Ki.sa;
+    ob += start(Ki, sk) * Ki.sb;
+    const int K = count(Ki, sk);
+    std::vector<int> cb;
+    for (const SIdx &x : Mi) cb.push_back(sp[x.l]);
+    for (const SIdx &x : Ni) cb.push_back(sp[x.l]);
+    c_blocks.insert(cb);
+    if (M == 0 || N == 0) continue;
+    require(M < (1LL << 31) && N < (1LL << 31), "spin block too large");
+    g.M = (int)M;
+    g.N = (int)N;
+    g.K = K;
+    g.A = base.A + oa;
+    g.B = base.B + ob;
+    g.C = base.C + oc;
+    g.D = base.D ? base.D + oc : nullptr;
+    g.sAk = Ki.sa;
+    g.sBk = Ki.sb;
+    // levels: the listed indices fill the innermost slots
+    const int nm = (int)Mi.size(), nn = (int)Ni.size();
+    const SIdx z{0, 'o', 0, 0, 0};
+    const SIdx &m0 = nm == 3 ? Mi[0] : z, &m1 = nm >= 2 ? Mi[nm - 2] : z,
+               &m2 = Mi[nm - 1];
+    const SIdx &n0 = nn == 3 ? Ni[0] : z, &n1 = nn >= 2 ? Ni[nn - 2] : z,
+               &n2 = Ni[nn - 1];
+    g.M1 = nm >= 2 ? cm[nm - 2] : 1;
+    g.M2 = cm[nm - 1];
+    g.N1 = nn >= 2 ? cn[nn - 2] : 1;
+    g.N2 = cn[nn - 1];
+    g.sAm0 = m0.sa;
+    g.sAm1 = m1.sa;
+    g.sAm = m2.sa;
+    g.sCm0 = g.sDm0 = m0.sc;
+    g.sCm1 = g.sDm1 = m1.sc;
+    g.sCm = g.sDm = m2.sc;
+    g.sBn0 = n0.sb;
+    g.sBn1 = n1.sb;
+    g.sBn = n2.sb;
+    g.sCn0 = g.sDn0 = n0.sc;
+    g.sCn1 = g.sDn1 = n1.sc;
+    g.sCn = g.sDn = n2.sc;
+    if (K == 0) {
+      if (!writes_all) continue;
+      g.K = 0;   // C = beta * (D or C) only
+    }
+    run(c, g, prec);
+  }
+  if (writes_all) {
+    // every spin-allowed block of C must have been written
+    const int nc = (int)(Mi.size() + Ni.size());
+    for (int mask = 0; mask < (1 << nc); ++mask) {
+      std::map<char, int> sp;
+      std::vector<int> cb;
+      int k = 0;
+      for (const SIdx &x : Mi) {
+        sp[x.l] = (mask >> k++) & 1;
+        cb.push_back(sp[x.l]);
+      }
+      for (const SIdx &x : Ni) {
+        sp[x.l] = (mask >> k++) & 1;
+        cb.push_back(sp[x.l]);
+      }
+      if (rule_holds(ruleC, sp))
+        require(c_blocks.count(cb) == 1,
+                "spin-restricted GEMM leaves an allowed output block unset");
+    }
+  }
 }
 
 // One of the six O(n^6) products: split-bf16 operands in bf16x3 mode
@@ -617,6 +772,24 @@ bool try_spin(ftcc_ctx *c, const std::vector<int> &lo,
   c->ps_oo = build_pairs(c, lo, lo, LADDER_ORDER);
   c->ps_vo = build_pairs(c, lv, lo, RING_ORDER);
   c->spin = 1;
+  // one run of alpha then one run of beta in each list
+  auto runs = [](const std::vector<int> &l, int start[2], int count[2]) {
+    int n0 = 0;
+    while (n0 < (int)l.size() && l[n0] == 0) ++n0;
+    for (size_t k = n0; k < l.size(); ++k)
+      if (l[k] != 1) return false;
+    start[0] = 0;
+    count[0] = n0;
+    start[1] = n0;
+    count[1] = (int)l.size() - n0;
+    return true;
+  };
+  const char *env = getenv("FTCC_SPIN5");
+  const bool want5 = !(env && atoi(env) == 0);
+  c->spin5 = (want5 && runs(lo, c->o_start, c->o_count) &&
+              runs(lv, c->v_start, c->v_count))
+                 ? 1
+                 : 0;
   return true;
 }
 
@@ -695,9 +868,22 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   const int O = c->no, V = c->nv;
   const int P = c->prec;           // contraction precision of n^5/n^6 terms
   const int P64 = FTCC_PREC_FP64;  // small terms
-  // the O(n^5) contractions stream n^4 operands once (HBM-bound), so they
-  // run in FP64 whatever the precision of the O(n^6) products
+  // O(n^5) contractions run in FP64 whatever the precision of the O(n^6)
+  // products: they stream n^4 operands once, and on these skinny shapes
+  // (N, K ~ n) the FP32 matrix-core variant of the GEMM measured 1.4-2x
+  // slower than the FP64 one at n = 114
   const int P5 = FTCC_PREC_FP64;
+  // O(n^5) contractions over spin-allowed blocks only (forbidden blocks of
+  // their outputs are left unwritten; no consumer reads them)
+  const bool s5 = c->spin && c->spin5;
+  auto sgem = [&](double alpha, double beta, const double *A, int64_t sA1,
+                  const double *B, int64_t sB1, double *C, int64_t sC1,
+                  const double *D) {
+    Gemm g = mk(0, 0, 0, alpha, beta, A, 0, 0, sA1, B, 0, 0, sB1, C, 0, 0,
+                sC1, nb);
+    g.D = D;
+    return g;
+  };
   hipStream_t s = c->stream;
   const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
   const int64_t S2 = VV * OO;           // doubles slice
@@ -755,14 +941,29 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
             OO, nb), P64);
 
   // Wmnij = oooo + P(ij) t_j^e <mn||ie> + 1/4 oovv.tau          (G1)
-  bcast(c->wmnij, oooo, O4, nb, 1.0, 0.0, s);
-  // + sum_e <mn||ie> t_j^e into [(m,n,i), j]
-  run(c, mk((int)(OO * O), O, V, 1.0, 1.0, ooov, V, 1, 0, t1, O, 1, VO,
-            c->wmnij, O, 1, O4, nb), P5);
-  {
+  // oooo + sum_e <mn||ie> t_j^e into [(m,n,i), j]
+  if (s5) {
+    run_spin(c, sgem(1.0, 1.0, ooov, 0, t1, VO, c->wmnij, O4, oooo), P5,
+             {{'m', 'o', OO * V, 0, O * OO}, {'n', 'o', O * V, 0, OO},
+              {'i', 'o', V, 0, O}},
+             {{'j', 'o', 0, 1, 1}}, {'e', 'v', 1, O, 0}, "mn|ie", "e|j",
+             "mn|ij");
+    // - sum_e t_i^e <mn||je>: C[i][(m,n,j)] -= t1^T[i,e] ooov[m,n,j,e]
+    run_spin(c, sgem(-1.0, 1.0, t1, VO, ooov, 0, c->wmnij, O4, nullptr), P5,
+             {{'i', 'o', 1, 0, O}},
+             {{'m', 'o', 0, OO * V, O * OO}, {'n', 'o', 0, O * V, OO},
+              {'j', 'o', 0, V, 1}},
+             {'e', 'v', O, 1, 0}, "e|i", "mn|je", "mn|ij");
+  } else {
+    Gemm g = mk((int)(OO * O), O, V, 1.0, 1.0, ooov, V, 1, 0, t1, O, 1, VO,
+                c->wmnij, O, 1, O4, nb);
+    g.D = oooo;
+    g.sDm = O;
+    g.sDn = 1;
+    run(c, g, P5);
     // - sum_e t_i^e <mn||je>: per (m,n), C[i][j] -= t1^T[i,e] ooov[m,n,j,e]
-    Gemm g = mk(O, O, V, -1.0, 1.0, t1, 1, O, VO, ooov, 1, V, 0, c->wmnij, O,
-                1, O4, nb);
+    g = mk(O, O, V, -1.0, 1.0, t1, 1, O, VO, ooov, 1, V, 0, c->wmnij, O, 1,
+           O4, nb);
     g.nb2 = (int)OO;
     g.sA2 = 0;
     g.sB2 = O * V;
@@ -774,16 +975,33 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
           LADDER);
 
   // Wabef = vvvv - P(ab) t_m^b <am||ef> + 1/4 tau.oovv          (G2)
-  bcast(c->wabef, vvvv, V4, nb, 1.0, 0.0, s);
-  {
-    // - sum_m t_m^b <am||ef>: per a, C[b][ef] -= t1[b,m] vovv[a,m,ef]
+  if (s5) {
+    // vvvv - sum_m t_m^b <am||ef>: C[b][(a,e,f)] = vvvv - t1[b,m] vovv[a,m,ef]
+    run_spin(c, sgem(-1.0, 1.0, t1, VO, vovv, 0, c->wabef, V4, vvvv), P5,
+             {{'b', 'v', O, 0, VV}},
+             {{'a', 'v', 0, O * VV, V * VV}, {'e', 'v', 0, V, V},
+              {'f', 'v', 0, 1, 1}},
+             {'m', 'o', 1, VV, 0}, "b|m", "am|ef", "ab|ef");
+    // + sum_m t_m^a <bm||ef>: C[a][(b,e,f)]
+    run_spin(c, sgem(1.0, 1.0, t1, VO, vovv, 0, c->wabef, V4, nullptr), P5,
+             {{'a', 'v', O, 0, V * VV}},
+             {{'b', 'v', 0, O * VV, VV}, {'e', 'v', 0, V, V},
+              {'f', 'v', 0, 1, 1}},
+             {'m', 'o', 1, VV, 0}, "a|m", "bm|ef", "ab|ef");
+  } else {
+    // vvvv - sum_m t_m^b <am||ef>: per a, C[b][ef] = vvvv - t1[b,m] vovv[a,m,ef]
     Gemm g = mk(V, (int)VV, O, -1.0, 1.0, t1, O, 1, VO, vovv, VV, 1, 0,
                 c->wabef, VV, 1, V4, nb);
     g.nb2 = V;
     g.sA2 = 0;
     g.sB2 = O * VV;
     g.sC2 = V * VV;
+    g.D = vvvv;
+    g.sDm = VV;
+    g.sDn = 1;
+    g.sD2 = V * VV;
     run(c, g, P5);
+    g.D = nullptr;
     // + sum_m t_m^a <bm||ef>: per b, rows a of W[a,b,ef]
     g.alpha = 1.0;
     g.sCm = V * VV;
@@ -795,11 +1013,31 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
           LADDER);
 
   // Wr[e,m,b,j] = <mb||ej> + t_j^f <mb||ef> - t_n^b <mn||ej> - A.<mn||ef>
-  bcast(c->wr, c->ovvo_ring, S2, nb, 1.0, 0.0, s);
-  // t_j^f <mb||ef> = -sum_f vovv[b,m,e,f] t1[f,j]: rows (e,m,b) of Wr
-  run(c, mk((int)(V * VO), O, V, -1.0, 1.0, c->vovv_embf, V, 1, 0, t1, O, 1,
-            VO, c->wr, O, 1, S2, nb), P5);
-  {
+  // <mb||ej> + t_j^f <mb||ef> = ovvo_ring - sum_f vovv[b,m,e,f] t1[f,j]:
+  // rows (e,m,b) of Wr
+  if (s5) {
+    run_spin(c, sgem(-1.0, 1.0, c->vovv_embf, 0, t1, VO, c->wr, S2,
+                     c->ovvo_ring),
+             P5,
+             {{'e', 'v', O * VV, 0, O * VO}, {'m', 'o', VV, 0, VO},
+              {'b', 'v', V, 0, O}},
+             {{'j', 'o', 0, 1, 1}}, {'f', 'v', 1, O, 0}, "bm|ef", "f|j",
+             "mb|ej");
+    // -t_n^b <mn||ej> = sum_n t1[b,n] ooov[m,n,j,e]: C[b][(e,m,j)]
+    run_spin(c, sgem(1.0, 1.0, t1, VO, c->ooov_emnj, 0, c->wr, S2, nullptr),
+             P5, {{'b', 'v', O, 0, O}},
+             {{'e', 'v', 0, O * OO, O * VO}, {'m', 'o', 0, OO, VO},
+              {'j', 'o', 0, 1, 1}},
+             {'n', 'o', 1, O, 0}, "b|n", "mn|je", "mb|ej");
+  } else {
+    Gemm g = mk((int)(V * VO), O, V, -1.0, 1.0, c->vovv_embf, V, 1, 0, t1, O,
+                1, VO, c->wr, O, 1, S2, nb);
+    g.D = c->ovvo_ring;
+    g.sDm = O;
+    g.sDn = 1;
+    run(c, g, P5);
+  }
+  if (!s5) {
     // -t_n^b <mn||ej> = sum_n t1[b,n] ooov[m,n,j,e]: per (e,m), C[b][j]
     Gemm g = mk(V, O, O, 1.0, 1.0, t1, O, 1, VO, c->ooov_emnj, O, 1, 0, c->wr,
                 O, 1, S2, nb);
@@ -816,12 +1054,26 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // Z[ai,bj] = t2r.Wr - t_i^e t_m^a <mb||ej>                    (G6)
   put_operand(c, OP_WR_B, c->wr, 1, VO, S2, nb);
   // spin blocks leave the spin-forbidden part of Z unwritten: clear it
-  if (c->spin) fill(c->z, S2 * nb, 0.0, s);
+  // (the spin-aware assembly below never reads it)
+  if (c->spin && !s5) fill(c->z, S2 * nb, 0.0, s);
   product(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr,
                 VO, 1, S2, c->z, VO, 1, S2, nb), OP_T2R_A, OP_WR_B, RING_G6);
-  run(c, mk(O, (int)(O * VO), V, 1.0, 0.0, t1, 1, O, VO, c->ovvo_ring, O * VO,
-            1, 0, c->u, O * VO, 1, OO * VO, nb), P5);
-  {
+  if (s5) {
+    // u[i,(m,b,j)] = t1[e,i] ovvo_ring[e,m,b,j]; Z[a,(i,b,j)] -= t1[a,m] u
+    run_spin(c, sgem(1.0, 0.0, t1, VO, c->ovvo_ring, 0, c->u, OO * VO,
+                     nullptr),
+             P5, {{'i', 'o', 1, 0, O * VO}},
+             {{'m', 'o', 0, VO, VO}, {'b', 'v', 0, O, O}, {'j', 'o', 0, 1, 1}},
+             {'e', 'v', O, O * VO, 0}, "e|i", "mb|ej", "mb|ij");
+    run_spin(c, sgem(-1.0, 1.0, t1, VO, c->u, OO * VO, c->z, S2, nullptr), P5,
+             {{'a', 'v', O, 0, O * VO}},
+             {{'i', 'o', 0, O * VO, VO}, {'b', 'v', 0, O, O},
+              {'j', 'o', 0, 1, 1}},
+             {'m', 'o', 1, VO, 0}, "a|m", "mb|ij", "ab|ij");
+  } else {
+    run(c, mk(O, (int)(O * VO), V, 1.0, 0.0, t1, 1, O, VO, c->ovvo_ring,
+              O * VO, 1, 0, c->u, O * VO, 1, OO * VO, nb),
+        P5);
     Gemm g = mk(V, (int)VO, O, -1.0, 1.0, t1, O, 1, VO, c->u, VO, 1, OO * VO,
                 c->z, O * VO, 1, S2, nb);
     g.nb2 = O;
@@ -843,7 +1095,29 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
           OP_TAU_B, LADDER);                                             // G4
 
   // S1 (antisymmetrised in ab): Fbe[b,e] t2[a,e,i,j] + t1[a,m] vooo[b,m,i,j]
-  {
+  // S2 (antisymmetrised in ij): -t2[a,b,i,m] Fmj[m,j] + t1[e,i] vvvo[a,b,e,j]
+  if (s5) {
+    run_spin(c, sgem(1.0, 0.0, c->Fbe, VV, t2, S2, c->s1, S2, nullptr), P5,
+             {{'b', 'v', V, 0, OO}},
+             {{'a', 'v', 0, V * OO, V * OO}, {'i', 'o', 0, O, O},
+              {'j', 'o', 0, 1, 1}},
+             {'e', 'v', 1, OO, 0}, "b|e", "ae|ij", "ab|ij");
+    run_spin(c, sgem(1.0, 1.0, t1, VO, vooo, 0, c->s1, S2, nullptr), P5,
+             {{'a', 'v', O, 0, V * OO}},
+             {{'b', 'v', 0, O * OO, OO}, {'i', 'o', 0, O, O},
+              {'j', 'o', 0, 1, 1}},
+             {'m', 'o', 1, OO, 0}, "a|m", "bm|ij", "ab|ij");
+    run_spin(c, sgem(-1.0, 0.0, t2, S2, c->Fmj, OO, c->s2, S2, nullptr), P5,
+             {{'a', 'v', V * OO, 0, V * OO}, {'b', 'v', OO, 0, OO},
+              {'i', 'o', O, 0, O}},
+             {{'j', 'o', 0, 1, 1}}, {'m', 'o', 1, O, 0}, "ab|im", "m|j",
+             "ab|ij");
+    run_spin(c, sgem(1.0, 1.0, t1, VO, vvvo, 0, c->s2, S2, nullptr), P5,
+             {{'i', 'o', 1, 0, O}},
+             {{'a', 'v', 0, V * VO, V * OO}, {'b', 'v', 0, VO, OO},
+              {'j', 'o', 0, 1, 1}},
+             {'e', 'v', O, O, 0}, "e|i", "ab|ej", "ab|ij");
+  } else {
     Gemm g = mk(V, (int)OO, V, 1.0, 0.0, c->Fbe, V, 1, VV, t2, OO, 1, S2,
                 c->s1, OO, 1, S2, nb);
     g.nb2 = V;
@@ -851,29 +1125,27 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sB2 = V * OO;
     g.sC2 = V * OO;
     run(c, g, P5);
-  }
-  {
-    Gemm g = mk(V, (int)OO, O, 1.0, 1.0, t1, O, 1, VO, vooo, OO, 1, 0, c->s1,
-                V * OO, 1, S2, nb);
+    g = mk(V, (int)OO, O, 1.0, 1.0, t1, O, 1, VO, vooo, OO, 1, 0, c->s1,
+           V * OO, 1, S2, nb);
     g.nb2 = V;
     g.sA2 = 0;
     g.sB2 = O * OO;
     g.sC2 = OO;
     run(c, g, P5);
-  }
-  // S2 (antisymmetrised in ij): -t2[a,b,i,m] Fmj[m,j] + t1[e,i] vvvo[a,b,e,j]
-  run(c, mk((int)(VV * O), O, O, -1.0, 0.0, t2, O, 1, S2, c->Fmj, O, 1, OO,
-            c->s2, O, 1, S2, nb), P5);
-  {
-    Gemm g = mk(O, O, V, 1.0, 1.0, t1, 1, O, VO, vvvo, O, 1, 0, c->s2, O, 1,
-                S2, nb);
+    run(c, mk((int)(VV * O), O, O, -1.0, 0.0, t2, O, 1, S2, c->Fmj, O, 1, OO,
+              c->s2, O, 1, S2, nb),
+        P5);
+    g = mk(O, O, V, 1.0, 1.0, t1, 1, O, VO, vvvo, O, 1, 0, c->s2, O, 1, S2,
+           nb);
     g.nb2 = (int)VV;
     g.sA2 = 0;
     g.sB2 = VO;
     g.sC2 = OO;
     run(c, g, P5);
   }
-  r2_final(t2bar, c->s1, c->s2, c->z, V, O, nb, s);
+  // forbidden blocks: written as zero without reading S1, S2, Z
+  r2_final(t2bar, c->s1, c->s2, c->z, V, O, nb, s,
+           s5 ? c->d_lab_v : nullptr, s5 ? c->d_lab_o : nullptr);
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
   //                    + 1/2 vovv.t2p + 1/2 t2.ooov_emni)
@@ -1192,7 +1464,7 @@ int ftcc_set_spin(ftcc_ctx *c, const int *lab_o, const int *lab_v) {
 int ftcc_get_spin(ftcc_ctx *c, int *on) {
   return guarded(c, [&] {
     require(on != nullptr, "null output pointer");
-    *on = c->spin;
+    *on = c->spin ? (c->spin5 ? 2 : 1) : 0;
   });
 }
 

@@ -51,6 +51,17 @@ struct Gemm {
   // optional scatter of the output: row m of the product is written to row
   // rmap[m] of C (column n to cmap[n]); -1 drops it
   const int *rmap = nullptr, *cmap = nullptr;
+  // optional addend: C = alpha*A.B + beta*D with D laid out like C (its own
+  // strides; sD1 = 0 broadcasts one matrix over the slices), C is not read
+  const double *D = nullptr;
+  int64_t sDm = 0, sDn = 0, sD1 = 0, sD2 = 0;
+  // optional rows/columns over up to three tensor indices: row
+  // m = (m0*M1 + m1)*M2 + m2 sits at m0*sXm0 + m1*sXm1 + m2*sXm for operand
+  // X (A, C, D); columns likewise with N1, N2 and sXn0, sXn1, sXn for B, C,
+  // D.  M2 = 0 (N2 = 0) means one index with stride sXm (sXn).
+  int M1 = 1, M2 = 0, N1 = 1, N2 = 0;
+  int64_t sAm0 = 0, sAm1 = 0, sCm0 = 0, sCm1 = 0, sDm0 = 0, sDm1 = 0;
+  int64_t sBn0 = 0, sBn1 = 0, sCn0 = 0, sCn1 = 0, sDn0 = 0, sDn1 = 0;
 };
 
 // Scratch the GEMM engine may use for split-K partial sums and low-precision
@@ -194,9 +205,12 @@ void antisym_add(double *W, const double *Q, int d0, int d1, int d2, int d3,
 // Wr[e,m,b,j] = ovvo_ring[e,m,b,j] - Q1[b,m,e,j] + Q2[m,b,j,e]
 void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
                    const double *Q2, int nv, int no, int nb, hipStream_t s);
-// T2bar[a,b,i,j] = -( acc + S1 - S1^(ab) + S2 - S2^(ij) + PP(Z ring) )
+// T2bar[a,b,i,j] = -( acc + S1 - S1^(ab) + S2 - S2^(ij) + PP(Z ring) );
+// with spin labels (device arrays, or nullptr), spin-forbidden elements are
+// written as zero without reading the operands
 void r2_final(double *t2bar, const double *S1, const double *S2,
-              const double *Z, int nv, int no, int nb, hipStream_t s);
+              const double *Z, int nv, int no, int nb, hipStream_t s,
+              const int *lab_v = nullptr, const int *lab_o = nullptr);
 // x[n][c] (nb batches of n*c) -> out[c][n] transposed copy for small mats
 void transpose2(const double *in, double *out, int r, int c, int nb,
                 hipStream_t s);

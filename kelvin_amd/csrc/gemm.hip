@@ -49,7 +49,30 @@ struct GemmDev {
   double *partial;   // [R][nbC][M][N] when splits*R > 1
   int nbC;           // number of distinct C batches (nb1 or nb1*nb2)
   const int *rmap, *cmap;   // output scatter (nullptr: identity)
+  const double *D;          // addend laid out like C (nullptr: C itself)
+  int64_t sDm, sDn, sD1, sD2;
+  // rows m = (m0*M1 + m1)*M2 + m2 and columns likewise (one index: M1 = 1,
+  // M2 = 2^30); the strides above belong to the innermost index
+  int M1, M2, N1, N2;
+  int64_t sAm0, sAm1, sCm0, sCm1, sDm0, sDm1;
+  int64_t sBn0, sBn1, sCn0, sCn1, sDn0, sDn1;
 };
+
+struct Lvl {
+  int x0, x1, x2;
+};
+__device__ inline Lvl levels(int x, int n1, int n2) {
+  Lvl l;
+  l.x2 = x % n2;
+  const int t = x / n2;
+  l.x1 = t % n1;
+  l.x0 = t / n1;
+  return l;
+}
+__device__ inline int64_t at(const Lvl &l, int64_t s0, int64_t s1,
+                             int64_t s2) {
+  return l.x0 * s0 + l.x1 * s1 + l.x2 * s2;
+}
 
 template <typename T> struct Mma;
 template <> struct Mma<double> {
@@ -89,37 +112,42 @@ __device__ inline BlockPos decode(const GemmDev &p) {
   return q;
 }
 
+// Element offsets of the tile's rows of A and columns of B, one table entry
+// per row/column of the tile (-1 past the end), kept in LDS so they hold no
+// registers through the K loop.
+__device__ inline void tile_offsets(const GemmDev &p, int m0, int n0,
+                                    int64_t *aoff, int64_t *boff) {
+  const int t = threadIdx.x;
+  if (t < BM) {
+    const int m = m0 + t;
+    aoff[t] = m < p.M ? at(levels(m, p.M1, p.M2), p.sAm0, p.sAm1, p.sAm) : -1;
+  } else if (t < BM + BN) {
+    const int n = n0 + t - BM;
+    boff[t - BM] =
+        n < p.N ? at(levels(n, p.N1, p.N2), p.sBn0, p.sBn1, p.sBn) : -1;
+  }
+}
+
 template <typename T, bool A_MFAST, bool B_NFAST>
 __device__ inline void load_slab(const GemmDev &p, const double *A,
-                                 const double *B, int m0, int n0, int k0,
-                                 int kend, T ra[8], T rb[8]) {
+                                 const double *B, const int64_t *aoff,
+                                 const int64_t *boff, int k0, int kend,
+                                 T ra[8], T rb[8]) {
   const int t = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
-    int m, k;
-    if (A_MFAST) {
-      m = m0 + (t & 127);
-      k = k0 + (t >> 7) + 2 * r;
-    } else {
-      m = m0 + (t >> 4) + 16 * r;
-      k = k0 + (t & 15);
-    }
+    const int64_t off = aoff[A_MFAST ? (t & 127) : (t >> 4) + 16 * r];
+    const int k = A_MFAST ? k0 + (t >> 7) + 2 * r : k0 + (t & 15);
     double v = 0.0;
-    if (m < p.M && k < kend) v = A[(int64_t)m * p.sAm + (int64_t)k * p.sAk];
+    if (off >= 0 && k < kend) v = A[off + (int64_t)k * p.sAk];
     ra[r] = (T)v;
   }
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
-    int n, k;
-    if (B_NFAST) {
-      n = n0 + (t & 127);
-      k = k0 + (t >> 7) + 2 * r;
-    } else {
-      n = n0 + (t >> 4) + 16 * r;
-      k = k0 + (t & 15);
-    }
+    const int64_t off = boff[B_NFAST ? (t & 127) : (t >> 4) + 16 * r];
+    const int k = B_NFAST ? k0 + (t >> 7) + 2 * r : k0 + (t & 15);
     double v = 0.0;
-    if (n < p.N && k < kend) v = B[(int64_t)k * p.sBk + (int64_t)n * p.sBn];
+    if (off >= 0 && k < kend) v = B[(int64_t)k * p.sBk + off];
     rb[r] = (T)v;
   }
 }
@@ -157,6 +185,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   constexpr int LDK = Mma<T>::LDK;
   __shared__ T As[2][BM * LDK];
   __shared__ T Bs[2][BN * LDK];
+  __shared__ int64_t aoff[BM], boff[BN], drow[BM], dcol[BN];
 
   const BlockPos q = decode(p);
   const double *A = p.A + q.b1 * p.sA1 + q.b2 * p.sA2;
@@ -177,15 +206,17 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
     for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   T ra[8], rb[8];
+  tile_offsets(p, m0, n0, aoff, boff);
+  __syncthreads();
   if (nslab > 0) {
-    load_slab<T, A_MFAST, B_NFAST>(p, A, B, m0, n0, kbeg, kend, ra, rb);
+    load_slab<T, A_MFAST, B_NFAST>(p, A, B, aoff, boff, kbeg, kend, ra, rb);
     store_slab<T, A_MFAST, B_NFAST>(As[0], Bs[0], ra, rb);
   }
   __syncthreads();
   for (int s = 0; s < nslab; ++s) {
     const int buf = s & 1;
     if (s + 1 < nslab)
-      load_slab<T, A_MFAST, B_NFAST>(p, A, B, m0, n0, kbeg + (s + 1) * BK,
+      load_slab<T, A_MFAST, B_NFAST>(p, A, B, aoff, boff, kbeg + (s + 1) * BK,
                                      kend, ra, rb);
     T a[4][4], b[4][4];
 #pragma unroll
@@ -206,9 +237,32 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
     __syncthreads();
   }
 
-  // epilogue: lane holds column li of each 16x16 block, rows Mma<T>::crow
+  // epilogue: lane holds column li of each 16x16 block, rows Mma<T>::crow.
+  // Output offsets of the tile's rows and columns (after the scatter maps;
+  // -1 where dropped or past the end) go to LDS first, so the unrolled
+  // stores below only look them up.  The K loop ended on a barrier, so the
+  // offset tables of A and B are free.
   const bool direct = (p.partial == nullptr);
   double *C = p.C + q.b1 * p.sC1 + q.b2 * p.sC2;
+  const double *D = p.D ? p.D + q.b1 * p.sD1 + q.b2 * p.sD2 : nullptr;
+  int64_t *crow = aoff, *ccol = boff;
+  if (direct) {
+    const int t = threadIdx.x;
+    if (t < BM) {
+      const int row = m0 + t;
+      const int rr = row < p.M ? (p.rmap ? p.rmap[row] : row) : -1;
+      const Lvl l = levels(rr < 0 ? 0 : rr, p.M1, p.M2);
+      crow[t] = rr < 0 ? -1 : at(l, p.sCm0, p.sCm1, p.sCm);
+      drow[t] = rr < 0 ? -1 : at(l, p.sDm0, p.sDm1, p.sDm);
+    } else if (t < BM + BN) {
+      const int col = n0 + t - BM;
+      const int cc = col < p.N ? (p.cmap ? p.cmap[col] : col) : -1;
+      const Lvl l = levels(cc < 0 ? 0 : cc, p.N1, p.N2);
+      ccol[t - BM] = cc < 0 ? -1 : at(l, p.sCn0, p.sCn1, p.sCn);
+      dcol[t - BM] = cc < 0 ? -1 : at(l, p.sDn0, p.sDn1, p.sDn);
+    }
+    __syncthreads();
+  }
   const int cb = p.reduce_b2 ? q.b1 : q.b1 * p.nb2 + q.b2;
   const int rsel = p.reduce_b2 ? q.b2 * p.splits + q.ks : q.ks;
   double *P = direct ? nullptr
@@ -218,54 +272,59 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
-      const int col = n0 + wn * 64 + ct * 16 + li;
+      const int lc = wn * 64 + ct * 16 + li;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm * 64 + rt * 16 + Mma<T>::crow(lg, v);
-        if (row < p.M && col < p.N) {
-          const double x = (double)acc[rt][ct][v];
-          if (direct) {
-            const int rr = p.rmap ? p.rmap[row] : row;
-            const int cc = p.cmap ? p.cmap[col] : col;
-            if (rr < 0 || cc < 0) continue;
-            double *c = C + (int64_t)rr * p.sCm + (int64_t)cc * p.sCn;
-            *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
-          } else {
-            P[(int64_t)row * p.N + col] = x;
+        const int lr = wm * 64 + rt * 16 + Mma<T>::crow(lg, v);
+        const double x = (double)acc[rt][ct][v];
+        if (direct) {
+          const int64_t oc = crow[lr], occ = ccol[lc];
+          if (oc >= 0 && occ >= 0) {
+            double *c = C + oc + occ;
+            if (D)
+              *c = p.alpha * x + p.beta * D[drow[lr] + dcol[lc]];
+            else
+              *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
           }
+        } else if (m0 + lr < p.M && n0 + lc < p.N) {
+          P[(int64_t)(m0 + lr) * p.N + n0 + lc] = x;
         }
       }
     }
 }
 
 // C[b][m,n] = alpha * sum_r partial[r][b][m][n] + beta*C  (fixed order)
-__global__ void reduce_partial_kernel(const double *partial, int R, int nbC,
-                                      int M, int N, int nb2, int reduce_b2,
-                                      double alpha, double beta, double *C,
-                                      int64_t sCm, int64_t sCn, int64_t sC1,
-                                      int64_t sC2, const int *rmap,
-                                      const int *cmap) {
-  const int64_t MN = (int64_t)M * N;
-  const int64_t total = MN * nbC;
+__global__ void reduce_partial_kernel(GemmDev p, int R) {
+  const int64_t MN = (int64_t)p.M * p.N;
+  const int64_t total = MN * p.nbC;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
        idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t cb = idx / MN;
     const int64_t mn = idx % MN;
-    const int m0 = (int)(mn / N), n0 = (int)(mn % N);
-    const int m = rmap ? rmap[m0] : m0, n = cmap ? cmap[n0] : n0;
+    const int m0 = (int)(mn / p.N), n0 = (int)(mn % p.N);
+    const int m = p.rmap ? p.rmap[m0] : m0, n = p.cmap ? p.cmap[n0] : n0;
     if (m < 0 || n < 0) continue;
     double s = 0.0;
-    for (int r = 0; r < R; ++r) s += partial[((int64_t)r * nbC + cb) * MN + mn];
+    for (int r = 0; r < R; ++r)
+      s += p.partial[((int64_t)r * p.nbC + cb) * MN + mn];
     int64_t b1, b2;
-    if (reduce_b2) {
+    if (p.reduce_b2) {
       b1 = cb;
       b2 = 0;
     } else {
-      b1 = cb / nb2;
-      b2 = cb % nb2;
+      b1 = cb / p.nb2;
+      b2 = cb % p.nb2;
     }
-    double *c = C + b1 * sC1 + b2 * sC2 + (int64_t)m * sCm + (int64_t)n * sCn;
-    *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
+    const Lvl lr = levels(m, p.M1, p.M2), lc = levels(n, p.N1, p.N2);
+    double *c = p.C + b1 * p.sC1 + b2 * p.sC2 + at(lr, p.sCm0, p.sCm1, p.sCm) +
+                at(lc, p.sCn0, p.sCn1, p.sCn);
+    if (p.D)
+      *c = p.alpha * s +
+           p.beta * p.D[b1 * p.sD1 + b2 * p.sD2 +
+                        at(lr, p.sDm0, p.sDm1, p.sDm) +
+                        at(lc, p.sDn0, p.sDn1, p.sDn)];
+    else
+      *c = (p.beta == 0.0) ? p.alpha * s : p.alpha * s + p.beta * *c;
   }
 }
 
@@ -348,6 +407,36 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   d.partial = nullptr;
   d.rmap = g.rmap;
   d.cmap = g.cmap;
+  d.D = g.D;
+  d.sDm = g.sDm;
+  d.sDn = g.sDn;
+  d.sD1 = g.sD1;
+  d.sD2 = g.sD2;
+  // index levels of rows and columns (one index: everything in the last)
+  require(g.M2 >= 0 && g.N2 >= 0 && g.M1 >= 1 && g.N1 >= 1,
+          "GEMM index levels must be positive");
+  if (g.M2 > 0)
+    require(g.M % ((int64_t)g.M1 * g.M2) == 0,
+            "GEMM rows are not a product of their index levels");
+  if (g.N2 > 0)
+    require(g.N % ((int64_t)g.N1 * g.N2) == 0,
+            "GEMM columns are not a product of their index levels");
+  d.M1 = g.M2 > 0 ? g.M1 : 1;
+  d.M2 = g.M2 > 0 ? g.M2 : (1 << 30);
+  d.N1 = g.N2 > 0 ? g.N1 : 1;
+  d.N2 = g.N2 > 0 ? g.N2 : (1 << 30);
+  d.sAm0 = g.sAm0;
+  d.sAm1 = g.sAm1;
+  d.sCm0 = g.sCm0;
+  d.sCm1 = g.sCm1;
+  d.sDm0 = g.sDm0;
+  d.sDm1 = g.sDm1;
+  d.sBn0 = g.sBn0;
+  d.sBn1 = g.sBn1;
+  d.sCn0 = g.sCn0;
+  d.sCn1 = g.sCn1;
+  d.sDn0 = g.sDn0;
+  d.sDn1 = g.sDn1;
   if (R > 1) {
     const size_t need = (size_t)R * d.nbC * g.M * g.N;
     require(need <= scr.partial_elems, "GEMM split-K scratch too small",
@@ -371,9 +460,7 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
     const int64_t total = (int64_t)d.nbC * g.M * g.N;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
     hipLaunchKernelGGL(reduce_partial_kernel, dim3(blocks), dim3(256), 0, s,
-                       d.partial, R, d.nbC, g.M, g.N, g.nb2, d.reduce_b2,
-                       g.alpha, g.beta, g.C, g.sCm, g.sCn, g.sC1, g.sC2,
-                       g.rmap, g.cmap);
+                       d, R);
     FTCC_CHECK_HIP(hipGetLastError());
   }
 }

@@ -274,43 +274,15 @@ __global__ void ring_assemble_kernel(double *Wr, const double *ovvo_ring,
 //                     + S1[a,b,i,j] - S1[b,a,i,j]
 //                     + S2[a,b,i,j] - S2[a,b,j,i]
 //                     + Z[a,i,b,j] - Z[b,i,a,j] - Z[a,j,b,i] + Z[b,j,a,i] )
-__global__ void r2_final_kernel(double *t2bar, const double *S1,
-                                const double *S2, const double *Z, int nv,
-                                int no, int nb) {
-  const int64_t per = (int64_t)nv * nv * no * no;
-  GRID_STRIDE(idx, per * nb) {
-    const int64_t y = idx / per;
-    int64_t r = idx % per;
-    const int j = (int)(r % no);
-    r /= no;
-    const int i = (int)(r % no);
-    r /= no;
-    const int b = (int)(r % nv);
-    const int a = (int)(r / nv);
-    const double *s1 = S1 + y * per;
-    const double *s2 = S2 + y * per;
-    const double *z = Z + y * per;
-    auto nat = [&](int p, int q, int k, int l) {
-      return (((int64_t)p * nv + q) * no + k) * no + l;
-    };
-    auto ring = [&](int p, int k, int q, int l) {
-      return (((int64_t)p * no + k) * nv + q) * no + l;
-    };
-    double v = t2bar[idx];
-    v += s1[nat(a, b, i, j)] - s1[nat(b, a, i, j)];
-    v += s2[nat(a, b, i, j)] - s2[nat(a, b, j, i)];
-    v += z[ring(a, i, b, j)] - z[ring(b, i, a, j)] - z[ring(a, j, b, i)] +
-         z[ring(b, j, a, i)];
-    t2bar[idx] = -v;
-  }
-}
-
 // Same assembly, one 32x32 (i,j) tile of one (a,b) pair per workgroup: the
 // three operands read transposed (S2[a,b,j,i], Z[a,j,b,i], Z[b,j,a,i]) are
-// loaded along i into LDS so every global access is coalesced.
+// loaded along i into LDS so every global access is coalesced.  With spin
+// labels (lv, lo), elements whose spins do not conserve (lv[a] + lv[b] !=
+// lo[i] + lo[j]) are written as zero and their operands are not read: every
+// operand element an allowed output reads is itself allowed.
 __global__ __launch_bounds__(256) void r2_final_tiled_kernel(
     double *t2bar, const double *S1, const double *S2, const double *Z,
-    int nv, int no) {
+    int nv, int no, const int *lv, const int *lo) {
   __shared__ double tS2[32][33], tZ1[32][33], tZ2[32][33];
   const int tiles = (no + 31) / 32;
   int64_t bid = blockIdx.x;
@@ -327,9 +299,10 @@ __global__ __launch_bounds__(256) void r2_final_tiled_kernel(
   double *out = t2bar + y * per;
   const int i0 = ti * 32, j0 = tj * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int sab = lv ? lv[a] + lv[b] : 0;
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + tx, j = j0 + r;
-    const bool ok = i < no && j < no;
+    const bool ok = i < no && j < no && (!lv || lo[i] + lo[j] == sab);
     tS2[r][tx] = ok ? s2[(((int64_t)a * nv + b) * no + j) * no + i] : 0.0;
     tZ1[r][tx] = ok ? z[(((int64_t)a * no + j) * nv + b) * no + i] : 0.0;
     tZ2[r][tx] = ok ? z[(((int64_t)b * no + j) * nv + a) * no + i] : 0.0;
@@ -339,6 +312,10 @@ __global__ __launch_bounds__(256) void r2_final_tiled_kernel(
     const int i = i0 + r, j = j0 + tx;
     if (i >= no || j >= no) continue;
     const int64_t nat = (((int64_t)a * nv + b) * no + i) * no + j;
+    if (lv && lo[i] + lo[j] != sab) {
+      out[nat] = 0.0;
+      continue;
+    }
     double v = out[nat];
     v += s1[nat] - s1[(((int64_t)b * nv + a) * no + i) * no + j];
     v += s2[nat] - tS2[tx][r];
@@ -520,12 +497,13 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
 }
 
 void r2_final(double *t2bar, const double *S1, const double *S2,
-              const double *Z, int nv, int no, int nb, hipStream_t s) {
+              const double *Z, int nv, int no, int nb, hipStream_t s,
+              const int *lab_v, const int *lab_o) {
   const int tiles = (no + 31) / 32;
   const int64_t blocks = (int64_t)nb * nv * nv * tiles * tiles;
   require(blocks < (1LL << 31), "r2 assembly grid too large");
   hipLaunchKernelGGL(r2_final_tiled_kernel, dim3((unsigned)blocks), dim3(256),
-                     0, s, t2bar, S1, S2, Z, nv, no);
+                     0, s, t2bar, S1, S2, Z, nv, no, lab_v, lab_o);
   FTCC_CHECK_HIP(hipGetLastError());
 }
 

@@ -164,9 +164,14 @@ class FTEngine:
 
     def spin_blocked(self):
         """True when the O(n^6) products run as spin blocks."""
+        return self.spin_mode() > 0
+
+    def spin_mode(self):
+        """0: dense; 1: O(n^6) products as spin blocks; 2: also the O(n^5)
+        contractions over spin-allowed blocks only."""
         on = ctypes.c_int()
         self._call("ftcc_get_spin", ctypes.byref(on))
-        return bool(on.value)
+        return on.value
 
     def ueg_eri(self, ivec, kvec, L):
         """Antisymmetrised spin-orbital UEG integrals (alpha, then beta) of

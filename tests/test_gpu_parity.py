@@ -438,11 +438,14 @@ def _spin_problem(no, nv, ng, lab, rng, scale=0.2):
 @pytest.mark.parametrize("no,nv,labels", [
     (5, 6, "alt"),        # unequal classes in every pair space
     (3, 7, "occ-alpha"),  # no beta occupied orbital: empty classes
+    (3, 7, "occ-alpha-runs"),  # the same with the virtual labels in runs
+    (5, 6, "uneven"),     # runs of unequal length, o != v
     (17, 17, "half")])    # classes above one 256 tile
-def test_spin_blocked_residual(prec, tol, no, nv, labels):
+def test_spin_blocked_residual(prec, tol, no, nv, labels, monkeypatch):
     """Spin-blocked O(n^6) products (three blocks each, written through the
-    pair maps) against the oracle's dense residual, and equal to the dense
-    engine path to rounding."""
+    pair maps) and, where the labels form runs, O(n^5) contractions over the
+    spin-allowed blocks only: against the oracle's dense residual, and equal
+    to the dense engine path to rounding."""
     from kelvin_amd.engine import FTEngine
     from kelvin_amd.ov_blocks import one_e_blocks, two_e_blocks
     d = dev()
@@ -452,34 +455,45 @@ def test_spin_blocked_residual(prec, tol, no, nv, labels):
         lab = numpy.arange(n) % 2
     elif labels == "occ-alpha":
         lab = numpy.concatenate([numpy.zeros(no, int), numpy.arange(nv) % 2])
+    elif labels == "occ-alpha-runs":
+        lab = numpy.concatenate([numpy.zeros(no, int),
+                                 numpy.arange(nv) >= 3]).astype(int)
+    elif labels == "uneven":
+        lab = numpy.array([0, 0, 1, 1, 1, 0, 0, 0, 0, 1, 1])
     else:
         lab = numpy.concatenate([numpy.arange(no) >= no // 2,
                                  numpy.arange(nv) >= nv // 2]).astype(int)
+    runs = labels not in ("alt", "occ-alpha")
     ng = 2
     F, I, t1, t2, lo, lv = _spin_problem(no, nv, ng, lab, rng)
     ref1, ref2 = oracle.residual_tbar(F, I, t1, t2)
     out = {}
-    for blocked in (True, False):
+    for mode in (2, 1, 0):
         eng = FTEngine(no, nv, ng, precision=prec)
         eng.set_blocks(one_e_blocks(**{k: F[k] for k in ('oo', 'ov', 'vo', 'vv')}),
                        two_e_blocks(**{k: I[k] for k in I}))
         eng.set_denominators(numpy.ones((nv, no)), numpy.ones((nv, nv, no, no)))
         eng.set_grid(numpy.zeros(ng), numpy.zeros((ng, ng)))
-        if blocked:
+        if mode:
+            if mode == 1:
+                monkeypatch.setenv("FTCC_SPIN5", "0")
             eng.set_spin(lo, lv)
-            assert eng.spin_blocked()
+            monkeypatch.delenv("FTCC_SPIN5", raising=False)
+            assert eng.spin_mode() == (2 if mode == 2 and runs else 1)
         else:
-            assert not eng.spin_blocked()
+            assert eng.spin_mode() == 0
         T1 = torch.as_tensor(t1, device=d)
         T2 = torch.as_tensor(t2, device=d)
         b1, b2 = torch.empty_like(T1), torch.empty_like(T2)
         eng.residual(T1, T2, b1, b2)
-        out[blocked] = (to_np(b1), to_np(b2))
-        assert rel(out[blocked][0], ref1) < tol
-        assert rel(out[blocked][1], ref2) < tol
+        out[mode] = (to_np(b1), to_np(b2))
+        assert rel(out[mode][0], ref1) < tol
+        assert rel(out[mode][1], ref2) < tol
     # blocked and dense differ only by summation order (and rounding of the
     # split operands, which is the same per element)
-    assert rel(out[True][1], out[False][1]) < max(tol, 1e-12)
+    for mode in (2, 1):
+        assert rel(out[mode][0], out[0][0]) < max(tol, 1e-12)
+        assert rel(out[mode][1], out[0][1]) < max(tol, 1e-12)
 
 
 def test_spin_labels_validated():
