@@ -181,6 +181,18 @@ int ftcc_point_step(ftcc_ctx *ctx, int ig, double *T1_dev, double *T2_dev,
                     double *T1bar_dev, double *T2bar_dev, double damp,
                     double out2[2]);
 
+/* Imaginary-time propagation (TD-CCSD, kelvin/td_ccsd.py:583-681 with the
+ * propagators of kelvin/propagation.py:5-52).  Each of the ng slices of
+ * the context is an independent state.  ftcc_td_rhs writes the right-hand
+ * side of td_ccsd.py:646-652, K = Tbar(T) - D T (K1 = 0 when singles == 0).
+ * ftcc_td_step advances T in place by one explicit Runge-Kutta step of
+ * size h: order 1 (rk1, Euler), 2 (rk2, Heun) or 4 (rk4, classical); with
+ * singles == 0, T1 is held fixed as the reference's singles=False does. */
+int ftcc_td_rhs(ftcc_ctx *ctx, const double *T1_dev, const double *T2_dev,
+                double *K1_dev, double *K2_dev, int singles);
+int ftcc_td_step(ftcc_ctx *ctx, int order, double h, int singles,
+                 double *T1_dev, double *T2_dev);
+
 /* Device bytes held by the context (integrals + workspace). */
 int ftcc_workspace_bytes(const ftcc_ctx *ctx, int64_t *bytes);
 

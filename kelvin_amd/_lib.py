@@ -49,6 +49,9 @@ SIGNATURES = {
                                         _vp]),
     "ftcc_point_step": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
                                        ctypes.c_double, _c_double_p]),
+    "ftcc_td_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "ftcc_td_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_int, _vp, _vp]),
     "ftcc_ueg_eri": (ctypes.c_int, [_vp, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int), _c_double_p,
                                     ctypes.c_double, _vp]),

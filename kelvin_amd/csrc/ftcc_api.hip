@@ -59,6 +59,9 @@ struct ftcc_ctx {
 
   // per-slot workspace (nb slots)
   int nb = 1;
+  // imaginary-time propagation: stage input, accumulator and right-hand
+  // side (singles / doubles), allocated on first use
+  double *td_var[2] = {}, *td_acc[2] = {}, *td_k[2] = {};
   double *tau = nullptr, *taut, *t2r, *aring, *t2p, *tautp, *wmnij, *wabef,
       *wr, *z, *u, *s1, *s2;
   double *t1T, *Fme, *FmeT, *Fae, *Fmi, *Fbe, *Fmj, *ytmp;
@@ -1664,6 +1667,61 @@ int ftcc_point_begin(ftcc_ctx *c, int ig, double *T1, double *T2,
       axpby(T2 + ig * S2, T2 + (ig - 2) * S2, S2, f, 1.0, s);
     }
     FTCC_CHECK_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int ftcc_td_rhs(ftcc_ctx *c, const double *T1, const double *T2, double *K1,
+                double *K2, int singles) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(T1 && T2 && K1 && K2, "null amplitude pointer");
+    residual_all(c, T1, T2, K1, K2, 0);
+    UpdateArgs a = upd(c);
+    a.D = c->have_D ? c->D1 : nullptr;
+    rk_stage(a, false, K1, const_cast<double *>(T1), nullptr, K1, 1.0, 0.0,
+             0.0, true, false, singles == 0, c->stream);
+    a.D = c->have_D ? c->D2 : nullptr;
+    rk_stage(a, true, K2, const_cast<double *>(T2), nullptr, K2, 1.0, 0.0,
+             0.0, true, false, false, c->stream);
+  });
+}
+
+int ftcc_td_step(ftcc_ctx *c, int order, double h, int singles, double *T1,
+                 double *T2) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(order == 1 || order == 2 || order == 4,
+            "Runge-Kutta order must be 1, 2 or 4");
+    require(T1 && T2, "null amplitude pointer");
+    const int64_t VO = (int64_t)c->nv * c->no, S2 = VO * VO;
+    const int64_t len[2] = {VO * c->ng, S2 * c->ng};
+    for (int k = 0; k < 2; ++k)
+      if (!c->td_var[k]) {
+        c->td_var[k] = dalloc(c, len[k]);
+        c->td_acc[k] = dalloc(c, len[k]);
+        c->td_k[k] = dalloc(c, len[k]);
+      }
+    // kelvin/propagation.py:5-52: rk1 (Euler), rk2 (Heun), rk4 (classical)
+    static const double b1[1] = {1.0}, b2[2] = {0.5, 0.5},
+                        b4[4] = {1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6};
+    static const double a2[1] = {1.0}, a4[3] = {0.5, 0.5, 1.0};
+    const double *bs = order == 1 ? b1 : order == 2 ? b2 : b4;
+    const double *an = order == 2 ? a2 : a4;
+    const double *in1 = T1, *in2 = T2;
+    for (int st = 0; st < order; ++st) {
+      residual_all(c, in1, in2, c->td_k[0], c->td_k[1], 0);
+      const bool first = st == 0, last = st == order - 1;
+      const double hn = last ? 0.0 : h * an[st];
+      UpdateArgs a = upd(c);
+      a.D = c->have_D ? c->D1 : nullptr;
+      rk_stage(a, false, c->td_k[0], T1, c->td_var[0], c->td_acc[0], bs[st],
+               hn, h, first, last, singles == 0, c->stream);
+      a.D = c->have_D ? c->D2 : nullptr;
+      rk_stage(a, true, c->td_k[1], T2, c->td_var[1], c->td_acc[1], bs[st], hn,
+               h, first, last, false, c->stream);
+      in1 = c->td_var[0];
+      in2 = c->td_var[1];
+    }
   });
 }
 

@@ -259,6 +259,13 @@ struct UpdateArgs {
 // |T - told|^2 in the first partial of each block (returns #blocks)
 int integrate_point(const UpdateArgs &a, bool doubles, int y, int mode,
                     hipStream_t s);
+// One explicit Runge-Kutta stage of dT/dtau = Tbar - D T over a.ng slices
+// (a.D or a.eo/a.ev give D): k = tb - D*(first ? T : var);
+// acc = (first ? 0 : acc) + b*k; last: T += h*acc, else var = T + hnext*k
+// (var == nullptr: only acc = k is formed).  zero: k = 0.
+int rk_stage(const UpdateArgs &a, bool doubles, const double *tb, double *T,
+             double *var, double *acc, double b, double hnext, double h,
+             bool first, bool last, bool zero, hipStream_t s);
 int update_t1(const UpdateArgs &a, int mode, hipStream_t s);   // returns #blocks
 int update_t2(const UpdateArgs &a, int mode, hipStream_t s);
 // true when in-place integration needs a source copy in UpdateArgs::src

@@ -498,6 +498,65 @@ int vsum(const double *x, int64_t n, double *partials, hipStream_t s) {
   return blocks;
 }
 
+// One stage of an explicit Runge-Kutta step of dT/dtau = Tbar(T) - D T (the
+// right-hand side of kelvin/td_ccsd.py:646-652), over p.ng independent
+// slices: with k = Tbar - D*in, where in is T on the first stage and var
+// after it, acc <- (first ? 0 : acc) + b*k; the last stage adds h*acc to T,
+// the others write the next stage's input T + hnext*k into var.  zero: the
+// right-hand side is zero (amplitudes held fixed).  With var == nullptr the
+// kernel only forms k into acc (first stage, b = 1, not last).
+template <bool DOUBLES>
+__global__ __launch_bounds__(UT) void rk_stage_kernel(
+    UpdateArgs p, const double *tb, double *T, double *var, double *acc,
+    double b, double hnext, double h, int first, int last, int zero) {
+  const int nv = p.nv, no = p.no;
+  const int64_t per = DOUBLES ? (int64_t)nv * nv * no * no : (int64_t)nv * no;
+  const int64_t total = per * p.ng;
+  for (int64_t x = blockIdx.x * (int64_t)UT + threadIdx.x; x < total;
+       x += (int64_t)gridDim.x * UT) {
+    double k = 0.0;
+    if (!zero) {
+      const int64_t e = x % per;
+      double d;
+      if (p.D) {
+        d = p.D[e];
+      } else {
+        const Elem q = decode<DOUBLES>(e, nv, no);
+        d = DOUBLES ? p.ev[q.a] + p.ev[q.b] - p.eo[q.i] - p.eo[q.j]
+                    : p.ev[q.a] - p.eo[q.i];
+      }
+      const double in = (first || !var) ? T[x] : var[x];
+      k = tb[x] - d * in;
+    }
+    const double a = (first ? 0.0 : acc[x]) + b * k;
+    if (last) {
+      T[x] += h * a;
+    } else {
+      acc[x] = a;
+      if (var) var[x] = T[x] + hnext * k;
+    }
+  }
+}
+
+int rk_stage(const UpdateArgs &a, bool doubles, const double *tb, double *T,
+             double *var, double *acc, double b, double hnext, double h,
+             bool first, bool last, bool zero, hipStream_t s) {
+  const int64_t per = doubles ? (int64_t)a.nv * a.nv * a.no * a.no
+                              : (int64_t)a.nv * a.no;
+  const int blocks = (int)std::max<int64_t>(
+      1, std::min<int64_t>((per * a.ng + UT - 1) / UT, MAX_BLOCKS));
+  if (doubles)
+    hipLaunchKernelGGL(rk_stage_kernel<true>, dim3(blocks), dim3(UT), 0, s, a,
+                       tb, T, var, acc, b, hnext, h, first ? 1 : 0,
+                       last ? 1 : 0, zero ? 1 : 0);
+  else
+    hipLaunchKernelGGL(rk_stage_kernel<false>, dim3(blocks), dim3(UT), 0, s,
+                       a, tb, T, var, acc, b, hnext, h, first ? 1 : 0,
+                       last ? 1 : 0, zero ? 1 : 0);
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
 int spin_violations(const double *a, int d0, int d1, int d2, int d3, int nb,
                     const int *l0, const int *l1, const int *l2, const int *l3,
                     double *partials, hipStream_t s) {

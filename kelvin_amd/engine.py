@@ -304,6 +304,20 @@ class FTEngine:
                    _vp(T2b), float(damp), out)
         return out[0], out[1]
 
+    def td_rhs(self, T1, T2, K1, K2, singles=True):
+        """K = Tbar(T) - D T at every slice (TD-CCSD right-hand side)."""
+        self._check_amps(T1, T2, K1, K2)
+        self._call("ftcc_td_rhs", _vp(T1), _vp(T2), _vp(K1), _vp(K2),
+                   int(bool(singles)))
+
+    def td_step(self, T1, T2, h, order=4, singles=True):
+        """One explicit Runge-Kutta step (order 1, 2 or 4) of size h, in
+        place, at every slice."""
+        self._check(T1, self.t1_shape(), "T1")
+        self._check(T2, self.t2_shape(), "T2")
+        self._call("ftcc_td_step", int(order), float(h), int(bool(singles)),
+                   _vp(T1), _vp(T2))
+
     def profile(self, on=True):
         """Bracket every O(n^6) product launch with HIP events."""
         self._call("ftcc_profile_enable", int(bool(on)))

@@ -1,0 +1,127 @@
+"""Imaginary-time propagated FT-CCSD (TD-CCSD) with the reference's user API.
+
+``TDCCSD(sys, prop, T=, mu=, ngrid=, ...).run()`` mirrors kelvin/td_ccsd.py:
+74-158 (construction, run) and the generalised-orbital propagation
+``_ccsd`` (kelvin/td_ccsd.py:583-681): starting from T = 0 at tau = 0, the
+amplitudes are propagated over the quadrature grid with
+dT/dtau = -D T - (F.vo, I.vvoo) - R_CCSD(T) and the grand potential
+accumulates Omega_cc = sum_{i>=1} g_i E_CC(T(tau_i)) / beta.  Every step runs
+on the GPU (``ftcc_td_step``: the residual engine plus one fused kernel per
+Runge-Kutta stage); the propagators are kelvin/propagation.py:5-52's
+explicit rk1, rk2 and rk4.
+
+It is an independent check of the fixed-point solver: both converge to the
+same Omega_cc as the grid is refined (kelvin/tests/test_td_ccsd.py).
+"""
+import logging
+
+import numpy
+
+from . import ft_utils
+from . import quadrature
+from .engine import FTEngine
+
+_ORDERS = {"rk1": 1, "rk2": 2, "rk4": 4}
+
+
+class TDCCSD(object):
+    """Time-dependent (imaginary-time propagated) CCSD driver."""
+
+    def __init__(self, sys, prop, T=0.0, mu=0.0, iprint=0, singles=True,
+                 ngrid=10, athresh=0.0, fthresh=0.0, quad='lin',
+                 saveT=False, saveL=False, tmem="mem", scratch="",
+                 precision="fp64"):
+        self.T = T
+        self.mu = mu
+        self.finite_T = False if T == 0 else True
+        self.iprint = iprint
+        self.singles = singles
+        self.ngrid = ngrid
+        self.athresh = athresh
+        self.fthresh = fthresh
+        self.quad = quad
+        self.prop = prop
+        self.saveT = saveT
+        self.saveL = saveL
+        self.tmem = tmem
+        self.scratch = scratch
+        self.precision = precision
+        if not sys.verify(self.T, self.mu):
+            raise Exception("Sytem temperature inconsistent with CC temp")
+        self.beta = 1.0 / T if self.finite_T else 80
+        self.ti, self.g, _ = quadrature.ft_quad(self.ngrid, self.beta,
+                                                self.quad)
+        self.sys = sys
+        self.T1 = [None] * ngrid
+        self.T2 = [None] * ngrid
+        self.G0 = self.G1 = self.Gcc = self.Gtot = None
+
+    def run(self, response=None):
+        """Run the propagation; returns (Omega_total, Omega_cc)."""
+        if not self.finite_T:
+            raise NotImplementedError(
+                "zero-temperature TD-CCSD is not part of the GPU path")
+        tprop = self.prop.get("tprop")
+        if tprop not in _ORDERS:
+            raise NotImplementedError(
+                "propagator %r: the explicit rk1, rk2 and rk4 are built "
+                "(kelvin/propagation.py:5-52)" % (tprop,))
+        logging.info('Running CCSD at an electronic temperature of %f K'
+                     % ft_utils.HtoK(self.T))
+        return self._ccsd(_ORDERS[tprop])
+
+    def _active(self, en):
+        """kelvin/td_ccsd.py:17-72 ('g' branch): orbitals with occupation
+        above athresh and vacancy above fthresh, and vice versa."""
+        fo = ft_utils.ff(self.beta, en, self.mu)
+        fv = ft_utils.ffv(self.beta, en, self.mu)
+        iocc = [i for i, x in enumerate(fo)
+                if x > self.athresh and fv[i] > self.fthresh]
+        ivir = [i for i, x in enumerate(fv)
+                if x > self.athresh and fo[i] > self.fthresh]
+        logging.info("FT-CCSD orbital info:")
+        logging.info("  nocc: {:d}".format(len(iocc)))
+        logging.info("  nvir: {:d}".format(len(ivir)))
+        return iocc, ivir
+
+    def _ccsd(self, order):
+        beta, mu = self.beta, self.mu
+        ng, ti, g = self.ngrid, self.ti, self.g
+        en = numpy.asarray(self.sys.g_energies_tot(), dtype=numpy.float64)
+        n = en.shape[0]
+        iocc, ivir = self._active(en)
+        full = len(iocc) == n and len(ivir) == n
+        eng = FTEngine(len(iocc), len(ivir), 1, precision=self.precision)
+        self.engine = eng
+        if hasattr(self.sys, "g_device_integrals"):
+            fock, eri, E1 = self.sys.g_device_integrals(eng)
+        else:
+            fock, eri = self.sys.g_fock_tot(), self.sys.g_aint_tot()
+            E1 = self.sys.get_mp1()
+        G0 = eng.set_system(en, fock, eri, beta, mu,
+                            iocc=None if full else iocc,
+                            ivir=None if full else ivir)
+        del fock, eri
+        E0 = G0 + self.sys.const_energy()
+        E01 = E0 + E1
+        # one state propagated; the grid of the context only sizes it
+        eng.set_grid(numpy.zeros(1), numpy.zeros((1, 1)))
+        T1, T2 = eng.empty_amplitudes()
+        T1.zero_()
+        T2.zero_()
+        Eccn = 0.0
+        if self.saveT:
+            self.T1[0], self.T2[0] = T1[0].clone(), T2[0].clone()
+        for i in range(1, ng):
+            h = ti[i] - ti[i - 1]
+            eng.td_step(T1, T2, h, order=order, singles=self.singles)
+            Eccn += eng.energy(T1, T2, numpy.array([g[i]]), beta, qterm=True)
+            if self.saveT:
+                self.T1[i], self.T2[i] = T1[0].clone(), T2[0].clone()
+        self.G0 = E0
+        self.G1 = E1
+        self.Gcc = Eccn
+        self.Gtot = E0 + E1 + Eccn
+        if not self.saveT:
+            self.T1[ng - 1], self.T2[ng - 1] = T1[0], T2[0]
+        return (Eccn + E01, Eccn)

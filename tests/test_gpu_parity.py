@@ -729,3 +729,79 @@ def test_ft_ucc_iter_golden_G1():
         "CCSD", T1a, T1b, T2aa, T2ab, T2bb, Fa, Fb, Ia, Ib, Iab, *Ds, g, G,
         beta, ng, ti, 0, conv)
     assert abs(E - GOLD["G1"]["Omega_cc"]) < GOLD["G1"]["tol"]
+
+
+# ------------------------------------------------------------ TD-CCSD
+@pytest.mark.parametrize("order", [1, 2, 4])
+@pytest.mark.parametrize("singles", [True, False])
+def test_td_step_matches_oracle(order, singles):
+    """ftcc_td_rhs / ftcc_td_step against the oracle's restatement of
+    kelvin/td_ccsd.py:646-652 and kelvin/propagation.py:5-52 (two
+    independent states, o != v, random non-Hermitian integrals)."""
+    from oracle import td_ccsd as otd
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd.ov_blocks import one_e_blocks, two_e_blocks
+    d = dev()
+    rng = numpy.random.default_rng(7 * order + singles)
+    no, nv, ng = 3, 4, 2
+    F, I = random_problem(no, nv, rng)
+    # denominators of orbital energies (symmetric within each pair, so the
+    # stage inputs T + h k keep the antisymmetry of T2)
+    eo, ev = rng.uniform(-1.0, -0.1, no), rng.uniform(0.1, 1.0, nv)
+    D1 = oracle.D1(ev, eo)
+    D2 = oracle.D2(ev, eo)
+    t1, t2 = random_T(ng, nv, no, rng, 0.1)
+    eng = FTEngine(no, nv, ng)
+    eng.set_blocks(one_e_blocks(**{k: F[k] for k in ('oo', 'ov', 'vo', 'vv')}),
+                   two_e_blocks(**{k: I[k] for k in I}))
+    eng.set_denominators(D1, D2)
+    eng.set_grid(numpy.zeros(ng), numpy.zeros((ng, ng)))
+    T1 = torch.as_tensor(t1, device=d).contiguous()
+    T2 = torch.as_tensor(t2, device=d).contiguous()
+    K1, K2 = torch.empty_like(T1), torch.empty_like(T2)
+    eng.td_rhs(T1, T2, K1, K2, singles=singles)
+    h = 0.07
+    eng.td_step(T1, T2, h, order=order, singles=singles)
+    for y in range(ng):
+        k1, k2 = otd.td_rhs(F, I, D1, D2, t1[y], t2[y], singles)
+        assert numpy.abs(to_np(K1)[y] - k1).max() <= \
+            1e-12 * max(numpy.abs(k1).max(), 1.0)
+        assert rel(to_np(K2)[y], k2) < 1e-12
+        d1, d2 = otd.rk_step(order, h, [t1[y], t2[y]],
+                             lambda v: otd.td_rhs(F, I, D1, D2, v[0], v[1],
+                                                  singles))
+        e1 = t1[y] + d1 if singles else t1[y]
+        assert numpy.abs(to_np(T1)[y] - e1).max() < 1e-13
+        assert rel(to_np(T2)[y], t2[y] + d2) < 1e-12
+
+
+@pytest.mark.parametrize("tprop,ng_td,ng_cc,tol", [("rk1", 380, 10, 1e-3),
+                                                   ("rk2", 60, 20, 1e-4),
+                                                   ("rk4", 60, 70, 1e-6)])
+def test_tdccsd_matches_fixed_point(tprop, ng_td, ng_cc, tol):
+    """kelvin/tests/test_td_ccsd.py:12-75 on the 7 k-point UEG: the
+    propagated Omega_cc agrees with the fixed-point one at the reference's
+    tolerances (rk1 1e-3, rk2 1e-4, rk4 1e-6), and equals the oracle's
+    propagation on the same grid to 1e-10."""
+    from oracle import td_ccsd as otd
+    from kelvin_amd.ccsd import ccsd
+    from kelvin_amd.td_ccsd import TDCCSD
+    dev()
+    sysg, s = _ueg_gold("G1")
+    T, mu = s["T"], s["mu"]
+    ref = ccsd(sysg, T=T, mu=mu, ngrid=ng_cc, damp=0.2, max_iter=80,
+               econv=1e-11)
+    Eref, Eccref = ref.run()
+    td = TDCCSD(sysg, {"tprop": tprop}, T=T, mu=mu, ngrid=ng_td)
+    Eout, Eccout = td.run()
+    assert abs(Eccout - Eccref) < tol
+    assert abs((Eout - Eccout) - (Eref - Eccref)) < 1e-12
+    if tprop != "rk1":
+        beta = 1.0 / T
+        en = sysg.g_energies_tot()
+        F, I = oracle.ft_integrals(sysg.g_fock_tot(), sysg.g_aint_tot(), en,
+                                   beta, mu)
+        ti, g, _ = oracle.ft_quad(ng_td, beta, 'lin')
+        Eo, _, _ = otd.td_ccsd(F, I, oracle.D1(en, en), oracle.D2(en, en),
+                               ti, g, beta, {"rk2": 2, "rk4": 4}[tprop])
+        assert abs(Eccout - Eo) < 1e-10

@@ -275,3 +275,59 @@ def test_pointwise_schedule_reaches_the_same_fixed_point():
     r2 = ftcc.ft_ccsd(u, 0.1, 0.1, rt_iter='point', **kw)
     assert abs(r1["Omega_cc"] - r2["Omega_cc"]) < 1e-8
     assert abs(r2["Omega_cc"] - GOLD["G1"]["Omega_cc"]) < 1e-8
+
+
+# ------------------------------------------------------------ TD-CCSD
+def _linear_td_problem(no, nv, rng):
+    """Interaction-free problem: only F.vo drives T1, so
+    T1(tau) = -F.vo (1 - exp(-D1 tau)) / D1 exactly and T2 stays zero."""
+    F = ftcc.Blocks(oo=numpy.zeros((no, no)), ov=numpy.zeros((no, nv)),
+                    vo=rng.normal(size=(nv, no)), vv=numpy.zeros((nv, nv)))
+    I = ftcc.Blocks({k: numpy.zeros([no if c == 'o' else nv for c in k])
+                     for k in ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo',
+                               'oovv', 'vooo', 'ooov', 'oooo')})
+    D1 = rng.uniform(0.5, 2.0, size=(nv, no))
+    D2 = rng.uniform(0.5, 2.0, size=(nv, nv, no, no))
+    return F, I, D1, D2
+
+
+@pytest.mark.parametrize("order", [1, 2, 4])
+def test_td_propagator_order_on_the_linear_problem(order):
+    """kelvin/propagation.py rk1/rk2/rk4 have global order 1, 2 and 4: the
+    error against the exact solution falls by 2^order when the step halves
+    (a wrong stage coefficient drops the order)."""
+    from oracle import td_ccsd
+    rng = numpy.random.default_rng(order)
+    no, nv = 2, 3
+    F, I, D1, D2 = _linear_td_problem(no, nv, rng)
+    tend = 2.0
+    exact = -F.vo * (1.0 - numpy.exp(-D1 * tend)) / D1
+    errs = []
+    for nstep in (20, 40):
+        ti = numpy.linspace(0.0, tend, nstep + 1)
+        _, t1, t2 = td_ccsd.td_ccsd(F, I, D1, D2, ti, numpy.zeros(nstep + 1),
+                                    1.0, order)
+        assert numpy.abs(t2).max() == 0.0
+        errs.append(numpy.abs(t1 - exact).max())
+    ratio = errs[0] / errs[1]
+    assert 0.8 * 2 ** order < ratio < 1.25 * 2 ** order, (errs, ratio)
+
+
+@pytest.mark.parametrize("order,ng_td,ng_cc,tol", [(2, 60, 20, 1e-4),
+                                                   (4, 60, 70, 1e-6)])
+def test_td_ccsd_matches_fixed_point(order, ng_td, ng_cc, tol):
+    """kelvin/tests/test_td_ccsd.py:33-75: the propagated Omega_cc agrees
+    with the fixed-point FT-CCSD one (rk2 at 1e-4, rk4 at 1e-6), here on the
+    7 k-point UEG of the reference's golden test."""
+    from oracle import td_ccsd
+    T, mu = 0.1, 0.1
+    u = ftcc.UEGSystem(T, 2 * numpy.pi, 1.2, mu, norb=7)
+    beta = 1.0 / T
+    ref = ftcc.ft_ccsd(u, T, mu, ngrid=ng_cc, damp=0.2, max_iter=80,
+                       econv=1e-11)
+    en = u.g_energies_tot()
+    F, I = ftcc.ft_integrals(u.g_fock_tot(), u.g_aint_tot(), en, beta, mu)
+    D1, D2 = ftcc.D1(en, en), ftcc.D2(en, en)
+    ti, g, _ = ftcc.ft_quad(ng_td, beta, 'lin')
+    Ecc, _, _ = td_ccsd.td_ccsd(F, I, D1, D2, ti, g, beta, order)
+    assert abs(Ecc - ref["Omega_cc"]) < tol, (Ecc, ref["Omega_cc"])
