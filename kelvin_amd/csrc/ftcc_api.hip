@@ -456,6 +456,26 @@ void big(ftcc_ctx *c, const Gemm &g, int prec, const SplitMat &A,
   });
 }
 
+struct Range {
+  int start = 0, len = 0, count = 0;
+};
+struct Blk {
+  Range r, k, c;
+};
+
+// classes c1 (and c2, adjacent after c1) of a pair space
+Range group(const PairSpace &p, int c1, int c2 = -1) {
+  Range g;
+  g.start = p.start[c1];
+  g.len = p.len[c1];
+  g.count = p.count[c1];
+  if (c2 >= 0) {
+    g.len += p.len[c2];
+    g.count += p.count[c2];
+  }
+  return g;
+}
+
 // ---------------------------------------------------------------------------
 // Operands of the six O(n^6) products.  Each is a matrix over two pair
 // spaces (rows, K); A operands are stored [M][K], B operands transposed as
@@ -534,24 +554,59 @@ void ensure_b64_buffers(ftcc_ctx *c) {
   c->b64_fixed_valid = false;
 }
 
+// The (row classes, K classes) blocks of operand `id` that its products read
+// (block_list below): ladder operands the three blocks of equal total spin;
+// ring operands the spin-keeping block and the two spin-flip blocks, crossed
+// (flip classes 1 x 2 and 2 x 1) except for G6's B operand Wr, whose K and
+// column classes agree.  Only these blocks are written in the spin layout.
+void operand_blocks(const ftcc_ctx *c, int id, Range rows[3], Range ks[3]) {
+  const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, OP_K[id]);
+  if (OP_ROWS[id] != 2) {
+    rows[0] = group(R, 0), ks[0] = group(K, 0);
+    rows[1] = group(R, 1, 2), ks[1] = group(K, 1, 2);
+    rows[2] = group(R, 3), ks[2] = group(K, 3);
+    return;
+  }
+  rows[0] = group(R, 0, 3), ks[0] = group(K, 0, 3);
+  if (id == OP_WR_B) {
+    rows[1] = group(R, 1), ks[1] = group(K, 1);
+    rows[2] = group(R, 2), ks[2] = group(K, 2);
+  } else {
+    rows[1] = group(R, 1), ks[1] = group(K, 2);
+    rows[2] = group(R, 2), ks[2] = group(K, 1);
+  }
+}
+
 // Write operand `id` of nb slots from its FP64 source: src[b*sb + row*sr +
 // k*sc] with row, k flat pair indices of the dense layout.
 void put_operand(ftcc_ctx *c, int id, const double *src, int64_t sr,
                  int64_t sc, int64_t sb, int nb) {
   hipStream_t s = c->stream;
-  if (c->prec == FTCC_PREC_BF16X3) {
-    SplitMat &m = split_of(c, id);
-    if (c->spin) {
-      const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, OP_K[id]);
-      split_gather(src, sr, sc, sb, R.d_map, R.total, K.d_map, m, nb, s);
-    } else {
+  if (!c->spin) {
+    if (c->prec == FTCC_PREC_BF16X3)
       split_convert(src, sr, sc, sb, space_len(c, OP_ROWS[id]),
-                    space_len(c, OP_K[id]), m, nb, s);
+                    space_len(c, OP_K[id]), split_of(c, id), nb, s);
+    return;
+  }
+  const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, OP_K[id]);
+  Range rows[3], ks[3];
+  operand_blocks(c, id, rows, ks);
+  for (int q = 0; q < 3; ++q) {
+    if (rows[q].len == 0 || ks[q].len == 0) continue;
+    const int *rmap = R.d_map + rows[q].start, *kmap = K.d_map + ks[q].start;
+    if (c->prec == FTCC_PREC_BF16X3) {
+      SplitMat m = split_of(c, id);
+      const int64_t off = (int64_t)rows[q].start * m.ld + ks[q].start;
+      m.hi += off;
+      m.lo += off;
+      split_gather(src, sr, sc, sb, rmap, rows[q].len, kmap, ks[q].len, m, nb,
+                   s);
+    } else {
+      const int64_t ld = K.total;
+      gather64(src, sr, sc, sb, rmap, rows[q].len, kmap, ks[q].len, ld,
+               c->b64[id] + (int64_t)rows[q].start * ld + ks[q].start,
+               (int64_t)R.total * ld, nb, s);
     }
-  } else if (c->spin) {
-    const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, OP_K[id]);
-    gather64(src, sr, sc, sb, R.d_map, R.total, K.d_map, K.total, c->b64[id],
-             (int64_t)R.total * K.total, nb, s);
   }
 }
 
@@ -613,26 +668,6 @@ PairSpace build_pairs(ftcc_ctx *c, const std::vector<int> &lx,
 // space ((a,i): 00 11 | 01 | 10).
 constexpr int LADDER_ORDER[4] = {0, 1, 2, 3};
 constexpr int RING_ORDER[4] = {0, 3, 1, 2};
-
-struct Range {
-  int start = 0, len = 0, count = 0;
-};
-struct Blk {
-  Range r, k, c;
-};
-
-// classes c1 (and c2, adjacent after c1) of a pair space
-Range group(const PairSpace &p, int c1, int c2 = -1) {
-  Range g;
-  g.start = p.start[c1];
-  g.len = p.len[c1];
-  g.count = p.count[c1];
-  if (c2 >= 0) {
-    g.len += p.len[c2];
-    g.count += p.count[c2];
-  }
-  return g;
-}
 
 // The three nonzero blocks of a product C[R,Cs] = A[R,K] B[K,Cs].
 //   ladder products (G1-G4): every operand conserves total spin of its pairs;

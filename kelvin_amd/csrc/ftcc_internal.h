@@ -103,14 +103,14 @@ int64_t split_ld(int K);
 void split_convert(const double *src, int64_t sr, int64_t sc, int64_t sb,
                    int R, int C, const SplitMat &d, int nb, hipStream_t s);
 // d[b][r][c] = src[b*sb + rmap[r]*sr + cmap[c]*sc] split into (hi, lo) for
-// r < R, c < d.ld; entries whose map is -1 are zero
+// r < R, c < W (row stride d.ld); entries whose map is -1 are zero
 void split_gather(const double *src, int64_t sr, int64_t sc, int64_t sb,
-                  const int *rmap, int R, const int *cmap, const SplitMat &d,
-                  int nb, hipStream_t s);
-// the same gather into FP64: dst[b*db + r*ld + c], c < ld
+                  const int *rmap, int R, const int *cmap, int W,
+                  const SplitMat &d, int nb, hipStream_t s);
+// the same gather into FP64: dst[b*db + r*os + c], c < W
 void gather64(const double *src, int64_t sr, int64_t sc, int64_t sb,
-              const int *rmap, int R, const int *cmap, int ld, double *dst,
-              int64_t db, int nb, hipStream_t s);
+              const int *rmap, int R, const int *cmap, int W, int64_t os,
+              double *dst, int64_t db, int nb, hipStream_t s);
 // C[b] = alpha * A[b] . B[b]^T + beta * C[b] with A as [M][K], B as [N][K];
 // output row m goes to row rmap[m] of C and column n to cmap[n] when the
 // maps are given (-1 drops the element)

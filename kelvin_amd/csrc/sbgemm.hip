@@ -232,25 +232,25 @@ __device__ inline int src_index(const int *map, int i, int n) {
   return map ? map[i] : (i < n ? i : -1);
 }
 
-// dst[b][r][c] = src[b*sb + R(r)*sr + C(c)*sc] for r < R, c < ld, zero where
-// a map gives -1 (or, without maps, for c >= C); threads run along c (use
-// when sc is the small stride).
+// dst[b][r][c] (row stride os) = src[b*sb + R(r)*sr + C(c)*sc] for r < R,
+// c < W, zero where a map gives -1 (or, without maps, for c >= C); threads
+// run along c (use when sc is the small stride).
 template <typename Out>
 __global__ void gather_rows_kernel(const double *src, int64_t sr, int64_t sc,
                                    int64_t sb, const int *rmap, int R,
-                                   const int *cmap, int C, int64_t ld,
+                                   const int *cmap, int C, int W, int64_t os,
                                    int64_t db, Out out, int nb) {
-  const int64_t per = (int64_t)R * ld;
+  const int64_t per = (int64_t)R * W;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
        idx < per * nb; idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = idx / per, e = idx % per;
-    const int r = (int)(e / ld);
-    const int c = (int)(e % ld);
+    const int r = (int)(e / W);
+    const int c = (int)(e % W);
     const int rr = src_index(rmap, r, R), cc = src_index(cmap, c, C);
     const double v = (rr >= 0 && cc >= 0)
                          ? src[b * sb + (int64_t)rr * sr + (int64_t)cc * sc]
                          : 0.0;
-    out(b * db + e, v);
+    out(b * db + (int64_t)r * os + c, v);
   }
 }
 
@@ -260,8 +260,8 @@ template <typename Out>
 __global__ void gather_transpose_kernel(const double *src, int64_t sr,
                                         int64_t sc, int64_t sb,
                                         const int *rmap, int R,
-                                        const int *cmap, int C, int64_t ld,
-                                        int64_t db, Out out) {
+                                        const int *cmap, int C, int W,
+                                        int64_t os, int64_t db, Out out) {
   __shared__ double tile[64][65];
   const int b = blockIdx.z;
   const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
@@ -269,7 +269,7 @@ __global__ void gather_transpose_kernel(const double *src, int64_t sr,
   for (int i = ty; i < 64; i += 4) {
     const int r = r0 + tx, c = c0 + i;
     const int rr = r < R ? src_index(rmap, r, R) : -1;
-    const int cc = c < ld ? src_index(cmap, c, C) : -1;
+    const int cc = c < W ? src_index(cmap, c, C) : -1;
     tile[i][tx] = (rr >= 0 && cc >= 0)
                       ? src[b * sb + (int64_t)rr * sr + (int64_t)cc * sc]
                       : 0.0;
@@ -277,25 +277,25 @@ __global__ void gather_transpose_kernel(const double *src, int64_t sr,
   __syncthreads();
   for (int i = ty; i < 64; i += 4) {
     const int r = r0 + i, c = c0 + tx;
-    if (r < R && c < ld) out(b * db + (int64_t)r * ld + c, tile[tx][i]);
+    if (r < R && c < W) out(b * db + (int64_t)r * os + c, tile[tx][i]);
   }
 }
 
 template <typename Out>
 void gather_launch(const double *src, int64_t sr, int64_t sc, int64_t sb,
-                   const int *rmap, int R, const int *cmap, int C, int64_t ld,
-                   int64_t db, Out out, int nb, hipStream_t s) {
-  if ((int64_t)R * ld == 0 || nb <= 0) return;
+                   const int *rmap, int R, const int *cmap, int C, int W,
+                   int64_t os, int64_t db, Out out, int nb, hipStream_t s) {
+  if ((int64_t)R * W == 0 || nb <= 0) return;
   if (sr == 1 && sc != 1) {
-    const int gx = (int)((ld + 63) / 64), gy = (R + 63) / 64;
+    const int gx = (W + 63) / 64, gy = (R + 63) / 64;
     hipLaunchKernelGGL(gather_transpose_kernel<Out>, dim3(gx, gy, nb),
-                       dim3(256), 0, s, src, sr, sc, sb, rmap, R, cmap, C, ld,
-                       db, out);
+                       dim3(256), 0, s, src, sr, sc, sb, rmap, R, cmap, C, W,
+                       os, db, out);
   } else {
-    const int64_t total = (int64_t)R * ld * nb;
+    const int64_t total = (int64_t)R * W * nb;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
     hipLaunchKernelGGL(gather_rows_kernel<Out>, dim3(blocks), dim3(256), 0, s,
-                       src, sr, sc, sb, rmap, R, cmap, C, ld, db, out, nb);
+                       src, sr, sc, sb, rmap, R, cmap, C, W, os, db, out, nb);
   }
   FTCC_CHECK_HIP(hipGetLastError());
 }
@@ -308,22 +308,22 @@ void split_convert(const double *src, int64_t sr, int64_t sc, int64_t sb,
                    int R, int C, const SplitMat &d, int nb, hipStream_t s) {
   require(d.ld >= C && d.ld % 32 == 0, "split operand leading dimension");
   if ((int64_t)R * C == 0) return;
-  gather_launch(src, sr, sc, sb, nullptr, R, nullptr, C, d.ld, d.bstride,
-                SplitOut{d.hi, d.lo}, nb, s);
+  gather_launch(src, sr, sc, sb, nullptr, R, nullptr, C, (int)d.ld, d.ld,
+                d.bstride, SplitOut{d.hi, d.lo}, nb, s);
 }
 
 void split_gather(const double *src, int64_t sr, int64_t sc, int64_t sb,
-                  const int *rmap, int R, const int *cmap, const SplitMat &d,
-                  int nb, hipStream_t s) {
-  require(d.ld % 32 == 0, "split operand leading dimension");
-  gather_launch(src, sr, sc, sb, rmap, R, cmap, (int)d.ld, d.ld, d.bstride,
+                  const int *rmap, int R, const int *cmap, int W,
+                  const SplitMat &d, int nb, hipStream_t s) {
+  require(d.ld % 32 == 0 && W <= d.ld, "split operand leading dimension");
+  gather_launch(src, sr, sc, sb, rmap, R, cmap, W, W, d.ld, d.bstride,
                 SplitOut{d.hi, d.lo}, nb, s);
 }
 
 void gather64(const double *src, int64_t sr, int64_t sc, int64_t sb,
-              const int *rmap, int R, const int *cmap, int ld, double *dst,
-              int64_t db, int nb, hipStream_t s) {
-  gather_launch(src, sr, sc, sb, rmap, R, cmap, ld, ld, db, F64Out{dst}, nb,
+              const int *rmap, int R, const int *cmap, int W, int64_t os,
+              double *dst, int64_t db, int nb, hipStream_t s) {
+  gather_launch(src, sr, sc, sb, rmap, R, cmap, W, W, os, db, F64Out{dst}, nb,
                 s);
 }
 
