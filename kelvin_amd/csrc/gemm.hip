@@ -34,7 +34,18 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef double d2_t __attribute__((ext_vector_type(2)));
 
-constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+constexpr int BK = 16, NT = 256;
+
+// Output tile: 128x128, or 64 on the side of a dimension of at most 64
+// (products over one spin run of an orbital index)
+struct Tile {
+  int m, n;
+};
+inline Tile pick_tile(int M, int N) {
+  if (M <= 64 && N > 64) return {64, 128};
+  if (N <= 64 && M > 64) return {128, 64};
+  return {128, 128};
+}
 
 struct GemmDev {
   int M, N, K, nb1, nb2, splits, kchunk, tilesM, tilesN;
@@ -115,59 +126,64 @@ __device__ inline BlockPos decode(const GemmDev &p) {
 // Element offsets of the tile's rows of A and columns of B, one table entry
 // per row/column of the tile (-1 past the end), kept in LDS so they hold no
 // registers through the K loop.
+template <int TM, int TN>
 __device__ inline void tile_offsets(const GemmDev &p, int m0, int n0,
                                     int64_t *aoff, int64_t *boff) {
   const int t = threadIdx.x;
-  if (t < BM) {
+  if (t < TM) {
     const int m = m0 + t;
     aoff[t] = m < p.M ? at(levels(m, p.M1, p.M2), p.sAm0, p.sAm1, p.sAm) : -1;
-  } else if (t < BM + BN) {
-    const int n = n0 + t - BM;
-    boff[t - BM] =
+  } else if (t < TM + TN) {
+    const int n = n0 + t - TM;
+    boff[t - TM] =
         n < p.N ? at(levels(n, p.N1, p.N2), p.sBn0, p.sBn1, p.sBn) : -1;
   }
 }
 
-template <typename T, bool A_MFAST, bool B_NFAST>
+// One K slab (16 deep) of the TM-row A tile and the TN-column B tile: each
+// of the 256 threads loads TM/16 values of A and TN/16 of B.  Along the
+// contiguous index: m = t % TM, k = t / TM + (256 / TM) r; otherwise
+// m = t / 16 + 16 r, k = t % 16.
+template <typename T, bool A_MFAST, bool B_NFAST, int TM, int TN>
 __device__ inline void load_slab(const GemmDev &p, const double *A,
                                  const double *B, const int64_t *aoff,
                                  const int64_t *boff, int k0, int kend,
-                                 T ra[8], T rb[8]) {
+                                 T ra[TM / 16], T rb[TN / 16]) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int64_t off = aoff[A_MFAST ? (t & 127) : (t >> 4) + 16 * r];
-    const int k = A_MFAST ? k0 + (t >> 7) + 2 * r : k0 + (t & 15);
+  for (int r = 0; r < TM / 16; ++r) {
+    const int64_t off = aoff[A_MFAST ? (t % TM) : (t >> 4) + 16 * r];
+    const int k = A_MFAST ? k0 + t / TM + (NT / TM) * r : k0 + (t & 15);
     double v = 0.0;
     if (off >= 0 && k < kend) v = A[off + (int64_t)k * p.sAk];
     ra[r] = (T)v;
   }
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int64_t off = boff[B_NFAST ? (t & 127) : (t >> 4) + 16 * r];
-    const int k = B_NFAST ? k0 + (t >> 7) + 2 * r : k0 + (t & 15);
+  for (int r = 0; r < TN / 16; ++r) {
+    const int64_t off = boff[B_NFAST ? (t % TN) : (t >> 4) + 16 * r];
+    const int k = B_NFAST ? k0 + t / TN + (NT / TN) * r : k0 + (t & 15);
     double v = 0.0;
     if (off >= 0 && k < kend) v = B[(int64_t)k * p.sBk + off];
     rb[r] = (T)v;
   }
 }
 
-template <typename T, bool A_MFAST, bool B_NFAST>
-__device__ inline void store_slab(T *As, T *Bs, const T ra[8],
-                                  const T rb[8]) {
+template <typename T, bool A_MFAST, bool B_NFAST, int TM, int TN>
+__device__ inline void store_slab(T *As, T *Bs, const T ra[TM / 16],
+                                  const T rb[TN / 16]) {
   constexpr int LDK = Mma<T>::LDK;
   const int t = threadIdx.x;
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
+  for (int r = 0; r < TM / 16; ++r) {
     if (A_MFAST)
-      As[(t & 127) * LDK + (t >> 7) + 2 * r] = ra[r];
+      As[(t % TM) * LDK + t / TM + (NT / TM) * r] = ra[r];
     else
       As[((t >> 4) + 16 * r) * LDK + (t & 15)] = ra[r];
   }
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
+  for (int r = 0; r < TN / 16; ++r) {
     if (B_NFAST)
-      Bs[(t & 127) * LDK + (t >> 7) + 2 * r] = rb[r];
+      Bs[(t % TN) * LDK + t / TN + (NT / TN) * r] = rb[r];
     else
       Bs[((t >> 4) + 16 * r) * LDK + (t & 15)] = rb[r];
   }
@@ -179,18 +195,22 @@ __device__ inline void load4(const T *src, T out[4]) {
   for (int s = 0; s < 4; ++s) out[s] = src[s];
 }
 
-template <typename T, bool A_MFAST, bool B_NFAST>
+// TM x TN output tile per workgroup (128x128, 64x128 or 128x64): the four
+// waves form a 2x2 grid, each owning (TM/2) x (TN/2) as RT x CT MFMA blocks.
+template <typename T, bool A_MFAST, bool B_NFAST, int TM, int TN>
 __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   using acc_t = typename Mma<T>::acc_t;
   constexpr int LDK = Mma<T>::LDK;
-  __shared__ T As[2][BM * LDK];
-  __shared__ T Bs[2][BN * LDK];
-  __shared__ int64_t aoff[BM], boff[BN], drow[BM], dcol[BN];
+  constexpr int RT = TM / 32, CT = TN / 32;
+  static_assert(TM + TN <= NT, "one offset-table entry per thread");
+  __shared__ T As[2][TM * LDK];
+  __shared__ T Bs[2][TN * LDK];
+  __shared__ int64_t aoff[TM], boff[TN], drow[TM], dcol[TN];
 
   const BlockPos q = decode(p);
   const double *A = p.A + q.b1 * p.sA1 + q.b2 * p.sA2;
   const double *B = p.B + q.b1 * p.sB1 + q.b2 * p.sB2;
-  const int m0 = q.tm * BM, n0 = q.tn * BN;
+  const int m0 = q.tm * TM, n0 = q.tn * TN;
   const int kbeg = q.ks * p.kchunk;
   const int kend = min(p.K, kbeg + p.kchunk);
   const int nslab = (kend - kbeg + BK - 1) / BK;
@@ -199,41 +219,46 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   const int wm = wave >> 1, wn = wave & 1;
   const int li = lane & 15, lg = lane >> 4;
 
-  acc_t acc[4][4];
+  acc_t acc[RT][CT];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RT; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < CT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  T ra[8], rb[8];
-  tile_offsets(p, m0, n0, aoff, boff);
+  T ra[TM / 16], rb[TN / 16];
+  tile_offsets<TM, TN>(p, m0, n0, aoff, boff);
   __syncthreads();
   if (nslab > 0) {
-    load_slab<T, A_MFAST, B_NFAST>(p, A, B, aoff, boff, kbeg, kend, ra, rb);
-    store_slab<T, A_MFAST, B_NFAST>(As[0], Bs[0], ra, rb);
+    load_slab<T, A_MFAST, B_NFAST, TM, TN>(p, A, B, aoff, boff, kbeg, kend,
+                                           ra, rb);
+    store_slab<T, A_MFAST, B_NFAST, TM, TN>(As[0], Bs[0], ra, rb);
   }
   __syncthreads();
   for (int s = 0; s < nslab; ++s) {
     const int buf = s & 1;
     if (s + 1 < nslab)
-      load_slab<T, A_MFAST, B_NFAST>(p, A, B, aoff, boff, kbeg + (s + 1) * BK,
-                                     kend, ra, rb);
-    T a[4][4], b[4][4];
+      load_slab<T, A_MFAST, B_NFAST, TM, TN>(p, A, B, aoff, boff,
+                                             kbeg + (s + 1) * BK, kend, ra,
+                                             rb);
+    T a[RT][4], b[CT][4];
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-      load4<T>(&As[buf][(wm * 64 + rt * 16 + li) * LDK + 4 * lg], a[rt]);
+    for (int rt = 0; rt < RT; ++rt)
+      load4<T>(&As[buf][(wm * (TM / 2) + rt * 16 + li) * LDK + 4 * lg],
+               a[rt]);
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
-      load4<T>(&Bs[buf][(wn * 64 + ct * 16 + li) * LDK + 4 * lg], b[ct]);
+    for (int ct = 0; ct < CT; ++ct)
+      load4<T>(&Bs[buf][(wn * (TN / 2) + ct * 16 + li) * LDK + 4 * lg],
+               b[ct]);
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
-      for (int rt = 0; rt < 4; ++rt)
+      for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct)
+        for (int ct = 0; ct < CT; ++ct)
           acc[rt][ct] = Mma<T>::op(a[rt][kk], b[ct][kk], acc[rt][ct]);
     if (s + 1 < nslab)
-      store_slab<T, A_MFAST, B_NFAST>(As[buf ^ 1], Bs[buf ^ 1], ra, rb);
+      store_slab<T, A_MFAST, B_NFAST, TM, TN>(As[buf ^ 1], Bs[buf ^ 1], ra,
+                                              rb);
     __syncthreads();
   }
 
@@ -248,18 +273,18 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   int64_t *crow = aoff, *ccol = boff;
   if (direct) {
     const int t = threadIdx.x;
-    if (t < BM) {
+    if (t < TM) {
       const int row = m0 + t;
       const int rr = row < p.M ? (p.rmap ? p.rmap[row] : row) : -1;
       const Lvl l = levels(rr < 0 ? 0 : rr, p.M1, p.M2);
       crow[t] = rr < 0 ? -1 : at(l, p.sCm0, p.sCm1, p.sCm);
       drow[t] = rr < 0 ? -1 : at(l, p.sDm0, p.sDm1, p.sDm);
-    } else if (t < BM + BN) {
-      const int col = n0 + t - BM;
+    } else if (t < TM + TN) {
+      const int col = n0 + t - TM;
       const int cc = col < p.N ? (p.cmap ? p.cmap[col] : col) : -1;
       const Lvl l = levels(cc < 0 ? 0 : cc, p.N1, p.N2);
-      ccol[t - BM] = cc < 0 ? -1 : at(l, p.sCn0, p.sCn1, p.sCn);
-      dcol[t - BM] = cc < 0 ? -1 : at(l, p.sDn0, p.sDn1, p.sDn);
+      ccol[t - TM] = cc < 0 ? -1 : at(l, p.sCn0, p.sCn1, p.sCn);
+      dcol[t - TM] = cc < 0 ? -1 : at(l, p.sDn0, p.sDn1, p.sDn);
     }
     __syncthreads();
   }
@@ -269,13 +294,13 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
                      : p.partial + ((int64_t)rsel * p.nbC + cb) *
                                        (int64_t)p.M * p.N;
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
+  for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
-      const int lc = wn * 64 + ct * 16 + li;
+    for (int ct = 0; ct < CT; ++ct) {
+      const int lc = wn * (TN / 2) + ct * 16 + li;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int lr = wm * 64 + rt * 16 + Mma<T>::crow(lg, v);
+        const int lr = wm * (TM / 2) + rt * 16 + Mma<T>::crow(lg, v);
         const double x = (double)acc[rt][ct][v];
         if (direct) {
           const int64_t oc = crow[lr], occ = ccol[lc];
@@ -347,10 +372,27 @@ void launch(K3 kernel, const GemmDev &d, hipStream_t s) {
   FTCC_CHECK_HIP(hipGetLastError());
 }
 
+template <typename T, int TM, int TN>
+void launch_tile(bool amf, bool bnf, const GemmDev &d, hipStream_t s) {
+  if (amf && bnf) launch(gemm_mfma_kernel<T, true, true, TM, TN>, d, s);
+  else if (amf) launch(gemm_mfma_kernel<T, true, false, TM, TN>, d, s);
+  else if (bnf) launch(gemm_mfma_kernel<T, false, true, TM, TN>, d, s);
+  else launch(gemm_mfma_kernel<T, false, false, TM, TN>, d, s);
+}
+
+template <typename T>
+void launch_layout(Tile tl, bool amf, bool bnf, const GemmDev &d,
+                   hipStream_t s) {
+  if (tl.m == 64) launch_tile<T, 64, 128>(amf, bnf, d, s);
+  else if (tl.n == 64) launch_tile<T, 128, 64>(amf, bnf, d, s);
+  else launch_tile<T, 128, 128>(amf, bnf, d, s);
+}
+
 }  // namespace
 
 size_t gemm_partial_elems(const Gemm &g) {
-  const int tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+  const Tile tl = pick_tile(g.M, g.N);
+  const int tiles = ((g.M + tl.m - 1) / tl.m) * ((g.N + tl.n - 1) / tl.n);
   const int bk = BK;  // conservative: smallest slab gives the most splits
   const int splits = choose_splits(g, tiles, bk);
   const int R = splits * (g.reduce_b2 ? g.nb2 : 1);
@@ -388,8 +430,9 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   d.sCn = g.sCn;
   d.sC1 = g.sC1;
   d.sC2 = g.sC2;
-  d.tilesM = (g.M + BM - 1) / BM;
-  d.tilesN = (g.N + BN - 1) / BN;
+  const Tile tl = pick_tile(g.M, g.N);
+  d.tilesM = (g.M + tl.m - 1) / tl.m;
+  d.tilesN = (g.N + tl.n - 1) / tl.n;
   if (g.K <= 0) {
     // C = beta*C only
     d.splits = 1;
@@ -445,17 +488,10 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   }
   const bool amf = (g.sAm == 1 && g.sAk != 1);
   const bool bnf = (g.sBn == 1 && g.sBk != 1);
-  if (precision == FTCC_PREC_FP64) {
-    if (amf && bnf) launch(gemm_mfma_kernel<double, true, true>, d, s);
-    else if (amf) launch(gemm_mfma_kernel<double, true, false>, d, s);
-    else if (bnf) launch(gemm_mfma_kernel<double, false, true>, d, s);
-    else launch(gemm_mfma_kernel<double, false, false>, d, s);
-  } else if (precision == FTCC_PREC_FP32) {
-    if (amf && bnf) launch(gemm_mfma_kernel<float, true, true>, d, s);
-    else if (amf) launch(gemm_mfma_kernel<float, true, false>, d, s);
-    else if (bnf) launch(gemm_mfma_kernel<float, false, true>, d, s);
-    else launch(gemm_mfma_kernel<float, false, false>, d, s);
-  }
+  if (precision == FTCC_PREC_FP64)
+    launch_layout<double>(tl, amf, bnf, d, s);
+  else
+    launch_layout<float>(tl, amf, bnf, d, s);
   if (R > 1) {
     const int64_t total = (int64_t)d.nbC * g.M * g.N;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
