@@ -202,9 +202,14 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   using acc_t = typename Mma<T>::acc_t;
   constexpr int LDK = Mma<T>::LDK;
   constexpr int RT = TM / 32, CT = TN / 32;
+  // the 128x128 tile is register-bound at two workgroups per CU and double
+  // buffers its slabs; the narrow tiles single-buffer so that three
+  // workgroups fit the LDS (two barriers per slab)
+  constexpr bool DB = (TM == 128 && TN == 128);
+  constexpr int NBUF = DB ? 2 : 1;
   static_assert(TM + TN <= NT, "one offset-table entry per thread");
-  __shared__ T As[2][TM * LDK];
-  __shared__ T Bs[2][TN * LDK];
+  __shared__ T As[NBUF][TM * LDK];
+  __shared__ T Bs[NBUF][TN * LDK];
   __shared__ int64_t aoff[TM], boff[TN], drow[TM], dcol[TN];
 
   const BlockPos q = decode(p);
@@ -235,7 +240,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   }
   __syncthreads();
   for (int s = 0; s < nslab; ++s) {
-    const int buf = s & 1;
+    const int buf = DB ? (s & 1) : 0;
     if (s + 1 < nslab)
       load_slab<T, A_MFAST, B_NFAST, TM, TN>(p, A, B, aoff, boff,
                                              kbeg + (s + 1) * BK, kend, ra,
@@ -249,6 +254,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
     for (int ct = 0; ct < CT; ++ct)
       load4<T>(&Bs[buf][(wn * (TN / 2) + ct * 16 + li) * LDK + 4 * lg],
                b[ct]);
+    // single buffer: every wave has its fragments before the slab is
+    // overwritten
+    if (!DB) __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
@@ -257,8 +265,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
         for (int ct = 0; ct < CT; ++ct)
           acc[rt][ct] = Mma<T>::op(a[rt][kk], b[ct][kk], acc[rt][ct]);
     if (s + 1 < nslab)
-      store_slab<T, A_MFAST, B_NFAST, TM, TN>(As[buf ^ 1], Bs[buf ^ 1], ra,
-                                              rb);
+      store_slab<T, A_MFAST, B_NFAST, TM, TN>(As[DB ? buf ^ 1 : 0],
+                                              Bs[DB ? buf ^ 1 : 0], ra, rb);
     __syncthreads();
   }
 
