@@ -94,6 +94,11 @@ struct ftcc_ctx {
   int o_start[2] = {0, 0}, o_count[2] = {0, 0};
   int v_start[2] = {0, 0}, v_count[2] = {0, 0};
   PairSpace ps_vv, ps_oo, ps_vo;   // (a,b), (i,j) and ring (a,i) pairs
+  // Ladder products contract over an antisymmetric pair, so they sum over
+  // the packed pairs only (e < f within a spin, alpha-beta once) and double
+  // the result: packed (e,f) and (m,n) spaces, used for their K index
+  int pack = 0;
+  PairSpace ps_vvp, ps_oop;
   double *b64[10] = {};            // FP64 blocked operands (fp64/fp32 modes)
   int b64_layout = -1;
   bool b64_fixed_valid = false;
@@ -491,8 +496,22 @@ constexpr bool OP_FIXED[NOPS] = {false, false, false, false, false,
                                  false, false, true,  true,  true};
 
 const PairSpace &space(const ftcc_ctx *c, int sp) {
-  return sp == 0 ? c->ps_vv : sp == 1 ? c->ps_oo : c->ps_vo;
+  return sp == 0   ? c->ps_vv
+         : sp == 1 ? c->ps_oo
+         : sp == 2 ? c->ps_vo
+         : sp == 3 ? c->ps_vvp
+                   : c->ps_oop;
 }
+
+// pair space of operand `id`'s K index: the packed ladder spaces (3, 4)
+// when packing is on
+int op_k(const ftcc_ctx *c, int id) {
+  return (c->spin && c->pack && OP_K[id] < 2) ? OP_K[id] + 3 : OP_K[id];
+}
+
+// layout of the product operands: 0 dense, 1 spin blocks, 2 spin blocks
+// with packed ladder K
+int op_layout(const ftcc_ctx *c) { return c->spin ? (c->pack ? 2 : 1) : 0; }
 
 // length of a pair space in the current layout
 int space_len(const ftcc_ctx *c, int sp) {
@@ -512,13 +531,14 @@ SplitMat &split_of(ftcc_ctx *c, int id) {
 // elements of one array of a split operand (hi or lo), all slots
 int64_t split_elems(const ftcc_ctx *c, int id) {
   const int64_t rows = space_len(c, OP_ROWS[id]);
-  return rows * split_ld(space_len(c, OP_K[id])) * (OP_FIXED[id] ? 1 : c->nb);
+  return rows * split_ld(space_len(c, op_k(c, id))) *
+         (OP_FIXED[id] ? 1 : c->nb);
 }
 
 // Split-bf16 operand buffers for the current layout (allocated on first use
 // of bf16x3, reallocated when the spin layout changes).
 void ensure_split_buffers(ftcc_ctx *c) {
-  if (c->split_layout == c->spin) return;
+  if (c->split_layout == op_layout(c)) return;
   for (int id = 0; id < NOPS; ++id) {
     SplitMat &m = split_of(c, id);
     dfree(c, m.hi);
@@ -527,7 +547,7 @@ void ensure_split_buffers(ftcc_ctx *c) {
   }
   for (int id = 0; id < NOPS; ++id) {
     SplitMat &m = split_of(c, id);
-    m.ld = split_ld(space_len(c, OP_K[id]));
+    m.ld = split_ld(space_len(c, op_k(c, id)));
     const int64_t rows = space_len(c, OP_ROWS[id]);
     m.bstride = OP_FIXED[id] ? 0 : rows * m.ld;
     const int64_t elems = split_elems(c, id);
@@ -535,22 +555,22 @@ void ensure_split_buffers(ftcc_ctx *c) {
     m.hi = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
     m.lo = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
   }
-  c->split_layout = c->spin;
+  c->split_layout = op_layout(c);
   c->split_fixed_valid = false;
 }
 
 // FP64 spin-blocked operand buffers (FP64 and FP32 modes)
 void ensure_b64_buffers(ftcc_ctx *c) {
-  if (c->b64_layout == c->spin && c->b64[0]) return;
+  if (c->b64_layout == op_layout(c) && c->b64[0]) return;
   for (int id = 0; id < NOPS; ++id) {
     dfree(c, c->b64[id]);
     c->b64[id] = nullptr;
   }
   for (int id = 0; id < NOPS; ++id)
     c->b64[id] = dalloc(c, (int64_t)space_len(c, OP_ROWS[id]) *
-                               space_len(c, OP_K[id]) *
+                               space_len(c, op_k(c, id)) *
                                (OP_FIXED[id] ? 1 : c->nb));
-  c->b64_layout = c->spin;
+  c->b64_layout = op_layout(c);
   c->b64_fixed_valid = false;
 }
 
@@ -560,7 +580,7 @@ void ensure_b64_buffers(ftcc_ctx *c) {
 // (flip classes 1 x 2 and 2 x 1) except for G6's B operand Wr, whose K and
 // column classes agree.  Only these blocks are written in the spin layout.
 void operand_blocks(const ftcc_ctx *c, int id, Range rows[3], Range ks[3]) {
-  const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, OP_K[id]);
+  const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, op_k(c, id));
   if (OP_ROWS[id] != 2) {
     rows[0] = group(R, 0), ks[0] = group(K, 0);
     rows[1] = group(R, 1, 2), ks[1] = group(K, 1, 2);
@@ -585,10 +605,10 @@ void put_operand(ftcc_ctx *c, int id, const double *src, int64_t sr,
   if (!c->spin) {
     if (c->prec == FTCC_PREC_BF16X3)
       split_convert(src, sr, sc, sb, space_len(c, OP_ROWS[id]),
-                    space_len(c, OP_K[id]), split_of(c, id), nb, s);
+                    space_len(c, op_k(c, id)), split_of(c, id), nb, s);
     return;
   }
-  const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, OP_K[id]);
+  const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, op_k(c, id));
   Range rows[3], ks[3];
   operand_blocks(c, id, rows, ks);
   for (int q = 0; q < 3; ++q) {
@@ -632,8 +652,12 @@ void ensure_fixed_operands(ftcc_ctx *c) {
 
 // Pairs of (lx, ly) grouped by class 2*lx + ly in `order`, each class padded
 // to a multiple of 256 (the product tile).
+// packed (lx and ly the same list): only x < y within a class of equal
+// spins, class 01 whole and class 10 empty -- one pair of each
+// antisymmetric couple.
 PairSpace build_pairs(ftcc_ctx *c, const std::vector<int> &lx,
-                      const std::vector<int> &ly, const int order[4]) {
+                      const std::vector<int> &ly, const int order[4],
+                      bool packed = false) {
   PairSpace ps;
   std::vector<int> map;
   const int ny = (int)ly.size();
@@ -642,8 +666,8 @@ PairSpace build_pairs(ftcc_ctx *c, const std::vector<int> &lx,
     ps.start[cls] = (int)map.size();
     int cnt = 0;
     for (int x = 0; x < (int)lx.size(); ++x) {
-      if (lx[x] != sx) continue;
-      for (int y = 0; y < ny; ++y)
+      if (lx[x] != sx || (packed && sx > sy)) continue;
+      for (int y = packed && sx == sy ? x + 1 : 0; y < ny; ++y)
         if (ly[y] == sy) {
           map.push_back(x * ny + y);
           ++cnt;
@@ -703,10 +727,13 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     big(c, g, P, split_of(c, ia), split_of(c, ib));
     return;
   }
-  const PairSpace &R = space(c, OP_ROWS[ia]), &K = space(c, OP_K[ia]);
+  const PairSpace &R = space(c, OP_ROWS[ia]), &K = space(c, op_k(c, ia));
   const PairSpace &Cs = space(c, OP_ROWS[ib]);
   Blk blk[3];
   block_list(kind, R, K, Cs, blk);
+  // a packed ladder K sums one pair of each antisymmetric couple: twice
+  const double alpha = (kind == LADDER && op_k(c, ia) >= 3) ? 2.0 * g.alpha
+                                                            : g.alpha;
   if (P == FTCC_PREC_BF16X3) {
     // the three blocks write disjoint parts of C: one grouped launch, the
     // largest block first (its tiles are the longest)
@@ -738,7 +765,7 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     });
     if (np == 0) return;
     timed(c, true, flops, [&] {
-      sbgemm_grouped(np, pb, g.nb1, g.alpha, g.beta, g.C, g.sCm, g.sCn,
+      sbgemm_grouped(np, pb, g.nb1, alpha, g.beta, g.C, g.sCm, g.sCn,
                      g.sC1, c->stream);
     });
     return;
@@ -747,7 +774,7 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     if (!b.r.count || !b.k.count || !b.c.count) continue;
     const double flops = 2.0 * b.r.count * (double)b.c.count * b.k.count * g.nb1;
     const int64_t ld = K.total;
-    Gemm gb = mk(b.r.len, b.c.len, b.k.len, g.alpha, g.beta,
+    Gemm gb = mk(b.r.len, b.c.len, b.k.len, alpha, g.beta,
                  c->b64[ia] + (int64_t)b.r.start * ld + b.k.start, ld, 1,
                  OP_FIXED[ia] ? 0 : (int64_t)R.total * ld,
                  c->b64[ib] + (int64_t)b.c.start * ld + b.k.start, 1, ld,
@@ -802,13 +829,18 @@ bool try_spin(ftcc_ctx *c, const std::vector<int> &lo,
     bad += c->host_red[2];
   }
   if (bad != 0.0) return false;
-  for (PairSpace *ps : {&c->ps_vv, &c->ps_oo, &c->ps_vo}) {
+  for (PairSpace *ps :
+       {&c->ps_vv, &c->ps_oo, &c->ps_vo, &c->ps_vvp, &c->ps_oop}) {
     dfree(c, ps->d_map);
     *ps = PairSpace();
   }
   c->ps_vv = build_pairs(c, lv, lv, LADDER_ORDER);
   c->ps_oo = build_pairs(c, lo, lo, LADDER_ORDER);
   c->ps_vo = build_pairs(c, lv, lo, RING_ORDER);
+  c->ps_vvp = build_pairs(c, lv, lv, LADDER_ORDER, true);
+  c->ps_oop = build_pairs(c, lo, lo, LADDER_ORDER, true);
+  const char *penv = getenv("FTCC_PACK");
+  c->pack = (penv && atoi(penv) == 0) ? 0 : 1;
   c->spin = 1;
   // one run of alpha then one run of beta in each list
   auto runs = [](const std::vector<int> &l, int start[2], int count[2]) {
