@@ -63,7 +63,7 @@ def measured_traffic(precision, n, ng, layout):
     """HBM bytes per launch of the dominant kernel, from the committed
     FETCH_SIZE/WRITE_SIZE profile of this workload (a line
     'traffic_bytes_per_launch <precision> <n> <ng> <layout> <bytes>',
-    layout 'spin' or 'dense'), or None."""
+    layout 'spin-packed', 'spin' or 'dense'), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
@@ -179,10 +179,15 @@ def main():
     eng.driving_terms(T1, T2)
     eng.integrate(T1, T2)
     skip = zero_slices(eng, T1, T2, G)
-    spin_blocked = eng.spin_blocked()
-    log("device set-up done in %.1f s (context %.1f GB, spin-blocked "
-        "products: %s)" % (time.time() - t0, eng.workspace_bytes() / 1e9,
-                           spin_blocked))
+    spin_mode = eng.spin_mode()
+    spin_blocked = bool(spin_mode & 1)
+    # operand layout of the dominant kernel: dense, spin blocks, or spin
+    # blocks with the ladder products over packed antisymmetric pairs
+    layout = ("dense" if not spin_blocked else
+              "spin-packed" if spin_mode & 4 else "spin")
+    log("device set-up done in %.1f s (context %.1f GB, product layout: %s, "
+        "spin flags %d)" % (time.time() - t0, eng.workspace_bytes() / 1e9,
+                            layout, spin_mode))
     if args.copy_calibration_mib > 0:
         nb = args.copy_calibration_mib << 20
         src = torch.ones(nb, dtype=torch.uint8, device=dev)
@@ -242,8 +247,7 @@ def main():
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
         "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
-        "traffic": measured_traffic(args.precision, n, ng,
-                                    "spin" if spin_blocked else "dense"),
+        "traffic": measured_traffic(args.precision, n, ng, layout),
         "kernel": "%s (the six O(n^6) products of the residual)"
                   % ("sbgemm_kernel" if args.precision == "bf16x3"
                      else "gemm_mfma_kernel<%s>" % ("double" if args.precision == "fp64" else "float")),
@@ -272,7 +276,8 @@ def main():
                                % (args.nk, n, args.rs, args.t_over_tf,
                                   args.nelec, ng),
                    "n": n, "ng": ng, "precision": args.precision,
-                   "damp": args.damp, "spin_blocked": spin_blocked},
+                   "damp": args.damp, "spin_blocked": spin_blocked,
+                   "layout": layout},
         "fp64": fp64,
         "omega_step_delta_vs_fp64": d_omega,
         "textbook_tflops": algo * value / 1e12,

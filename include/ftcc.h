@@ -88,11 +88,14 @@ int ftcc_set_blocks(ftcc_ctx *ctx, const double *const F_dev[4],
  * the amplitude entries that do not and runs dense if there are any. */
 int ftcc_set_spin(ftcc_ctx *ctx, const int *lab_o_host,
                   const int *lab_v_host);
-/* *on = 0 when the products run dense, 1 when the O(n^6) products run
- * spin-blocked, 2 when in addition the O(n^5) contractions run over the
+/* *on = 0 when everything runs dense; otherwise bit flags: 1 the O(n^6)
+ * products run spin-blocked; 2 the O(n^5) contractions run over the
  * spin-allowed blocks only (labels form one run of 0s then one of 1s in the
- * occupied and in the virtual list; FTCC_SPIN5=0 in the environment at
- * ftcc_set_system/ftcc_set_spin time keeps them dense). */
+ * occupied and in the virtual list); 4 the ladder products sum over packed
+ * antisymmetric pairs (e < f within a spin, alpha-beta once: the amplitudes
+ * must be antisymmetric, as every CC iterate is).  FTCC_SPIN5=0 and
+ * FTCC_PACK=0 in the environment at ftcc_set_system/ftcc_set_spin time
+ * switch off flags 2 and 4. */
 int ftcc_get_spin(ftcc_ctx *ctx, int *on);
 
 /* Uniform-electron-gas integrals built on the device (the provider of

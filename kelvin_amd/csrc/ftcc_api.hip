@@ -1534,7 +1534,7 @@ int ftcc_set_spin(ftcc_ctx *c, const int *lab_o, const int *lab_v) {
 int ftcc_get_spin(ftcc_ctx *c, int *on) {
   return guarded(c, [&] {
     require(on != nullptr, "null output pointer");
-    *on = c->spin ? (c->spin5 ? 2 : 1) : 0;
+    *on = c->spin ? (1 | (c->spin5 ? 2 : 0) | (c->pack ? 4 : 0)) : 0;
   });
 }
 

@@ -167,8 +167,9 @@ class FTEngine:
         return self.spin_mode() > 0
 
     def spin_mode(self):
-        """0: dense; 1: O(n^6) products as spin blocks; 2: also the O(n^5)
-        contractions over spin-allowed blocks only."""
+        """0: dense; else bit flags: 1 O(n^6) products as spin blocks, 2 the
+        O(n^5) contractions over spin-allowed blocks only, 4 ladder products
+        over packed antisymmetric pairs."""
         on = ctypes.c_int()
         self._call("ftcc_get_spin", ctypes.byref(on))
         return on.value

@@ -468,7 +468,9 @@ def test_spin_blocked_residual(prec, tol, no, nv, labels, monkeypatch):
     F, I, t1, t2, lo, lv = _spin_problem(no, nv, ng, lab, rng)
     ref1, ref2 = oracle.residual_tbar(F, I, t1, t2)
     out = {}
-    for mode in (2, 1, 0):
+    # modes: 7 all blocking (restricted O(n^5) only where labels form runs),
+    # 1 spin-blocked products alone, 0 dense
+    for mode in (7, 1, 0):
         eng = FTEngine(no, nv, ng, precision=prec)
         eng.set_blocks(one_e_blocks(**{k: F[k] for k in ('oo', 'ov', 'vo', 'vv')}),
                        two_e_blocks(**{k: I[k] for k in I}))
@@ -477,9 +479,11 @@ def test_spin_blocked_residual(prec, tol, no, nv, labels, monkeypatch):
         if mode:
             if mode == 1:
                 monkeypatch.setenv("FTCC_SPIN5", "0")
+                monkeypatch.setenv("FTCC_PACK", "0")
             eng.set_spin(lo, lv)
             monkeypatch.delenv("FTCC_SPIN5", raising=False)
-            assert eng.spin_mode() == (2 if mode == 2 and runs else 1)
+            monkeypatch.delenv("FTCC_PACK", raising=False)
+            assert eng.spin_mode() == (mode if runs else mode & ~2)
         else:
             assert eng.spin_mode() == 0
         T1 = torch.as_tensor(t1, device=d)
@@ -491,7 +495,7 @@ def test_spin_blocked_residual(prec, tol, no, nv, labels, monkeypatch):
         assert rel(out[mode][1], ref2) < tol
     # blocked and dense differ only by summation order (and rounding of the
     # split operands, which is the same per element)
-    for mode in (2, 1):
+    for mode in (7, 1):
         assert rel(out[mode][0], out[0][0]) < max(tol, 1e-12)
         assert rel(out[mode][1], out[0][1]) < max(tol, 1e-12)
 
