@@ -195,6 +195,18 @@ int ftcc_td_rhs(ftcc_ctx *ctx, const double *T1_dev, const double *T2_dev,
                 double *K1_dev, double *K2_dev, int singles);
 int ftcc_td_step(ftcc_ctx *ctx, int order, double h, int singles,
                  double *T1_dev, double *T2_dev);
+/* One implicit step of size h (kelvin/propagation.py:55-128): scheme 0 be
+ * (backward Euler), 1 cn (Crank-Nicolson: the backward-Euler iterate
+ * averaged with the explicit predictor), 2 am2 (trapezoid).  From the
+ * predictor d = h f(T), d <- (1 - damp) d_new + damp d with d_new = h f(T + d)
+ * (am2: (h f(T + d) + h f(T)) / 2) until sum over T1, T2 of
+ * ||d_new - d|| / (||d_new|| + 0.01) < thresh (norms over all slices), at
+ * most max_iter times; *iters (may be NULL) receives the count.  Not
+ * converging returns FTCC_EINVAL ("BE: / AM2: Failed to compute implicit
+ * step", the reference's message) and leaves T unchanged. */
+int ftcc_td_implicit_step(ftcc_ctx *ctx, int scheme, double h, int max_iter,
+                          double damp, double thresh, int singles,
+                          double *T1_dev, double *T2_dev, int *iters);
 
 /* Device bytes held by the context (integrals + workspace). */
 int ftcc_workspace_bytes(const ftcc_ctx *ctx, int64_t *bytes);

@@ -52,6 +52,11 @@ SIGNATURES = {
     "ftcc_td_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
     "ftcc_td_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_int, _vp, _vp]),
+    "ftcc_td_implicit_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int,
+                                             _vp, _vp,
+                                             ctypes.POINTER(ctypes.c_int)]),
     "ftcc_ueg_eri": (ctypes.c_int, [_vp, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int), _c_double_p,
                                     ctypes.c_double, _vp]),

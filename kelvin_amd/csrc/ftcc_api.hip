@@ -61,7 +61,7 @@ struct ftcc_ctx {
   int nb = 1;
   // imaginary-time propagation: stage input, accumulator and right-hand
   // side (singles / doubles), allocated on first use
-  double *td_var[2] = {}, *td_acc[2] = {}, *td_k[2] = {};
+  double *td_var[2] = {}, *td_acc[2] = {}, *td_k[2] = {}, *td_k1[2] = {};
   double *tau = nullptr, *taut, *t2r, *aring, *t2p, *tautp, *wmnij, *wabef,
       *wr, *z, *u, *s1, *s2;
   double *t1T, *Fme, *FmeT, *Fae, *Fmi, *Fbe, *Fmj, *ytmp;
@@ -1789,6 +1789,61 @@ int ftcc_td_step(ftcc_ctx *c, int order, double h, int singles, double *T1,
       in1 = c->td_var[0];
       in2 = c->td_var[1];
     }
+  });
+}
+
+int ftcc_td_implicit_step(ftcc_ctx *c, int scheme, double h, int max_iter,
+                          double damp, double thresh, int singles, double *T1,
+                          double *T2, int *iters) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(scheme >= 0 && scheme <= 2, "implicit scheme must be 0, 1 or 2");
+    require(max_iter >= 1, "max_iter must be positive");
+    require(T1 && T2, "null amplitude pointer");
+    hipStream_t s = c->stream;
+    const int64_t VO = (int64_t)c->nv * c->no, S2 = VO * VO;
+    const int64_t len[2] = {VO * c->ng, S2 * c->ng};
+    for (int k = 0; k < 2; ++k) {
+      if (!c->td_var[k]) {
+        c->td_var[k] = dalloc(c, len[k]);
+        c->td_acc[k] = dalloc(c, len[k]);
+        c->td_k[k] = dalloc(c, len[k]);
+      }
+      if (!c->td_k1[k]) c->td_k1[k] = dalloc(c, len[k]);
+    }
+    double *T[2] = {T1, T2};
+    auto pass = [&](int k, int mode) {
+      UpdateArgs a = upd(c);
+      a.D = c->have_D ? (k ? c->D2 : c->D1) : nullptr;
+      a.partials = c->partials;
+      return implicit_pass(a, k == 1, c->td_k[k], T[k], c->td_var[k],
+                           c->td_acc[k], c->td_k1[k], h, damp, mode, scheme,
+                           k == 0 && singles == 0, s);
+    };
+    // predictor: k1 = h f(T)   (kelvin/propagation.py:83-128)
+    residual_all(c, T1, T2, c->td_k[0], c->td_k[1], 0);
+    pass(0, 0);
+    pass(1, 0);
+    bool converged = false;
+    int it = 0;
+    while (it < max_iter && !converged) {
+      residual_all(c, c->td_var[0], c->td_var[1], c->td_k[0], c->td_k[1], 0);
+      int nb = pass(0, 1);
+      reduce_partials(c->partials, nb, c->red, s);
+      nb = pass(1, 1);
+      reduce_partials(c->partials, nb, c->red + 3, s);
+      read_scalars(c, 6);
+      const double err =
+          sqrt(c->host_red[0]) / (sqrt(c->host_red[1]) + 0.01) +
+          sqrt(c->host_red[3]) / (sqrt(c->host_red[4]) + 0.01);
+      ++it;
+      converged = err < thresh;
+    }
+    if (iters) *iters = it;
+    require(converged, scheme == 2 ? "AM2: Failed to compute implicit step"
+                                   : "BE: Failed to compute implicit step");
+    pass(0, 2);
+    pass(1, 2);
   });
 }
 

@@ -266,6 +266,15 @@ int integrate_point(const UpdateArgs &a, bool doubles, int y, int mode,
 int rk_stage(const UpdateArgs &a, bool doubles, const double *tb, double *T,
              double *var, double *acc, double b, double hnext, double h,
              bool first, bool last, bool zero, hipStream_t s);
+// One pass of an implicit step (be = 0, cn = 1, am2 = 2) of dT/dtau =
+// Tbar - D T over a.ng slices: mode 0 predictor (k1 = dold = h k, var =
+// T + h k), mode 1 iteration (d from Tbar(var); sums of (d - dold)^2 and
+// d^2 per block in a.partials; dold, var updated), mode 2 finish (T +=
+// dold, or the cn average).  Returns #blocks.
+int implicit_pass(const UpdateArgs &a, bool doubles, const double *tb,
+                  double *T, double *var, double *dold, double *k1, double h,
+                  double alpha, int mode, int scheme, bool zero,
+                  hipStream_t s);
 int update_t1(const UpdateArgs &a, int mode, hipStream_t s);   // returns #blocks
 int update_t2(const UpdateArgs &a, int mode, hipStream_t s);
 // true when in-place integration needs a source copy in UpdateArgs::src

@@ -538,6 +538,77 @@ __global__ __launch_bounds__(UT) void rk_stage_kernel(
   }
 }
 
+// One pass of an implicit step of dT/dtau = Tbar(T) - D T (kelvin/
+// propagation.py:55-128) over p.ng slices.  mode 0 (predictor): with
+// k = Tbar - D T, k1 = dold = h k and var = T + h k.  mode 1 (iteration):
+// k = Tbar - D var, d = h k (be, cn) or (h k + k1) / 2 (am2); the block's
+// sums of (d - dold)^2 and d^2 go to p.partials; dold <- (1 - alpha) d +
+// alpha dold and var = T + dold.  mode 2 (finish): T += dold, or
+// (dold + k1) / 2 for cn.  zero: the right-hand side is zero.
+template <bool DOUBLES>
+__global__ __launch_bounds__(UT) void implicit_kernel(
+    UpdateArgs p, const double *tb, double *T, double *var, double *dold,
+    double *k1, double h, double alpha, int mode, int scheme, int zero) {
+  const int nv = p.nv, no = p.no;
+  const int64_t per = DOUBLES ? (int64_t)nv * nv * no * no : (int64_t)nv * no;
+  const int64_t total = per * p.ng;
+  double sdiff = 0.0, snorm = 0.0;
+  for (int64_t x = blockIdx.x * (int64_t)UT + threadIdx.x; x < total;
+       x += (int64_t)gridDim.x * UT) {
+    if (mode == 2) {
+      if (!zero) T[x] += scheme == 1 ? 0.5 * (dold[x] + k1[x]) : dold[x];
+      continue;
+    }
+    double k = 0.0;
+    if (!zero) {
+      const int64_t e = x % per;
+      double dd;
+      if (p.D) {
+        dd = p.D[e];
+      } else {
+        const Elem q = decode<DOUBLES>(e, nv, no);
+        dd = DOUBLES ? p.ev[q.a] + p.ev[q.b] - p.eo[q.i] - p.eo[q.j]
+                     : p.ev[q.a] - p.eo[q.i];
+      }
+      k = tb[x] - dd * (mode == 0 ? T[x] : var[x]);
+    }
+    if (mode == 0) {
+      const double hk = h * k;
+      k1[x] = hk;
+      dold[x] = hk;
+      var[x] = T[x] + hk;
+    } else {
+      const double d = scheme == 2 ? 0.5 * (h * k + k1[x]) : h * k;
+      const double o = dold[x];
+      sdiff += (d - o) * (d - o);
+      snorm += d * d;
+      const double n = (1.0 - alpha) * d + alpha * o;
+      dold[x] = n;
+      var[x] = T[x] + n;
+    }
+  }
+  if (mode == 1) block_reduce3(sdiff, snorm, 0.0, p.partials);
+}
+
+int implicit_pass(const UpdateArgs &a, bool doubles, const double *tb,
+                  double *T, double *var, double *dold, double *k1, double h,
+                  double alpha, int mode, int scheme, bool zero,
+                  hipStream_t s) {
+  const int64_t per = doubles ? (int64_t)a.nv * a.nv * a.no * a.no
+                              : (int64_t)a.nv * a.no;
+  const int blocks = nblocks_for(per * a.ng);
+  if (doubles)
+    hipLaunchKernelGGL(implicit_kernel<true>, dim3(blocks), dim3(UT), 0, s, a,
+                       tb, T, var, dold, k1, h, alpha, mode, scheme,
+                       zero ? 1 : 0);
+  else
+    hipLaunchKernelGGL(implicit_kernel<false>, dim3(blocks), dim3(UT), 0, s,
+                       a, tb, T, var, dold, k1, h, alpha, mode, scheme,
+                       zero ? 1 : 0);
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
 int rk_stage(const UpdateArgs &a, bool doubles, const double *tb, double *T,
              double *var, double *acc, double b, double hnext, double h,
              bool first, bool last, bool zero, hipStream_t s) {

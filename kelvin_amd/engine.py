@@ -319,6 +319,19 @@ class FTEngine:
         self._call("ftcc_td_step", int(order), float(h), int(bool(singles)),
                    _vp(T1), _vp(T2))
 
+    def td_implicit_step(self, T1, T2, h, scheme, max_iter, damp, thresh,
+                         singles=True):
+        """One implicit step ('be', 'cn' or 'am2') of size h, in place;
+        returns the number of inner iterations."""
+        self._check(T1, self.t1_shape(), "T1")
+        self._check(T2, self.t2_shape(), "T2")
+        code = {"be": 0, "cn": 1, "am2": 2}[scheme]
+        it = ctypes.c_int()
+        self._call("ftcc_td_implicit_step", code, float(h), int(max_iter),
+                   float(damp), float(thresh), int(bool(singles)), _vp(T1),
+                   _vp(T2), ctypes.byref(it))
+        return it.value
+
     def profile(self, on=True):
         """Bracket every O(n^6) product launch with HIP events."""
         self._call("ftcc_profile_enable", int(bool(on)))

@@ -7,8 +7,11 @@ amplitudes are propagated over the quadrature grid with
 dT/dtau = -D T - (F.vo, I.vvoo) - R_CCSD(T) and the grand potential
 accumulates Omega_cc = sum_{i>=1} g_i E_CC(T(tau_i)) / beta.  Every step runs
 on the GPU (``ftcc_td_step``: the residual engine plus one fused kernel per
-Runge-Kutta stage); the propagators are kelvin/propagation.py:5-52's
-explicit rk1, rk2 and rk4.
+Runge-Kutta stage; ``ftcc_td_implicit_step``: the damped inner iteration of
+the implicit schemes, one fused update-and-norm kernel per pass); the
+propagators are kelvin/propagation.py's explicit rk1, rk2, rk4 (:5-52) and
+implicit be, cn, am2 (:55-128, with the ``max_iter``, ``damp`` and
+``thresh`` entries of ``prop``).
 
 It is an independent check of the fixed-point solver: both converge to the
 same Omega_cc as the grid is refined (kelvin/tests/test_td_ccsd.py).
@@ -22,6 +25,7 @@ from . import quadrature
 from .engine import FTEngine
 
 _ORDERS = {"rk1": 1, "rk2": 2, "rk4": 4}
+_IMPLICIT = ("be", "cn", "am2")
 
 
 class TDCCSD(object):
@@ -62,13 +66,11 @@ class TDCCSD(object):
             raise NotImplementedError(
                 "zero-temperature TD-CCSD is not part of the GPU path")
         tprop = self.prop.get("tprop")
-        if tprop not in _ORDERS:
-            raise NotImplementedError(
-                "propagator %r: the explicit rk1, rk2 and rk4 are built "
-                "(kelvin/propagation.py:5-52)" % (tprop,))
+        if tprop not in _ORDERS and tprop not in _IMPLICIT:
+            raise Exception("Unrecognized propagation scheme: " + str(self.prop))
         logging.info('Running CCSD at an electronic temperature of %f K'
                      % ft_utils.HtoK(self.T))
-        return self._ccsd(_ORDERS[tprop])
+        return self._ccsd(tprop)
 
     def _active(self, en):
         """kelvin/td_ccsd.py:17-72 ('g' branch): orbitals with occupation
@@ -84,7 +86,7 @@ class TDCCSD(object):
         logging.info("  nvir: {:d}".format(len(ivir)))
         return iocc, ivir
 
-    def _ccsd(self, order):
+    def _ccsd(self, tprop):
         beta, mu = self.beta, self.mu
         ng, ti, g = self.ngrid, self.ti, self.g
         en = numpy.asarray(self.sys.g_energies_tot(), dtype=numpy.float64)
@@ -114,7 +116,13 @@ class TDCCSD(object):
             self.T1[0], self.T2[0] = T1[0].clone(), T2[0].clone()
         for i in range(1, ng):
             h = ti[i] - ti[i - 1]
-            eng.td_step(T1, T2, h, order=order, singles=self.singles)
+            if tprop in _ORDERS:
+                eng.td_step(T1, T2, h, order=_ORDERS[tprop],
+                            singles=self.singles)
+            else:
+                eng.td_implicit_step(T1, T2, h, tprop, self.prop["max_iter"],
+                                     self.prop["damp"], self.prop["thresh"],
+                                     singles=self.singles)
             Eccn += eng.energy(T1, T2, numpy.array([g[i]]), beta, qterm=True)
             if self.saveT:
                 self.T1[i], self.T2[i] = T1[0].clone(), T2[0].clone()

@@ -809,3 +809,59 @@ def test_tdccsd_matches_fixed_point(tprop, ng_td, ng_cc, tol):
         Eo, _, _ = otd.td_ccsd(F, I, oracle.D1(en, en), oracle.D2(en, en),
                                ti, g, beta, {"rk2": 2, "rk4": 4}[tprop])
         assert abs(Eccout - Eo) < 1e-10
+
+
+@pytest.mark.parametrize("scheme", ["be", "cn", "am2"])
+def test_td_implicit_step_matches_oracle(scheme):
+    """ftcc_td_implicit_step against the oracle's restatement of
+    kelvin/propagation.py:55-128: same inner-iteration count, same step."""
+    from oracle import td_ccsd as otd
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd.ov_blocks import one_e_blocks, two_e_blocks
+    d = dev()
+    rng = numpy.random.default_rng({"be": 1, "cn": 2, "am2": 3}[scheme])
+    no, nv = 3, 4
+    F, I = random_problem(no, nv, rng)
+    eo, ev = rng.uniform(-1.0, -0.1, no), rng.uniform(0.1, 1.0, nv)
+    D1, D2 = oracle.D1(ev, eo), oracle.D2(ev, eo)
+    t1, t2 = random_T(1, nv, no, rng, 0.1)
+    eng = FTEngine(no, nv, 1)
+    eng.set_blocks(one_e_blocks(**{k: F[k] for k in ('oo', 'ov', 'vo', 'vv')}),
+                   two_e_blocks(**{k: I[k] for k in I}))
+    eng.set_denominators(D1, D2)
+    eng.set_grid(numpy.zeros(1), numpy.zeros((1, 1)))
+    T1 = torch.as_tensor(t1, device=d).contiguous()
+    T2 = torch.as_tensor(t2, device=d).contiguous()
+    h, mi, damp, thresh = 0.2, 200, 0.3, 1e-9
+    its = eng.td_implicit_step(T1, T2, h, scheme, mi, damp, thresh)
+    (d1, d2), oits = otd.implicit_step(
+        scheme, h, [t1[0], t2[0]],
+        lambda v: otd.td_rhs(F, I, D1, D2, v[0], v[1]), mi, damp, thresh)
+    assert its == oits
+    assert numpy.abs(to_np(T1)[0] - (t1[0] + d1)).max() < 1e-12
+    assert rel(to_np(T2)[0], t2[0] + d2) < 1e-11
+    # not converging: the reference's error, amplitudes untouched
+    from kelvin_amd._lib import FTCCError
+    before = to_np(T2).copy()
+    with pytest.raises(FTCCError, match="Failed to compute implicit step"):
+        eng.td_implicit_step(T1, T2, h, scheme, 1, damp, 1e-300)
+    assert numpy.array_equal(to_np(T2), before)
+
+
+@pytest.mark.parametrize("tprop,ng_td,ng_rk4", [("cn", 60, 160),
+                                                ("am2", 50, 20)])
+def test_tdccsd_implicit_matches_rk4(tprop, ng_td, ng_rk4):
+    """kelvin/tests/test_td_ccsd.py:77-117 on the 7 k-point UEG: cn and am2
+    (200 inner iterations at most, damping 0.4 / 0.3, threshold 1e-5)
+    against rk4 at 1e-4."""
+    from kelvin_amd.td_ccsd import TDCCSD
+    dev()
+    sysg, s = _ueg_gold("G1")
+    T, mu = s["T"], s["mu"]
+    ref = TDCCSD(sysg, {"tprop": "rk4"}, T=T, mu=mu, ngrid=ng_rk4)
+    Eref, Eccref = ref.run()
+    prop = {"tprop": tprop, "max_iter": 200,
+            "damp": 0.4 if tprop == "cn" else 0.3, "thresh": 1e-5}
+    td = TDCCSD(sysg, prop, T=T, mu=mu, ngrid=ng_td)
+    Eout, Eccout = td.run()
+    assert abs(Eccout - Eccref) < 1e-4

@@ -331,3 +331,24 @@ def test_td_ccsd_matches_fixed_point(order, ng_td, ng_cc, tol):
     ti, g, _ = ftcc.ft_quad(ng_td, beta, 'lin')
     Ecc, _, _ = td_ccsd.td_ccsd(F, I, D1, D2, ti, g, beta, order)
     assert abs(Ecc - ref["Omega_cc"]) < tol, (Ecc, ref["Omega_cc"])
+
+
+@pytest.mark.parametrize("scheme,order", [("be", 1), ("cn", 2), ("am2", 2)])
+def test_td_implicit_propagator_order(scheme, order):
+    """kelvin/propagation.py be/cn/am2 with the inner iteration converged
+    tightly: error ratios 2, 4, 4 on halving the step of the linear problem."""
+    from oracle import td_ccsd
+    rng = numpy.random.default_rng(11 + order)
+    no, nv = 2, 3
+    F, I, D1, D2 = _linear_td_problem(no, nv, rng)
+    tend = 2.0
+    exact = -F.vo * (1.0 - numpy.exp(-D1 * tend)) / D1
+    errs = []
+    for nstep in (20, 40):
+        ti = numpy.linspace(0.0, tend, nstep + 1)
+        _, t1, _ = td_ccsd.td_ccsd(F, I, D1, D2, ti, numpy.zeros(nstep + 1),
+                                   1.0, None,
+                                   implicit=(scheme, 400, 0.0, 1e-13))
+        errs.append(numpy.abs(t1 - exact).max())
+    ratio = errs[0] / errs[1]
+    assert 0.8 * 2 ** order < ratio < 1.25 * 2 ** order, (errs, ratio)
