@@ -741,9 +741,10 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     int np = 0;
     double flops = 0.0;
     for (const Blk &b : blk) {
-      // an empty K range leaves C as it is: the callers either accumulate
-      // (beta = 1) or have cleared C beforehand
-      if (!b.r.count || !b.k.count || !b.c.count) continue;
+      // an empty K range still writes beta * C (zero for beta = 0); a
+      // block without rows or columns has nothing to write
+      if (!b.r.count || !b.c.count || (!b.k.count && g.beta != 0.0))
+        continue;
       SbProblem &q = pb[np++];
       q.M = b.r.len;
       q.N = b.c.len;
@@ -771,7 +772,7 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     return;
   }
   for (const Blk &b : blk) {
-    if (!b.r.count || !b.k.count || !b.c.count) continue;
+    if (!b.r.count || !b.c.count || (!b.k.count && g.beta != 0.0)) continue;
     const double flops = 2.0 * b.r.count * (double)b.c.count * b.k.count * g.nb1;
     const int64_t ld = K.total;
     Gemm gb = mk(b.r.len, b.c.len, b.k.len, alpha, g.beta,
@@ -1153,13 +1154,15 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
 
-  // doubles accumulator in t2bar: vvoo + 1/2 tau.Wmnij + 1/2 Wabef.tau
-  bcast(t2bar, vvoo, S2, nb, 1.0, 0.0, s);
+  // doubles accumulator in t2bar: vvoo + 1/2 tau.Wmnij + 1/2 Wabef.tau; in
+  // the spin layout G3 writes every allowed element (beta = 0) and the
+  // assembly adds vvoo, reading neither on forbidden elements
+  if (!s5) bcast(t2bar, vvoo, S2, nb, 1.0, 0.0, s);
   put_operand(c, OP_WMNIJ_B, c->wmnij, 1, OO, O4, nb);
   put_operand(c, OP_WABEF_A, c->wabef, VV, 1, V4, nb);
-  product(c, mk((int)VV, (int)OO, (int)OO, 0.5, 1.0, c->tau, OO, 1, S2,
-                c->wmnij, OO, 1, O4, t2bar, OO, 1, S2, nb), OP_TAU_A,
-          OP_WMNIJ_B, LADDER);                                           // G3
+  product(c, mk((int)VV, (int)OO, (int)OO, 0.5, s5 ? 0.0 : 1.0, c->tau, OO,
+                1, S2, c->wmnij, OO, 1, O4, t2bar, OO, 1, S2, nb),
+          OP_TAU_A, OP_WMNIJ_B, LADDER);                                 // G3
   product(c, mk((int)VV, (int)OO, (int)VV, 0.5, 1.0, c->wabef, VV, 1, V4,
                 c->tau, OO, 1, S2, t2bar, OO, 1, S2, nb), OP_WABEF_A,
           OP_TAU_B, LADDER);                                             // G4
@@ -1215,7 +1218,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   }
   // forbidden blocks: written as zero without reading S1, S2, Z
   r2_final(t2bar, c->s1, c->s2, c->z, V, O, nb, s,
-           s5 ? c->d_lab_v : nullptr, s5 ? c->d_lab_o : nullptr);
+           s5 ? c->d_lab_v : nullptr, s5 ? c->d_lab_o : nullptr,
+           s5 ? vvoo : nullptr);
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
   //                    + 1/2 vovv.t2p + 1/2 t2.ooov_emni)

@@ -205,12 +205,14 @@ void antisym_add(double *W, const double *Q, int d0, int d1, int d2, int d3,
 // Wr[e,m,b,j] = ovvo_ring[e,m,b,j] - Q1[b,m,e,j] + Q2[m,b,j,e]
 void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
                    const double *Q2, int nv, int no, int nb, hipStream_t s);
-// T2bar[a,b,i,j] = -( acc + S1 - S1^(ab) + S2 - S2^(ij) + PP(Z ring) );
-// with spin labels (device arrays, or nullptr), spin-forbidden elements are
-// written as zero without reading the operands
+// T2bar[a,b,i,j] = -( acc [+ add] + S1 - S1^(ab) + S2 - S2^(ij) + PP(Z ring) )
+// with add one slice for every grid point (or nullptr); with spin labels
+// (device arrays, or nullptr), spin-forbidden elements are written as zero
+// without reading the operands
 void r2_final(double *t2bar, const double *S1, const double *S2,
               const double *Z, int nv, int no, int nb, hipStream_t s,
-              const int *lab_v = nullptr, const int *lab_o = nullptr);
+              const int *lab_v = nullptr, const int *lab_o = nullptr,
+              const double *add = nullptr);
 // x[n][c] (nb batches of n*c) -> out[c][n] transposed copy for small mats
 void transpose2(const double *in, double *out, int r, int c, int nb,
                 hipStream_t s);

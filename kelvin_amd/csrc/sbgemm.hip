@@ -132,10 +132,13 @@ __global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
 
   u4_t ra[2048 / NT], rb[2048 / NT];
   constexpr int STAGE = 2 * BM * LDR;   // bf16 per stage (A then B)
-  load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, 0, ra);
-  load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, 0, rb);
-  store_operand<NT>(lds, ra);
-  store_operand<NT>(lds + BM * LDR, rb);
+  // an empty K (a spin class without pairs) only writes beta * C
+  if (nslab > 0) {
+    load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, 0, ra);
+    load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, 0, rb);
+    store_operand<NT>(lds, ra);
+    store_operand<NT>(lds + BM * LDR, rb);
+  }
   __syncthreads();
   for (int s = 0; s < nslab; ++s) {
     const __bf16 *As = lds + (s & 1) * STAGE;

@@ -271,58 +271,77 @@ __global__ void ring_assemble_kernel(double *Wr, const double *ovvo_ring,
 }
 
 // t2bar[a,b,i,j] = -( acc[a,b,i,j]            (already in t2bar)
+//                     [+ add[a,b,i,j]]         (one slice, all grid points)
 //                     + S1[a,b,i,j] - S1[b,a,i,j]
 //                     + S2[a,b,i,j] - S2[a,b,j,i]
 //                     + Z[a,i,b,j] - Z[b,i,a,j] - Z[a,j,b,i] + Z[b,j,a,i] )
-// Same assembly, one 32x32 (i,j) tile of one (a,b) pair per workgroup: the
-// three operands read transposed (S2[a,b,j,i], Z[a,j,b,i], Z[b,j,a,i]) are
-// loaded along i into LDS so every global access is coalesced.  With spin
-// labels (lv, lo), elements whose spins do not conserve (lv[a] + lv[b] !=
-// lo[i] + lo[j]) are written as zero and their operands are not read: every
-// operand element an allowed output reads is itself allowed.
-__global__ __launch_bounds__(256) void r2_final_tiled_kernel(
+// One workgroup per (a <= b) pair and 32x32 (i,j) tile writes both (a,b)
+// and (b,a): their S1 and Z terms are the same four and two reads with
+// opposite signs.  The operands read transposed (S2[a,b,j,i], S2[b,a,j,i],
+// Z[a,j,b,i], Z[b,j,a,i]) are loaded along i into LDS so every global
+// access is coalesced.  With spin labels (lv, lo), elements whose spins do
+// not conserve (lv[a] + lv[b] != lo[i] + lo[j]) are written as zero and
+// their operands are not read: every operand element an allowed output
+// reads is itself allowed.
+__global__ __launch_bounds__(256) void r2_final_pair_kernel(
     double *t2bar, const double *S1, const double *S2, const double *Z,
-    int nv, int no, const int *lv, const int *lo) {
-  __shared__ double tS2[32][33], tZ1[32][33], tZ2[32][33];
+    const double *add, int nv, int no, const int *lv, const int *lo) {
+  __shared__ double tA[32][33], tB[32][33], tZ1[32][33], tZ2[32][33];
   const int tiles = (no + 31) / 32;
+  const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   int64_t bid = blockIdx.x;
   const int tj = (int)(bid % tiles);
   bid /= tiles;
   const int ti = (int)(bid % tiles);
   bid /= tiles;
-  const int b = (int)(bid % nv);
-  bid /= nv;
-  const int a = (int)(bid % nv);
-  const int64_t y = bid / nv;
+  const int64_t q = bid % npair;
+  const int64_t y = bid / npair;
+  // pair q -> (a, b), a <= b, q = b (b + 1) / 2 + a
+  int b = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while ((int64_t)b * (b + 1) / 2 > q) --b;
+  while ((int64_t)(b + 1) * (b + 2) / 2 <= q) ++b;
+  const int a = (int)(q - (int64_t)b * (b + 1) / 2);
   const int64_t per = (int64_t)nv * nv * no * no;
   const double *s1 = S1 + y * per, *s2 = S2 + y * per, *z = Z + y * per;
   double *out = t2bar + y * per;
   const int i0 = ti * 32, j0 = tj * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int sab = lv ? lv[a] + lv[b] : 0;
+  auto nat = [&](int p, int r, int k, int l) {
+    return (((int64_t)p * nv + r) * no + k) * no + l;
+  };
+  auto ring = [&](int p, int k, int r, int l) {
+    return (((int64_t)p * no + k) * nv + r) * no + l;
+  };
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + tx, j = j0 + r;
     const bool ok = i < no && j < no && (!lv || lo[i] + lo[j] == sab);
-    tS2[r][tx] = ok ? s2[(((int64_t)a * nv + b) * no + j) * no + i] : 0.0;
-    tZ1[r][tx] = ok ? z[(((int64_t)a * no + j) * nv + b) * no + i] : 0.0;
-    tZ2[r][tx] = ok ? z[(((int64_t)b * no + j) * nv + a) * no + i] : 0.0;
+    tA[r][tx] = ok ? s2[nat(a, b, j, i)] : 0.0;
+    tB[r][tx] = ok ? s2[nat(b, a, j, i)] : 0.0;
+    tZ1[r][tx] = ok ? z[ring(a, j, b, i)] : 0.0;
+    tZ2[r][tx] = ok ? z[ring(b, j, a, i)] : 0.0;
   }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + r, j = j0 + tx;
     if (i >= no || j >= no) continue;
-    const int64_t nat = (((int64_t)a * nv + b) * no + i) * no + j;
+    const int64_t nab = nat(a, b, i, j), nba = nat(b, a, i, j);
     if (lv && lo[i] + lo[j] != sab) {
-      out[nat] = 0.0;
+      out[nab] = 0.0;
+      out[nba] = 0.0;
       continue;
     }
-    double v = out[nat];
-    v += s1[nat] - s1[(((int64_t)b * nv + a) * no + i) * no + j];
-    v += s2[nat] - tS2[tx][r];
-    v += z[(((int64_t)a * no + i) * nv + b) * no + j] -
-         z[(((int64_t)b * no + i) * nv + a) * no + j] - tZ1[tx][r] +
-         tZ2[tx][r];
-    out[nat] = -v;
+    const double s1ab = s1[nab], s1ba = s1[nba];
+    const double zz = z[ring(a, i, b, j)] - z[ring(b, i, a, j)] -
+                      tZ1[tx][r] + tZ2[tx][r];
+    double vab = out[nab] + s1ab - s1ba + s2[nab] - tA[tx][r] + zz;
+    double vba = out[nba] + s1ba - s1ab + s2[nba] - tB[tx][r] - zz;
+    if (add) {
+      vab += add[nab];
+      vba += add[nba];
+    }
+    out[nab] = -vab;
+    if (a != b) out[nba] = -vba;
   }
 }
 
@@ -498,12 +517,13 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
 
 void r2_final(double *t2bar, const double *S1, const double *S2,
               const double *Z, int nv, int no, int nb, hipStream_t s,
-              const int *lab_v, const int *lab_o) {
+              const int *lab_v, const int *lab_o, const double *add) {
   const int tiles = (no + 31) / 32;
-  const int64_t blocks = (int64_t)nb * nv * nv * tiles * tiles;
+  const int64_t npair = (int64_t)nv * (nv + 1) / 2;
+  const int64_t blocks = (int64_t)nb * npair * tiles * tiles;
   require(blocks < (1LL << 31), "r2 assembly grid too large");
-  hipLaunchKernelGGL(r2_final_tiled_kernel, dim3((unsigned)blocks), dim3(256),
-                     0, s, t2bar, S1, S2, Z, nv, no, lab_v, lab_o);
+  hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks), dim3(256),
+                     0, s, t2bar, S1, S2, Z, add, nv, no, lab_v, lab_o);
   FTCC_CHECK_HIP(hipGetLastError());
 }
 

@@ -440,6 +440,7 @@ def _spin_problem(no, nv, ng, lab, rng, scale=0.2):
     (3, 7, "occ-alpha"),  # no beta occupied orbital: empty classes
     (3, 7, "occ-alpha-runs"),  # the same with the virtual labels in runs
     (5, 6, "uneven"),     # runs of unequal length, o != v
+    (4, 5, "one-beta-occ"),  # packed beta-beta class empty, full one not
     (17, 17, "half")])    # classes above one 256 tile
 def test_spin_blocked_residual(prec, tol, no, nv, labels, monkeypatch):
     """Spin-blocked O(n^6) products (three blocks each, written through the
@@ -460,6 +461,8 @@ def test_spin_blocked_residual(prec, tol, no, nv, labels, monkeypatch):
                                  numpy.arange(nv) >= 3]).astype(int)
     elif labels == "uneven":
         lab = numpy.array([0, 0, 1, 1, 1, 0, 0, 0, 0, 1, 1])
+    elif labels == "one-beta-occ":
+        lab = numpy.array([0, 0, 0, 1, 0, 0, 1, 1, 1])
     else:
         lab = numpy.concatenate([numpy.arange(no) >= no // 2,
                                  numpy.arange(nv) >= nv // 2]).astype(int)
