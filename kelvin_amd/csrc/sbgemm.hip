@@ -14,9 +14,11 @@
 // fragments (hi and lo) from LDS and issues the three MFMAs of each block in
 // turn.  K advances in slabs of 32 through double-buffered LDS; each LDS row
 // holds the 32 hi values, then the 32 lo values, then 8 bf16 of padding
-// (144 B rows: 16 lanes reading one row each land on distinct banks).  The
-// next slab is prefetched into registers with 16-byte loads while the
-// current one feeds the MFMAs.  Workgroup ids are remapped so that each XCD
+// (144 B rows: 16 lanes reading one row each land on distinct banks).  Slabs
+// are loaded into registers (16-byte loads) two slabs ahead of their use,
+// alternating two register sets, and stored to the other LDS stage one slab
+// ahead, so each load has a slab of MFMA work besides its own to land
+// (5.6% shorter launches than prefetching one slab ahead at n = 114).  Workgroup ids are remapped so that each XCD
 // works on a compact group of tiles (shared A/B panels stay in that XCD's
 // L2).  Measured alternatives -- 4 waves of 128x128 with accumulators in
 // AGPRs, with and without a software pipeline that double-buffers the
@@ -130,22 +132,31 @@ __global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
 
-  u4_t ra[2048 / NT], rb[2048 / NT];
+  // Slab k is loaded into register set k % 2 two slabs ahead of its use and
+  // stored to LDS stage k % 2 one slab ahead, so each load has a whole slab
+  // of MFMA work, besides its own, to land (the loop is unrolled by two so
+  // both register sets are addressed statically).
+  u4_t ra0[2048 / NT], rb0[2048 / NT], ra1[2048 / NT], rb1[2048 / NT];
   constexpr int STAGE = 2 * BM * LDR;   // bf16 per stage (A then B)
   // an empty K (a spin class without pairs) only writes beta * C
   if (nslab > 0) {
-    load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, 0, ra);
-    load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, 0, rb);
-    store_operand<NT>(lds, ra);
-    store_operand<NT>(lds + BM * LDR, rb);
+    load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, 0, ra0);
+    load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, 0, rb0);
+    store_operand<NT>(lds, ra0);
+    store_operand<NT>(lds + BM * LDR, rb0);
+  }
+  if (nslab > 1) {
+    load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, BK, ra1);
+    load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, BK, rb1);
   }
   __syncthreads();
-  for (int s = 0; s < nslab; ++s) {
+  auto step = [&](int s, u4_t(&lda_)[2048 / NT], u4_t(&ldb_)[2048 / NT],
+                  const u4_t(&sta)[2048 / NT], const u4_t(&stb)[2048 / NT]) {
     const __bf16 *As = lds + (s & 1) * STAGE;
     const __bf16 *Bs = As + BM * LDR;
-    if (s + 1 < nslab) {
-      load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, (s + 1) * BK, ra);
-      load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, (s + 1) * BK, rb);
+    if (s + 2 < nslab) {
+      load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, (s + 2) * BK, lda_);
+      load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, (s + 2) * BK, ldb_);
     }
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
@@ -177,10 +188,14 @@ __global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
     }
     if (s + 1 < nslab) {
       __bf16 *nxt = lds + ((s + 1) & 1) * STAGE;
-      store_operand<NT>(nxt, ra);
-      store_operand<NT>(nxt + BM * LDR, rb);
+      store_operand<NT>(nxt, sta);
+      store_operand<NT>(nxt + BM * LDR, stb);
     }
     __syncthreads();
+  };
+  for (int s = 0; s < nslab; s += 2) {
+    step(s, ra0, rb0, ra1, rb1);
+    if (s + 1 < nslab) step(s + 1, ra1, rb1, ra0, rb0);
   }
 
   // lane holds column li, rows 8*(v/4) + 4*lg + v%4 of each 32x32 block
