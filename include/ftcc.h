@@ -208,6 +208,37 @@ int ftcc_td_implicit_step(ftcc_ctx *ctx, int scheme, double h, int max_iter,
                           double damp, double thresh, int singles,
                           double *T1_dev, double *T2_dev, int *iters);
 
+/* Grid points per residual chunk (the slots whose O(n^6) products go out as
+ * one launch): nb = 0 lets the context choose from its workspace budget
+ * (FTCC_WORKSPACE_GB, read when the workspace is allocated) -- the
+ * default; nb > 0 fixes it, at most ng and what that budget allows.
+ * Changing it releases the residual workspace.  ftcc_get_chunk reports the
+ * value in use (allocating the workspace).  No reference counterpart: the
+ * reference evaluates the grid points one at a time. */
+int ftcc_set_chunk(ftcc_ctx *ctx, int nb);
+int ftcc_get_chunk(ftcc_ctx *ctx, int *nb);
+
+/* on != 0: the caller promises that the amplitudes it passes conserve spin
+ * (true of every iterate of the solver started from the driving terms), so
+ * residual calls skip the counting pass over T and need no host
+ * synchronisation.  Results are wrong if the promise is broken. */
+int ftcc_set_trusted(ftcc_ctx *ctx, int on);
+
+/* Quadrature weights g_host[ng] for ftcc_step_graph. */
+int ftcc_set_weights(ftcc_ctx *ctx, const double *g_host);
+
+/* ftcc_step with the weights of ftcc_set_weights, replayed as a HIP graph:
+ * the first call with a set of arguments (pointers, beta, damp, skip mask,
+ * precision) runs eagerly, the second captures the step's device work into
+ * a graph and launches it, later calls relaunch it; out5 as ftcc_step (one
+ * host synchronisation per call).  The context's stream must not be the
+ * null stream; in the spin layout it needs ftcc_set_trusted; product
+ * profiling is off.  Any ftcc_set_* call drops
+ * the graph.  No reference counterpart (host-side launch overhead). */
+int ftcc_step_graph(ftcc_ctx *ctx, double *T1old_dev, double *T2old_dev,
+                    double *T1bar_dev, double *T2bar_dev, double beta,
+                    double damp, uint64_t skip_mask, double out5[5]);
+
 /* Device bytes held by the context (integrals + workspace). */
 int ftcc_workspace_bytes(const ftcc_ctx *ctx, int64_t *bytes);
 

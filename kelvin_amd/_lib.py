@@ -50,6 +50,13 @@ SIGNATURES = {
     "ftcc_point_step": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp,
                                        ctypes.c_double, _c_double_p]),
     "ftcc_td_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "ftcc_set_chunk": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ftcc_get_chunk": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "ftcc_set_trusted": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ftcc_set_weights": (ctypes.c_int, [_vp, _c_double_p]),
+    "ftcc_step_graph": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp,
+                                       ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_uint64, _c_double_p]),
     "ftcc_td_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_int, _vp, _vp]),
     "ftcc_td_implicit_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double,

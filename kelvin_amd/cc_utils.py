@@ -97,9 +97,16 @@ def ft_cc_iter(method, T1old, T2old, F, I, D1, D2, g, G, beta, ng, ti,
     Eold = 888888888.888888888
     nl1 = math.sqrt(eng.sumsq(T1old)) + 0.1
     nl2 = math.sqrt(eng.sumsq(T2old)) + 0.1
+    graph = getattr(eng, "use_graph", False)
+    if graph:
+        eng.set_weights(g)
     while i < max_iter and not converged:
-        d1, d2, n1, n2, E = eng.step(T1old, T2old, T1b, T2b, g, beta, alpha,
-                                     skip)
+        if graph:
+            d1, d2, n1, n2, E = eng.step_graph(T1old, T2old, T1b, T2b, beta,
+                                               alpha, skip)
+        else:
+            d1, d2, n1, n2, E = eng.step(T1old, T2old, T1b, T2b, g, beta,
+                                         alpha, skip)
         res1 = math.sqrt(d1) / nl1
         res2 = math.sqrt(d2) / nl2
         nl1 = math.sqrt(n1) + 0.1

@@ -12,6 +12,13 @@ Extensions over the reference (all optional):
               damping, norms and energies stay FP64.
   refine      with a mixed precision, continue in FP64 after convergence
               until the FP64 iteration converges (default True).
+  graph       run the solve on a stream of its own with every fixed-point
+              step replayed as a HIP graph (launch overhead of small
+              systems); the solver's own iterates are trusted to conserve
+              spin (default False).
+  chunk       grid points per residual chunk (default: chosen from the
+              workspace budget; ``kelvin_amd.batch.autotune_chunk`` times
+              the choices).
 """
 import copy
 import logging
@@ -33,7 +40,8 @@ class ccsd(object):
     def __init__(self, sys, T=0.0, mu=0.0, iprint=0, singles=True, econv=1e-8,
                  tconv=None, max_iter=40, damp=0.0, ngrid=10, realtime=False,
                  athresh=0.0, quad='lin', rt_iter="all", precision="fp64",
-                 refine=True, refine_max_iter=None):
+                 refine=True, refine_max_iter=None, graph=False,
+                 chunk=None):
         self.T = T
         self.mu = mu
         self.finite_T = False if T == 0 else True
@@ -51,6 +59,8 @@ class ccsd(object):
         self.precision = precision
         self.refine = refine
         self.refine_max_iter = refine_max_iter
+        self.graph = graph
+        self.chunk = chunk
         if self.finite_T:
             self.realtime = True
         if not sys.verify(self.T, self.mu):
@@ -85,7 +95,14 @@ class ccsd(object):
                      % ft_utils.HtoK(self.T))
         if not self.singles:
             raise NotImplementedError("CCD is not built for the GPU path")
-        return self._ft_ccsd(T1in=T1, T2in=T2)
+        if not self.graph:
+            return self._ft_ccsd(T1in=T1, T2in=T2)
+        # graph steps need a stream of their own (not the null stream)
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            out = self._ft_ccsd(T1in=T1, T2in=T2)
+        torch.cuda.current_stream().wait_stream(stream)
+        return out
 
     def compute_ESN(self, L1=None, L2=None, gderiv=True, delta=1e-4,
                     richardson=False, fd_econv=1e-12):
@@ -226,6 +243,12 @@ class ccsd(object):
         prec0 = self.precision
         eng = FTEngine(no, nv, ng, precision=prec0)
         self.engine = eng
+        if self.chunk is not None:
+            eng.set_chunk(self.chunk)
+        if self.graph:
+            eng.use_graph = True
+            # iterates started from the driving terms conserve spin
+            eng.set_trusted(T1in is None and T2in is None)
         t0 = time.time()
         if hasattr(self.sys, "g_device_integrals"):
             # integrals built in HBM (the UEG provider)

@@ -59,6 +59,18 @@ struct ftcc_ctx {
 
   // per-slot workspace (nb slots)
   int nb = 1;
+  int nb_fixed = 0;          // grid points per chunk set by the caller (0: auto)
+  bool trusted = false;      // caller promises spin-conserving amplitudes
+  // one FT-CCSD step captured as a HIP graph, and the arguments it was
+  // captured for (the first call with new arguments runs eagerly, so that
+  // every lazily allocated buffer exists before capture)
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
+  const void *gkey[4] = {nullptr, nullptr, nullptr, nullptr};
+  double gbeta = 0.0, gdamp = 0.0;
+  uint64_t gskip = 0;
+  int gprec = -1;
+  bool gwarm = false;
   // imaginary-time propagation: stage input, accumulator and right-hand
   // side (singles / doubles), allocated on first use
   double *td_var[2] = {}, *td_acc[2] = {}, *td_k[2] = {}, *td_k1[2] = {};
@@ -906,8 +918,10 @@ void ensure_workspace(ftcc_ctx *c) {
   // about 2^24 doubles per chunk, but at least two grid points when the
   // budget allows (more tiles per product launch at large n)
   const int64_t want = std::max<int64_t>(2, (1LL << 24) / std::max<int64_t>(1, S2));
-  const int nb = (int)std::min<int64_t>(
-      ng, std::max<int64_t>(1, std::min<int64_t>(want, budget / per_slot)));
+  const int64_t cap = std::max<int64_t>(1, budget / per_slot);
+  int nb = (int)std::min<int64_t>(ng, std::min<int64_t>(want, cap));
+  if (c->nb_fixed > 0)
+    nb = (int)std::min<int64_t>(std::min<int64_t>(ng, c->nb_fixed), cap);
   c->nb = nb;
   c->tau = dalloc(c, S2 * nb);
   c->taut = dalloc(c, S2 * nb);
@@ -930,6 +944,38 @@ void ensure_workspace(ftcc_ctx *c) {
   c->Fbe = dalloc(c, VV * nb);
   c->Fmj = dalloc(c, OO * nb);
   c->ytmp = dalloc(c, sm * nb);
+}
+
+// Release the residual workspace and the operand copies sized by it (they
+// are rebuilt on the next residual with the current chunk size).
+void free_workspace(ftcc_ctx *c) {
+  FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+  for (double **p : {&c->tau, &c->taut, &c->t2r, &c->aring, &c->t2p,
+                     &c->tautp, &c->wr, &c->z, &c->s1, &c->s2, &c->wmnij,
+                     &c->wabef, &c->u, &c->t1T, &c->Fme, &c->FmeT, &c->Fae,
+                     &c->Fmi, &c->Fbe, &c->Fmj, &c->ytmp}) {
+    dfree(c, *p);
+    *p = nullptr;
+  }
+  for (int id = 0; id < NOPS; ++id) {
+    SplitMat &m = split_of(c, id);
+    dfree(c, m.hi);
+    dfree(c, m.lo);
+    m = SplitMat();
+    dfree(c, c->b64[id]);
+    c->b64[id] = nullptr;
+  }
+  c->split_layout = -1;
+  c->b64_layout = -1;
+}
+
+// Forget the captured step (its buffers or kernels may change).
+void drop_graph(ftcc_ctx *c) {
+  if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+  if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
+  c->step_exec = nullptr;
+  c->step_graph = nullptr;
+  c->gwarm = false;
 }
 
 // Residual for `nb` consecutive slices starting at t1/t2 (slice strides
@@ -1287,7 +1333,8 @@ void residual_all(ftcc_ctx *c, const double *T1, const double *T2,
   // spin blocks are exact only for spin-conserving amplitudes (every CC
   // iterate is); anything else runs dense
   const int spin_saved = c->spin;
-  if (c->spin && !amplitudes_conserve_spin(c, T1, T2, c->ng)) c->spin = 0;
+  if (c->spin && !c->trusted && !amplitudes_conserve_spin(c, T1, T2, c->ng))
+    c->spin = 0;
   struct Restore {
     ftcc_ctx *c;
     int v;
@@ -1389,6 +1436,8 @@ int ftcc_destroy(ftcc_ctx *c) {
     (void)hipEventDestroy(p.e1);
   }
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+  if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->grid.d_ti) (void)hipFree(c->grid.d_ti);
   if (c->grid.d_g) (void)hipFree(c->grid.d_g);
@@ -1405,6 +1454,7 @@ const char *ftcc_last_error(const ftcc_ctx *c) {
 
 int ftcc_set_precision(ftcc_ctx *c, int precision) {
   return guarded(c, [&] {
+    drop_graph(c);   // buffers or kernels of the captured step change
     require(precision >= 0 && precision <= 2, "unknown precision");
     c->prec = precision;
   });
@@ -1414,6 +1464,7 @@ int ftcc_set_system(ftcc_ctx *c, int n, const double *en, const double *fock,
                     const double *eri, double beta, double mu,
                     const int *iocc, const int *ivir, double *omega0) {
   return guarded(c, [&] {
+    drop_graph(c);   // buffers or kernels of the captured step change
     require(n > 0 && en && fock && eri, "bad system arguments");
     require(beta > 0.0 && std::isfinite(beta), "beta must be positive");
     const int O = c->no, V = c->nv;
@@ -1493,6 +1544,7 @@ int ftcc_set_system(ftcc_ctx *c, int n, const double *en, const double *fock,
 int ftcc_set_blocks(ftcc_ctx *c, const double *const F[4],
                     const double *const I[9]) {
   return guarded(c, [&] {
+    drop_graph(c);   // buffers or kernels of the captured step change
     require(F && I, "null block pointer");
     const int O = c->no, V = c->nv;
     const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
@@ -1520,6 +1572,7 @@ int ftcc_set_blocks(ftcc_ctx *c, const double *const F[4],
 
 int ftcc_set_spin(ftcc_ctx *c, const int *lab_o, const int *lab_v) {
   return guarded(c, [&] {
+    drop_graph(c);   // buffers or kernels of the captured step change
     require(c->have_ints, "integrals not set (ftcc_set_system/ftcc_set_blocks)",
             FTCC_ESTATE);
     if (!lab_o && !lab_v) {
@@ -1544,6 +1597,7 @@ int ftcc_get_spin(ftcc_ctx *c, int *on) {
 
 int ftcc_set_denominators(ftcc_ctx *c, const double *D1, const double *D2) {
   return guarded(c, [&] {
+    drop_graph(c);   // buffers or kernels of the captured step change
     require(D1 && D2, "null denominator pointer");
     const int O = c->no, V = c->nv;
     const int64_t VO = (int64_t)V * O, S2 = VO * VO;
@@ -1586,6 +1640,7 @@ int ftcc_driving_terms(ftcc_ctx *c, double *T1b, double *T2b) {
 
 int ftcc_set_grid(ftcc_ctx *c, int ng, const double *ti, const double *G) {
   return guarded(c, [&] {
+    drop_graph(c);   // buffers or kernels of the captured step change
     require(ti && G, "null grid pointer");
     require(ng > 0, "ng must be positive");
     Grid &gr = c->grid;
@@ -1960,6 +2015,122 @@ int ftcc_sumsq(ftcc_ctx *c, const double *x, int64_t n, double *out) {
   });
 }
 
+// The device work of one step (residual, integration, damping, norms and
+// energy partial sums reduced into c->red), with no host synchronisation
+// when the amplitudes are trusted.
+void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
+                 double *T2b, double damp, uint64_t skip) {
+  hipStream_t s = c->stream;
+  residual_all(c, T1old, T2old, T1b, T2b, skip);
+  UpdateArgs a = upd(c);
+  a.damp = damp;
+  a.tbar = T1b;
+  a.told = T1old;
+  a.D = c->have_D ? c->D1 : nullptr;
+  a.eint = c->fovT;
+  int nb = update_t1(a, 1, s);
+  reduce_partials(c->partials, nb, c->red, s);
+  a.tbar = T2b;
+  a.told = T2old;
+  a.D = c->have_D ? c->D2 : nullptr;
+  a.eint = c->oovvT;
+  a.t1new = T1old;
+  nb = update_t2(a, 1, s);
+  reduce_partials(c->partials, nb, c->red + 3, s);
+}
+
+void step_outputs(ftcc_ctx *c, double beta, double out5[5]) {
+  read_scalars(c, 6);
+  out5[0] = c->host_red[0];
+  out5[1] = c->host_red[3];
+  out5[2] = c->host_red[1];
+  out5[3] = c->host_red[4];
+  out5[4] = (c->host_red[2] + c->host_red[5]) / beta;
+}
+
+int ftcc_set_chunk(ftcc_ctx *c, int nb) {
+  return guarded(c, [&] {
+    require(nb >= 0, "chunk size must be >= 0");
+    if (nb == c->nb_fixed) return;
+    drop_graph(c);
+    c->nb_fixed = nb;
+    free_workspace(c);
+  });
+}
+
+int ftcc_get_chunk(ftcc_ctx *c, int *nb) {
+  return guarded(c, [&] {
+    require(nb != nullptr, "null output pointer");
+    ensure_workspace(c);
+    *nb = c->nb;
+  });
+}
+
+int ftcc_set_trusted(ftcc_ctx *c, int on) {
+  return guarded(c, [&] {
+    drop_graph(c);
+    c->trusted = on != 0;
+  });
+}
+
+int ftcc_set_weights(ftcc_ctx *c, const double *g) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(g != nullptr, "null weights");
+    upload_g(c, g);
+  });
+}
+
+int ftcc_step_graph(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
+                    double *T2b, double beta, double damp, uint64_t skip,
+                    double out5[5]) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(T1old && T2old && T1b && T2b && out5, "null pointer");
+    require(beta > 0.0, "beta must be positive");
+    require(!c->spin || c->trusted,
+            "graph steps need trusted amplitudes (ftcc_set_trusted) in the "
+            "spin layout");
+    require(!c->prof_on, "product profiling is off in graph steps");
+    require(c->stream != nullptr,
+            "graph steps need a context created on a non-default stream");
+    const void *key[4] = {T1old, T2old, T1b, T2b};
+    const bool same = c->gwarm && std::equal(key, key + 4, c->gkey) &&
+                      c->gbeta == beta && c->gdamp == damp &&
+                      c->gskip == skip && c->gprec == c->prec;
+    if (same && c->step_exec) {
+      FTCC_CHECK_HIP(hipGraphLaunch(c->step_exec, c->stream));
+    } else if (same) {
+      // second call with these arguments: capture, then launch
+      FTCC_CHECK_HIP(hipStreamBeginCapture(c->stream,
+                                           hipStreamCaptureModeThreadLocal));
+      try {
+        step_device(c, T1old, T2old, T1b, T2b, damp, skip);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      FTCC_CHECK_HIP(hipStreamEndCapture(c->stream, &c->step_graph));
+      FTCC_CHECK_HIP(
+          hipGraphInstantiate(&c->step_exec, c->step_graph, nullptr, nullptr, 0));
+      FTCC_CHECK_HIP(hipGraphLaunch(c->step_exec, c->stream));
+    } else {
+      // new arguments: run eagerly (allocations happen here)
+      drop_graph(c);
+      step_device(c, T1old, T2old, T1b, T2b, damp, skip);
+      std::copy(key, key + 4, c->gkey);
+      c->gbeta = beta;
+      c->gdamp = damp;
+      c->gskip = skip;
+      c->gprec = c->prec;
+      c->gwarm = true;
+    }
+    step_outputs(c, beta, out5);
+  });
+}
+
 int ftcc_step(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
               double *T2b, const double *g, double beta, double damp,
               uint64_t skip, double out5[5]) {
@@ -1967,31 +2138,9 @@ int ftcc_step(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
     check_ready(c);
     require(T1old && T2old && T1b && T2b && g && out5, "null pointer");
     require(beta > 0.0, "beta must be positive");
-    hipStream_t s = c->stream;
     upload_g(c, g);
-    residual_all(c, T1old, T2old, T1b, T2b, skip);
-    UpdateArgs a = upd(c);
-    a.damp = damp;
-    // singles first: the doubles energy needs the new T1
-    a.tbar = T1b;
-    a.told = T1old;
-    a.D = c->have_D ? c->D1 : nullptr;
-    a.eint = c->fovT;
-    int nb = update_t1(a, 1, s);
-    reduce_partials(c->partials, nb, c->red, s);
-    a.tbar = T2b;
-    a.told = T2old;
-    a.D = c->have_D ? c->D2 : nullptr;
-    a.eint = c->oovvT;
-    a.t1new = T1old;
-    nb = update_t2(a, 1, s);
-    reduce_partials(c->partials, nb, c->red + 3, s);
-    read_scalars(c, 6);
-    out5[0] = c->host_red[0];
-    out5[1] = c->host_red[3];
-    out5[2] = c->host_red[1];
-    out5[3] = c->host_red[4];
-    out5[4] = (c->host_red[2] + c->host_red[5]) / beta;
+    step_device(c, T1old, T2old, T1b, T2b, damp, skip);
+    step_outputs(c, beta, out5);
   });
 }
 

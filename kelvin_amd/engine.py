@@ -283,6 +283,41 @@ class FTEngine:
                    out)
         return tuple(out)
 
+    def step_graph(self, T1old, T2old, T1b, T2b, beta, damp, skip_mask=0):
+        """``step`` replayed as a HIP graph (weights from ``set_weights``;
+        the engine must live on a non-default stream; spin layout needs
+        ``set_trusted(True)``)."""
+        for t, s, nm in ((T1old, self.t1_shape(), "T1old"),
+                         (T2old, self.t2_shape(), "T2old"),
+                         (T1b, self.t1_shape(), "T1bar"),
+                         (T2b, self.t2_shape(), "T2bar")):
+            self._check(t, s, nm)
+        out = (ctypes.c_double * 5)()
+        self._call("ftcc_step_graph", _vp(T1old), _vp(T2old), _vp(T1b),
+                   _vp(T2b), float(beta), float(damp),
+                   ctypes.c_uint64(skip_mask), out)
+        return tuple(out)
+
+    def set_weights(self, g):
+        """Quadrature weights for ``step_graph``."""
+        g_a, g_p = _dbuf(g)
+        self._call("ftcc_set_weights", g_p)
+
+    def set_trusted(self, on=True):
+        """Promise spin-conserving amplitudes: no counting pass, no host
+        synchronisation in the residual."""
+        self._call("ftcc_set_trusted", int(bool(on)))
+
+    def set_chunk(self, nb):
+        """Grid points per residual chunk (0: automatic)."""
+        self._call("ftcc_set_chunk", int(nb))
+
+    def chunk(self):
+        """Grid points per residual chunk in use."""
+        nb = ctypes.c_int()
+        self._call("ftcc_get_chunk", ctypes.byref(nb))
+        return nb.value
+
     def _check_amps(self, T1, T2, T1b, T2b):
         for t, s, nm in ((T1, self.t1_shape(), "T1"),
                          (T2, self.t2_shape(), "T2"),

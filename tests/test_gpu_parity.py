@@ -868,3 +868,45 @@ def test_tdccsd_implicit_matches_rk4(tprop, ng_td, ng_rk4):
     td = TDCCSD(sysg, prop, T=T, mu=mu, ngrid=ng_td)
     Eout, Eccout = td.run()
     assert abs(Eccout - Eccref) < 1e-4
+
+
+# ------------------------------------------------------------ graphs, batch
+def test_graph_steps_equal_eager_steps():
+    """ccsd(graph=True) (trusted iterates, every step a replayed HIP graph on
+    a stream of its own) gives the eager solver's grand potential and
+    kelvin's golden G1; a fixed chunk size does not change it."""
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    g = GOLD["G1"]
+    sysg, s = _ueg_gold("G1")
+    ref = ccsd(sysg, T=s["T"], mu=s["mu"], **g["ccsd"])
+    E0, Ecc0 = ref.run()
+    for chunk in (None, 1, 3):
+        cc = ccsd(sysg, T=s["T"], mu=s["mu"], graph=True, chunk=chunk,
+                  **g["ccsd"])
+        E, Ecc = cc.run()
+        assert abs(Ecc - Ecc0) < 1e-12
+        assert len(cc.history) == len(ref.history)
+        if chunk is not None:
+            assert cc.engine.chunk() == chunk
+    assert abs(Ecc - g["Omega_cc"]) < g["tol"]
+
+
+def test_batch_equals_loop():
+    """Concurrent graph solves on four streams give the plain loop's
+    results (UEG, 7 and 19 k-points, uneven grids)."""
+    from kelvin_amd import batch
+    dev()
+    insts = [dict(nk=7, ng=6, t_over_tf=0.5, rs=2.0, N=14),
+             dict(nk=19, ng=8, t_over_tf=0.3, rs=3.0, N=14),
+             dict(nk=7, ng=10, t_over_tf=0.8, rs=1.5, N=14),
+             dict(nk=19, ng=4, t_over_tf=1.0, rs=4.0, N=14)]
+    kw = dict(damp=0.5, max_iter=40, econv=1e-9)
+    best, times = batch.autotune_chunk(19, 8)
+    assert best in times and 1 <= best <= 8
+    loop, _ = batch.solve_loop(insts, **kw)
+    bat, _ = batch.solve_batch(insts, workers=4, chunks={(19, 8): best},
+                               **kw)
+    for a, b in zip(loop, bat):
+        assert abs(a["Omega"] - b["Omega"]) < 1e-10
+        assert a["iterations"] == b["iterations"]
