@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--precision", default="bf16x3",
                     choices=["bf16x3", "fp32", "fp64"])
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="grid points per residual chunk (0: the library's "
+                         "choice from its workspace budget)")
     ap.add_argument("--nk", type=int, default=57)
     ap.add_argument("--ng", type=int, default=24)
     ap.add_argument("--rs", type=float, default=2.0)
@@ -168,6 +171,8 @@ def main():
     ng = args.ng
     ti, g, G = quadrature.ft_quad(ng, beta, 'lin')
     eng = FTEngine(n, n, ng, precision=args.precision)
+    if args.chunk:
+        eng.set_chunk(args.chunk)
     fock, eri, e1 = sysg.g_device_integrals(eng)     # built in HBM
     log("UEG nk=%d n=%d beta=%.4f mu=%.6f, integrals built on the device "
         "in %.1f s" % (args.nk, n, beta, mu, time.time() - t0))
@@ -277,7 +282,7 @@ def main():
                                   args.nelec, ng),
                    "n": n, "ng": ng, "precision": args.precision,
                    "damp": args.damp, "spin_blocked": spin_blocked,
-                   "layout": layout},
+                   "layout": layout, "chunk": eng.chunk()},
         "fp64": fp64,
         "omega_step_delta_vs_fp64": d_omega,
         "textbook_tflops": algo * value / 1e12,

@@ -902,24 +902,27 @@ void ensure_integrals(ftcc_ctx *c) {
 }
 
 // Residual workspace: as many grid points per chunk as fit the budget
-// (FTCC_WORKSPACE_GB, default 24 GiB) and are needed to fill the GPU.
+// (FTCC_WORKSPACE_GB, default 120 GiB: four grid points at n = 114, the
+// whole grid at n <= 66) -- larger chunks measured faster at every size
+// tried (fewer, fuller launches), 2.8% at n = 114 from two to four.
 void ensure_workspace(ftcc_ctx *c) {
   if (c->tau) return;
   const int O = c->no, V = c->nv, ng = c->ng;
   const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
   const int64_t S2 = VV * OO;
   const int64_t sm = std::max<int64_t>(VV, std::max<int64_t>(VO, OO));
+  // ten n^4 intermediates, Wmnij/Wabef/U, the seven per-slot product
+  // operands split into bf16 pairs (bf16x3) and as FP64 spin-blocked copies
+  // (FP64/FP32 modes; both exist after a precision switch)
   const int64_t per_slot =
       8 * (10 * S2 + OO * OO + VV * VV + OO * VO + 8 * sm) +
-      4 * 7 * (S2 + 64 * std::max<int64_t>(VV, std::max<int64_t>(VO, OO)));
-  double budget_gb = 48.0;
+      4 * 7 * (S2 + 64 * std::max<int64_t>(VV, std::max<int64_t>(VO, OO))) +
+      8 * 7 * S2;
+  double budget_gb = 120.0;
   if (const char *env = getenv("FTCC_WORKSPACE_GB")) budget_gb = atof(env);
   const int64_t budget = (int64_t)(budget_gb * (1LL << 30));
-  // about 2^24 doubles per chunk, but at least two grid points when the
-  // budget allows (more tiles per product launch at large n)
-  const int64_t want = std::max<int64_t>(2, (1LL << 24) / std::max<int64_t>(1, S2));
   const int64_t cap = std::max<int64_t>(1, budget / per_slot);
-  int nb = (int)std::min<int64_t>(ng, std::min<int64_t>(want, cap));
+  int nb = (int)std::min<int64_t>(ng, cap);
   if (c->nb_fixed > 0)
     nb = (int)std::min<int64_t>(std::min<int64_t>(ng, c->nb_fixed), cap);
   c->nb = nb;
