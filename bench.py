@@ -66,7 +66,8 @@ def measured_traffic(precision, n, ng, layout):
     """HBM bytes per launch of the dominant kernel, from the committed
     FETCH_SIZE/WRITE_SIZE profile of this workload (a line
     'traffic_bytes_per_launch <precision> <n> <ng> <layout> <bytes>',
-    layout 'spin-packed', 'spin' or 'dense'), or None."""
+    layout e.g. 'spin-packed-c4': the operand layout and the grid points per
+    launch), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
@@ -252,7 +253,8 @@ def main():
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
         "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
-        "traffic": measured_traffic(args.precision, n, ng, layout),
+        "traffic": measured_traffic(args.precision, n, ng,
+                                    "%s-c%d" % (layout, eng.chunk())),
         "kernel": "%s (the six O(n^6) products of the residual)"
                   % ("sbgemm_kernel" if args.precision == "bf16x3"
                      else "gemm_mfma_kernel<%s>" % ("double" if args.precision == "fp64" else "float")),
