@@ -1060,8 +1060,16 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   run(c, mk(O, O, V, 0.5, 1.0, c->Fme, V, 1, VO, t1, O, 1, VO, c->Fmj, O, 1,
             OO, nb), P64);
 
+  // Each W intermediate starts with its O(n^6) product written with
+  // beta = 0 (the split-bf16 GEMM then never reads C, whose latency it
+  // cannot hide); the O(n^5) terms and the integral (an addend D) accumulate
+  // onto it in the FP64 GEMM.  In the spin layout the product writes the
+  // allowed blocks only, and nothing reads the others.
   // Wmnij = oooo + P(ij) t_j^e <mn||ie> + 1/4 oovv.tau          (G1)
-  // oooo + sum_e <mn||ie> t_j^e into [(m,n,i), j]
+  product(c, mk((int)OO, (int)OO, (int)VV, 0.25, 0.0, oovv, VV, 1, 0, c->tau,
+                OO, 1, S2, c->wmnij, OO, 1, O4, nb), OP_OOVV_A, OP_TAU_B,
+          LADDER);
+  // + oooo + sum_e <mn||ie> t_j^e into [(m,n,i), j]
   if (s5) {
     run_spin(c, sgem(1.0, 1.0, ooov, 0, t1, VO, c->wmnij, O4, oooo), P5,
              {{'m', 'o', OO * V, 0, O * OO}, {'n', 'o', O * V, 0, OO},
@@ -1090,11 +1098,11 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sC2 = OO;
     run(c, g, P5);
   }
-  product(c, mk((int)OO, (int)OO, (int)VV, 0.25, 1.0, oovv, VV, 1, 0, c->tau,
-                OO, 1, S2, c->wmnij, OO, 1, O4, nb), OP_OOVV_A, OP_TAU_B,
-          LADDER);
 
   // Wabef = vvvv - P(ab) t_m^b <am||ef> + 1/4 tau.oovv          (G2)
+  product(c, mk((int)VV, (int)VV, (int)OO, 0.25, 0.0, c->tau, OO, 1, S2, oovv,
+                VV, 1, 0, c->wabef, VV, 1, V4, nb), OP_TAU_A, OP_OOVV_B,
+          LADDER);
   if (s5) {
     // vvvv - sum_m t_m^b <am||ef>: C[b][(a,e,f)] = vvvv - t1[b,m] vovv[a,m,ef]
     run_spin(c, sgem(-1.0, 1.0, t1, VO, vovv, 0, c->wabef, V4, vvvv), P5,
@@ -1128,11 +1136,11 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sC2 = VV;
     run(c, g, P5);
   }
-  product(c, mk((int)VV, (int)VV, (int)OO, 0.25, 1.0, c->tau, OO, 1, S2, oovv,
-                VV, 1, 0, c->wabef, VV, 1, V4, nb), OP_TAU_A, OP_OOVV_B,
-          LADDER);
 
   // Wr[e,m,b,j] = <mb||ej> + t_j^f <mb||ef> - t_n^b <mn||ej> - A.<mn||ef>
+  product(c, mk((int)VO, (int)VO, (int)VO, -1.0, 0.0, c->oovv_ring, VO, 1, 0,
+                c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), OP_RING_A,
+          OP_ARING_B, RING_G5);                                          // G5
   // <mb||ej> + t_j^f <mb||ef> = ovvo_ring - sum_f vovv[b,m,e,f] t1[f,j]:
   // rows (e,m,b) of Wr
   if (s5) {
@@ -1167,15 +1175,11 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sC2 = VO;
     run(c, g, P5);
   }
-  product(c, mk((int)VO, (int)VO, (int)VO, -1.0, 1.0, c->oovv_ring, VO, 1, 0,
-                c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), OP_RING_A,
-          OP_ARING_B, RING_G5);                                          // G5
 
   // Z[ai,bj] = t2r.Wr - t_i^e t_m^a <mb||ej>                    (G6)
   put_operand(c, OP_WR_B, c->wr, 1, VO, S2, nb);
-  // spin blocks leave the spin-forbidden part of Z unwritten: clear it
-  // (the spin-aware assembly below never reads it)
-  if (c->spin && !s5) fill(c->z, S2 * nb, 0.0, s);
+  // spin blocks leave the spin-forbidden part of Z unwritten; the
+  // spin-aware assembly below never reads it
   product(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr,
                 VO, 1, S2, c->z, VO, 1, S2, nb), OP_T2R_A, OP_WR_B, RING_G6);
   if (s5) {
@@ -1203,17 +1207,16 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
 
-  // doubles accumulator in t2bar: vvoo + 1/2 tau.Wmnij + 1/2 Wabef.tau; in
-  // the spin layout G3 writes every allowed element (beta = 0) and the
-  // assembly adds vvoo, reading neither on forbidden elements
-  if (!s5) bcast(t2bar, vvoo, S2, nb, 1.0, 0.0, s);
+  // doubles: G3 = 1/2 tau.Wmnij into t2bar and G4 = 1/2 Wabef.tau into
+  // aring (free after G5), both with beta = 0; the assembly adds them, vvoo
+  // and the rest
   put_operand(c, OP_WMNIJ_B, c->wmnij, 1, OO, O4, nb);
   put_operand(c, OP_WABEF_A, c->wabef, VV, 1, V4, nb);
-  product(c, mk((int)VV, (int)OO, (int)OO, 0.5, s5 ? 0.0 : 1.0, c->tau, OO,
-                1, S2, c->wmnij, OO, 1, O4, t2bar, OO, 1, S2, nb),
-          OP_TAU_A, OP_WMNIJ_B, LADDER);                                 // G3
-  product(c, mk((int)VV, (int)OO, (int)VV, 0.5, 1.0, c->wabef, VV, 1, V4,
-                c->tau, OO, 1, S2, t2bar, OO, 1, S2, nb), OP_WABEF_A,
+  product(c, mk((int)VV, (int)OO, (int)OO, 0.5, 0.0, c->tau, OO, 1, S2,
+                c->wmnij, OO, 1, O4, t2bar, OO, 1, S2, nb), OP_TAU_A,
+          OP_WMNIJ_B, LADDER);                                           // G3
+  product(c, mk((int)VV, (int)OO, (int)VV, 0.5, 0.0, c->wabef, VV, 1, V4,
+                c->tau, OO, 1, S2, c->aring, OO, 1, S2, nb), OP_WABEF_A,
           OP_TAU_B, LADDER);                                             // G4
 
   // S1 (antisymmetrised in ab): Fbe[b,e] t2[a,e,i,j] + t1[a,m] vooo[b,m,i,j]
@@ -1265,10 +1268,10 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sC2 = OO;
     run(c, g, P5);
   }
-  // forbidden blocks: written as zero without reading S1, S2, Z
+  // forbidden blocks (spin layout): written as zero without reading
   r2_final(t2bar, c->s1, c->s2, c->z, V, O, nb, s,
-           s5 ? c->d_lab_v : nullptr, s5 ? c->d_lab_o : nullptr,
-           s5 ? vvoo : nullptr);
+           c->spin ? c->d_lab_v : nullptr, c->spin ? c->d_lab_o : nullptr,
+           vvoo, c->aring);
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
   //                    + 1/2 vovv.t2p + 1/2 t2.ooov_emni)

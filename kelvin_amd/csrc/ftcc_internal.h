@@ -51,9 +51,11 @@ struct Gemm {
   // optional scatter of the output: row m of the product is written to row
   // rmap[m] of C (column n to cmap[n]); -1 drops it
   const int *rmap = nullptr, *cmap = nullptr;
-  // optional addend: C = alpha*A.B + beta*D with D laid out like C (its own
-  // strides; sD1 = 0 broadcasts one matrix over the slices), C is not read
+  // optional addend: C = alpha*A.B + beta*C + gamma*D with D laid out like
+  // C (its own strides; sD1 = 0 broadcasts one matrix over the slices); C is
+  // read only when beta != 0
   const double *D = nullptr;
+  double gamma = 1.0;
   int64_t sDm = 0, sDn = 0, sD1 = 0, sD2 = 0;
   // optional rows/columns over up to three tensor indices: row
   // m = (m0*M1 + m1)*M2 + m2 sits at m0*sXm0 + m1*sXm1 + m2*sXm for operand
@@ -205,14 +207,15 @@ void antisym_add(double *W, const double *Q, int d0, int d1, int d2, int d3,
 // Wr[e,m,b,j] = ovvo_ring[e,m,b,j] - Q1[b,m,e,j] + Q2[m,b,j,e]
 void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
                    const double *Q2, int nv, int no, int nb, hipStream_t s);
-// T2bar[a,b,i,j] = -( acc [+ add] + S1 - S1^(ab) + S2 - S2^(ij) + PP(Z ring) )
-// with add one slice for every grid point (or nullptr); with spin labels
+// T2bar[a,b,i,j] = -( acc [+ add] [+ add2] + S1 - S1^(ab) + S2 - S2^(ij)
+//                    + PP(Z ring) ) with add one slice for every grid point
+// and add2 one per grid point (either may be nullptr); with spin labels
 // (device arrays, or nullptr), spin-forbidden elements are written as zero
 // without reading the operands
 void r2_final(double *t2bar, const double *S1, const double *S2,
               const double *Z, int nv, int no, int nb, hipStream_t s,
               const int *lab_v = nullptr, const int *lab_o = nullptr,
-              const double *add = nullptr);
+              const double *add = nullptr, const double *add2 = nullptr);
 // x[n][c] (nb batches of n*c) -> out[c][n] transposed copy for small mats
 void transpose2(const double *in, double *out, int r, int c, int nb,
                 hipStream_t s);

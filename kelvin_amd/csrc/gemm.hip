@@ -60,7 +60,8 @@ struct GemmDev {
   double *partial;   // [R][nbC][M][N] when splits*R > 1
   int nbC;           // number of distinct C batches (nb1 or nb1*nb2)
   const int *rmap, *cmap;   // output scatter (nullptr: identity)
-  const double *D;          // addend laid out like C (nullptr: C itself)
+  const double *D;          // addend gamma*D laid out like C (or nullptr)
+  double gamma;
   int64_t sDm, sDn, sD1, sD2;
   // rows m = (m0*M1 + m1)*M2 + m2 and columns likewise (one index: M1 = 1,
   // M2 = 2^30); the strides above belong to the innermost index
@@ -314,10 +315,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
           const int64_t oc = crow[lr], occ = ccol[lc];
           if (oc >= 0 && occ >= 0) {
             double *c = C + oc + occ;
-            if (D)
-              *c = p.alpha * x + p.beta * D[drow[lr] + dcol[lc]];
-            else
-              *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
+            double y = p.alpha * x;
+            if (D) y += p.gamma * D[drow[lr] + dcol[lc]];
+            if (p.beta != 0.0) y += p.beta * *c;
+            *c = y;
           }
         } else if (m0 + lr < p.M && n0 + lc < p.N) {
           P[(int64_t)(m0 + lr) * p.N + n0 + lc] = x;
@@ -351,13 +352,13 @@ __global__ void reduce_partial_kernel(GemmDev p, int R) {
     const Lvl lr = levels(m, p.M1, p.M2), lc = levels(n, p.N1, p.N2);
     double *c = p.C + b1 * p.sC1 + b2 * p.sC2 + at(lr, p.sCm0, p.sCm1, p.sCm) +
                 at(lc, p.sCn0, p.sCn1, p.sCn);
+    double y = p.alpha * s;
     if (p.D)
-      *c = p.alpha * s +
-           p.beta * p.D[b1 * p.sD1 + b2 * p.sD2 +
-                        at(lr, p.sDm0, p.sDm1, p.sDm) +
-                        at(lc, p.sDn0, p.sDn1, p.sDn)];
-    else
-      *c = (p.beta == 0.0) ? p.alpha * s : p.alpha * s + p.beta * *c;
+      y += p.gamma * p.D[b1 * p.sD1 + b2 * p.sD2 +
+                         at(lr, p.sDm0, p.sDm1, p.sDm) +
+                         at(lc, p.sDn0, p.sDn1, p.sDn)];
+    if (p.beta != 0.0) y += p.beta * *c;
+    *c = y;
   }
 }
 
@@ -459,6 +460,7 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   d.rmap = g.rmap;
   d.cmap = g.cmap;
   d.D = g.D;
+  d.gamma = g.gamma;
   d.sDm = g.sDm;
   d.sDn = g.sDn;
   d.sD1 = g.sD1;

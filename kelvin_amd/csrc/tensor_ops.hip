@@ -272,6 +272,7 @@ __global__ void ring_assemble_kernel(double *Wr, const double *ovvo_ring,
 
 // t2bar[a,b,i,j] = -( acc[a,b,i,j]            (already in t2bar)
 //                     [+ add[a,b,i,j]]         (one slice, all grid points)
+//                     [+ add2[y][a,b,i,j]]     (one per grid point)
 //                     + S1[a,b,i,j] - S1[b,a,i,j]
 //                     + S2[a,b,i,j] - S2[a,b,j,i]
 //                     + Z[a,i,b,j] - Z[b,i,a,j] - Z[a,j,b,i] + Z[b,j,a,i] )
@@ -285,7 +286,8 @@ __global__ void ring_assemble_kernel(double *Wr, const double *ovvo_ring,
 // reads is itself allowed.
 __global__ __launch_bounds__(256) void r2_final_pair_kernel(
     double *t2bar, const double *S1, const double *S2, const double *Z,
-    const double *add, int nv, int no, const int *lv, const int *lo) {
+    const double *add, const double *add2, int nv, int no, const int *lv,
+    const int *lo) {
   __shared__ double tA[32][33], tB[32][33], tZ1[32][33], tZ2[32][33];
   const int tiles = (no + 31) / 32;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
@@ -339,6 +341,10 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
     if (add) {
       vab += add[nab];
       vba += add[nba];
+    }
+    if (add2) {
+      vab += add2[y * per + nab];
+      vba += add2[y * per + nba];
     }
     out[nab] = -vab;
     if (a != b) out[nba] = -vba;
@@ -517,13 +523,15 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
 
 void r2_final(double *t2bar, const double *S1, const double *S2,
               const double *Z, int nv, int no, int nb, hipStream_t s,
-              const int *lab_v, const int *lab_o, const double *add) {
+              const int *lab_v, const int *lab_o, const double *add,
+              const double *add2) {
   const int tiles = (no + 31) / 32;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   const int64_t blocks = (int64_t)nb * npair * tiles * tiles;
   require(blocks < (1LL << 31), "r2 assembly grid too large");
   hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks), dim3(256),
-                     0, s, t2bar, S1, S2, Z, add, nv, no, lab_v, lab_o);
+                     0, s, t2bar, S1, S2, Z, add, add2, nv, no, lab_v,
+                     lab_o);
   FTCC_CHECK_HIP(hipGetLastError());
 }
 
