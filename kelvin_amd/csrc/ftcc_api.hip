@@ -42,10 +42,9 @@ struct ftcc_ctx {
   double *I[9] = {};  // vvvv vvvo vovv vvoo vovo oovv vooo ooov oooo
   // derived layouts
   double *oovv_ring = nullptr;  // [e,m,f,n] = oovv[m,n,e,f]
-  double *oovv_fmne = nullptr;  // [f,m,n,e] = oovv[m,n,e,f]
   double *ovvo_ring = nullptr;  // [e,m,b,j] = <mb||ej> = -vovo[b,m,e,j]
   double *ooov_p1 = nullptr;    // [m,i,n,e] = ooov[m,n,i,e]
-  double *ooov_emni = nullptr;  // [e,m,n,i] = ooov[n,m,i,e]
+  double *ooov_iemn = nullptr;  // [i,e,m,n] = ooov[n,m,i,e]
   double *oovvT = nullptr;      // [a,b,i,j] = oovv[i,j,a,b]
   double *vovv_embf = nullptr;  // [e,m,b,f] = vovv[b,m,e,f]
   double *ooov_emnj = nullptr;  // [e,m,n,j] = ooov[m,n,j,e]
@@ -193,8 +192,6 @@ void build_derived(ftcc_ctx *c) {
   const int doovv[4] = {O, O, V, V};
   const int pr[4] = {2, 0, 3, 1};
   derive(c->I[5], c->oovv_ring, doovv, pr, 1.0, s);
-  const int pf[4] = {3, 0, 1, 2};
-  derive(c->I[5], c->oovv_fmne, doovv, pf, 1.0, s);
   const int pt[4] = {2, 3, 0, 1};
   derive(c->I[5], c->oovvT, doovv, pt, 1.0, s);
   const int dvovo[4] = {V, O, V, O};
@@ -203,8 +200,8 @@ void build_derived(ftcc_ctx *c) {
   const int dooov[4] = {O, O, O, V};
   const int p1[4] = {0, 2, 1, 3};
   derive(c->I[7], c->ooov_p1, dooov, p1, 1.0, s);
-  const int pe[4] = {3, 1, 0, 2};
-  derive(c->I[7], c->ooov_emni, dooov, pe, 1.0, s);
+  const int pe[4] = {2, 3, 1, 0};
+  derive(c->I[7], c->ooov_iemn, dooov, pe, 1.0, s);
   const int pj[4] = {3, 0, 1, 2};
   derive(c->I[7], c->ooov_emnj, dooov, pj, 1.0, s);
   const int dvovv[4] = {V, O, V, V};
@@ -334,17 +331,18 @@ bool rule_holds(const char *rule, const std::map<char, int> &sp) {
   return left == right;
 }
 
-// C[M,N] = alpha * A[M,K] B[K,N] + beta * (D or C) over the spin-allowed
-// blocks only: M and N run over up to three orbital indices each, K over
-// one, and every assignment of spins to the letters under which the rules of
+// C[M,N] = alpha * A[M,K] B[K,N] + beta * C + gamma * D over the
+// spin-allowed blocks only: M, N and K run over up to three orbital indices
+// each (K over several only with both operands K-fastest), and every
+// assignment of spins to the letters under which the rules of
 // A, B and C all hold is one launch over the product of the letters' spin
 // runs (the other blocks of A and B are zero, those of C are not written).
 // `base` carries the pointers, alpha/beta, D, the slice count and the slice
 // strides; the rest is filled here.
 void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
               const std::vector<SIdx> &Mi, const std::vector<SIdx> &Ni,
-              const SIdx &Ki, const char *ruleA, const char *ruleB,
-              const char *ruleC) {
+              const std::vector<SIdx> &Ki, const char *ruleA,
+              const char *ruleB, const char *ruleC) {
   std::vector<char> letters;
   auto add = [&](char l) {
     if (std::find(letters.begin(), letters.end(), l) == letters.end())
@@ -352,9 +350,10 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
   };
   for (const SIdx &x : Mi) add(x.l);
   for (const SIdx &x : Ni) add(x.l);
-  add(Ki.l);
+  for (const SIdx &x : Ki) add(x.l);
   require(Mi.size() >= 1 && Mi.size() <= 3 && Ni.size() >= 1 &&
-              Ni.size() <= 3 && letters.size() == Mi.size() + Ni.size() + 1,
+              Ni.size() <= 3 && Ki.size() >= 1 && Ki.size() <= 3 &&
+              letters.size() == Mi.size() + Ni.size() + Ki.size(),
           "spin-restricted GEMM: bad index lists");
   auto start = [&](const SIdx &x, int sp) {
     return x.role == 'o' ? c->o_start[sp] : c->v_start[sp];
@@ -365,6 +364,7 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
   const bool writes_all = base.beta == 0.0 || base.D != nullptr;
   std::set<std::vector<int>> c_blocks;   // C spin blocks that were written
   const int L = (int)letters.size();
+  const SIdx z{0, 'o', 0, 0, 0};
   for (int mask = 0; mask < (1 << L); ++mask) {
     std::map<char, int> sp;
     for (int k = 0; k < L; ++k) sp[letters[k]] = (mask >> k) & 1;
@@ -373,8 +373,8 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
       continue;
     Gemm g = base;
     int64_t oa = 0, ob = 0, oc = 0;
-    int64_t M = 1, N = 1;
-    std::vector<int> cm, cn;
+    int64_t M = 1, N = 1, K = 1;
+    std::vector<int> cm, cn, ck;
     for (const SIdx &x : Mi) {
       const int s0 = sp[x.l];
       oa += start(x, s0) * x.sa;
@@ -389,32 +389,35 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
       cn.push_back(count(x, s0));
       N *= count(x, s0);
     }
-    const int sk = sp[Ki.l];
-    oa += start(Ki, sk) * Ki.sa;
-    ob += start(Ki, sk) * Ki.sb;
-    const int K = count(Ki, sk);
+    for (const SIdx &x : Ki) {
+      const int s0 = sp[x.l];
+      oa += start(x, s0) * x.sa;
+      ob += start(x, s0) * x.sb;
+      ck.push_back(count(x, s0));
+      K *= count(x, s0);
+    }
     std::vector<int> cb;
     for (const SIdx &x : Mi) cb.push_back(sp[x.l]);
     for (const SIdx &x : Ni) cb.push_back(sp[x.l]);
     c_blocks.insert(cb);
     if (M == 0 || N == 0) continue;
-    require(M < (1LL << 31) && N < (1LL << 31), "spin block too large");
+    require(M < (1LL << 31) && N < (1LL << 31) && K < (1LL << 31),
+            "spin block too large");
     g.M = (int)M;
     g.N = (int)N;
-    g.K = K;
+    g.K = (int)K;
     g.A = base.A + oa;
     g.B = base.B + ob;
     g.C = base.C + oc;
     g.D = base.D ? base.D + oc : nullptr;
-    g.sAk = Ki.sa;
-    g.sBk = Ki.sb;
     // levels: the listed indices fill the innermost slots
-    const int nm = (int)Mi.size(), nn = (int)Ni.size();
-    const SIdx z{0, 'o', 0, 0, 0};
+    const int nm = (int)Mi.size(), nn = (int)Ni.size(), nk = (int)Ki.size();
     const SIdx &m0 = nm == 3 ? Mi[0] : z, &m1 = nm >= 2 ? Mi[nm - 2] : z,
                &m2 = Mi[nm - 1];
     const SIdx &n0 = nn == 3 ? Ni[0] : z, &n1 = nn >= 2 ? Ni[nn - 2] : z,
                &n2 = Ni[nn - 1];
+    const SIdx &k0 = nk == 3 ? Ki[0] : z, &k1 = nk >= 2 ? Ki[nk - 2] : z,
+               &k2 = Ki[nk - 1];
     g.M1 = nm >= 2 ? cm[nm - 2] : 1;
     g.M2 = cm[nm - 1];
     g.N1 = nn >= 2 ? cn[nn - 2] : 1;
@@ -431,9 +434,21 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
     g.sCn0 = g.sDn0 = n0.sc;
     g.sCn1 = g.sDn1 = n1.sc;
     g.sCn = g.sDn = n2.sc;
+    g.sAk = k2.sa;
+    g.sBk = k2.sb;
+    if (nk >= 2) {
+      g.K1 = ck[nk - 2];
+      g.K2 = ck[nk - 1];
+      g.sAk0 = k0.sa;
+      g.sAk1 = k1.sa;
+      g.sBk0 = k0.sb;
+      g.sBk1 = k1.sb;
+    }
     if (K == 0) {
       if (!writes_all) continue;
-      g.K = 0;   // C = beta * (D or C) only
+      g.K = 0;   // C = beta * C + gamma * D only
+      g.K1 = 1;
+      g.K2 = 0;
     }
     run(c, g, prec);
   }
@@ -457,6 +472,13 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
                 "spin-restricted GEMM leaves an allowed output block unset");
     }
   }
+}
+
+void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
+              const std::vector<SIdx> &Mi, const std::vector<SIdx> &Ni,
+              const SIdx &Ki, const char *ruleA, const char *ruleB,
+              const char *ruleC) {
+  run_spin(c, base, prec, Mi, Ni, std::vector<SIdx>{Ki}, ruleA, ruleB, ruleC);
 }
 
 // One of the six O(n^6) products: split-bf16 operands in bf16x3 mode
@@ -889,11 +911,10 @@ void ensure_integrals(ftcc_ctx *c) {
                           S2,      V * OO * O, OO * VO, OO * OO};
   for (int k = 0; k < 9; ++k) c->I[k] = dalloc(c, isz[k]);
   c->oovv_ring = dalloc(c, S2);
-  c->oovv_fmne = dalloc(c, S2);
   c->oovvT = dalloc(c, S2);
   c->ovvo_ring = dalloc(c, S2);
   c->ooov_p1 = dalloc(c, OO * VO);
-  c->ooov_emni = dalloc(c, OO * VO);
+  c->ooov_iemn = dalloc(c, OO * VO);
   c->ooov_emnj = dalloc(c, OO * VO);
   c->vovv_embf = dalloc(c, VO * VV);
   c->fovT = dalloc(c, VO);
@@ -1040,8 +1061,16 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     gemv_cols(vovv, V, VO * V, c->t1T + y * VO, 0, c->Fae + y * VV, V,
               (int)VO, V, V, -1.0, 1.0, c->scr.partial, c->scr.partial_elems,
               s);
-  run(c, mk(V, V, (int)(V * OO), -0.5, 1.0, c->taut, V * OO, 1, S2,
-            c->oovv_fmne, V, 1, 0, c->Fae, V, 1, VV, nb), P5);
+  // - 1/2 sum_{fmn} taut[a,f,m,n] <mn||ef> (oovvT[e,f,m,n]: K fastest)
+  if (s5)
+    run_spin(c, sgem(-0.5, 1.0, c->taut, S2, c->oovvT, 0, c->Fae, VV,
+                     nullptr),
+             P5, {{'a', 'v', V * OO, 0, V}}, {{'e', 'v', 0, V * OO, 1}},
+             {{'f', 'v', OO, OO, 0}, {'m', 'o', O, O, 0}, {'n', 'o', 1, 1, 0}},
+             "af|mn", "mn|ef", "a|e");
+  else
+    run(c, mk(V, V, (int)(V * OO), -0.5, 1.0, c->taut, V * OO, 1, S2,
+              c->oovvT, 1, V * OO, 0, c->Fae, V, 1, VV, nb), P5);
 
   // Fmi[m,i] = Foo + 1/2 Fov.t1 + sum t_n^e <mn||ie> + 1/2 oovv.tautp
   bcast(c->Fmi, Foo, OO, nb, 1.0, 0.0, s);
@@ -1049,8 +1078,15 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
             nb), P64);
   gemv_rows(c->ooov_p1, VO, 0, c->t1T, VO, c->Fmi, OO, (int)OO, (int)VO, nb,
             1.0, 1.0, s);
-  run(c, mk(O, O, (int)(O * VV), 0.5, 1.0, oovv, O * VV, 1, 0, c->tautp, O, 1,
-            S2, c->Fmi, O, 1, OO, nb), P5);
+  // + 1/2 sum_{nef} <mn||ef> taut[e,f,i,n] (tautp[i,n,e,f]: K fastest)
+  if (s5)
+    run_spin(c, sgem(0.5, 1.0, oovv, 0, c->tautp, S2, c->Fmi, OO, nullptr),
+             P5, {{'m', 'o', O * VV, 0, O}}, {{'i', 'o', 0, O * VV, 1}},
+             {{'n', 'o', VV, VV, 0}, {'e', 'v', V, V, 0}, {'f', 'v', 1, 1, 0}},
+             "mn|ef", "ef|in", "m|i");
+  else
+    run(c, mk(O, O, (int)(O * VV), 0.5, 1.0, oovv, O * VV, 1, 0, c->tautp, 1,
+              O * VV, S2, c->Fmi, O, 1, OO, nb), P5);
 
   // Fbe = Fae - 1/2 t1.Fme ; Fmj = Fmi + 1/2 Fme.t1
   axpby(c->Fbe, c->Fae, VV * nb, 1.0, 0.0, s);
@@ -1274,7 +1310,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
            vvoo, c->aring);
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
-  //                    + 1/2 vovv.t2p + 1/2 t2.ooov_emni)
+  //                    + 1/2 vovv.t2p + 1/2 t2.ooov_iemn)
   bcast(t1bar, Fvo, VO, nb, 1.0, 0.0, s);
   run(c, mk(V, O, V, 1.0, 1.0, c->Fae, V, 1, VV, t1, O, 1, VO, t1bar, O, 1, VO,
             nb), P64);
@@ -1285,10 +1321,23 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   for (int y = 0; y < nb; ++y)
     gemv_cols(c->ovvo_ring, VO, 0, t1 + y * VO, 0, t1bar + y * VO, 0, (int)VO,
               (int)VO, 1, 1.0, 1.0, c->scr.partial, c->scr.partial_elems, s);
-  run(c, mk(V, O, (int)(O * VV), 0.5, 1.0, vovv, O * VV, 1, 0, c->t2p, O, 1,
-            S2, t1bar, O, 1, VO, nb), P5);
-  run(c, mk(V, O, (int)(V * OO), 0.5, 1.0, t2, V * OO, 1, S2, c->ooov_emni, O,
-            1, 0, t1bar, O, 1, VO, nb), P5);
+  // + 1/2 sum_{mef} <am||ef> t2[e,f,i,m] (t2p[i,m,e,f])
+  // + 1/2 sum_{emn} t2[a,e,m,n] <nm||ie> (ooov_iemn[i,e,m,n])
+  if (s5) {
+    run_spin(c, sgem(0.5, 1.0, vovv, 0, c->t2p, S2, t1bar, VO, nullptr), P5,
+             {{'a', 'v', O * VV, 0, O}}, {{'i', 'o', 0, O * VV, 1}},
+             {{'m', 'o', VV, VV, 0}, {'e', 'v', V, V, 0}, {'f', 'v', 1, 1, 0}},
+             "am|ef", "ef|im", "a|i");
+    run_spin(c, sgem(0.5, 1.0, t2, S2, c->ooov_iemn, 0, t1bar, VO, nullptr),
+             P5, {{'a', 'v', V * OO, 0, O}}, {{'i', 'o', 0, V * OO, 1}},
+             {{'e', 'v', OO, OO, 0}, {'m', 'o', O, O, 0}, {'n', 'o', 1, 1, 0}},
+             "ae|mn", "nm|ie", "a|i");
+  } else {
+    run(c, mk(V, O, (int)(O * VV), 0.5, 1.0, vovv, O * VV, 1, 0, c->t2p, 1,
+              O * VV, S2, t1bar, O, 1, VO, nb), P5);
+    run(c, mk(V, O, (int)(V * OO), 0.5, 1.0, t2, V * OO, 1, S2, c->ooov_iemn,
+              1, V * OO, 0, t1bar, O, 1, VO, nb), P5);
+  }
   axpby(t1bar, t1bar, VO * nb, -1.0, 0.0, s);
 }
 

@@ -64,6 +64,11 @@ struct Gemm {
   int M1 = 1, M2 = 0, N1 = 1, N2 = 0;
   int64_t sAm0 = 0, sAm1 = 0, sCm0 = 0, sCm1 = 0, sDm0 = 0, sDm1 = 0;
   int64_t sBn0 = 0, sBn1 = 0, sCn0 = 0, sCn1 = 0, sDn0 = 0, sDn1 = 0;
+  // K over up to three indices likewise (K1, K2 and sAk0, sAk1 / sBk0,
+  // sBk1 with sAk, sBk innermost; K2 = 0: one index); needs A stored with
+  // K fastest and B with K fastest ([N][K])
+  int K1 = 1, K2 = 0;
+  int64_t sAk0 = 0, sAk1 = 0, sBk0 = 0, sBk1 = 0;
 };
 
 // Scratch the GEMM engine may use for split-K partial sums and low-precision

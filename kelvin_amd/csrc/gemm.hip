@@ -68,6 +68,8 @@ struct GemmDev {
   int M1, M2, N1, N2;
   int64_t sAm0, sAm1, sCm0, sCm1, sDm0, sDm1;
   int64_t sBn0, sBn1, sCn0, sCn1, sDn0, sDn1;
+  int K1, K2;                // K levels (kernels instantiated with KL)
+  int64_t sAk0, sAk1, sBk0, sBk1;
 };
 
 struct Lvl {
@@ -145,26 +147,46 @@ __device__ inline void tile_offsets(const GemmDev &p, int m0, int n0,
 // of the 256 threads loads TM/16 values of A and TN/16 of B.  Along the
 // contiguous index: m = t % TM, k = t / TM + (256 / TM) r; otherwise
 // m = t / 16 + 16 r, k = t % 16.
-template <typename T, bool A_MFAST, bool B_NFAST, int TM, int TN>
+// KL (K over several indices, both operands K-fastest): every load of a
+// thread shares k = k0 + t % 16, decoded once per slab.
+template <typename T, bool A_MFAST, bool B_NFAST, int TM, int TN, bool KL>
 __device__ inline void load_slab(const GemmDev &p, const double *A,
                                  const double *B, const int64_t *aoff,
                                  const int64_t *boff, int k0, int kend,
                                  T ra[TM / 16], T rb[TN / 16]) {
+  static_assert(!KL || (!A_MFAST && !B_NFAST), "K levels need K-fast loads");
   const int t = threadIdx.x;
+  int64_t ka = 0, kb = 0;
+  bool kin = true;
+  if (KL) {
+    const int k = k0 + (t & 15);
+    kin = k < kend;
+    const Lvl l = levels(kin ? k : 0, p.K1, p.K2);
+    ka = at(l, p.sAk0, p.sAk1, p.sAk);
+    kb = at(l, p.sBk0, p.sBk1, p.sBk);
+  }
 #pragma unroll
   for (int r = 0; r < TM / 16; ++r) {
     const int64_t off = aoff[A_MFAST ? (t % TM) : (t >> 4) + 16 * r];
-    const int k = A_MFAST ? k0 + t / TM + (NT / TM) * r : k0 + (t & 15);
     double v = 0.0;
-    if (off >= 0 && k < kend) v = A[off + (int64_t)k * p.sAk];
+    if (KL) {
+      if (off >= 0 && kin) v = A[off + ka];
+    } else {
+      const int k = A_MFAST ? k0 + t / TM + (NT / TM) * r : k0 + (t & 15);
+      if (off >= 0 && k < kend) v = A[off + (int64_t)k * p.sAk];
+    }
     ra[r] = (T)v;
   }
 #pragma unroll
   for (int r = 0; r < TN / 16; ++r) {
     const int64_t off = boff[B_NFAST ? (t % TN) : (t >> 4) + 16 * r];
-    const int k = B_NFAST ? k0 + t / TN + (NT / TN) * r : k0 + (t & 15);
     double v = 0.0;
-    if (off >= 0 && k < kend) v = B[(int64_t)k * p.sBk + off];
+    if (KL) {
+      if (off >= 0 && kin) v = B[kb + off];
+    } else {
+      const int k = B_NFAST ? k0 + t / TN + (NT / TN) * r : k0 + (t & 15);
+      if (off >= 0 && k < kend) v = B[(int64_t)k * p.sBk + off];
+    }
     rb[r] = (T)v;
   }
 }
@@ -198,7 +220,7 @@ __device__ inline void load4(const T *src, T out[4]) {
 
 // TM x TN output tile per workgroup (128x128, 64x128 or 128x64): the four
 // waves form a 2x2 grid, each owning (TM/2) x (TN/2) as RT x CT MFMA blocks.
-template <typename T, bool A_MFAST, bool B_NFAST, int TM, int TN>
+template <typename T, bool A_MFAST, bool B_NFAST, int TM, int TN, bool KL>
 __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   using acc_t = typename Mma<T>::acc_t;
   constexpr int LDK = Mma<T>::LDK;
@@ -235,17 +257,17 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   tile_offsets<TM, TN>(p, m0, n0, aoff, boff);
   __syncthreads();
   if (nslab > 0) {
-    load_slab<T, A_MFAST, B_NFAST, TM, TN>(p, A, B, aoff, boff, kbeg, kend,
-                                           ra, rb);
+    load_slab<T, A_MFAST, B_NFAST, TM, TN, KL>(p, A, B, aoff, boff, kbeg,
+                                               kend, ra, rb);
     store_slab<T, A_MFAST, B_NFAST, TM, TN>(As[0], Bs[0], ra, rb);
   }
   __syncthreads();
   for (int s = 0; s < nslab; ++s) {
     const int buf = DB ? (s & 1) : 0;
     if (s + 1 < nslab)
-      load_slab<T, A_MFAST, B_NFAST, TM, TN>(p, A, B, aoff, boff,
-                                             kbeg + (s + 1) * BK, kend, ra,
-                                             rb);
+      load_slab<T, A_MFAST, B_NFAST, TM, TN, KL>(p, A, B, aoff, boff,
+                                                 kbeg + (s + 1) * BK, kend,
+                                                 ra, rb);
     T a[RT][4], b[CT][4];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -382,19 +404,22 @@ void launch(K3 kernel, const GemmDev &d, hipStream_t s) {
 }
 
 template <typename T, int TM, int TN>
-void launch_tile(bool amf, bool bnf, const GemmDev &d, hipStream_t s) {
-  if (amf && bnf) launch(gemm_mfma_kernel<T, true, true, TM, TN>, d, s);
-  else if (amf) launch(gemm_mfma_kernel<T, true, false, TM, TN>, d, s);
-  else if (bnf) launch(gemm_mfma_kernel<T, false, true, TM, TN>, d, s);
-  else launch(gemm_mfma_kernel<T, false, false, TM, TN>, d, s);
+void launch_tile(bool amf, bool bnf, bool kl, const GemmDev &d,
+                 hipStream_t s) {
+  if (kl) launch(gemm_mfma_kernel<T, false, false, TM, TN, true>, d, s);
+  else if (amf && bnf)
+    launch(gemm_mfma_kernel<T, true, true, TM, TN, false>, d, s);
+  else if (amf) launch(gemm_mfma_kernel<T, true, false, TM, TN, false>, d, s);
+  else if (bnf) launch(gemm_mfma_kernel<T, false, true, TM, TN, false>, d, s);
+  else launch(gemm_mfma_kernel<T, false, false, TM, TN, false>, d, s);
 }
 
 template <typename T>
-void launch_layout(Tile tl, bool amf, bool bnf, const GemmDev &d,
+void launch_layout(Tile tl, bool amf, bool bnf, bool kl, const GemmDev &d,
                    hipStream_t s) {
-  if (tl.m == 64) launch_tile<T, 64, 128>(amf, bnf, d, s);
-  else if (tl.n == 64) launch_tile<T, 128, 64>(amf, bnf, d, s);
-  else launch_tile<T, 128, 128>(amf, bnf, d, s);
+  if (tl.m == 64) launch_tile<T, 64, 128>(amf, bnf, kl, d, s);
+  else if (tl.n == 64) launch_tile<T, 128, 64>(amf, bnf, kl, d, s);
+  else launch_tile<T, 128, 128>(amf, bnf, kl, d, s);
 }
 
 }  // namespace
@@ -490,6 +515,16 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   d.sCn1 = g.sCn1;
   d.sDn0 = g.sDn0;
   d.sDn1 = g.sDn1;
+  require(g.K2 >= 0 && g.K1 >= 1, "GEMM K levels must be positive");
+  if (g.K2 > 0)
+    require(g.K % ((int64_t)g.K1 * g.K2) == 0,
+            "GEMM K is not a product of its index levels");
+  d.K1 = g.K2 > 0 ? g.K1 : 1;
+  d.K2 = g.K2 > 0 ? g.K2 : (1 << 30);
+  d.sAk0 = g.sAk0;
+  d.sAk1 = g.sAk1;
+  d.sBk0 = g.sBk0;
+  d.sBk1 = g.sBk1;
   if (R > 1) {
     const size_t need = (size_t)R * d.nbC * g.M * g.N;
     require(need <= scr.partial_elems, "GEMM split-K scratch too small",
@@ -498,10 +533,13 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   }
   const bool amf = (g.sAm == 1 && g.sAk != 1);
   const bool bnf = (g.sBn == 1 && g.sBk != 1);
+  const bool kl = g.K2 > 0;
+  require(!kl || (!amf && !bnf),
+          "K over several indices needs both operands K-fastest");
   if (precision == FTCC_PREC_FP64)
-    launch_layout<double>(tl, amf, bnf, d, s);
+    launch_layout<double>(tl, amf, bnf, kl, d, s);
   else
-    launch_layout<float>(tl, amf, bnf, d, s);
+    launch_layout<float>(tl, amf, bnf, kl, d, s);
   if (R > 1) {
     const int64_t total = (int64_t)d.nbC * g.M * g.N;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);

@@ -196,8 +196,8 @@ __global__ void denominators_kernel(const double *ev, const double *eo, int nv,
 //   taut  [a,b,i,j] = t2 + (t1[a,i]t1[b,j] - t1[b,i]t1[a,j])/2
 //   t2r   [a,i,b,j] = t2[a,b,i,j]                    (ring layout)
 //   aring [a,i,b,j] = -t2[a,b,i,j]/2 + t1[a,j]t1[b,i]
-//   t2p   [j,a,b,i] = t2[a,b,i,j]
-//   tautp [j,a,b,i] = taut[a,b,i,j]
+//   t2p   [i,j,a,b] = t2[a,b,i,j]     (the [ab][ij] matrix transposed)
+//   tautp [i,j,a,b] = taut[a,b,i,j]
 __global__ void prep_kernel(const double *t1s, const double *t2s, int nv,
                             int no, int nb, double *tau, double *taut,
                             double *t2r, double *aring, double *t2p,
@@ -497,15 +497,17 @@ void prep_amplitudes(const double *t1, const double *t2, int nv, int no,
                      int nb, const PrepOut &o, hipStream_t s) {
   LAUNCH(prep_kernel, (int64_t)nv * nv * no * no * nb, t1, t2, nv, no, nb,
          o.tau, o.taut, o.t2r, o.aring, o.t2p, o.tautp);
-  // t2p / tautp: transpose of the [(a,b,i)][j] matrix of each slice
-  const int64_t R = (int64_t)nv * nv * no;
-  const int64_t tiles = ((R + 31) / 32) * ((no + 31) / 32);
+  // t2p / tautp: transpose of the [(a,b)][(i,j)] matrix of each slice, so
+  // that their (j, a, b) index runs contiguously for every i
+  const int64_t R = (int64_t)nv * nv;
+  const int C = no * no;
+  const int64_t tiles = ((R + 31) / 32) * ((C + 31) / 32);
   require(tiles < (1LL << 31) && nb < 65536, "transpose grid too large");
   hipLaunchKernelGGL(transpose_tiled_kernel, dim3((unsigned)tiles, nb),
-                     dim3(256), 0, s, t2, o.t2p, R, no);
+                     dim3(256), 0, s, t2, o.t2p, R, C);
   FTCC_CHECK_HIP(hipGetLastError());
   hipLaunchKernelGGL(transpose_tiled_kernel, dim3((unsigned)tiles, nb),
-                     dim3(256), 0, s, o.taut, o.tautp, R, no);
+                     dim3(256), 0, s, o.taut, o.tautp, R, C);
   FTCC_CHECK_HIP(hipGetLastError());
 }
 
