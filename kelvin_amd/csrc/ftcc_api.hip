@@ -2,17 +2,20 @@
 // residual at every grid point, integration, damping and energies.
 //
 // Residual schedule for a chunk of nb grid points (all indices over the
-// occupied-role list o and virtual-role list v; O = no, V = nv).  Six
+// occupied-role list o and virtual-role list v; O = no, V = nv).  Five
 // O(n^6) products run on the matrix cores:
-//   G1  Wmnij += 1/4 oovv[mn,ef] . tau[ef,ij]
-//   G2  Wabef += 1/4 tau[ab,mn] . oovv[mn,ef]
-//   G3  R2    += 1/2 tau[ab,mn] . Wmnij[mn,ij]
-//   G4  R2    += 1/2 Wabef[ab,ef] . tau[ef,ij]
-//   G5  Wr[em,bj] -= oovv[m,n,e,f](as [em,fn]) . A[fn,bj],
+//   G1  Wmnij += 1/2 oovv[mn,ef] . tau[ef,ij]
+//   G2  R2    += 1/2 tau[ab,mn] . Wmnij[mn,ij]
+//   G3  R2    += 1/2 Wabef[ab,ef] . tau[ef,ij]
+//   G4  Wr[em,bj] -= oovv[m,n,e,f](as [em,fn]) . A[fn,bj],
 //                    A[f,n,b,j] = 1/2 t_jn^fb + t_j^f t_n^b
-//   G6  Z[ai,bj]   = t2[a,e,i,m](as [ai,em]) . Wr[em,bj]
-// ("ring" tensors are stored [p,k,q,l] for <pq..kl> so G5/G6 are plain
+//   G5  Z[ai,bj]   = t2[a,e,i,m](as [ai,em]) . Wr[em,bj]
+// ("ring" tensors are stored [p,k,q,l] for <pq..kl> so G4/G5 are plain
 // products), and the O(n^5) terms are batched GEMMs on the same engine.
+// The Stanton-Gauss intermediates carry 1/4 tau.oovv each in Wmnij and
+// Wabef, so that R2 holds 1/2 . 1/4 tau.oovv.tau through each ladder;
+// here Wmnij carries all of it (1/2) and Wabef none, which is the same
+// 1/4 tau.oovv.tau in R2 for one product fewer.
 // Equations: Stanton, Gauss, Watts, Bartlett, J. Chem. Phys. 94, 4334 (1991)
 // in kelvin's block conventions (kelvin/ft_cc_equations.py:96-113).
 #include "ftcc_internal.h"
@@ -86,9 +89,9 @@ struct ftcc_ctx {
     hipEvent_t e0, e1;
     double flops;
   };
-  // split-bf16 operands of the six O(n^6) products (bf16x3 mode)
+  // split-bf16 operands of the five O(n^6) products (bf16x3 mode)
   SplitMat sp_tauA, sp_tauB, sp_t2rA, sp_aringB, sp_wmnijB, sp_wabefA, sp_wrB;
-  SplitMat sp_oovvA, sp_oovvB, sp_ringA;
+  SplitMat sp_oovvA, sp_ringA;
   bool split_fixed_valid = false;
   int split_layout = -1;        // spin state the split buffers are sized for
 
@@ -481,7 +484,7 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
   run_spin(c, base, prec, Mi, Ni, std::vector<SIdx>{Ki}, ruleA, ruleB, ruleC);
 }
 
-// One of the six O(n^6) products: split-bf16 operands in bf16x3 mode
+// One of the five O(n^6) products: split-bf16 operands in bf16x3 mode
 // (A as [M][K], B as [N][K]), the generic engine otherwise.
 void big(ftcc_ctx *c, const Gemm &g, int prec, const SplitMat &A,
          const SplitMat &B) {
@@ -516,18 +519,18 @@ Range group(const PairSpace &p, int c1, int c2 = -1) {
 }
 
 // ---------------------------------------------------------------------------
-// Operands of the six O(n^6) products.  Each is a matrix over two pair
+// Operands of the five O(n^6) products.  Each is a matrix over two pair
 // spaces (rows, K); A operands are stored [M][K], B operands transposed as
 // [N][K].  Pair spaces: 0 = (a,b) virtual pairs, 1 = (i,j) occupied pairs,
-// 2 = ring (a,i) pairs.  Ids 7..9 depend on the integrals only.
+// 2 = ring (a,i) pairs.  Ids 7 and 8 depend on the integrals only.
 enum Op {
   OP_TAU_A, OP_TAU_B, OP_T2R_A, OP_ARING_B, OP_WMNIJ_B, OP_WABEF_A, OP_WR_B,
-  OP_OOVV_A, OP_OOVV_B, OP_RING_A, NOPS
+  OP_OOVV_A, OP_RING_A, NOPS
 };
-constexpr int OP_ROWS[NOPS] = {0, 1, 2, 2, 1, 0, 2, 1, 0, 2};
-constexpr int OP_K[NOPS] = {1, 0, 2, 2, 1, 0, 2, 0, 1, 2};
+constexpr int OP_ROWS[NOPS] = {0, 1, 2, 2, 1, 0, 2, 1, 2};
+constexpr int OP_K[NOPS] = {1, 0, 2, 2, 1, 0, 2, 0, 2};
 constexpr bool OP_FIXED[NOPS] = {false, false, false, false, false,
-                                 false, false, true,  true,  true};
+                                 false, false, true,  true};
 
 const PairSpace &space(const ftcc_ctx *c, int sp) {
   return sp == 0   ? c->ps_vv
@@ -557,8 +560,7 @@ int space_len(const ftcc_ctx *c, int sp) {
 SplitMat &split_of(ftcc_ctx *c, int id) {
   SplitMat *t[NOPS] = {&c->sp_tauA,   &c->sp_tauB,   &c->sp_t2rA,
                        &c->sp_aringB, &c->sp_wmnijB, &c->sp_wabefA,
-                       &c->sp_wrB,    &c->sp_oovvA,  &c->sp_oovvB,
-                       &c->sp_ringA};
+                       &c->sp_wrB,    &c->sp_oovvA,  &c->sp_ringA};
   return *t[id];
 }
 
@@ -611,7 +613,7 @@ void ensure_b64_buffers(ftcc_ctx *c) {
 // The (row classes, K classes) blocks of operand `id` that its products read
 // (block_list below): ladder operands the three blocks of equal total spin;
 // ring operands the spin-keeping block and the two spin-flip blocks, crossed
-// (flip classes 1 x 2 and 2 x 1) except for G6's B operand Wr, whose K and
+// (flip classes 1 x 2 and 2 x 1) except for G5's B operand Wr, whose K and
 // column classes agree.  Only these blocks are written in the spin layout.
 void operand_blocks(const ftcc_ctx *c, int id, Range rows[3], Range ks[3]) {
   const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, op_k(c, id));
@@ -680,7 +682,6 @@ void ensure_fixed_operands(ftcc_ctx *c) {
     return;
   }
   put_operand(c, OP_OOVV_A, c->I[5], VV, 1, 0, 1);   // [mn][ef]
-  put_operand(c, OP_OOVV_B, c->I[5], 1, VV, 0, 1);   // [ef][mn]
   put_operand(c, OP_RING_A, c->oovv_ring, VO, 1, 0, 1);
 }
 
@@ -728,11 +729,11 @@ constexpr int LADDER_ORDER[4] = {0, 1, 2, 3};
 constexpr int RING_ORDER[4] = {0, 3, 1, 2};
 
 // The three nonzero blocks of a product C[R,Cs] = A[R,K] B[K,Cs].
-//   ladder products (G1-G4): every operand conserves total spin of its pairs;
-//   G5: Wr[em,bj] -= oovv_ring[em,fn] aring[fn,bj]: ring pairs of A and B
+//   ladder products (G1-G3): every operand conserves total spin of its pairs;
+//   G4: Wr[em,bj] -= oovv_ring[em,fn] aring[fn,bj]: ring pairs of A and B
 //       flip spin with opposite signs, so C keeps it;
-//   G6: Z[ai,bj] = t2r[ai,em] Wr[em,bj]: A flips sign, B keeps it.
-enum BlockKind { LADDER, RING_G5, RING_G6 };
+//   G5: Z[ai,bj] = t2r[ai,em] Wr[em,bj]: A flips sign, B keeps it.
+enum BlockKind { LADDER, RING_G4, RING_G5 };
 void block_list(int kind, const PairSpace &R, const PairSpace &K,
                 const PairSpace &C, Blk out[3]) {
   if (kind == LADDER) {
@@ -743,7 +744,7 @@ void block_list(int kind, const PairSpace &R, const PairSpace &K,
   }
   // ring classes: 0 and 3 keep spin, 1 lowers it (a alpha, i beta), 2 raises
   out[0] = {group(R, 0, 3), group(K, 0, 3), group(C, 0, 3)};
-  if (kind == RING_G5) {
+  if (kind == RING_G4) {
     out[1] = {group(R, 1), group(K, 2), group(C, 1)};
     out[2] = {group(R, 2), group(K, 1), group(C, 2)};
   } else {
@@ -1101,8 +1102,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // cannot hide); the O(n^5) terms and the integral (an addend D) accumulate
   // onto it in the FP64 GEMM.  In the spin layout the product writes the
   // allowed blocks only, and nothing reads the others.
-  // Wmnij = oooo + P(ij) t_j^e <mn||ie> + 1/4 oovv.tau          (G1)
-  product(c, mk((int)OO, (int)OO, (int)VV, 0.25, 0.0, oovv, VV, 1, 0, c->tau,
+  // Wmnij = oooo + P(ij) t_j^e <mn||ie> + 1/2 oovv.tau          (G1)
+  product(c, mk((int)OO, (int)OO, (int)VV, 0.5, 0.0, oovv, VV, 1, 0, c->tau,
                 OO, 1, S2, c->wmnij, OO, 1, O4, nb), OP_OOVV_A, OP_TAU_B,
           LADDER);
   // + oooo + sum_e <mn||ie> t_j^e into [(m,n,i), j]
@@ -1135,13 +1136,10 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
 
-  // Wabef = vvvv - P(ab) t_m^b <am||ef> + 1/4 tau.oovv          (G2)
-  product(c, mk((int)VV, (int)VV, (int)OO, 0.25, 0.0, c->tau, OO, 1, S2, oovv,
-                VV, 1, 0, c->wabef, VV, 1, V4, nb), OP_TAU_A, OP_OOVV_B,
-          LADDER);
+  // Wabef = vvvv - P(ab) t_m^b <am||ef> (its tau.oovv share is in Wmnij)
   if (s5) {
     // vvvv - sum_m t_m^b <am||ef>: C[b][(a,e,f)] = vvvv - t1[b,m] vovv[a,m,ef]
-    run_spin(c, sgem(-1.0, 1.0, t1, VO, vovv, 0, c->wabef, V4, vvvv), P5,
+    run_spin(c, sgem(-1.0, 0.0, t1, VO, vovv, 0, c->wabef, V4, vvvv), P5,
              {{'b', 'v', O, 0, VV}},
              {{'a', 'v', 0, O * VV, V * VV}, {'e', 'v', 0, V, V},
               {'f', 'v', 0, 1, 1}},
@@ -1154,7 +1152,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              {'m', 'o', 1, VV, 0}, "a|m", "bm|ef", "ab|ef");
   } else {
     // vvvv - sum_m t_m^b <am||ef>: per a, C[b][ef] = vvvv - t1[b,m] vovv[a,m,ef]
-    Gemm g = mk(V, (int)VV, O, -1.0, 1.0, t1, O, 1, VO, vovv, VV, 1, 0,
+    Gemm g = mk(V, (int)VV, O, -1.0, 0.0, t1, O, 1, VO, vovv, VV, 1, 0,
                 c->wabef, VV, 1, V4, nb);
     g.nb2 = V;
     g.sA2 = 0;
@@ -1168,6 +1166,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.D = nullptr;
     // + sum_m t_m^a <bm||ef>: per b, rows a of W[a,b,ef]
     g.alpha = 1.0;
+    g.beta = 1.0;
     g.sCm = V * VV;
     g.sC2 = VV;
     run(c, g, P5);
@@ -1176,7 +1175,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // Wr[e,m,b,j] = <mb||ej> + t_j^f <mb||ef> - t_n^b <mn||ej> - A.<mn||ef>
   product(c, mk((int)VO, (int)VO, (int)VO, -1.0, 0.0, c->oovv_ring, VO, 1, 0,
                 c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), OP_RING_A,
-          OP_ARING_B, RING_G5);                                          // G5
+          OP_ARING_B, RING_G4);                                          // G4
   // <mb||ej> + t_j^f <mb||ef> = ovvo_ring - sum_f vovv[b,m,e,f] t1[f,j]:
   // rows (e,m,b) of Wr
   if (s5) {
@@ -1212,12 +1211,12 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
 
-  // Z[ai,bj] = t2r.Wr - t_i^e t_m^a <mb||ej>                    (G6)
+  // Z[ai,bj] = t2r.Wr - t_i^e t_m^a <mb||ej>                    (G5)
   put_operand(c, OP_WR_B, c->wr, 1, VO, S2, nb);
   // spin blocks leave the spin-forbidden part of Z unwritten; the
   // spin-aware assembly below never reads it
   product(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr,
-                VO, 1, S2, c->z, VO, 1, S2, nb), OP_T2R_A, OP_WR_B, RING_G6);
+                VO, 1, S2, c->z, VO, 1, S2, nb), OP_T2R_A, OP_WR_B, RING_G5);
   if (s5) {
     // u[i,(m,b,j)] = t1[e,i] ovvo_ring[e,m,b,j]; Z[a,(i,b,j)] -= t1[a,m] u
     run_spin(c, sgem(1.0, 0.0, t1, VO, c->ovvo_ring, 0, c->u, OO * VO,
@@ -1243,17 +1242,17 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
 
-  // doubles: G3 = 1/2 tau.Wmnij into t2bar and G4 = 1/2 Wabef.tau into
-  // aring (free after G5), both with beta = 0; the assembly adds them, vvoo
+  // doubles: G2 = 1/2 tau.Wmnij into t2bar and G3 = 1/2 Wabef.tau into
+  // aring (free after G4), both with beta = 0; the assembly adds them, vvoo
   // and the rest
   put_operand(c, OP_WMNIJ_B, c->wmnij, 1, OO, O4, nb);
   put_operand(c, OP_WABEF_A, c->wabef, VV, 1, V4, nb);
   product(c, mk((int)VV, (int)OO, (int)OO, 0.5, 0.0, c->tau, OO, 1, S2,
                 c->wmnij, OO, 1, O4, t2bar, OO, 1, S2, nb), OP_TAU_A,
-          OP_WMNIJ_B, LADDER);                                           // G3
+          OP_WMNIJ_B, LADDER);                                           // G2
   product(c, mk((int)VV, (int)OO, (int)VV, 0.5, 0.0, c->wabef, VV, 1, V4,
                 c->tau, OO, 1, S2, c->aring, OO, 1, S2, nb), OP_WABEF_A,
-          OP_TAU_B, LADDER);                                             // G4
+          OP_TAU_B, LADDER);                                             // G3
 
   // S1 (antisymmetrised in ab): Fbe[b,e] t2[a,e,i,j] + t1[a,m] vooo[b,m,i,j]
   // S2 (antisymmetrised in ij): -t2[a,b,i,m] Fmj[m,j] + t1[e,i] vvvo[a,b,e,j]
