@@ -185,6 +185,9 @@ def main():
     eng.driving_terms(T1, T2)
     eng.integrate(T1, T2)
     skip = zero_slices(eng, T1, T2, G)
+    # the iterates are the library's own (driving terms, integration,
+    # steps): no per-step spin check of the amplitudes
+    eng.set_trusted(True)
     spin_mode = eng.spin_mode()
     spin_blocked = bool(spin_mode & 1)
     # operand layout of the dominant kernel: dense, spin blocks, or spin

@@ -14,8 +14,9 @@ Extensions over the reference (all optional):
               until the FP64 iteration converges (default True).
   graph       run the solve on a stream of its own with every fixed-point
               step replayed as a HIP graph (launch overhead of small
-              systems); the solver's own iterates are trusted to conserve
-              spin (default False).
+              systems) (default False).  Without a guess (T1/T2 on
+              entry) the solver's own iterates are trusted to conserve
+              spin and the per-step check of the amplitudes is skipped.
   chunk       grid points per residual chunk (default: chosen from the
               workspace budget; ``kelvin_amd.batch.autotune_chunk`` times
               the choices).
@@ -247,8 +248,9 @@ class ccsd(object):
             eng.set_chunk(self.chunk)
         if self.graph:
             eng.use_graph = True
-            # iterates started from the driving terms conserve spin
-            eng.set_trusted(T1in is None and T2in is None)
+        # iterates started from the driving terms conserve spin: the
+        # per-step check of the amplitudes is needed for a guess only
+        eng.set_trusted(T1in is None and T2in is None)
         t0 = time.time()
         if hasattr(self.sys, "g_device_integrals"):
             # integrals built in HBM (the UEG provider)

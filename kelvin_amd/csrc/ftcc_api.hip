@@ -49,6 +49,8 @@ struct ftcc_ctx {
   double *ooov_p1 = nullptr;    // [m,i,n,e] = ooov[m,n,i,e]
   double *ooov_iemn = nullptr;  // [i,e,m,n] = ooov[n,m,i,e]
   double *oovvT = nullptr;      // [a,b,i,j] = oovv[i,j,a,b]
+  double *oovv_mefn = nullptr;  // [m,e,f,n] = oovv[m,n,e,f]
+  double *vovv_aefm = nullptr;  // [a,e,f,m] = vovv[a,m,e,f]
   double *vovv_embf = nullptr;  // [e,m,b,f] = vovv[b,m,e,f]
   double *ooov_emnj = nullptr;  // [e,m,n,j] = ooov[m,n,j,e]
   double *fovT = nullptr;       // [a,i] = F.ov[i,a]
@@ -76,7 +78,7 @@ struct ftcc_ctx {
   // imaginary-time propagation: stage input, accumulator and right-hand
   // side (singles / doubles), allocated on first use
   double *td_var[2] = {}, *td_acc[2] = {}, *td_k[2] = {}, *td_k1[2] = {};
-  double *tau = nullptr, *taut, *t2r, *aring, *t2p, *tautp, *wmnij, *wabef,
+  double *tau = nullptr, *taut, *t2r, *aring, *wmnij, *wabef,
       *wr, *z, *u, *s1, *s2;
   double *t1T, *Fme, *FmeT, *Fae, *Fmi, *Fbe, *Fmj, *ytmp;
   Scratch scr;
@@ -197,6 +199,8 @@ void build_derived(ftcc_ctx *c) {
   derive(c->I[5], c->oovv_ring, doovv, pr, 1.0, s);
   const int pt[4] = {2, 3, 0, 1};
   derive(c->I[5], c->oovvT, doovv, pt, 1.0, s);
+  const int pk[4] = {0, 2, 3, 1};
+  derive(c->I[5], c->oovv_mefn, doovv, pk, 1.0, s);
   const int dvovo[4] = {V, O, V, O};
   const int pv[4] = {2, 1, 0, 3};
   derive(c->I[4], c->ovvo_ring, dvovo, pv, -1.0, s);
@@ -210,6 +214,7 @@ void build_derived(ftcc_ctx *c) {
   const int dvovv[4] = {V, O, V, V};
   const int pb[4] = {2, 1, 0, 3};
   derive(c->I[2], c->vovv_embf, dvovv, pb, 1.0, s);
+  derive(c->I[2], c->vovv_aefm, dvovv, pk, 1.0, s);
   transpose2(c->F[1], c->fovT, O, V, 1, s);
   c->have_ints = true;
   c->split_fixed_valid = false;
@@ -306,6 +311,30 @@ double gemm_flops(const Gemm &g) {
   return 2.0 * g.M * (double)g.N * g.K * g.nb1 * g.nb2;
 }
 
+// A batched GEMM as one launch whose outer row (column) level runs over
+// its slices: the operand shared by every slice (slice stride 0) is then
+// read once for all of them.  g has one row (column) index.
+Gemm fold_m(Gemm g) {
+  g.M1 = g.nb1;
+  g.M2 = g.M;
+  g.M *= g.nb1;
+  g.sAm1 = g.sA1;
+  g.sCm1 = g.sC1;
+  g.sDm1 = g.sD1;
+  g.nb1 = 1;
+  return g;
+}
+Gemm fold_n(Gemm g) {
+  g.N1 = g.nb1;
+  g.N2 = g.N;
+  g.N *= g.nb1;
+  g.sBn1 = g.sB1;
+  g.sCn1 = g.sC1;
+  g.sDn1 = g.sD1;
+  g.nb1 = 1;
+  return g;
+}
+
 void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false) {
   ensure_partial(c, g);
   timed(c, big, gemm_flops(g),
@@ -341,29 +370,45 @@ bool rule_holds(const char *rule, const std::map<char, int> &sp) {
 // A, B and C all hold is one launch over the product of the letters' spin
 // runs (the other blocks of A and B are zero, those of C are not written).
 // `base` carries the pointers, alpha/beta, D, the slice count and the slice
-// strides; the rest is filled here.
+// strides; the rest is filled here.  An M or N index of role 'y' runs over
+// the base's slices (grid points) inside one launch: a fixed integral
+// operand is then read once for all of them; it takes no spin, and D (an
+// integral) does not move along it.
 void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
               const std::vector<SIdx> &Mi, const std::vector<SIdx> &Ni,
               const std::vector<SIdx> &Ki, const char *ruleA,
               const char *ruleB, const char *ruleC) {
   std::vector<char> letters;
-  auto add = [&](char l) {
-    if (std::find(letters.begin(), letters.end(), l) == letters.end())
-      letters.push_back(l);
+  int batch = 0;
+  auto add = [&](const SIdx &x) {
+    if (x.role == 'y') {
+      ++batch;
+      return;
+    }
+    if (std::find(letters.begin(), letters.end(), x.l) == letters.end())
+      letters.push_back(x.l);
   };
-  for (const SIdx &x : Mi) add(x.l);
-  for (const SIdx &x : Ni) add(x.l);
-  for (const SIdx &x : Ki) add(x.l);
+  for (const SIdx &x : Mi) add(x);
+  for (const SIdx &x : Ni) add(x);
+  for (const SIdx &x : Ki) {
+    require(x.role != 'y', "spin-restricted GEMM: slice index in K");
+    add(x);
+  }
   require(Mi.size() >= 1 && Mi.size() <= 3 && Ni.size() >= 1 &&
               Ni.size() <= 3 && Ki.size() >= 1 && Ki.size() <= 3 &&
-              letters.size() == Mi.size() + Ni.size() + Ki.size(),
+              batch <= 1 &&
+              letters.size() + batch == Mi.size() + Ni.size() + Ki.size(),
           "spin-restricted GEMM: bad index lists");
   auto start = [&](const SIdx &x, int sp) {
-    return x.role == 'o' ? c->o_start[sp] : c->v_start[sp];
+    return x.role == 'y' ? 0 : x.role == 'o' ? c->o_start[sp] : c->v_start[sp];
   };
   auto count = [&](const SIdx &x, int sp) {
-    return x.role == 'o' ? c->o_count[sp] : c->v_count[sp];
+    return x.role == 'y'   ? base.nb1
+           : x.role == 'o' ? c->o_count[sp]
+                           : c->v_count[sp];
   };
+  // D's stride along an index: none along the slices
+  auto sd = [](const SIdx &x) { return x.role == 'y' ? 0 : x.sc; };
   const bool writes_all = base.beta == 0.0 || base.D != nullptr;
   std::set<std::vector<int>> c_blocks;   // C spin blocks that were written
   const int L = (int)letters.size();
@@ -428,15 +473,22 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
     g.sAm0 = m0.sa;
     g.sAm1 = m1.sa;
     g.sAm = m2.sa;
-    g.sCm0 = g.sDm0 = m0.sc;
-    g.sCm1 = g.sDm1 = m1.sc;
-    g.sCm = g.sDm = m2.sc;
+    g.sCm0 = m0.sc;
+    g.sCm1 = m1.sc;
+    g.sCm = m2.sc;
+    g.sDm0 = sd(m0);
+    g.sDm1 = sd(m1);
+    g.sDm = sd(m2);
     g.sBn0 = n0.sb;
     g.sBn1 = n1.sb;
     g.sBn = n2.sb;
-    g.sCn0 = g.sDn0 = n0.sc;
-    g.sCn1 = g.sDn1 = n1.sc;
-    g.sCn = g.sDn = n2.sc;
+    g.sCn0 = n0.sc;
+    g.sCn1 = n1.sc;
+    g.sCn = n2.sc;
+    g.sDn0 = sd(n0);
+    g.sDn1 = sd(n1);
+    g.sDn = sd(n2);
+    if (batch) g.nb1 = 1;   // the slices are an index of the launch
     g.sAk = k2.sa;
     g.sBk = k2.sb;
     if (nk >= 2) {
@@ -457,19 +509,17 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
   }
   if (writes_all) {
     // every spin-allowed block of C must have been written
-    const int nc = (int)(Mi.size() + Ni.size());
+    const int nc = (int)(Mi.size() + Ni.size()) - batch;
     for (int mask = 0; mask < (1 << nc); ++mask) {
       std::map<char, int> sp;
       std::vector<int> cb;
       int k = 0;
-      for (const SIdx &x : Mi) {
-        sp[x.l] = (mask >> k++) & 1;
+      auto take = [&](const SIdx &x) {
+        sp[x.l] = x.role == 'y' ? 0 : (mask >> k++) & 1;
         cb.push_back(sp[x.l]);
-      }
-      for (const SIdx &x : Ni) {
-        sp[x.l] = (mask >> k++) & 1;
-        cb.push_back(sp[x.l]);
-      }
+      };
+      for (const SIdx &x : Mi) take(x);
+      for (const SIdx &x : Ni) take(x);
       if (rule_holds(ruleC, sp))
         require(c_blocks.count(cb) == 1,
                 "spin-restricted GEMM leaves an allowed output block unset");
@@ -913,6 +963,8 @@ void ensure_integrals(ftcc_ctx *c) {
   for (int k = 0; k < 9; ++k) c->I[k] = dalloc(c, isz[k]);
   c->oovv_ring = dalloc(c, S2);
   c->oovvT = dalloc(c, S2);
+  c->oovv_mefn = dalloc(c, S2);
+  c->vovv_aefm = dalloc(c, VO * VV);
   c->ovvo_ring = dalloc(c, S2);
   c->ooov_p1 = dalloc(c, OO * VO);
   c->ooov_iemn = dalloc(c, OO * VO);
@@ -933,11 +985,11 @@ void ensure_workspace(ftcc_ctx *c) {
   const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
   const int64_t S2 = VV * OO;
   const int64_t sm = std::max<int64_t>(VV, std::max<int64_t>(VO, OO));
-  // ten n^4 intermediates, Wmnij/Wabef/U, the seven per-slot product
+  // eight n^4 intermediates, Wmnij/Wabef/U, the seven per-slot product
   // operands split into bf16 pairs (bf16x3) and as FP64 spin-blocked copies
   // (FP64/FP32 modes; both exist after a precision switch)
   const int64_t per_slot =
-      8 * (10 * S2 + OO * OO + VV * VV + OO * VO + 8 * sm) +
+      8 * (8 * S2 + OO * OO + VV * VV + OO * VO + 8 * sm) +
       4 * 7 * (S2 + 64 * std::max<int64_t>(VV, std::max<int64_t>(VO, OO))) +
       8 * 7 * S2;
   double budget_gb = 120.0;
@@ -952,8 +1004,6 @@ void ensure_workspace(ftcc_ctx *c) {
   c->taut = dalloc(c, S2 * nb);
   c->t2r = dalloc(c, S2 * nb);
   c->aring = dalloc(c, S2 * nb);
-  c->t2p = dalloc(c, S2 * nb);
-  c->tautp = dalloc(c, S2 * nb);
   c->wr = dalloc(c, S2 * nb);
   c->z = dalloc(c, S2 * nb);
   c->s1 = dalloc(c, S2 * nb);
@@ -975,8 +1025,8 @@ void ensure_workspace(ftcc_ctx *c) {
 // are rebuilt on the next residual with the current chunk size).
 void free_workspace(ftcc_ctx *c) {
   FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
-  for (double **p : {&c->tau, &c->taut, &c->t2r, &c->aring, &c->t2p,
-                     &c->tautp, &c->wr, &c->z, &c->s1, &c->s2, &c->wmnij,
+  for (double **p : {&c->tau, &c->taut, &c->t2r, &c->aring, &c->wr, &c->z,
+                     &c->s1, &c->s2, &c->wmnij,
                      &c->wabef, &c->u, &c->t1T, &c->Fme, &c->FmeT, &c->Fae,
                      &c->Fmi, &c->Fbe, &c->Fmj, &c->ytmp}) {
     dfree(c, *p);
@@ -1036,7 +1086,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
                *ooov = c->I[7], *oooo = c->I[8];
 
   // amplitude-derived operands
-  PrepOut po{c->tau, c->taut, c->t2r, c->aring, c->t2p, c->tautp};
+  PrepOut po{c->tau, c->taut, c->t2r, c->aring};
   prep_amplitudes(t1, t2, V, O, nb, po, s);
   transpose2(t1, c->t1T, V, O, nb, s);   // t1T[m,f] = t1[f,m]
   // product operands (split into bf16 pairs, and/or reordered by spin
@@ -1063,31 +1113,44 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
               (int)VO, V, V, -1.0, 1.0, c->scr.partial, c->scr.partial_elems,
               s);
   // - 1/2 sum_{fmn} taut[a,f,m,n] <mn||ef> (oovvT[e,f,m,n]: K fastest)
+  // (the chunk's grid points are rows of one launch: oovvT is read once)
   if (s5)
     run_spin(c, sgem(-0.5, 1.0, c->taut, S2, c->oovvT, 0, c->Fae, VV,
                      nullptr),
-             P5, {{'a', 'v', V * OO, 0, V}}, {{'e', 'v', 0, V * OO, 1}},
+             P5, {{'y', 'y', S2, 0, VV}, {'a', 'v', V * OO, 0, V}},
+             {{'e', 'v', 0, V * OO, 1}},
              {{'f', 'v', OO, OO, 0}, {'m', 'o', O, O, 0}, {'n', 'o', 1, 1, 0}},
              "af|mn", "mn|ef", "a|e");
   else
-    run(c, mk(V, V, (int)(V * OO), -0.5, 1.0, c->taut, V * OO, 1, S2,
-              c->oovvT, 1, V * OO, 0, c->Fae, V, 1, VV, nb), P5);
+    run(c, fold_m(mk(V, V, (int)(V * OO), -0.5, 1.0, c->taut, V * OO, 1, S2,
+                     c->oovvT, 1, V * OO, 0, c->Fae, V, 1, VV, nb)),
+        P5);
 
-  // Fmi[m,i] = Foo + 1/2 Fov.t1 + sum t_n^e <mn||ie> + 1/2 oovv.tautp
+  // Fmi[m,i] = Foo + 1/2 Fov.t1 + sum t_n^e <mn||ie> + 1/2 oovv.taut
   bcast(c->Fmi, Foo, OO, nb, 1.0, 0.0, s);
   run(c, mk(O, O, V, 0.5, 1.0, Fov, V, 1, 0, t1, O, 1, VO, c->Fmi, O, 1, OO,
             nb), P64);
   gemv_rows(c->ooov_p1, VO, 0, c->t1T, VO, c->Fmi, OO, (int)OO, (int)VO, nb,
             1.0, 1.0, s);
-  // + 1/2 sum_{nef} <mn||ef> taut[e,f,i,n] (tautp[i,n,e,f]: K fastest)
-  if (s5)
-    run_spin(c, sgem(0.5, 1.0, oovv, 0, c->tautp, S2, c->Fmi, OO, nullptr),
-             P5, {{'m', 'o', O * VV, 0, O}}, {{'i', 'o', 0, O * VV, 1}},
-             {{'n', 'o', VV, VV, 0}, {'e', 'v', V, V, 0}, {'f', 'v', 1, 1, 0}},
+  // + 1/2 sum_{nef} <mn||ef> taut[e,f,i,n] over K = (e, f, n), n fastest in
+  // both (oovv_mefn[m,e,f,n]); the grid points are columns of one launch
+  if (s5) {
+    run_spin(c, sgem(0.5, 1.0, c->oovv_mefn, 0, c->taut, S2, c->Fmi, OO,
+                     nullptr),
+             P5, {{'m', 'o', V * VO, 0, O}},
+             {{'y', 'y', 0, S2, OO}, {'i', 'o', 0, O, 1}},
+             {{'e', 'v', VO, V * OO, 0}, {'f', 'v', O, OO, 0},
+              {'n', 'o', 1, 1, 0}},
              "mn|ef", "ef|in", "m|i");
-  else
-    run(c, mk(O, O, (int)(O * VV), 0.5, 1.0, oovv, O * VV, 1, 0, c->tautp, 1,
-              O * VV, S2, c->Fmi, O, 1, OO, nb), P5);
+  } else {
+    Gemm g = fold_n(mk(O, O, (int)(O * VV), 0.5, 1.0, c->oovv_mefn, V * VO, 1,
+                       0, c->taut, 1, O, S2, c->Fmi, O, 1, OO, nb));
+    g.K1 = (int)VV;   // k = (e,f) * O + n
+    g.K2 = O;
+    g.sAk1 = O;
+    g.sBk1 = OO;
+    run(c, g, P5);
+  }
 
   // Fbe = Fae - 1/2 t1.Fme ; Fmj = Fmi + 1/2 Fme.t1
   axpby(c->Fbe, c->Fae, VV * nb, 1.0, 0.0, s);
@@ -1111,11 +1174,11 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run_spin(c, sgem(1.0, 1.0, ooov, 0, t1, VO, c->wmnij, O4, oooo), P5,
              {{'m', 'o', OO * V, 0, O * OO}, {'n', 'o', O * V, 0, OO},
               {'i', 'o', V, 0, O}},
-             {{'j', 'o', 0, 1, 1}}, {'e', 'v', 1, O, 0}, "mn|ie", "e|j",
-             "mn|ij");
+             {{'y', 'y', 0, VO, O4}, {'j', 'o', 0, 1, 1}}, {'e', 'v', 1, O, 0},
+             "mn|ie", "e|j", "mn|ij");
     // - sum_e t_i^e <mn||je>: C[i][(m,n,j)] -= t1^T[i,e] ooov[m,n,j,e]
     run_spin(c, sgem(-1.0, 1.0, t1, VO, ooov, 0, c->wmnij, O4, nullptr), P5,
-             {{'i', 'o', 1, 0, O}},
+             {{'y', 'y', VO, 0, O4}, {'i', 'o', 1, 0, O}},
              {{'m', 'o', 0, OO * V, O * OO}, {'n', 'o', 0, O * V, OO},
               {'j', 'o', 0, V, 1}},
              {'e', 'v', O, 1, 0}, "e|i", "mn|je", "mn|ij");
@@ -1140,13 +1203,13 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   if (s5) {
     // vvvv - sum_m t_m^b <am||ef>: C[b][(a,e,f)] = vvvv - t1[b,m] vovv[a,m,ef]
     run_spin(c, sgem(-1.0, 0.0, t1, VO, vovv, 0, c->wabef, V4, vvvv), P5,
-             {{'b', 'v', O, 0, VV}},
+             {{'y', 'y', VO, 0, V4}, {'b', 'v', O, 0, VV}},
              {{'a', 'v', 0, O * VV, V * VV}, {'e', 'v', 0, V, V},
               {'f', 'v', 0, 1, 1}},
              {'m', 'o', 1, VV, 0}, "b|m", "am|ef", "ab|ef");
     // + sum_m t_m^a <bm||ef>: C[a][(b,e,f)]
     run_spin(c, sgem(1.0, 1.0, t1, VO, vovv, 0, c->wabef, V4, nullptr), P5,
-             {{'a', 'v', O, 0, V * VV}},
+             {{'y', 'y', VO, 0, V4}, {'a', 'v', O, 0, V * VV}},
              {{'b', 'v', 0, O * VV, VV}, {'e', 'v', 0, V, V},
               {'f', 'v', 0, 1, 1}},
              {'m', 'o', 1, VV, 0}, "a|m", "bm|ef", "ab|ef");
@@ -1184,11 +1247,11 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              P5,
              {{'e', 'v', O * VV, 0, O * VO}, {'m', 'o', VV, 0, VO},
               {'b', 'v', V, 0, O}},
-             {{'j', 'o', 0, 1, 1}}, {'f', 'v', 1, O, 0}, "bm|ef", "f|j",
-             "mb|ej");
+             {{'y', 'y', 0, VO, S2}, {'j', 'o', 0, 1, 1}}, {'f', 'v', 1, O, 0},
+             "bm|ef", "f|j", "mb|ej");
     // -t_n^b <mn||ej> = sum_n t1[b,n] ooov[m,n,j,e]: C[b][(e,m,j)]
     run_spin(c, sgem(1.0, 1.0, t1, VO, c->ooov_emnj, 0, c->wr, S2, nullptr),
-             P5, {{'b', 'v', O, 0, O}},
+             P5, {{'y', 'y', VO, 0, S2}, {'b', 'v', O, 0, O}},
              {{'e', 'v', 0, O * OO, O * VO}, {'m', 'o', 0, OO, VO},
               {'j', 'o', 0, 1, 1}},
              {'n', 'o', 1, O, 0}, "b|n", "mn|je", "mb|ej");
@@ -1221,7 +1284,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     // u[i,(m,b,j)] = t1[e,i] ovvo_ring[e,m,b,j]; Z[a,(i,b,j)] -= t1[a,m] u
     run_spin(c, sgem(1.0, 0.0, t1, VO, c->ovvo_ring, 0, c->u, OO * VO,
                      nullptr),
-             P5, {{'i', 'o', 1, 0, O * VO}},
+             P5, {{'y', 'y', VO, 0, OO * VO}, {'i', 'o', 1, 0, O * VO}},
              {{'m', 'o', 0, VO, VO}, {'b', 'v', 0, O, O}, {'j', 'o', 0, 1, 1}},
              {'e', 'v', O, O * VO, 0}, "e|i", "mb|ej", "mb|ij");
     run_spin(c, sgem(-1.0, 1.0, t1, VO, c->u, OO * VO, c->z, S2, nullptr), P5,
@@ -1263,7 +1326,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
               {'j', 'o', 0, 1, 1}},
              {'e', 'v', 1, OO, 0}, "b|e", "ae|ij", "ab|ij");
     run_spin(c, sgem(1.0, 1.0, t1, VO, vooo, 0, c->s1, S2, nullptr), P5,
-             {{'a', 'v', O, 0, V * OO}},
+             {{'y', 'y', VO, 0, S2}, {'a', 'v', O, 0, V * OO}},
              {{'b', 'v', 0, O * OO, OO}, {'i', 'o', 0, O, O},
               {'j', 'o', 0, 1, 1}},
              {'m', 'o', 1, OO, 0}, "a|m", "bm|ij", "ab|ij");
@@ -1273,7 +1336,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              {{'j', 'o', 0, 1, 1}}, {'m', 'o', 1, O, 0}, "ab|im", "m|j",
              "ab|ij");
     run_spin(c, sgem(1.0, 1.0, t1, VO, vvvo, 0, c->s2, S2, nullptr), P5,
-             {{'i', 'o', 1, 0, O}},
+             {{'y', 'y', VO, 0, S2}, {'i', 'o', 1, 0, O}},
              {{'a', 'v', 0, V * VO, V * OO}, {'b', 'v', 0, VO, OO},
               {'j', 'o', 0, 1, 1}},
              {'e', 'v', O, O, 0}, "e|i", "ab|ej", "ab|ij");
@@ -1309,7 +1372,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
            vvoo, c->aring);
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
-  //                    + 1/2 vovv.t2p + 1/2 t2.ooov_iemn)
+  //                    + 1/2 vovv.t2 + 1/2 t2.ooov)
   bcast(t1bar, Fvo, VO, nb, 1.0, 0.0, s);
   run(c, mk(V, O, V, 1.0, 1.0, c->Fae, V, 1, VV, t1, O, 1, VO, t1bar, O, 1, VO,
             nb), P64);
@@ -1320,22 +1383,33 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   for (int y = 0; y < nb; ++y)
     gemv_cols(c->ovvo_ring, VO, 0, t1 + y * VO, 0, t1bar + y * VO, 0, (int)VO,
               (int)VO, 1, 1.0, 1.0, c->scr.partial, c->scr.partial_elems, s);
-  // + 1/2 sum_{mef} <am||ef> t2[e,f,i,m] (t2p[i,m,e,f])
+  // + 1/2 sum_{mef} <am||ef> t2[e,f,i,m] over K = (e, f, m), m fastest in
+  // both (vovv_aefm[a,e,f,m])
   // + 1/2 sum_{emn} t2[a,e,m,n] <nm||ie> (ooov_iemn[i,e,m,n])
+  // (the grid points are columns, resp. rows, of one launch)
   if (s5) {
-    run_spin(c, sgem(0.5, 1.0, vovv, 0, c->t2p, S2, t1bar, VO, nullptr), P5,
-             {{'a', 'v', O * VV, 0, O}}, {{'i', 'o', 0, O * VV, 1}},
-             {{'m', 'o', VV, VV, 0}, {'e', 'v', V, V, 0}, {'f', 'v', 1, 1, 0}},
+    run_spin(c, sgem(0.5, 1.0, c->vovv_aefm, 0, t2, S2, t1bar, VO, nullptr),
+             P5, {{'a', 'v', V * VO, 0, O}},
+             {{'y', 'y', 0, S2, VO}, {'i', 'o', 0, O, 1}},
+             {{'e', 'v', VO, V * OO, 0}, {'f', 'v', O, OO, 0},
+              {'m', 'o', 1, 1, 0}},
              "am|ef", "ef|im", "a|i");
     run_spin(c, sgem(0.5, 1.0, t2, S2, c->ooov_iemn, 0, t1bar, VO, nullptr),
-             P5, {{'a', 'v', V * OO, 0, O}}, {{'i', 'o', 0, V * OO, 1}},
+             P5, {{'y', 'y', S2, 0, VO}, {'a', 'v', V * OO, 0, O}},
+             {{'i', 'o', 0, V * OO, 1}},
              {{'e', 'v', OO, OO, 0}, {'m', 'o', O, O, 0}, {'n', 'o', 1, 1, 0}},
              "ae|mn", "nm|ie", "a|i");
   } else {
-    run(c, mk(V, O, (int)(O * VV), 0.5, 1.0, vovv, O * VV, 1, 0, c->t2p, 1,
-              O * VV, S2, t1bar, O, 1, VO, nb), P5);
-    run(c, mk(V, O, (int)(V * OO), 0.5, 1.0, t2, V * OO, 1, S2, c->ooov_iemn,
-              1, V * OO, 0, t1bar, O, 1, VO, nb), P5);
+    Gemm g = fold_n(mk(V, O, (int)(O * VV), 0.5, 1.0, c->vovv_aefm, V * VO, 1,
+                       0, t2, 1, O, S2, t1bar, O, 1, VO, nb));
+    g.K1 = (int)VV;   // k = (e,f) * O + m
+    g.K2 = O;
+    g.sAk1 = O;
+    g.sBk1 = OO;
+    run(c, g, P5);
+    run(c, fold_m(mk(V, O, (int)(V * OO), 0.5, 1.0, t2, V * OO, 1, S2,
+                     c->ooov_iemn, 1, V * OO, 0, t1bar, O, 1, VO, nb)),
+        P5);
   }
   axpby(t1bar, t1bar, VO * nb, -1.0, 0.0, s);
 }

@@ -201,7 +201,7 @@ void denominators(const double *ev, const double *eo, int nv, int no,
 
 // Per-slice preparation of the amplitude-derived operands (see ftcc_api.hip)
 struct PrepOut {
-  double *tau, *taut, *t2r, *aring, *t2p, *tautp;
+  double *tau, *taut, *t2r, *aring;
 };
 void prep_amplitudes(const double *t1, const double *t2, int nv, int no,
                      int nb, const PrepOut &o, hipStream_t s);

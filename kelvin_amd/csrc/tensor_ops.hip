@@ -196,12 +196,9 @@ __global__ void denominators_kernel(const double *ev, const double *eo, int nv,
 //   taut  [a,b,i,j] = t2 + (t1[a,i]t1[b,j] - t1[b,i]t1[a,j])/2
 //   t2r   [a,i,b,j] = t2[a,b,i,j]                    (ring layout)
 //   aring [a,i,b,j] = -t2[a,b,i,j]/2 + t1[a,j]t1[b,i]
-//   t2p   [i,j,a,b] = t2[a,b,i,j]     (the [ab][ij] matrix transposed)
-//   tautp [i,j,a,b] = taut[a,b,i,j]
 __global__ void prep_kernel(const double *t1s, const double *t2s, int nv,
                             int no, int nb, double *tau, double *taut,
-                            double *t2r, double *aring, double *t2p,
-                            double *tautp) {
+                            double *t2r, double *aring) {
   const int64_t per = (int64_t)nv * nv * no * no;
   const int64_t per1 = (int64_t)nv * no;
   GRID_STRIDE(idx, per * nb) {
@@ -351,30 +348,6 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
   }
 }
 
-// out[b][c][r] = in[b][r][c] for an R x C matrix per batch, 32x32 tiles
-__global__ __launch_bounds__(256) void transpose_tiled_kernel(
-    const double *in, double *out, int64_t R, int C) {
-  __shared__ double t[32][33];
-  const int64_t tilesC = (C + 31) / 32;
-  const int64_t r0 = (blockIdx.x / tilesC) * 32;
-  const int c0 = (int)(blockIdx.x % tilesC) * 32;
-  const int64_t b = blockIdx.y;
-  const double *src = in + b * R * C;
-  double *dst = out + b * R * C;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int k = ty; k < 32; k += 8) {
-    const int64_t r = r0 + k;
-    const int c = c0 + tx;
-    t[k][tx] = (r < R && c < C) ? src[r * C + c] : 0.0;
-  }
-  __syncthreads();
-  for (int k = ty; k < 32; k += 8) {
-    const int c = c0 + k;
-    const int64_t r = r0 + tx;
-    if (r < R && c < C) dst[(int64_t)c * R + r] = t[tx][k];
-  }
-}
-
 __global__ void transpose2_kernel(const double *in, double *out, int r, int c,
                                   int nb) {
   const int64_t per = (int64_t)r * c;
@@ -496,19 +469,7 @@ void denominators(const double *ev, const double *eo, int nv, int no,
 void prep_amplitudes(const double *t1, const double *t2, int nv, int no,
                      int nb, const PrepOut &o, hipStream_t s) {
   LAUNCH(prep_kernel, (int64_t)nv * nv * no * no * nb, t1, t2, nv, no, nb,
-         o.tau, o.taut, o.t2r, o.aring, o.t2p, o.tautp);
-  // t2p / tautp: transpose of the [(a,b)][(i,j)] matrix of each slice, so
-  // that their (j, a, b) index runs contiguously for every i
-  const int64_t R = (int64_t)nv * nv;
-  const int C = no * no;
-  const int64_t tiles = ((R + 31) / 32) * ((C + 31) / 32);
-  require(tiles < (1LL << 31) && nb < 65536, "transpose grid too large");
-  hipLaunchKernelGGL(transpose_tiled_kernel, dim3((unsigned)tiles, nb),
-                     dim3(256), 0, s, t2, o.t2p, R, C);
-  FTCC_CHECK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(transpose_tiled_kernel, dim3((unsigned)tiles, nb),
-                     dim3(256), 0, s, o.taut, o.tautp, R, C);
-  FTCC_CHECK_HIP(hipGetLastError());
+         o.tau, o.taut, o.t2r, o.aring);
 }
 
 void antisym_add(double *W, const double *Q, int d0, int d1, int d2, int d3,
