@@ -46,6 +46,7 @@ struct ftcc_ctx {
   // derived layouts
   double *oovv_ring = nullptr;  // [e,m,f,n] = oovv[m,n,e,f]
   double *ovvo_ring = nullptr;  // [e,m,b,j] = <mb||ej> = -vovo[b,m,e,j]
+  double *ovvo_ringT = nullptr; // [b,j,e,m] = ovvo_ring[e,m,b,j]
   double *ooov_p1 = nullptr;    // [m,i,n,e] = ooov[m,n,i,e]
   double *ooov_iemn = nullptr;  // [i,e,m,n] = ooov[n,m,i,e]
   double *oovvT = nullptr;      // [a,b,i,j] = oovv[i,j,a,b]
@@ -204,6 +205,8 @@ void build_derived(ftcc_ctx *c) {
   const int dvovo[4] = {V, O, V, O};
   const int pv[4] = {2, 1, 0, 3};
   derive(c->I[4], c->ovvo_ring, dvovo, pv, -1.0, s);
+  const int dring[4] = {V, O, V, O};
+  derive(c->ovvo_ring, c->ovvo_ringT, dring, pt, 1.0, s);
   const int dooov[4] = {O, O, O, V};
   const int p1[4] = {0, 2, 1, 3};
   derive(c->I[7], c->ooov_p1, dooov, p1, 1.0, s);
@@ -373,16 +376,18 @@ bool rule_holds(const char *rule, const std::map<char, int> &sp) {
 // strides; the rest is filled here.  An M or N index of role 'y' runs over
 // the base's slices (grid points) inside one launch: a fixed integral
 // operand is then read once for all of them; it takes no spin, and D (an
-// integral) does not move along it.
+// integral) does not move along it.  Role '1' is an index of extent one
+// (a matrix-vector product).
 void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
               const std::vector<SIdx> &Mi, const std::vector<SIdx> &Ni,
               const std::vector<SIdx> &Ki, const char *ruleA,
               const char *ruleB, const char *ruleC) {
   std::vector<char> letters;
   int batch = 0;
+  int unit = 0;
   auto add = [&](const SIdx &x) {
-    if (x.role == 'y') {
-      ++batch;
+    if (x.role == 'y' || x.role == '1') {
+      ++(x.role == 'y' ? batch : unit);
       return;
     }
     if (std::find(letters.begin(), letters.end(), x.l) == letters.end())
@@ -391,19 +396,24 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
   for (const SIdx &x : Mi) add(x);
   for (const SIdx &x : Ni) add(x);
   for (const SIdx &x : Ki) {
-    require(x.role != 'y', "spin-restricted GEMM: slice index in K");
+    require(x.role != 'y' && x.role != '1',
+            "spin-restricted GEMM: slice index in K");
     add(x);
   }
   require(Mi.size() >= 1 && Mi.size() <= 3 && Ni.size() >= 1 &&
               Ni.size() <= 3 && Ki.size() >= 1 && Ki.size() <= 3 &&
               batch <= 1 &&
-              letters.size() + batch == Mi.size() + Ni.size() + Ki.size(),
+              letters.size() + batch + unit ==
+                  Mi.size() + Ni.size() + Ki.size(),
           "spin-restricted GEMM: bad index lists");
   auto start = [&](const SIdx &x, int sp) {
-    return x.role == 'y' ? 0 : x.role == 'o' ? c->o_start[sp] : c->v_start[sp];
+    return (x.role == 'y' || x.role == '1') ? 0
+           : x.role == 'o'                  ? c->o_start[sp]
+                                            : c->v_start[sp];
   };
   auto count = [&](const SIdx &x, int sp) {
     return x.role == 'y'   ? base.nb1
+           : x.role == '1' ? 1
            : x.role == 'o' ? c->o_count[sp]
                            : c->v_count[sp];
   };
@@ -447,7 +457,13 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
     std::vector<int> cb;
     for (const SIdx &x : Mi) cb.push_back(sp[x.l]);
     for (const SIdx &x : Ni) cb.push_back(sp[x.l]);
-    c_blocks.insert(cb);
+    // several K spin blocks onto one C block: the first applies beta and
+    // D, the others accumulate
+    const bool first = c_blocks.insert(cb).second;
+    if (!first) {
+      g.beta = 1.0;
+      g.D = nullptr;
+    }
     if (M == 0 || N == 0) continue;
     require(M < (1LL << 31) && N < (1LL << 31) && K < (1LL << 31),
             "spin block too large");
@@ -457,7 +473,7 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
     g.A = base.A + oa;
     g.B = base.B + ob;
     g.C = base.C + oc;
-    g.D = base.D ? base.D + oc : nullptr;
+    g.D = (first && base.D) ? base.D + oc : nullptr;
     // levels: the listed indices fill the innermost slots
     const int nm = (int)Mi.size(), nn = (int)Ni.size(), nk = (int)Ki.size();
     const SIdx &m0 = nm == 3 ? Mi[0] : z, &m1 = nm >= 2 ? Mi[nm - 2] : z,
@@ -500,7 +516,7 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
       g.sBk1 = k1.sb;
     }
     if (K == 0) {
-      if (!writes_all) continue;
+      if (!writes_all || !first) continue;
       g.K = 0;   // C = beta * C + gamma * D only
       g.K1 = 1;
       g.K2 = 0;
@@ -509,13 +525,13 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
   }
   if (writes_all) {
     // every spin-allowed block of C must have been written
-    const int nc = (int)(Mi.size() + Ni.size()) - batch;
+    const int nc = (int)(Mi.size() + Ni.size()) - batch - unit;
     for (int mask = 0; mask < (1 << nc); ++mask) {
       std::map<char, int> sp;
       std::vector<int> cb;
       int k = 0;
       auto take = [&](const SIdx &x) {
-        sp[x.l] = x.role == 'y' ? 0 : (mask >> k++) & 1;
+        sp[x.l] = (x.role == 'y' || x.role == '1') ? 0 : (mask >> k++) & 1;
         cb.push_back(sp[x.l]);
       };
       for (const SIdx &x : Mi) take(x);
@@ -966,6 +982,7 @@ void ensure_integrals(ftcc_ctx *c) {
   c->oovv_mefn = dalloc(c, S2);
   c->vovv_aefm = dalloc(c, VO * VV);
   c->ovvo_ring = dalloc(c, S2);
+  c->ovvo_ringT = dalloc(c, S2);
   c->ooov_p1 = dalloc(c, OO * VO);
   c->ooov_iemn = dalloc(c, OO * VO);
   c->ooov_emnj = dalloc(c, OO * VO);
@@ -1097,9 +1114,21 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   put_operand(c, OP_T2R_A, c->t2r, VO, 1, S2, nb);
   put_operand(c, OP_ARING_B, c->aring, 1, VO, S2, nb);
 
-  // Fme[m,e] = Fov[m,e] + sum_{fn} <mn||ef> t_n^f
-  gemv_rows(c->oovv_ring, VO, 0, t1, VO, c->FmeT, VO, (int)VO, (int)VO, nb,
-            1.0, 0.0, s);
+  // Fme[m,e] = Fov[m,e] + sum_{fn} <mn||ef> t_n^f: FmeT[y][e,m] over the
+  // grid points y as columns (oovv_ring[e,m,f,n] read once)
+  if (s5) {
+    fill(c->FmeT, VO * nb, 0.0, s);   // spin-forbidden entries stay zero
+    run_spin(c, sgem(1.0, 0.0, c->oovv_ring, 0, t1, VO, c->FmeT, VO,
+                     nullptr),
+             P5, {{'e', 'v', O * VO, 0, O}, {'m', 'o', VO, 0, 1}},
+             {{'y', 'y', 0, VO, VO}},
+             {{'f', 'v', O, O, 0}, {'n', 'o', 1, 1, 0}}, "mn|ef", "f|n",
+             "e|m");
+  } else {
+    run(c, mk((int)VO, nb, (int)VO, 1.0, 0.0, c->oovv_ring, VO, 1, 0, t1, 1,
+              VO, 0, c->FmeT, 1, VO, 0, 1),
+        P5);
+  }
   transpose2(c->FmeT, c->Fme, V, O, nb, s);
   bcast(c->Fme, Fov, VO, nb, 1.0, 1.0, s);
   transpose2(c->Fme, c->FmeT, O, V, nb, s);
@@ -1108,10 +1137,28 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   bcast(c->Fae, Fvv, VV, nb, 1.0, 0.0, s);
   run(c, mk(V, V, O, -0.5, 1.0, t1, O, 1, VO, Fov, V, 1, 0, c->Fae, V, 1, VV,
             nb), P64);
-  for (int y = 0; y < nb; ++y)
-    gemv_cols(vovv, V, VO * V, c->t1T + y * VO, 0, c->Fae + y * VV, V,
-              (int)VO, V, V, -1.0, 1.0, c->scr.partial, c->scr.partial_elems,
-              s);
+  // + sum_{mf} t_m^f <ma||fe> = - sum_{fm} t1[f,m] vovv[a,m,f,e], vovv[a,m,f,e]
+  // = vovv_aefm[a,f,e,m]: rows y, columns (a,e), K = (f,m)
+  if (s5) {
+    run_spin(c, sgem(-1.0, 1.0, t1, VO, c->vovv_aefm, 0, c->Fae, VV,
+                     nullptr),
+             P5, {{'y', 'y', VO, 0, VV}},
+             {{'a', 'v', 0, V * VO, V}, {'e', 'v', 0, O, 1}},
+             {{'f', 'v', O, VO, 0}, {'m', 'o', 1, 1, 0}}, "f|m", "am|fe",
+             "a|e");
+  } else {
+    Gemm g = mk(nb, (int)VV, (int)VO, -1.0, 1.0, t1, VO, 1, 0, c->vovv_aefm,
+                1, O, 0, c->Fae, VV, 1, 0, 1);
+    g.N1 = V;        // columns (a, e)
+    g.N2 = V;
+    g.sBn1 = V * VO;
+    g.sCn1 = V;
+    g.K1 = V;        // k = f * O + m
+    g.K2 = O;
+    g.sAk1 = O;
+    g.sBk1 = VO;
+    run(c, g, P5);
+  }
   // - 1/2 sum_{fmn} taut[a,f,m,n] <mn||ef> (oovvT[e,f,m,n]: K fastest)
   // (the chunk's grid points are rows of one launch: oovvT is read once)
   if (s5)
@@ -1130,8 +1177,18 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   bcast(c->Fmi, Foo, OO, nb, 1.0, 0.0, s);
   run(c, mk(O, O, V, 0.5, 1.0, Fov, V, 1, 0, t1, O, 1, VO, c->Fmi, O, 1, OO,
             nb), P64);
-  gemv_rows(c->ooov_p1, VO, 0, c->t1T, VO, c->Fmi, OO, (int)OO, (int)VO, nb,
-            1.0, 1.0, s);
+  // + sum_{ne} <mn||ie> t_n^e: ooov_p1[m,i,n,e] rows (m,i), columns y
+  if (s5)
+    run_spin(c, sgem(1.0, 1.0, c->ooov_p1, 0, c->t1T, VO, c->Fmi, OO,
+                     nullptr),
+             P5, {{'m', 'o', O * VO, 0, O}, {'i', 'o', VO, 0, 1}},
+             {{'y', 'y', 0, VO, OO}},
+             {{'n', 'o', V, V, 0}, {'e', 'v', 1, 1, 0}}, "mn|ie", "e|n",
+             "m|i");
+  else
+    run(c, mk((int)OO, nb, (int)VO, 1.0, 1.0, c->ooov_p1, VO, 1, 0, c->t1T, 1,
+              VO, 0, c->Fmi, 1, OO, 0, 1),
+        P5);
   // + 1/2 sum_{nef} <mn||ef> taut[e,f,i,n] over K = (e, f, n), n fastest in
   // both (oovv_mefn[m,e,f,n]); the grid points are columns of one launch
   if (s5) {
@@ -1378,11 +1435,25 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
             nb), P64);
   run(c, mk(V, O, O, -1.0, 1.0, t1, O, 1, VO, c->Fmi, O, 1, OO, t1bar, O, 1,
             VO, nb), P64);
-  gemv_rows(c->t2r, VO, S2, c->FmeT, VO, t1bar, VO, (int)VO, (int)VO, nb, 1.0,
-            1.0, s);
-  for (int y = 0; y < nb; ++y)
-    gemv_cols(c->ovvo_ring, VO, 0, t1 + y * VO, 0, t1bar + y * VO, 0, (int)VO,
-              (int)VO, 1, 1.0, 1.0, c->scr.partial, c->scr.partial_elems, s);
+  // + sum_{em} t2r[a,i,e,m] FmeT[e,m] (per grid point) and
+  // + sum_{em} t1[e,m] ovvo_ring[e,m,a,i] (ovvo_ringT[a,i,e,m]; rows y)
+  if (s5) {
+    run_spin(c, sgem(1.0, 1.0, c->t2r, S2, c->FmeT, VO, t1bar, VO, nullptr),
+             P5, {{'a', 'v', O * VO, 0, O}, {'i', 'o', VO, 0, 1}},
+             {{'_', '1', 0, 0, 0}}, {{'e', 'v', O, O, 0}, {'m', 'o', 1, 1, 0}},
+             "ae|im", "e|m", "a|i");
+    run_spin(c, sgem(1.0, 1.0, t1, VO, c->ovvo_ringT, 0, t1bar, VO, nullptr),
+             P5, {{'y', 'y', VO, 0, VO}},
+             {{'a', 'v', 0, O * VO, O}, {'i', 'o', 0, VO, 1}},
+             {{'e', 'v', O, O, 0}, {'m', 'o', 1, 1, 0}}, "e|m", "ma|ei",
+             "a|i");
+  } else {
+    gemv_rows(c->t2r, VO, S2, c->FmeT, VO, t1bar, VO, (int)VO, (int)VO, nb,
+              1.0, 1.0, s);
+    run(c, mk(nb, (int)VO, (int)VO, 1.0, 1.0, t1, VO, 1, 0, c->ovvo_ringT, 1,
+              VO, 0, t1bar, VO, 1, 0, 1),
+        P5);
+  }
   // + 1/2 sum_{mef} <am||ef> t2[e,f,i,m] over K = (e, f, m), m fastest in
   // both (vovv_aefm[a,e,f,m])
   // + 1/2 sum_{emn} t2[a,e,m,n] <nm||ie> (ooov_iemn[i,e,m,n])

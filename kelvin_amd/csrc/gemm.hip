@@ -350,37 +350,67 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
 }
 
 // C[b][m,n] = alpha * sum_r partial[r][b][m][n] + beta*C  (fixed order)
-__global__ void reduce_partial_kernel(GemmDev p, int R) {
+// Sum of the R split-K partials of each output element, then the epilogue.
+// A block of RG x 64 threads takes 64 consecutive elements (coalesced rows
+// of the partial buffer); its RG groups sum interleaved partials (several
+// loads in flight per thread) and combine in LDS, so that even a small
+// output (a few thousand elements) spreads its R-deep sums over the chip.
+constexpr int RG = 16;
+__global__ __launch_bounds__(64 * RG) void reduce_partial_kernel(GemmDev p,
+                                                                 int R) {
+  __shared__ double part[RG][64];
   const int64_t MN = (int64_t)p.M * p.N;
   const int64_t total = MN * p.nbC;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  for (int64_t base = (int64_t)blockIdx.x * 64; base < total;
+       base += (int64_t)gridDim.x * 64) {
+    const int64_t idx = base + lane;
     const int64_t cb = idx / MN;
     const int64_t mn = idx % MN;
-    const int m0 = (int)(mn / p.N), n0 = (int)(mn % p.N);
-    const int m = p.rmap ? p.rmap[m0] : m0, n = p.cmap ? p.cmap[n0] : n0;
-    if (m < 0 || n < 0) continue;
-    double s = 0.0;
-    for (int r = 0; r < R; ++r)
-      s += p.partial[((int64_t)r * p.nbC + cb) * MN + mn];
-    int64_t b1, b2;
-    if (p.reduce_b2) {
-      b1 = cb;
-      b2 = 0;
-    } else {
-      b1 = cb / p.nb2;
-      b2 = cb % p.nb2;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (idx < total) {
+      const double *src = p.partial + cb * MN + mn;
+      const int64_t step = (int64_t)p.nbC * MN;
+      int r = g;
+      for (; r + 3 * RG < R; r += 4 * RG) {
+        s0 += src[(int64_t)r * step];
+        s1 += src[(int64_t)(r + RG) * step];
+        s2 += src[(int64_t)(r + 2 * RG) * step];
+        s3 += src[(int64_t)(r + 3 * RG) * step];
+      }
+      for (; r < R; r += RG) s0 += src[(int64_t)r * step];
     }
-    const Lvl lr = levels(m, p.M1, p.M2), lc = levels(n, p.N1, p.N2);
-    double *c = p.C + b1 * p.sC1 + b2 * p.sC2 + at(lr, p.sCm0, p.sCm1, p.sCm) +
-                at(lc, p.sCn0, p.sCn1, p.sCn);
-    double y = p.alpha * s;
-    if (p.D)
-      y += p.gamma * p.D[b1 * p.sD1 + b2 * p.sD2 +
-                         at(lr, p.sDm0, p.sDm1, p.sDm) +
-                         at(lc, p.sDn0, p.sDn1, p.sDn)];
-    if (p.beta != 0.0) y += p.beta * *c;
-    *c = y;
+    part[g][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (g == 0 && idx < total) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < RG; ++q) s += part[q][lane];
+      const int m0 = (int)(mn / p.N), n0 = (int)(mn % p.N);
+      const int m = p.rmap ? p.rmap[m0] : m0, n = p.cmap ? p.cmap[n0] : n0;
+      if (m >= 0 && n >= 0) {
+        int64_t b1, b2;
+        if (p.reduce_b2) {
+          b1 = cb;
+          b2 = 0;
+        } else {
+          b1 = cb / p.nb2;
+          b2 = cb % p.nb2;
+        }
+        const Lvl lr = levels(m, p.M1, p.M2), lc = levels(n, p.N1, p.N2);
+        double *c = p.C + b1 * p.sC1 + b2 * p.sC2 +
+                    at(lr, p.sCm0, p.sCm1, p.sCm) +
+                    at(lc, p.sCn0, p.sCn1, p.sCn);
+        double y = p.alpha * s;
+        if (p.D)
+          y += p.gamma * p.D[b1 * p.sD1 + b2 * p.sD2 +
+                             at(lr, p.sDm0, p.sDm1, p.sDm) +
+                             at(lc, p.sDn0, p.sDn1, p.sDn)];
+        if (p.beta != 0.0) y += p.beta * *c;
+        *c = y;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -542,9 +572,9 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
     launch_layout<float>(tl, amf, bnf, kl, d, s);
   if (R > 1) {
     const int64_t total = (int64_t)d.nbC * g.M * g.N;
-    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(reduce_partial_kernel, dim3(blocks), dim3(256), 0, s,
-                       d, R);
+    const int blocks = (int)std::min<int64_t>((total + 63) / 64, 65536);
+    hipLaunchKernelGGL(reduce_partial_kernel, dim3(blocks), dim3(64 * RG), 0,
+                       s, d, R);
     FTCC_CHECK_HIP(hipGetLastError());
   }
 }
