@@ -324,29 +324,50 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   double *P = direct ? nullptr
                      : p.partial + ((int64_t)rsel * p.nbC + cb) *
                                        (int64_t)p.M * p.N;
+  // Direct stores go one 16-row block at a time: all its reads of D and C
+  // are issued before any store (a store may alias them as far as the
+  // compiler knows, which would otherwise serialise every element on a
+  // round trip to memory).
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt)
+  for (int rt = 0; rt < RT; ++rt) {
+    if (direct) {
+      double dv[CT][4], cv[CT][4];
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-      const int lc = wn * (TN / 2) + ct * 16 + li;
+      for (int ct = 0; ct < CT; ++ct) {
+        const int lc = wn * (TN / 2) + ct * 16 + li;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int lr = wm * (TM / 2) + rt * 16 + Mma<T>::crow(lg, v);
-        const double x = (double)acc[rt][ct][v];
-        if (direct) {
+        for (int v = 0; v < 4; ++v) {
+          const int lr = wm * (TM / 2) + rt * 16 + Mma<T>::crow(lg, v);
+          const bool ok = crow[lr] >= 0 && ccol[lc] >= 0;
+          dv[ct][v] = (D && ok) ? D[drow[lr] + dcol[lc]] : 0.0;
+          cv[ct][v] = (p.beta != 0.0 && ok) ? C[crow[lr] + ccol[lc]] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const int lc = wn * (TN / 2) + ct * 16 + li;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int lr = wm * (TM / 2) + rt * 16 + Mma<T>::crow(lg, v);
           const int64_t oc = crow[lr], occ = ccol[lc];
-          if (oc >= 0 && occ >= 0) {
-            double *c = C + oc + occ;
-            double y = p.alpha * x;
-            if (D) y += p.gamma * D[drow[lr] + dcol[lc]];
-            if (p.beta != 0.0) y += p.beta * *c;
-            *c = y;
-          }
-        } else if (m0 + lr < p.M && n0 + lc < p.N) {
-          P[(int64_t)(m0 + lr) * p.N + n0 + lc] = x;
+          if (oc >= 0 && occ >= 0)
+            C[oc + occ] = p.alpha * (double)acc[rt][ct][v] +
+                          p.gamma * dv[ct][v] + p.beta * cv[ct][v];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const int lc = wn * (TN / 2) + ct * 16 + li;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int lr = wm * (TM / 2) + rt * 16 + Mma<T>::crow(lg, v);
+          if (m0 + lr < p.M && n0 + lc < p.N)
+            P[(int64_t)(m0 + lr) * p.N + n0 + lc] = (double)acc[rt][ct][v];
         }
       }
     }
+  }
 }
 
 // C[b][m,n] = alpha * sum_r partial[r][b][m][n] + beta*C  (fixed order)
