@@ -611,6 +611,13 @@ const PairSpace &space(const ftcc_ctx *c, int sp) {
 int op_k(const ftcc_ctx *c, int id) {
   return (c->spin && c->pack && OP_K[id] < 2) ? OP_K[id] + 3 : OP_K[id];
 }
+// and of its rows: every ladder operand and output is antisymmetric in both
+// of its pairs, so packing covers its rows too (the outputs then hold one
+// pair of each couple)
+int op_rows(const ftcc_ctx *c, int id) {
+  return (c->spin && c->pack && OP_ROWS[id] < 2) ? OP_ROWS[id] + 3
+                                                 : OP_ROWS[id];
+}
 
 // layout of the product operands: 0 dense, 1 spin blocks, 2 spin blocks
 // with packed ladder K
@@ -632,7 +639,7 @@ SplitMat &split_of(ftcc_ctx *c, int id) {
 
 // elements of one array of a split operand (hi or lo), all slots
 int64_t split_elems(const ftcc_ctx *c, int id) {
-  const int64_t rows = space_len(c, OP_ROWS[id]);
+  const int64_t rows = space_len(c, op_rows(c, id));
   return rows * split_ld(space_len(c, op_k(c, id))) *
          (OP_FIXED[id] ? 1 : c->nb);
 }
@@ -650,7 +657,7 @@ void ensure_split_buffers(ftcc_ctx *c) {
   for (int id = 0; id < NOPS; ++id) {
     SplitMat &m = split_of(c, id);
     m.ld = split_ld(space_len(c, op_k(c, id)));
-    const int64_t rows = space_len(c, OP_ROWS[id]);
+    const int64_t rows = space_len(c, op_rows(c, id));
     m.bstride = OP_FIXED[id] ? 0 : rows * m.ld;
     const int64_t elems = split_elems(c, id);
     // dalloc counts doubles: four bf16 per double
@@ -669,7 +676,7 @@ void ensure_b64_buffers(ftcc_ctx *c) {
     c->b64[id] = nullptr;
   }
   for (int id = 0; id < NOPS; ++id)
-    c->b64[id] = dalloc(c, (int64_t)space_len(c, OP_ROWS[id]) *
+    c->b64[id] = dalloc(c, (int64_t)space_len(c, op_rows(c, id)) *
                                space_len(c, op_k(c, id)) *
                                (OP_FIXED[id] ? 1 : c->nb));
   c->b64_layout = op_layout(c);
@@ -682,7 +689,7 @@ void ensure_b64_buffers(ftcc_ctx *c) {
 // (flip classes 1 x 2 and 2 x 1) except for G5's B operand Wr, whose K and
 // column classes agree.  Only these blocks are written in the spin layout.
 void operand_blocks(const ftcc_ctx *c, int id, Range rows[3], Range ks[3]) {
-  const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, op_k(c, id));
+  const PairSpace &R = space(c, op_rows(c, id)), &K = space(c, op_k(c, id));
   if (OP_ROWS[id] != 2) {
     rows[0] = group(R, 0), ks[0] = group(K, 0);
     rows[1] = group(R, 1, 2), ks[1] = group(K, 1, 2);
@@ -710,7 +717,7 @@ void put_operand(ftcc_ctx *c, int id, const double *src, int64_t sr,
                     space_len(c, op_k(c, id)), split_of(c, id), nb, s);
     return;
   }
-  const PairSpace &R = space(c, OP_ROWS[id]), &K = space(c, op_k(c, id));
+  const PairSpace &R = space(c, op_rows(c, id)), &K = space(c, op_k(c, id));
   Range rows[3], ks[3];
   operand_blocks(c, id, rows, ks);
   for (int q = 0; q < 3; ++q) {
@@ -828,8 +835,8 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     big(c, g, P, split_of(c, ia), split_of(c, ib));
     return;
   }
-  const PairSpace &R = space(c, OP_ROWS[ia]), &K = space(c, op_k(c, ia));
-  const PairSpace &Cs = space(c, OP_ROWS[ib]);
+  const PairSpace &R = space(c, op_rows(c, ia)), &K = space(c, op_k(c, ia));
+  const PairSpace &Cs = space(c, op_rows(c, ib));
   Blk blk[3];
   block_list(kind, R, K, Cs, blk);
   // a packed ladder K sums one pair of each antisymmetric couple: twice
@@ -1362,13 +1369,14 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
 
-  // doubles: G2 = 1/2 tau.Wmnij into t2bar and G3 = 1/2 Wabef.tau into
-  // aring (free after G4), both with beta = 0; the assembly adds them, vvoo
-  // and the rest
+  // doubles: G2 = 1/2 tau.Wmnij into wr (free after the G5 operand was
+  // gathered) and G3 = 1/2 Wabef.tau into aring (free after G4), both with
+  // beta = 0 (with packed pairs: one pair of each antisymmetric couple);
+  // the assembly adds them, vvoo and the rest into t2bar
   put_operand(c, OP_WMNIJ_B, c->wmnij, 1, OO, O4, nb);
   put_operand(c, OP_WABEF_A, c->wabef, VV, 1, V4, nb);
   product(c, mk((int)VV, (int)OO, (int)OO, 0.5, 0.0, c->tau, OO, 1, S2,
-                c->wmnij, OO, 1, O4, t2bar, OO, 1, S2, nb), OP_TAU_A,
+                c->wmnij, OO, 1, O4, c->wr, OO, 1, S2, nb), OP_TAU_A,
           OP_WMNIJ_B, LADDER);                                           // G2
   product(c, mk((int)VV, (int)OO, (int)VV, 0.5, 0.0, c->wabef, VV, 1, V4,
                 c->tau, OO, 1, S2, c->aring, OO, 1, S2, nb), OP_WABEF_A,
@@ -1424,9 +1432,9 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
   // forbidden blocks (spin layout): written as zero without reading
-  r2_final(t2bar, c->s1, c->s2, c->z, V, O, nb, s,
+  r2_final(t2bar, c->wr, c->aring, c->s1, c->s2, c->z, V, O, nb, s,
            c->spin ? c->d_lab_v : nullptr, c->spin ? c->d_lab_o : nullptr,
-           vvoo, c->aring);
+           vvoo, c->spin && c->pack);
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
   //                    + 1/2 vovv.t2 + 1/2 t2.ooov)

@@ -212,15 +212,17 @@ void antisym_add(double *W, const double *Q, int d0, int d1, int d2, int d3,
 // Wr[e,m,b,j] = ovvo_ring[e,m,b,j] - Q1[b,m,e,j] + Q2[m,b,j,e]
 void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
                    const double *Q2, int nv, int no, int nb, hipStream_t s);
-// T2bar[a,b,i,j] = -( acc [+ add] [+ add2] + S1 - S1^(ab) + S2 - S2^(ij)
-//                    + PP(Z ring) ) with add one slice for every grid point
-// and add2 one per grid point (either may be nullptr); with spin labels
+// T2bar[a,b,i,j] = -( acc1 + acc2 [+ add] + S1 - S1^(ab) + S2 - S2^(ij)
+//                    + PP(Z ring) ) with acc1/acc2 one per grid point and
+// add one slice for every grid point (or nullptr); with spin labels
 // (device arrays, or nullptr), spin-forbidden elements are written as zero
-// without reading the operands
-void r2_final(double *t2bar, const double *S1, const double *S2,
-              const double *Z, int nv, int no, int nb, hipStream_t s,
-              const int *lab_v = nullptr, const int *lab_o = nullptr,
-              const double *add = nullptr, const double *add2 = nullptr);
+// without reading the operands.  packed: acc1/acc2 hold one pair of each
+// antisymmetric (a,b) and (i,j) couple only (x < y for equal spins, the
+// alpha index first for mixed spins); needs the labels.
+void r2_final(double *t2bar, const double *acc1, const double *acc2,
+              const double *S1, const double *S2, const double *Z, int nv,
+              int no, int nb, hipStream_t s, const int *lab_v,
+              const int *lab_o, const double *add, bool packed);
 // x[n][c] (nb batches of n*c) -> out[c][n] transposed copy for small mats
 void transpose2(const double *in, double *out, int r, int c, int nb,
                 hipStream_t s);

@@ -267,25 +267,29 @@ __global__ void ring_assemble_kernel(double *Wr, const double *ovvo_ring,
   }
 }
 
-// t2bar[a,b,i,j] = -( acc[a,b,i,j]            (already in t2bar)
+// t2bar[a,b,i,j] = -( acc1[a,b,i,j] + acc2[a,b,i,j]   (ladder products)
 //                     [+ add[a,b,i,j]]         (one slice, all grid points)
-//                     [+ add2[y][a,b,i,j]]     (one per grid point)
 //                     + S1[a,b,i,j] - S1[b,a,i,j]
 //                     + S2[a,b,i,j] - S2[a,b,j,i]
 //                     + Z[a,i,b,j] - Z[b,i,a,j] - Z[a,j,b,i] + Z[b,j,a,i] )
 // One workgroup per (a <= b) pair and 32x32 (i,j) tile writes both (a,b)
 // and (b,a): their S1 and Z terms are the same four and two reads with
-// opposite signs.  The operands read transposed (S2[a,b,j,i], S2[b,a,j,i],
-// Z[a,j,b,i], Z[b,j,a,i]) are loaded along i into LDS so every global
-// access is coalesced.  With spin labels (lv, lo), elements whose spins do
+// opposite signs.  The operands read transposed (S2[a,b,j,i], S2[b,a,j,i]
+// and the difference Z[b,j,a,i] - Z[a,j,b,i]) are loaded along i into LDS
+// so every global access is coalesced (four 32x33 tiles: four workgroups
+// per CU).  With spin labels (lv, lo), elements whose spins do
 // not conserve (lv[a] + lv[b] != lo[i] + lo[j]) are written as zero and
 // their operands are not read: every operand element an allowed output
 // reads is itself allowed.
+// packed: acc1/acc2 hold only one pair of each antisymmetric couple in
+// (a,b) and in (i,j) -- x < y for equal spins, the alpha index first for
+// mixed spins -- and every other element follows by antisymmetry; the
+// stored element of a transposed (i,j) is read along i through LDS.
 __global__ __launch_bounds__(256) void r2_final_pair_kernel(
-    double *t2bar, const double *S1, const double *S2, const double *Z,
-    const double *add, const double *add2, int nv, int no, const int *lv,
-    const int *lo) {
-  __shared__ double tA[32][33], tB[32][33], tZ1[32][33], tZ2[32][33];
+    double *t2bar, const double *acc1, const double *acc2, const double *S1,
+    const double *S2, const double *Z, const double *add, int nv, int no,
+    const int *lv, const int *lo, int packed) {
+  __shared__ double tA[32][33], tB[32][33], tZ[32][33], tP[32][33];
   const int tiles = (no + 31) / 32;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   int64_t bid = blockIdx.x;
@@ -302,6 +306,7 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
   const int a = (int)(q - (int64_t)b * (b + 1) / 2);
   const int64_t per = (int64_t)nv * nv * no * no;
   const double *s1 = S1 + y * per, *s2 = S2 + y * per, *z = Z + y * per;
+  const double *c1 = acc1 + y * per, *c2 = acc2 + y * per;
   double *out = t2bar + y * per;
   const int i0 = ti * 32, j0 = tj * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -312,13 +317,32 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
   auto ring = [&](int p, int k, int r, int l) {
     return (((int64_t)p * no + k) * nv + r) * no + l;
   };
+  // packed: the stored (a,b) pair and its sign; a == b of equal spins is
+  // zero (never stored)
+  int pa = a, pb = b;
+  double sg = 1.0;
+  const bool zab = packed && a == b;
+  if (packed && lv[a] != lv[b] && lv[a] == 1) {
+    pa = b;
+    pb = a;
+    sg = -1.0;
+  }
+  // packed (i,j): 1 = stored as (i,j), -1 = stored as (j,i), 0 = zero
+  auto ij_sign = [&](int i, int j) {
+    if (lo[i] == lo[j]) return i < j ? 1 : (i > j ? -1 : 0);
+    return lo[i] == 0 ? 1 : -1;
+  };
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + tx, j = j0 + r;
     const bool ok = i < no && j < no && (!lv || lo[i] + lo[j] == sab);
     tA[r][tx] = ok ? s2[nat(a, b, j, i)] : 0.0;
     tB[r][tx] = ok ? s2[nat(b, a, j, i)] : 0.0;
-    tZ1[r][tx] = ok ? z[ring(a, j, b, i)] : 0.0;
-    tZ2[r][tx] = ok ? z[ring(b, j, a, i)] : 0.0;
+    tZ[r][tx] = ok ? z[ring(b, j, a, i)] - z[ring(a, j, b, i)] : 0.0;
+    if (packed) {
+      const bool tr = ok && !zab && ij_sign(i, j) < 0;
+      const int64_t e = nat(pa, pb, j, i);
+      tP[r][tx] = tr ? c1[e] + c2[e] : 0.0;
+    }
   }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
@@ -330,18 +354,31 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
       out[nba] = 0.0;
       continue;
     }
+    double accab, accba;
+    if (packed) {
+      double v = 0.0;
+      if (!zab) {
+        const int sij = ij_sign(i, j);
+        if (sij > 0) {
+          const int64_t e = nat(pa, pb, i, j);
+          v = c1[e] + c2[e];
+        } else if (sij < 0) {
+          v = -tP[tx][r];
+        }
+      }
+      accab = sg * v;
+      accba = -accab;
+    } else {
+      accab = c1[nab] + c2[nab];
+      accba = c1[nba] + c2[nba];
+    }
     const double s1ab = s1[nab], s1ba = s1[nba];
-    const double zz = z[ring(a, i, b, j)] - z[ring(b, i, a, j)] -
-                      tZ1[tx][r] + tZ2[tx][r];
-    double vab = out[nab] + s1ab - s1ba + s2[nab] - tA[tx][r] + zz;
-    double vba = out[nba] + s1ba - s1ab + s2[nba] - tB[tx][r] - zz;
+    const double zz = z[ring(a, i, b, j)] - z[ring(b, i, a, j)] + tZ[tx][r];
+    double vab = accab + s1ab - s1ba + s2[nab] - tA[tx][r] + zz;
+    double vba = accba + s1ba - s1ab + s2[nba] - tB[tx][r] - zz;
     if (add) {
       vab += add[nab];
       vba += add[nba];
-    }
-    if (add2) {
-      vab += add2[y * per + nab];
-      vba += add2[y * per + nba];
     }
     out[nab] = -vab;
     if (a != b) out[nba] = -vba;
@@ -484,17 +521,18 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
          ovvo_ring, Q1, Q2, nv, no, nb);
 }
 
-void r2_final(double *t2bar, const double *S1, const double *S2,
-              const double *Z, int nv, int no, int nb, hipStream_t s,
-              const int *lab_v, const int *lab_o, const double *add,
-              const double *add2) {
+void r2_final(double *t2bar, const double *acc1, const double *acc2,
+              const double *S1, const double *S2, const double *Z, int nv,
+              int no, int nb, hipStream_t s, const int *lab_v,
+              const int *lab_o, const double *add, bool packed) {
+  require(!packed || (lab_v && lab_o), "packed pairs need spin labels");
   const int tiles = (no + 31) / 32;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   const int64_t blocks = (int64_t)nb * npair * tiles * tiles;
   require(blocks < (1LL << 31), "r2 assembly grid too large");
   hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks), dim3(256),
-                     0, s, t2bar, S1, S2, Z, add, add2, nv, no, lab_v,
-                     lab_o);
+                     0, s, t2bar, acc1, acc2, S1, S2, Z, add, nv, no, lab_v,
+                     lab_o, packed ? 1 : 0);
   FTCC_CHECK_HIP(hipGetLastError());
 }
 
