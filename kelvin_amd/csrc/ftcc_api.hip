@@ -79,7 +79,7 @@ struct ftcc_ctx {
   // imaginary-time propagation: stage input, accumulator and right-hand
   // side (singles / doubles), allocated on first use
   double *td_var[2] = {}, *td_acc[2] = {}, *td_k[2] = {}, *td_k1[2] = {};
-  double *tau = nullptr, *taut, *t2r, *aring, *wmnij, *wabef,
+  double *tau = nullptr, *t2r, *aring, *wmnij, *wabef,
       *wr, *z, *u, *s1, *s2;
   double *t1T, *Fme, *FmeT, *Fae, *Fmi, *Fbe, *Fmj, *ytmp;
   Scratch scr;
@@ -706,17 +706,11 @@ void operand_blocks(const ftcc_ctx *c, int id, Range rows[3], Range ks[3]) {
   }
 }
 
-// Write operand `id` of nb slots from its FP64 source: src[b*sb + row*sr +
-// k*sc] with row, k flat pair indices of the dense layout.
-void put_operand(ftcc_ctx *c, int id, const double *src, int64_t sr,
-                 int64_t sc, int64_t sb, int nb) {
-  hipStream_t s = c->stream;
-  if (!c->spin) {
-    if (c->prec == FTCC_PREC_BF16X3)
-      split_convert(src, sr, sc, sb, space_len(c, OP_ROWS[id]),
-                    space_len(c, op_k(c, id)), split_of(c, id), nb, s);
-    return;
-  }
+// The spin blocks of operand `id` (operand_blocks) through the pair maps:
+// split(rmap, rows, kmap, K, dst) in bf16x3 mode, f64(rmap, rows, kmap, K,
+// ld, dst, slice stride) otherwise.
+template <typename SplitFn, typename F64Fn>
+void gather_blocks(ftcc_ctx *c, int id, SplitFn split, F64Fn f64) {
   const PairSpace &R = space(c, op_rows(c, id)), &K = space(c, op_k(c, id));
   Range rows[3], ks[3];
   operand_blocks(c, id, rows, ks);
@@ -728,15 +722,55 @@ void put_operand(ftcc_ctx *c, int id, const double *src, int64_t sr,
       const int64_t off = (int64_t)rows[q].start * m.ld + ks[q].start;
       m.hi += off;
       m.lo += off;
-      split_gather(src, sr, sc, sb, rmap, rows[q].len, kmap, ks[q].len, m, nb,
-                   s);
+      split(rmap, rows[q].len, kmap, ks[q].len, m);
     } else {
       const int64_t ld = K.total;
-      gather64(src, sr, sc, sb, rmap, rows[q].len, kmap, ks[q].len, ld,
-               c->b64[id] + (int64_t)rows[q].start * ld + ks[q].start,
-               (int64_t)R.total * ld, nb, s);
+      f64(rmap, rows[q].len, kmap, ks[q].len, ld,
+          c->b64[id] + (int64_t)rows[q].start * ld + ks[q].start,
+          (int64_t)R.total * ld);
     }
   }
+}
+
+// Write operand `id` of nb slots from its FP64 source: src[b*sb + row*sr +
+// k*sc] with row, k flat pair indices of the dense layout.
+void put_operand(ftcc_ctx *c, int id, const double *src, int64_t sr,
+                 int64_t sc, int64_t sb, int nb) {
+  hipStream_t s = c->stream;
+  if (!c->spin) {
+    if (c->prec == FTCC_PREC_BF16X3)
+      split_convert(src, sr, sc, sb, space_len(c, OP_ROWS[id]),
+                    space_len(c, op_k(c, id)), split_of(c, id), nb, s);
+    return;
+  }
+  gather_blocks(
+      c, id,
+      [&](const int *rmap, int R, const int *kmap, int W, const SplitMat &m) {
+        split_gather(src, sr, sc, sb, rmap, R, kmap, W, m, nb, s);
+      },
+      [&](const int *rmap, int R, const int *kmap, int W, int64_t ld,
+          double *dst, int64_t db) {
+        gather64(src, sr, sc, sb, rmap, R, kmap, W, ld, dst, db, nb, s);
+      });
+}
+
+// Spin layout: operand `id` gathered straight from the amplitudes t1, t2 of
+// nb slices (tau, t2 in ring order, the ring A of G4: see AmpKind)
+void put_amp_operand(ftcc_ctx *c, int id, int kind, const double *t1,
+                     const double *t2, int nb) {
+  require(c->spin, "amplitude gathers need the spin layout");
+  hipStream_t s = c->stream;
+  const int V = c->nv, O = c->no;
+  gather_blocks(
+      c, id,
+      [&](const int *rmap, int R, const int *kmap, int W, const SplitMat &m) {
+        amp_split_gather(kind, t1, t2, V, O, rmap, R, kmap, W, m, nb, s);
+      },
+      [&](const int *rmap, int R, const int *kmap, int W, int64_t ld,
+          double *dst, int64_t db) {
+        amp_gather64(kind, t1, t2, V, O, rmap, R, kmap, W, ld, dst, db, nb,
+                     s);
+      });
 }
 
 // operands that depend on the integrals only
@@ -1004,16 +1038,17 @@ void ensure_integrals(ftcc_ctx *c) {
 // whole grid at n <= 66) -- larger chunks measured faster at every size
 // tried (fewer, fuller launches), 2.8% at n = 114 from two to four.
 void ensure_workspace(ftcc_ctx *c) {
-  if (c->tau) return;
+  if (c->aring) return;
   const int O = c->no, V = c->nv, ng = c->ng;
   const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
   const int64_t S2 = VV * OO;
   const int64_t sm = std::max<int64_t>(VV, std::max<int64_t>(VO, OO));
-  // eight n^4 intermediates, Wmnij/Wabef/U, the seven per-slot product
-  // operands split into bf16 pairs (bf16x3) and as FP64 spin-blocked copies
-  // (FP64/FP32 modes; both exist after a precision switch)
+  // five n^4 intermediates (seven in the dense layout, which also forms tau
+  // and t2r), Wmnij/Wabef/U, the seven per-slot product operands split into
+  // bf16 pairs (bf16x3) and as FP64 spin-blocked copies (FP64/FP32 modes;
+  // both exist after a precision switch)
   const int64_t per_slot =
-      8 * (8 * S2 + OO * OO + VV * VV + OO * VO + 8 * sm) +
+      8 * ((c->spin ? 5 : 7) * S2 + OO * OO + VV * VV + OO * VO + 8 * sm) +
       4 * 7 * (S2 + 64 * std::max<int64_t>(VV, std::max<int64_t>(VO, OO))) +
       8 * 7 * S2;
   double budget_gb = 120.0;
@@ -1024,9 +1059,6 @@ void ensure_workspace(ftcc_ctx *c) {
   if (c->nb_fixed > 0)
     nb = (int)std::min<int64_t>(std::min<int64_t>(ng, c->nb_fixed), cap);
   c->nb = nb;
-  c->tau = dalloc(c, S2 * nb);
-  c->taut = dalloc(c, S2 * nb);
-  c->t2r = dalloc(c, S2 * nb);
   c->aring = dalloc(c, S2 * nb);
   c->wr = dalloc(c, S2 * nb);
   c->z = dalloc(c, S2 * nb);
@@ -1049,7 +1081,7 @@ void ensure_workspace(ftcc_ctx *c) {
 // are rebuilt on the next residual with the current chunk size).
 void free_workspace(ftcc_ctx *c) {
   FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
-  for (double **p : {&c->tau, &c->taut, &c->t2r, &c->aring, &c->wr, &c->z,
+  for (double **p : {&c->tau, &c->t2r, &c->aring, &c->wr, &c->z,
                      &c->s1, &c->s2, &c->wmnij,
                      &c->wabef, &c->u, &c->t1T, &c->Fme, &c->FmeT, &c->Fae,
                      &c->Fmi, &c->Fbe, &c->Fmj, &c->ytmp}) {
@@ -1109,17 +1141,31 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
                *vvoo = c->I[3], *oovv = c->I[5], *vooo = c->I[6],
                *ooov = c->I[7], *oooo = c->I[8];
 
-  // amplitude-derived operands
-  PrepOut po{c->tau, c->taut, c->t2r, c->aring};
-  prep_amplitudes(t1, t2, V, O, nb, po, s);
   transpose2(t1, c->t1T, V, O, nb, s);   // t1T[m,f] = t1[f,m]
   // product operands (split into bf16 pairs, and/or reordered by spin
-  // class): A as [M][K], B as [N][K] (K contiguous)
+  // class): A as [M][K], B as [N][K] (K contiguous).  The spin layout
+  // gathers the amplitude-derived ones straight from t1 and t2; the dense
+  // layout reads them whole from their FP64 forms.
   ensure_fixed_operands(c);
-  put_operand(c, OP_TAU_A, c->tau, OO, 1, S2, nb);
-  put_operand(c, OP_TAU_B, c->tau, 1, OO, S2, nb);
-  put_operand(c, OP_T2R_A, c->t2r, VO, 1, S2, nb);
-  put_operand(c, OP_ARING_B, c->aring, 1, VO, S2, nb);
+  if (c->spin) {
+    put_amp_operand(c, OP_TAU_A, AMP_TAU_AB, t1, t2, nb);
+    put_amp_operand(c, OP_TAU_B, AMP_TAU_IJ, t1, t2, nb);
+    put_amp_operand(c, OP_T2R_A, AMP_T2R, t1, t2, nb);
+    put_amp_operand(c, OP_ARING_B, AMP_ARING, t1, t2, nb);
+  } else {
+    // dense layout (also a spin-blocked system's rare dense fallback):
+    // tau and t2r are formed whole
+    if (!c->tau) {
+      c->tau = dalloc(c, S2 * nb);
+      c->t2r = dalloc(c, S2 * nb);
+    }
+    PrepOut po{c->tau, c->t2r, c->aring};
+    prep_amplitudes(t1, t2, V, O, nb, po, s);
+    put_operand(c, OP_TAU_A, c->tau, OO, 1, S2, nb);
+    put_operand(c, OP_TAU_B, c->tau, 1, OO, S2, nb);
+    put_operand(c, OP_T2R_A, c->t2r, VO, 1, S2, nb);
+    put_operand(c, OP_ARING_B, c->aring, 1, VO, S2, nb);
+  }
 
   // Fme[m,e] = Fov[m,e] + sum_{fn} <mn||ef> t_n^f: FmeT[y][e,m] over the
   // grid points y as columns (oovv_ring[e,m,f,n] read once)
@@ -1140,10 +1186,13 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   bcast(c->Fme, Fov, VO, nb, 1.0, 1.0, s);
   transpose2(c->Fme, c->FmeT, O, V, nb, s);
 
-  // Fae[a,e] = Fvv - 1/2 t1.Fov + sum t_m^f <ma||fe> - 1/2 taut.oovv
+  // Fae[a,e] = Fvv - 1/2 t1.Fov + sum t_m^f <ma||fe> - 1/2 taut.oovv with
+  // taut = t2 + 1/2 (t1 t1 - t1 t1): its t1 t1 part contracted with oovv is
+  // 1/2 t1.(Fme - Fov), so Fae = Fvv - 1/2 t1.Fme + sum t1.vovv
+  // - 1/2 t2.oovv (and likewise for Fmi below): no taut is formed
   bcast(c->Fae, Fvv, VV, nb, 1.0, 0.0, s);
-  run(c, mk(V, V, O, -0.5, 1.0, t1, O, 1, VO, Fov, V, 1, 0, c->Fae, V, 1, VV,
-            nb), P64);
+  run(c, mk(V, V, O, -0.5, 1.0, t1, O, 1, VO, c->Fme, V, 1, VO, c->Fae, V, 1,
+            VV, nb), P64);
   // + sum_{mf} t_m^f <ma||fe> = - sum_{fm} t1[f,m] vovv[a,m,f,e], vovv[a,m,f,e]
   // = vovv_aefm[a,f,e,m]: rows y, columns (a,e), K = (f,m)
   if (s5) {
@@ -1166,24 +1215,25 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     g.sBk1 = VO;
     run(c, g, P5);
   }
-  // - 1/2 sum_{fmn} taut[a,f,m,n] <mn||ef> (oovvT[e,f,m,n]: K fastest)
+  // - 1/2 sum_{fmn} t2[a,f,m,n] <mn||ef> (oovvT[e,f,m,n]: K fastest)
   // (the chunk's grid points are rows of one launch: oovvT is read once)
   if (s5)
-    run_spin(c, sgem(-0.5, 1.0, c->taut, S2, c->oovvT, 0, c->Fae, VV,
+    run_spin(c, sgem(-0.5, 1.0, t2, S2, c->oovvT, 0, c->Fae, VV,
                      nullptr),
              P5, {{'y', 'y', S2, 0, VV}, {'a', 'v', V * OO, 0, V}},
              {{'e', 'v', 0, V * OO, 1}},
              {{'f', 'v', OO, OO, 0}, {'m', 'o', O, O, 0}, {'n', 'o', 1, 1, 0}},
              "af|mn", "mn|ef", "a|e");
   else
-    run(c, fold_m(mk(V, V, (int)(V * OO), -0.5, 1.0, c->taut, V * OO, 1, S2,
+    run(c, fold_m(mk(V, V, (int)(V * OO), -0.5, 1.0, t2, V * OO, 1, S2,
                      c->oovvT, 1, V * OO, 0, c->Fae, V, 1, VV, nb)),
         P5);
 
   // Fmi[m,i] = Foo + 1/2 Fov.t1 + sum t_n^e <mn||ie> + 1/2 oovv.taut
+  //         = Foo + 1/2 Fme.t1 + sum t_n^e <mn||ie> + 1/2 oovv.t2
   bcast(c->Fmi, Foo, OO, nb, 1.0, 0.0, s);
-  run(c, mk(O, O, V, 0.5, 1.0, Fov, V, 1, 0, t1, O, 1, VO, c->Fmi, O, 1, OO,
-            nb), P64);
+  run(c, mk(O, O, V, 0.5, 1.0, c->Fme, V, 1, VO, t1, O, 1, VO, c->Fmi, O, 1,
+            OO, nb), P64);
   // + sum_{ne} <mn||ie> t_n^e: ooov_p1[m,i,n,e] rows (m,i), columns y
   if (s5)
     run_spin(c, sgem(1.0, 1.0, c->ooov_p1, 0, c->t1T, VO, c->Fmi, OO,
@@ -1196,10 +1246,10 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, mk((int)OO, nb, (int)VO, 1.0, 1.0, c->ooov_p1, VO, 1, 0, c->t1T, 1,
               VO, 0, c->Fmi, 1, OO, 0, 1),
         P5);
-  // + 1/2 sum_{nef} <mn||ef> taut[e,f,i,n] over K = (e, f, n), n fastest in
+  // + 1/2 sum_{nef} <mn||ef> t2[e,f,i,n] over K = (e, f, n), n fastest in
   // both (oovv_mefn[m,e,f,n]); the grid points are columns of one launch
   if (s5) {
-    run_spin(c, sgem(0.5, 1.0, c->oovv_mefn, 0, c->taut, S2, c->Fmi, OO,
+    run_spin(c, sgem(0.5, 1.0, c->oovv_mefn, 0, t2, S2, c->Fmi, OO,
                      nullptr),
              P5, {{'m', 'o', V * VO, 0, O}},
              {{'y', 'y', 0, S2, OO}, {'i', 'o', 0, O, 1}},
@@ -1208,7 +1258,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              "mn|ef", "ef|in", "m|i");
   } else {
     Gemm g = fold_n(mk(O, O, (int)(O * VV), 0.5, 1.0, c->oovv_mefn, V * VO, 1,
-                       0, c->taut, 1, O, S2, c->Fmi, O, 1, OO, nb));
+                       0, t2, 1, O, S2, c->Fmi, O, 1, OO, nb));
     g.K1 = (int)VV;   // k = (e,f) * O + n
     g.K2 = O;
     g.sAk1 = O;
@@ -1443,12 +1493,12 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
             nb), P64);
   run(c, mk(V, O, O, -1.0, 1.0, t1, O, 1, VO, c->Fmi, O, 1, OO, t1bar, O, 1,
             VO, nb), P64);
-  // + sum_{em} t2r[a,i,e,m] FmeT[e,m] (per grid point) and
+  // + sum_{em} t2[a,e,i,m] FmeT[e,m] (per grid point) and
   // + sum_{em} t1[e,m] ovvo_ring[e,m,a,i] (ovvo_ringT[a,i,e,m]; rows y)
   if (s5) {
-    run_spin(c, sgem(1.0, 1.0, c->t2r, S2, c->FmeT, VO, t1bar, VO, nullptr),
-             P5, {{'a', 'v', O * VO, 0, O}, {'i', 'o', VO, 0, 1}},
-             {{'_', '1', 0, 0, 0}}, {{'e', 'v', O, O, 0}, {'m', 'o', 1, 1, 0}},
+    run_spin(c, sgem(1.0, 1.0, t2, S2, c->FmeT, VO, t1bar, VO, nullptr), P5,
+             {{'a', 'v', V * OO, 0, O}, {'i', 'o', O, 0, 1}},
+             {{'_', '1', 0, 0, 0}}, {{'e', 'v', OO, O, 0}, {'m', 'o', 1, 1, 0}},
              "ae|im", "e|m", "a|i");
     run_spin(c, sgem(1.0, 1.0, t1, VO, c->ovvo_ringT, 0, t1bar, VO, nullptr),
              P5, {{'y', 'y', VO, 0, VO}},
@@ -1456,8 +1506,18 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              {{'e', 'v', O, O, 0}, {'m', 'o', 1, 1, 0}}, "e|m", "ma|ei",
              "a|i");
   } else {
-    gemv_rows(c->t2r, VO, S2, c->FmeT, VO, t1bar, VO, (int)VO, (int)VO, nb,
-              1.0, 1.0, s);
+    // rows (a,i) and K = (e,m) over two index levels each
+    Gemm g = mk((int)VO, 1, (int)VO, 1.0, 1.0, t2, O, 1, S2, c->FmeT, 1, 0, VO,
+                t1bar, 1, 0, VO, nb);
+    g.M1 = V;
+    g.M2 = O;
+    g.sAm1 = V * OO;
+    g.sCm1 = O;
+    g.K1 = V;
+    g.K2 = O;
+    g.sAk1 = OO;
+    g.sBk1 = O;
+    run(c, g, P5);
     run(c, mk(nb, (int)VO, (int)VO, 1.0, 1.0, t1, VO, 1, 0, c->ovvo_ringT, 1,
               VO, 0, t1bar, VO, 1, 0, 1),
         P5);

@@ -118,6 +118,19 @@ void split_gather(const double *src, int64_t sr, int64_t sc, int64_t sb,
 void gather64(const double *src, int64_t sr, int64_t sc, int64_t sb,
               const int *rmap, int R, const int *cmap, int W, int64_t os,
               double *dst, int64_t db, int nb, hipStream_t s);
+// Amplitude-derived operands gathered straight from t1 [nb][V][O] and
+// t2 [nb][V][V][O][O] (rows / columns are pair indices x * ny + y):
+//   AMP_TAU_AB  rows (a,b), columns (i,j): t2 + t1[a,i]t1[b,j] - t1[b,i]t1[a,j]
+//   AMP_TAU_IJ  rows (i,j), columns (a,b): the same
+//   AMP_T2R     rows (a,i), columns (e,m): t2[a,e,i,m]
+//   AMP_ARING   rows (b,j), columns (f,n): -1/2 t2[f,b,n,j] + t1[f,j]t1[b,n]
+enum AmpKind { AMP_TAU_AB, AMP_TAU_IJ, AMP_T2R, AMP_ARING };
+void amp_split_gather(int kind, const double *t1, const double *t2, int nv,
+                      int no, const int *rmap, int R, const int *cmap, int W,
+                      const SplitMat &d, int nb, hipStream_t s);
+void amp_gather64(int kind, const double *t1, const double *t2, int nv,
+                  int no, const int *rmap, int R, const int *cmap, int W,
+                  int64_t os, double *dst, int64_t db, int nb, hipStream_t s);
 // C[b] = alpha * A[b] . B[b]^T + beta * C[b] with A as [M][K], B as [N][K];
 // output row m goes to row rmap[m] of C and column n to cmap[n] when the
 // maps are given (-1 drops the element)
@@ -201,7 +214,7 @@ void denominators(const double *ev, const double *eo, int nv, int no,
 
 // Per-slice preparation of the amplitude-derived operands (see ftcc_api.hip)
 struct PrepOut {
-  double *tau, *taut, *t2r, *aring;
+  double *tau, *t2r, *aring;
 };
 void prep_amplitudes(const double *t1, const double *t2, int nv, int no,
                      int nb, const PrepOut &o, hipStream_t s);

@@ -250,12 +250,57 @@ __device__ inline int src_index(const int *map, int i, int n) {
   return map ? map[i] : (i < n ? i : -1);
 }
 
-// dst[b][r][c] (row stride os) = src[b*sb + R(r)*sr + C(c)*sc] for r < R,
-// c < W, zero where a map gives -1 (or, without maps, for c >= C); threads
-// run along c (use when sc is the small stride).
-template <typename Out>
-__global__ void gather_rows_kernel(const double *src, int64_t sr, int64_t sc,
-                                   int64_t sb, const int *rmap, int R,
+// Sources of the gather kernels: value of the dense matrix at (slice b,
+// row pair rr, column pair cc).  Plain: a strided FP64 tensor.
+struct PlainSrc {
+  const double *src;
+  int64_t sr, sc, sb;
+  __device__ double operator()(int64_t b, int rr, int cc) const {
+    return src[b * sb + (int64_t)rr * sr + (int64_t)cc * sc];
+  }
+};
+// Amplitude-derived operands computed from t1 [V][O] and t2 [V][V][O][O]
+// of each slice as they are gathered (nothing is materialised):
+//   AMP_TAU_AB: rows (a,b), columns (i,j): tau = t2 + t1 t1 - t1 t1
+//   AMP_TAU_IJ: rows (i,j), columns (a,b): the same tau
+//   AMP_T2R:    rows (a,i), columns (e,m): t2[a,e,i,m]
+//   AMP_ARING:  rows (b,j), columns (f,n): -1/2 t2[f,b,n,j] + t1[f,j] t1[b,n]
+// (pair indices x * ny + y of the pair maps).
+struct AmpSrc {
+  const double *t1, *t2;
+  int nv, no, kind;
+  __device__ double operator()(int64_t b, int rr, int cc) const {
+    const int64_t VO = (int64_t)nv * no;
+    const double *x1 = t1 + b * VO, *x2 = t2 + b * VO * VO;
+    auto T2 = [&](int a, int e, int i, int m) {
+      return x2[(((int64_t)a * nv + e) * no + i) * no + m];
+    };
+    auto T1 = [&](int a, int i) { return x1[(int64_t)a * no + i]; };
+    switch (kind) {
+      case AMP_TAU_AB:
+      case AMP_TAU_IJ: {
+        const int ab = kind == AMP_TAU_AB ? rr : cc;
+        const int ij = kind == AMP_TAU_AB ? cc : rr;
+        const int a = ab / nv, bb = ab % nv, i = ij / no, j = ij % no;
+        return T2(a, bb, i, j) + T1(a, i) * T1(bb, j) - T1(bb, i) * T1(a, j);
+      }
+      case AMP_T2R: {
+        const int a = rr / no, i = rr % no, e = cc / no, m = cc % no;
+        return T2(a, e, i, m);
+      }
+      default: {   // AMP_ARING
+        const int bb = rr / no, j = rr % no, f = cc / no, n = cc % no;
+        return -0.5 * T2(f, bb, n, j) + T1(f, j) * T1(bb, n);
+      }
+    }
+  }
+};
+
+// dst[b][r][c] (row stride os) = src(b, R(r), C(c)) for r < R, c < W, zero
+// where a map gives -1 (or, without maps, for c >= C); threads run along c
+// (use when the source is contiguous along columns).
+template <typename Src, typename Out>
+__global__ void gather_rows_kernel(Src src, const int *rmap, int R,
                                    const int *cmap, int C, int W, int64_t os,
                                    int64_t db, Out out, int nb) {
   const int64_t per = (int64_t)R * W;
@@ -265,19 +310,15 @@ __global__ void gather_rows_kernel(const double *src, int64_t sr, int64_t sc,
     const int r = (int)(e / W);
     const int c = (int)(e % W);
     const int rr = src_index(rmap, r, R), cc = src_index(cmap, c, C);
-    const double v = (rr >= 0 && cc >= 0)
-                         ? src[b * sb + (int64_t)rr * sr + (int64_t)cc * sc]
-                         : 0.0;
+    const double v = (rr >= 0 && cc >= 0) ? src(b, rr, cc) : 0.0;
     out(b * db + (int64_t)r * os + c, v);
   }
 }
 
-// Same map through a 64x64 LDS tile: reads run along r (sr is the small
-// stride), writes along c.
-template <typename Out>
-__global__ void gather_transpose_kernel(const double *src, int64_t sr,
-                                        int64_t sc, int64_t sb,
-                                        const int *rmap, int R,
+// Same map through a 64x64 LDS tile: reads run along r (the source is
+// contiguous along rows), writes along c.
+template <typename Src, typename Out>
+__global__ void gather_transpose_kernel(Src src, const int *rmap, int R,
                                         const int *cmap, int C, int W,
                                         int64_t os, int64_t db, Out out) {
   __shared__ double tile[64][65];
@@ -288,9 +329,7 @@ __global__ void gather_transpose_kernel(const double *src, int64_t sr,
     const int r = r0 + tx, c = c0 + i;
     const int rr = r < R ? src_index(rmap, r, R) : -1;
     const int cc = c < W ? src_index(cmap, c, C) : -1;
-    tile[i][tx] = (rr >= 0 && cc >= 0)
-                      ? src[b * sb + (int64_t)rr * sr + (int64_t)cc * sc]
-                      : 0.0;
+    tile[i][tx] = (rr >= 0 && cc >= 0) ? src(b, rr, cc) : 0.0;
   }
   __syncthreads();
   for (int i = ty; i < 64; i += 4) {
@@ -299,23 +338,38 @@ __global__ void gather_transpose_kernel(const double *src, int64_t sr,
   }
 }
 
-template <typename Out>
-void gather_launch(const double *src, int64_t sr, int64_t sc, int64_t sb,
-                   const int *rmap, int R, const int *cmap, int C, int W,
-                   int64_t os, int64_t db, Out out, int nb, hipStream_t s) {
+template <typename Src, typename Out>
+void gather_launch(Src src, bool along_rows, const int *rmap, int R,
+                   const int *cmap, int C, int W, int64_t os, int64_t db,
+                   Out out, int nb, hipStream_t s) {
   if ((int64_t)R * W == 0 || nb <= 0) return;
-  if (sr == 1 && sc != 1) {
+  if (along_rows) {
     const int gx = (W + 63) / 64, gy = (R + 63) / 64;
-    hipLaunchKernelGGL(gather_transpose_kernel<Out>, dim3(gx, gy, nb),
-                       dim3(256), 0, s, src, sr, sc, sb, rmap, R, cmap, C, W,
-                       os, db, out);
+    require(gy < 65536 && nb < 65536, "gather grid too large");
+    hipLaunchKernelGGL((gather_transpose_kernel<Src, Out>), dim3(gx, gy, nb),
+                       dim3(256), 0, s, src, rmap, R, cmap, C, W, os, db,
+                       out);
   } else {
     const int64_t total = (int64_t)R * W * nb;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
-    hipLaunchKernelGGL(gather_rows_kernel<Out>, dim3(blocks), dim3(256), 0, s,
-                       src, sr, sc, sb, rmap, R, cmap, C, W, os, db, out, nb);
+    hipLaunchKernelGGL((gather_rows_kernel<Src, Out>), dim3(blocks),
+                       dim3(256), 0, s, src, rmap, R, cmap, C, W, os, db, out,
+                       nb);
   }
   FTCC_CHECK_HIP(hipGetLastError());
+}
+
+// a plain source is read along rows when its row stride is the unit one
+PlainSrc plain(const double *src, int64_t sr, int64_t sc, int64_t sb,
+               bool &along_rows) {
+  along_rows = (sr == 1 && sc != 1);
+  return PlainSrc{src, sr, sc, sb};
+}
+// tau by (i,j) rows and aring by (b,j) rows are contiguous along rows
+AmpSrc amp(int kind, const double *t1, const double *t2, int nv, int no,
+           bool &along_rows) {
+  along_rows = (kind == AMP_TAU_IJ || kind == AMP_ARING);
+  return AmpSrc{t1, t2, nv, no, kind};
 }
 
 }  // namespace
@@ -326,23 +380,49 @@ void split_convert(const double *src, int64_t sr, int64_t sc, int64_t sb,
                    int R, int C, const SplitMat &d, int nb, hipStream_t s) {
   require(d.ld >= C && d.ld % 32 == 0, "split operand leading dimension");
   if ((int64_t)R * C == 0) return;
-  gather_launch(src, sr, sc, sb, nullptr, R, nullptr, C, (int)d.ld, d.ld,
-                d.bstride, SplitOut{d.hi, d.lo}, nb, s);
+  bool ar;
+  const PlainSrc src_ = plain(src, sr, sc, sb, ar);
+  gather_launch(src_, ar, nullptr, R, nullptr, C, (int)d.ld, d.ld, d.bstride,
+                SplitOut{d.hi, d.lo}, nb, s);
 }
 
 void split_gather(const double *src, int64_t sr, int64_t sc, int64_t sb,
                   const int *rmap, int R, const int *cmap, int W,
                   const SplitMat &d, int nb, hipStream_t s) {
   require(d.ld % 32 == 0 && W <= d.ld, "split operand leading dimension");
-  gather_launch(src, sr, sc, sb, rmap, R, cmap, W, W, d.ld, d.bstride,
+  bool ar;
+  const PlainSrc src_ = plain(src, sr, sc, sb, ar);
+  gather_launch(src_, ar, rmap, R, cmap, W, W, d.ld, d.bstride,
                 SplitOut{d.hi, d.lo}, nb, s);
+}
+
+void amp_split_gather(int kind, const double *t1, const double *t2, int nv,
+                      int no, const int *rmap, int R, const int *cmap, int W,
+                      const SplitMat &d, int nb, hipStream_t s) {
+  require(d.ld % 32 == 0 && W <= d.ld, "split operand leading dimension");
+  require(rmap && cmap, "amplitude operands are gathered through pair maps");
+  bool ar;
+  const AmpSrc src_ = amp(kind, t1, t2, nv, no, ar);
+  gather_launch(src_, ar, rmap, R, cmap, W, W, d.ld, d.bstride,
+                SplitOut{d.hi, d.lo}, nb, s);
+}
+
+void amp_gather64(int kind, const double *t1, const double *t2, int nv,
+                  int no, const int *rmap, int R, const int *cmap, int W,
+                  int64_t os, double *dst, int64_t db, int nb,
+                  hipStream_t s) {
+  require(rmap && cmap, "amplitude operands are gathered through pair maps");
+  bool ar;
+  const AmpSrc src_ = amp(kind, t1, t2, nv, no, ar);
+  gather_launch(src_, ar, rmap, R, cmap, W, W, os, db, F64Out{dst}, nb, s);
 }
 
 void gather64(const double *src, int64_t sr, int64_t sc, int64_t sb,
               const int *rmap, int R, const int *cmap, int W, int64_t os,
               double *dst, int64_t db, int nb, hipStream_t s) {
-  gather_launch(src, sr, sc, sb, rmap, R, cmap, W, W, os, db, F64Out{dst}, nb,
-                s);
+  bool ar;
+  const PlainSrc src_ = plain(src, sr, sc, sb, ar);
+  gather_launch(src_, ar, rmap, R, cmap, W, W, os, db, F64Out{dst}, nb, s);
 }
 
 void sbgemm_grouped(int nprob, const SbProblem *prob, int nb, double alpha,

@@ -193,12 +193,11 @@ __global__ void denominators_kernel(const double *ev, const double *eo, int nv,
 
 // One thread per t2 element (a,b,i,j) of each slice:
 //   tau   [a,b,i,j] = t2 + t1[a,i]t1[b,j] - t1[b,i]t1[a,j]
-//   taut  [a,b,i,j] = t2 + (t1[a,i]t1[b,j] - t1[b,i]t1[a,j])/2
 //   t2r   [a,i,b,j] = t2[a,b,i,j]                    (ring layout)
 //   aring [a,i,b,j] = -t2[a,b,i,j]/2 + t1[a,j]t1[b,i]
 __global__ void prep_kernel(const double *t1s, const double *t2s, int nv,
-                            int no, int nb, double *tau, double *taut,
-                            double *t2r, double *aring) {
+                            int no, int nb, double *tau, double *t2r,
+                            double *aring) {
   const int64_t per = (int64_t)nv * nv * no * no;
   const int64_t per1 = (int64_t)nv * no;
   GRID_STRIDE(idx, per * nb) {
@@ -215,9 +214,7 @@ __global__ void prep_kernel(const double *t1s, const double *t2s, int nv,
     const double v = t2s[y * per + e];
     const double x = t1[a * no + i] * t1[b * no + j] -
                      t1[b * no + i] * t1[a * no + j];
-    const double tt = v + 0.5 * x;
     tau[y * per + e] = v + x;
-    taut[y * per + e] = tt;
     const int64_t ring = (((int64_t)a * no + i) * nv + b) * no + j;
     t2r[y * per + ring] = v;
     aring[y * per + ring] = -0.5 * v + t1[a * no + j] * t1[b * no + i];
@@ -506,7 +503,7 @@ void denominators(const double *ev, const double *eo, int nv, int no,
 void prep_amplitudes(const double *t1, const double *t2, int nv, int no,
                      int nb, const PrepOut &o, hipStream_t s) {
   LAUNCH(prep_kernel, (int64_t)nv * nv * no * no * nb, t1, t2, nv, no, nb,
-         o.tau, o.taut, o.t2r, o.aring);
+         o.tau, o.t2r, o.aring);
 }
 
 void antisym_add(double *W, const double *Q, int d0, int d1, int d2, int d3,
