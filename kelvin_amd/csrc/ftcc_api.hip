@@ -1594,7 +1594,8 @@ bool amplitudes_conserve_spin(ftcc_ctx *c, const double *T1,
   return bad == 0.0;
 }
 
-void residual_all(ftcc_ctx *c, const double *T1, const double *T2,
+// Returns whether the spin layout was used (the amplitudes conserve spin).
+bool residual_all(ftcc_ctx *c, const double *T1, const double *T2,
                   double *T1b, double *T2b, uint64_t skip) {
   ensure_workspace(c);
   // spin blocks are exact only for spin-conserving amplitudes (every CC
@@ -1625,6 +1626,7 @@ void residual_all(ftcc_ctx *c, const double *T1, const double *T2,
                    nb);
     y += nb;
   }
+  return c->spin != 0;
 }
 
 UpdateArgs upd(ftcc_ctx *c) {
@@ -2288,7 +2290,7 @@ int ftcc_sumsq(ftcc_ctx *c, const double *x, int64_t n, double *out) {
 void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
                  double *T2b, double damp, uint64_t skip) {
   hipStream_t s = c->stream;
-  residual_all(c, T1old, T2old, T1b, T2b, skip);
+  const bool spin_used = residual_all(c, T1old, T2old, T1b, T2b, skip);
   UpdateArgs a = upd(c);
   a.damp = damp;
   a.tbar = T1b;
@@ -2302,6 +2304,10 @@ void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
   a.D = c->have_D ? c->D2 : nullptr;
   a.eint = c->oovvT;
   a.t1new = T1old;
+  if (spin_used) {   // spin-forbidden doubles are zero: skipped
+    a.lv = c->d_lab_v;
+    a.lo = c->d_lab_o;
+  }
   nb = update_t2(a, 1, s);
   reduce_partials(c->partials, nb, c->red + 3, s);
 }

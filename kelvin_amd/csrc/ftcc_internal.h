@@ -278,6 +278,10 @@ struct UpdateArgs {
   const double *eint;           // energy integral in amplitude layout
                                 // (F.ov^T for T1, oovv^T for T2)
   double *partials;             // per-block {d, n, e} triples
+  // spin labels of the virtual / occupied lists (device) when the doubles
+  // are known to conserve spin: spin-forbidden elements (zero in Tbar and
+  // T) are then skipped; nullptr: every element
+  const int *lv = nullptr, *lo = nullptr;
 };
 // One grid point y of the integral from the ng slices of a.tbar into the
 // single slice a.told: mode 0 stores it, mode 1 damps into it and leaves

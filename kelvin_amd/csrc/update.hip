@@ -97,6 +97,8 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
   for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
        e += (int64_t)gridDim.x * UT) {
     const Elem q = decode<DOUBLES>(e, nv, no);
+    if (DOUBLES && p.lv && p.lv[q.a] + p.lv[q.b] != p.lo[q.i] + p.lo[q.j])
+      continue;   // zero in Tbar and T: contributes nothing
     double d;
     if (p.D)
       d = p.D[e];
@@ -189,6 +191,8 @@ __global__ __launch_bounds__(UT) void update_rec_kernel(UpdateArgs p,
   for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
        e += (int64_t)gridDim.x * UT) {
     const Elem q = decode<DOUBLES>(e, nv, no);
+    if (DOUBLES && p.lv && p.lv[q.a] + p.lv[q.b] != p.lo[q.i] + p.lo[q.j])
+      continue;   // zero in Tbar and T: contributes nothing
     double d;
     if (p.D)
       d = p.D[e];
@@ -299,6 +303,8 @@ __global__ __launch_bounds__(UT) void point_kernel(UpdateArgs p, int y,
   for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
        e += (int64_t)gridDim.x * UT) {
     const Elem q = decode<DOUBLES>(e, nv, no);
+    if (DOUBLES && p.lv && p.lv[q.a] + p.lv[q.b] != p.lo[q.i] + p.lo[q.j])
+      continue;   // zero in Tbar and T: contributes nothing
     double d;
     if (p.D)
       d = p.D[e];
