@@ -349,6 +349,7 @@ void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false) {
 struct SIdx {
   char l, role;
   int64_t sa, sb, sc;
+  char l2 = 0;   // pair roles: the second letter
 };
 
 // Spin conservation rule of an operand, "mn|ie": the spins of the letters
@@ -385,37 +386,56 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
   std::vector<char> letters;
   int batch = 0;
   int unit = 0;
+  auto is_pair = [](const SIdx &x) { return x.role == 'P' || x.role == 'Q'; };
+  auto add1 = [&](char l) {
+    require(std::find(letters.begin(), letters.end(), l) == letters.end(),
+            "spin-restricted GEMM: repeated index");
+    letters.push_back(l);
+  };
   auto add = [&](const SIdx &x) {
     if (x.role == 'y' || x.role == '1') {
       ++(x.role == 'y' ? batch : unit);
       return;
     }
-    if (std::find(letters.begin(), letters.end(), x.l) == letters.end())
-      letters.push_back(x.l);
+    add1(x.l);
+    if (is_pair(x)) add1(x.l2);
   };
-  for (const SIdx &x : Mi) add(x);
-  for (const SIdx &x : Ni) add(x);
+  int pm = 0, pn = 0;
+  for (const SIdx &x : Mi) {
+    add(x);
+    pm += is_pair(x);
+  }
+  for (const SIdx &x : Ni) {
+    add(x);
+    pn += is_pair(x);
+  }
   for (const SIdx &x : Ki) {
-    require(x.role != 'y' && x.role != '1',
-            "spin-restricted GEMM: slice index in K");
+    require(x.role != 'y' && x.role != '1' && !is_pair(x),
+            "spin-restricted GEMM: slice or pair index in K");
     add(x);
   }
   require(Mi.size() >= 1 && Mi.size() <= 3 && Ni.size() >= 1 &&
               Ni.size() <= 3 && Ki.size() >= 1 && Ki.size() <= 3 &&
-              batch <= 1 &&
-              letters.size() + batch + unit ==
-                  Mi.size() + Ni.size() + Ki.size(),
+              batch <= 1 && pm <= 1 && pn <= 1 && (!(pm + pn) || c->pack),
           "spin-restricted GEMM: bad index lists");
-  auto start = [&](const SIdx &x, int sp) {
-    return (x.role == 'y' || x.role == '1') ? 0
-           : x.role == 'o'                  ? c->o_start[sp]
-                                            : c->v_start[sp];
+  // a pair index runs over the packed pairs of its spin class (x < y for
+  // equal spins, every mixed pair once, none for the beta-alpha class);
+  // its level maps the packed index to the dense pair x * n + y
+  auto pair_space = [&](const SIdx &x) -> const PairSpace & {
+    return x.role == 'P' ? c->ps_oop : c->ps_vvp;
   };
-  auto count = [&](const SIdx &x, int sp) {
-    return x.role == 'y'   ? base.nb1
-           : x.role == '1' ? 1
-           : x.role == 'o' ? c->o_count[sp]
-                           : c->v_count[sp];
+  auto cls = [&](const SIdx &x, std::map<char, int> &sp) {
+    return 2 * sp[x.l] + sp[x.l2];
+  };
+  auto start = [&](const SIdx &x, std::map<char, int> &sp) -> int64_t {
+    if (x.role == 'y' || x.role == '1' || is_pair(x)) return 0;
+    return x.role == 'o' ? c->o_start[sp[x.l]] : c->v_start[sp[x.l]];
+  };
+  auto count = [&](const SIdx &x, std::map<char, int> &sp) -> int64_t {
+    if (x.role == 'y') return base.nb1;
+    if (x.role == '1') return 1;
+    if (is_pair(x)) return pair_space(x).count[cls(x, sp)];
+    return x.role == 'o' ? c->o_count[sp[x.l]] : c->v_count[sp[x.l]];
   };
   // D's stride along an index: none along the slices
   auto sd = [](const SIdx &x) { return x.role == 'y' ? 0 : x.sc; };
@@ -434,29 +454,32 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
     int64_t M = 1, N = 1, K = 1;
     std::vector<int> cm, cn, ck;
     for (const SIdx &x : Mi) {
-      const int s0 = sp[x.l];
-      oa += start(x, s0) * x.sa;
-      oc += start(x, s0) * x.sc;
-      cm.push_back(count(x, s0));
-      M *= count(x, s0);
+      oa += start(x, sp) * x.sa;
+      oc += start(x, sp) * x.sc;
+      cm.push_back((int)count(x, sp));
+      M *= count(x, sp);
     }
     for (const SIdx &x : Ni) {
-      const int s0 = sp[x.l];
-      ob += start(x, s0) * x.sb;
-      oc += start(x, s0) * x.sc;
-      cn.push_back(count(x, s0));
-      N *= count(x, s0);
+      ob += start(x, sp) * x.sb;
+      oc += start(x, sp) * x.sc;
+      cn.push_back((int)count(x, sp));
+      N *= count(x, sp);
     }
     for (const SIdx &x : Ki) {
-      const int s0 = sp[x.l];
-      oa += start(x, s0) * x.sa;
-      ob += start(x, s0) * x.sb;
-      ck.push_back(count(x, s0));
-      K *= count(x, s0);
+      oa += start(x, sp) * x.sa;
+      ob += start(x, sp) * x.sb;
+      ck.push_back((int)count(x, sp));
+      K *= count(x, sp);
     }
     std::vector<int> cb;
-    for (const SIdx &x : Mi) cb.push_back(sp[x.l]);
-    for (const SIdx &x : Ni) cb.push_back(sp[x.l]);
+    for (const SIdx &x : Mi) {
+      cb.push_back(sp[x.l]);
+      if (is_pair(x)) cb.push_back(sp[x.l2]);
+    }
+    for (const SIdx &x : Ni) {
+      cb.push_back(sp[x.l]);
+      if (is_pair(x)) cb.push_back(sp[x.l2]);
+    }
     // several K spin blocks onto one C block: the first applies beta and
     // D, the others accumulate
     const bool first = c_blocks.insert(cb).second;
@@ -476,6 +499,16 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
     g.D = (first && base.D) ? base.D + oc : nullptr;
     // levels: the listed indices fill the innermost slots
     const int nm = (int)Mi.size(), nn = (int)Ni.size(), nk = (int)Ki.size();
+    for (int q = 0; q < nm; ++q)
+      if (is_pair(Mi[q])) {
+        g.mmap = pair_space(Mi[q]).d_map + pair_space(Mi[q]).start[cls(Mi[q], sp)];
+        g.mlvl = 3 - nm + q;
+      }
+    for (int q = 0; q < nn; ++q)
+      if (is_pair(Ni[q])) {
+        g.nmap = pair_space(Ni[q]).d_map + pair_space(Ni[q]).start[cls(Ni[q], sp)];
+        g.nlvl = 3 - nn + q;
+      }
     const SIdx &m0 = nm == 3 ? Mi[0] : z, &m1 = nm >= 2 ? Mi[nm - 2] : z,
                &m2 = Mi[nm - 1];
     const SIdx &n0 = nn == 3 ? Ni[0] : z, &n1 = nn >= 2 ? Ni[nn - 2] : z,
@@ -524,19 +557,25 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
     run(c, g, prec);
   }
   if (writes_all) {
-    // every spin-allowed block of C must have been written
-    const int nc = (int)(Mi.size() + Ni.size()) - batch - unit;
+    // every spin-allowed, nonempty block of C must have been written
+    const int nc = (int)(Mi.size() + Ni.size()) - batch - unit + pm + pn;
     for (int mask = 0; mask < (1 << nc); ++mask) {
       std::map<char, int> sp;
       std::vector<int> cb;
       int k = 0;
+      int64_t cells = 1;
       auto take = [&](const SIdx &x) {
         sp[x.l] = (x.role == 'y' || x.role == '1') ? 0 : (mask >> k++) & 1;
         cb.push_back(sp[x.l]);
+        if (is_pair(x)) {
+          sp[x.l2] = (mask >> k++) & 1;
+          cb.push_back(sp[x.l2]);
+        }
+        cells *= count(x, sp);
       };
       for (const SIdx &x : Mi) take(x);
       for (const SIdx &x : Ni) take(x);
-      if (rule_holds(ruleC, sp))
+      if (rule_holds(ruleC, sp) && cells > 0)
         require(c_blocks.count(cb) == 1,
                 "spin-restricted GEMM leaves an allowed output block unset");
     }
@@ -1283,18 +1322,28 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   product(c, mk((int)OO, (int)OO, (int)VV, 0.5, 0.0, oovv, VV, 1, 0, c->tau,
                 OO, 1, S2, c->wmnij, OO, 1, O4, nb), OP_OOVV_A, OP_TAU_B,
           LADDER);
-  // + oooo + sum_e <mn||ie> t_j^e into [(m,n,i), j]
+  // + oooo + sum_e <mn||ie> t_j^e into [(m,n,i), j]; with packed pairs
+  // only the stored (m,n) of each couple (all the G2 operand reads)
   if (s5) {
-    run_spin(c, sgem(1.0, 1.0, ooov, 0, t1, VO, c->wmnij, O4, oooo), P5,
-             {{'m', 'o', OO * V, 0, O * OO}, {'n', 'o', O * V, 0, OO},
-              {'i', 'o', V, 0, O}},
+    const std::vector<SIdx> mn =
+        c->pack ? std::vector<SIdx>{{'m', 'P', O * V, 0, OO, 'n'}}
+                : std::vector<SIdx>{{'m', 'o', OO * V, 0, O * OO},
+                                    {'n', 'o', O * V, 0, OO}};
+    std::vector<SIdx> Mi = mn;
+    Mi.push_back({'i', 'o', V, 0, O});
+    run_spin(c, sgem(1.0, 1.0, ooov, 0, t1, VO, c->wmnij, O4, oooo), P5, Mi,
              {{'y', 'y', 0, VO, O4}, {'j', 'o', 0, 1, 1}}, {'e', 'v', 1, O, 0},
              "mn|ie", "e|j", "mn|ij");
     // - sum_e t_i^e <mn||je>: C[i][(m,n,j)] -= t1^T[i,e] ooov[m,n,j,e]
+    std::vector<SIdx> Ni;
+    for (SIdx x : mn) {   // the same (m,n) as columns: strides of B and C
+      x.sb = x.sa;
+      x.sa = 0;
+      Ni.push_back(x);
+    }
+    Ni.push_back({'j', 'o', 0, V, 1});
     run_spin(c, sgem(-1.0, 1.0, t1, VO, ooov, 0, c->wmnij, O4, nullptr), P5,
-             {{'y', 'y', VO, 0, O4}, {'i', 'o', 1, 0, O}},
-             {{'m', 'o', 0, OO * V, O * OO}, {'n', 'o', 0, O * V, OO},
-              {'j', 'o', 0, V, 1}},
+             {{'y', 'y', VO, 0, O4}, {'i', 'o', 1, 0, O}}, Ni,
              {'e', 'v', O, 1, 0}, "e|i", "mn|je", "mn|ij");
   } else {
     Gemm g = mk((int)(OO * O), O, V, 1.0, 1.0, ooov, V, 1, 0, t1, O, 1, VO,
@@ -1313,19 +1362,23 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
 
-  // Wabef = vvvv - P(ab) t_m^b <am||ef> (its tau.oovv share is in Wmnij)
+  // Wabef = vvvv - P(ab) t_m^b <am||ef> (its tau.oovv share is in Wmnij);
+  // with packed pairs only the stored (e,f) of each couple
   if (s5) {
+    const std::vector<SIdx> ef =
+        c->pack ? std::vector<SIdx>{{'e', 'Q', 0, 1, 1, 'f'}}
+                : std::vector<SIdx>{{'e', 'v', 0, V, V}, {'f', 'v', 0, 1, 1}};
     // vvvv - sum_m t_m^b <am||ef>: C[b][(a,e,f)] = vvvv - t1[b,m] vovv[a,m,ef]
+    std::vector<SIdx> Ni{{'a', 'v', 0, O * VV, V * VV}};
+    Ni.insert(Ni.end(), ef.begin(), ef.end());
     run_spin(c, sgem(-1.0, 0.0, t1, VO, vovv, 0, c->wabef, V4, vvvv), P5,
-             {{'y', 'y', VO, 0, V4}, {'b', 'v', O, 0, VV}},
-             {{'a', 'v', 0, O * VV, V * VV}, {'e', 'v', 0, V, V},
-              {'f', 'v', 0, 1, 1}},
+             {{'y', 'y', VO, 0, V4}, {'b', 'v', O, 0, VV}}, Ni,
              {'m', 'o', 1, VV, 0}, "b|m", "am|ef", "ab|ef");
     // + sum_m t_m^a <bm||ef>: C[a][(b,e,f)]
+    Ni = {{'b', 'v', 0, O * VV, VV}};
+    Ni.insert(Ni.end(), ef.begin(), ef.end());
     run_spin(c, sgem(1.0, 1.0, t1, VO, vovv, 0, c->wabef, V4, nullptr), P5,
-             {{'y', 'y', VO, 0, V4}, {'a', 'v', O, 0, V * VV}},
-             {{'b', 'v', 0, O * VV, VV}, {'e', 'v', 0, V, V},
-              {'f', 'v', 0, 1, 1}},
+             {{'y', 'y', VO, 0, V4}, {'a', 'v', O, 0, V * VV}}, Ni,
              {'m', 'o', 1, VV, 0}, "a|m", "bm|ef", "ab|ef");
   } else {
     // vvvv - sum_m t_m^b <am||ef>: per a, C[b][ef] = vvvv - t1[b,m] vovv[a,m,ef]

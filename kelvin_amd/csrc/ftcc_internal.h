@@ -68,6 +68,10 @@ struct Gemm {
   // sBk1 with sAk, sBk innermost; K2 = 0: one index); needs A stored with
   // K fastest and B with K fastest ([N][K])
   int K1 = 1, K2 = 0;
+  // optional map of one row (column) level: index x of level mlvl (nlvl,
+  // 0..2 as above) is replaced by mmap[x] (nmap[x]) before the strides
+  const int *mmap = nullptr, *nmap = nullptr;
+  int mlvl = -1, nlvl = -1;
   int64_t sAk0 = 0, sAk1 = 0, sBk0 = 0, sBk1 = 0;
 };
 

@@ -70,11 +70,21 @@ struct GemmDev {
   int64_t sBn0, sBn1, sCn0, sCn1, sDn0, sDn1;
   int K1, K2;                // K levels (kernels instantiated with KL)
   int64_t sAk0, sAk1, sBk0, sBk1;
+  // optional map of one row level and one column level (level 0, 1 or 2;
+  // -1: none): that level's index x becomes map[x] before the strides
+  const int *mmap, *nmap;
+  int mlvl, nlvl;
 };
 
 struct Lvl {
   int x0, x1, x2;
 };
+__device__ inline Lvl map_level(Lvl l, const int *map, int lvl) {
+  if (lvl == 0) l.x0 = map[l.x0];
+  else if (lvl == 1) l.x1 = map[l.x1];
+  else if (lvl == 2) l.x2 = map[l.x2];
+  return l;
+}
 __device__ inline Lvl levels(int x, int n1, int n2) {
   Lvl l;
   l.x2 = x % n2;
@@ -135,11 +145,15 @@ __device__ inline void tile_offsets(const GemmDev &p, int m0, int n0,
   const int t = threadIdx.x;
   if (t < TM) {
     const int m = m0 + t;
-    aoff[t] = m < p.M ? at(levels(m, p.M1, p.M2), p.sAm0, p.sAm1, p.sAm) : -1;
+    aoff[t] = m < p.M ? at(map_level(levels(m, p.M1, p.M2), p.mmap, p.mlvl),
+                           p.sAm0, p.sAm1, p.sAm)
+                      : -1;
   } else if (t < TM + TN) {
     const int n = n0 + t - TM;
     boff[t - TM] =
-        n < p.N ? at(levels(n, p.N1, p.N2), p.sBn0, p.sBn1, p.sBn) : -1;
+        n < p.N ? at(map_level(levels(n, p.N1, p.N2), p.nmap, p.nlvl),
+                     p.sBn0, p.sBn1, p.sBn)
+                : -1;
   }
 }
 
@@ -307,13 +321,15 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
     if (t < TM) {
       const int row = m0 + t;
       const int rr = row < p.M ? (p.rmap ? p.rmap[row] : row) : -1;
-      const Lvl l = levels(rr < 0 ? 0 : rr, p.M1, p.M2);
+      const Lvl l =
+          map_level(levels(rr < 0 ? 0 : rr, p.M1, p.M2), p.mmap, p.mlvl);
       crow[t] = rr < 0 ? -1 : at(l, p.sCm0, p.sCm1, p.sCm);
       drow[t] = rr < 0 ? -1 : at(l, p.sDm0, p.sDm1, p.sDm);
     } else if (t < TM + TN) {
       const int col = n0 + t - TM;
       const int cc = col < p.N ? (p.cmap ? p.cmap[col] : col) : -1;
-      const Lvl l = levels(cc < 0 ? 0 : cc, p.N1, p.N2);
+      const Lvl l =
+          map_level(levels(cc < 0 ? 0 : cc, p.N1, p.N2), p.nmap, p.nlvl);
       ccol[t - TM] = cc < 0 ? -1 : at(l, p.sCn0, p.sCn1, p.sCn);
       dcol[t - TM] = cc < 0 ? -1 : at(l, p.sDn0, p.sDn1, p.sDn);
     }
@@ -418,7 +434,8 @@ __global__ __launch_bounds__(64 * RG) void reduce_partial_kernel(GemmDev p,
           b1 = cb / p.nb2;
           b2 = cb % p.nb2;
         }
-        const Lvl lr = levels(m, p.M1, p.M2), lc = levels(n, p.N1, p.N2);
+        const Lvl lr = map_level(levels(m, p.M1, p.M2), p.mmap, p.mlvl),
+                  lc = map_level(levels(n, p.N1, p.N2), p.nmap, p.nlvl);
         double *c = p.C + b1 * p.sC1 + b2 * p.sC2 +
                     at(lr, p.sCm0, p.sCm1, p.sCm) +
                     at(lc, p.sCn0, p.sCn1, p.sCn);
@@ -574,6 +591,13 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   d.K2 = g.K2 > 0 ? g.K2 : (1 << 30);
   d.sAk0 = g.sAk0;
   d.sAk1 = g.sAk1;
+  require((g.mlvl < 0 || (g.mmap && g.mlvl <= 2)) &&
+              (g.nlvl < 0 || (g.nmap && g.nlvl <= 2)),
+          "GEMM level map without a map or level");
+  d.mmap = g.mmap;
+  d.nmap = g.nmap;
+  d.mlvl = g.mmap ? g.mlvl : -1;
+  d.nlvl = g.nmap ? g.nlvl : -1;
   d.sBk0 = g.sBk0;
   d.sBk1 = g.sBk1;
   if (R > 1) {
