@@ -258,7 +258,7 @@ def main():
         "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
         "traffic": measured_traffic(args.precision, n, ng,
                                     "%s-c%d" % (layout, eng.chunk())),
-        "kernel": "%s (the six O(n^6) products of the residual)"
+        "kernel": "%s (the five O(n^6) products of the residual)"
                   % ("sbgemm_kernel" if args.precision == "bf16x3"
                      else "gemm_mfma_kernel<%s>" % ("double" if args.precision == "fp64" else "float")),
         "launch_flops": launch_flops, "avg_launch_ms": round(avg_ms, 4),
