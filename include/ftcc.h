@@ -74,12 +74,12 @@ int ftcc_set_system(ftcc_ctx *ctx, int n, const double *en_dev,
 int ftcc_set_blocks(ftcc_ctx *ctx, const double *const F_dev[4],
                     const double *const I_dev[9]);
 
-/* Spin blocking of the six O(n^6) products (no counterpart in the
- * reference, whose einsums are dense; the results are the same).  Labels
- * (0 or 1, host arrays) of the occupied-role and virtual-role orbitals: if
- * every Fock and integral block is exactly zero wherever spin is not
- * conserved, each product runs as its three nonzero spin blocks (10/64 of
- * the dense work); otherwise FTCC_EINVAL and the products stay dense.  Both
+/* Spin blocking of the O(n^6) products (no counterpart in the reference,
+ * whose einsums are dense; the results are the same).  Labels (0 or 1,
+ * host arrays) of the occupied-role and virtual-role orbitals: if every
+ * Fock and integral block is exactly zero wherever spin is not conserved,
+ * each product runs as its three nonzero spin blocks (10/64 of the dense
+ * work); otherwise FTCC_EINVAL and the products stay dense.  Both
  * NULL switches blocking off.  ftcc_set_system tries the labels "first
  * half alpha, second half beta" of the n orbitals by itself (environment
  * FTCC_SPIN=0 prevents it); ftcc_set_blocks switches blocking off.
@@ -91,9 +91,10 @@ int ftcc_set_spin(ftcc_ctx *ctx, const int *lab_o_host,
 /* *on = 0 when everything runs dense; otherwise bit flags: 1 the O(n^6)
  * products run spin-blocked; 2 the O(n^5) contractions run over the
  * spin-allowed blocks only (labels form one run of 0s then one of 1s in the
- * occupied and in the virtual list); 4 the ladder products sum over packed
- * antisymmetric pairs (e < f within a spin, alpha-beta once: the amplitudes
- * must be antisymmetric, as every CC iterate is).  FTCC_SPIN5=0 and
+ * occupied and in the virtual list); 4 the ladder products run over packed
+ * antisymmetric pairs in rows, columns and K (e < f within a spin,
+ * alpha-beta once: the amplitudes must be antisymmetric, as every CC
+ * iterate is).  FTCC_SPIN5=0 and
  * FTCC_PACK=0 in the environment at ftcc_set_system/ftcc_set_spin time
  * switch off flags 2 and 4. */
 int ftcc_get_spin(ftcc_ctx *ctx, int *on);
