@@ -1487,26 +1487,36 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
 
   // S1 (antisymmetrised in ab): Fbe[b,e] t2[a,e,i,j] + t1[a,m] vooo[b,m,i,j]
   // S2 (antisymmetrised in ij): -t2[a,b,i,m] Fmj[m,j] + t1[e,i] vvvo[a,b,e,j]
+  // S1 is antisymmetric in (i,j) and S2 in (a,b): with packed pairs each is
+  // formed for the stored pair of that couple only
   if (s5) {
+    const std::vector<SIdx> ij =
+        c->pack ? std::vector<SIdx>{{'i', 'P', 0, 1, 1, 'j'}}
+                : std::vector<SIdx>{{'i', 'o', 0, O, O}, {'j', 'o', 0, 1, 1}};
+    std::vector<SIdx> Ni{{'a', 'v', 0, V * OO, V * OO}};
+    Ni.insert(Ni.end(), ij.begin(), ij.end());
     run_spin(c, sgem(1.0, 0.0, c->Fbe, VV, t2, S2, c->s1, S2, nullptr), P5,
-             {{'b', 'v', V, 0, OO}},
-             {{'a', 'v', 0, V * OO, V * OO}, {'i', 'o', 0, O, O},
-              {'j', 'o', 0, 1, 1}},
-             {'e', 'v', 1, OO, 0}, "b|e", "ae|ij", "ab|ij");
+             {{'b', 'v', V, 0, OO}}, Ni, {'e', 'v', 1, OO, 0}, "b|e",
+             "ae|ij", "ab|ij");
+    Ni = {{'b', 'v', 0, O * OO, OO}};
+    Ni.insert(Ni.end(), ij.begin(), ij.end());
     run_spin(c, sgem(1.0, 1.0, t1, VO, vooo, 0, c->s1, S2, nullptr), P5,
-             {{'y', 'y', VO, 0, S2}, {'a', 'v', O, 0, V * OO}},
-             {{'b', 'v', 0, O * OO, OO}, {'i', 'o', 0, O, O},
-              {'j', 'o', 0, 1, 1}},
+             {{'y', 'y', VO, 0, S2}, {'a', 'v', O, 0, V * OO}}, Ni,
              {'m', 'o', 1, OO, 0}, "a|m", "bm|ij", "ab|ij");
+    std::vector<SIdx> Mi =
+        c->pack ? std::vector<SIdx>{{'a', 'Q', OO, 0, OO, 'b'}}
+                : std::vector<SIdx>{{'a', 'v', V * OO, 0, V * OO},
+                                    {'b', 'v', OO, 0, OO}};
+    Mi.push_back({'i', 'o', O, 0, O});
     run_spin(c, sgem(-1.0, 0.0, t2, S2, c->Fmj, OO, c->s2, S2, nullptr), P5,
-             {{'a', 'v', V * OO, 0, V * OO}, {'b', 'v', OO, 0, OO},
-              {'i', 'o', O, 0, O}},
-             {{'j', 'o', 0, 1, 1}}, {'m', 'o', 1, O, 0}, "ab|im", "m|j",
+             Mi, {{'j', 'o', 0, 1, 1}}, {'m', 'o', 1, O, 0}, "ab|im", "m|j",
              "ab|ij");
+    Ni = c->pack ? std::vector<SIdx>{{'a', 'Q', 0, VO, OO, 'b'}}
+                 : std::vector<SIdx>{{'a', 'v', 0, V * VO, V * OO},
+                                     {'b', 'v', 0, VO, OO}};
+    Ni.push_back({'j', 'o', 0, 1, 1});
     run_spin(c, sgem(1.0, 1.0, t1, VO, vvvo, 0, c->s2, S2, nullptr), P5,
-             {{'y', 'y', VO, 0, S2}, {'i', 'o', 1, 0, O}},
-             {{'a', 'v', 0, V * VO, V * OO}, {'b', 'v', 0, VO, OO},
-              {'j', 'o', 0, 1, 1}},
+             {{'y', 'y', VO, 0, S2}, {'i', 'o', 1, 0, O}}, Ni,
              {'e', 'v', O, O, 0}, "e|i", "ab|ej", "ab|ij");
   } else {
     Gemm g = mk(V, (int)OO, V, 1.0, 0.0, c->Fbe, V, 1, VV, t2, OO, 1, S2,

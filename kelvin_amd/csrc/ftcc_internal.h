@@ -233,9 +233,10 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
 //                    + PP(Z ring) ) with acc1/acc2 one per grid point and
 // add one slice for every grid point (or nullptr); with spin labels
 // (device arrays, or nullptr), spin-forbidden elements are written as zero
-// without reading the operands.  packed: acc1/acc2 hold one pair of each
-// antisymmetric (a,b) and (i,j) couple only (x < y for equal spins, the
-// alpha index first for mixed spins); needs the labels.
+// without reading the operands.  packed (needs the labels): every input
+// holds one pair of each antisymmetric couple (x < y for equal spins, the
+// alpha index first for mixed spins): acc1/acc2 in (a,b) and (i,j), S1 in
+// (i,j), S2 in (a,b); add must be antisymmetric in (a,b).
 void r2_final(double *t2bar, const double *acc1, const double *acc2,
               const double *S1, const double *S2, const double *Z, int nv,
               int no, int nb, hipStream_t s, const int *lab_v,

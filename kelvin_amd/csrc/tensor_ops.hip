@@ -273,20 +273,16 @@ __global__ void ring_assemble_kernel(double *Wr, const double *ovvo_ring,
 // and (b,a): their S1 and Z terms are the same four and two reads with
 // opposite signs.  The operands read transposed (S2[a,b,j,i], S2[b,a,j,i]
 // and the difference Z[b,j,a,i] - Z[a,j,b,i]) are loaded along i into LDS
-// so every global access is coalesced (four 32x33 tiles: four workgroups
-// per CU).  With spin labels (lv, lo), elements whose spins do
-// not conserve (lv[a] + lv[b] != lo[i] + lo[j]) are written as zero and
-// their operands are not read: every operand element an allowed output
-// reads is itself allowed.
-// packed: acc1/acc2 hold only one pair of each antisymmetric couple in
-// (a,b) and in (i,j) -- x < y for equal spins, the alpha index first for
-// mixed spins -- and every other element follows by antisymmetry; the
-// stored element of a transposed (i,j) is read along i through LDS.
+// so every global access is coalesced (three 32x33 tiles).  With spin
+// labels (lv, lo), elements whose spins do not conserve
+// (lv[a] + lv[b] != lo[i] + lo[j]) are written as zero and their operands
+// are not read: every operand element an allowed output reads is itself
+// allowed.
 __global__ __launch_bounds__(256) void r2_final_pair_kernel(
     double *t2bar, const double *acc1, const double *acc2, const double *S1,
     const double *S2, const double *Z, const double *add, int nv, int no,
-    const int *lv, const int *lo, int packed) {
-  __shared__ double tA[32][33], tB[32][33], tZ[32][33], tP[32][33];
+    const int *lv, const int *lo) {
+  __shared__ double tA[32][33], tB[32][33], tZ[32][33];
   const int tiles = (no + 31) / 32;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   int64_t bid = blockIdx.x;
@@ -314,32 +310,12 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
   auto ring = [&](int p, int k, int r, int l) {
     return (((int64_t)p * no + k) * nv + r) * no + l;
   };
-  // packed: the stored (a,b) pair and its sign; a == b of equal spins is
-  // zero (never stored)
-  int pa = a, pb = b;
-  double sg = 1.0;
-  const bool zab = packed && a == b;
-  if (packed && lv[a] != lv[b] && lv[a] == 1) {
-    pa = b;
-    pb = a;
-    sg = -1.0;
-  }
-  // packed (i,j): 1 = stored as (i,j), -1 = stored as (j,i), 0 = zero
-  auto ij_sign = [&](int i, int j) {
-    if (lo[i] == lo[j]) return i < j ? 1 : (i > j ? -1 : 0);
-    return lo[i] == 0 ? 1 : -1;
-  };
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + tx, j = j0 + r;
     const bool ok = i < no && j < no && (!lv || lo[i] + lo[j] == sab);
     tA[r][tx] = ok ? s2[nat(a, b, j, i)] : 0.0;
     tB[r][tx] = ok ? s2[nat(b, a, j, i)] : 0.0;
     tZ[r][tx] = ok ? z[ring(b, j, a, i)] - z[ring(a, j, b, i)] : 0.0;
-    if (packed) {
-      const bool tr = ok && !zab && ij_sign(i, j) < 0;
-      const int64_t e = nat(pa, pb, j, i);
-      tP[r][tx] = tr ? c1[e] + c2[e] : 0.0;
-    }
   }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
@@ -351,34 +327,110 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
       out[nba] = 0.0;
       continue;
     }
-    double accab, accba;
-    if (packed) {
-      double v = 0.0;
-      if (!zab) {
-        const int sij = ij_sign(i, j);
-        if (sij > 0) {
-          const int64_t e = nat(pa, pb, i, j);
-          v = c1[e] + c2[e];
-        } else if (sij < 0) {
-          v = -tP[tx][r];
-        }
-      }
-      accab = sg * v;
-      accba = -accab;
-    } else {
-      accab = c1[nab] + c2[nab];
-      accba = c1[nba] + c2[nba];
-    }
     const double s1ab = s1[nab], s1ba = s1[nba];
     const double zz = z[ring(a, i, b, j)] - z[ring(b, i, a, j)] + tZ[tx][r];
-    double vab = accab + s1ab - s1ba + s2[nab] - tA[tx][r] + zz;
-    double vba = accba + s1ba - s1ab + s2[nba] - tB[tx][r] - zz;
+    double vab = c1[nab] + c2[nab] + s1ab - s1ba + s2[nab] - tA[tx][r] + zz;
+    double vba = c1[nba] + c2[nba] + s1ba - s1ab + s2[nba] - tB[tx][r] - zz;
     if (add) {
       vab += add[nab];
       vba += add[nba];
     }
     out[nab] = -vab;
     if (a != b) out[nba] = -vba;
+  }
+}
+
+// The same assembly in the packed spin layout, where every input holds
+// one pair of each antisymmetric couple (x < y for equal spins, the alpha
+// index first for mixed spins; the other follows by antisymmetry):
+//   acc1, acc2  the stored (a,b) and the stored (i,j)
+//   S1          every (a,b), the stored (i,j)   (S1 is antisymmetric in ij)
+//   S2          the stored (a,b), every (i,j)   (S2 is antisymmetric in ab)
+// and add (<ab||ij>) is antisymmetric, so that R2[b,a,i,j] = -R2[a,b,i,j]
+// is written without further reads.  Transposed reads (the stored element
+// of a transposed (i,j), S2[.,.,j,i], the Z difference) go through four
+// 32x33 LDS tiles.
+__global__ __launch_bounds__(256) void r2_packed_kernel(
+    double *t2bar, const double *acc1, const double *acc2, const double *S1,
+    const double *S2, const double *Z, const double *add, int nv, int no,
+    const int *lv, const int *lo) {
+  __shared__ double tS2[32][33], tS1[32][33], tZ[32][33], tP[32][33];
+  const int tiles = (no + 31) / 32;
+  const int64_t npair = (int64_t)nv * (nv + 1) / 2;
+  int64_t bid = blockIdx.x;
+  const int tj = (int)(bid % tiles);
+  bid /= tiles;
+  const int ti = (int)(bid % tiles);
+  bid /= tiles;
+  const int64_t q = bid % npair;
+  const int64_t y = bid / npair;
+  int b = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while ((int64_t)b * (b + 1) / 2 > q) --b;
+  while ((int64_t)(b + 1) * (b + 2) / 2 <= q) ++b;
+  const int a = (int)(q - (int64_t)b * (b + 1) / 2);
+  const int64_t per = (int64_t)nv * nv * no * no;
+  const double *s1 = S1 + y * per, *s2 = S2 + y * per, *z = Z + y * per;
+  const double *c1 = acc1 + y * per, *c2 = acc2 + y * per;
+  double *out = t2bar + y * per;
+  const int i0 = ti * 32, j0 = tj * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int sab = lv[a] + lv[b];
+  auto nat = [&](int p, int r, int k, int l) {
+    return (((int64_t)p * nv + r) * no + k) * no + l;
+  };
+  auto ring = [&](int p, int k, int r, int l) {
+    return (((int64_t)p * no + k) * nv + r) * no + l;
+  };
+  // the stored (a,b) pair and its sign; a == b is zero throughout
+  int pa = a, pb = b;
+  double sg = 1.0;
+  const bool zab = a == b;
+  if (lv[a] != lv[b] && lv[a] == 1) {
+    pa = b;
+    pb = a;
+    sg = -1.0;
+  }
+  // (i,j): 1 = stored as (i,j), -1 = stored as (j,i), 0 = zero
+  auto ij_sign = [&](int i, int j) {
+    if (lo[i] == lo[j]) return i < j ? 1 : (i > j ? -1 : 0);
+    return lo[i] == 0 ? 1 : -1;
+  };
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    const bool ok = !zab && i < no && j < no && lo[i] + lo[j] == sab;
+    const bool tr = ok && ij_sign(i, j) < 0;
+    const int64_t e = nat(pa, pb, j, i);
+    tS2[r][tx] = ok ? s2[e] : 0.0;
+    tS1[r][tx] = tr ? s1[nat(a, b, j, i)] - s1[nat(b, a, j, i)] : 0.0;
+    tZ[r][tx] = ok ? z[ring(b, j, a, i)] - z[ring(a, j, b, i)] : 0.0;
+    tP[r][tx] = tr ? c1[e] + c2[e] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + r, j = j0 + tx;
+    if (i >= no || j >= no) continue;
+    const int64_t nab = nat(a, b, i, j), nba = nat(b, a, i, j);
+    if (zab || lo[i] + lo[j] != sab) {
+      out[nab] = 0.0;
+      out[nba] = 0.0;
+      continue;
+    }
+    const int sij = ij_sign(i, j);
+    const int64_t e = nat(pa, pb, i, j);
+    double acc = 0.0, s1d = 0.0;
+    if (sij > 0) {
+      acc = c1[e] + c2[e];
+      s1d = s1[nab] - s1[nba];
+    } else if (sij < 0) {
+      acc = -tP[tx][r];
+      s1d = -tS1[tx][r];
+    }
+    const double s2d = s2[e] - tS2[tx][r];
+    const double zz = z[ring(a, i, b, j)] - z[ring(b, i, a, j)] + tZ[tx][r];
+    double v = sg * (acc + s2d) + s1d + zz;
+    if (add) v += add[nab];
+    out[nab] = -v;
+    out[nba] = v;
   }
 }
 
@@ -527,9 +579,14 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   const int64_t blocks = (int64_t)nb * npair * tiles * tiles;
   require(blocks < (1LL << 31), "r2 assembly grid too large");
-  hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks), dim3(256),
-                     0, s, t2bar, acc1, acc2, S1, S2, Z, add, nv, no, lab_v,
-                     lab_o, packed ? 1 : 0);
+  if (packed)
+    hipLaunchKernelGGL(r2_packed_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       s, t2bar, acc1, acc2, S1, S2, Z, add, nv, no, lab_v,
+                       lab_o);
+  else
+    hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks),
+                       dim3(256), 0, s, t2bar, acc1, acc2, S1, S2, Z, add, nv,
+                       no, lab_v, lab_o);
   FTCC_CHECK_HIP(hipGetLastError());
 }
 
