@@ -347,19 +347,23 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
 //   S1          every (a,b), the stored (i,j)   (S1 is antisymmetric in ij)
 //   S2          the stored (a,b), every (i,j)   (S2 is antisymmetric in ab)
 // and add (<ab||ij>) is antisymmetric, so that R2[b,a,i,j] = -R2[a,b,i,j]
-// is written without further reads.  Transposed reads (the stored element
-// of a transposed (i,j), S2[.,.,j,i], the Z difference) go through four
-// 32x33 LDS tiles.
+// is written without further reads.  One workgroup per (a <= b) pair and
+// R2P_ROWS rows i, all j: direct reads and writes run along whole rows of
+// j; the four values read transposed (the stored element of a transposed
+// (i,j), S2[.,.,j,i], the S1 and Z differences at (j,i)) are first staged
+// in LDS as [j][i] strips (row stride R2P_LD doubles, conflict-free reads
+// along j).
+constexpr int R2P_ROWS = 8, R2P_LD = 9;
 __global__ __launch_bounds__(256) void r2_packed_kernel(
     double *t2bar, const double *acc1, const double *acc2, const double *S1,
     const double *S2, const double *Z, const double *add, int nv, int no,
     const int *lv, const int *lo) {
-  __shared__ double tS2[32][33], tS1[32][33], tZ[32][33], tP[32][33];
-  const int tiles = (no + 31) / 32;
+  extern __shared__ double sm[];   // 4 strips of no * R2P_LD
+  double *tS2 = sm, *tS1 = sm + no * R2P_LD, *tZ = sm + 2 * no * R2P_LD,
+         *tP = sm + 3 * no * R2P_LD;
+  const int tiles = (no + R2P_ROWS - 1) / R2P_ROWS;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   int64_t bid = blockIdx.x;
-  const int tj = (int)(bid % tiles);
-  bid /= tiles;
   const int ti = (int)(bid % tiles);
   bid /= tiles;
   const int64_t q = bid % npair;
@@ -372,8 +376,7 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
   const double *s1 = S1 + y * per, *s2 = S2 + y * per, *z = Z + y * per;
   const double *c1 = acc1 + y * per, *c2 = acc2 + y * per;
   double *out = t2bar + y * per;
-  const int i0 = ti * 32, j0 = tj * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int i0 = ti * R2P_ROWS;
   const int sab = lv[a] + lv[b];
   auto nat = [&](int p, int r, int k, int l) {
     return (((int64_t)p * nv + r) * no + k) * no + l;
@@ -395,42 +398,47 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
     if (lo[i] == lo[j]) return i < j ? 1 : (i > j ? -1 : 0);
     return lo[i] == 0 ? 1 : -1;
   };
-  for (int r = ty; r < 32; r += 8) {
-    const int i = i0 + tx, j = j0 + r;
-    const bool ok = !zab && i < no && j < no && lo[i] + lo[j] == sab;
+  // stage the values at (j, i) for the block's rows i, contiguous along i
+  for (int t = threadIdx.x; t < no * R2P_ROWS; t += blockDim.x) {
+    const int j = t / R2P_ROWS, il = t % R2P_ROWS, i = i0 + il;
+    const bool ok = !zab && i < no && lo[i] + lo[j] == sab;
     const bool tr = ok && ij_sign(i, j) < 0;
     const int64_t e = nat(pa, pb, j, i);
-    tS2[r][tx] = ok ? s2[e] : 0.0;
-    tS1[r][tx] = tr ? s1[nat(a, b, j, i)] - s1[nat(b, a, j, i)] : 0.0;
-    tZ[r][tx] = ok ? z[ring(b, j, a, i)] - z[ring(a, j, b, i)] : 0.0;
-    tP[r][tx] = tr ? c1[e] + c2[e] : 0.0;
+    tS2[j * R2P_LD + il] = ok ? s2[e] : 0.0;
+    tS1[j * R2P_LD + il] = tr ? s1[nat(a, b, j, i)] - s1[nat(b, a, j, i)] : 0.0;
+    tZ[j * R2P_LD + il] = ok ? z[ring(b, j, a, i)] - z[ring(a, j, b, i)] : 0.0;
+    tP[j * R2P_LD + il] = tr ? c1[e] + c2[e] : 0.0;
   }
   __syncthreads();
-  for (int r = ty; r < 32; r += 8) {
-    const int i = i0 + r, j = j0 + tx;
-    if (i >= no || j >= no) continue;
-    const int64_t nab = nat(a, b, i, j), nba = nat(b, a, i, j);
-    if (zab || lo[i] + lo[j] != sab) {
-      out[nab] = 0.0;
-      out[nba] = 0.0;
-      continue;
+  const int lanes = blockDim.x / 2;   // two rows at a time
+  for (int il = threadIdx.x / lanes; il < R2P_ROWS; il += 2) {
+    const int i = i0 + il;
+    if (i >= no) break;
+    for (int j = threadIdx.x % lanes; j < no; j += lanes) {
+      const int64_t nab = nat(a, b, i, j), nba = nat(b, a, i, j);
+      if (zab || lo[i] + lo[j] != sab) {
+        out[nab] = 0.0;
+        out[nba] = 0.0;
+        continue;
+      }
+      const int sij = ij_sign(i, j);
+      const int64_t e = nat(pa, pb, i, j);
+      double acc = 0.0, s1d = 0.0;
+      if (sij > 0) {
+        acc = c1[e] + c2[e];
+        s1d = s1[nab] - s1[nba];
+      } else if (sij < 0) {
+        acc = -tP[j * R2P_LD + il];
+        s1d = -tS1[j * R2P_LD + il];
+      }
+      const double s2d = s2[e] - tS2[j * R2P_LD + il];
+      const double zz =
+          z[ring(a, i, b, j)] - z[ring(b, i, a, j)] + tZ[j * R2P_LD + il];
+      double v = sg * (acc + s2d) + s1d + zz;
+      if (add) v += add[nab];
+      out[nab] = -v;
+      out[nba] = v;
     }
-    const int sij = ij_sign(i, j);
-    const int64_t e = nat(pa, pb, i, j);
-    double acc = 0.0, s1d = 0.0;
-    if (sij > 0) {
-      acc = c1[e] + c2[e];
-      s1d = s1[nab] - s1[nba];
-    } else if (sij < 0) {
-      acc = -tP[tx][r];
-      s1d = -tS1[tx][r];
-    }
-    const double s2d = s2[e] - tS2[tx][r];
-    const double zz = z[ring(a, i, b, j)] - z[ring(b, i, a, j)] + tZ[tx][r];
-    double v = sg * (acc + s2d) + s1d + zz;
-    if (add) v += add[nab];
-    out[nab] = -v;
-    out[nba] = v;
   }
 }
 
@@ -579,11 +587,23 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   const int64_t blocks = (int64_t)nb * npair * tiles * tiles;
   require(blocks < (1LL << 31), "r2 assembly grid too large");
-  if (packed)
-    hipLaunchKernelGGL(r2_packed_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                       s, t2bar, acc1, acc2, S1, S2, Z, add, nv, no, lab_v,
-                       lab_o);
-  else
+  if (packed) {
+    const int64_t pblocks =
+        (int64_t)nb * npair * ((no + R2P_ROWS - 1) / R2P_ROWS);
+    require(pblocks < (1LL << 31), "r2 assembly grid too large");
+    const size_t lds = sizeof(double) * 4 * (size_t)no * R2P_LD;
+    require(lds <= 160 * 1024, "r2 assembly: too many occupied orbitals");
+    static bool attr = false;
+    if (!attr) {
+      FTCC_CHECK_HIP(hipFuncSetAttribute(
+          (const void *)r2_packed_kernel,
+          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+      attr = true;
+    }
+    hipLaunchKernelGGL(r2_packed_kernel, dim3((unsigned)pblocks), dim3(256),
+                       lds, s, t2bar, acc1, acc2, S1, S2, Z, add, nv, no,
+                       lab_v, lab_o);
+  } else
     hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks),
                        dim3(256), 0, s, t2bar, acc1, acc2, S1, S2, Z, add, nv,
                        no, lab_v, lab_o);
