@@ -230,6 +230,7 @@ __device__ inline void split1(double v, __bf16 &hi, __bf16 &lo) {
 
 // Destination writers of the gather kernels: split into (hi, lo) bf16, or
 // plain FP64 (the spin-blocked operands of the FP64 and FP32 modes).
+// store4 writes four consecutive elements (o a multiple of 4).
 struct SplitOut {
   __bf16 *hi, *lo;
   __device__ void operator()(int64_t o, double v) const {
@@ -238,10 +239,28 @@ struct SplitOut {
     hi[o] = h;
     lo[o] = l;
   }
+  __device__ void store4(int64_t o, const double v[4]) const {
+    typedef __bf16 b4_t __attribute__((ext_vector_type(4)));
+    b4_t h, l;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      __bf16 hk, lk;
+      split1(v[k], hk, lk);
+      h[k] = hk;
+      l[k] = lk;
+    }
+    *(b4_t *)(hi + o) = h;
+    *(b4_t *)(lo + o) = l;
+  }
 };
 struct F64Out {
   double *p;
   __device__ void operator()(int64_t o, double v) const { p[o] = v; }
+  __device__ void store4(int64_t o, const double v[4]) const {
+    typedef double d2_t __attribute__((ext_vector_type(2)));
+    *(d2_t *)(p + o) = d2_t{v[0], v[1]};
+    *(d2_t *)(p + o + 2) = d2_t{v[2], v[3]};
+  }
 };
 
 // source row / column of destination index i: map[i] (-1: zero) or i itself
@@ -299,19 +318,27 @@ struct AmpSrc {
 // dst[b][r][c] (row stride os) = src(b, R(r), C(c)) for r < R, c < W, zero
 // where a map gives -1 (or, without maps, for c >= C); threads run along c
 // (use when the source is contiguous along columns).
+// Each thread takes four consecutive columns (W, os and db multiples of 4,
+// destinations aligned to four elements).
 template <typename Src, typename Out>
 __global__ void gather_rows_kernel(Src src, const int *rmap, int R,
                                    const int *cmap, int C, int W, int64_t os,
                                    int64_t db, Out out, int nb) {
-  const int64_t per = (int64_t)R * W;
+  const int W4 = W / 4;
+  const int64_t per = (int64_t)R * W4;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
        idx < per * nb; idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = idx / per, e = idx % per;
-    const int r = (int)(e / W);
-    const int c = (int)(e % W);
-    const int rr = src_index(rmap, r, R), cc = src_index(cmap, c, C);
-    const double v = (rr >= 0 && cc >= 0) ? src(b, rr, cc) : 0.0;
-    out(b * db + (int64_t)r * os + c, v);
+    const int r = (int)(e / W4);
+    const int c = 4 * (int)(e % W4);
+    const int rr = src_index(rmap, r, R);
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int cc = src_index(cmap, c + k, C);
+      v[k] = (rr >= 0 && cc >= 0) ? src(b, rr, cc) : 0.0;
+    }
+    out.store4(b * db + (int64_t)r * os + c, v);
   }
 }
 
@@ -332,9 +359,16 @@ __global__ void gather_transpose_kernel(Src src, const int *rmap, int R,
     tile[i][tx] = (rr >= 0 && cc >= 0) ? src(b, rr, cc) : 0.0;
   }
   __syncthreads();
-  for (int i = ty; i < 64; i += 4) {
-    const int r = r0 + i, c = c0 + tx;
-    if (r < R && c < W) out(b * db + (int64_t)r * os + c, tile[tx][i]);
+  // four consecutive columns per thread: 16 quads per row, 16 rows a pass
+  const int cq = threadIdx.x & 15, rq = threadIdx.x >> 4;
+  for (int i = rq; i < 64; i += 16) {
+    const int r = r0 + i, c = c0 + 4 * cq;
+    if (r < R && c < W) {
+      double v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[k] = tile[4 * cq + k][i];
+      out.store4(b * db + (int64_t)r * os + c, v);
+    }
   }
 }
 
@@ -343,6 +377,8 @@ void gather_launch(Src src, bool along_rows, const int *rmap, int R,
                    const int *cmap, int C, int W, int64_t os, int64_t db,
                    Out out, int nb, hipStream_t s) {
   if ((int64_t)R * W == 0 || nb <= 0) return;
+  require(W % 4 == 0 && os % 4 == 0 && db % 4 == 0,
+          "gather: widths and strides must be multiples of 4");
   if (along_rows) {
     const int gx = (W + 63) / 64, gy = (R + 63) / 64;
     require(gy < 65536 && nb < 65536, "gather grid too large");
@@ -350,7 +386,7 @@ void gather_launch(Src src, bool along_rows, const int *rmap, int R,
                        dim3(256), 0, s, src, rmap, R, cmap, C, W, os, db,
                        out);
   } else {
-    const int64_t total = (int64_t)R * W * nb;
+    const int64_t total = (int64_t)R * (W / 4) * nb;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
     hipLaunchKernelGGL((gather_rows_kernel<Src, Out>), dim3(blocks),
                        dim3(256), 0, s, src, rmap, R, cmap, C, W, os, db, out,
