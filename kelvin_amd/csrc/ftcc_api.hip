@@ -1073,9 +1073,10 @@ void ensure_integrals(ftcc_ctx *c) {
 }
 
 // Residual workspace: as many grid points per chunk as fit the budget
-// (FTCC_WORKSPACE_GB, default 120 GiB: four grid points at n = 114, the
-// whole grid at n <= 66) -- larger chunks measured faster at every size
-// tried (fewer, fuller launches), 2.8% at n = 114 from two to four.
+// (FTCC_WORKSPACE_GB, default 120 GiB: eight grid points at n = 114 in
+// bf16x3, six in FP64, the whole grid at n <= 66) -- larger chunks
+// measured faster at every size tried (fewer, fuller launches; integrals
+// read once per chunk).
 void ensure_workspace(ftcc_ctx *c) {
   if (c->aring) return;
   const int O = c->no, V = c->nv, ng = c->ng;
@@ -1083,13 +1084,18 @@ void ensure_workspace(ftcc_ctx *c) {
   const int64_t S2 = VV * OO;
   const int64_t sm = std::max<int64_t>(VV, std::max<int64_t>(VO, OO));
   // five n^4 intermediates (seven in the dense layout, which also forms tau
-  // and t2r), Wmnij/Wabef/U, the seven per-slot product operands split into
-  // bf16 pairs (bf16x3) and as FP64 spin-blocked copies (FP64/FP32 modes;
-  // both exist after a precision switch)
+  // and t2r), Wmnij/Wabef/U, and the seven per-slot product operands of the
+  // current precision: split into bf16 pairs (bf16x3) or as FP64
+  // spin-blocked copies (FP64/FP32 spin layout).  A precision switch
+  // releases the workspace, so only one of the two ever exists.
+  const int64_t operands =
+      c->prec == FTCC_PREC_BF16X3
+          ? 4 * 7 *
+                (S2 + 64 * std::max<int64_t>(VV, std::max<int64_t>(VO, OO)))
+          : (c->spin ? 8 * 7 * S2 : 0);
   const int64_t per_slot =
       8 * ((c->spin ? 5 : 7) * S2 + OO * OO + VV * VV + OO * VO + 8 * sm) +
-      4 * 7 * (S2 + 64 * std::max<int64_t>(VV, std::max<int64_t>(VO, OO))) +
-      8 * 7 * S2;
+      operands;
   double budget_gb = 120.0;
   if (const char *env = getenv("FTCC_WORKSPACE_GB")) budget_gb = atof(env);
   const int64_t budget = (int64_t)(budget_gb * (1LL << 30));
@@ -1788,6 +1794,10 @@ int ftcc_set_precision(ftcc_ctx *c, int precision) {
   return guarded(c, [&] {
     drop_graph(c);   // buffers or kernels of the captured step change
     require(precision >= 0 && precision <= 2, "unknown precision");
+    if (precision == c->prec) return;
+    // the product operands and the chunk size depend on the precision:
+    // release the workspace (rebuilt by the next residual)
+    free_workspace(c);
     c->prec = precision;
   });
 }
@@ -2367,7 +2377,25 @@ void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
   a.D = c->have_D ? c->D2 : nullptr;
   a.eint = c->oovvT;
   a.t1new = T1old;
-  if (spin_used) {   // spin-forbidden doubles are zero: skipped
+  if (spin_used && c->spin5) {
+    // label runs: only the spin-allowed doubles are enumerated (the others
+    // are zero in Tbar and T)
+    int64_t n = 0;
+    for (int sp = 0; sp < 2; ++sp) {
+      a.vrun[2 * sp] = c->v_start[sp];
+      a.vrun[2 * sp + 1] = c->v_count[sp];
+      a.orun[2 * sp] = c->o_start[sp];
+      a.orun[2 * sp + 1] = c->o_count[sp];
+    }
+    for (int box = 0; box < 16; ++box) {
+      const int sa = box >> 3, sb = (box >> 2) & 1, si = (box >> 1) & 1,
+                sj = box & 1;
+      if (sa + sb == si + sj)
+        n += (int64_t)c->v_count[sa] * c->v_count[sb] * c->o_count[si] *
+             c->o_count[sj];
+    }
+    a.nallowed = n;
+  } else if (spin_used) {   // spin-forbidden doubles are zero: skipped
     a.lv = c->d_lab_v;
     a.lo = c->d_lab_o;
   }

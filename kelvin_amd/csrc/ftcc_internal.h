@@ -287,6 +287,11 @@ struct UpdateArgs {
   // are known to conserve spin: spin-forbidden elements (zero in Tbar and
   // T) are then skipped; nullptr: every element
   const int *lv = nullptr, *lo = nullptr;
+  // the same when each label list is one run of 0s then one of 1s: {start,
+  // count} of the alpha and beta runs of the virtual and occupied lists and
+  // the number of spin-allowed doubles (> 0: only those are enumerated)
+  int vrun[4] = {0, 0, 0, 0}, orun[4] = {0, 0, 0, 0};
+  int64_t nallowed = 0;
 };
 // One grid point y of the integral from the ng slices of a.tbar into the
 // single slice a.told: mode 0 stores it, mode 1 damps into it and leaves

@@ -174,6 +174,32 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
 // written and earlier slices are carried in registers.
 constexpr int RCH = 8;
 
+// Allowed doubles of the spin layout when the labels are runs (alpha then
+// beta in each list): the six (sa, sb, si, sj) boxes with sa + sb = si + sj,
+// each [a range] x [b range] x [i range] x [j range], enumerated with j
+// innermost.  Returns false past the last element.
+__device__ inline bool run_elem(const UpdateArgs &p, int64_t k, Elem &q) {
+  for (int box = 0; box < 16; ++box) {
+    const int sa = box >> 3, sb = (box >> 2) & 1, si = (box >> 1) & 1,
+              sj = box & 1;
+    if (sa + sb != si + sj) continue;
+    const int64_t na = p.vrun[2 * sa + 1], nb = p.vrun[2 * sb + 1],
+                  ni = p.orun[2 * si + 1], nj = p.orun[2 * sj + 1];
+    const int64_t size = na * nb * ni * nj;
+    if (k < size) {
+      q.j = p.orun[2 * sj] + (int)(k % nj);
+      k /= nj;
+      q.i = p.orun[2 * si] + (int)(k % ni);
+      k /= ni;
+      q.b = p.vrun[2 * sb] + (int)(k % nb);
+      q.a = p.vrun[2 * sa] + (int)(k / nb);
+      return true;
+    }
+    k -= size;
+  }
+  return false;
+}
+
 template <bool DOUBLES>
 __global__ __launch_bounds__(UT) void update_rec_kernel(UpdateArgs p,
                                                         int mode) {
@@ -188,9 +214,19 @@ __global__ __launch_bounds__(UT) void update_rec_kernel(UpdateArgs p,
   const int64_t per1 = (int64_t)nv * no;
   const double *src = p.src ? p.src : p.tbar;
   double dsum = 0.0, nsum = 0.0, esum = 0.0;
-  for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
-       e += (int64_t)gridDim.x * UT) {
-    const Elem q = decode<DOUBLES>(e, nv, no);
+  // with label runs only the allowed doubles are enumerated
+  const bool runs = DOUBLES && p.nallowed > 0;
+  const int64_t count = runs ? p.nallowed : per;
+  for (int64_t k = blockIdx.x * (int64_t)UT + threadIdx.x; k < count;
+       k += (int64_t)gridDim.x * UT) {
+    Elem q;
+    int64_t e = k;
+    if (runs) {
+      if (!run_elem(p, k, q)) break;
+      e = (((int64_t)q.a * nv + q.b) * no + q.i) * no + q.j;
+    } else {
+      q = decode<DOUBLES>(e, nv, no);
+    }
     if (DOUBLES && p.lv && p.lv[q.a] + p.lv[q.b] != p.lo[q.i] + p.lo[q.j])
       continue;   // zero in Tbar and T: contributes nothing
     double d;
@@ -250,7 +286,9 @@ __global__ __launch_bounds__(UT) void update_rec_kernel(UpdateArgs p,
 
 template <bool DOUBLES>
 int launch_update(const UpdateArgs &a, int mode, hipStream_t s) {
-  const int64_t per = DOUBLES ? (int64_t)a.nv * a.nv * a.no * a.no
+  const int64_t per = DOUBLES ? (a.rec && a.nallowed > 0
+                                     ? a.nallowed
+                                     : (int64_t)a.nv * a.nv * a.no * a.no)
                               : (int64_t)a.nv * a.no;
   const int blocks =
       (int)std::max<int64_t>(1, std::min<int64_t>((per + UT - 1) / UT, MAX_BLOCKS));
