@@ -41,9 +41,13 @@ constexpr int BK = 16, NT = 256;
 struct Tile {
   int m, n;
 };
-inline Tile pick_tile(int M, int N) {
+// Short K (the O(n^5) contractions over one index): the narrow,
+// single-buffered tiles (three workgroups per CU) stream the output faster
+// than 128x128 (3.5% on those launches at n = 114).
+inline Tile pick_tile(int M, int N, int K = 1 << 30) {
   if (M <= 64 && N > 64) return {64, 128};
   if (N <= 64 && M > 64) return {128, 64};
+  if (K <= 64) return M >= N ? Tile{128, 64} : Tile{64, 128};
   return {128, 128};
 }
 
@@ -493,7 +497,7 @@ void launch_layout(Tile tl, bool amf, bool bnf, bool kl, const GemmDev &d,
 }  // namespace
 
 size_t gemm_partial_elems(const Gemm &g) {
-  const Tile tl = pick_tile(g.M, g.N);
+  const Tile tl = pick_tile(g.M, g.N, g.K);
   const int tiles = ((g.M + tl.m - 1) / tl.m) * ((g.N + tl.n - 1) / tl.n);
   const int bk = BK;  // conservative: smallest slab gives the most splits
   const int splits = choose_splits(g, tiles, bk);
@@ -532,7 +536,7 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   d.sCn = g.sCn;
   d.sC1 = g.sC1;
   d.sC2 = g.sC2;
-  const Tile tl = pick_tile(g.M, g.N);
+  const Tile tl = pick_tile(g.M, g.N, g.K);
   d.tilesM = (g.M + tl.m - 1) / tl.m;
   d.tilesN = (g.N + tl.n - 1) / tl.n;
   if (g.K <= 0) {
