@@ -351,9 +351,11 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
 // R2P_ROWS rows i, all j: direct reads and writes run along whole rows of
 // j; the four values read transposed (the stored element of a transposed
 // (i,j), S2[.,.,j,i], the S1 and Z differences at (j,i)) are first staged
-// in LDS as [j][i] strips (row stride R2P_LD doubles, conflict-free reads
-// along j).
-constexpr int R2P_ROWS = 8, R2P_LD = 9;
+// in LDS as [j][i] strips (row stride R2P_LD doubles: two-way bank
+// conflicts at most on the reads along j).  Four rows per workgroup
+// measured fastest at n = 114 (2, 4, 8, 16 rows: 38.6, 28.5, 33.7, 71.0 ms
+// per iteration).
+constexpr int R2P_ROWS = 4, R2P_LD = 5;
 __global__ __launch_bounds__(256) void r2_packed_kernel(
     double *t2bar, const double *acc1, const double *acc2, const double *S1,
     const double *S2, const double *Z, const double *add, int nv, int no,
