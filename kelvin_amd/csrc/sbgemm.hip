@@ -342,26 +342,28 @@ __global__ void gather_rows_kernel(Src src, const int *rmap, int R,
   }
 }
 
-// Same map through a 64x64 LDS tile: reads run along r (the source is
-// contiguous along rows), writes along c.
+// Same map through a GT x GT LDS tile: reads run along r (the source is
+// contiguous along rows), writes along c, four columns per thread.  32x32
+// tiles measured faster than 64x64 at n = 114 (24.2 against 30.0 ms of
+// gathers per iteration: more workgroups in flight).
+constexpr int GT = 32;
 template <typename Src, typename Out>
 __global__ void gather_transpose_kernel(Src src, const int *rmap, int R,
                                         const int *cmap, int C, int W,
                                         int64_t os, int64_t db, Out out) {
-  __shared__ double tile[64][65];
+  __shared__ double tile[GT][GT + 1];
   const int b = blockIdx.z;
-  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 256 threads
-  for (int i = ty; i < 64; i += 4) {
+  const int r0 = blockIdx.y * GT, c0 = blockIdx.x * GT;
+  const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;   // 256 threads
+  for (int i = ty; i < GT; i += 256 / GT) {
     const int r = r0 + tx, c = c0 + i;
     const int rr = r < R ? src_index(rmap, r, R) : -1;
     const int cc = c < W ? src_index(cmap, c, C) : -1;
     tile[i][tx] = (rr >= 0 && cc >= 0) ? src(b, rr, cc) : 0.0;
   }
   __syncthreads();
-  // four consecutive columns per thread: 16 quads per row, 16 rows a pass
-  const int cq = threadIdx.x & 15, rq = threadIdx.x >> 4;
-  for (int i = rq; i < 64; i += 16) {
+  const int cq = threadIdx.x % (GT / 4), rq = threadIdx.x / (GT / 4);
+  for (int i = rq; i < GT; i += 1024 / GT) {
     const int r = r0 + i, c = c0 + 4 * cq;
     if (r < R && c < W) {
       double v[4];
@@ -380,7 +382,7 @@ void gather_launch(Src src, bool along_rows, const int *rmap, int R,
   require(W % 4 == 0 && os % 4 == 0 && db % 4 == 0,
           "gather: widths and strides must be multiples of 4");
   if (along_rows) {
-    const int gx = (W + 63) / 64, gy = (R + 63) / 64;
+    const int gx = (W + GT - 1) / GT, gy = (R + GT - 1) / GT;
     require(gy < 65536 && nb < 65536, "gather grid too large");
     hipLaunchKernelGGL((gather_transpose_kernel<Src, Out>), dim3(gx, gy, nb),
                        dim3(256), 0, s, src, rmap, R, cmap, C, W, os, db,
