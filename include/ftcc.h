@@ -50,6 +50,9 @@ int ftcc_create(ftcc_ctx **ctx, int no, int nv, int ng, int precision,
                 void *stream);
 int ftcc_destroy(ftcc_ctx *ctx);
 const char *ftcc_last_error(const ftcc_ctx *ctx);
+/* Contraction precision of the O(n^6) products (FTCC_PREC_*).  A change
+ * releases the residual workspace (its product operands and chunk size
+ * depend on the precision); the next residual rebuilds it. */
 int ftcc_set_precision(ftcc_ctx *ctx, int precision);
 
 /* Fermi-weighted integrals (kelvin/cc_utils.py:569-602 ft_integrals and
@@ -211,8 +214,9 @@ int ftcc_td_implicit_step(ftcc_ctx *ctx, int scheme, double h, int max_iter,
 
 /* Grid points per residual chunk (the slots whose O(n^6) products go out as
  * one launch): nb = 0 lets the context choose from its workspace budget
- * (FTCC_WORKSPACE_GB, read when the workspace is allocated) -- the
- * default; nb > 0 fixes it, at most ng and what that budget allows.
+ * (FTCC_WORKSPACE_GB, default 120 GiB, read when the workspace is
+ * allocated; it counts the product operands of the current precision) --
+ * the default; nb > 0 fixes it, at most ng and what that budget allows.
  * Changing it releases the residual workspace.  ftcc_get_chunk reports the
  * value in use (allocating the workspace).  No reference counterpart: the
  * reference evaluates the grid points one at a time. */
