@@ -97,9 +97,13 @@ int ftcc_set_spin(ftcc_ctx *ctx, const int *lab_o_host,
  * occupied and in the virtual list); 4 the ladder products run over packed
  * antisymmetric pairs in rows, columns and K (e < f within a spin,
  * alpha-beta once: the amplitudes must be antisymmetric, as every CC
- * iterate is).  FTCC_SPIN5=0 and
- * FTCC_PACK=0 in the environment at ftcc_set_system/ftcc_set_spin time
- * switch off flags 2 and 4. */
+ * iterate is); 8 the system is symmetric under the spin flip (equal alpha
+ * and beta runs; every integral, Fock block and orbital energy unchanged
+ * when each index moves to its partner of the other spin): for trusted
+ * amplitudes (ftcc_set_trusted) the two ring products compute the output
+ * rows with an alpha virtual index and copy the others from their flipped
+ * images.  FTCC_SPIN5=0, FTCC_PACK=0 and FTCC_FLIP=0 in the environment at
+ * ftcc_set_system/ftcc_set_spin time switch off flags 2, 4 and 8. */
 int ftcc_get_spin(ftcc_ctx *ctx, int *on);
 
 /* Uniform-electron-gas integrals built on the device (the provider of
@@ -224,9 +228,11 @@ int ftcc_set_chunk(ftcc_ctx *ctx, int nb);
 int ftcc_get_chunk(ftcc_ctx *ctx, int *nb);
 
 /* on != 0: the caller promises that the amplitudes it passes conserve spin
- * (true of every iterate of the solver started from the driving terms), so
- * residual calls skip the counting pass over T and need no host
- * synchronisation.  Results are wrong if the promise is broken. */
+ * and, for a spin-flip symmetric system (ftcc_get_spin flag 8), are
+ * unchanged under the spin flip (both true of every iterate of the solver
+ * started from the driving terms), so residual calls skip the counting
+ * pass over T, need no host synchronisation and use the flip symmetry.
+ * Results are wrong if the promise is broken. */
 int ftcc_set_trusted(ftcc_ctx *ctx, int on);
 
 /* Quadrature weights g_host[ng] for ftcc_step_graph. */

@@ -110,6 +110,14 @@ struct ftcc_ctx {
   int spin5 = 0;
   int o_start[2] = {0, 0}, o_count[2] = {0, 0};
   int v_start[2] = {0, 0}, v_count[2] = {0, 0};
+  // Spin-flip symmetry (flip = 1): equal alpha and beta runs and every
+  // integral, Fock block and orbital energy unchanged when each index moves
+  // to its partner of the other spin.  Trusted amplitudes (the solver's own
+  // iterates) then share the symmetry, and the ring products compute the
+  // output rows with an alpha virtual index only (flip_now: in force for
+  // the residual being evaluated).
+  int flip = 0;
+  bool flip_now = false;
   PairSpace ps_vv, ps_oo, ps_vo;   // (a,b), (i,j) and ring (a,i) pairs
   // Ladder products contract over an antisymmetric pair, so they sum over
   // the packed pairs only (e < f within a spin, alpha-beta once) and double
@@ -736,6 +744,13 @@ void operand_blocks(const ftcc_ctx *c, int id, Range rows[3], Range ks[3]) {
     return;
   }
   rows[0] = group(R, 0, 3), ks[0] = group(K, 0, 3);
+  if (id == OP_T2R_A && c->flip_now) {
+    // the spin-flip symmetric G5 reads the rows with an alpha virtual only
+    rows[0] = group(R, 0);
+    rows[1] = group(R, 1), ks[1] = group(K, 2);
+    rows[2] = Range(), ks[2] = Range();
+    return;
+  }
   if (id == OP_WR_B) {
     rows[1] = group(R, 1), ks[1] = group(K, 1);
     rows[2] = group(R, 2), ks[2] = group(K, 2);
@@ -899,6 +914,18 @@ void block_list(int kind, const PairSpace &R, const PairSpace &K,
   }
 }
 
+// The rows of a ring product output (ring layout [e][m][b][j], slices of
+// g.sC1) whose virtual index is beta, from their spin-flipped images.
+// G4's Wr[e,m,b,j] conserves se + sj = sm + sb, G5's Z[a,i,b,j] sa + sb =
+// si + sj.
+void flip_rows(ftcc_ctx *c, const Gemm &g, int kind) {
+  const int O = c->no, V = c->nv;
+  require(g.sCm == (int64_t)V * O && g.sCn == 1,
+          "flip copy: ring output layout expected");
+  flip_copy_ring(g.C, V, O, c->v_count[0], c->o_count[0], g.nb1, g.sC1,
+                 kind == RING_G5, c->stream);
+}
+
 // One O(n^6) product C = alpha A.B + beta C over g.nb1 slots, g describing
 // it in the dense layout and ia/ib naming its operands: dense, or as the
 // three nonzero spin blocks written straight into C through the pair maps.
@@ -912,6 +939,14 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
   const PairSpace &Cs = space(c, op_rows(c, ib));
   Blk blk[3];
   block_list(kind, R, K, Cs, blk);
+  // spin-flip symmetry: the ring products compute the rows whose virtual
+  // index is alpha (classes 00 and 01 of the ring pairs); the others are
+  // copied from their flipped images below
+  const bool flip = c->flip_now && kind != LADDER;
+  if (flip) {
+    blk[0].r = group(R, 0);
+    blk[2].r = Range();
+  }
   // a packed ladder K sums one pair of each antisymmetric couple: twice
   const double alpha = (kind == LADDER && op_k(c, ia) >= 3) ? 2.0 * g.alpha
                                                             : g.alpha;
@@ -945,11 +980,12 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     std::sort(pb, pb + np, [](const SbProblem &x, const SbProblem &y) {
       return (double)x.M * x.N * x.K > (double)y.M * y.N * y.K;
     });
-    if (np == 0) return;
-    timed(c, true, flops, [&] {
-      sbgemm_grouped(np, pb, g.nb1, alpha, g.beta, g.C, g.sCm, g.sCn,
-                     g.sC1, c->stream);
-    });
+    if (np > 0)
+      timed(c, true, flops, [&] {
+        sbgemm_grouped(np, pb, g.nb1, alpha, g.beta, g.C, g.sCm, g.sCn,
+                       g.sC1, c->stream);
+      });
+    if (flip) flip_rows(c, g, kind);
     return;
   }
   for (const Blk &b : blk) {
@@ -967,6 +1003,7 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     ensure_partial(c, gb);
     timed(c, true, flops, [&] { gemm_run(gb, P, c->scr, c->stream); });
   }
+  if (flip) flip_rows(c, g, kind);
 }
 
 void read_scalars(ftcc_ctx *c, int n);
@@ -1042,6 +1079,34 @@ bool try_spin(ftcc_ctx *c, const std::vector<int> &lo,
               runs(lv, c->v_start, c->v_count))
                  ? 1
                  : 0;
+  c->flip = 0;
+  const char *fenv = getenv("FTCC_FLIP");
+  if (c->spin5 && !(fenv && atoi(fenv) == 0) && c->have_energies &&
+      !c->have_D && c->o_count[0] == c->o_count[1] &&
+      c->v_count[0] == c->v_count[1]) {
+    const int ho = c->o_count[0], hv = c->v_count[0];
+    auto half = [&](char x) { return x == 'o' ? ho : hv; };
+    double fbad = 0.0;
+    auto check = [&](const double *a, int d0, int d1, int d2, int d3, int h0,
+                     int h1, int h2, int h3) {
+      const int nbk =
+          flip_violations(a, d0, d1, d2, d3, h0, h1, h2, h3, c->partials, s);
+      reduce_partials(c->partials, nbk, c->red, s);
+      read_scalars(c, 3);
+      fbad += c->host_red[2];
+    };
+    check(c->eo, O, 1, 1, 1, ho, 0, 0, 0);
+    check(c->ev, V, 1, 1, 1, hv, 0, 0, 0);
+    for (int k = 0; k < 4 && fbad == 0.0; ++k)
+      check(c->F[k], dim(fn[k][0]), dim(fn[k][1]), 1, 1, half(fn[k][0]),
+            half(fn[k][1]), 0, 0);
+    for (int k = 0; k < 9 && fbad == 0.0; ++k) {
+      const char *b = in[k];
+      check(c->I[k], dim(b[0]), dim(b[1]), dim(b[2]), dim(b[3]), half(b[0]),
+            half(b[1]), half(b[2]), half(b[3]));
+    }
+    c->flip = fbad == 0.0 ? 1 : 0;
+  }
   return true;
 }
 
@@ -1672,10 +1737,15 @@ bool residual_all(ftcc_ctx *c, const double *T1, const double *T2,
   const int spin_saved = c->spin;
   if (c->spin && !c->trusted && !amplitudes_conserve_spin(c, T1, T2, c->ng))
     c->spin = 0;
+  // the flip symmetry of the amplitudes is part of the trusted promise
+  c->flip_now = c->flip && c->spin && c->trusted;
   struct Restore {
     ftcc_ctx *c;
     int v;
-    ~Restore() { c->spin = v; }
+    ~Restore() {
+      c->spin = v;
+      c->flip_now = false;
+    }
   } restore{c, spin_saved};
   const int O = c->no, V = c->nv;
   const int64_t VO = (int64_t)V * O, S2 = (int64_t)V * V * O * O;
@@ -1933,7 +2003,9 @@ int ftcc_set_spin(ftcc_ctx *c, const int *lab_o, const int *lab_v) {
 int ftcc_get_spin(ftcc_ctx *c, int *on) {
   return guarded(c, [&] {
     require(on != nullptr, "null output pointer");
-    *on = c->spin ? (1 | (c->spin5 ? 2 : 0) | (c->pack ? 4 : 0)) : 0;
+    *on = c->spin ? (1 | (c->spin5 ? 2 : 0) | (c->pack ? 4 : 0) |
+                     (c->flip ? 8 : 0))
+                  : 0;
   });
 }
 
@@ -1950,6 +2022,7 @@ int ftcc_set_denominators(ftcc_ctx *c, const double *D1, const double *D2) {
     FTCC_CHECK_HIP(hipMemcpyAsync(c->D2, D2, 8 * S2, hipMemcpyDeviceToDevice,
                                   c->stream));
     c->have_D = true;
+    c->flip = 0;   // the symmetry check covered the orbital energies only
     FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
   });
 }

@@ -241,6 +241,13 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
               const double *S1, const double *S2, const double *Z, int nv,
               int no, int nb, hipStream_t s, const int *lab_v,
               const int *lab_o, const double *add, bool packed);
+// Spin-flip images of a ring tensor X[nb][nv][no][nv][no] (slice stride
+// sb) whose rows with a virtual alpha index are computed: every allowed
+// element with that index beta is copied from its flipped partner (alpha
+// runs [0, hv) and [0, ho), beta runs of the same lengths after them).
+// vv_pair: X[a,i,b,j] conserves sa + sb = si + sj (else se + sj = sm + sb).
+void flip_copy_ring(double *X, int nv, int no, int hv, int ho, int nb,
+                    int64_t sb, bool vv_pair, hipStream_t s);
 // x[n][c] (nb batches of n*c) -> out[c][n] transposed copy for small mats
 void transpose2(const double *in, double *out, int r, int c, int nb,
                 hipStream_t s);
@@ -337,5 +344,10 @@ int vsum(const double *x, int64_t n, double *partials, hipStream_t s);
 int spin_violations(const double *a, int d0, int d1, int d2, int d3, int nb,
                     const int *l0, const int *l1, const int *l2, const int *l3,
                     double *partials, hipStream_t s);
+// count (third slot) of the entries of a (d0,d1,d2,d3) block that differ
+// from their image under the spin flip: index x of dimension k maps to
+// x + h_k below h_k and x - h_k from h_k on (h_k = 0: not flipped)
+int flip_violations(const double *a, int d0, int d1, int d2, int d3, int h0,
+                    int h1, int h2, int h3, double *partials, hipStream_t s);
 
 }  // namespace ftcc

@@ -444,6 +444,36 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
   }
 }
 
+// Ring tensor X[y][p][k][q][l] (virtual p, q; occupied k, l) of a
+// spin-flip symmetric problem whose rows with p alpha are computed: the
+// allowed elements with p beta are their flipped images
+// X[y][p - hv][fl(k)][fl(q)][fl(l)] (fl: alpha <-> beta partner). Allowed
+// means sk + sq == sp + sl for a ring intermediate W[e,m,b,j] (vv_pair = 0)
+// and sp + sq == sk + sl for an amplitude-like Z[a,i,b,j] (vv_pair = 1).
+__global__ void flip_copy_ring_kernel(double *X, int nv, int no, int hv,
+                                      int ho, int nb, int64_t sb,
+                                      int vv_pair) {
+  const int64_t per = (int64_t)(nv - hv) * no * nv * no;
+  GRID_STRIDE(idx, per * nb) {
+    const int64_t y = idx / per;
+    int64_t r = idx % per;
+    const int l = (int)(r % no);
+    r /= no;
+    const int q = (int)(r % nv);
+    r /= nv;
+    const int k = (int)(r % no);
+    const int p = hv + (int)(r / no);
+    const int sk = k >= ho, sq = q >= hv, sl = l >= ho;
+    // spin-forbidden (p is beta)
+    if (vv_pair ? 1 + sq != sk + sl : sk + sq != 1 + sl) continue;
+    const int fk = sk ? k - ho : k + ho, fq = sq ? q - hv : q + hv,
+              fl = sl ? l - ho : l + ho;
+    double *x = X + y * sb;
+    x[(((int64_t)p * no + k) * nv + q) * no + l] =
+        x[(((int64_t)(p - hv) * no + fk) * nv + fq) * no + fl];
+  }
+}
+
 __global__ void transpose2_kernel(const double *in, double *out, int r, int c,
                                   int nb) {
   const int64_t per = (int64_t)r * c;
@@ -610,6 +640,13 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
                        dim3(256), 0, s, t2bar, acc1, acc2, S1, S2, Z, add, nv,
                        no, lab_v, lab_o);
   FTCC_CHECK_HIP(hipGetLastError());
+}
+
+void flip_copy_ring(double *X, int nv, int no, int hv, int ho, int nb,
+                    int64_t sb, bool vv_pair, hipStream_t s) {
+  require(2 * hv == nv && 2 * ho == no, "flip copy needs equal spin halves");
+  LAUNCH(flip_copy_ring_kernel, (int64_t)(nv - hv) * no * nv * no * nb, X, nv,
+         no, hv, ho, nb, sb, vv_pair ? 1 : 0);
 }
 
 void transpose2(const double *in, double *out, int r, int c, int nb,

@@ -482,6 +482,31 @@ __global__ __launch_bounds__(UT) void spin_violations_kernel(
   block_reduce3(0.0, 0.0, cnt, partials);
 }
 
+// Elements that differ from their image under the spin flip (index x of a
+// dimension with half h maps to x + h below h, x - h above; h = 0: fixed).
+__global__ __launch_bounds__(UT) void flip_violations_kernel(
+    const double *a, int d0, int d1, int d2, int d3, int h0, int h1, int h2,
+    int h3, double *partials) {
+  const int64_t per = (int64_t)d0 * d1 * d2 * d3;
+  double cnt = 0.0;
+  auto fl = [](int x, int h) { return h ? (x < h ? x + h : x - h) : x; };
+  for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
+       e += (int64_t)gridDim.x * UT) {
+    int64_t r = e;
+    const int i3 = (int)(r % d3);
+    r /= d3;
+    const int i2 = (int)(r % d2);
+    r /= d2;
+    const int i1 = (int)(r % d1);
+    const int i0 = (int)(r / d1);
+    const int64_t f =
+        (((int64_t)fl(i0, h0) * d1 + fl(i1, h1)) * d2 + fl(i2, h2)) * d3 +
+        fl(i3, h3);
+    if (a[e] != a[f]) cnt += 1.0;
+  }
+  block_reduce3(0.0, 0.0, cnt, partials);
+}
+
 int nblocks_for(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + UT - 1) / UT, MAX_BLOCKS));
 }
@@ -678,6 +703,15 @@ int spin_violations(const double *a, int d0, int d1, int d2, int d3, int nb,
   const int blocks = nblocks_for((int64_t)d0 * d1 * d2 * d3 * nb);
   hipLaunchKernelGGL(spin_violations_kernel, dim3(blocks), dim3(UT), 0, s, a,
                      d0, d1, d2, d3, nb, l0, l1, l2, l3, partials);
+  FTCC_CHECK_HIP(hipGetLastError());
+  return blocks;
+}
+
+int flip_violations(const double *a, int d0, int d1, int d2, int d3, int h0,
+                    int h1, int h2, int h3, double *partials, hipStream_t s) {
+  const int blocks = nblocks_for((int64_t)d0 * d1 * d2 * d3);
+  hipLaunchKernelGGL(flip_violations_kernel, dim3(blocks), dim3(UT), 0, s, a,
+                     d0, d1, d2, d3, h0, h1, h2, h3, partials);
   FTCC_CHECK_HIP(hipGetLastError());
   return blocks;
 }

@@ -169,7 +169,8 @@ class FTEngine:
     def spin_mode(self):
         """0: dense; else bit flags: 1 O(n^6) products as spin blocks, 2 the
         O(n^5) contractions over spin-allowed blocks only, 4 ladder products
-        over packed antisymmetric pairs."""
+        over packed antisymmetric pairs, 8 a spin-flip symmetric system (the
+        ring products of trusted amplitudes compute half their rows)."""
         on = ctypes.c_int()
         self._call("ftcc_get_spin", ctypes.byref(on))
         return on.value

@@ -870,6 +870,48 @@ def test_tdccsd_implicit_matches_rk4(tprop, ng_td, ng_rk4):
     assert abs(Eccout - Eccref) < 1e-4
 
 
+# ------------------------------------------------------------ spin flip
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_spin_flip_residual_equals_full(precision):
+    """A spin-unpolarised UEG (19 k-points, n = 38) is symmetric under the
+    spin flip: with trusted amplitudes the ring products compute the rows of
+    an alpha virtual index only and copy the rest.  The residual equals the
+    one of untrusted amplitudes (every row computed) to rounding, for
+    amplitudes from the driving terms and after two damped steps."""
+    from kelvin_amd import quadrature
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd.ueg_system import ueg_from_config
+    dev()
+    sysg = ueg_from_config(19, 2.0, N=14, t_over_tf=0.3)
+    en = sysg.g_energies_tot()
+    n, ng = en.shape[0], 6
+    beta = 1.0 / sysg.T
+    ti, g, G = quadrature.ft_quad(ng, beta, 'lin')
+    eng = FTEngine(n, n, ng, precision=precision)
+    fock, eri, _ = sysg.g_device_integrals(eng)
+    eng.set_system(en, fock, eri, beta, sysg.mu)
+    del fock, eri
+    eng.set_grid(ti, G)
+    assert eng.spin_mode() == 15
+    T1, T2 = eng.empty_amplitudes()
+    T1b, T2b = eng.empty_amplitudes()
+    eng.driving_terms(T1, T2)
+    eng.integrate(T1, T2)
+    for it in range(3):
+        out = []
+        for trusted in (False, True):
+            eng.set_trusted(trusted)
+            R1, R2 = eng.empty_amplitudes()
+            eng.residual(T1, T2, R1, R2)
+            out.append((to_np(R1), to_np(R2)))
+        tol = 1e-12 if precision == "fp64" else 1e-6
+        assert rel(out[1][0], out[0][0]) < tol
+        assert rel(out[1][1], out[0][1]) < tol
+        eng.set_trusted(True)
+        eng.step(T1, T2, T1b, T2b, g, beta, 0.5)
+    eng.close()
+
+
 # ------------------------------------------------------------ graphs, batch
 def test_graph_steps_equal_eager_steps():
     """ccsd(graph=True) (trusted iterates, every step a replayed HIP graph on
