@@ -66,8 +66,9 @@ def measured_traffic(precision, n, ng, layout):
     """HBM bytes per launch of the dominant kernel, from the committed
     FETCH_SIZE/WRITE_SIZE profile of this workload (a line
     'traffic_bytes_per_launch <precision> <n> <ng> <layout> <bytes>',
-    layout e.g. 'spin-packed-c4': the operand layout and the grid points per
-    launch), or None."""
+    layout e.g. 'spin-packed-flip-c8': the operand layout, '-flip' when the
+    ring products compute the alpha-virtual rows only, and the grid points
+    per launch), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
@@ -194,6 +195,8 @@ def main():
     # blocks with the ladder products over packed antisymmetric pairs
     layout = ("dense" if not spin_blocked else
               "spin-packed" if spin_mode & 4 else "spin")
+    if spin_blocked and spin_mode & 8:
+        layout += "-flip"    # ring products over the alpha-virtual rows
     log("device set-up done in %.1f s (context %.1f GB, product layout: %s, "
         "spin flags %d)" % (time.time() - t0, eng.workspace_bytes() / 1e9,
                             layout, spin_mode))
