@@ -450,27 +450,36 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
 // X[y][p - hv][fl(k)][fl(q)][fl(l)] (fl: alpha <-> beta partner). Allowed
 // means sk + sq == sp + sl for a ring intermediate W[e,m,b,j] (vv_pair = 0)
 // and sp + sq == sk + sl for an amplitude-like Z[a,i,b,j] (vv_pair = 1).
+// With p beta each (k, q) spin pair allows at most one l run, and three of
+// the four pairs allow one:
+//   vv_pair 0: (sk, sq) = (1,0), (0,1) -> sl = 0; (1,1) -> sl = 1
+//   vv_pair 1: (sk, sq) = (1,0) -> sl = 0; (0,0), (1,1) -> sl = 1
+// so the kernel enumerates the allowed elements only: per slice (blockIdx.y)
+// hv x 3 x ho x hv x ho of them, l innermost (source and destination runs of
+// ho contiguous elements).
 __global__ void flip_copy_ring_kernel(double *X, int nv, int no, int hv,
-                                      int ho, int nb, int64_t sb,
-                                      int vv_pair) {
-  const int64_t per = (int64_t)(nv - hv) * no * nv * no;
-  GRID_STRIDE(idx, per * nb) {
-    const int64_t y = idx / per;
-    int64_t r = idx % per;
-    const int l = (int)(r % no);
-    r /= no;
-    const int q = (int)(r % nv);
-    r /= nv;
-    const int k = (int)(r % no);
-    const int p = hv + (int)(r / no);
-    const int sk = k >= ho, sq = q >= hv, sl = l >= ho;
-    // spin-forbidden (p is beta)
-    if (vv_pair ? 1 + sq != sk + sl : sk + sq != 1 + sl) continue;
-    const int fk = sk ? k - ho : k + ho, fq = sq ? q - hv : q + hv,
-              fl = sl ? l - ho : l + ho;
-    double *x = X + y * sb;
-    x[(((int64_t)p * no + k) * nv + q) * no + l] =
-        x[(((int64_t)(p - hv) * no + fk) * nv + fq) * no + fl];
+                                      int ho, int64_t sb, int vv_pair) {
+  const int per = 3 * hv * ho * hv * ho;
+  double *x = X + blockIdx.y * sb;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < per;
+       idx += gridDim.x * blockDim.x) {
+    int r = idx;
+    const int lh = r % ho;
+    r /= ho;
+    const int qh = r % hv;
+    r /= hv;
+    const int kh = r % ho;
+    r /= ho;
+    const int c = r % 3;
+    const int ph = r / 3;
+    const int sk = c == 1 ? 0 : 1;
+    const int sq = c == 0 ? 0 : (c == 1 ? vv_pair ^ 1 : 1);
+    const int sl = c == 0 ? 0 : (c == 1 ? vv_pair : 1);
+    const int k = kh + sk * ho, q = qh + sq * hv, l = lh + sl * ho;
+    const int fk = kh + (sk ^ 1) * ho, fq = qh + (sq ^ 1) * hv,
+              fl = lh + (sl ^ 1) * ho;
+    x[(((int64_t)(hv + ph) * no + k) * nv + q) * no + l] =
+        x[(((int64_t)ph * no + fk) * nv + fq) * no + fl];
   }
 }
 
@@ -645,8 +654,13 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
 void flip_copy_ring(double *X, int nv, int no, int hv, int ho, int nb,
                     int64_t sb, bool vv_pair, hipStream_t s) {
   require(2 * hv == nv && 2 * ho == no, "flip copy needs equal spin halves");
-  LAUNCH(flip_copy_ring_kernel, (int64_t)(nv - hv) * no * nv * no * nb, X, nv,
-         no, hv, ho, nb, sb, vv_pair ? 1 : 0);
+  require(nb >= 1 && nb <= 65535, "flip copy: slice count out of range");
+  const int64_t per = 3 * (int64_t)hv * ho * hv * ho;
+  require(per < (int64_t)1 << 31, "flip copy: slice too large");
+  const int blocks = (int)std::min<int64_t>((per + TPB - 1) / TPB, 1 << 16);
+  hipLaunchKernelGGL(flip_copy_ring_kernel, dim3(blocks, nb), dim3(TPB), 0, s,
+                     X, nv, no, hv, ho, sb, vv_pair ? 1 : 0);
+  FTCC_CHECK_HIP(hipGetLastError());
 }
 
 void transpose2(const double *in, double *out, int r, int c, int nb,
