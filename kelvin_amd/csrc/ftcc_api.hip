@@ -118,6 +118,10 @@ struct ftcc_ctx {
   // the residual being evaluated).
   int flip = 0;
   bool flip_now = false;
+  // under flip_now: an output index letter of the spin-restricted GEMMs
+  // whose beta assignments are skipped (their blocks are flipped images,
+  // copied once the output is complete); 0 = none
+  char flip_lead = 0;
   PairSpace ps_vv, ps_oo, ps_vo;   // (a,b), (i,j) and ring (a,i) pairs
   // Ladder products contract over an antisymmetric pair, so they sum over
   // the packed pairs only (e < f within a spin, alpha-beta once) and double
@@ -426,6 +430,19 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
               Ni.size() <= 3 && Ki.size() >= 1 && Ki.size() <= 3 &&
               batch <= 1 && pm <= 1 && pn <= 1 && (!(pm + pn) || c->pack),
           "spin-restricted GEMM: bad index lists");
+  // a flip lead must index C (skipping a K spin would drop partial sums)
+  const char lead = c->flip_lead;
+  if (lead) {
+    bool in_c = false;
+    for (const auto *v : {&Mi, &Ni})
+      for (const SIdx &x : *v)
+        in_c |= (x.role == 'o' || x.role == 'v') && x.l == lead;
+    require(in_c, "spin-restricted GEMM: flip lead is not an output index");
+  }
+  auto lead_beta = [&](const std::map<char, int> &sp) {
+    const auto it = lead ? sp.find(lead) : sp.end();
+    return it != sp.end() && it->second == 1;
+  };
   // a pair index runs over the packed pairs of its spin class (x < y for
   // equal spins, every mixed pair once, none for the beta-alpha class);
   // its level maps the packed index to the dense pair x * n + y
@@ -455,7 +472,7 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
     std::map<char, int> sp;
     for (int k = 0; k < L; ++k) sp[letters[k]] = (mask >> k) & 1;
     if (!rule_holds(ruleA, sp) || !rule_holds(ruleB, sp) ||
-        !rule_holds(ruleC, sp))
+        !rule_holds(ruleC, sp) || lead_beta(sp))
       continue;
     Gemm g = base;
     int64_t oa = 0, ob = 0, oc = 0;
@@ -583,7 +600,7 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
       };
       for (const SIdx &x : Mi) take(x);
       for (const SIdx &x : Ni) take(x);
-      if (rule_holds(ruleC, sp) && cells > 0)
+      if (rule_holds(ruleC, sp) && cells > 0 && !lead_beta(sp))
         require(c_blocks.count(cb) == 1,
                 "spin-restricted GEMM leaves an allowed output block unset");
     }
@@ -914,16 +931,13 @@ void block_list(int kind, const PairSpace &R, const PairSpace &K,
   }
 }
 
-// The rows of a ring product output (ring layout [e][m][b][j], slices of
-// g.sC1) whose virtual index is beta, from their spin-flipped images.
-// G4's Wr[e,m,b,j] conserves se + sj = sm + sb, G5's Z[a,i,b,j] sa + sb =
-// si + sj.
-void flip_rows(ftcc_ctx *c, const Gemm &g, int kind) {
+// The rows of a ring tensor (layout [e][m][b][j], nb slices of V*O*V*O)
+// whose virtual index e is beta, from their spin-flipped images: Wr[e,m,b,j]
+// conserves se + sj = sm + sb, Z[a,i,b,j] sa + sb = si + sj (vv_pair).
+void flip_rows(ftcc_ctx *c, double *X, int nb, bool vv_pair) {
   const int O = c->no, V = c->nv;
-  require(g.sCm == (int64_t)V * O && g.sCn == 1,
-          "flip copy: ring output layout expected");
-  flip_copy_ring(g.C, V, O, c->v_count[0], c->o_count[0], g.nb1, g.sC1,
-                 kind == RING_G5, c->stream);
+  flip_copy_ring(X, V, O, c->v_count[0], c->o_count[0], nb,
+                 (int64_t)V * O * V * O, vv_pair, c->stream);
 }
 
 // One O(n^6) product C = alpha A.B + beta C over g.nb1 slots, g describing
@@ -940,8 +954,8 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
   Blk blk[3];
   block_list(kind, R, K, Cs, blk);
   // spin-flip symmetry: the ring products compute the rows whose virtual
-  // index is alpha (classes 00 and 01 of the ring pairs); the others are
-  // copied from their flipped images below
+  // index is alpha (classes 00 and 01 of the ring pairs); the caller copies
+  // the others from their flipped images once the output is complete
   const bool flip = c->flip_now && kind != LADDER;
   if (flip) {
     blk[0].r = group(R, 0);
@@ -985,7 +999,6 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
         sbgemm_grouped(np, pb, g.nb1, alpha, g.beta, g.C, g.sCm, g.sCn,
                        g.sC1, c->stream);
       });
-    if (flip) flip_rows(c, g, kind);
     return;
   }
   for (const Blk &b : blk) {
@@ -1003,7 +1016,6 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     ensure_partial(c, gb);
     timed(c, true, flops, [&] { gemm_run(gb, P, c->scr, c->stream); });
   }
-  if (flip) flip_rows(c, g, kind);
 }
 
 void read_scalars(ftcc_ctx *c, int n);
@@ -1478,8 +1490,14 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
                 c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), OP_RING_A,
           OP_ARING_B, RING_G4);                                          // G4
   // <mb||ej> + t_j^f <mb||ef> = ovvo_ring - sum_f vovv[b,m,e,f] t1[f,j]:
-  // rows (e,m,b) of Wr
+  // rows (e,m,b) of Wr.  Under the spin-flip symmetry (flip_now, spin5
+  // implied) the terms of Wr and Z, like G4 and G5, compute the rows of an
+  // alpha virtual e (a) only, and their beta rows are copied once each
+  // tensor is complete
+  const bool flip = c->flip_now;
+  require(!flip || s5, "flip symmetry needs the spin-restricted terms");
   if (s5) {
+    c->flip_lead = flip ? 'e' : 0;
     run_spin(c, sgem(-1.0, 1.0, c->vovv_embf, 0, t1, VO, c->wr, S2,
                      c->ovvo_ring),
              P5,
@@ -1493,6 +1511,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              {{'e', 'v', 0, O * OO, O * VO}, {'m', 'o', 0, OO, VO},
               {'j', 'o', 0, 1, 1}},
              {'n', 'o', 1, O, 0}, "b|n", "mn|je", "mb|ej");
+    c->flip_lead = 0;
   } else {
     Gemm g = mk((int)(V * VO), O, V, -1.0, 1.0, c->vovv_embf, V, 1, 0, t1, O,
                 1, VO, c->wr, O, 1, S2, nb);
@@ -1512,6 +1531,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
 
+  if (flip) flip_rows(c, c->wr, nb, false);
+
   // Z[ai,bj] = t2r.Wr - t_i^e t_m^a <mb||ej>                    (G5)
   put_operand(c, OP_WR_B, c->wr, 1, VO, S2, nb);
   // spin blocks leave the spin-forbidden part of Z unwritten; the
@@ -1520,16 +1541,21 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
                 VO, 1, S2, c->z, VO, 1, S2, nb), OP_T2R_A, OP_WR_B, RING_G5);
   if (s5) {
     // u[i,(m,b,j)] = t1[e,i] ovvo_ring[e,m,b,j]; Z[a,(i,b,j)] -= t1[a,m] u
+    // (Z rows of an alpha a read the u of an alpha m only)
+    c->flip_lead = flip ? 'm' : 0;
     run_spin(c, sgem(1.0, 0.0, t1, VO, c->ovvo_ring, 0, c->u, OO * VO,
                      nullptr),
              P5, {{'y', 'y', VO, 0, OO * VO}, {'i', 'o', 1, 0, O * VO}},
              {{'m', 'o', 0, VO, VO}, {'b', 'v', 0, O, O}, {'j', 'o', 0, 1, 1}},
              {'e', 'v', O, O * VO, 0}, "e|i", "mb|ej", "mb|ij");
+    c->flip_lead = flip ? 'a' : 0;
     run_spin(c, sgem(-1.0, 1.0, t1, VO, c->u, OO * VO, c->z, S2, nullptr), P5,
              {{'a', 'v', O, 0, O * VO}},
              {{'i', 'o', 0, O * VO, VO}, {'b', 'v', 0, O, O},
               {'j', 'o', 0, 1, 1}},
              {'m', 'o', 1, VO, 0}, "a|m", "mb|ij", "ab|ij");
+    c->flip_lead = 0;
+    if (flip) flip_rows(c, c->z, nb, true);
   } else {
     run(c, mk(O, (int)(O * VO), V, 1.0, 0.0, t1, 1, O, VO, c->ovvo_ring,
               O * VO, 1, 0, c->u, O * VO, 1, OO * VO, nb),
@@ -1745,6 +1771,7 @@ bool residual_all(ftcc_ctx *c, const double *T1, const double *T2,
     ~Restore() {
       c->spin = v;
       c->flip_now = false;
+      c->flip_lead = 0;
     }
   } restore{c, spin_saved};
   const int O = c->no, V = c->nv;
