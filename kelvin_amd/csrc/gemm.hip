@@ -122,12 +122,19 @@ template <> struct Mma<float> {
   }
 };
 
-// Decode the flat block index into (tile n, tile m, split, b2, b1).
+// Decode the flat block index into (tile n, tile m, split, b2, b1).  The
+// tiles along n of one tile row share its A panel (for the O(n^5) terms the
+// large integral operand), so the index is first regrouped per XCD:
+// workgroups are dispatched to the eight XCDs round robin and each XCD has
+// its own L2; the hardware ids x, x + 8, x + 16, ... of XCD x take a
+// contiguous range of tiles (a bijection on [0, gridDim.x)).
 struct BlockPos {
   int tn, tm, ks, b1, b2;
 };
 __device__ inline BlockPos decode(const GemmDev &p) {
-  int64_t bid = blockIdx.x;
+  const int64_t G = gridDim.x, hw = blockIdx.x;
+  const int64_t q8 = G / 8, r8 = G % 8, x = hw % 8, loc = hw / 8;
+  int64_t bid = x < r8 ? x * (q8 + 1) + loc : r8 * (q8 + 1) + (x - r8) * q8 + loc;
   BlockPos q;
   q.tn = (int)(bid % p.tilesN);
   bid /= p.tilesN;

@@ -354,8 +354,18 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
 // in LDS as [j][i] strips (row stride R2P_LD doubles: two-way bank
 // conflicts at most on the reads along j).  Four rows per workgroup
 // measured fastest at n = 114 (2, 4, 8, 16 rows: 38.6, 28.5, 33.7, 71.0 ms
-// per iteration).
+// per iteration).  The row tiles of one (a,b) pair share the cache lines of
+// their staged reads (R2P_ROWS doubles of each line per tile), so the block
+// index is regrouped to run consecutive tiles on one XCD (workgroups are
+// dispatched to the eight XCDs round robin; each XCD has its own L2).
 constexpr int R2P_ROWS = 4, R2P_LD = 5;
+
+// Logical block of a hardware block id: the ids x, x + 8, x + 16, ... (XCD
+// x) take a contiguous range of logical blocks; a bijection on [0, G).
+__device__ inline int64_t xcd_grouped(int64_t bid, int64_t G) {
+  const int64_t q = G / 8, r = G % 8, x = bid % 8, loc = bid / 8;
+  return x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+}
 __global__ __launch_bounds__(256) void r2_packed_kernel(
     double *t2bar, const double *acc1, const double *acc2, const double *S1,
     const double *S2, const double *Z, const double *add, int nv, int no,
@@ -365,7 +375,7 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
          *tP = sm + 3 * no * R2P_LD;
   const int tiles = (no + R2P_ROWS - 1) / R2P_ROWS;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
-  int64_t bid = blockIdx.x;
+  int64_t bid = xcd_grouped(blockIdx.x, gridDim.x);
   const int ti = (int)(bid % tiles);
   bid /= tiles;
   const int64_t q = bid % npair;
