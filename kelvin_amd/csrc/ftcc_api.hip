@@ -1246,6 +1246,14 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // O(n^5) contractions over spin-allowed blocks only (forbidden blocks of
   // their outputs are left unwritten; no consumer reads them)
   const bool s5 = c->spin && c->spin5;
+  // Under the spin-flip symmetry (flip_now, spin5 implied) the spin-
+  // restricted terms of FmeT, Fae, Fmi, Wr, Z and t1bar compute the blocks
+  // of an alpha lead index only (flip_lead), and the beta blocks of each are
+  // copied from their images once the tensor is complete
+  const bool flip = c->flip_now;
+  require(!flip || s5, "flip symmetry needs the spin-restricted terms");
+  const int hv = c->v_count[0], ho = c->o_count[0];
+  auto lead = [&](char l) { c->flip_lead = flip ? l : 0; };
   auto sgem = [&](double alpha, double beta, const double *A, int64_t sA1,
                   const double *B, int64_t sB1, double *C, int64_t sC1,
                   const double *D) {
@@ -1293,12 +1301,15 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // grid points y as columns (oovv_ring[e,m,f,n] read once)
   if (s5) {
     fill(c->FmeT, VO * nb, 0.0, s);   // spin-forbidden entries stay zero
+    lead('e');
     run_spin(c, sgem(1.0, 0.0, c->oovv_ring, 0, t1, VO, c->FmeT, VO,
                      nullptr),
              P5, {{'e', 'v', O * VO, 0, O}, {'m', 'o', VO, 0, 1}},
              {{'y', 'y', 0, VO, VO}},
              {{'f', 'v', O, O, 0}, {'n', 'o', 1, 1, 0}}, "mn|ef", "f|n",
              "e|m");
+    lead(0);
+    if (flip) flip_copy_mat(c->FmeT, V, O, hv, ho, nb, VO, s);
   } else {
     run(c, mk((int)VO, nb, (int)VO, 1.0, 0.0, c->oovv_ring, VO, 1, 0, t1, 1,
               VO, 0, c->FmeT, 1, VO, 0, 1),
@@ -1318,6 +1329,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // + sum_{mf} t_m^f <ma||fe> = - sum_{fm} t1[f,m] vovv[a,m,f,e], vovv[a,m,f,e]
   // = vovv_aefm[a,f,e,m]: rows y, columns (a,e), K = (f,m)
   if (s5) {
+    lead('a');
     run_spin(c, sgem(-1.0, 1.0, t1, VO, c->vovv_aefm, 0, c->Fae, VV,
                      nullptr),
              P5, {{'y', 'y', VO, 0, VV}},
@@ -1339,14 +1351,16 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   }
   // - 1/2 sum_{fmn} t2[a,f,m,n] <mn||ef> (oovvT[e,f,m,n]: K fastest)
   // (the chunk's grid points are rows of one launch: oovvT is read once)
-  if (s5)
+  if (s5) {
     run_spin(c, sgem(-0.5, 1.0, t2, S2, c->oovvT, 0, c->Fae, VV,
                      nullptr),
              P5, {{'y', 'y', S2, 0, VV}, {'a', 'v', V * OO, 0, V}},
              {{'e', 'v', 0, V * OO, 1}},
              {{'f', 'v', OO, OO, 0}, {'m', 'o', O, O, 0}, {'n', 'o', 1, 1, 0}},
              "af|mn", "mn|ef", "a|e");
-  else
+    lead(0);
+    if (flip) flip_copy_mat(c->Fae, V, V, hv, hv, nb, VV, s);
+  } else
     run(c, fold_m(mk(V, V, (int)(V * OO), -0.5, 1.0, t2, V * OO, 1, S2,
                      c->oovvT, 1, V * OO, 0, c->Fae, V, 1, VV, nb)),
         P5);
@@ -1357,14 +1371,15 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   run(c, mk(O, O, V, 0.5, 1.0, c->Fme, V, 1, VO, t1, O, 1, VO, c->Fmi, O, 1,
             OO, nb), P64);
   // + sum_{ne} <mn||ie> t_n^e: ooov_p1[m,i,n,e] rows (m,i), columns y
-  if (s5)
+  if (s5) {
+    lead('m');
     run_spin(c, sgem(1.0, 1.0, c->ooov_p1, 0, c->t1T, VO, c->Fmi, OO,
                      nullptr),
              P5, {{'m', 'o', O * VO, 0, O}, {'i', 'o', VO, 0, 1}},
              {{'y', 'y', 0, VO, OO}},
              {{'n', 'o', V, V, 0}, {'e', 'v', 1, 1, 0}}, "mn|ie", "e|n",
              "m|i");
-  else
+  } else
     run(c, mk((int)OO, nb, (int)VO, 1.0, 1.0, c->ooov_p1, VO, 1, 0, c->t1T, 1,
               VO, 0, c->Fmi, 1, OO, 0, 1),
         P5);
@@ -1378,6 +1393,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              {{'e', 'v', VO, V * OO, 0}, {'f', 'v', O, OO, 0},
               {'n', 'o', 1, 1, 0}},
              "mn|ef", "ef|in", "m|i");
+    lead(0);
+    if (flip) flip_copy_mat(c->Fmi, O, O, ho, ho, nb, OO, s);
   } else {
     Gemm g = fold_n(mk(O, O, (int)(O * VV), 0.5, 1.0, c->oovv_mefn, V * VO, 1,
                        0, t2, 1, O, S2, c->Fmi, O, 1, OO, nb));
@@ -1490,14 +1507,10 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
                 c->aring, VO, 1, S2, c->wr, VO, 1, S2, nb), OP_RING_A,
           OP_ARING_B, RING_G4);                                          // G4
   // <mb||ej> + t_j^f <mb||ef> = ovvo_ring - sum_f vovv[b,m,e,f] t1[f,j]:
-  // rows (e,m,b) of Wr.  Under the spin-flip symmetry (flip_now, spin5
-  // implied) the terms of Wr and Z, like G4 and G5, compute the rows of an
-  // alpha virtual e (a) only, and their beta rows are copied once each
-  // tensor is complete
-  const bool flip = c->flip_now;
-  require(!flip || s5, "flip symmetry needs the spin-restricted terms");
+  // rows (e,m,b) of Wr (like G4 and G5, over the rows of an alpha e, a
+  // under the flip symmetry)
   if (s5) {
-    c->flip_lead = flip ? 'e' : 0;
+    lead('e');
     run_spin(c, sgem(-1.0, 1.0, c->vovv_embf, 0, t1, VO, c->wr, S2,
                      c->ovvo_ring),
              P5,
@@ -1542,13 +1555,13 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   if (s5) {
     // u[i,(m,b,j)] = t1[e,i] ovvo_ring[e,m,b,j]; Z[a,(i,b,j)] -= t1[a,m] u
     // (Z rows of an alpha a read the u of an alpha m only)
-    c->flip_lead = flip ? 'm' : 0;
+    lead('m');
     run_spin(c, sgem(1.0, 0.0, t1, VO, c->ovvo_ring, 0, c->u, OO * VO,
                      nullptr),
              P5, {{'y', 'y', VO, 0, OO * VO}, {'i', 'o', 1, 0, O * VO}},
              {{'m', 'o', 0, VO, VO}, {'b', 'v', 0, O, O}, {'j', 'o', 0, 1, 1}},
              {'e', 'v', O, O * VO, 0}, "e|i", "mb|ej", "mb|ij");
-    c->flip_lead = flip ? 'a' : 0;
+    lead('a');
     run_spin(c, sgem(-1.0, 1.0, t1, VO, c->u, OO * VO, c->z, S2, nullptr), P5,
              {{'a', 'v', O, 0, O * VO}},
              {{'i', 'o', 0, O * VO, VO}, {'b', 'v', 0, O, O},
@@ -1656,6 +1669,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // + sum_{em} t2[a,e,i,m] FmeT[e,m] (per grid point) and
   // + sum_{em} t1[e,m] ovvo_ring[e,m,a,i] (ovvo_ringT[a,i,e,m]; rows y)
   if (s5) {
+    lead('a');
     run_spin(c, sgem(1.0, 1.0, t2, S2, c->FmeT, VO, t1bar, VO, nullptr), P5,
              {{'a', 'v', V * OO, 0, O}, {'i', 'o', O, 0, 1}},
              {{'_', '1', 0, 0, 0}}, {{'e', 'v', OO, O, 0}, {'m', 'o', 1, 1, 0}},
@@ -1698,6 +1712,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              {{'i', 'o', 0, V * OO, 1}},
              {{'e', 'v', OO, OO, 0}, {'m', 'o', O, O, 0}, {'n', 'o', 1, 1, 0}},
              "ae|mn", "nm|ie", "a|i");
+    lead(0);
+    if (flip) flip_copy_mat(t1bar, V, O, hv, ho, nb, VO, s);
   } else {
     Gemm g = fold_n(mk(V, O, (int)(O * VV), 0.5, 1.0, c->vovv_aefm, V * VO, 1,
                        0, t2, 1, O, S2, t1bar, O, 1, VO, nb));

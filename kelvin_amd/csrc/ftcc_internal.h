@@ -248,6 +248,11 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
 // vv_pair: X[a,i,b,j] conserves sa + sb = si + sj (else se + sj = sm + sb).
 void flip_copy_ring(double *X, int nv, int no, int hv, int ho, int nb,
                     int64_t sb, bool vv_pair, hipStream_t s);
+// Spin-flip image of a spin-diagonal matrix X[nb][P][Q] (slice stride sb;
+// alpha runs [0, hp) and [0, hq), beta runs after them): the beta-beta
+// block is copied from the alpha-alpha one.
+void flip_copy_mat(double *X, int P, int Q, int hp, int hq, int nb,
+                   int64_t sb, hipStream_t s);
 // x[n][c] (nb batches of n*c) -> out[c][n] transposed copy for small mats
 void transpose2(const double *in, double *out, int r, int c, int nb,
                 hipStream_t s);

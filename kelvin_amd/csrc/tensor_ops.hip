@@ -493,6 +493,20 @@ __global__ void flip_copy_ring_kernel(double *X, int nv, int no, int hv,
   }
 }
 
+// Spin-diagonal matrices X[y][p][q] (P x Q, slice stride sb) of a
+// spin-flip symmetric problem whose alpha-alpha block is computed: the
+// beta-beta block is its image, X[y][hp + p][hq + q] = X[y][p][q].
+__global__ void flip_copy_mat_kernel(double *X, int Q, int hp, int hq,
+                                     int64_t sb) {
+  const int per = hp * hq;
+  double *x = X + blockIdx.y * sb;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < per;
+       idx += gridDim.x * blockDim.x) {
+    const int p = idx / hq, q = idx % hq;
+    x[(int64_t)(hp + p) * Q + hq + q] = x[(int64_t)p * Q + q];
+  }
+}
+
 __global__ void transpose2_kernel(const double *in, double *out, int r, int c,
                                   int nb) {
   const int64_t per = (int64_t)r * c;
@@ -658,6 +672,18 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
     hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks),
                        dim3(256), 0, s, t2bar, acc1, acc2, S1, S2, Z, add, nv,
                        no, lab_v, lab_o);
+  FTCC_CHECK_HIP(hipGetLastError());
+}
+
+void flip_copy_mat(double *X, int P, int Q, int hp, int hq, int nb,
+                   int64_t sb, hipStream_t s) {
+  require(2 * hp == P && 2 * hq == Q, "flip copy needs equal spin halves");
+  require(nb >= 1 && nb <= 65535 && (int64_t)P * Q <= sb,
+          "flip copy: bad slice layout");
+  const int per = hp * hq;
+  const int blocks = std::max(1, std::min((per + TPB - 1) / TPB, 1 << 16));
+  hipLaunchKernelGGL(flip_copy_mat_kernel, dim3(blocks, nb), dim3(TPB), 0, s,
+                     X, Q, hp, hq, sb);
   FTCC_CHECK_HIP(hipGetLastError());
 }
 
