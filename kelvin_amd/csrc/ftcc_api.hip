@@ -1568,7 +1568,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
               {'j', 'o', 0, 1, 1}},
              {'m', 'o', 1, VO, 0}, "a|m", "mb|ij", "ab|ij");
     c->flip_lead = 0;
-    if (flip) flip_rows(c, c->z, nb, true);
+    // the packed assembly reads the beta rows of Z from their images
+    if (flip && !c->pack) flip_rows(c, c->z, nb, true);
   } else {
     run(c, mk(O, (int)(O * VO), V, 1.0, 0.0, t1, 1, O, VO, c->ovvo_ring,
               O * VO, 1, 0, c->u, O * VO, 1, OO * VO, nb),
@@ -1655,9 +1656,10 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
   // forbidden blocks (spin layout): written as zero without reading
+  const bool zflip = flip && c->pack;
   r2_final(t2bar, c->wr, c->aring, c->s1, c->s2, c->z, V, O, nb, s,
            c->spin ? c->d_lab_v : nullptr, c->spin ? c->d_lab_o : nullptr,
-           vvoo, c->spin && c->pack);
+           vvoo, c->spin && c->pack, zflip ? hv : 0, zflip ? ho : 0);
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
   //                    + 1/2 vovv.t2 + 1/2 t2.ooov)

@@ -236,11 +236,15 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
 // without reading the operands.  packed (needs the labels): every input
 // holds one pair of each antisymmetric couple (x < y for equal spins, the
 // alpha index first for mixed spins): acc1/acc2 in (a,b) and (i,j), S1 in
-// (i,j), S2 in (a,b); add must be antisymmetric in (a,b).
+// (i,j), S2 in (a,b); add must be antisymmetric in (a,b).  zh > 0 (packed
+// only; alpha runs [0, zh) and [0, zo) with beta runs of the same lengths
+// after them): Z holds the rows of an alpha first index only, the others
+// are read from their spin-flipped images.
 void r2_final(double *t2bar, const double *acc1, const double *acc2,
               const double *S1, const double *S2, const double *Z, int nv,
               int no, int nb, hipStream_t s, const int *lab_v,
-              const int *lab_o, const double *add, bool packed);
+              const int *lab_o, const double *add, bool packed, int zh = 0,
+              int zo = 0);
 // Spin-flip images of a ring tensor X[nb][nv][no][nv][no] (slice stride
 // sb) whose rows with a virtual alpha index are computed: every allowed
 // element with that index beta is copied from its flipped partner (alpha

@@ -347,7 +347,10 @@ __global__ __launch_bounds__(256) void r2_final_pair_kernel(
 //   S1          every (a,b), the stored (i,j)   (S1 is antisymmetric in ij)
 //   S2          the stored (a,b), every (i,j)   (S2 is antisymmetric in ab)
 // and add (<ab||ij>) is antisymmetric, so that R2[b,a,i,j] = -R2[a,b,i,j]
-// is written without further reads.  One workgroup per (a <= b) pair and
+// is written without further reads.  zh > 0 (spin-flip symmetric problem,
+// alpha runs [0, zh) and [0, zo), beta runs after them): Z holds the rows of
+// an alpha first index only, and a row of a beta one is read from its image
+// Z[p - zh][fl(k)][fl(q)][fl(l)].  One workgroup per (a <= b) pair and
 // R2P_ROWS rows i, all j: direct reads and writes run along whole rows of
 // j; the four values read transposed (the stored element of a transposed
 // (i,j), S2[.,.,j,i], the S1 and Z differences at (j,i)) are first staged
@@ -369,7 +372,7 @@ __device__ inline int64_t xcd_grouped(int64_t bid, int64_t G) {
 __global__ __launch_bounds__(256) void r2_packed_kernel(
     double *t2bar, const double *acc1, const double *acc2, const double *S1,
     const double *S2, const double *Z, const double *add, int nv, int no,
-    const int *lv, const int *lo) {
+    const int *lv, const int *lo, int zh, int zo) {
   extern __shared__ double sm[];   // 4 strips of no * R2P_LD
   double *tS2 = sm, *tS1 = sm + no * R2P_LD, *tZ = sm + 2 * no * R2P_LD,
          *tP = sm + 3 * no * R2P_LD;
@@ -396,6 +399,15 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
   auto ring = [&](int p, int k, int r, int l) {
     return (((int64_t)p * no + k) * nv + r) * no + l;
   };
+  auto zr = [&](int p, int k, int r, int l) {
+    if (zh > 0 && p >= zh) {
+      p -= zh;
+      k = k < zo ? k + zo : k - zo;
+      r = r < zh ? r + zh : r - zh;
+      l = l < zo ? l + zo : l - zo;
+    }
+    return z[ring(p, k, r, l)];
+  };
   // the stored (a,b) pair and its sign; a == b is zero throughout
   int pa = a, pb = b;
   double sg = 1.0;
@@ -418,7 +430,7 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
     const int64_t e = nat(pa, pb, j, i);
     tS2[j * R2P_LD + il] = ok ? s2[e] : 0.0;
     tS1[j * R2P_LD + il] = tr ? s1[nat(a, b, j, i)] - s1[nat(b, a, j, i)] : 0.0;
-    tZ[j * R2P_LD + il] = ok ? z[ring(b, j, a, i)] - z[ring(a, j, b, i)] : 0.0;
+    tZ[j * R2P_LD + il] = ok ? zr(b, j, a, i) - zr(a, j, b, i) : 0.0;
     tP[j * R2P_LD + il] = tr ? c1[e] + c2[e] : 0.0;
   }
   __syncthreads();
@@ -445,7 +457,7 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
       }
       const double s2d = s2[e] - tS2[j * R2P_LD + il];
       const double zz =
-          z[ring(a, i, b, j)] - z[ring(b, i, a, j)] + tZ[j * R2P_LD + il];
+          zr(a, i, b, j) - zr(b, i, a, j) + tZ[j * R2P_LD + il];
       double v = sg * (acc + s2d) + s1d + zz;
       if (add) v += add[nab];
       out[nab] = -v;
@@ -646,8 +658,11 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
 void r2_final(double *t2bar, const double *acc1, const double *acc2,
               const double *S1, const double *S2, const double *Z, int nv,
               int no, int nb, hipStream_t s, const int *lab_v,
-              const int *lab_o, const double *add, bool packed) {
+              const int *lab_o, const double *add, bool packed, int zh,
+              int zo) {
   require(!packed || (lab_v && lab_o), "packed pairs need spin labels");
+  require(zh == 0 || (packed && 2 * zh == nv && 2 * zo == no),
+          "r2 assembly: flipped Z needs the packed layout and equal halves");
   const int tiles = (no + 31) / 32;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   const int64_t blocks = (int64_t)nb * npair * tiles * tiles;
@@ -667,7 +682,7 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
     }
     hipLaunchKernelGGL(r2_packed_kernel, dim3((unsigned)pblocks), dim3(256),
                        lds, s, t2bar, acc1, acc2, S1, S2, Z, add, nv, no,
-                       lab_v, lab_o);
+                       lab_v, lab_o, zh, zo);
   } else
     hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks),
                        dim3(256), 0, s, t2bar, acc1, acc2, S1, S2, Z, add, nv,
