@@ -122,6 +122,10 @@ struct ftcc_ctx {
   // whose beta assignments are skipped (their blocks are flipped images,
   // copied once the output is complete); 0 = none
   char flip_lead = 0;
+  // output index letters (x, y) of the spin-restricted GEMMs whose
+  // assignments with x beta and y alpha are skipped: the packed ladder
+  // operands never read that class of a pair; 0 = none
+  char skip_ba[2] = {0, 0};
   PairSpace ps_vv, ps_oo, ps_vo;   // (a,b), (i,j) and ring (a,i) pairs
   // Ladder products contract over an antisymmetric pair, so they sum over
   // the packed pairs only (e < f within a spin, alpha-beta once) and double
@@ -439,9 +443,23 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
         in_c |= (x.role == 'o' || x.role == 'v') && x.l == lead;
     require(in_c, "spin-restricted GEMM: flip lead is not an output index");
   }
+  // an output pair of class beta-alpha that no reader reads: both letters
+  // index C
+  const char bx = c->skip_ba[0], by = c->skip_ba[1];
+  if (bx) {
+    int found = 0;
+    for (const auto *v : {&Mi, &Ni})
+      for (const SIdx &x : *v)
+        found += (x.role == 'o' || x.role == 'v') && (x.l == bx || x.l == by);
+    require(found == 2, "spin-restricted GEMM: skipped pair is not in C");
+  }
   auto lead_beta = [&](const std::map<char, int> &sp) {
     const auto it = lead ? sp.find(lead) : sp.end();
-    return it != sp.end() && it->second == 1;
+    if (it != sp.end() && it->second == 1) return true;
+    if (!bx) return false;
+    const auto ix = sp.find(bx), iy = sp.find(by);
+    return ix != sp.end() && iy != sp.end() && ix->second == 1 &&
+           iy->second == 0;
   };
   // a pair index runs over the packed pairs of its spin class (x < y for
   // equal spins, every mixed pair once, none for the beta-alpha class);
@@ -1431,6 +1449,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
                                     {'n', 'o', O * V, 0, OO}};
     std::vector<SIdx> Mi = mn;
     Mi.push_back({'i', 'o', V, 0, O});
+    // G2 reads the stored (i,j) of each couple only: no (i beta, j alpha)
+    if (c->pack) c->skip_ba[0] = 'i', c->skip_ba[1] = 'j';
     run_spin(c, sgem(1.0, 1.0, ooov, 0, t1, VO, c->wmnij, O4, oooo), P5, Mi,
              {{'y', 'y', 0, VO, O4}, {'j', 'o', 0, 1, 1}}, {'e', 'v', 1, O, 0},
              "mn|ie", "e|j", "mn|ij");
@@ -1445,6 +1465,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run_spin(c, sgem(-1.0, 1.0, t1, VO, ooov, 0, c->wmnij, O4, nullptr), P5,
              {{'y', 'y', VO, 0, O4}, {'i', 'o', 1, 0, O}}, Ni,
              {'e', 'v', O, 1, 0}, "e|i", "mn|je", "mn|ij");
+    c->skip_ba[0] = c->skip_ba[1] = 0;
   } else {
     Gemm g = mk((int)(OO * O), O, V, 1.0, 1.0, ooov, V, 1, 0, t1, O, 1, VO,
                 c->wmnij, O, 1, O4, nb);
@@ -1471,6 +1492,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     // vvvv - sum_m t_m^b <am||ef>: C[b][(a,e,f)] = vvvv - t1[b,m] vovv[a,m,ef]
     std::vector<SIdx> Ni{{'a', 'v', 0, O * VV, V * VV}};
     Ni.insert(Ni.end(), ef.begin(), ef.end());
+    // G3 reads the stored (a,b) of each couple only: no (a beta, b alpha)
+    if (c->pack) c->skip_ba[0] = 'a', c->skip_ba[1] = 'b';
     run_spin(c, sgem(-1.0, 0.0, t1, VO, vovv, 0, c->wabef, V4, vvvv), P5,
              {{'y', 'y', VO, 0, V4}, {'b', 'v', O, 0, VV}}, Ni,
              {'m', 'o', 1, VV, 0}, "b|m", "am|ef", "ab|ef");
@@ -1480,6 +1503,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run_spin(c, sgem(1.0, 1.0, t1, VO, vovv, 0, c->wabef, V4, nullptr), P5,
              {{'y', 'y', VO, 0, V4}, {'a', 'v', O, 0, V * VV}}, Ni,
              {'m', 'o', 1, VV, 0}, "a|m", "bm|ef", "ab|ef");
+    c->skip_ba[0] = c->skip_ba[1] = 0;
   } else {
     // vvvv - sum_m t_m^b <am||ef>: per a, C[b][ef] = vvvv - t1[b,m] vovv[a,m,ef]
     Gemm g = mk(V, (int)VV, O, -1.0, 0.0, t1, O, 1, VO, vovv, VV, 1, 0,
@@ -1790,6 +1814,7 @@ bool residual_all(ftcc_ctx *c, const double *T1, const double *T2,
       c->spin = v;
       c->flip_now = false;
       c->flip_lead = 0;
+      c->skip_ba[0] = c->skip_ba[1] = 0;
     }
   } restore{c, spin_saved};
   const int O = c->no, V = c->nv;
