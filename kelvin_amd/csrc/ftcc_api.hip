@@ -126,6 +126,9 @@ struct ftcc_ctx {
   // assignments with x beta and y alpha are skipped: the packed ladder
   // operands never read that class of a pair; 0 = none
   char skip_ba[2] = {0, 0};
+  // set by ftcc_step around its residual: its update reads the spin-allowed
+  // doubles of Tbar only, so the assembly leaves the forbidden ones unwritten
+  bool tbar_scratch = false;
   PairSpace ps_vv, ps_oo, ps_vo;   // (a,b), (i,j) and ring (a,i) pairs
   // Ladder products contract over an antisymmetric pair, so they sum over
   // the packed pairs only (e < f within a spin, alpha-beta once) and double
@@ -1683,7 +1686,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   const bool zflip = flip && c->pack;
   r2_final(t2bar, c->wr, c->aring, c->s1, c->s2, c->z, V, O, nb, s,
            c->spin ? c->d_lab_v : nullptr, c->spin ? c->d_lab_o : nullptr,
-           vvoo, c->spin && c->pack, zflip ? hv : 0, zflip ? ho : 0);
+           vvoo, c->spin && c->pack, zflip ? hv : 0, zflip ? ho : 0,
+           !(c->tbar_scratch && c->spin && c->pack));
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
   //                    + 1/2 vovv.t2 + 1/2 t2.ooov)
@@ -2506,7 +2510,19 @@ int ftcc_sumsq(ftcc_ctx *c, const double *x, int64_t n, double *out) {
 void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
                  double *T2b, double damp, uint64_t skip) {
   hipStream_t s = c->stream;
-  const bool spin_used = residual_all(c, T1old, T2old, T1b, T2b, skip);
+  // Tbar is this step's scratch: its spin-forbidden doubles are read by
+  // nobody when the residual ran in the spin layout (the update below skips
+  // them), so the assembly need not write their zeros
+  struct Scratch {
+    ftcc_ctx *c;
+    Scratch(ftcc_ctx *c_) : c(c_) { c->tbar_scratch = true; }
+    ~Scratch() { c->tbar_scratch = false; }
+  };
+  bool spin_used;
+  {
+    Scratch sc(c);
+    spin_used = residual_all(c, T1old, T2old, T1b, T2b, skip);
+  }
   UpdateArgs a = upd(c);
   a.damp = damp;
   a.tbar = T1b;
@@ -2538,6 +2554,10 @@ void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
              c->o_count[sj];
     }
     a.nallowed = n;
+    if (!a.rec) {   // the per-grid-point update skips by label instead
+      a.lv = c->d_lab_v;
+      a.lo = c->d_lab_o;
+    }
   } else if (spin_used) {   // spin-forbidden doubles are zero: skipped
     a.lv = c->d_lab_v;
     a.lo = c->d_lab_o;

@@ -239,12 +239,13 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
 // (i,j), S2 in (a,b); add must be antisymmetric in (a,b).  zh > 0 (packed
 // only; alpha runs [0, zh) and [0, zo) with beta runs of the same lengths
 // after them): Z holds the rows of an alpha first index only, the others
-// are read from their spin-flipped images.
+// are read from their spin-flipped images.  zero_forbidden = false (packed
+// only): the spin-forbidden doubles of t2bar are left unwritten.
 void r2_final(double *t2bar, const double *acc1, const double *acc2,
               const double *S1, const double *S2, const double *Z, int nv,
               int no, int nb, hipStream_t s, const int *lab_v,
               const int *lab_o, const double *add, bool packed, int zh = 0,
-              int zo = 0);
+              int zo = 0, bool zero_forbidden = true);
 // Spin-flip images of a ring tensor X[nb][nv][no][nv][no] (slice stride
 // sb) whose rows with a virtual alpha index are computed: every allowed
 // element with that index beta is copied from its flipped partner (alpha

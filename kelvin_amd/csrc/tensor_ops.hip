@@ -372,7 +372,7 @@ __device__ inline int64_t xcd_grouped(int64_t bid, int64_t G) {
 __global__ __launch_bounds__(256) void r2_packed_kernel(
     double *t2bar, const double *acc1, const double *acc2, const double *S1,
     const double *S2, const double *Z, const double *add, int nv, int no,
-    const int *lv, const int *lo, int zh, int zo) {
+    const int *lv, const int *lo, int zh, int zo, int zero_forbidden) {
   extern __shared__ double sm[];   // 4 strips of no * R2P_LD
   double *tS2 = sm, *tS1 = sm + no * R2P_LD, *tZ = sm + 2 * no * R2P_LD,
          *tP = sm + 3 * no * R2P_LD;
@@ -441,8 +441,11 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
     for (int j = threadIdx.x % lanes; j < no; j += lanes) {
       const int64_t nab = nat(a, b, i, j), nba = nat(b, a, i, j);
       if (zab || lo[i] + lo[j] != sab) {
-        out[nab] = 0.0;
-        out[nba] = 0.0;
+        // a == b is allowed for some (i,j) and always zero: written
+        if (zab || zero_forbidden) {
+          out[nab] = 0.0;
+          out[nba] = 0.0;
+        }
         continue;
       }
       const int sij = ij_sign(i, j);
@@ -659,10 +662,12 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
               const double *S1, const double *S2, const double *Z, int nv,
               int no, int nb, hipStream_t s, const int *lab_v,
               const int *lab_o, const double *add, bool packed, int zh,
-              int zo) {
+              int zo, bool zero_forbidden) {
   require(!packed || (lab_v && lab_o), "packed pairs need spin labels");
   require(zh == 0 || (packed && 2 * zh == nv && 2 * zo == no),
           "r2 assembly: flipped Z needs the packed layout and equal halves");
+  require(zero_forbidden || packed,
+          "r2 assembly: skipping forbidden doubles needs the packed layout");
   const int tiles = (no + 31) / 32;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   const int64_t blocks = (int64_t)nb * npair * tiles * tiles;
@@ -682,7 +687,7 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
     }
     hipLaunchKernelGGL(r2_packed_kernel, dim3((unsigned)pblocks), dim3(256),
                        lds, s, t2bar, acc1, acc2, S1, S2, Z, add, nv, no,
-                       lab_v, lab_o, zh, zo);
+                       lab_v, lab_o, zh, zo, zero_forbidden ? 1 : 0);
   } else
     hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks),
                        dim3(256), 0, s, t2bar, acc1, acc2, S1, S2, Z, add, nv,
