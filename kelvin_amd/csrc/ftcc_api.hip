@@ -104,6 +104,10 @@ struct ftcc_ctx {
   // three dense block products (10/64 of the dense work).
   int spin = 0;
   int *d_lab_o = nullptr, *d_lab_v = nullptr;
+  // occupied indices grouped by spin label: d_jl[0, jl_cnt[0]) the label-0
+  // ones, then the label-1 ones (in index order)
+  int *d_jl = nullptr;
+  int jl_cnt[2] = {0, 0};
   // O(n^5) contractions restricted to spin-allowed blocks: needs each label
   // list to be one run of 0s then one run of 1s (spin5 = 1); starts/counts
   // of the two runs in the occupied and virtual index lists
@@ -1051,11 +1055,25 @@ bool try_spin(ftcc_ctx *c, const std::vector<int> &lo,
   if (!c->d_lab_o) {
     c->d_lab_o = (int *)dalloc(c, (O + 1) / 2 + 1);
     c->d_lab_v = (int *)dalloc(c, (V + 1) / 2 + 1);
+    c->d_jl = (int *)dalloc(c, (O + 1) / 2 + 1);
   }
   FTCC_CHECK_HIP(hipMemcpyAsync(c->d_lab_o, lo.data(), sizeof(int) * O,
                                 hipMemcpyHostToDevice, s));
   FTCC_CHECK_HIP(hipMemcpyAsync(c->d_lab_v, lv.data(), sizeof(int) * V,
                                 hipMemcpyHostToDevice, s));
+  std::vector<int> jl;
+  for (int sp = 0; sp < 2; ++sp) {
+    c->jl_cnt[sp] = 0;
+    for (int k = 0; k < O; ++k)
+      if (lo[k] == sp) {
+        jl.push_back(k);
+        ++c->jl_cnt[sp];
+      }
+  }
+  if (!jl.empty())
+    FTCC_CHECK_HIP(hipMemcpyAsync(c->d_jl, jl.data(), sizeof(int) * jl.size(),
+                                  hipMemcpyHostToDevice, s));
+  FTCC_CHECK_HIP(hipStreamSynchronize(s));   // the host lists are local
   auto lab = [&](char x) { return x == 'o' ? c->d_lab_o : c->d_lab_v; };
   auto dim = [&](char x) { return x == 'o' ? O : V; };
   double bad = 0.0;
@@ -1687,7 +1705,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   r2_final(t2bar, c->wr, c->aring, c->s1, c->s2, c->z, V, O, nb, s,
            c->spin ? c->d_lab_v : nullptr, c->spin ? c->d_lab_o : nullptr,
            vvoo, c->spin && c->pack, zflip ? hv : 0, zflip ? ho : 0,
-           !(c->tbar_scratch && c->spin && c->pack));
+           !(c->tbar_scratch && c->spin && c->pack), c->d_jl, c->jl_cnt[0],
+           c->jl_cnt[1]);
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
   //                    + 1/2 vovv.t2 + 1/2 t2.ooov)

@@ -240,12 +240,15 @@ void ring_assemble(double *Wr, const double *ovvo_ring, const double *Q1,
 // only; alpha runs [0, zh) and [0, zo) with beta runs of the same lengths
 // after them): Z holds the rows of an alpha first index only, the others
 // are read from their spin-flipped images.  zero_forbidden = false (packed
-// only): the spin-forbidden doubles of t2bar are left unwritten.
+// only): the spin-forbidden doubles of t2bar are left unwritten.  packed
+// needs jl: the occupied indices of label 0 (jc0 of them), then those of
+// label 1 (jc1).
 void r2_final(double *t2bar, const double *acc1, const double *acc2,
               const double *S1, const double *S2, const double *Z, int nv,
               int no, int nb, hipStream_t s, const int *lab_v,
               const int *lab_o, const double *add, bool packed, int zh = 0,
-              int zo = 0, bool zero_forbidden = true);
+              int zo = 0, bool zero_forbidden = true,
+              const int *jl = nullptr, int jc0 = 0, int jc1 = 0);
 // Spin-flip images of a ring tensor X[nb][nv][no][nv][no] (slice stride
 // sb) whose rows with a virtual alpha index are computed: every allowed
 // element with that index beta is copied from its flipped partner (alpha

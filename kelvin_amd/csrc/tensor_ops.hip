@@ -372,7 +372,8 @@ __device__ inline int64_t xcd_grouped(int64_t bid, int64_t G) {
 __global__ __launch_bounds__(256) void r2_packed_kernel(
     double *t2bar, const double *acc1, const double *acc2, const double *S1,
     const double *S2, const double *Z, const double *add, int nv, int no,
-    const int *lv, const int *lo, int zh, int zo, int zero_forbidden) {
+    const int *lv, const int *lo, int zh, int zo, int zero_forbidden,
+    const int *jl, int jc0, int jc1) {
   extern __shared__ double sm[];   // 4 strips of no * R2P_LD
   double *tS2 = sm, *tS1 = sm + no * R2P_LD, *tZ = sm + 2 * no * R2P_LD,
          *tP = sm + 3 * no * R2P_LD;
@@ -422,49 +423,77 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
     if (lo[i] == lo[j]) return i < j ? 1 : (i > j ? -1 : 0);
     return lo[i] == 0 ? 1 : -1;
   };
-  // stage the values at (j, i) for the block's rows i, contiguous along i
-  for (int t = threadIdx.x; t < no * R2P_ROWS; t += blockDim.x) {
-    const int j = t / R2P_ROWS, il = t % R2P_ROWS, i = i0 + il;
-    const bool ok = !zab && i < no && lo[i] + lo[j] == sab;
-    const bool tr = ok && ij_sign(i, j) < 0;
-    const int64_t e = nat(pa, pb, j, i);
-    tS2[j * R2P_LD + il] = ok ? s2[e] : 0.0;
-    tS1[j * R2P_LD + il] = tr ? s1[nat(a, b, j, i)] - s1[nat(b, a, j, i)] : 0.0;
-    tZ[j * R2P_LD + il] = ok ? zr(b, j, a, i) - zr(a, j, b, i) : 0.0;
-    tP[j * R2P_LD + il] = tr ? c1[e] + c2[e] : 0.0;
-  }
-  __syncthreads();
-  const int lanes = blockDim.x / 2;   // two rows at a time
-  for (int il = threadIdx.x / lanes; il < R2P_ROWS; il += 2) {
+  // stage the values at (j, i) for the block's rows i, contiguous along i;
+  // only the spin-allowed (i, j) are staged, the only ones read below: row
+  // i allows the occupied j of label sab - lo[i] (jl lists them by label)
+  const int cmax = jc0 > jc1 ? jc0 : jc1;
+  auto allowed_j = [&](int il, int k) {   // k-th allowed j of row il, or -1
     const int i = i0 + il;
-    if (i >= no) break;
-    for (int j = threadIdx.x % lanes; j < no; j += lanes) {
-      const int64_t nab = nat(a, b, i, j), nba = nat(b, a, i, j);
-      if (zab || lo[i] + lo[j] != sab) {
-        // a == b is allowed for some (i,j) and always zero: written
-        if (zab || zero_forbidden) {
-          out[nab] = 0.0;
-          out[nba] = 0.0;
+    if (i >= no) return -1;
+    const int sj = sab - lo[i];
+    if (sj < 0 || sj > 1 || k >= (sj ? jc1 : jc0)) return -1;
+    return jl[sj ? jc0 + k : k];
+  };
+  if (!zab)
+    for (int t = threadIdx.x; t < cmax * R2P_ROWS; t += blockDim.x) {
+      const int il = t % R2P_ROWS, j = allowed_j(il, t / R2P_ROWS);
+      if (j < 0) continue;
+      const int i = i0 + il;
+      const bool tr = ij_sign(i, j) < 0;
+      const int64_t e = nat(pa, pb, j, i);
+      tS2[j * R2P_LD + il] = s2[e];
+      tS1[j * R2P_LD + il] =
+          tr ? s1[nat(a, b, j, i)] - s1[nat(b, a, j, i)] : 0.0;
+      tZ[j * R2P_LD + il] = zr(b, j, a, i) - zr(a, j, b, i);
+      tP[j * R2P_LD + il] = tr ? c1[e] + c2[e] : 0.0;
+    }
+  __syncthreads();
+  // one spin-allowed element (a != b): R2[a,b,i,j] and R2[b,a,i,j]
+  auto element = [&](int il, int i, int j) {
+    const int64_t nab = nat(a, b, i, j), nba = nat(b, a, i, j);
+    const int sij = ij_sign(i, j);
+    const int64_t e = nat(pa, pb, i, j);
+    double acc = 0.0, s1d = 0.0;
+    if (sij > 0) {
+      acc = c1[e] + c2[e];
+      s1d = s1[nab] - s1[nba];
+    } else if (sij < 0) {
+      acc = -tP[j * R2P_LD + il];
+      s1d = -tS1[j * R2P_LD + il];
+    }
+    const double s2d = s2[e] - tS2[j * R2P_LD + il];
+    const double zz = zr(a, i, b, j) - zr(b, i, a, j) + tZ[j * R2P_LD + il];
+    double v = sg * (acc + s2d) + s1d + zz;
+    if (add) v += add[nab];
+    out[nab] = -v;
+    out[nba] = v;
+  };
+  if (zero_forbidden) {
+    const int lanes = blockDim.x / 2;   // two rows at a time
+    for (int il = threadIdx.x / lanes; il < R2P_ROWS; il += 2) {
+      const int i = i0 + il;
+      if (i >= no) break;
+      for (int j = threadIdx.x % lanes; j < no; j += lanes) {
+        if (zab || lo[i] + lo[j] != sab) {
+          out[nat(a, b, i, j)] = 0.0;
+          out[nat(b, a, i, j)] = 0.0;
+          continue;
         }
+        element(il, i, j);
+      }
+    }
+  } else {
+    // the allowed (i, j) only, rows after one another along the threads (a
+    // == b is allowed for some (i,j) and always zero: written)
+    for (int t = threadIdx.x; t < R2P_ROWS * cmax; t += blockDim.x) {
+      const int il = t / cmax, j = allowed_j(il, t % cmax);
+      if (j < 0) continue;
+      const int i = i0 + il;
+      if (zab) {
+        out[nat(a, b, i, j)] = 0.0;
         continue;
       }
-      const int sij = ij_sign(i, j);
-      const int64_t e = nat(pa, pb, i, j);
-      double acc = 0.0, s1d = 0.0;
-      if (sij > 0) {
-        acc = c1[e] + c2[e];
-        s1d = s1[nab] - s1[nba];
-      } else if (sij < 0) {
-        acc = -tP[j * R2P_LD + il];
-        s1d = -tS1[j * R2P_LD + il];
-      }
-      const double s2d = s2[e] - tS2[j * R2P_LD + il];
-      const double zz =
-          zr(a, i, b, j) - zr(b, i, a, j) + tZ[j * R2P_LD + il];
-      double v = sg * (acc + s2d) + s1d + zz;
-      if (add) v += add[nab];
-      out[nab] = -v;
-      out[nba] = v;
+      element(il, i, j);
     }
   }
 }
@@ -662,12 +691,15 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
               const double *S1, const double *S2, const double *Z, int nv,
               int no, int nb, hipStream_t s, const int *lab_v,
               const int *lab_o, const double *add, bool packed, int zh,
-              int zo, bool zero_forbidden) {
+              int zo, bool zero_forbidden, const int *jl, int jc0,
+              int jc1) {
   require(!packed || (lab_v && lab_o), "packed pairs need spin labels");
   require(zh == 0 || (packed && 2 * zh == nv && 2 * zo == no),
           "r2 assembly: flipped Z needs the packed layout and equal halves");
   require(zero_forbidden || packed,
           "r2 assembly: skipping forbidden doubles needs the packed layout");
+  require(!packed || (jl && jc0 >= 0 && jc1 >= 0 && jc0 + jc1 == no),
+          "r2 assembly: packed pairs need the occupied lists by label");
   const int tiles = (no + 31) / 32;
   const int64_t npair = (int64_t)nv * (nv + 1) / 2;
   const int64_t blocks = (int64_t)nb * npair * tiles * tiles;
@@ -687,7 +719,8 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
     }
     hipLaunchKernelGGL(r2_packed_kernel, dim3((unsigned)pblocks), dim3(256),
                        lds, s, t2bar, acc1, acc2, S1, S2, Z, add, nv, no,
-                       lab_v, lab_o, zh, zo, zero_forbidden ? 1 : 0);
+                       lab_v, lab_o, zh, zo, zero_forbidden ? 1 : 0, jl, jc0,
+                       jc1);
   } else
     hipLaunchKernelGGL(r2_final_pair_kernel, dim3((unsigned)blocks),
                        dim3(256), 0, s, t2bar, acc1, acc2, S1, S2, Z, add, nv,
