@@ -394,6 +394,13 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
   double *out = t2bar + y * per;
   const int i0 = ti * R2P_ROWS;
   const int sab = lv[a] + lv[b];
+  // spin-flip symmetric step (zh > 0, forbidden doubles not written): the
+  // pairs of two beta virtuals are the images of the alpha-alpha pairs,
+  // whose workgroups write them
+  const bool fimage = zh > 0 && !zero_forbidden;
+  if (fimage && a >= zh) return;   // a <= b: both beta
+  const bool wimage = fimage && b < zh;   // both alpha
+  auto flo = [&](int k) { return k < zo ? k + zo : k - zo; };
   auto nat = [&](int p, int r, int k, int l) {
     return (((int64_t)p * nv + r) * no + k) * no + l;
   };
@@ -467,6 +474,10 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
     if (add) v += add[nab];
     out[nab] = -v;
     out[nba] = v;
+    if (wimage) {
+      out[nat(a + zh, b + zh, flo(i), flo(j))] = -v;
+      out[nat(b + zh, a + zh, flo(i), flo(j))] = v;
+    }
   };
   if (zero_forbidden) {
     const int lanes = blockDim.x / 2;   // two rows at a time
@@ -491,6 +502,7 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
       const int i = i0 + il;
       if (zab) {
         out[nat(a, b, i, j)] = 0.0;
+        if (wimage) out[nat(a + zh, b + zh, flo(i), flo(j))] = 0.0;
         continue;
       }
       element(il, i, j);
