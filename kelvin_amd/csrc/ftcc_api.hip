@@ -130,6 +130,11 @@ struct ftcc_ctx {
   // assignments with x beta and y alpha are skipped: the packed ladder
   // operands never read that class of a pair; 0 = none
   char skip_ba[2] = {0, 0};
+  // output index letters (x, y) whose assignments with both beta are
+  // skipped (letters of a pair index allowed); 0 = none
+  char skip_bb[2] = {0, 0};
+  // the ladder products skip their beta-beta | beta-beta block
+  bool ladder_skip_bb = false;
   // set by ftcc_step around its residual: its update reads the spin-allowed
   // doubles of Tbar only, so the assembly leaves the forbidden ones unwritten
   bool tbar_scratch = false;
@@ -450,23 +455,30 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
         in_c |= (x.role == 'o' || x.role == 'v') && x.l == lead;
     require(in_c, "spin-restricted GEMM: flip lead is not an output index");
   }
-  // an output pair of class beta-alpha that no reader reads: both letters
-  // index C
+  // output pairs of class beta-alpha (skip_ba) or beta-beta (skip_bb) that
+  // no reader reads: both letters index C
   const char bx = c->skip_ba[0], by = c->skip_ba[1];
-  if (bx) {
+  const char qx = c->skip_bb[0], qy = c->skip_bb[1];
+  auto in_c = [&](char l, bool pairs) {
     int found = 0;
     for (const auto *v : {&Mi, &Ni})
       for (const SIdx &x : *v)
-        found += (x.role == 'o' || x.role == 'v') && (x.l == bx || x.l == by);
-    require(found == 2, "spin-restricted GEMM: skipped pair is not in C");
-  }
+        found += ((x.role == 'o' || x.role == 'v') && x.l == l) ||
+                 (pairs && is_pair(x) && (x.l == l || x.l2 == l));
+    return found == 1;
+  };
+  require(!bx || (in_c(bx, false) && in_c(by, false)),
+          "spin-restricted GEMM: skipped pair is not in C");
+  require(!qx || (in_c(qx, true) && in_c(qy, true)),
+          "spin-restricted GEMM: skipped pair is not in C");
+  auto spin_of = [](const std::map<char, int> &sp, char l) {
+    const auto it = sp.find(l);
+    return it == sp.end() ? -1 : it->second;
+  };
   auto lead_beta = [&](const std::map<char, int> &sp) {
-    const auto it = lead ? sp.find(lead) : sp.end();
-    if (it != sp.end() && it->second == 1) return true;
-    if (!bx) return false;
-    const auto ix = sp.find(bx), iy = sp.find(by);
-    return ix != sp.end() && iy != sp.end() && ix->second == 1 &&
-           iy->second == 0;
+    if (lead && spin_of(sp, lead) == 1) return true;
+    if (bx && spin_of(sp, bx) == 1 && spin_of(sp, by) == 0) return true;
+    return qx && spin_of(sp, qx) == 1 && spin_of(sp, qy) == 1;
   };
   // a pair index runs over the packed pairs of its spin class (x < y for
   // equal spins, every mixed pair once, none for the beta-alpha class);
@@ -986,6 +998,8 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
     blk[0].r = group(R, 0);
     blk[2].r = Range();
   }
+  // the beta-beta | beta-beta block of a ladder product that nothing reads
+  if (kind == LADDER && c->ladder_skip_bb) blk[2].r = Range();
   // a packed ladder K sums one pair of each antisymmetric couple: twice
   const double alpha = (kind == LADDER && op_k(c, ia) >= 3) ? 2.0 * g.alpha
                                                             : g.alpha;
@@ -1293,6 +1307,11 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   require(!flip || s5, "flip symmetry needs the spin-restricted terms");
   const int hv = c->v_count[0], ho = c->o_count[0];
   auto lead = [&](char l) { c->flip_lead = flip ? l : 0; };
+  // ftcc_step's assembly under the flip symmetry reads nothing of a pair of
+  // two beta virtuals (their R2 is written as the image of the alpha-alpha
+  // pair): the beta-beta | beta-beta blocks of the ladder products, of
+  // W_mnij and W_abef, and the beta-beta (a,b) of S1 and S2 are skipped
+  const bool flip_step = flip && c->pack && c->tbar_scratch;
   auto sgem = [&](double alpha, double beta, const double *A, int64_t sA1,
                   const double *B, int64_t sB1, double *C, int64_t sC1,
                   const double *D) {
@@ -1458,9 +1477,11 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // onto it in the FP64 GEMM.  In the spin layout the product writes the
   // allowed blocks only, and nothing reads the others.
   // Wmnij = oooo + P(ij) t_j^e <mn||ie> + 1/2 oovv.tau          (G1)
+  c->ladder_skip_bb = flip_step;
   product(c, mk((int)OO, (int)OO, (int)VV, 0.5, 0.0, oovv, VV, 1, 0, c->tau,
                 OO, 1, S2, c->wmnij, OO, 1, O4, nb), OP_OOVV_A, OP_TAU_B,
           LADDER);
+  c->ladder_skip_bb = false;
   // + oooo + sum_e <mn||ie> t_j^e into [(m,n,i), j]; with packed pairs
   // only the stored (m,n) of each couple (all the G2 operand reads)
   if (s5) {
@@ -1472,6 +1493,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     Mi.push_back({'i', 'o', V, 0, O});
     // G2 reads the stored (i,j) of each couple only: no (i beta, j alpha)
     if (c->pack) c->skip_ba[0] = 'i', c->skip_ba[1] = 'j';
+    c->flip_lead = flip_step ? 'i' : 0;   // i beta: (m,n) and (i,j) beta-beta
     run_spin(c, sgem(1.0, 1.0, ooov, 0, t1, VO, c->wmnij, O4, oooo), P5, Mi,
              {{'y', 'y', 0, VO, O4}, {'j', 'o', 0, 1, 1}}, {'e', 'v', 1, O, 0},
              "mn|ie", "e|j", "mn|ij");
@@ -1487,6 +1509,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              {{'y', 'y', VO, 0, O4}, {'i', 'o', 1, 0, O}}, Ni,
              {'e', 'v', O, 1, 0}, "e|i", "mn|je", "mn|ij");
     c->skip_ba[0] = c->skip_ba[1] = 0;
+    c->flip_lead = 0;
   } else {
     Gemm g = mk((int)(OO * O), O, V, 1.0, 1.0, ooov, V, 1, 0, t1, O, 1, VO,
                 c->wmnij, O, 1, O4, nb);
@@ -1515,6 +1538,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     Ni.insert(Ni.end(), ef.begin(), ef.end());
     // G3 reads the stored (a,b) of each couple only: no (a beta, b alpha)
     if (c->pack) c->skip_ba[0] = 'a', c->skip_ba[1] = 'b';
+    c->flip_lead = flip_step ? 'a' : 0;   // a beta: (a,b) and (e,f) beta-beta
     run_spin(c, sgem(-1.0, 0.0, t1, VO, vovv, 0, c->wabef, V4, vvvv), P5,
              {{'y', 'y', VO, 0, V4}, {'b', 'v', O, 0, VV}}, Ni,
              {'m', 'o', 1, VV, 0}, "b|m", "am|ef", "ab|ef");
@@ -1525,6 +1549,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
              {{'y', 'y', VO, 0, V4}, {'a', 'v', O, 0, V * VV}}, Ni,
              {'m', 'o', 1, VV, 0}, "a|m", "bm|ef", "ab|ef");
     c->skip_ba[0] = c->skip_ba[1] = 0;
+    c->flip_lead = 0;
   } else {
     // vvvv - sum_m t_m^b <am||ef>: per a, C[b][ef] = vvvv - t1[b,m] vovv[a,m,ef]
     Gemm g = mk(V, (int)VV, O, -1.0, 0.0, t1, O, 1, VO, vovv, VV, 1, 0,
@@ -1634,12 +1659,14 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // the assembly adds them, vvoo and the rest into t2bar
   put_operand(c, OP_WMNIJ_B, c->wmnij, 1, OO, O4, nb);
   put_operand(c, OP_WABEF_A, c->wabef, VV, 1, V4, nb);
+  c->ladder_skip_bb = flip_step;
   product(c, mk((int)VV, (int)OO, (int)OO, 0.5, 0.0, c->tau, OO, 1, S2,
                 c->wmnij, OO, 1, O4, c->wr, OO, 1, S2, nb), OP_TAU_A,
           OP_WMNIJ_B, LADDER);                                           // G2
   product(c, mk((int)VV, (int)OO, (int)VV, 0.5, 0.0, c->wabef, VV, 1, V4,
                 c->tau, OO, 1, S2, c->aring, OO, 1, S2, nb), OP_WABEF_A,
           OP_TAU_B, LADDER);                                             // G3
+  c->ladder_skip_bb = false;
 
   // S1 (antisymmetrised in ab): Fbe[b,e] t2[a,e,i,j] + t1[a,m] vooo[b,m,i,j]
   // S2 (antisymmetrised in ij): -t2[a,b,i,m] Fmj[m,j] + t1[e,i] vvvo[a,b,e,j]
@@ -1651,6 +1678,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
                 : std::vector<SIdx>{{'i', 'o', 0, O, O}, {'j', 'o', 0, 1, 1}};
     std::vector<SIdx> Ni{{'a', 'v', 0, V * OO, V * OO}};
     Ni.insert(Ni.end(), ij.begin(), ij.end());
+    if (flip_step) c->skip_bb[0] = 'a', c->skip_bb[1] = 'b';
     run_spin(c, sgem(1.0, 0.0, c->Fbe, VV, t2, S2, c->s1, S2, nullptr), P5,
              {{'b', 'v', V, 0, OO}}, Ni, {'e', 'v', 1, OO, 0}, "b|e",
              "ae|ij", "ab|ij");
@@ -1674,6 +1702,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run_spin(c, sgem(1.0, 1.0, t1, VO, vvvo, 0, c->s2, S2, nullptr), P5,
              {{'y', 'y', VO, 0, S2}, {'i', 'o', 1, 0, O}}, Ni,
              {'e', 'v', O, O, 0}, "e|i", "ab|ej", "ab|ij");
+    c->skip_bb[0] = c->skip_bb[1] = 0;
   } else {
     Gemm g = mk(V, (int)OO, V, 1.0, 0.0, c->Fbe, V, 1, VV, t2, OO, 1, S2,
                 c->s1, OO, 1, S2, nb);
@@ -1838,6 +1867,8 @@ bool residual_all(ftcc_ctx *c, const double *T1, const double *T2,
       c->flip_now = false;
       c->flip_lead = 0;
       c->skip_ba[0] = c->skip_ba[1] = 0;
+      c->skip_bb[0] = c->skip_bb[1] = 0;
+      c->ladder_skip_bb = false;
     }
   } restore{c, spin_saved};
   const int O = c->no, V = c->nv;
