@@ -396,10 +396,17 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
   const int sab = lv[a] + lv[b];
   // spin-flip symmetric step (zh > 0, forbidden doubles not written): the
   // pairs of two beta virtuals are the images of the alpha-alpha pairs,
-  // whose workgroups write them
+  // whose workgroups write them; half of the mixed pairs likewise
   const bool fimage = zh > 0 && !zero_forbidden;
   if (fimage && a >= zh) return;   // a <= b: both beta
   const bool wimage = fimage && b < zh;   // both alpha
+  // a mixed pair (a alpha, b beta) and (b - zh, a + zh) are images of each
+  // other: the one with a < b - zh writes both (a == b - zh: itself only)
+  bool mimage = false;
+  if (fimage && b >= zh) {
+    if (a > b - zh) return;
+    mimage = a < b - zh;
+  }
   auto flo = [&](int k) { return k < zo ? k + zo : k - zo; };
   auto nat = [&](int p, int r, int k, int l) {
     return (((int64_t)p * nv + r) * no + k) * no + l;
@@ -477,6 +484,10 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
     if (wimage) {
       out[nat(a + zh, b + zh, flo(i), flo(j))] = -v;
       out[nat(b + zh, a + zh, flo(i), flo(j))] = v;
+    }
+    if (mimage) {
+      out[nat(a + zh, b - zh, flo(i), flo(j))] = -v;
+      out[nat(b - zh, a + zh, flo(i), flo(j))] = v;
     }
   };
   if (zero_forbidden) {
