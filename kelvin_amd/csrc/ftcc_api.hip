@@ -135,6 +135,9 @@ struct ftcc_ctx {
   char skip_bb[2] = {0, 0};
   // the ladder products skip their beta-beta | beta-beta block
   bool ladder_skip_bb = false;
+  // ftcc_step under the flip symmetry (for the chunk being evaluated): the
+  // ladder operands' beta-beta blocks are read by no product
+  bool flip_step_now = false;
   // set by ftcc_step around its residual: its update reads the spin-allowed
   // doubles of Tbar only, so the assembly leaves the forbidden ones unwritten
   bool tbar_scratch = false;
@@ -795,6 +798,8 @@ void operand_blocks(const ftcc_ctx *c, int id, Range rows[3], Range ks[3]) {
     rows[0] = group(R, 0), ks[0] = group(K, 0);
     rows[1] = group(R, 1, 2), ks[1] = group(K, 1, 2);
     rows[2] = group(R, 3), ks[2] = group(K, 3);
+    // (the fixed integral operand is gathered once for every later call)
+    if (c->flip_step_now && !OP_FIXED[id]) rows[2] = Range(), ks[2] = Range();
     return;
   }
   rows[0] = group(R, 0, 3), ks[0] = group(K, 0, 3);
@@ -808,9 +813,13 @@ void operand_blocks(const ftcc_ctx *c, int id, Range rows[3], Range ks[3]) {
   if (id == OP_WR_B) {
     rows[1] = group(R, 1), ks[1] = group(K, 1);
     rows[2] = group(R, 2), ks[2] = group(K, 2);
+    // G5 over the alpha-virtual rows never reads the class-1 columns
+    if (c->flip_now) rows[1] = Range(), ks[1] = Range();
   } else {
     rows[1] = group(R, 1), ks[1] = group(K, 2);
     rows[2] = group(R, 2), ks[2] = group(K, 1);
+    // G4 over the alpha-virtual rows never reads the class-2 columns of B
+    if (c->flip_now && id == OP_ARING_B) rows[2] = Range(), ks[2] = Range();
   }
 }
 
@@ -1312,6 +1321,7 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   // pair): the beta-beta | beta-beta blocks of the ladder products, of
   // W_mnij and W_abef, and the beta-beta (a,b) of S1 and S2 are skipped
   const bool flip_step = flip && c->pack && c->tbar_scratch;
+  c->flip_step_now = flip_step;
   auto sgem = [&](double alpha, double beta, const double *A, int64_t sA1,
                   const double *B, int64_t sB1, double *C, int64_t sC1,
                   const double *D) {
@@ -1869,6 +1879,7 @@ bool residual_all(ftcc_ctx *c, const double *T1, const double *T2,
       c->skip_ba[0] = c->skip_ba[1] = 0;
       c->skip_bb[0] = c->skip_bb[1] = 0;
       c->ladder_skip_bb = false;
+      c->flip_step_now = false;
     }
   } restore{c, spin_saved};
   const int O = c->no, V = c->nv;
