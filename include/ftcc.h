@@ -171,8 +171,10 @@ int ftcc_sumsq(ftcc_ctx *ctx, const double *x_dev, int64_t n, double *out);
  *   d1 = |T1 - T1old|^2, d2 = |T2 - T2old|^2
  *   Told <- damp*Told + (1-damp)*T      (in place in T1old_dev / T2old_dev)
  *   n1 = |T1old|^2, n2 = |T2old|^2, energy = ft_cc_energy(T1old, T2old, g)
- * T1bar_dev / T2bar_dev are caller-provided scratch of amplitude size.
- * out5 receives {d1, d2, n1, n2, energy}. */
+ * T1bar_dev / T2bar_dev are caller-provided scratch of amplitude size:
+ * with spin blocking on, the spin-forbidden doubles of T2bar are left as
+ * they were (the update skips them), and the whole T2bar of the last call
+ * is unspecified.  out5 receives {d1, d2, n1, n2, energy}. */
 int ftcc_step(ftcc_ctx *ctx, double *T1old_dev, double *T2old_dev,
               double *T1bar_dev, double *T2bar_dev, const double *g_host,
               double beta, double damp, uint64_t skip_mask, double out5[5]);
