@@ -871,6 +871,24 @@ void put_operand(ftcc_ctx *c, int id, const double *src, int64_t sr,
       });
 }
 
+// G5's B operand Wr of a spin-flip symmetric residual, whose rows of an
+// alpha e only are computed: the beta ones are read from their images
+void put_flip_wr(ftcc_ctx *c, const double *wr, int64_t sb, int nb) {
+  const int V = c->nv, O = c->no, hv = c->v_count[0], ho = c->o_count[0];
+  hipStream_t s = c->stream;
+  gather_blocks(
+      c, OP_WR_B,
+      [&](const int *rmap, int R, const int *kmap, int W, const SplitMat &m) {
+        flip_ring_gather(wr, V, O, hv, ho, sb, rmap, R, kmap, W, &m, 0,
+                         nullptr, 0, nb, s);
+      },
+      [&](const int *rmap, int R, const int *kmap, int W, int64_t ld,
+          double *dst, int64_t db) {
+        flip_ring_gather(wr, V, O, hv, ho, sb, rmap, R, kmap, W, nullptr, ld,
+                         dst, db, nb, s);
+      });
+}
+
 // Spin layout: operand `id` gathered straight from the amplitudes t1, t2 of
 // nb slices (tau, t2 in ring order, the ring A of G4: see AmpKind)
 void put_amp_operand(ftcc_ctx *c, int id, int kind, const double *t1,
@@ -1624,10 +1642,13 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     run(c, g, P5);
   }
 
-  if (flip) flip_rows(c, c->wr, nb, false);
-
   // Z[ai,bj] = t2r.Wr - t_i^e t_m^a <mb||ej>                    (G5)
-  put_operand(c, OP_WR_B, c->wr, 1, VO, S2, nb);
+  // (under the flip symmetry the gather reads Wr's beta rows from their
+  // images: nothing else reads them)
+  if (flip)
+    put_flip_wr(c, c->wr, S2, nb);
+  else
+    put_operand(c, OP_WR_B, c->wr, 1, VO, S2, nb);
   // spin blocks leave the spin-forbidden part of Z unwritten; the
   // spin-aware assembly below never reads it
   product(c, mk((int)VO, (int)VO, (int)VO, 1.0, 0.0, c->t2r, VO, 1, S2, c->wr,

@@ -122,6 +122,15 @@ void split_gather(const double *src, int64_t sr, int64_t sc, int64_t sb,
 void gather64(const double *src, int64_t sr, int64_t sc, int64_t sb,
               const int *rmap, int R, const int *cmap, int W, int64_t os,
               double *dst, int64_t db, int nb, hipStream_t s);
+// The gather of a ring tensor W[nb][nv][no][nv][no] (slice stride sb) by
+// rows (b,j) and columns (e,m) whose rows of an alpha e only are stored
+// (spin-flip symmetric problem; alpha runs [0, hv) and [0, ho), beta runs
+// after them): a beta e is read from its image.  Into the split pair *d
+// when d is given, else into FP64 dst as gather64.
+void flip_ring_gather(const double *src, int nv, int no, int hv, int ho,
+                      int64_t sb, const int *rmap, int R, const int *cmap,
+                      int W, const SplitMat *d, int64_t os, double *dst,
+                      int64_t db, int nb, hipStream_t s);
 // Amplitude-derived operands gathered straight from t1 [nb][V][O] and
 // t2 [nb][V][V][O][O] (rows / columns are pair indices x * ny + y):
 //   AMP_TAU_AB  rows (a,b), columns (i,j): t2 + t1[a,i]t1[b,j] - t1[b,i]t1[a,j]

@@ -278,6 +278,28 @@ struct PlainSrc {
     return src[b * sb + (int64_t)rr * sr + (int64_t)cc * sc];
   }
 };
+// Ring tensor W[e][m][b][j] (slice stride sb) of a spin-flip symmetric
+// problem whose rows of an alpha e are stored (alpha runs [0, hv), [0, ho),
+// beta runs after them), read by rows (b,j) and columns (e,m) (pair indices
+// x * no + y): an element of a beta e is read from its image
+// W[e - hv][fl m][fl b][fl j].
+struct FlipRingSrc {
+  const double *src;
+  int64_t sb;
+  int nv, no, hv, ho;
+  __device__ double operator()(int64_t b, int rr, int cc) const {
+    int e = cc / no, m = cc - e * no;
+    int q = rr / no, l = rr - q * no;
+    if (e >= hv) {
+      e -= hv;
+      m = m < ho ? m + ho : m - ho;
+      q = q < hv ? q + hv : q - hv;
+      l = l < ho ? l + ho : l - ho;
+    }
+    return src[b * sb + ((int64_t)e * no + m) * ((int64_t)nv * no) +
+               (int64_t)q * no + l];
+  }
+};
 // Amplitude-derived operands computed from t1 [V][O] and t2 [V][V][O][O]
 // of each slice as they are gathered (nothing is materialised):
 //   AMP_TAU_AB: rows (a,b), columns (i,j): tau = t2 + t1 t1 - t1 t1
@@ -453,6 +475,22 @@ void amp_gather64(int kind, const double *t1, const double *t2, int nv,
   bool ar;
   const AmpSrc src_ = amp(kind, t1, t2, nv, no, ar);
   gather_launch(src_, ar, rmap, R, cmap, W, W, os, db, F64Out{dst}, nb, s);
+}
+
+void flip_ring_gather(const double *src, int nv, int no, int hv, int ho,
+                      int64_t sb, const int *rmap, int R, const int *cmap,
+                      int W, const SplitMat *d, int64_t os, double *dst,
+                      int64_t db, int nb, hipStream_t s) {
+  require(2 * hv == nv && 2 * ho == no, "flip gather needs equal spin halves");
+  require(rmap && cmap, "flip gather reads through pair maps");
+  const FlipRingSrc src_{src, sb, nv, no, hv, ho};
+  if (d) {
+    require(d->ld % 32 == 0 && W <= d->ld, "split operand leading dimension");
+    gather_launch(src_, true, rmap, R, cmap, W, W, d->ld, d->bstride,
+                  SplitOut{d->hi, d->lo}, nb, s);
+  } else {
+    gather_launch(src_, true, rmap, R, cmap, W, W, os, db, F64Out{dst}, nb, s);
+  }
 }
 
 void gather64(const double *src, int64_t sr, int64_t sc, int64_t sb,
