@@ -100,9 +100,11 @@ int ftcc_set_spin(ftcc_ctx *ctx, const int *lab_o_host,
  * iterate is); 8 the system is symmetric under the spin flip (equal alpha
  * and beta runs; every integral, Fock block and orbital energy unchanged
  * when each index moves to its partner of the other spin): for trusted
- * amplitudes (ftcc_set_trusted) the two ring products compute the output
- * rows with an alpha virtual index and copy the others from their flipped
- * images.  FTCC_SPIN5=0, FTCC_PACK=0 and FTCC_FLIP=0 in the environment at
+ * amplitudes (ftcc_set_trusted) the two ring products and the
+ * spin-restricted terms of the Fock-type, ring and singles intermediates
+ * compute the blocks of an alpha lead index and take the others from their
+ * flipped images, and ftcc_step also skips the beta-beta blocks its update
+ * never reads.  FTCC_SPIN5=0, FTCC_PACK=0 and FTCC_FLIP=0 in the environment at
  * ftcc_set_system/ftcc_set_spin time switch off flags 2, 4 and 8. */
 int ftcc_get_spin(ftcc_ctx *ctx, int *on);
 
