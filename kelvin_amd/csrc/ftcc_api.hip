@@ -117,14 +117,14 @@ struct ftcc_ctx {
   // Spin-flip symmetry (flip = 1): equal alpha and beta runs and every
   // integral, Fock block and orbital energy unchanged when each index moves
   // to its partner of the other spin.  Trusted amplitudes (the solver's own
-  // iterates) then share the symmetry, and the ring products compute the
-  // output rows with an alpha virtual index only (flip_now: in force for
-  // the residual being evaluated).
+  // iterates) then share the symmetry, and the ring products and the
+  // alpha-lead O(n^5) terms compute half their blocks, the rest being their
+  // flipped images (flip_now: in force for the residual being evaluated).
   int flip = 0;
   bool flip_now = false;
   // under flip_now: an output index letter of the spin-restricted GEMMs
   // whose beta assignments are skipped (their blocks are flipped images,
-  // copied once the output is complete); 0 = none
+  // copied or read in place once the output is complete); 0 = none
   char flip_lead = 0;
   // output index letters (x, y) of the spin-restricted GEMMs whose
   // assignments with x beta and y alpha are skipped: the packed ladder
@@ -1018,8 +1018,9 @@ void product(ftcc_ctx *c, const Gemm &g, int ia, int ib, int kind) {
   Blk blk[3];
   block_list(kind, R, K, Cs, blk);
   // spin-flip symmetry: the ring products compute the rows whose virtual
-  // index is alpha (classes 00 and 01 of the ring pairs); the caller copies
-  // the others from their flipped images once the output is complete
+  // index is alpha (classes 00 and 01 of the ring pairs); the others are
+  // read from their flipped images once the output is complete (G5's B
+  // gather for Wr, the assembly for Z)
   const bool flip = c->flip_now && kind != LADDER;
   if (flip) {
     blk[0].r = group(R, 0);
@@ -1328,8 +1329,9 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   const bool s5 = c->spin && c->spin5;
   // Under the spin-flip symmetry (flip_now, spin5 implied) the spin-
   // restricted terms of FmeT, Fae, Fmi, Wr, Z and t1bar compute the blocks
-  // of an alpha lead index only (flip_lead), and the beta blocks of each are
-  // copied from their images once the tensor is complete
+  // of an alpha lead index only (flip_lead); once each tensor is complete
+  // its beta blocks are copied from their images (the Fock-type matrices
+  // and t1bar) or read from them in place (Wr, Z)
   const bool flip = c->flip_now;
   require(!flip || s5, "flip symmetry needs the spin-restricted terms");
   const int hv = c->v_count[0], ho = c->o_count[0];
