@@ -169,8 +169,9 @@ class FTEngine:
     def spin_mode(self):
         """0: dense; else bit flags: 1 O(n^6) products as spin blocks, 2 the
         O(n^5) contractions over spin-allowed blocks only, 4 ladder products
-        over packed antisymmetric pairs, 8 a spin-flip symmetric system (the
-        ring products of trusted amplitudes compute half their rows)."""
+        over packed antisymmetric pairs, 8 a spin-flip symmetric system (for
+        trusted amplitudes the ring products and the alpha-lead O(n^5) terms
+        compute half their blocks, the rest being their flipped images)."""
         on = ctypes.c_int()
         self._call("ftcc_get_spin", ctypes.byref(on))
         return on.value
