@@ -73,7 +73,9 @@ int ftcc_set_system(ftcc_ctx *ctx, int n, const double *en_dev,
  * two_e_blocks as passed to kelvin/ft_cc_equations.py:96 ccsd_stanton):
  *   F: oo[no,no] ov[no,nv] vo[nv,no] vv[nv,nv]
  *   I: vvvv vvvo vovv vvoo vovo oovv vooo ooov oooo
- * (pointer order exactly as listed; device pointers, copied). */
+ * (pointer order exactly as listed; device pointers, copied).  The orbital
+ * energies of an earlier ftcc_set_system are dropped with its integrals:
+ * give the denominators with ftcc_set_denominators before integrating. */
 int ftcc_set_blocks(ftcc_ctx *ctx, const double *const F_dev[4],
                     const double *const I_dev[9]);
 
@@ -236,8 +238,36 @@ int ftcc_get_chunk(ftcc_ctx *ctx, int *nb);
  * unchanged under the spin flip (both true of every iterate of the solver
  * started from the driving terms), so residual calls skip the counting
  * pass over T, need no host synchronisation and use the flip symmetry.
- * Results are wrong if the promise is broken. */
+ * The first residual-evaluating call after the promise (ftcc_residual,
+ * ftcc_step, the eager first call of ftcc_step_graph, ...) checks it once on
+ * the amplitudes it receives (one host synchronisation; entries must match
+ * their flipped images to 1e-10 relative): if they break it, the promise is
+ * withdrawn and that call returns FTCC_EINVAL without computing.  Later
+ * calls are not checked.  No counterpart in the reference (whose residual
+ * is dense). */
 int ftcc_set_trusted(ftcc_ctx *ctx, int on);
+
+/* Fixed corrections dT1bar_dev[ng][nv][no], dT2bar_dev[ng][nv][nv][no][no]
+ * (device, caller-owned, read by every later step until cleared with NULL,
+ * NULL) added to the residual of ftcc_step / ftcc_step_graph before the
+ * integration: the step then iterates Tbar(T) + dTbar.  With dTbar =
+ * Tbar_fp64(T0) - Tbar_low(T0) a low-precision step reproduces the FP64 step
+ * at T0 (mixed-precision defect correction; no counterpart in the
+ * reference, which iterates in FP64 only). */
+int ftcc_set_correction(ftcc_ctx *ctx, const double *dT1bar_dev,
+                        const double *dT2bar_dev);
+/* The defect of a low-precision residual at T:
+ *   dTbar = Tbar_fp64(T) - Tbar_low(T)      (every grid point, skip_mask as
+ * ftcc_residual), with S1_dev/S2_dev amplitude-size scratch.  Both residuals
+ * are evaluated as ftcc_step evaluates its own, so that a low-precision step
+ * from T with this correction is the FP64 step from T to rounding (the
+ * spin-forbidden doubles of dT2bar, which no step reads, are zero).  The
+ * context's precision is unchanged on return (its workspace is rebuilt for
+ * the other precision in between).  The source of ftcc_set_correction's
+ * dTbar. */
+int ftcc_defect(ftcc_ctx *ctx, const double *T1_dev, const double *T2_dev,
+                int low_precision, double *dT1bar_dev, double *dT2bar_dev,
+                double *S1_dev, double *S2_dev, uint64_t skip_mask);
 
 /* Quadrature weights g_host[ng] for ftcc_step_graph. */
 int ftcc_set_weights(ftcc_ctx *ctx, const double *g_host);

@@ -53,6 +53,9 @@ SIGNATURES = {
     "ftcc_set_chunk": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ftcc_get_chunk": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "ftcc_set_trusted": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ftcc_set_correction": (ctypes.c_int, [_vp, _vp, _vp]),
+    "ftcc_defect": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, _vp,
+                                   _vp, _vp, ctypes.c_uint64]),
     "ftcc_set_weights": (ctypes.c_int, [_vp, _c_double_p]),
     "ftcc_step_graph": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp,
                                        ctypes.c_double, ctypes.c_double,

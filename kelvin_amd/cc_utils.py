@@ -126,6 +126,108 @@ def ft_cc_iter(method, T1old, T2old, F, I, D1, D2, g, G, beta, ng, ti,
     return Eold, T1old, T2old
 
 
+def ft_cc_iter_refined(method, T1old, T2old, F, I, D1, D2, g, G, beta, ng,
+                       ti, iprint, conv_options, low, history=None,
+                       stats=None):
+    """Mixed-precision damped fixed point with an FP64 finish.
+
+    The loop, norms, damping and convergence test are those of
+    ``ft_cc_iter`` (kelvin/cc_utils.py:111-173) in three phases:
+
+    1. residuals in the low precision ``low`` ('fp32' or 'bf16x3') until the
+       relative change falls below tconv or stops decreasing (rounding of
+       the low-precision products sets a floor under it);
+    2. the defect dTbar = Tbar_fp64(T_c) - Tbar_low(T_c) at the current
+       iterate T_c is added to every further low-precision residual, which
+       moves the fixed point from the low-precision one (about 1e-7 Ha away
+       in bf16x3 at n = 114) to the FP64 one, until the change stops
+       decreasing again;
+    3. plain FP64 steps until the reference's test |E - Eold| < econv and
+       res < tconv holds, so that convergence is always certified by an FP64
+       step.
+
+    ``stats`` (a dict) receives the number of steps in each phase.  T1old /
+    T2old are updated in place; returns (Eold, T1old, T2old).
+    """
+    if method != "CCSD":
+        return _other_method(method)
+    eng = ft_cc_equations.engine_of(F, I, D1, D2, ng)
+    eng.set_grid(ti, G)
+    eng.set_precision(low)
+    tbeg = time.time()
+    ethresh = conv_options["econv"]
+    tthresh = conv_options["tconv"]
+    max_iter = conv_options["max_iter"]
+    alpha = conv_options["damp"]
+    T1b = torch.empty_like(T1old)
+    T2b = torch.empty_like(T2old)
+    dT1 = dT2 = None
+    skip = zero_slices(eng, T1old, T2old, G)
+    graph = getattr(eng, "use_graph", False)
+    if graph:
+        eng.set_weights(g)
+    nl1 = math.sqrt(eng.sumsq(T1old)) + 0.1
+    nl2 = math.sqrt(eng.sumsq(T2old)) + 0.1
+    Eold = 888888888.888888888
+    i, phase = 0, 1
+    steps = {1: 0, 2: 0, 3: 0}
+    best, flat = float("inf"), 0
+    converged = False
+    try:
+        while i < max_iter and not converged:
+            if graph:
+                d1, d2, n1, n2, E = eng.step_graph(T1old, T2old, T1b, T2b,
+                                                   beta, alpha, skip)
+            else:
+                d1, d2, n1, n2, E = eng.step(T1old, T2old, T1b, T2b, g, beta,
+                                             alpha, skip)
+            res = math.sqrt(d1) / nl1 + math.sqrt(d2) / nl2
+            nl1 = math.sqrt(n1) + 0.1
+            nl2 = math.sqrt(n2) + 0.1
+            logging.info(' %2d  %.10f   %.4E' % (i + 1, E, res))
+            if history is not None:
+                history.append((E, res))
+            i += 1
+            steps[phase] += 1
+            ok = abs(E - Eold) < ethresh and res < tthresh
+            Eold = E
+            if phase == 3:
+                converged = ok
+                continue
+            if res < 0.9 * best:
+                best, flat = res, 0
+            else:
+                flat += 1
+            if phase == 1 and (ok or res < tthresh or flat >= 5):
+                t = time.time()
+                dT1 = torch.empty_like(T1old)
+                dT2 = torch.empty_like(T2old)
+                eng.defect(T1old, T2old, low, dT1, dT2, T1b, T2b, skip)
+                eng.set_correction(dT1, dT2)
+                phase, best, flat = 2, float("inf"), 0
+                logging.info("FP64 defect at step %d: %.3f s" %
+                             (i, time.time() - t))
+            elif phase == 2 and (ok or (flat >= 4 and steps[2] >= 6)):
+                eng.set_correction(None, None)
+                del dT1, dT2
+                dT1 = dT2 = None
+                eng.set_precision("fp64")
+                phase = 3
+                logging.info("FP64 finish from step %d" % i)
+    finally:
+        eng.set_correction(None, None)
+    if not converged:
+        logging.warning("{} did not converge!".format(method))
+    logging.info("Total {} time: {:.4f} s ({} {} steps, {} corrected, {} "
+                 "FP64)".format(method, time.time() - tbeg, steps[1], low,
+                                steps[2], steps[3]))
+    if stats is not None:
+        stats["steps"] = dict(steps)
+        stats["converged"] = converged
+    del T1b, T2b, dT1, dT2
+    return Eold, T1old, T2old
+
+
 def ft_cc_iter_extrap(method, F, I, D1, D2, g, G, beta, ng, ti,
                       iprint, conv_options):
     """Pointwise schedule (kelvin/cc_utils.py:176-242): converge the grid

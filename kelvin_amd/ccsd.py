@@ -10,8 +10,14 @@ Extensions over the reference (all optional):
   precision   'fp64' (default), 'fp32' or 'bf16x3' for the O(n^6) and
               O(n^5) contractions; Fermi factors, propagators, quadrature,
               damping, norms and energies stay FP64.
-  refine      with a mixed precision, continue in FP64 after convergence
-              until the FP64 iteration converges (default True).
+  refine      with a mixed precision: True (default) iterates in the low
+              precision until the change reaches its rounding floor, then
+              with the FP64 defect (FP64 minus low-precision residual at
+              that iterate) added to every residual, and finishes with FP64
+              steps, so that convergence is certified by an FP64 step
+              (``cc_utils.ft_cc_iter_refined``); "fp64" continues with
+              plain FP64 iterations after the low-precision solve
+              converges; False stops at the low-precision fixed point.
   graph       run the solve on a stream of its own with every fixed-point
               step replayed as a HIP graph (launch overhead of small
               systems) (default False).  Without a guess (T1/T2 on
@@ -277,6 +283,7 @@ class ccsd(object):
                 "CCSD", F, I, D1, D2, g, G, self.beta_max, ng, ti,
                 self.iprint, conv)
             if prec0 != "fp64" and self.refine:
+                # pointwise schedule: plain FP64 iterations of the whole grid
                 logging.info("FP64 refinement after %s iterations" % prec0)
                 eng.set_precision("fp64")
                 conv_r = dict(conv)
@@ -304,11 +311,33 @@ class ccsd(object):
         logging.info('MP2 Energy: {:.10f}'.format(E2))
         self.mp2 = E2
         self.history = []
-        Eccn, T1, T2 = cc_utils.ft_cc_iter(
-            "CCSD", T1old, T2old, F, I, D1, D2, g, G, self.beta_max, ng, ti,
-            self.iprint, conv, history=self.history)
-        if prec0 != "fp64" and self.refine:
+        self.corrections = 0
+        self.phase_steps = None
+        t_mixed = time.time()
+        if prec0 != "fp64" and self.refine is True:
+            # low-precision iterations with FP64 defect corrections; the
+            # accepted step is an FP64 step
+            stats = {}
+            Eccn, T1, T2 = cc_utils.ft_cc_iter_refined(
+                "CCSD", T1old, T2old, F, I, D1, D2, g, G, self.beta_max, ng,
+                ti, self.iprint, conv, prec0, history=self.history,
+                stats=stats)
+            self.phase_steps = stats["steps"]
+            self.corrections = 1 if stats["steps"][2] else 0
+        else:
+            Eccn, T1, T2 = cc_utils.ft_cc_iter(
+                "CCSD", T1old, T2old, F, I, D1, D2, g, G, self.beta_max, ng,
+                ti, self.iprint, conv, history=self.history)
+        torch.cuda.synchronize(eng.device)
+        # the solve in the chosen precision, before any continuation in FP64
+        self.Gcc_unrefined = Eccn
+        self.niter_unrefined = len(self.history)
+        self.time_unrefined = time.time() - t_mixed
+        self.niter_refine = 0
+        self.time_refine = 0.0
+        if prec0 != "fp64" and self.refine == "fp64":
             logging.info("FP64 refinement after %s iterations" % prec0)
+            t_ref = time.time()
             eng.set_precision("fp64")
             conv_r = dict(conv)
             if self.refine_max_iter is not None:
@@ -316,6 +345,9 @@ class ccsd(object):
             Eccn, T1, T2 = cc_utils.ft_cc_iter(
                 "CCSD", T1, T2, F, I, D1, D2, g, G, self.beta_max, ng, ti,
                 self.iprint, conv_r, history=self.history)
+            torch.cuda.synchronize(eng.device)
+            self.niter_refine = len(self.history) - self.niter_unrefined
+            self.time_refine = time.time() - t_ref
         self.T1 = T1
         self.T2 = T2
         self.G0 = E0

@@ -66,6 +66,12 @@ struct ftcc_ctx {
   int nb = 1;
   int nb_fixed = 0;          // grid points per chunk set by the caller (0: auto)
   bool trusted = false;      // caller promises spin-conserving amplitudes
+  // the promise was given and the next residual call checks it once on the
+  // amplitudes it receives
+  bool trust_pending = false;
+  // fixed corrections added to Tbar before the integration of ftcc_step
+  // (caller-owned, amplitude layout; nullptr: none)
+  const double *corr1 = nullptr, *corr2 = nullptr;
   // one FT-CCSD step captured as a HIP graph, and the arguments it was
   // captured for (the first call with new arguments runs eagerly, so that
   // every lazily allocated buffer exists before capture)
@@ -1881,10 +1887,57 @@ bool amplitudes_conserve_spin(ftcc_ctx *c, const double *T1,
   return bad == 0.0;
 }
 
+// The trusted promise, checked once on the first amplitudes after it is
+// given: every slice conserves spin under the labels and, for a spin-flip
+// symmetric system, equals its flipped image (to 1e-10 relative: iterates of
+// an untrusted run are symmetric up to rounding).  A broken promise is
+// withdrawn and reported.
+void verify_trust(ftcc_ctx *c, const double *T1, const double *T2) {
+  c->trust_pending = false;
+  if (!c->spin) return;   // dense layout: nothing relies on the promise
+  hipStream_t s = c->stream;
+  if (s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    FTCC_CHECK_HIP(hipStreamIsCapturing(s, &st));
+    require(st == hipStreamCaptureStatusNone,
+            "ftcc_set_trusted: the first residual after the promise checks "
+            "the amplitudes and cannot be captured");
+  }
+  if (!amplitudes_conserve_spin(c, T1, T2, c->ng)) {
+    c->trusted = false;
+    throw Error(FTCC_EINVAL,
+                "ftcc_set_trusted: the amplitudes do not conserve spin under "
+                "the orbital labels (promise withdrawn)");
+  }
+  if (!c->flip) return;
+  const int O = c->no, V = c->nv;
+  const int ho = c->o_count[0], hv = c->v_count[0];
+  double bad = 0.0;
+  int nbk = flip_violations(T1, V, O, 1, 1, hv, ho, 0, 0, c->partials, s,
+                            c->ng, 1e-10);
+  reduce_partials(c->partials, nbk, c->red, s);
+  read_scalars(c, 3);
+  bad += c->host_red[2];
+  nbk = flip_violations(T2, V, V, O, O, hv, hv, ho, ho, c->partials, s, c->ng,
+                        1e-10);
+  reduce_partials(c->partials, nbk, c->red, s);
+  read_scalars(c, 3);
+  bad += c->host_red[2];
+  if (bad != 0.0) {
+    c->trusted = false;
+    throw Error(FTCC_EINVAL,
+                "ftcc_set_trusted: the amplitudes are not symmetric under the "
+                "spin flip of this flip-symmetric system (" +
+                    std::to_string((long long)bad) +
+                    " entries differ; promise withdrawn)");
+  }
+}
+
 // Returns whether the spin layout was used (the amplitudes conserve spin).
 bool residual_all(ftcc_ctx *c, const double *T1, const double *T2,
                   double *T1b, double *T2b, uint64_t skip) {
   ensure_workspace(c);
+  if (c->trusted && c->trust_pending) verify_trust(c, T1, T2);
   // spin blocks are exact only for spin-conserving amplitudes (every CC
   // iterate is); anything else runs dense
   const int spin_saved = c->spin;
@@ -2136,6 +2189,11 @@ int ftcc_set_blocks(ftcc_ctx *c, const double *const F[4],
     }
     build_derived(c);
     c->spin = 0;   // labels unknown: ftcc_set_spin may switch blocking on
+    // the orbital energies of an earlier ftcc_set_system belong to other
+    // integrals: denominators must come from ftcc_set_denominators, and the
+    // flip symmetry (checked on those energies) no longer holds
+    c->have_energies = false;
+    c->flip = 0;
     FTCC_CHECK_HIP(hipStreamSynchronize(s));
   });
 }
@@ -2286,6 +2344,55 @@ int ftcc_residual(ftcc_ctx *c, const double *T1, const double *T2,
     check_ready(c);
     require(T1 && T2 && T1b && T2b, "null amplitude pointer");
     residual_all(c, T1, T2, T1b, T2b, skip);
+  });
+}
+
+int ftcc_defect(ftcc_ctx *c, const double *T1, const double *T2,
+                int low_precision, double *dT1b, double *dT2b, double *S1,
+                double *S2, uint64_t skip) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(T1 && T2 && dT1b && dT2b && S1 && S2, "null amplitude pointer");
+    require(low_precision >= 0 && low_precision <= 2, "unknown precision");
+    require(c->corr1 != dT1b && c->corr2 != dT2b,
+            "the defect cannot overwrite the correction in use");
+    drop_graph(c);
+    const int keep = c->prec;
+    struct Back {
+      ftcc_ctx *c;
+      int p;
+      ~Back() {
+        if (c->prec != p) {
+          free_workspace(c);
+          c->prec = p;
+        }
+      }
+    } back{c, keep};
+    // the residual of ftcc_step (spin-forbidden doubles left unwritten,
+    // the step's flip images), so that the correction cancels the step's
+    // own low-precision residual at T exactly
+    auto with = [&](int prec, double *o1, double *o2) {
+      if (c->prec != prec) {
+        free_workspace(c);
+        c->prec = prec;
+      }
+      c->tbar_scratch = true;
+      try {
+        residual_all(c, T1, T2, o1, o2, skip);
+      } catch (...) {
+        c->tbar_scratch = false;
+        throw;
+      }
+      c->tbar_scratch = false;
+    };
+    const int64_t VO = (int64_t)c->nv * c->no;
+    // zeros where the step's residual leaves forbidden doubles unwritten
+    fill(dT2b, c->ng * VO * VO, 0.0, c->stream);
+    fill(S2, c->ng * VO * VO, 0.0, c->stream);
+    with(FTCC_PREC_FP64, dT1b, dT2b);
+    with(low_precision, S1, S2);
+    axpby(dT1b, S1, c->ng * VO, -1.0, 1.0, c->stream);
+    axpby(dT2b, S2, c->ng * VO * VO, -1.0, 1.0, c->stream);
   });
 }
 
@@ -2611,12 +2718,14 @@ void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
   a.damp = damp;
   a.tbar = T1b;
   a.told = T1old;
+  a.corr = c->corr1;
   a.D = c->have_D ? c->D1 : nullptr;
   a.eint = c->fovT;
   int nb = update_t1(a, 1, s);
   reduce_partials(c->partials, nb, c->red, s);
   a.tbar = T2b;
   a.told = T2old;
+  a.corr = c->corr2;
   a.D = c->have_D ? c->D2 : nullptr;
   a.eint = c->oovvT;
   a.t1new = T1old;
@@ -2681,6 +2790,18 @@ int ftcc_set_trusted(ftcc_ctx *c, int on) {
   return guarded(c, [&] {
     drop_graph(c);
     c->trusted = on != 0;
+    c->trust_pending = c->trusted;
+  });
+}
+
+int ftcc_set_correction(ftcc_ctx *c, const double *dT1bar,
+                        const double *dT2bar) {
+  return guarded(c, [&] {
+    require((dT1bar == nullptr) == (dT2bar == nullptr),
+            "give both correction arrays or neither");
+    drop_graph(c);   // the captured update reads the pointers
+    c->corr1 = dT1bar;
+    c->corr2 = dT2bar;
   });
 }
 

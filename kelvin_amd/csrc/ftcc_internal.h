@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,16 @@ struct Error {
 
 inline void require(bool ok, const std::string &msg, int code = FTCC_EINVAL) {
   if (!ok) throw Error(code, msg);
+}
+
+// Raise a kernel's dynamic LDS limit to `bytes` once per process; safe when
+// several host threads launch (a failed attempt is retried by the next).
+inline void set_max_lds_once(std::once_flag &flag, const void *kernel,
+                             int bytes) {
+  std::call_once(flag, [&] {
+    FTCC_CHECK_HIP(hipFuncSetAttribute(
+        kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -303,6 +314,9 @@ struct UpdateArgs {
                                 // large for LDS staging integrate in place
                                 // only from a copy)
   double *told;                 // damped iterate (mode 1)
+  // a fixed correction added to every element of Tbar before integration
+  // (same layout as tbar; nullptr: none)
+  const double *corr = nullptr;
   double damp;
   int uniform;                  // grid equally spaced: tau_x - tau_y =
   double delta;                 //   (x - y) * delta
@@ -366,10 +380,12 @@ int vsum(const double *x, int64_t n, double *partials, hipStream_t s);
 int spin_violations(const double *a, int d0, int d1, int d2, int d3, int nb,
                     const int *l0, const int *l1, const int *l2, const int *l3,
                     double *partials, hipStream_t s);
-// count (third slot) of the entries of a (d0,d1,d2,d3) block that differ
-// from their image under the spin flip: index x of dimension k maps to
-// x + h_k below h_k and x - h_k from h_k on (h_k = 0: not flipped)
+// count (third slot) of the entries of nb consecutive (d0,d1,d2,d3) blocks
+// that differ from their image under the spin flip: index x of dimension k
+// maps to x + h_k below h_k and x - h_k from h_k on (h_k = 0: not flipped).
+// tol = 0 counts any difference; tol > 0 only |x - y| > tol * max(|x|, |y|).
 int flip_violations(const double *a, int d0, int d1, int d2, int d3, int h0,
-                    int h1, int h2, int h3, double *partials, hipStream_t s);
+                    int h1, int h2, int h3, double *partials, hipStream_t s,
+                    int nb = 1, double tol = 0.0);
 
 }  // namespace ftcc

@@ -505,13 +505,8 @@ void sbgemm_grouped(int nprob, const SbProblem *prob, int nb, double alpha,
                     double beta, double *C, int64_t sCm, int64_t sCn,
                     int64_t sCb, hipStream_t s) {
   require(nprob >= 1 && nprob <= MAXP, "1 to 3 problems per grouped launch");
-  static bool attr_set = false;
-  if (!attr_set) {
-    FTCC_CHECK_HIP(hipFuncSetAttribute(
-        (const void *)sbgemm_kernel,
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES));
-    attr_set = true;
-  }
+  static std::once_flag attr;
+  set_max_lds_once(attr, (const void *)sbgemm_kernel, (int)LDS_BYTES);
   SbDev p;
   p.nb = nb;
   p.C = C;

@@ -733,13 +733,8 @@ void r2_final(double *t2bar, const double *acc1, const double *acc2,
     require(pblocks < (1LL << 31), "r2 assembly grid too large");
     const size_t lds = sizeof(double) * 4 * (size_t)no * R2P_LD;
     require(lds <= 160 * 1024, "r2 assembly: too many occupied orbitals");
-    static bool attr = false;
-    if (!attr) {
-      FTCC_CHECK_HIP(hipFuncSetAttribute(
-          (const void *)r2_packed_kernel,
-          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-      attr = true;
-    }
+    static std::once_flag attr;
+    set_max_lds_once(attr, (const void *)r2_packed_kernel, 160 * 1024);
     hipLaunchKernelGGL(r2_packed_kernel, dim3((unsigned)pblocks), dim3(256),
                        lds, s, t2bar, acc1, acc2, S1, S2, Z, add, nv, no,
                        lab_v, lab_o, zh, zo, zero_forbidden ? 1 : 0, jl, jc0,

@@ -108,7 +108,9 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
       d = p.ev[q.a] - p.eo[q.i];
     if (STAGE_LDS) {
       // issue every load of this element before any arithmetic
-      for (int x = 0; x < ng; ++x) col[x * UT + threadIdx.x] = src[x * per + e];
+      for (int x = 0; x < ng; ++x)
+        col[x * UT + threadIdx.x] =
+            src[x * per + e] + (p.corr ? p.corr[x * per + e] : 0.0);
       if (mode == 1)
         for (int x = 0; x < ng; ++x)
           ocol[x * UT + threadIdx.x] = p.told[x * per + e];
@@ -124,16 +126,18 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
         if (p.uniform) w *= r;
         const double gyx = Gy[x];
         if (gyx == 0.0) continue;
-        const double tb = STAGE_LDS ? col[x * UT + threadIdx.x]
-                                    : src[x * per + e];
+        const double tb =
+            STAGE_LDS ? col[x * UT + threadIdx.x]
+                      : src[x * per + e] + (p.corr ? p.corr[x * per + e] : 0.0);
         const double wx = p.uniform ? w : exp((tis[x] - ty) * d);
         tn += gyx * (wx * tb);
       }
       for (int x = y; x < ng; ++x) {
         const double gyx = Gy[x];
         if (gyx == 0.0) continue;
-        const double tb = STAGE_LDS ? col[x * UT + threadIdx.x]
-                                    : src[x * per + e];
+        const double tb =
+            STAGE_LDS ? col[x * UT + threadIdx.x]
+                      : src[x * per + e] + (p.corr ? p.corr[x * per + e] : 0.0);
         tn += gyx * tb;
       }
       if (mode == 0) {
@@ -244,6 +248,11 @@ __global__ __launch_bounds__(UT) void update_rec_kernel(UpdateArgs p,
 #pragma unroll
       for (int k = 0; k < RCH; ++k)
         if (y0 + k < ng) tb[k] = src[(y0 + k) * per + e];
+      if (p.corr) {
+#pragma unroll
+        for (int k = 0; k < RCH; ++k)
+          if (y0 + k < ng) tb[k] += p.corr[(y0 + k) * per + e];
+      }
       if (mode == 1) {
 #pragma unroll
         for (int k = 0; k < RCH; ++k)
@@ -300,15 +309,10 @@ int launch_update(const UpdateArgs &a, int mode, hipStream_t s) {
   }
   if (a.ng <= MAX_LDS_NG) {
     const size_t lds = sizeof(double) * (a.ng * a.ng + a.ng + 2 * a.ng * UT);
-    static bool attr = false;
-    if (!attr) {
-      FTCC_CHECK_HIP(hipFuncSetAttribute(
-          (const void *)update_kernel<DOUBLES, true>,
-          hipFuncAttributeMaxDynamicSharedMemorySize,
-          (int)(sizeof(double) * (MAX_LDS_NG * MAX_LDS_NG + MAX_LDS_NG +
-                                  2 * MAX_LDS_NG * UT))));
-      attr = true;
-    }
+    static std::once_flag attr;
+    set_max_lds_once(attr, (const void *)update_kernel<DOUBLES, true>,
+                     (int)(sizeof(double) * (MAX_LDS_NG * MAX_LDS_NG +
+                                             MAX_LDS_NG + 2 * MAX_LDS_NG * UT)));
     hipLaunchKernelGGL((update_kernel<DOUBLES, true>), dim3(blocks), dim3(UT),
                        lds, s, a, mode);
   } else {
@@ -486,13 +490,14 @@ __global__ __launch_bounds__(UT) void spin_violations_kernel(
 // dimension with half h maps to x + h below h, x - h above; h = 0: fixed).
 __global__ __launch_bounds__(UT) void flip_violations_kernel(
     const double *a, int d0, int d1, int d2, int d3, int h0, int h1, int h2,
-    int h3, double *partials) {
+    int h3, int nb, double tol, double *partials) {
   const int64_t per = (int64_t)d0 * d1 * d2 * d3;
   double cnt = 0.0;
   auto fl = [](int x, int h) { return h ? (x < h ? x + h : x - h) : x; };
-  for (int64_t e = blockIdx.x * (int64_t)UT + threadIdx.x; e < per;
-       e += (int64_t)gridDim.x * UT) {
-    int64_t r = e;
+  for (int64_t g = blockIdx.x * (int64_t)UT + threadIdx.x; g < per * nb;
+       g += (int64_t)gridDim.x * UT) {
+    const int64_t base = (g / per) * per;
+    int64_t r = g - base;
     const int i3 = (int)(r % d3);
     r /= d3;
     const int i2 = (int)(r % d2);
@@ -500,9 +505,13 @@ __global__ __launch_bounds__(UT) void flip_violations_kernel(
     const int i1 = (int)(r % d1);
     const int i0 = (int)(r / d1);
     const int64_t f =
+        base +
         (((int64_t)fl(i0, h0) * d1 + fl(i1, h1)) * d2 + fl(i2, h2)) * d3 +
         fl(i3, h3);
-    if (a[e] != a[f]) cnt += 1.0;
+    const double x = a[g], y = a[f];
+    if (tol == 0.0 ? x != y
+                   : !(fabs(x - y) <= tol * fmax(fabs(x), fabs(y))))
+      cnt += 1.0;
   }
   block_reduce3(0.0, 0.0, cnt, partials);
 }
@@ -708,10 +717,11 @@ int spin_violations(const double *a, int d0, int d1, int d2, int d3, int nb,
 }
 
 int flip_violations(const double *a, int d0, int d1, int d2, int d3, int h0,
-                    int h1, int h2, int h3, double *partials, hipStream_t s) {
-  const int blocks = nblocks_for((int64_t)d0 * d1 * d2 * d3);
+                    int h1, int h2, int h3, double *partials, hipStream_t s,
+                    int nb, double tol) {
+  const int blocks = nblocks_for((int64_t)d0 * d1 * d2 * d3 * nb);
   hipLaunchKernelGGL(flip_violations_kernel, dim3(blocks), dim3(UT), 0, s, a,
-                     d0, d1, d2, d3, h0, h1, h2, h3, partials);
+                     d0, d1, d2, d3, h0, h1, h2, h3, nb, tol, partials);
   FTCC_CHECK_HIP(hipGetLastError());
   return blocks;
 }

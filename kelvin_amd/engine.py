@@ -310,6 +310,29 @@ class FTEngine:
         synchronisation in the residual."""
         self._call("ftcc_set_trusted", int(bool(on)))
 
+    def set_correction(self, dT1b=None, dT2b=None):
+        """Add fixed corrections to the residual of every later ``step``
+        (None, None clears them); the tensors must stay alive meanwhile."""
+        if (dT1b is None) != (dT2b is None):
+            raise ValueError("give both corrections or neither")
+        if dT1b is None:
+            self._corr = None
+            self._call("ftcc_set_correction", None, None)
+            return
+        self._check(dT1b, self.t1_shape(), "dT1bar")
+        self._check(dT2b, self.t2_shape(), "dT2bar")
+        self._corr = (dT1b, dT2b)
+        self._call("ftcc_set_correction", _vp(dT1b), _vp(dT2b))
+
+    def defect(self, T1, T2, low, dT1b, dT2b, S1, S2, skip_mask=0):
+        """dTbar = Tbar_fp64(T) - Tbar_low(T) at every grid point (S1/S2
+        amplitude-size scratch); the engine's precision is unchanged."""
+        self._check_amps(T1, T2, dT1b, dT2b)
+        self._check(S1, self.t1_shape(), "S1")
+        self._check(S2, self.t2_shape(), "S2")
+        self._call("ftcc_defect", _vp(T1), _vp(T2), PREC[low], _vp(dT1b),
+                   _vp(dT2b), _vp(S1), _vp(S2), ctypes.c_uint64(skip_mask))
+
     def set_chunk(self, nb):
         """Grid points per residual chunk (0: automatic)."""
         self._call("ftcc_set_chunk", int(nb))

@@ -400,6 +400,362 @@ def residual_tbar(F, I, T1, T2):
 
 
 # ---------------------------------------------------------------------------
+# Adjoint of the residual (the Lambda equations).  kelvin solves them with
+# cqcpy cc_equations._Lambda_opt (absent, called at
+# kelvin/ft_cc_equations.py:385-409 with fac=-1), whose content is the
+# transpose of the Jacobian of the Stanton-Gauss residual contracted with the
+# integrated lambda amplitudes.  stanton_rhs_transpose restates that operator
+# as the exact reverse-mode derivative of stanton_rhs, term by term.
+# ---------------------------------------------------------------------------
+def stanton_rhs_transpose(F, I, t1, t2, v1, v2):
+    """Return (g1, g2) with <g, w> = d/dh <v, stanton_rhs(t + h w)> at h = 0
+    for every direction w (all n^4 components of t2 independent: no
+    antisymmetry is assumed of v2 or w2).  Layouts as stanton_rhs."""
+    es = _es
+    vovv, ooov, oovv, vovo = I.vovv, I.ooov, I.oovv, I.vovo
+    fov = F.ov
+    # forward intermediates, as stanton_rhs
+    tt0 = es('ai,bj->abij', t1, t1)
+    tt = tt0 - tt0.transpose(1, 0, 2, 3)
+    tau = t2 + tt
+    taut = t2 + 0.5 * tt
+    Fme = fov + es('fn,mnef->me', t1, oovv)
+    Fae = (F.vv - 0.5 * es('me,am->ae', fov, t1)
+           - es('fm,amfe->ae', t1, vovv)
+           - 0.5 * es('afmn,mnef->ae', taut, oovv))
+    Fmi = (F.oo + 0.5 * es('ei,me->mi', t1, fov)
+           + es('en,mnie->mi', t1, ooov)
+           + 0.5 * es('efin,mnef->mi', taut, oovv))
+    tmp = es('ej,mnie->mnij', t1, ooov)
+    Wmnij = (I.oooo + tmp - tmp.transpose(0, 1, 3, 2)
+             + 0.25 * es('efij,mnef->mnij', tau, oovv))
+    tmp = es('bm,amef->abef', t1, vovv)
+    Wabef = (I.vvvv - tmp + tmp.transpose(1, 0, 2, 3)
+             + 0.25 * es('abmn,mnef->abef', tau, oovv))
+    ovvo = -vovo.transpose(1, 0, 2, 3)
+    Wmbej = (ovvo - es('fj,bmef->mbej', t1, vovv)
+             + es('bn,mnje->mbej', t1, ooov)
+             - 0.5 * es('fbjn,mnef->mbej', t2, oovv)
+             - es('fj,bn,mnef->mbej', t1, t1, oovv))
+    Fbe = Fae - 0.5 * es('bm,me->be', t1, Fme)
+    Fmj = Fmi + 0.5 * es('ej,me->mj', t1, Fme)
+
+    def pab(x):
+        return x.transpose(1, 0, 2, 3)
+
+    def pij(x):
+        return x.transpose(0, 1, 3, 2)
+
+    g1 = numpy.zeros_like(t1)
+    g2 = numpy.zeros_like(t2)
+    # doubles terms: transposes of the antisymmetrisers
+    gX3 = v2 - pab(v2)
+    gX4 = -(v2 - pij(v2))
+    gX5 = v2 - pab(v2) - pij(v2) + pab(pij(v2))
+    gX6 = v2 - pij(v2)
+    gX7 = v2 - pab(v2)
+    g1 += es('abij,bmij->am', gX7, I.vooo)          # t1[a,m] vooo[b,m,i,j]
+    g1 += es('abij,abej->ei', gX6, I.vvvo)          # t1[e,i] vvvo[a,b,e,j]
+    g2 += es('abij,mbej->aeim', gX5, Wmbej)         # t2[a,e,i,m] Wmbej
+    gW = es('aeim,abij->mbej', t2, gX5)
+    Z = es('abij,mbej->aime', gX5, ovvo)            # -t1[e,i] t1[a,m] ovvo
+    g1 -= es('aime,am->ei', Z, t1)
+    g1 -= es('aime,ei->am', Z, t1)
+    gtau = 0.5 * es('abij,abef->efij', v2, Wabef)   # 1/2 tau.Wabef
+    gWabef = 0.5 * es('abij,efij->abef', v2, tau)
+    gtau += 0.5 * es('abij,mnij->abmn', v2, Wmnij)  # 1/2 tau.Wmnij
+    gWmnij = 0.5 * es('abmn,abij->mnij', tau, v2)
+    g2 += es('abij,mj->abim', gX4, Fmj)             # t2[a,b,i,m] Fmj
+    gFmj = es('abim,abij->mj', t2, gX4)
+    g2 += es('abij,be->aeij', gX3, Fbe)             # t2[a,e,i,j] Fbe
+    gFbe = es('aeij,abij->be', t2, gX3)
+    # Fmj = Fmi + 1/2 t1 Fme ; Fbe = Fae - 1/2 t1 Fme
+    gFmi = gFmj.copy()
+    g1 += 0.5 * es('mj,me->ej', gFmj, Fme)
+    gFme = 0.5 * es('mj,ej->me', gFmj, t1)
+    gFae = gFbe.copy()
+    g1 -= 0.5 * es('be,me->bm', gFbe, Fme)
+    gFme -= 0.5 * es('be,bm->me', gFbe, t1)
+    # singles terms
+    gFae += es('ai,ei->ae', v1, t1)
+    g1 += es('ai,ae->ei', v1, Fae)
+    gFmi -= es('ai,am->mi', v1, t1)
+    g1 -= es('ai,mi->am', v1, Fmi)
+    g2 += es('ai,me->aeim', v1, Fme)
+    gFme += es('ai,aeim->me', v1, t2)
+    g1 -= es('ai,anfi->fn', v1, vovo)
+    g2 += 0.5 * es('ai,amef->efim', v1, vovv)
+    g2 += 0.5 * es('ai,nmie->aemn', v1, ooov)
+    # Wmbej
+    g1 -= es('mbej,bmef->fj', gW, vovv)
+    g1 += es('mbej,mnje->bn', gW, ooov)
+    g2 -= 0.5 * es('mbej,mnef->fbjn', gW, oovv)
+    Q = es('mbej,mnef->bjnf', gW, oovv)
+    g1 -= es('bjnf,bn->fj', Q, t1)
+    g1 -= es('bjnf,fj->bn', Q, t1)
+    # Wabef: -t1[b,m] vovv[a,m,e,f] + t1[a,m] vovv[b,m,e,f] + 1/4 tau.oovv
+    gX2 = -gWabef + pab(gWabef)
+    g1 += es('abef,amef->bm', gX2, vovv)
+    gtau += 0.25 * es('abef,mnef->abmn', gWabef, oovv)
+    # Wmnij: t1[e,j] ooov[m,n,i,e] - (i <-> j) + 1/4 tau.oovv
+    gX1 = gWmnij - pij(gWmnij)
+    g1 += es('mnij,mnie->ej', gX1, ooov)
+    gtau += 0.25 * es('mnij,mnef->efij', gWmnij, oovv)
+    # Fmi
+    g1 += 0.5 * es('mi,me->ei', gFmi, fov)
+    g1 += es('mi,mnie->en', gFmi, ooov)
+    gtaut = 0.5 * es('mi,mnef->efin', gFmi, oovv)
+    # Fae
+    g1 -= 0.5 * es('ae,me->am', gFae, fov)
+    g1 -= es('ae,amfe->fm', gFae, vovv)
+    gtaut -= 0.5 * es('ae,mnef->afmn', gFae, oovv)
+    # Fme
+    g1 += es('me,mnef->fn', gFme, oovv)
+    # tau = t2 + tt, taut = t2 + tt/2, tt = t1 t1 - (a <-> b)
+    g2 += gtau + gtaut
+    gtt = gtau + 0.5 * gtaut
+    gtt0 = gtt - pab(gtt)
+    g1 += es('abij,bj->ai', gtt0, t1) + es('abij,ai->bj', gtt0, t1)
+    return g1, g2
+
+
+def antisymmetrize(x2):
+    """Orthogonal projection of (..., a, b, i, j) onto tensors antisymmetric
+    in (a, b) and in (i, j)."""
+    y = x2 - numpy.swapaxes(x2, -4, -3)
+    y = y - numpy.swapaxes(y, -2, -1)
+    return 0.25 * y
+
+
+def int_adjoint(ng, lam, ti, D, G):
+    """Transpose of int_tbar1/int_tbar2 (kelvin/quadrature.py:292-317) in
+    the slice index: out[x] = sum_y G[y,x] P_yx lam[y], with the integrator's
+    propagator P_yx = exp((ti[x] - ti[y]) D) for x < y and 1 for x >= y."""
+    out = numpy.zeros_like(lam)
+    for x in range(ng):
+        acc = numpy.zeros_like(lam[0])
+        for y in range(ng):
+            if G[y, x] == 0.0:
+                continue
+            w = numpy.exp((ti[x] - ti[y]) * D) if x < y else 1.0
+            acc += G[y, x] * w * lam[y]
+        out[x] = acc
+    return out
+
+
+def int_L1(ng, L1old, ti, D1, g, G):
+    """kelvin/quadrature.py:320-332 (L1 layout [y,i,a])."""
+    L1_out = numpy.zeros(L1old.shape, dtype=L1old.dtype)
+    for s in range(ng):
+        dt = numpy.zeros(ng)
+        for y in range(s, ng):
+            dt[y] = ti[s] - ti[y]
+        gtemp = numpy.exp(dt[:, None, None] * D1[None, :, :])
+        L1temp = einsum('yai,yia->yia', gtemp, L1old)
+        L1_out[s] = einsum('y,yia,y->ia', g, L1temp, G[:, s]) / g[s]
+    return L1_out
+
+
+def int_L2(ng, L2old, ti, D2, g, G):
+    """kelvin/quadrature.py:335-345 (L2 layout [y,i,j,a,b])."""
+    L2_out = numpy.zeros(L2old.shape, dtype=L2old.dtype)
+    for s in range(ng):
+        dt = numpy.zeros(ng)
+        for y in range(s, ng):
+            dt[y] = ti[s] - ti[y]
+        gtemp = numpy.exp(dt[:, None, None, None, None] * D2[None])
+        L2temp = einsum('yabij,yijab->yijab', gtemp, L2old)
+        L2_out[s] = einsum('y,yijab,y->ijab', g, L2temp, G[:, s]) / g[s]
+    return L2_out
+
+
+def ccsd_lambda_guess(F, I, T1, beta, ng):
+    """kelvin/ft_cc_equations.py:502-512 with cqcpy _LS_TS (the T1 term of
+    the energy's derivative, +fac * sum_jb oovv[i,j,a,b] t1[b,j])."""
+    L1 = numpy.broadcast_to(F.ov / beta, (ng,) + F.ov.shape).copy()
+    L2 = numpy.broadcast_to(I.oovv / beta, (ng,) + I.oovv.shape).copy()
+    for y in range(ng):
+        L1[y] += einsum('ijab,bj->ia', I.oovv, T1[y]) / beta
+    return L1, L2
+
+
+def ccsd_lambda_opt(F, I, T1, T2, L1old, L2old, D1, D2, ti, ng, g, G, beta):
+    """kelvin/ft_cc_equations.py:385-409 in kelvin's lambda convention
+    (L1[y,i,a], L2[y,i,j,a,b]; the derivative of Omega_cc along a change dT
+    is -(1/beta) sum_y g_y [L1.dT1 + 1/4 L2.dT2], kelvin/ccsd.py:1113-1146):
+    L = -dE_y/dT_y - J_y^T (int_L(L)) with the Jacobian transpose of
+    stanton_rhs_transpose, the doubles projected onto antisymmetric
+    tensors."""
+    L1int = int_L1(ng, L1old, ti, D1, g, G)
+    L2int = int_L2(ng, L2old, ti, D2, g, G)
+    L1 = numpy.empty_like(L1old)
+    L2 = numpy.empty_like(L2old)
+    for y in range(ng):
+        v1 = L1int[y].T
+        v2 = 0.25 * L2int[y].transpose(2, 3, 0, 1)
+        g1, g2 = stanton_rhs_transpose(F, I, T1[y], T2[y], v1, v2)
+        g2 = antisymmetrize(g2)
+        L1[y] = (-F.ov - einsum('ijab,bj->ia', I.oovv, T1[y]) - g1.T)
+        L2[y] = -I.oovv - 4.0 * g2.transpose(2, 3, 0, 1)
+    return L1, L2
+
+
+def ft_lambda_iter(L1old, L2old, T1, T2, F, I, D1, D2, g, G, beta, ng, ti,
+                   conv, history=None):
+    """kelvin/cc_utils.py:414-480: damped iteration of ccsd_lambda_opt with
+    norms ||L|| + 0.1 and the test res1 + res2 < tconv."""
+    thresh, max_iter, alpha = conv["tconv"], conv["max_iter"], conv["damp"]
+    i, converged = 0, False
+    nl1 = numpy.linalg.norm(L1old) + 0.1
+    nl2 = numpy.linalg.norm(L2old) + 0.1
+    while i < max_iter and not converged:
+        L1, L2 = ccsd_lambda_opt(F, I, T1, T2, L1old, L2old, D1, D2, ti, ng,
+                                 g, G, beta)
+        res1 = numpy.linalg.norm(L1 - L1old) / nl1
+        res2 = numpy.linalg.norm(L2 - L2old) / nl2
+        L1old = alpha * L1old + (1.0 - alpha) * L1
+        L2old = alpha * L2old + (1.0 - alpha) * L2
+        nl1 = numpy.linalg.norm(L1old) + 0.1
+        nl2 = numpy.linalg.norm(L2old) + 0.1
+        if history is not None:
+            history.append(res1 + res2)
+        i += 1
+        converged = res1 + res2 < thresh
+    return L1old, L2old, converged, i
+
+
+def response_blocks(F, I, dfock, fo, fv, dx, iocc, ivir):
+    """Derivative of the Fermi-weighted blocks F, I (kelvin/cc_utils.py:
+    569-602, :825-858, over the orbital lists iocc / ivir) when
+    x_p = beta (e_p - mu) moves by dx_p (all n orbitals) and the bare Fock
+    matrix by dfock (n x n).  With d ln sqrt(fo_p) = -fv_p dx_p / 2 and
+    d ln sqrt(fv_p) = fo_p dx_p / 2 a block w x y z changes by
+    I * (c^w_p + c^x_q + c^y_r + c^z_s).  These are the quantities kelvin
+    contracts with its response densities in g_d_on_oo, g_d_on_vv (the
+    dso = fv, dsv = fo factors) and g_Fd_on (kelvin/cc_utils.py:1628-1700)."""
+    o, v = numpy.asarray(iocc), numpy.asarray(ivir)
+    so, sv = numpy.sqrt(fo[o]), numpy.sqrt(fv[v])
+    c = {'o': -0.5 * fv[o] * dx[o], 'v': 0.5 * fo[v] * dx[v]}
+    s_ = {'o': so, 'v': sv}
+    idx = {'o': o, 'v': v}
+    dF = Blocks()
+    for name in ('oo', 'ov', 'vo', 'vv'):
+        x, y = name
+        dF[name] = (dfock[numpy.ix_(idx[x], idx[y])] * s_[x][:, None]
+                    * s_[y][None, :]
+                    + F[name] * (c[x][:, None] + c[y][None, :]))
+    dI = Blocks()
+    for name in ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo', 'oovv', 'vooo',
+                 'ooov', 'oooo'):
+        w = (c[name[0]][:, None, None, None] + c[name[1]][None, :, None, None]
+             + c[name[2]][None, None, :, None]
+             + c[name[3]][None, None, None, :])
+        dI[name] = I[name] * w
+    return dF, dI
+
+
+def grid_derivative(Lam1, Lam2, Tb1, Tb2, ti, G, Gd, D1, D2, beta):
+    """sum over y, x of Lam_y . (Gd[y,x] P_yx - G[y,x] (ti_y - ti_x)/beta D
+    P_yx) Tbar_x: the change of the integrated amplitudes with beta through
+    the weights and the propagators (ti scales with beta), contracted with
+    the adjoint amplitudes (kelvin/ccsd.py:1113-1146 in the unscaled
+    convention; P as in int_adjoint)."""
+    ng = ti.shape[0]
+    out = 0.0
+    for Lam, Tb, D in ((Lam1, Tb1, D1), (Lam2, Tb2, D2)):
+        for y in range(ng):
+            for x in range(ng):
+                if G[y, x] == 0.0 and Gd[y, x] == 0.0:
+                    continue
+                P = numpy.exp((ti[x] - ti[y]) * D) if x < y else 1.0
+                K = (Gd[y, x] - G[y, x] * (ti[y] - ti[x]) / beta * D) * P
+                out += numpy.sum(Lam[y] * K * Tb[x])
+    return out
+
+
+def ft_ccsd_esn(sys, T, mu, ngrid=10, damp=0.0, max_iter=40, econv=1e-8,
+                tconv=None, quad='lin', athresh=0.0, gderiv=True):
+    """E, S, N of the FT-CCSD grand potential (kelvin/ccsd.py:128-214,
+    generalised path) from the adjoint of the converged fixed point:
+    Omega_cc(beta, mu) with T = Phi(T) has dOmega_cc/dtheta =
+    dE/dtheta + sum_y <Lam_y, dPhi_y/dtheta> where Lam solves
+    Lam = dE/dT + (dPhi/dT)^T Lam (the lambda equations of
+    kelvin/cc_utils.py:414-480, here in kelvin's L convention and converted:
+    Lam1_y = -(g_y/beta) L1_y^T, Lam2_y = -(g_y/(4 beta)) L2_y^T).  The
+    residual is linear in the integrals, so dPhi/dtheta through them is the
+    integrated residual with the derivative blocks of response_blocks; the
+    grid scales with beta (d_ft_quad, kelvin/ccsd.py:1075-1148).  First
+    order: sys.g_mp1_den (kelvin/ccsd.py:1434-1445, :186-190)."""
+    tconv = tconv if tconv is not None else 1000.0 * econv
+    r = ft_ccsd(sys, T, mu, ngrid=ngrid, damp=damp, max_iter=max_iter,
+                econv=econv, tconv=tconv, quad=quad, athresh=athresh)
+    beta = 1.0 / T
+    ti, g, G = ft_quad(ngrid, beta, quad)
+    en = sys.g_energies_tot()
+    n = en.shape[0]
+    fo, fv = ff(beta, en, mu), ffv(beta, en, mu)
+    fock, eri = sys.g_fock_tot(), sys.g_aint_tot()
+    if athresh > 0.0:
+        iocc = [i for i, x in enumerate(fo) if x > athresh]
+        ivir = [i for i, x in enumerate(fv) if x > athresh]
+    else:
+        iocc = ivir = list(range(n))
+    F, I = ft_active_integrals(fock, eri, en, fo[iocc], fv[ivir], iocc, ivir)
+    d1 = D1(en, en)[numpy.ix_(ivir, iocc)]
+    d2 = D2(en, en)[numpy.ix_(ivir, ivir, iocc, iocc)]
+    T1, T2, ng = r["T1"], r["T2"], ngrid
+    conv = {"tconv": tconv, "max_iter": max_iter, "damp": damp}
+    L1, L2 = ccsd_lambda_guess(F, I, T1, beta, ng)
+    L1, L2, lconv, lit = ft_lambda_iter(L1, L2, T1, T2, F, I, d1, d2, g, G,
+                                        beta, ng, ti, conv)
+    Lam1 = -(g / beta)[:, None, None] * L1.transpose(0, 2, 1)
+    Lam2 = (-(g / (4.0 * beta))[:, None, None, None, None]
+            * L2.transpose(0, 3, 4, 1, 2))
+    Lt1 = int_adjoint(ng, Lam1, ti, d1, G)
+    Lt2 = int_adjoint(ng, Lam2, ti, d2, G)
+    Tb1, Tb2 = residual_tbar(F, I, T1, T2)
+    Fd = sys.g_fock_d_den()                     # [p,q,k] = <pk||qk> fo fv
+
+    def h_part(dx):
+        dfock = -einsum('pqk,k->pq', Fd, dx)
+        dF, dI = response_blocks(F, I, dfock, fo, fv, dx, iocc, ivir)
+        out = ft_cc_energy(T1, T2, dF.ov, dI.oovv, g, beta)
+        for y in range(ng):
+            r1, r2 = stanton_rhs(dF, dI, T1[y], T2[y])
+            out -= numpy.sum(Lt1[y] * r1) + numpy.sum(Lt2[y] * r2)
+        return out
+
+    dmu = h_part(-beta * numpy.ones(n))
+    dbeta = h_part(en - mu) - r["Omega_cc"] / beta
+    if gderiv:
+        gd, Gd = d_ft_quad(ng, beta, quad)
+        dbeta += ft_cc_energy(T1, T2, F.ov, I.oovv, gd, beta)
+        dbeta += grid_derivative(Lam1, Lam2, Tb1, Tb2, ti, G, Gd, d1, d2,
+                                 beta)
+    else:                                   # kelvin/ccsd.py:1262-1288
+        e_last = numpy.zeros(ng)
+        e_last[-1] = 1.0
+        dbeta += ft_cc_energy(T1, T2, F.ov, I.oovv, e_last, beta)
+    Ncc = -dmu
+    Ecc = beta * dbeta + mu * Ncc + r["Omega_cc"]
+    # zero and first order (kelvin/ccsd.py:172-190)
+    G0, G1 = r["G0"], r["G1"]
+    N0 = fo.sum()
+    E0 = beta * dGP0(beta, en, mu).sum() + mu * N0 + G0
+    ron1 = sys.g_mp1_den()
+    N1 = ron1.sum()
+    B1 = numpy.dot((en - mu) / beta, ron1)
+    E1 = beta * B1 + mu * N1 + G1
+    N = N0 + N1 + Ncc
+    E = E0 + E1 + Ecc
+    Om = r["Omega"]
+    S = -beta * (Om - E + mu * N)
+    return dict(E=E, S=S, N=N, E0=E0, E1=E1, Ecc=Ecc, N0=N0, N1=N1,
+                Ncc=Ncc, Omega=Om, Omega_cc=r["Omega_cc"], L1=L1, L2=L2,
+                lambda_converged=lconv, lambda_iter=lit, T1=T1, T2=T2)
+
+
+# ---------------------------------------------------------------------------
 # Grand potential: kelvin/ft_cc_energy.py:7-32
 # ---------------------------------------------------------------------------
 def ft_cc_energy(T1, T2, f, eri, g, beta, Qterm=True):
@@ -654,6 +1010,20 @@ class UEGSystem:
         fo = ff(1.0 / self.T, self.g_energies_tot(), self.mu)
         return 0.5 * einsum('ijij,i,j->', self.g_aint_tot(), fo, fo)
 
+    def g_mp1_den(self):
+        """kelvin/ueg_system.py:204-215: beta dE1/dx_p with x = beta (e - mu)."""
+        beta = 1.0 / self.T
+        en = self.g_energies_tot()
+        fo, fv = ff(beta, en, self.mu), ffv(beta, en, self.mu)
+        return -beta * einsum('ijij,i,j->i', self.g_aint_tot(), fo * fv, fo)
+
+    def g_fock_d_den(self):
+        """kelvin/ueg_system.py:498-510: JK[p,q,k] = <pk||qk> fo_k fv_k."""
+        beta = 1.0 / self.T
+        en = self.g_energies_tot()
+        fo, fv = ff(beta, en, self.mu), ffv(beta, en, self.mu)
+        return einsum('piqi,i->pqi', self.g_aint_tot(), fo * fv)
+
 
 class DFModelSystem:
     """Molecule-like model with density-fitted ERIs.
@@ -710,6 +1080,17 @@ class DFModelSystem:
         fo = ff(1.0 / self.T, self._en, self.mu)
         return 0.5 * einsum('ijij,i,j->', self._eri, fo, fo)
 
+    def g_mp1_den(self):
+        """beta dE1/dx_p (E1 as get_mp1; kelvin/ueg_system.py:204-215)."""
+        beta = 1.0 / self.T
+        fo, fv = ff(beta, self._en, self.mu), ffv(beta, self._en, self.mu)
+        return -beta * einsum('ijij,i,j->i', self._eri, fo * fv, fo)
+
+    def g_fock_d_den(self):
+        """The Fock matrix is diag(e): no occupation dependence."""
+        n = self._en.shape[0]
+        return numpy.zeros((n, n, n))
+
 
 class GenericSystem:
     """Explicit (h, <pq||rs>, reference energies) system for brute-force
@@ -745,6 +1126,19 @@ class GenericSystem:
         fo = self._fo()
         return (numpy.dot(numpy.diag(self.h) - self._en, fo)
                 + 0.5 * einsum('ijij,i,j->', self._eri, fo, fo))
+
+    def g_mp1_den(self):
+        """beta dE1/dx_p with x = beta (e - mu) (E1 as get_mp1)."""
+        beta = 1.0 / self.T
+        fo, fv = self._fo(), ffv(beta, self._en, self.mu)
+        return -beta * fo * fv * ((numpy.diag(self.h) - self._en)
+                                  + einsum('ijij,j->i', self._eri, fo))
+
+    def g_fock_d_den(self):
+        """[p,q,k] = <pk||qk> fo_k fv_k (Fock = h + sum_k <pk||qk> fo_k)."""
+        beta = 1.0 / self.T
+        fo, fv = self._fo(), ffv(beta, self._en, self.mu)
+        return einsum('piqi,i->pqi', self._eri, fo * fv)
 
 
 def ueg_params(nk, rs, N, t_over_tf):

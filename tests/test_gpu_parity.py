@@ -952,3 +952,136 @@ def test_batch_equals_loop():
     for a, b in zip(loop, bat):
         assert abs(a["Omega"] - b["Omega"]) < 1e-10
         assert a["iterations"] == b["iterations"]
+
+
+# ------------------------------------------------------------ trusted promise
+def _flip_engine(precision="fp64", ng=6, chunk=None):
+    """A spin-unpolarised UEG (19 k-points, n = 38): spin-flip symmetric."""
+    from kelvin_amd import quadrature
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd.ueg_system import ueg_from_config
+    sysg = ueg_from_config(19, 2.0, N=14, t_over_tf=0.3)
+    en = sysg.g_energies_tot()
+    n = en.shape[0]
+    beta = 1.0 / sysg.T
+    ti, g, G = quadrature.ft_quad(ng, beta, 'lin')
+    eng = FTEngine(n, n, ng, precision=precision)
+    if chunk:
+        eng.set_chunk(chunk)
+    fock, eri, _ = sysg.g_device_integrals(eng)
+    eng.set_system(en, fock, eri, beta, sysg.mu)
+    del fock, eri
+    eng.set_grid(ti, G)
+    return eng, ti, g, G, beta
+
+
+def test_trusted_promise_is_checked():
+    """ftcc_set_trusted checks its promise once on the next amplitudes: a
+    broken spin-flip symmetry or spin conservation gives FTCCError and
+    withdraws the promise (later calls then run the untrusted path and
+    equal a context that was never trusted); iterates of the solver pass."""
+    from kelvin_amd._lib import FTCCError
+    dev()
+    eng, ti, g, G, beta = _flip_engine()
+    assert eng.spin_mode() == 15
+    n = eng.no
+    h = n // 2
+    T1, T2 = eng.empty_amplitudes()
+    eng.driving_terms(T1, T2)
+    eng.integrate(T1, T2)
+    R1, R2 = eng.empty_amplitudes()
+    eng.set_trusted(True)
+    eng.residual(T1, T2, R1, R2)            # the solver's own iterate: kept
+    ref1, ref2 = to_np(R1), to_np(R2)
+    # an alpha-alpha-alpha-alpha element changed without its beta image
+    bad2 = T2.clone()
+    bad2[3, 0, 1, 0, 1] += 1e-3
+    bad2[3, 1, 0, 0, 1] -= 1e-3
+    bad2[3, 0, 1, 1, 0] -= 1e-3
+    bad2[3, 1, 0, 1, 0] += 1e-3
+    eng.set_trusted(True)
+    with pytest.raises(FTCCError, match="spin flip"):
+        eng.residual(T1, bad2, R1, R2)
+    eng.residual(T1, bad2, R1, R2)          # promise withdrawn: untrusted
+    got = to_np(R2)
+    eng2, *_ = _flip_engine()
+    S1, S2 = eng2.empty_amplitudes()
+    eng2.residual(T1, bad2, S1, S2)
+    assert rel(got, to_np(S2)) < 1e-12
+    # a spin-forbidden amplitude (alpha a, alpha b <- alpha i, beta j)
+    bad1 = T2.clone()
+    bad1[2, 0, 1, 2, h + 3] = 1e-4
+    eng.set_trusted(True)
+    with pytest.raises(FTCCError, match="conserve spin"):
+        eng.residual(T1, bad1, R1, R2)
+    # and the solver's iterate again under a fresh promise
+    eng.set_trusted(True)
+    eng.residual(T1, T2, R1, R2)
+    assert rel(to_np(R2), ref2) < 1e-13 and rel(to_np(R1), ref1) < 1e-13
+    eng.close()
+    eng2.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_trusted_flip_steps_equal_untrusted_steps(precision):
+    """ftcc_step with the trusted flip path (ring products and the
+    step-only skips of the beta-beta blocks, chunks of two grid points
+    smaller than ng, the tau_0 slice skipped) equals the untrusted step
+    element by element over two steps, and so do the step's norms and
+    energy."""
+    from kelvin_amd.cc_utils import zero_slices
+    dev()
+    outs = []
+    for trusted in (True, False):
+        eng, ti, g, G, beta = _flip_engine(precision, ng=6, chunk=2)
+        T1, T2 = eng.empty_amplitudes()
+        eng.driving_terms(T1, T2)
+        eng.integrate(T1, T2)
+        skip = zero_slices(eng, T1, T2, G)
+        assert skip == 1
+        eng.set_trusted(trusted)
+        T1b, T2b = eng.empty_amplitudes()
+        o = [eng.step(T1, T2, T1b, T2b, g, beta, 0.3, skip) for _ in range(2)]
+        outs.append((to_np(T1), to_np(T2), numpy.array(o)))
+        eng.close()
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    assert rel(outs[0][0], outs[1][0]) < tol
+    assert rel(outs[0][1], outs[1][1]) < tol
+    assert numpy.abs(outs[0][2] - outs[1][2]).max() <= tol * numpy.abs(outs[1][2]).max()
+
+
+@pytest.mark.parametrize("low", ["fp32", "bf16x3"])
+def test_defect_correction_reproduces_fp64_step(low):
+    """With the defect Tbar_fp64(T) - Tbar_low(T) as the step's correction,
+    a low-precision step from T equals the FP64 step from T to rounding
+    (the corrected residual is the FP64 one plus a difference of two
+    low-precision evaluations of the same amplitudes, which cancel)."""
+    from kelvin_amd.cc_utils import zero_slices
+    dev()
+    eng, ti, g, G, beta = _flip_engine(low, ng=6)
+    T1, T2 = eng.empty_amplitudes()
+    eng.driving_terms(T1, T2)
+    eng.integrate(T1, T2)
+    skip = zero_slices(eng, T1, T2, G)
+    eng.set_trusted(True)
+    T1b, T2b = eng.empty_amplitudes()
+    eng.step(T1, T2, T1b, T2b, g, beta, 0.0, skip)   # a non-trivial T
+    d1, d2 = eng.empty_amplitudes()
+    eng.defect(T1, T2, low, d1, d2, T1b, T2b, skip)
+    assert eng.precision == low
+    A1, A2 = T1.clone(), T2.clone()
+    eng.set_correction(d1, d2)
+    out_low = eng.step(A1, A2, T1b, T2b, g, beta, 0.3, skip)
+    eng.set_correction(None, None)
+    eng.set_precision("fp64")
+    B1, B2 = T1.clone(), T2.clone()
+    out_64 = eng.step(B1, B2, T1b, T2b, g, beta, 0.3, skip)
+    assert rel(to_np(A1), to_np(B1)) < 1e-12
+    assert rel(to_np(A2), to_np(B2)) < 1e-12
+    assert abs(out_low[4] - out_64[4]) < 1e-13
+    # without the correction the low-precision step differs visibly
+    eng.set_precision(low)
+    C1, C2 = T1.clone(), T2.clone()
+    eng.step(C1, C2, T1b, T2b, g, beta, 0.3, skip)
+    assert rel(to_np(C2), to_np(B2)) > 1e-12
+    eng.close()

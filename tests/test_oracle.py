@@ -352,3 +352,209 @@ def test_td_implicit_propagator_order(scheme, order):
         errs.append(numpy.abs(t1 - exact).max())
     ratio = errs[0] / errs[1]
     assert 0.8 * 2 ** order < ratio < 1.25 * 2 ** order, (errs, ratio)
+
+
+# ------------------------------------------------------------ C1 model
+def test_df_model_integrals_pinned():
+    """The density-fitted model's <pq||rs> carries (pq|rs) = sum_Q B_Qpq
+    B_Qrs in the right slots: its mixed-spin block (p, r alpha; q, s beta)
+    is <pq|rs> = (pr|qs) alone, which as a matrix over (p,r) x (q,s) must be
+    positive semidefinite of rank at most 2*norb (it is B^T B), with the
+    8-fold symmetry of real orbitals; the spin-orbital tensor vanishes unless
+    the spins of (p, q) equal those of (r, s) as a set.  A (pq|rs)-for-
+    (pr|qs) slip breaks the semidefiniteness and the rank."""
+    norb = 8
+    s = ftcc.DFModelSystem(0.1, 0.0, norb=norb, seed=0)
+    eri = s.g_aint_tot()
+    n = 2 * norb
+    a, b = slice(0, norb), slice(norb, n)
+    chem = eri[a, b, a, b].transpose(0, 2, 1, 3)     # (pr|qs) -> [p,r,q,s]
+    assert numpy.abs(chem).max() > 1e-4
+    M = chem.reshape(norb * norb, norb * norb)
+    w = numpy.linalg.eigvalsh(0.5 * (M + M.T))
+    assert w.min() > -1e-14 * w.max()
+    assert int((w > 1e-12 * w.max()).sum()) <= 2 * norb
+    assert numpy.abs(M - M.T).max() < 1e-16
+    for perm in ((1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1), (1, 0, 3, 2)):
+        assert numpy.abs(chem - chem.transpose(perm)).max() < 1e-16
+    # the same B gives the same (pq|rs)
+    ref = numpy.einsum('Qpq,Qrs->pqrs', s.B, s.B)
+    assert numpy.abs(chem - ref).max() < 1e-16
+    # spin pattern of the antisymmetrised integrals
+    sp = numpy.arange(n) // norb
+    allowed = ((sp[:, None, None, None] == sp[None, None, :, None]) &
+               (sp[None, :, None, None] == sp[None, None, None, :])) | \
+              ((sp[:, None, None, None] == sp[None, None, None, :]) &
+               (sp[None, :, None, None] == sp[None, None, :, None]))
+    assert numpy.abs(eri[~allowed]).max() == 0.0
+    assert numpy.abs(eri[allowed]).max() > 1e-4
+    # same-spin block: <pq||rs> = (pr|qs) - (ps|qr)
+    same = chem.transpose(0, 2, 1, 3) - chem.transpose(0, 2, 3, 1)
+    assert numpy.abs(eri[a, a, a, a] - same).max() < 1e-16
+
+
+@pytest.mark.skipif(os.environ.get("FTCC_SLOW") != "1",
+                    reason="several minutes of numpy at n = 38 "
+                           "(FTCC_SLOW=1 runs it)")
+def test_c3_corner_diverges_in_the_reference_algorithm():
+    """At rs = 4, T = 0.05 T_F (the one point of the 33-k-point sweep where
+    the GPU's damped iteration does not converge) the reference's own
+    algorithm diverges too: the oracle at 19 k-points (n = 38), ng = 16,
+    damping 0.6 runs away (|E| > 1e3 within 30 iterations; NaN at iteration
+    28 when continued)."""
+    L, T, Ef = ftcc.ueg_params(19, 4.0, 14, 0.05)
+    Emax = 0.5 * (2 * numpy.pi / L) ** 2 * 8.0
+    s0 = ftcc.UEGSystem(T, L, Emax, 0.0, norb=19)
+    mu = ftcc.solve_mu(s0.g_energies_tot(), 1.0 / T, 14)
+    s = ftcc.UEGSystem(T, L, Emax, mu, norb=19)
+
+    class Stop(Exception):
+        pass
+
+    class Watch(list):
+        def append(self, x):
+            super().append(x)
+            if not numpy.isfinite(x[0]) or abs(x[0]) > 1e3:
+                raise Stop()
+
+    hist = Watch()
+    with numpy.errstate(all="ignore"):
+        with pytest.raises(Stop):
+            ftcc.ft_ccsd(s, T, mu, ngrid=16, damp=0.6, max_iter=30,
+                         econv=1e-10, tconv=1e-7, history=hist)
+    assert len(hist) <= 30
+
+
+# ------------------------------------------------------------ adjoint
+def _random_blocks(no, nv, rng):
+    n = no + nv
+    f = rng.normal(size=(n, n)) * 0.3
+    eri = fs.random_antisym(n, rng, hermitian=False) * 0.1
+    sl = {'o': slice(0, no), 'v': slice(no, n)}
+    F = ftcc.Blocks(oo=f[sl['o'], sl['o']], ov=f[sl['o'], sl['v']],
+                    vo=f[sl['v'], sl['o']], vv=f[sl['v'], sl['v']])
+    I = ftcc.Blocks({k: numpy.ascontiguousarray(
+        eri[sl[k[0]], sl[k[1]], sl[k[2]], sl[k[3]]])
+        for k in ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo', 'oovv', 'vooo',
+                  'ooov', 'oooo')})
+    return F, I
+
+
+def _jt_mismatch(F, I, t1, t2, v1, v2, w1, w2, g1, g2, h=1.0):
+    """|<g, w> - d/dh <v, stanton_rhs(t + h w)>| / |...|: the residual is a
+    polynomial of degree 4 in h, so the five-point central difference is
+    its derivative at 0 exactly (up to rounding), at any h."""
+    def f(x):
+        r1, r2 = ftcc.stanton_rhs(F, I, t1 + x * w1, t2 + x * w2)
+        return (v1 * r1).sum() + (v2 * r2).sum()
+    d = (f(-2 * h) - 8 * f(-h) + 8 * f(h) - f(2 * h)) / (12 * h)
+    a = (g1 * w1).sum() + (g2 * w2).sum()
+    return abs(a - d) / abs(d)
+
+
+@pytest.mark.parametrize("no,nv,seed", [(3, 4, 3), (4, 2, 7), (2, 5, 11)])
+def test_residual_transpose_is_exact(no, nv, seed):
+    """stanton_rhs_transpose (the Lambda-equation operator) is the transpose
+    of the residual's Jacobian: <J^T v, w> equals the derivative of
+    <v, R(t + h w)> for random non-Hermitian blocks with o != v and
+    unsymmetric v, w (every component independent), to rounding; a 1e-6
+    change of one component of the result is detected."""
+    rng = numpy.random.default_rng(seed)
+    F, I = _random_blocks(no, nv, rng)
+    t1 = rng.normal(size=(nv, no)) * 0.3
+    t2 = rng.normal(size=(nv, nv, no, no)) * 0.3
+    v1, v2 = rng.normal(size=t1.shape), rng.normal(size=t2.shape)
+    w1, w2 = rng.normal(size=t1.shape), rng.normal(size=t2.shape)
+    g1, g2 = ftcc.stanton_rhs_transpose(F, I, t1, t2, v1, v2)
+    for h in (0.5, 1.0):
+        assert _jt_mismatch(F, I, t1, t2, v1, v2, w1, w2, g1, g2, h) < 1e-12
+    # sensitivity: a planted error far below any single term is caught
+    g2b = g2.copy()
+    g2b[0, 1, 1, 0] += 1e-6 * numpy.abs(g2).max()
+    w2b = numpy.zeros_like(w2)
+    w2b[0, 1, 1, 0] = 1.0
+    assert _jt_mismatch(F, I, t1, t2, v1, v2, w1 * 0, w2b, g1, g2b) > 1e-8
+
+
+def test_adjoint_integration_is_the_transpose():
+    """int_adjoint is the transpose of int_tbar1/2 over the slices (uniform
+    Simpson and the non-uniform 'sin' grid, any G), and kelvin's int_L1 /
+    int_L2 (kelvin/quadrature.py:320-345) are it in the g-scaled convention:
+    int_L(L)[s] = int_adjoint(g L)[s] / g[s]."""
+    rng = numpy.random.default_rng(4)
+    nv, no = 3, 2
+    D1 = rng.uniform(-2, 2, size=(nv, no))
+    D2 = rng.uniform(-2, 2, size=(nv, nv, no, no))
+    for quad in ('lin', 'sin'):
+        ng = 7
+        ti, g, G = ftcc.ft_quad(ng, 2.0, quad)
+        for D in (D1, D2):
+            tb = rng.normal(size=(ng,) + D.shape)
+            lam = rng.normal(size=(ng,) + D.shape)
+            fwd = ftcc.int_tbar_scan(ng, tb, ti, D, G)
+            adj = ftcc.int_adjoint(ng, lam, ti, D, G)
+            a, b = (lam * fwd).sum(), (adj * tb).sum()
+            assert abs(a - b) < 1e-13 * abs(a)
+        L1 = rng.normal(size=(ng, no, nv))
+        ref = ftcc.int_L1(ng, L1, ti, D1, g, G)
+        got = ftcc.int_adjoint(ng, g[:, None, None] * L1.transpose(0, 2, 1),
+                               ti, D1, G) / g[:, None, None]
+        assert numpy.abs(got.transpose(0, 2, 1) - ref).max() < 1e-13
+        L2 = rng.normal(size=(ng, no, no, nv, nv))
+        ref = ftcc.int_L2(ng, L2, ti, D2, g, G)
+        got = ftcc.int_adjoint(
+            ng, g[:, None, None, None, None] * L2.transpose(0, 3, 4, 1, 2),
+            ti, D2, G) / g[:, None, None, None, None]
+        assert numpy.abs(got.transpose(0, 3, 4, 1, 2) - ref).max() < 1e-13
+
+
+def test_esn_golden_G4():
+    """E, S, N from the adjoint of the fixed point equal the values kelvin's
+    lambda-equation compute_ESN printed for examples/ueg_ft_cc_compare.py
+    (examples/ueg_ft_cc_compare.out:46-48; 'u' orbitals there, spin orbitals
+    here), measured 9e-12, 9e-11 and 3e-11 off."""
+    g = GOLD["G4"]
+    s = g["system"]
+    sysg = ftcc.UEGSystem(s["T"], 2 * numpy.pi, s["Emax"], s["mu"],
+                          norb=s["norb"])
+    r = ftcc.ft_ccsd_esn(sysg, s["T"], s["mu"], **g["ccsd"])
+    assert r["lambda_converged"]
+    for k in ("E", "S", "N"):
+        assert abs(r[k] - g[k]) < 1e-9, (k, r[k], g[k])
+
+
+def _omega(make, T, mu, kw):
+    return ftcc.ft_ccsd(make(T, mu), T, mu, **kw)["Omega"]
+
+
+@pytest.mark.parametrize("kind", ["df", "generic"])
+def test_esn_matches_finite_differences(kind):
+    """N = -dOmega/dmu, S = -dOmega/dT and E = Omega + T S + mu N from the
+    adjoint equal central differences (h = 1e-4) of fully converged solves
+    on two small systems: the density-fitted model (E1 occupation
+    dependent, Fock fixed) and a generic system whose Fock matrix depends
+    on the occupations (kelvin's g_fock_d_den term)."""
+    rng = numpy.random.default_rng(5)
+    if kind == "df":
+        def make(T, mu):
+            return ftcc.DFModelSystem(T, mu, norb=3, seed=1, sigma=0.2)
+        T, mu = 0.3, 0.05
+    else:
+        n = 6
+        h = rng.normal(size=(n, n)) * 0.2
+        h = 0.5 * (h + h.T) + numpy.diag(numpy.linspace(-0.6, 0.6, n))
+        eri = fs.random_antisym(n, rng, hermitian=True) * 0.08
+        en = numpy.diag(h).copy() + 0.03
+
+        def make(T, mu):
+            return ftcc.GenericSystem(T, mu, h, eri, en)
+        T, mu = 0.25, -0.02
+    kw = dict(ngrid=8, damp=0.2, max_iter=200, econv=1e-13, tconv=1e-11)
+    r = ftcc.ft_ccsd_esn(make(T, mu), T, mu, **kw)
+    d = 1e-4
+    N = -(_omega(make, T, mu + d, kw) - _omega(make, T, mu - d, kw)) / (2 * d)
+    S = -(_omega(make, T + d, mu, kw) - _omega(make, T - d, mu, kw)) / (2 * d)
+    E = r["Omega"] + T * S + mu * N
+    assert abs(r["N"] - N) < 1e-7
+    assert abs(r["S"] - S) < 1e-7
+    assert abs(r["E"] - E) < 1e-7
