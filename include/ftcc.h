@@ -269,6 +269,85 @@ int ftcc_defect(ftcc_ctx *ctx, const double *T1_dev, const double *T2_dev,
                 int low_precision, double *dT1bar_dev, double *dT2bar_dev,
                 double *S1_dev, double *S2_dev, uint64_t skip_mask);
 
+/* ---- Adjoint of the converged fixed point (E, S, N) ----------------------
+ * kelvin's compute_ESN (kelvin/ccsd.py:128-214) solves lambda equations
+ * (ft_lambda_iter, kelvin/cc_utils.py:414-480, over cqcpy _Lambda_opt via
+ * kelvin/ft_cc_equations.py:385-409) and contracts response densities with
+ * the derivative integrals (kelvin/ccsd.py:1075-1148, 1333-1492).  These
+ * entry points give the same derivative as the adjoint of T = Phi(T):
+ *   Lam = dE/dT - J^T(Int^T Lam),  dOmega_cc = dE + sum_y <Lam_y, dPhi_y>.
+ * Adjoint amplitudes Lam1[y][a][i], Lam2[y][a][b][i][j] (the amplitude
+ * layout) are kelvin's L up to a factor per grid point:
+ *   Lam1_y = -(g_y/beta) L1_y^T,   Lam2_y = -(g_y/(4 beta)) L2_y^T. */
+
+/* Fix the amplitudes T1_dev/T2_dev (read again by ftcc_adjoint_apply: keep
+ * them unchanged until ftcc_adjoint_end) and build the amplitude-only
+ * intermediates of J^T for as many grid points as FTCC_ADJOINT_CACHE_GB
+ * (default 64) holds; the others rebuild theirs at each application.
+ * Releases the residual workspace of the forward iteration. */
+int ftcc_adjoint_begin(ftcc_ctx *ctx, const double *T1_dev,
+                       const double *T2_dev);
+/* Grid points whose intermediates are kept. */
+int ftcc_adjoint_cached(ftcc_ctx *ctx, int *nslices);
+/* G = J^T Lt at every grid point: the transpose of the Jacobian of the
+ * residual (cqcpy _Lambda_opt's operator), Lt2 antisymmetric in (a,b) and
+ * in (i,j); G2 is projected onto such tensors.  FP64. */
+int ftcc_adjoint_apply(ftcc_ctx *ctx, const double *Lt1_dev,
+                       const double *Lt2_dev, double *G1_dev, double *G2_dev);
+int ftcc_adjoint_end(ftcc_ctx *ctx);
+/* out[x] = sum_y G[y,x] exp((t_x - t_y) D) L[y] (x < y; weight 1 for
+ * x >= y): the transpose of ftcc_integrate over the grid points
+ * (kelvin/quadrature.py:320-345 int_L1/int_L2 without their g weights). */
+int ftcc_integrate_adjoint(ftcc_ctx *ctx, const double *L1_dev,
+                           const double *L2_dev, double *out1_dev,
+                           double *out2_dev);
+/* One damped lambda step after ftcc_adjoint_apply:
+ *   Lam <- damp Lam + (1 - damp) (dE/dT - G)
+ * with dE/dT1_y = (g_y/beta)(F.ov^T + oovv.t1_y), dE/dT2_y = (g_y/(4 beta))
+ * oovv^T; out4 = {|dL1|^2, |dL2|^2, |L1|^2, |L2|^2} in kelvin's lambda
+ * scale (the norms of kelvin/cc_utils.py:440-470). */
+int ftcc_lambda_update(ftcc_ctx *ctx, double *Lam1_dev, double *Lam2_dev,
+                       const double *G1_dev, const double *G2_dev,
+                       const double *T1_dev, const double *g_host,
+                       double beta, double damp, double out4[4]);
+/* kelvin's first lambda amplitudes (kelvin/ft_cc_equations.py:502-512) in
+ * the adjoint layout. */
+int ftcc_lambda_guess(ftcc_ctx *ctx, const double *T1_dev,
+                      const double *g_host, double beta, double *Lam1_dev,
+                      double *Lam2_dev);
+/* Adjoint amplitudes <-> kelvin's L1[y][i][a], L2[y][i][j][a][b]
+ * (to_kelvin != 0: src is Lam, dst is L). */
+int ftcc_lambda_convert(ftcc_ctx *ctx, int to_kelvin, const double *g_host,
+                        double beta, const double *src1_dev,
+                        const double *src2_dev, double *dst1_dev,
+                        double *dst2_dev);
+/* The change of the integrated amplitudes with beta through the grid,
+ * contracted with Lam: sum_yx Lam_y (Gd[y,x] - G[y,x] (t_y - t_x) D / beta)
+ * P_yx Tbar_x, with Tbar the residual at the solution and Gd_host = dG/dbeta
+ * (kelvin/quadrature.py:238-256 d_ft_quad); the grid terms of
+ * kelvin/ccsd.py:1113-1146. */
+int ftcc_grid_derivative(ftcc_ctx *ctx, const double *Lam1_dev,
+                         const double *Lam2_dev, const double *T1bar_dev,
+                         const double *T2bar_dev, const double *Gd_host,
+                         double beta, double *out);
+/* sum_k x[k] y[k] over n doubles. */
+int ftcc_dot(ftcc_ctx *ctx, const double *x_dev, const double *y_dev,
+             int64_t n, double *out);
+/* Replace the context's integrals by their derivative along a change of
+ * x_p = beta (e_p - mu) by dx_dev[p], the bare Fock matrix moving by
+ * dfock_dev[n*n]: every Fermi-weighted block of ftcc_set_system
+ * differentiated through sqrt(fo), sqrt(fv) and the Fock matrix (what
+ * kelvin contracts with its response densities in g_d_on_oo, g_d_on_vv and
+ * g_Fd_on, kelvin/cc_utils.py:1628-1700).  The residual is linear in the
+ * integrals, so ftcc_residual then gives dTbar/dtheta and ftcc_energy the
+ * explicit derivative of the grand potential.  ftcc_set_system restores
+ * the integrals. */
+int ftcc_set_system_response(ftcc_ctx *ctx, int n, const double *en_dev,
+                             const double *fock_dev, const double *dfock_dev,
+                             const double *eri_dev, const double *dx_dev,
+                             double beta, double mu, const int *iocc_host,
+                             const int *ivir_host);
+
 /* Quadrature weights g_host[ng] for ftcc_step_graph. */
 int ftcc_set_weights(ftcc_ctx *ctx, const double *g_host);
 

@@ -54,6 +54,30 @@ SIGNATURES = {
     "ftcc_get_chunk": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "ftcc_set_trusted": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ftcc_set_correction": (ctypes.c_int, [_vp, _vp, _vp]),
+    "ftcc_adjoint_begin": (ctypes.c_int, [_vp, _vp, _vp]),
+    "ftcc_adjoint_cached": (ctypes.c_int, [_vp,
+                                           ctypes.POINTER(ctypes.c_int)]),
+    "ftcc_adjoint_apply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "ftcc_adjoint_end": (ctypes.c_int, [_vp]),
+    "ftcc_integrate_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "ftcc_lambda_update": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp,
+                                          _c_double_p, ctypes.c_double,
+                                          ctypes.c_double, _c_double_p]),
+    "ftcc_grid_derivative": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp,
+                                            _c_double_p, ctypes.c_double,
+                                            _c_double_p]),
+    "ftcc_dot": (ctypes.c_int, [_vp, _vp, _vp, _i64, _c_double_p]),
+    "ftcc_lambda_guess": (ctypes.c_int, [_vp, _vp, _c_double_p,
+                                         ctypes.c_double, _vp, _vp]),
+    "ftcc_lambda_convert": (ctypes.c_int, [_vp, ctypes.c_int, _c_double_p,
+                                           ctypes.c_double, _vp, _vp, _vp,
+                                           _vp]),
+    "ftcc_set_system_response": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp,
+                                                _vp, _vp, _vp,
+                                                ctypes.c_double,
+                                                ctypes.c_double,
+                                                ctypes.POINTER(ctypes.c_int),
+                                                ctypes.POINTER(ctypes.c_int)]),
     "ftcc_defect": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, _vp,
                                    _vp, _vp, ctypes.c_uint64]),
     "ftcc_set_weights": (ctypes.c_int, [_vp, _c_double_p]),

@@ -135,13 +135,14 @@ def ft_cc_iter_refined(method, T1old, T2old, F, I, D1, D2, g, G, beta, ng,
     ``ft_cc_iter`` (kelvin/cc_utils.py:111-173) in three phases:
 
     1. residuals in the low precision ``low`` ('fp32' or 'bf16x3') until the
-       relative change falls below tconv or stops decreasing (rounding of
-       the low-precision products sets a floor under it);
+       relative change falls below 10 tconv or, once below 1e-4, stops
+       decreasing (rounding of the low-precision products sets a floor
+       under it);
     2. the defect dTbar = Tbar_fp64(T_c) - Tbar_low(T_c) at the current
        iterate T_c is added to every further low-precision residual, which
        moves the fixed point from the low-precision one (about 1e-7 Ha away
-       in bf16x3 at n = 114) to the FP64 one, until the change stops
-       decreasing again;
+       in bf16x3 at n = 114) to the FP64 one, until the change falls below
+       tconv or stops decreasing again;
     3. plain FP64 steps until the reference's test |E - Eold| < econv and
        res < tconv holds, so that convergence is always certified by an FP64
        step.
@@ -198,7 +199,9 @@ def ft_cc_iter_refined(method, T1old, T2old, F, I, D1, D2, g, G, beta, ng,
                 best, flat = res, 0
             else:
                 flat += 1
-            if phase == 1 and (ok or res < tthresh or flat >= 5):
+            near = res < 1e-4        # past the transient of the iteration
+            if phase == 1 and (ok or res < 10.0 * tthresh or
+                               (near and flat >= 5)):
                 t = time.time()
                 dT1 = torch.empty_like(T1old)
                 dT2 = torch.empty_like(T2old)
@@ -207,7 +210,8 @@ def ft_cc_iter_refined(method, T1old, T2old, F, I, D1, D2, g, G, beta, ng,
                 phase, best, flat = 2, float("inf"), 0
                 logging.info("FP64 defect at step %d: %.3f s" %
                              (i, time.time() - t))
-            elif phase == 2 and (ok or (flat >= 4 and steps[2] >= 6)):
+            elif phase == 2 and (ok or res < tthresh or
+                                 (flat >= 4 and steps[2] >= 6)):
                 eng.set_correction(None, None)
                 del dT1, dT2
                 dT1 = dT2 = None

@@ -111,31 +111,44 @@ class ccsd(object):
         torch.cuda.current_stream().wait_stream(stream)
         return out
 
-    def compute_ESN(self, L1=None, L2=None, gderiv=True, delta=1e-4,
-                    richardson=False, fd_econv=1e-12):
+    def compute_ESN(self, L1=None, L2=None, gderiv=True):
         """Energy, entropy and particle number (kelvin/ccsd.py:128-214).
 
-        kelvin differentiates the grand potential of the discretised problem
-        analytically (lambda equations, response densities and the grid
-        derivative, kelvin/ccsd.py:167-214).  Here the same derivatives
-        N = -dOmega/dmu and S = -dOmega/dT are taken by central differences
-        of full GPU solves at (T, mu +- delta) and (T +- delta, mu), each
-        warm-started from the converged amplitudes and converged to
-        ``fd_econv``, with the grid rescaled with beta as in the reference;
-        the truncation error is O(delta^2) (O(delta^4) with
-        ``richardson=True``, which adds four solves at 2*delta).  The zero
-        order parts are the closed forms of kelvin/ccsd.py:175-180;
-        E = Omega + T S + mu N for every part.  L1/L2 are accepted for
-        signature compatibility and unused.  The system must derive its
-        (T, mu)-dependent quantities from its ``T`` and ``mu`` attributes
-        (true of the systems in this package).
-        """
-        if not gderiv:
-            raise NotImplementedError(
-                "gderiv=False (approximate grid derivative) is not built")
+        The derivatives of the discretised grand potential with beta and mu
+        come from the adjoint of the converged fixed point
+        (``kelvin_amd.adjoint.compute_esn``): lambda amplitudes from
+        kelvin's damped iteration (L1 / L2 in kelvin's layout start it, as
+        in kelvin/ccsd.py:926-946), the response of the Fermi-weighted
+        integrals, of the Fock matrix and of the grid, all on the GPU.
+        Sets E, S, N, their 0 / 1 / cc parts and L1, L2 as the reference
+        does; gderiv=False takes the reference's approximate grid
+        derivative (kelvin/ccsd.py:1262-1288).  ``compute_ESN_fd`` gives the
+        same quantities by finite differences of full solves."""
         if not self.finite_T:
             logging.warning(
                 "Computing thermodynamic quantities at zero temperature")
+            raise NotImplementedError(
+                "zero-temperature CCSD is not part of the GPU path")
+        if self.T1 is None or self.engine is None:
+            self.run()
+        from .adjoint import compute_esn
+        return compute_esn(self, L1=L1, L2=L2, gderiv=gderiv)
+
+    def compute_ESN_fd(self, delta=1e-4, richardson=False, fd_econv=1e-12):
+        """E, S and N by central differences of full GPU solves.
+
+        N = -dOmega/dmu and S = -dOmega/dT from solves at (T, mu +- delta)
+        and (T +- delta, mu), each warm-started from the converged
+        amplitudes and converged to ``fd_econv``, with the grid rescaled
+        with beta as in the reference; the truncation error is O(delta^2)
+        (O(delta^4) with ``richardson=True``, which adds four solves at
+        2*delta).  The zero order parts are the closed forms of
+        kelvin/ccsd.py:175-180; E = Omega + T S + mu N for every part.  The
+        system must derive its (T, mu)-dependent quantities from its ``T``
+        and ``mu`` attributes (true of the systems in this package).  A
+        check of ``compute_ESN``, costing several solves.
+        """
+        if not self.finite_T:
             raise NotImplementedError(
                 "zero-temperature CCSD is not part of the GPU path")
         if self.T1 is None:
@@ -284,7 +297,10 @@ class ccsd(object):
                 self.iprint, conv)
             if prec0 != "fp64" and self.refine:
                 # pointwise schedule: plain FP64 iterations of the whole grid
+                # from amplitudes whose spin-flip images agree only to the
+                # low precision's rounding: no trusted promise
                 logging.info("FP64 refinement after %s iterations" % prec0)
+                eng.set_trusted(False)
                 eng.set_precision("fp64")
                 conv_r = dict(conv)
                 if self.refine_max_iter is not None:

@@ -32,139 +32,9 @@
 
 using namespace ftcc;
 
-struct ftcc_ctx {
-  int no = 0, nv = 0, ng = 0, prec = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  std::vector<void *> allocs;
-  std::map<void *, int64_t> alloc_doubles;   // sizes of dalloc's buffers
-  int64_t bytes = 0;
+#include "ftcc_ctx.h"
 
-  // integrals in the reference layouts
-  double *F[4] = {nullptr, nullptr, nullptr, nullptr};   // oo ov vo vv
-  double *I[9] = {};  // vvvv vvvo vovv vvoo vovo oovv vooo ooov oooo
-  // derived layouts
-  double *oovv_ring = nullptr;  // [e,m,f,n] = oovv[m,n,e,f]
-  double *ovvo_ring = nullptr;  // [e,m,b,j] = <mb||ej> = -vovo[b,m,e,j]
-  double *ovvo_ringT = nullptr; // [b,j,e,m] = ovvo_ring[e,m,b,j]
-  double *ooov_p1 = nullptr;    // [m,i,n,e] = ooov[m,n,i,e]
-  double *ooov_iemn = nullptr;  // [i,e,m,n] = ooov[n,m,i,e]
-  double *oovvT = nullptr;      // [a,b,i,j] = oovv[i,j,a,b]
-  double *oovv_mefn = nullptr;  // [m,e,f,n] = oovv[m,n,e,f]
-  double *vovv_aefm = nullptr;  // [a,e,f,m] = vovv[a,m,e,f]
-  double *vovv_embf = nullptr;  // [e,m,b,f] = vovv[b,m,e,f]
-  double *ooov_emnj = nullptr;  // [e,m,n,j] = ooov[m,n,j,e]
-  double *fovT = nullptr;       // [a,i] = F.ov[i,a]
-  double *eo = nullptr, *ev = nullptr;
-  double *D1 = nullptr, *D2 = nullptr;   // explicit denominators (optional)
-  bool have_D = false;
-  bool have_ints = false;
-  bool have_energies = false;
-  Grid grid;
-
-  // per-slot workspace (nb slots)
-  int nb = 1;
-  int nb_fixed = 0;          // grid points per chunk set by the caller (0: auto)
-  bool trusted = false;      // caller promises spin-conserving amplitudes
-  // the promise was given and the next residual call checks it once on the
-  // amplitudes it receives
-  bool trust_pending = false;
-  // fixed corrections added to Tbar before the integration of ftcc_step
-  // (caller-owned, amplitude layout; nullptr: none)
-  const double *corr1 = nullptr, *corr2 = nullptr;
-  // one FT-CCSD step captured as a HIP graph, and the arguments it was
-  // captured for (the first call with new arguments runs eagerly, so that
-  // every lazily allocated buffer exists before capture)
-  hipGraph_t step_graph = nullptr;
-  hipGraphExec_t step_exec = nullptr;
-  const void *gkey[4] = {nullptr, nullptr, nullptr, nullptr};
-  double gbeta = 0.0, gdamp = 0.0;
-  uint64_t gskip = 0;
-  int gprec = -1;
-  bool gwarm = false;
-  // imaginary-time propagation: stage input, accumulator and right-hand
-  // side (singles / doubles), allocated on first use
-  double *td_var[2] = {}, *td_acc[2] = {}, *td_k[2] = {}, *td_k1[2] = {};
-  double *tau = nullptr, *t2r, *aring, *wmnij, *wabef,
-      *wr, *z, *u, *s1, *s2;
-  double *t1T, *Fme, *FmeT, *Fae, *Fmi, *Fbe, *Fmj, *ytmp;
-  Scratch scr;
-  double *partials = nullptr;   // 3*MAX blocks
-  double *red = nullptr;        // device scalars
-  double *host_red = nullptr;   // pinned
-
-  // event timing of the O(n^6) products
-  struct Pending {
-    hipEvent_t e0, e1;
-    double flops;
-  };
-  // split-bf16 operands of the five O(n^6) products (bf16x3 mode)
-  SplitMat sp_tauA, sp_tauB, sp_t2rA, sp_aringB, sp_wmnijB, sp_wabefA, sp_wrB;
-  SplitMat sp_oovvA, sp_ringA;
-  bool split_fixed_valid = false;
-  int split_layout = -1;        // spin state the split buffers are sized for
-
-  // Spin blocking of the O(n^6) products: when every integral block
-  // conserves spin under the orbital labels, the pair indices of the ten
-  // product operands are reordered by spin class and each product runs as
-  // three dense block products (10/64 of the dense work).
-  int spin = 0;
-  int *d_lab_o = nullptr, *d_lab_v = nullptr;
-  // occupied indices grouped by spin label: d_jl[0, jl_cnt[0]) the label-0
-  // ones, then the label-1 ones (in index order)
-  int *d_jl = nullptr;
-  int jl_cnt[2] = {0, 0};
-  // O(n^5) contractions restricted to spin-allowed blocks: needs each label
-  // list to be one run of 0s then one run of 1s (spin5 = 1); starts/counts
-  // of the two runs in the occupied and virtual index lists
-  int spin5 = 0;
-  int o_start[2] = {0, 0}, o_count[2] = {0, 0};
-  int v_start[2] = {0, 0}, v_count[2] = {0, 0};
-  // Spin-flip symmetry (flip = 1): equal alpha and beta runs and every
-  // integral, Fock block and orbital energy unchanged when each index moves
-  // to its partner of the other spin.  Trusted amplitudes (the solver's own
-  // iterates) then share the symmetry, and the ring products and the
-  // alpha-lead O(n^5) terms compute half their blocks, the rest being their
-  // flipped images (flip_now: in force for the residual being evaluated).
-  int flip = 0;
-  bool flip_now = false;
-  // under flip_now: an output index letter of the spin-restricted GEMMs
-  // whose beta assignments are skipped (their blocks are flipped images,
-  // copied or read in place once the output is complete); 0 = none
-  char flip_lead = 0;
-  // output index letters (x, y) of the spin-restricted GEMMs whose
-  // assignments with x beta and y alpha are skipped: the packed ladder
-  // operands never read that class of a pair; 0 = none
-  char skip_ba[2] = {0, 0};
-  // output index letters (x, y) whose assignments with both beta are
-  // skipped (letters of a pair index allowed); 0 = none
-  char skip_bb[2] = {0, 0};
-  // the ladder products skip their beta-beta | beta-beta block
-  bool ladder_skip_bb = false;
-  // ftcc_step under the flip symmetry (for the chunk being evaluated): the
-  // ladder operands' beta-beta blocks are read by no product
-  bool flip_step_now = false;
-  // set by ftcc_step around its residual: its update reads the spin-allowed
-  // doubles of Tbar only, so the assembly leaves the forbidden ones unwritten
-  bool tbar_scratch = false;
-  PairSpace ps_vv, ps_oo, ps_vo;   // (a,b), (i,j) and ring (a,i) pairs
-  // Ladder products contract over an antisymmetric pair, so they sum over
-  // the packed pairs only (e < f within a spin, alpha-beta once) and double
-  // the result: packed (e,f) and (m,n) spaces, used for their K index
-  int pack = 0;
-  PairSpace ps_vvp, ps_oop;
-  double *b64[10] = {};            // FP64 blocked operands (fp64/fp32 modes)
-  int b64_layout = -1;
-  bool b64_fixed_valid = false;
-
-  bool prof_on = false;
-  std::vector<Pending> prof_pending;
-  std::vector<hipEvent_t> ev_pool;
-  double prof_ms = 0.0, prof_flops = 0.0;
-  int64_t prof_launches = 0;
-};
-
-namespace {
+namespace ftcc_core {
 
 constexpr int MAX_PARTIAL_BLOCKS = 8192;
 
@@ -194,21 +64,6 @@ void dfree(ftcc_ctx *c, void *p) {
   FTCC_CHECK_HIP(hipFree(p));
 }
 
-template <typename Fn>
-int guarded(ftcc_ctx *c, Fn fn) {
-  if (!c) return FTCC_EINVAL;
-  try {
-    fn();
-    c->err.clear();
-    return FTCC_OK;
-  } catch (const Error &e) {
-    c->err = e.msg;
-    return e.code;
-  } catch (const std::exception &e) {
-    c->err = e.what();
-    return FTCC_EINVAL;
-  }
-}
 
 int64_t p4(int a, int b, int c, int d) {
   return (int64_t)a * b * c * d;
@@ -381,13 +236,6 @@ void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false) {
         [&] { gemm_run(g, prec, c->scr, c->stream); });
 }
 
-// One index of a spin-restricted contraction: its letter, its orbital role
-// ('o' or 'v') and its stride in A, B and C (C's strides also address D).
-struct SIdx {
-  char l, role;
-  int64_t sa, sb, sc;
-  char l2 = 0;   // pair roles: the second letter
-};
 
 // Spin conservation rule of an operand, "mn|ie": the spins of the letters
 // left of '|' sum to those right of it ("e|j": a spin-diagonal matrix).
@@ -2011,7 +1859,9 @@ void read_scalars(ftcc_ctx *c, int n) {
   flush_profile(c);
 }
 
-}  // namespace
+}  // namespace ftcc_core
+
+using namespace ftcc_core;
 
 extern "C" {
 
@@ -2050,6 +1900,7 @@ int ftcc_create(ftcc_ctx **out, int no, int nv, int ng, int precision,
 int ftcc_destroy(ftcc_ctx *c) {
   if (!c) return FTCC_OK;
   (void)hipStreamSynchronize(c->stream);
+  adjoint_release(c);
   for (auto &p : c->prof_pending) {
     (void)hipEventDestroy(p.e0);
     (void)hipEventDestroy(p.e1);

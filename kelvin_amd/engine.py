@@ -333,6 +333,130 @@ class FTEngine:
         self._call("ftcc_defect", _vp(T1), _vp(T2), PREC[low], _vp(dT1b),
                    _vp(dT2b), _vp(S1), _vp(S2), ctypes.c_uint64(skip_mask))
 
+    # -- adjoint ---------------------------------------------------------------
+    def adjoint_begin(self, T1, T2):
+        """Fix the amplitudes of the adjoint (kept by reference: unchanged
+        until ``adjoint_end``) and build their intermediates."""
+        self._check(T1, self.t1_shape(), "T1")
+        self._check(T2, self.t2_shape(), "T2")
+        self._adj_T = (T1, T2)
+        self._call("ftcc_adjoint_begin", _vp(T1), _vp(T2))
+
+    def adjoint_cached(self):
+        """Grid points whose adjoint intermediates are kept."""
+        n = ctypes.c_int()
+        self._call("ftcc_adjoint_cached", ctypes.byref(n))
+        return n.value
+
+    def adjoint_apply(self, Lt1, Lt2, G1, G2):
+        """(G1, G2) = J^T (Lt1, Lt2) at every grid point (Lt2 antisymmetric;
+        G2 projected onto antisymmetric tensors)."""
+        self._check_amps(Lt1, Lt2, G1, G2)
+        self._call("ftcc_adjoint_apply", _vp(Lt1), _vp(Lt2), _vp(G1),
+                   _vp(G2))
+
+    def adjoint_end(self):
+        self._call("ftcc_adjoint_end")
+        self._adj_T = None
+
+    def integrate_adjoint(self, L1, L2, out1, out2):
+        """out[x] = sum_y G[y,x] exp(-(t_y - t_x) D) L[y] (x < y; weight 1
+        otherwise): the transpose of ``integrate``."""
+        self._check_amps(L1, L2, out1, out2)
+        self._call("ftcc_integrate_adjoint", _vp(L1), _vp(L2), _vp(out1),
+                   _vp(out2))
+
+    def lambda_update(self, L1, L2, G1, G2, T1, g, beta, damp):
+        """L <- damp L + (1 - damp) (dE/dT - G); returns the squared norms
+        (|dL1|^2, |dL2|^2, |L1|^2, |L2|^2) in kelvin's lambda scale."""
+        self._check_amps(L1, L2, G1, G2)
+        self._check(T1, self.t1_shape(), "T1")
+        g_a, g_p = _dbuf(g)
+        out = (ctypes.c_double * 4)()
+        self._call("ftcc_lambda_update", _vp(L1), _vp(L2), _vp(G1), _vp(G2),
+                   _vp(T1), g_p, float(beta), float(damp), out)
+        return tuple(out)
+
+    def lambda_guess(self, T1, g, beta, L1, L2):
+        """kelvin's first lambda amplitudes (kelvin/ft_cc_equations.py:
+        502-512) in the adjoint layout: -(1/beta) dE/dT."""
+        self._check_amps(T1, L2, L1, L2)
+        g_a, g_p = _dbuf(g)
+        self._call("ftcc_lambda_guess", _vp(T1), g_p, float(beta), _vp(L1),
+                   _vp(L2))
+
+    def lambda_to_kelvin(self, L1, L2, g, beta):
+        """kelvin's L1[y,i,a], L2[y,i,j,a,b] (scale of kelvin/ccsd.py:
+        1113-1146) from the adjoint amplitudes."""
+        self._check(L1, self.t1_shape(), "Lam1")
+        self._check(L2, self.t2_shape(), "Lam2")
+        z = dict(dtype=torch.float64, device=self.device)
+        K1 = torch.empty((self.ng, self.no, self.nv), **z)
+        K2 = torch.empty((self.ng, self.no, self.no, self.nv, self.nv), **z)
+        g_a, g_p = _dbuf(g)
+        self._call("ftcc_lambda_convert", 1, g_p, float(beta), _vp(L1),
+                   _vp(L2), _vp(K1), _vp(K2))
+        return K1, K2
+
+    def lambda_from_kelvin(self, K1, K2, g, beta):
+        """The adjoint amplitudes of kelvin's L1, L2."""
+        K1 = self.to_device(K1)
+        K2 = self.to_device(K2)
+        self._check(K1, (self.ng, self.no, self.nv), "L1")
+        self._check(K2, (self.ng, self.no, self.no, self.nv, self.nv), "L2")
+        L1, L2 = self.empty_amplitudes()
+        g_a, g_p = _dbuf(g)
+        self._call("ftcc_lambda_convert", 0, g_p, float(beta), _vp(K1),
+                   _vp(K2), _vp(L1), _vp(L2))
+        return L1, L2
+
+    def grid_derivative(self, L1, L2, T1b, T2b, Gd, beta):
+        """sum_yx L_y (Gd[y,x] - G[y,x] (t_y - t_x) D / beta) P_yx Tbar_x."""
+        self._check_amps(L1, L2, T1b, T2b)
+        Gd_a, Gd_p = _dbuf(Gd)
+        if Gd_a.shape != (self.ng, self.ng):
+            raise ValueError("Gd must be (ng, ng)")
+        out = ctypes.c_double()
+        self._call("ftcc_grid_derivative", _vp(L1), _vp(L2), _vp(T1b),
+                   _vp(T2b), Gd_p, float(beta), ctypes.byref(out))
+        return out.value
+
+    def dot(self, x, y):
+        if x.shape != y.shape or not (x.is_contiguous() and y.is_contiguous()):
+            raise ValueError("dot needs two contiguous tensors of one shape")
+        for t in (x, y):
+            if t.device != self.device or t.dtype != torch.float64:
+                raise ValueError("dot needs float64 device tensors")
+        out = ctypes.c_double()
+        self._call("ftcc_dot", _vp(x), _vp(y), ctypes.c_int64(x.numel()),
+                   ctypes.byref(out))
+        return out.value
+
+    def set_system_response(self, en, fock, dfock, eri, dx, beta, mu,
+                            iocc=None, ivir=None):
+        """Replace the integrals by their derivative along a change of
+        x_p = beta (e_p - mu) by dx_p, with the bare Fock matrix moving by
+        dfock (a later ``set_system`` restores them)."""
+        en = self.to_device(en)
+        n = en.shape[0]
+        fock = self.to_device(fock)
+        dfock = self.to_device(dfock)
+        dx = self.to_device(dx)
+        self._check(fock, (n, n), "fock")
+        self._check(dfock, (n, n), "dfock")
+        self._check(eri, (n, n, n, n), "eri")
+        self._check(dx, (n,), "dx")
+        io = numpy.arange(n, dtype=numpy.int32) if iocc is None \
+            else numpy.asarray(iocc, dtype=numpy.int32)
+        iv = numpy.arange(n, dtype=numpy.int32) if ivir is None \
+            else numpy.asarray(ivir, dtype=numpy.int32)
+        if io.size != self.no or iv.size != self.nv:
+            raise ValueError("active lists do not match (no, nv)")
+        ip = ctypes.POINTER(ctypes.c_int)
+        self._call("ftcc_set_system_response", n, _vp(en), _vp(fock),
+                   _vp(dfock), _vp(eri), _vp(dx), float(beta), float(mu),
+                   io.ctypes.data_as(ip), iv.ctypes.data_as(ip))
+
     def set_chunk(self, nb):
         """Grid points per residual chunk (0: automatic)."""
         self._call("ftcc_set_chunk", int(nb))

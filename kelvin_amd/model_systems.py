@@ -62,6 +62,19 @@ class DFModelSystem(System):
         fo = ft_utils.ff(1.0 / self.T, self._en, self.mu)
         return 0.5 * numpy.einsum('ijij,i,j->', self._eri, fo, fo)
 
+    def g_mp1_den(self):
+        """beta dE1/dx_p with x_p = beta (e_p - mu) (the content of
+        kelvin/ueg_system.py:204-215 for this E1)."""
+        beta = 1.0 / self.T
+        fo = ft_utils.ff(beta, self._en, self.mu)
+        fv = ft_utils.ffv(beta, self._en, self.mu)
+        return -beta * numpy.einsum('ijij,i,j->i', self._eri, fo * fv, fo)
+
+    def g_fock_d_den(self):
+        """The Fock matrix diag(e) does not depend on the occupations."""
+        n = self._en.shape[0]
+        return numpy.zeros((n, n, n))
+
 
 class ArraySystem(System):
     """Provider from explicit arrays: one-body h, antisymmetrised eri,
@@ -107,3 +120,19 @@ class ArraySystem(System):
         fo = self._fo()
         return (numpy.dot(numpy.diag(self.h) - self._en, fo)
                 + 0.5 * numpy.einsum('ijij,i,j->', self._eri, fo, fo))
+
+    def g_mp1_den(self):
+        """beta dE1/dx_p with x_p = beta (e_p - mu)."""
+        beta = 1.0 / self.T
+        fo = self._fo()
+        fv = ft_utils.ffv(beta, self._en, self.mu)
+        return -beta * fo * fv * ((numpy.diag(self.h) - self._en)
+                                  + numpy.einsum('ijij,j->i', self._eri, fo))
+
+    def g_fock_d_den(self):
+        """[p,q,k] = <pk||qk> fo_k fv_k (kelvin/ueg_system.py:498-510 for
+        this mean-field Fock matrix)."""
+        beta = 1.0 / self.T
+        fo = self._fo()
+        fv = ft_utils.ffv(beta, self._en, self.mu)
+        return numpy.einsum('piqi,i->pqi', self._eri, fo * fv)

@@ -1,0 +1,258 @@
+"""The adjoint of the FT-CCSD fixed point on the GPU: the transposed
+residual and integrator against the oracle, the lambda amplitudes and
+E, S, N against kelvin's printed lambda-equation values, finite differences
+of full solves and the oracle's adjoint, and the autograd wrapper."""
+import json
+import os
+
+import numpy
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from oracle import ftcc as oracle  # noqa: E402
+from oracle import fock_space as fs  # noqa: E402
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_goldens.json")))
+
+
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return numpy.abs(a - b).max() / max(numpy.abs(b).max(), 1e-300)
+
+
+def random_problem(no, nv, rng):
+    n = no + nv
+    f = rng.normal(size=(n, n)) * 0.3
+    eri = fs.random_antisym(n, rng, hermitian=False) * 0.1
+    sl = {'o': slice(0, no), 'v': slice(no, n)}
+    F = oracle.Blocks(oo=f[sl['o'], sl['o']], ov=f[sl['o'], sl['v']],
+                      vo=f[sl['v'], sl['o']], vv=f[sl['v'], sl['v']])
+    I = oracle.Blocks({k: numpy.ascontiguousarray(
+        eri[sl[k[0]], sl[k[1]], sl[k[2]], sl[k[3]]])
+        for k in ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo', 'oovv', 'vooo',
+                  'ooov', 'oooo')})
+    return F, I
+
+
+def antisym(x):
+    x = x - numpy.swapaxes(x, -4, -3)
+    return x - numpy.swapaxes(x, -2, -1)
+
+
+def _engine(F, I, no, nv, ng, rng):
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd.ov_blocks import one_e_blocks, two_e_blocks
+    eng = FTEngine(no, nv, ng)
+    eng.set_blocks(one_e_blocks(**{k: F[k] for k in ('oo', 'ov', 'vo', 'vv')}),
+                   two_e_blocks(**{k: I[k] for k in I}))
+    eng.set_denominators(rng.normal(size=(nv, no)),
+                         rng.normal(size=(nv, nv, no, no)))
+    eng.set_grid(numpy.zeros(ng), numpy.zeros((ng, ng)))
+    return eng
+
+
+@pytest.mark.parametrize("no,nv,ng", [(3, 4, 2), (5, 5, 3), (9, 11, 2)])
+def test_transposed_residual_matches_oracle(no, nv, ng):
+    """ftcc_adjoint_apply: J^T of the residual at every grid point (the
+    kernel of the lambda equations) against the oracle's exact transpose,
+    projected onto antisymmetric doubles, on random non-Hermitian blocks
+    with o != v; with every grid point's intermediates kept and with none
+    kept (rebuilt from the amplitudes at each application)."""
+    d = dev()
+    rng = numpy.random.default_rng(31 * no + nv)
+    F, I = random_problem(no, nv, rng)
+    t1 = rng.normal(size=(ng, nv, no)) * 0.2
+    t2 = antisym(rng.normal(size=(ng, nv, nv, no, no))) * 0.1
+    v1 = rng.normal(size=(ng, nv, no))
+    v2 = antisym(rng.normal(size=(ng, nv, nv, no, no)))
+    ref1 = numpy.empty_like(v1)
+    ref2 = numpy.empty_like(v2)
+    for y in range(ng):
+        g1, g2 = oracle.stanton_rhs_transpose(F, I, t1[y], t2[y], v1[y],
+                                              v2[y])
+        ref1[y], ref2[y] = g1, oracle.antisymmetrize(g2)
+    T1, T2 = (torch.as_tensor(x, device=d) for x in (t1, t2))
+    V1, V2 = (torch.as_tensor(x, device=d) for x in (v1, v2))
+    for cache_gb in ("64", "0"):
+        os.environ["FTCC_ADJOINT_CACHE_GB"] = cache_gb
+        try:
+            eng = _engine(F, I, no, nv, ng, rng)
+            eng.adjoint_begin(T1, T2)
+            assert eng.adjoint_cached() == (ng if cache_gb == "64" else 0)
+            G1, G2 = eng.empty_amplitudes()
+            eng.adjoint_apply(V1, V2, G1, G2)
+            eng.adjoint_end()
+        finally:
+            del os.environ["FTCC_ADJOINT_CACHE_GB"]
+        assert rel(to_np(G1), ref1) < 1e-12
+        assert rel(to_np(G2), ref2) < 1e-12
+        eng.close()
+
+
+@pytest.mark.parametrize("quad", ['lin', 'sin', 'mid', 'L'])
+def test_adjoint_integration_matches_oracle(quad):
+    """ftcc_integrate_adjoint is the oracle's transpose of the integrator
+    (uniform grids by the running power, others by one exp per pair), with
+    denominators from orbital energies."""
+    from kelvin_amd import quadrature
+    from kelvin_amd.engine import FTEngine
+    d = dev()
+    rng = numpy.random.default_rng(41)
+    ng, n = 9, 6
+    ti, g, G = quadrature.ft_quad(ng, 4.0, quad)
+    en = numpy.sort(rng.uniform(-1, 1, n))
+    eng = FTEngine(n, n, ng)
+    eri = fs.random_antisym(n, rng, hermitian=True) * 0.1
+    eng.set_system(en, numpy.diag(en), eri, 1.0 / 0.25, 0.0)
+    eng.set_grid(ti, G)
+    l1 = rng.normal(size=(ng, n, n))
+    l2 = rng.normal(size=(ng, n, n, n, n))
+    D1, D2 = oracle.D1(en, en), oracle.D2(en, en)
+    L1, L2 = (torch.as_tensor(x, device=d) for x in (l1, l2))
+    O1, O2 = eng.empty_amplitudes()
+    eng.integrate_adjoint(L1, L2, O1, O2)
+    assert rel(to_np(O1), oracle.int_adjoint(ng, l1, ti, D1, G)) < 1e-13
+    assert rel(to_np(O2), oracle.int_adjoint(ng, l2, ti, D2, G)) < 1e-13
+    eng.close()
+
+
+def _ueg_gold(key):
+    from kelvin_amd.ueg_system import UEGSystem
+    s = GOLD[key]["system"]
+    L = 2 * numpy.pi if s["L"] == "2*pi" else s["L"]
+    return UEGSystem(s["T"], L, s["Emax"], mu=s["mu"], norb=s["norb"],
+                     orbtype='g'), s
+
+
+@pytest.mark.parametrize("key,precision", [("G4", "fp64"), ("G4", "bf16x3"),
+                                           ("G3", "fp64"), ("G5", "fp64")])
+def test_esn_goldens(key, precision):
+    """compute_ESN against the E, S, N that kelvin's lambda-equation
+    compute_ESN printed (examples/ueg_ft_cc_compare.out:46-48,
+    examples/ueg_ft_ccsd_ESN.out:43-45, bench/ueg_ft_ccsd_ESN19/
+    ulambda_19_04_17.out:42-44), at 1e-6."""
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    g = GOLD[key]
+    sysg, s = _ueg_gold(key)
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], precision=precision, **g["ccsd"])
+    cc.run()
+    E, S, N = cc.compute_ESN()
+    print("ESN %s %s: dE %.2e dS %.2e dN %.2e lambda %d it, %.2f s"
+          % (key, precision, E - g["E"], S - g["S"], N - g["N"],
+             cc.lambda_iter, cc.time_esn))
+    assert cc.lambda_converged
+    assert abs(E - g["E"]) < 1e-6
+    assert abs(S - g["S"]) < 1e-6
+    assert abs(N - g["N"]) < 1e-6
+    assert abs(cc.N0 + cc.N1 + cc.Ncc - N) < 1e-12
+    assert abs(cc.E0 + cc.E1 + cc.Ecc - E) < 1e-12
+
+
+def test_esn_and_lambda_match_oracle_G4():
+    """The G4 system: E, S, N and the lambda amplitudes in kelvin's layout
+    equal the oracle's adjoint (kelvin's lambda iteration restated), the
+    lambda iteration taking the same number of steps."""
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    g = GOLD["G4"]
+    sysg, s = _ueg_gold("G4")
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], **g["ccsd"])
+    cc.run()
+    E, S, N = cc.compute_ESN()
+    osys = oracle.UEGSystem(s["T"], 2 * numpy.pi, s["Emax"], s["mu"],
+                            norb=s["norb"])
+    r = oracle.ft_ccsd_esn(osys, s["T"], s["mu"], **g["ccsd"])
+    assert cc.lambda_iter == r["lambda_iter"]
+    assert abs(E - r["E"]) < 1e-10
+    assert abs(S - r["S"]) < 1e-9
+    assert abs(N - r["N"]) < 1e-10
+    assert rel(to_np(cc.L1), r["L1"]) < 1e-8
+    assert rel(to_np(cc.L2), r["L2"]) < 1e-8
+    # warm start from the converged lambda amplitudes: one iteration
+    cc.compute_ESN(L1=cc.L1.clone(), L2=cc.L2.clone())
+    assert cc.lambda_iter == 1
+    assert abs(cc.E - E) < 1e-9
+
+
+@pytest.mark.parametrize("kind", ["df", "generic"])
+def test_esn_matches_finite_differences(kind):
+    """The adjoint against central differences of full solves (h = 1e-4)
+    on the density-fitted model and on a system whose Fock matrix depends
+    on the occupations (kelvin's g_fock_d_den term), and against the
+    oracle's adjoint (1e-10); gderiv=False against the oracle too."""
+    from kelvin_amd.ccsd import ccsd
+    from kelvin_amd.model_systems import ArraySystem, DFModelSystem
+    dev()
+    rng = numpy.random.default_rng(5)
+    if kind == "df":
+        T, mu = 0.3, 0.05
+        sysg = DFModelSystem(T, mu, norb=3, seed=1, sigma=0.2)
+        osys = oracle.DFModelSystem(T, mu, norb=3, seed=1, sigma=0.2)
+    else:
+        n = 6
+        h = rng.normal(size=(n, n)) * 0.2
+        h = 0.5 * (h + h.T) + numpy.diag(numpy.linspace(-0.6, 0.6, n))
+        eri = fs.random_antisym(n, rng, hermitian=True) * 0.08
+        en = numpy.diag(h).copy() + 0.03
+        T, mu = 0.25, -0.02
+        sysg = ArraySystem(T, mu, h, eri, en)
+        osys = oracle.GenericSystem(T, mu, h, eri, en)
+    kw = dict(ngrid=8, damp=0.2, max_iter=200, econv=1e-13, tconv=1e-11)
+    cc = ccsd(sysg, T=T, mu=mu, **kw)
+    cc.run()
+    E, S, N = cc.compute_ESN()
+    r = oracle.ft_ccsd_esn(osys, T, mu, **kw)
+    assert abs(E - r["E"]) < 1e-10
+    assert abs(S - r["S"]) < 1e-9
+    assert abs(N - r["N"]) < 1e-10
+    Ea, Sa, Na = cc.compute_ESN(gderiv=False)
+    ra = oracle.ft_ccsd_esn(osys, T, mu, gderiv=False, **kw)
+    assert abs(Ea - ra["E"]) < 1e-10
+    assert abs(Na - ra["N"]) < 1e-10
+    fd = ccsd(sysg, T=T, mu=mu, **kw)
+    fd.run()
+    Ef, Sf, Nf = fd.compute_ESN_fd(delta=1e-4, fd_econv=1e-13)
+    assert abs(E - Ef) < 1e-7
+    assert abs(S - Sf) < 1e-7
+    assert abs(N - Nf) < 1e-7
+
+
+def test_autograd_grand_potential():
+    """kelvin_amd.adjoint.grand_potential: Omega(beta, mu) with the adjoint
+    as its backward; the gradient equals (E - mu N - Omega)/beta and -N of
+    compute_ESN and central differences of the forward (h = 1e-4)."""
+    from kelvin_amd.adjoint import grand_potential
+    from kelvin_amd.model_systems import DFModelSystem
+    dev()
+
+    def make(T, mu):
+        return DFModelSystem(T, mu, norb=3, seed=1, sigma=0.2)
+    kw = dict(ngrid=8, damp=0.2, max_iter=200, econv=1e-13, tconv=1e-11)
+    beta = torch.tensor(1.0 / 0.3, dtype=torch.float64, requires_grad=True)
+    mu = torch.tensor(0.05, dtype=torch.float64, requires_grad=True)
+    om = grand_potential(make, beta, mu, **kw)
+    om.backward()
+    h = 1e-4
+
+    def f(b, m):
+        return float(grand_potential(make, torch.tensor(b, dtype=torch.float64),
+                                     torch.tensor(m, dtype=torch.float64), **kw))
+    b0, m0 = float(beta), float(mu)
+    gb = (f(b0 + h, m0) - f(b0 - h, m0)) / (2 * h)
+    gm = (f(b0, m0 + h) - f(b0, m0 - h)) / (2 * h)
+    assert abs(float(beta.grad) - gb) < 1e-7 * max(1.0, abs(gb))
+    assert abs(float(mu.grad) - gm) < 1e-7
