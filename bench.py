@@ -27,6 +27,10 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 MFMA_PER_PRODUCT = {"fp64": 1, "fp32": 1, "bf16x3": 3}
 
 
+def peak_of(args):
+    return PEAK_TFLOPS[args.precision]
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -39,6 +43,7 @@ def parse():
     ap.add_argument("--precision", default="bf16x3",
                     choices=["bf16x3", "fp32", "fp64"])
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp32-steps", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=0,
                     help="grid points per residual chunk (0: the library's "
                          "choice from its workspace budget)")
@@ -62,24 +67,71 @@ def parse():
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sbgemm_traffic_ueg_n114.txt")
 
 
+KERNEL_SOURCE = os.path.join(ROOT, "kelvin_amd", "csrc", "sbgemm.hip")
+
+
+def source_digest(path):
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
 def measured_traffic(precision, n, ng, layout):
     """HBM bytes per launch of the dominant kernel, from the committed
     FETCH_SIZE/WRITE_SIZE profile of this workload (a line
     'traffic_bytes_per_launch <precision> <n> <ng> <layout> <bytes>',
     layout e.g. 'spin-packed-flip-c8': the operand layout, '-flip' when the
     ring products compute the alpha-virtual rows only, and the grid points
-    per launch), or None."""
+    per launch), or None.  The profile names the digest of the kernel
+    source it was taken with ('kernel_source_sha256 <16 hex>'); a profile
+    of another source is not used: hardware counters cannot be read from
+    inside this process, so a stale figure is dropped, not reported."""
     try:
+        digest = source_digest(KERNEL_SOURCE)
+        found, same_source = None, False
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 w = line.split()
+                if len(w) == 2 and w[0] == "kernel_source_sha256":
+                    same_source = w[1] == digest
                 if (len(w) == 6 and w[0] == "traffic_bytes_per_launch"
                         and w[1] == precision and int(w[2]) == n
                         and int(w[3]) == ng and w[4] == layout):
-                    return float(w[5])
+                    found = float(w[5])
+        return found if same_source else None
     except OSError:
         pass
     return None
+
+
+def host_description():
+    """CPU model, logical CPUs and the BLAS numpy calls (the cpu leg's
+    hardware and library)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = None
+    try:
+        from threadpoolctl import threadpool_info
+        info = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+        if info:
+            blas = "%s %s (%s, %d threads)" % (
+                info[0].get("internal_api"), info[0].get("version"),
+                info[0].get("threading_layer"), info[0]["num_threads"])
+    except Exception:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"cpu": model, "logical_cpus": os.cpu_count(),
+            "cpus_allowed": affinity, "blas": blas}
 
 
 def cpu_threads():
@@ -216,7 +268,7 @@ def main():
     save1, save2 = T1.clone(), T2.clone()
 
     eng.profile(True)
-    eng.profile_read()
+    eng.profile_classes()
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     energies = []
@@ -226,18 +278,16 @@ def main():
         log("step %d: E=%.10f" % (k + 1, out[4]))
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t_start
-    launches, gemm_ms, gemm_flops = eng.profile_read()
+    classes = eng.profile_classes()
     eng.profile(False)
+    launches, gemm_ms, gemm_flops, _ = classes["products"]
     value = args.steps / t_mixed
 
-    # FP64 form from the same amplitudes
-    fp64 = None
-    d_omega = None
-    if args.precision != "fp64" and args.fp64_steps > 0:
-        eng.set_precision("fp64")
-        # one untimed step in the new precision (its workspace and operand
-        # buffers are built on first use), then the timed ones from the same
-        # amplitudes as the mixed-precision steps
+    def other_form(prec, steps):
+        """The same step in another precision from the same amplitudes:
+        one untimed step (its workspace and operand buffers are built on
+        first use), then ``steps`` timed ones."""
+        eng.set_precision(prec)
         T1.copy_(save1)
         T2.copy_(save2)
         eng.step(T1, T2, T1b, T2b, g, beta, args.damp, skip)
@@ -245,23 +295,33 @@ def main():
         T2.copy_(save2)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        e64 = []
-        for k in range(args.fp64_steps):
+        es = []
+        for k in range(steps):
             out = eng.step(T1, T2, T1b, T2b, g, beta, args.damp, skip)
-            e64.append(out[4])
-            log("fp64 step %d: E=%.10f" % (k + 1, out[4]))
+            es.append(out[4])
+            log("%s step %d: E=%.10f" % (prec, k + 1, out[4]))
         torch.cuda.synchronize()
-        t64 = time.perf_counter() - t
-        fp64 = {"value": args.fp64_steps / t64, "unit": "iter/s",
-                "ms_per_step": 1e3 * t64 / args.fp64_steps}
-        d_omega = abs(energies[0] - e64[0])
+        dt = time.perf_counter() - t
         eng.set_precision(args.precision)
+        return ({"value": steps / dt, "unit": "iter/s",
+                 "ms_per_step": 1e3 * dt / steps}, es[0])
+
+    fp64 = fp32 = None
+    d_omega = d_omega32 = None
+    e64 = None
+    if args.precision != "fp64" and args.fp64_steps > 0:
+        fp64, e64 = other_form("fp64", args.fp64_steps)
+        d_omega = abs(energies[0] - e64)
+    if args.precision != "fp32" and args.fp32_steps > 0:
+        fp32, e32 = other_form("fp32", args.fp32_steps)
+        if e64 is not None:
+            d_omega32 = abs(e32 - e64)
     del save1, save2
 
     avg_ms = gemm_ms / max(1, launches)
     launch_flops = gemm_flops / max(1, launches)
     achieved = launch_flops / (avg_ms * 1e-3) / 1e12 if launches else 0.0
-    peak = PEAK_TFLOPS[args.precision]
+    peak = peak_of(args)
     roofline = {
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
         "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
@@ -274,15 +334,45 @@ def main():
         "launches": launches,
         "mfma_ops_per_product": MFMA_PER_PRODUCT[args.precision],
     }
+    # every kernel class of the step against the roofline that bounds it:
+    # the O(n^5) GEMMs run in FP64 in every precision; an arithmetic
+    # intensity above the FP64 ridge (78.6 TFLOP/s / 8 TB/s) makes a class
+    # MFMA-bound, else HBM-bound
+    per_class = {}
+    for name, (nl, ms, fl, by) in classes.items():
+        if nl == 0 or ms <= 0.0:
+            continue
+        sec = ms * 1e-3
+        entry = {"launches_per_step": nl / args.steps,
+                 "ms_per_step": ms / args.steps,
+                 "avg_launch_ms": ms / nl}
+        if name == "products":
+            entry.update(bound="mfma", unit="TFLOP/s",
+                         achieved=fl / sec / 1e12, peak=peak_of(args))
+        elif fl > 0.0 and by > 0.0 and fl / by > 78.6 / 8.0:
+            entry.update(bound="mfma", unit="TFLOP/s",
+                         achieved=fl / sec / 1e12, peak=78.6,
+                         hbm_gbs=by / sec / 1e9)
+        else:
+            entry.update(bound="hbm", unit="GB/s", achieved=by / sec / 1e9,
+                         peak=8000.0)
+            if fl > 0.0:
+                entry["tflops"] = fl / sec / 1e12
+        entry["frac"] = entry["achieved"] / entry["peak"]
+        entry["algorithmic_flops_per_step"] = fl / args.steps
+        entry["algorithmic_bytes_per_step"] = by / args.steps
+        per_class[name] = entry
     cpu = None
     if not args.no_cpu_baseline:
         try:
             rate, sample = cpu_baseline(eng, T1, T2, n, ng, args.cpu_budget_s)
             cpu = {"value": rate, "unit": "iter/s", "cores": cpu_threads(),
-                   "kind": "port", "sample": sample}
+                   "kind": "port", "sample": sample,
+                   "host": host_description()}
         except Exception as exc:   # report, do not hide the GPU numbers
             cpu = {"value": None, "unit": "iter/s", "cores": cpu_threads(),
-                   "kind": "port", "sample": "failed: %r" % (exc,)}
+                   "kind": "port", "sample": "failed: %r" % (exc,),
+                   "host": host_description()}
     algo = 12.0 * n ** 6 * ng
     line = {
         "metric": METRIC, "value": value, "unit": "iter/s", "n_gpus": 1,
@@ -298,12 +388,15 @@ def main():
                    "damp": args.damp, "spin_blocked": spin_blocked,
                    "layout": layout, "chunk": eng.chunk()},
         "fp64": fp64,
+        "fp32": fp32,
         "omega_step_delta_vs_fp64": d_omega,
+        "omega_step_delta_fp32_vs_fp64": d_omega32,
         "textbook_tflops": algo * value / 1e12,
         "product_flops_per_iter": gemm_flops / args.steps,
         "omega0_plus_e1": omega0 + e1,
         "energies": energies,
         "roofline": roofline,
+        "kernel_classes": per_class,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -375,6 +375,13 @@ int ftcc_workspace_bytes(const ftcc_ctx *ctx, int64_t *bytes);
 int ftcc_profile_enable(ftcc_ctx *ctx, int on);
 int ftcc_profile_read(ftcc_ctx *ctx, int64_t *launches, double *ms,
                       double *flops);
+/* The same for every kernel class while profiling is on: 0 the O(n^6)
+ * products (as ftcc_profile_read), 1 the other GEMMs (O(n^5) terms), 2 the
+ * product operand gathers, 3 the doubles assembly, 4 the integration and
+ * damping update of ftcc_step; flops and bytes are algorithmic (2MNK; each
+ * operand read and each output written once). */
+int ftcc_profile_class(ftcc_ctx *ctx, int cls, int64_t *launches, double *ms,
+                       double *flops, double *bytes);
 
 /* Batched strided GEMM of the residual engine, exposed for testing:
  * C[b] = alpha * A[b] . B[b] + beta * C[b] with element strides

@@ -55,6 +55,7 @@ def solve_lambda(eng, T1, T2, g, beta, conv, L1=None, L2=None, history=None):
     w2 = 4.0 * beta / g
     thresh, max_iter, damp = conv["tconv"], conv["max_iter"], conv["damp"]
     eng.adjoint_begin(T1, T2)
+    eng.adjoint_slices_kept = eng.adjoint_cached()
     try:
         if L1 is None and L2 is None:
             Lam1, Lam2 = eng.empty_amplitudes()
@@ -149,6 +150,7 @@ def compute_esn(cc, L1=None, L2=None, gderiv=True):
     Lam1, Lam2, lconv, lit = solve_lambda(eng, T1, T2, g, beta, conv, L1=L1,
                                           L2=L2, history=cc.lambda_history)
     cc.lambda_converged, cc.lambda_iter = lconv, lit
+    cc.adjoint_slices_kept = eng.adjoint_slices_kept
     t_lambda = time.time() - t0
     Omega_cc = cc.Gcc
     # the quadrature's dependence on beta

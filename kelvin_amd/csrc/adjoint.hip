@@ -39,6 +39,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -359,7 +360,7 @@ struct Ten {
   const double *p = nullptr;
   std::string l;
   std::vector<int64_t> st;
-  int64_t size = 0;
+  std::string rule;   // spin rule: letters left of '|' sum to those right
   int64_t stride(char x) const {
     const size_t k = l.find(x);
     return k == std::string::npos ? -1 : st[k];
@@ -372,7 +373,11 @@ int letter_dim(const ftcc_ctx *c, char x) {
   return (x >= 'i' && x <= 'n') ? c->no : c->nv;
 }
 
-Ten ten(const ftcc_ctx *c, const double *p, const char *letters) {
+// A dense tensor over `letters` (C order).  Its spin rule is `rule`, or for
+// four letters "pq|rs" and for two "p|q" (true of the integrals, the
+// amplitudes and every intermediate stored in the <pq||rs> order).
+Ten ten(const ftcc_ctx *c, const double *p, const char *letters,
+        const char *rule = nullptr) {
   Ten t;
   t.p = p;
   t.l = letters;
@@ -382,10 +387,16 @@ Ten ten(const ftcc_ctx *c, const double *p, const char *letters) {
     t.st[k] = s;
     s *= letter_dim(c, t.l[k]);
   }
-  t.size = s;
   for (size_t a = 0; a < t.l.size(); ++a)
     for (size_t b = a + 1; b < t.l.size(); ++b)
       require(t.l[a] != t.l[b], "contraction: repeated letter");
+  if (rule) {
+    t.rule = rule;
+  } else if (t.l.size() == 4) {
+    t.rule = t.l.substr(0, 2) + "|" + t.l.substr(2, 2);
+  } else if (t.l.size() == 2) {
+    t.rule = t.l.substr(0, 1) + "|" + t.l.substr(1, 1);
+  }
   return t;
 }
 
@@ -394,8 +405,15 @@ struct Level {
   std::vector<char> ls;   // letters, outer to inner
 };
 
+// the extent of every letter of one (sub-)contraction
+struct Extents {
+  int64_t d[128] = {};
+  int64_t operator()(char x) const { return d[(unsigned char)x]; }
+};
+
 // merge consecutive letters of `order` that are contiguous in every operand
-std::vector<Level> levels_of(const ftcc_ctx *c, const std::vector<char> &order,
+std::vector<Level> levels_of(const Extents &ext,
+                             const std::vector<char> &order,
                              const std::vector<const Ten *> &ops) {
   std::vector<Level> out;
   for (char x : order) {
@@ -403,15 +421,15 @@ std::vector<Level> levels_of(const ftcc_ctx *c, const std::vector<char> &order,
     if (merge) {
       const char prev = out.back().ls.back();
       for (const Ten *t : ops)
-        if (t->stride(prev) != t->stride(x) * letter_dim(c, x)) merge = false;
+        if (t->stride(prev) != t->stride(x) * ext(x)) merge = false;
     }
     if (merge) {
       out.back().ls.push_back(x);
-      out.back().dim *= letter_dim(c, x);
+      out.back().dim *= ext(x);
     } else {
       Level L;
       L.ls.push_back(x);
-      L.dim = letter_dim(c, x);
+      L.dim = ext(x);
       out.push_back(L);
     }
   }
@@ -420,28 +438,49 @@ std::vector<Level> levels_of(const ftcc_ctx *c, const std::vector<char> &order,
 
 int64_t lstride(const Ten &t, const Level &L) { return t.stride(L.ls.back()); }
 
-void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
-         const Ten &C) {
-  // letter groups
+struct Groups {
   std::vector<char> M, N, K;
+};
+
+Groups groups_of(const Ten &A0, const Ten &B0, const Ten &C) {
+  Groups g;
   for (char x : C.l) {
     const bool inA = A0.stride(x) >= 0, inB = B0.stride(x) >= 0;
     require(inA != inB, "contraction: an output letter must be in A or B");
-    (inA ? M : N).push_back(x);
+    (inA ? g.M : g.N).push_back(x);
   }
   for (char x : A0.l)
     if (C.stride(x) < 0) {
       require(B0.stride(x) >= 0, "contraction: a letter of A only");
-      K.push_back(x);
+      g.K.push_back(x);
     }
   for (char x : B0.l)
     if (C.stride(x) < 0)
       require(A0.stride(x) >= 0, "contraction: a letter of B only");
-  struct Best {
-    int score = -1;
-    bool swap = false;
-    std::vector<Level> m, n, k;
-  } best;
+  return g;
+}
+
+struct Shape {
+  int score = -1;
+  bool swap = false;
+  std::vector<Level> m, n, k;
+};
+
+// The level orders of C = A.B over the letter extents `ext` (score < 0:
+// none satisfies the engine's layout rules); a search over the orders of
+// each group and the choice of A, cached per thread by letters, strides
+// and extents.
+Shape shape_of(const Extents &ext, const Ten &A0, const Ten &B0,
+               const Ten &C, const Groups &G) {
+  static thread_local std::map<std::string, Shape> cache;
+  std::string key = A0.l + "," + B0.l + ">" + C.l + ":";
+  for (const Ten *t : {&A0, &B0, &C})
+    for (int64_t x : t->st) key += std::to_string(x) + ",";
+  for (const auto *grp : {&G.M, &G.N, &G.K})
+    for (char x : *grp) key += std::to_string(ext(x)) + ";";
+  const auto hit = cache.find(key);
+  if (hit != cache.end()) return hit->second;
+  Shape best;
   auto perms = [](std::vector<char> v) {
     std::vector<std::vector<char>> out;
     std::sort(v.begin(), v.end());
@@ -449,16 +488,16 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
     while (std::next_permutation(v.begin(), v.end()));
     return out;
   };
-  const auto pM = perms(M), pN = perms(N), pK = perms(K);
+  const auto pM = perms(G.M), pN = perms(G.N), pK = perms(G.K);
   for (int sw = 0; sw < 2; ++sw) {
     const Ten &A = sw ? B0 : A0, &B = sw ? A0 : B0;
     const auto &pm = sw ? pN : pM, &pn = sw ? pM : pN;
     for (const auto &om : pm)
       for (const auto &on : pn)
         for (const auto &ok : pK) {
-          const auto lm = levels_of(c, om, {&A, &C});
-          const auto ln = levels_of(c, on, {&B, &C});
-          const auto lk = levels_of(c, ok, {&A, &B});
+          const auto lm = levels_of(ext, om, {&A, &C});
+          const auto ln = levels_of(ext, on, {&B, &C});
+          const auto lk = levels_of(ext, ok, {&A, &B});
           if (lm.size() > 3 || ln.size() > 3 || lk.size() > 3) continue;
           const int64_t sAm = lm.empty() ? 0 : lstride(A, lm.back());
           const int64_t sBn = ln.empty() ? 0 : lstride(B, ln.back());
@@ -483,10 +522,18 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
           }
         }
   }
-  require(best.score >= 0, "contraction " + A0.l + "," + B0.l + "->" + C.l +
-                               ": no layout the GEMM engine takes");
+  cache[key] = best;
+  return best;
+}
+
+// The GEMM of C = alpha A.B + beta C over the letter extents `ext`, or
+// false when no level order satisfies the engine's layout rules.
+bool plan(const Extents &ext, double alpha, const Ten &A0, const Ten &B0,
+          double beta, const Ten &C, const Groups &G, Gemm &g) {
+  const Shape best = shape_of(ext, A0, B0, C, G);
+  if (best.score < 0) return false;
   const Ten &A = best.swap ? B0 : A0, &B = best.swap ? A0 : B0;
-  Gemm g;
+  g = Gemm();
   g.alpha = alpha;
   g.beta = beta;
   g.A = A.p;
@@ -504,12 +551,8 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
   g.M = (int)Mt;
   g.N = (int)Nt;
   g.K = (int)Kt;
-  // rows
   const auto &m = best.m;
-  if (m.empty()) {
-    g.sAm = 0;
-    g.sCm = 0;
-  } else {
+  if (!m.empty()) {
     g.sAm = lstride(A, m.back());
     g.sCm = lstride(C, m.back());
     if (m.size() >= 2) {
@@ -524,10 +567,7 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
     }
   }
   const auto &n = best.n;
-  if (n.empty()) {
-    g.sBn = 0;
-    g.sCn = 0;
-  } else {
+  if (!n.empty()) {
     g.sBn = lstride(B, n.back());
     g.sCn = lstride(C, n.back());
     if (n.size() >= 2) {
@@ -542,10 +582,7 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
     }
   }
   const auto &k = best.k;
-  if (k.empty()) {
-    g.sAk = 0;
-    g.sBk = 0;
-  } else {
+  if (!k.empty()) {
     g.sAk = lstride(A, k.back());
     g.sBk = lstride(B, k.back());
     if (k.size() >= 2) {
@@ -559,6 +596,96 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
       g.sBk0 = lstride(B, k[0]);
     }
   }
+  return true;
+}
+
+bool rule_ok(const std::string &rule, const int *sp) {
+  if (rule.empty()) return true;
+  int left = 0, right = 0;
+  bool r = false;
+  for (char q : rule) {
+    if (q == '|') {
+      r = true;
+      continue;
+    }
+    (r ? right : left) += sp[(unsigned char)q];
+  }
+  return left == right;
+}
+
+// Element count of a tensor over the letter extents.
+int64_t elems(const Ten &t, const Extents &ext) {
+  int64_t s = 1;
+  for (char x : t.l) s *= ext(x);
+  return s;
+}
+
+// C[...] = alpha sum_K A[...] B[...] + beta C[...] (beta 0 or 1).  When the
+// orbital labels form one alpha run and one beta run in each list, only the
+// spin blocks allowed by the three operands' rules are multiplied (a
+// forbidden block of A or B is zero, one of C stays zero); otherwise, or
+// when a block has no layout the engine takes, the whole product runs
+// dense.
+void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
+         const Ten &C) {
+  require(beta == 0.0 || beta == 1.0, "contraction: beta must be 0 or 1");
+  const Groups G = groups_of(A0, B0, C);
+  Extents full;
+  std::vector<char> letters;
+  for (const Ten *t : {&A0, &B0, &C})
+    for (char x : t->l)
+      if (!full(x)) {
+        full.d[(unsigned char)x] = letter_dim(c, x);
+        letters.push_back(x);
+      }
+  if (c->spin && c->spin5 && letters.size() <= 10) {
+    struct Block {
+      Gemm g;
+    };
+    std::vector<Block> blocks;
+    bool ok = true;
+    const int L = (int)letters.size();
+    for (int mask = 0; mask < (1 << L) && ok; ++mask) {
+      int sp[128] = {};
+      for (int k = 0; k < L; ++k)
+        sp[(unsigned char)letters[k]] = (mask >> k) & 1;
+      if (!rule_ok(A0.rule, sp) || !rule_ok(B0.rule, sp) ||
+          !rule_ok(C.rule, sp))
+        continue;
+      Extents ext;
+      int64_t off[3] = {0, 0, 0};
+      bool empty = false;
+      for (char x : letters) {
+        const bool occ = x >= 'i' && x <= 'n';
+        const int s = sp[(unsigned char)x];
+        const int64_t start = occ ? c->o_start[s] : c->v_start[s];
+        const int64_t cnt = occ ? c->o_count[s] : c->v_count[s];
+        ext.d[(unsigned char)x] = cnt;
+        if (cnt == 0) empty = true;
+        const Ten *ts[3] = {&A0, &B0, &C};
+        for (int q = 0; q < 3; ++q)
+          if (ts[q]->stride(x) >= 0) off[q] += start * ts[q]->stride(x);
+      }
+      if (empty) continue;
+      Ten a = A0, b = B0, cc = C;
+      a.p += off[0];
+      b.p += off[1];
+      cc.p += off[2];
+      Block blk;
+      if (!plan(ext, alpha, a, b, 1.0, cc, G, blk.g)) ok = false;
+      blocks.push_back(blk);
+    }
+    if (ok) {
+      if (beta == 0.0) fill(const_cast<double *>(C.p), elems(C, full), 0.0,
+                            c->stream);
+      for (const Block &blk : blocks) run(c, blk.g, FTCC_PREC_FP64, false);
+      return;
+    }
+  }
+  Gemm g;
+  require(plan(full, alpha, A0, B0, beta, C, G, g),
+          "contraction " + A0.l + "," + B0.l + "->" + C.l +
+              ": no layout the GEMM engine takes");
   run(c, g, FTCC_PREC_FP64, false);
 }
 
@@ -734,7 +861,7 @@ void prepare_slice(ftcc_ctx *c, Adjoint *A, const double *t1,
       ten(c, s.wmbej, "mbej"));
   // Zt[e,a,b,i] = tau.ooov, Zv[a,m,i,j] = vovv.tau
   con(c, 1.0, ten(c, A->tau, "abmn"), ten(c, I[7], "mnie"), 0.0,
-      ten(c, s.zt, "eabi"));
+      ten(c, s.zt, "eabi", "ab|ie"));
   con(c, 1.0, ten(c, I[2], "amef"), ten(c, A->tau, "efij"), 0.0,
       ten(c, s.zv, "amij"));
 }
@@ -761,7 +888,7 @@ void apply_slice(ftcc_ctx *c, Adjoint *A, const double *t1, const double *t2,
   con(c, 2.0, V2, ten(c, I[6], "bmij"), 1.0, ten(c, g1, "am"));      // vooo
   con(c, 2.0, ten(c, I[1], "abej"), V2, 1.0, ten(c, g1, "ei"));      // vvvo
   con(c, 4.0, V2, ten(c, s.wmbej, "mbej"), 0.0, ten(c, A->acc, "aeim"));
-  con(c, 4.0, ten(c, t2, "aeim"), V2, 0.0, ten(c, A->gw, "jbme"));
+  con(c, 4.0, ten(c, t2, "aeim"), V2, 0.0, ten(c, A->gw, "jbme", "mb|ej"));
   // -t1 t1 <mb||ej> term: <mb||ej> = -vovo[b,m,e,j]
   con(c, 1.0, ten(c, t1, "am"), V2, 0.0, ten(c, A->u, "mbij"));
   con(c, 4.0, ten(c, I[4], "bmej"), ten(c, A->u, "mbij"), 1.0,
@@ -773,7 +900,7 @@ void apply_slice(ftcc_ctx *c, Adjoint *A, const double *t1, const double *t2,
   con(c, 1.0, ten(c, s.wefab, "efab"), VT, 0.0, ten(c, A->gtau, "efij"));
   con(c, 1.0, V2, ten(c, s.wmnij, "mnij"), 1.0, ten(c, A->gtau, "abmn"));
   // t1 parts of W_mnij and W_abef through Zt, Zv
-  con(c, 1.0, ten(c, s.zt, "eabi"), VT, 1.0, ten(c, g1, "ej"));
+  con(c, 1.0, ten(c, s.zt, "eabi", "ab|ie"), VT, 1.0, ten(c, g1, "ej"));
   con(c, -1.0, V2, ten(c, s.zv, "amij"), 1.0, ten(c, g1, "bm"));
   // Fock-type terms of the doubles
   con(c, -2.0, V2, ten(c, F.mj, "mj"), 1.0, ten(c, A->acc, "abim"));
@@ -805,18 +932,18 @@ void apply_slice(ftcc_ctx *c, Adjoint *A, const double *t1, const double *t2,
   con(c, 0.5, ten(c, v1, "ai"), ten(c, I[7], "nmie"), 1.0,
       ten(c, A->acc, "aemn"));
   // Wmbej: t1.vovv, t1.ooov and the ring amplitude t2/2 + t1 t1
-  con(c, -1.0, ten(c, A->gw, "jbme"), ten(c, I[2], "bmef"), 1.0,
+  con(c, -1.0, ten(c, A->gw, "jbme", "mb|ej"), ten(c, I[2], "bmef"), 1.0,
       ten(c, g1, "fj"));
-  con(c, 1.0, ten(c, A->gw, "jbme"), ten(c, I[7], "mnje"), 1.0,
+  con(c, 1.0, ten(c, A->gw, "jbme", "mb|ej"), ten(c, I[7], "mnje"), 1.0,
       ten(c, g1, "bn"));
-  con(c, 1.0, ten(c, A->gw, "jbme"), ten(c, I[5], "mnef"), 0.0,
-      ten(c, A->q, "bjnf"));
+  con(c, 1.0, ten(c, A->gw, "jbme", "mb|ej"), ten(c, I[5], "mnef"), 0.0,
+      ten(c, A->q, "bjnf", "bf|jn"));
   // acc[f,b,j,n] -= q[b,j,n,f] / 2
   permute4(A->q, A->acc, V, O, O, V, (int64_t)O * O, (int64_t)O, 1,
            (int64_t)V * O * O, -0.5, 1.0, 1, 0, 0, st);
-  con(c, -1.0, ten(c, A->q, "bjnf"), ten(c, t1, "bn"), 1.0,
+  con(c, -1.0, ten(c, A->q, "bjnf", "bf|jn"), ten(c, t1, "bn"), 1.0,
       ten(c, g1, "fj"));
-  con(c, -1.0, ten(c, A->q, "bjnf"), ten(c, t1, "fj"), 1.0,
+  con(c, -1.0, ten(c, A->q, "bjnf", "bf|jn"), ten(c, t1, "fj"), 1.0,
       ten(c, g1, "bn"));
   // Fmi, Fae, Fme
   con(c, 0.5, ten(c, c->F[1], "me"), ten(c, gFmi, "mi"), 1.0,

@@ -176,23 +176,27 @@ void flush_profile(ftcc_ctx *c) {
   for (auto &p : c->prof_pending) {
     float ms = 0.f;
     FTCC_CHECK_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
-    c->prof_ms += ms;
-    c->prof_flops += p.flops;
-    c->prof_launches += 1;
+    c->prof_ms[p.cls] += ms;
+    c->prof_flops[p.cls] += p.flops;
+    c->prof_bytes[p.cls] += p.bytes;
+    c->prof_launches[p.cls] += 1;
     c->ev_pool.push_back(p.e0);
     c->ev_pool.push_back(p.e1);
   }
   c->prof_pending.clear();
 }
 
-// Run fn() bracketed by events when profiling is on (O(n^6) products only).
+// Run fn() bracketed by events of kernel class cls when profiling is on,
+// with its algorithmic flops and bytes (cls < 0: never timed).
 template <typename Fn>
-void timed(ftcc_ctx *c, bool big, double flops, Fn fn) {
-  if (big && c->prof_on) {
+void timed_class(ftcc_ctx *c, int cls, double flops, double bytes, Fn fn) {
+  if (cls >= 0 && c->prof_on) {
     ftcc_ctx::Pending p;
     p.e0 = take_event(c);
     p.e1 = take_event(c);
+    p.cls = cls;
     p.flops = flops;
+    p.bytes = bytes;
     FTCC_CHECK_HIP(hipEventRecord(p.e0, c->stream));
     fn();
     FTCC_CHECK_HIP(hipEventRecord(p.e1, c->stream));
@@ -202,8 +206,24 @@ void timed(ftcc_ctx *c, bool big, double flops, Fn fn) {
   }
 }
 
+// the O(n^6) products (big) are class PROF_PRODUCTS
+template <typename Fn>
+void timed(ftcc_ctx *c, bool big, double flops, Fn fn) {
+  timed_class(c, big ? (int)ftcc_ctx::PROF_PRODUCTS : -1, flops, 0.0, fn);
+}
+
 double gemm_flops(const Gemm &g) {
   return 2.0 * g.M * (double)g.N * g.K * g.nb1 * g.nb2;
+}
+
+// bytes of a GEMM that reads A and B and writes C once each (C also read
+// when beta != 0); an operand shared by the slices is read once
+double gemm_bytes(const Gemm &g) {
+  const double sl = (double)g.nb1 * g.nb2;
+  const double a = (double)g.M * g.K * ((g.sA1 || g.sA2) ? sl : 1.0);
+  const double b = (double)g.K * g.N * ((g.sB1 || g.sB2) ? sl : 1.0);
+  const double cm = (double)g.M * g.N * (g.reduce_b2 ? g.nb1 : sl);
+  return 8.0 * (a + b + cm * (g.beta != 0.0 ? 2.0 : 1.0));
 }
 
 // A batched GEMM as one launch whose outer row (column) level runs over
@@ -232,8 +252,9 @@ Gemm fold_n(Gemm g) {
 
 void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false) {
   ensure_partial(c, g);
-  timed(c, big, gemm_flops(g),
-        [&] { gemm_run(g, prec, c->scr, c->stream); });
+  timed_class(c, big ? ftcc_ctx::PROF_PRODUCTS : ftcc_ctx::PROF_N5,
+              gemm_flops(g), big ? 0.0 : gemm_bytes(g),
+              [&] { gemm_run(g, prec, c->scr, c->stream); });
 }
 
 
@@ -681,10 +702,18 @@ void operand_blocks(const ftcc_ctx *c, int id, Range rows[3], Range ks[3]) {
 // split(rmap, rows, kmap, K, dst) in bf16x3 mode, f64(rmap, rows, kmap, K,
 // ld, dst, slice stride) otherwise.
 template <typename SplitFn, typename F64Fn>
-void gather_blocks(ftcc_ctx *c, int id, SplitFn split, F64Fn f64) {
+void gather_blocks(ftcc_ctx *c, int id, SplitFn split, F64Fn f64,
+                   int nb = 1) {
   const PairSpace &R = space(c, op_rows(c, id)), &K = space(c, op_k(c, id));
   Range rows[3], ks[3];
   operand_blocks(c, id, rows, ks);
+  // algorithmic bytes: one FP64 read and one write (two bf16 halves or an
+  // FP64 copy) per gathered element
+  double elems = 0.0;
+  for (int q = 0; q < 3; ++q)
+    elems += (double)rows[q].count * ks[q].count * nb;
+  const double per = c->prec == FTCC_PREC_BF16X3 ? 12.0 : 16.0;
+  timed_class(c, ftcc_ctx::PROF_GATHER, 0.0, per * elems, [&] {
   for (int q = 0; q < 3; ++q) {
     if (rows[q].len == 0 || ks[q].len == 0) continue;
     const int *rmap = R.d_map + rows[q].start, *kmap = K.d_map + ks[q].start;
@@ -701,6 +730,7 @@ void gather_blocks(ftcc_ctx *c, int id, SplitFn split, F64Fn f64) {
           (int64_t)R.total * ld);
     }
   }
+  });
 }
 
 // Write operand `id` of nb slots from its FP64 source: src[b*sb + row*sr +
@@ -722,7 +752,8 @@ void put_operand(ftcc_ctx *c, int id, const double *src, int64_t sr,
       [&](const int *rmap, int R, const int *kmap, int W, int64_t ld,
           double *dst, int64_t db) {
         gather64(src, sr, sc, sb, rmap, R, kmap, W, ld, dst, db, nb, s);
-      });
+      },
+      nb);
 }
 
 // G5's B operand Wr of a spin-flip symmetric residual, whose rows of an
@@ -740,7 +771,8 @@ void put_flip_wr(ftcc_ctx *c, const double *wr, int64_t sb, int nb) {
           double *dst, int64_t db) {
         flip_ring_gather(wr, V, O, hv, ho, sb, rmap, R, kmap, W, nullptr, ld,
                          dst, db, nb, s);
-      });
+      },
+      nb);
 }
 
 // Spin layout: operand `id` gathered straight from the amplitudes t1, t2 of
@@ -759,7 +791,8 @@ void put_amp_operand(ftcc_ctx *c, int id, int kind, const double *t1,
           double *dst, int64_t db) {
         amp_gather64(kind, t1, t2, V, O, rmap, R, kmap, W, ld, dst, db, nb,
                      s);
-      });
+      },
+      nb);
 }
 
 // operands that depend on the integrals only
@@ -1618,11 +1651,26 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   }
   // forbidden blocks (spin layout): written as zero without reading
   const bool zflip = flip && c->pack;
-  r2_final(t2bar, c->wr, c->aring, c->s1, c->s2, c->z, V, O, nb, s,
-           c->spin ? c->d_lab_v : nullptr, c->spin ? c->d_lab_o : nullptr,
-           vvoo, c->spin && c->pack, zflip ? hv : 0, zflip ? ho : 0,
-           !(c->tbar_scratch && c->spin && c->pack), c->d_jl, c->jl_cnt[0],
-           c->jl_cnt[1]);
+  // algorithmic bytes: one read of each of the six inputs and one write per
+  // spin-allowed double
+  double allowed = (double)S2;
+  if (c->spin5) {
+    allowed = 0.0;
+    for (int box = 0; box < 16; ++box) {
+      const int sa = box >> 3, sb = (box >> 2) & 1, si = (box >> 1) & 1,
+                sj = box & 1;
+      if (sa + sb == si + sj)
+        allowed += (double)c->v_count[sa] * c->v_count[sb] *
+                   c->o_count[si] * c->o_count[sj];
+    }
+  }
+  timed_class(c, ftcc_ctx::PROF_ASSEMBLY, 0.0, 56.0 * allowed * nb, [&] {
+    r2_final(t2bar, c->wr, c->aring, c->s1, c->s2, c->z, V, O, nb, s,
+             c->spin ? c->d_lab_v : nullptr, c->spin ? c->d_lab_o : nullptr,
+             vvoo, c->spin && c->pack, zflip ? hv : 0, zflip ? ho : 0,
+             !(c->tbar_scratch && c->spin && c->pack), c->d_jl, c->jl_cnt[0],
+             c->jl_cnt[1]);
+  });
 
   // singles: t1bar = -(Fvo + Fae.t1 - t1.Fmi + t2r.FmeT + ovvo_ring^T.t1
   //                    + 1/2 vovv.t2 + 1/2 t2.ooov)
@@ -2572,7 +2620,13 @@ void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
   a.corr = c->corr1;
   a.D = c->have_D ? c->D1 : nullptr;
   a.eint = c->fovT;
-  int nb = update_t1(a, 1, s);
+  // algorithmic bytes: Tbar read, Told read and written at every grid point
+  // (and the correction read), the energy integral read once, per element
+  const double per_elem = (24.0 + (c->corr1 ? 8.0 : 0.0)) * c->ng + 8.0;
+  int nb = 0;
+  timed_class(c, ftcc_ctx::PROF_UPDATE, 0.0,
+              per_elem * c->nv * (double)c->no,
+              [&] { nb = update_t1(a, 1, s); });
   reduce_partials(c->partials, nb, c->red, s);
   a.tbar = T2b;
   a.told = T2old;
@@ -2606,7 +2660,11 @@ void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
     a.lv = c->d_lab_v;
     a.lo = c->d_lab_o;
   }
-  nb = update_t2(a, 1, s);
+  const double elems2 = a.nallowed > 0 && a.rec
+                            ? (double)a.nallowed
+                            : (double)c->nv * c->nv * c->no * c->no;
+  timed_class(c, ftcc_ctx::PROF_UPDATE, 0.0, per_elem * elems2,
+              [&] { nb = update_t2(a, 1, s); });
   reduce_partials(c->partials, nb, c->red + 3, s);
 }
 
@@ -2743,12 +2801,31 @@ int ftcc_profile_read(ftcc_ctx *c, int64_t *launches, double *ms,
     require(launches && ms && flops, "null pointer");
     FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
     flush_profile(c);
-    *launches = c->prof_launches;
-    *ms = c->prof_ms;
-    *flops = c->prof_flops;
-    c->prof_launches = 0;
-    c->prof_ms = 0.0;
-    c->prof_flops = 0.0;
+    *launches = c->prof_launches[0];
+    *ms = c->prof_ms[0];
+    *flops = c->prof_flops[0];
+    c->prof_launches[0] = 0;
+    c->prof_ms[0] = 0.0;
+    c->prof_flops[0] = 0.0;
+    c->prof_bytes[0] = 0.0;
+  });
+}
+
+int ftcc_profile_class(ftcc_ctx *c, int cls, int64_t *launches, double *ms,
+                       double *flops, double *bytes) {
+  return guarded(c, [&] {
+    require(launches && ms && flops && bytes, "null pointer");
+    require(cls >= 0 && cls < ftcc_ctx::NPROF, "unknown kernel class");
+    FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+    flush_profile(c);
+    *launches = c->prof_launches[cls];
+    *ms = c->prof_ms[cls];
+    *flops = c->prof_flops[cls];
+    *bytes = c->prof_bytes[cls];
+    c->prof_launches[cls] = 0;
+    c->prof_ms[cls] = 0.0;
+    c->prof_flops[cls] = 0.0;
+    c->prof_bytes[cls] = 0.0;
   });
 }
 

@@ -76,9 +76,13 @@ struct ftcc_ctx {
   double *host_red = nullptr;   // pinned
 
   // event timing of the O(n^6) products
+  // kernel classes timed with events while profiling is on
+  enum { PROF_PRODUCTS = 0, PROF_N5 = 1, PROF_GATHER = 2, PROF_ASSEMBLY = 3,
+         PROF_UPDATE = 4, NPROF = 5 };
   struct Pending {
     hipEvent_t e0, e1;
-    double flops;
+    int cls;
+    double flops, bytes;
   };
   // split-bf16 operands of the five O(n^6) products (bf16x3 mode)
   SplitMat sp_tauA, sp_tauB, sp_t2rA, sp_aringB, sp_wmnijB, sp_wabefA, sp_wrB;
@@ -142,8 +146,8 @@ struct ftcc_ctx {
   bool prof_on = false;
   std::vector<Pending> prof_pending;
   std::vector<hipEvent_t> ev_pool;
-  double prof_ms = 0.0, prof_flops = 0.0;
-  int64_t prof_launches = 0;
+  double prof_ms[NPROF] = {}, prof_flops[NPROF] = {}, prof_bytes[NPROF] = {};
+  int64_t prof_launches[NPROF] = {};
 
   // adjoint (lambda) workspace and the T-dependent intermediates it keeps
   // between ftcc_adjoint_begin and ftcc_adjoint_end (adjoint.hip)

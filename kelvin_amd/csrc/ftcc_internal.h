@@ -373,6 +373,8 @@ int energy_t2(const double *T1, const double *T2, const double *g,
 int energy_t1(const double *T1, const double *g, const double *fovT, int ng,
               int nv, int no, double *partials, hipStream_t s);
 int sumsq(const double *x, int64_t n, double *partials, hipStream_t s);
+// blocks (= partial triples) of the reductions above over n elements
+int reduction_blocks(int64_t n);
 int vsum(const double *x, int64_t n, double *partials, hipStream_t s);
 // count (third slot of the partial triples) of the nonzero entries of nb
 // consecutive (d0,d1,d2,d3) blocks with l0[i0] + l1[i1] != l2[i2] + l3[i3];

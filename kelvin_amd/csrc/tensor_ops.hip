@@ -401,11 +401,16 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
   if (fimage && a >= zh) return;   // a <= b: both beta
   const bool wimage = fimage && b < zh;   // both alpha
   // a mixed pair (a alpha, b beta) and (b - zh, a + zh) are images of each
-  // other: the one with a < b - zh writes both (a == b - zh: itself only)
-  bool mimage = false;
+  // other: the one with a < b - zh writes both.  A pair with a == b - zh is
+  // its own image, element by element: R2[a,b,i,j] = R2[a,b,fl(j),fl(i)],
+  // and of each such couple of elements the one with the smaller (i, j)
+  // writes both (computed separately, the two would differ by rounding in
+  // the low-precision modes)
+  bool mimage = false, simage = false;
   if (fimage && b >= zh) {
     if (a > b - zh) return;
     mimage = a < b - zh;
+    simage = a == b - zh;
   }
   auto flo = [&](int k) { return k < zo ? k + zo : k - zo; };
   auto nat = [&](int p, int r, int k, int l) {
@@ -489,6 +494,10 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
       out[nat(a + zh, b - zh, flo(i), flo(j))] = -v;
       out[nat(b - zh, a + zh, flo(i), flo(j))] = v;
     }
+    if (simage && i != flo(j)) {
+      out[nat(a, b, flo(j), flo(i))] = -v;
+      out[nat(b, a, flo(j), flo(i))] = v;
+    }
   };
   if (zero_forbidden) {
     const int lanes = blockDim.x / 2;   // two rows at a time
@@ -516,6 +525,8 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
         if (wimage) out[nat(a + zh, b + zh, flo(i), flo(j))] = 0.0;
         continue;
       }
+      // a self-image pair: (fl(j), fl(i)) before (i, j) writes this one
+      if (simage && (flo(j) < i || (flo(j) == i && flo(i) < j))) continue;
       element(il, i, j);
     }
   }

@@ -560,6 +560,8 @@ int energy_t1(const double *T1, const double *g, const double *fovT, int ng,
   return blocks;
 }
 
+int reduction_blocks(int64_t n) { return nblocks_for(n); }
+
 int sumsq(const double *x, int64_t n, double *partials, hipStream_t s) {
   const int blocks = nblocks_for(n);
   hipLaunchKernelGGL(sumsq_kernel, dim3(blocks), dim3(UT), 0, s, x, n,

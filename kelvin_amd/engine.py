@@ -529,6 +529,23 @@ class FTEngine:
                    ctypes.byref(fl))
         return n.value, ms.value, fl.value
 
+    PROFILE_CLASSES = ("products", "n5_gemms", "gathers", "assembly",
+                       "update")
+
+    def profile_classes(self):
+        """{class: (launches, device ms, algorithmic flops, algorithmic
+        bytes)} since the last read, for every kernel class."""
+        out = {}
+        for k, name in enumerate(self.PROFILE_CLASSES):
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            fl = ctypes.c_double()
+            by = ctypes.c_double()
+            self._call("ftcc_profile_class", k, ctypes.byref(n),
+                       ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by))
+            out[name] = (n.value, ms.value, fl.value, by.value)
+        return out
+
     def workspace_bytes(self):
         b = ctypes.c_int64()
         self._call("ftcc_workspace_bytes", ctypes.byref(b))

@@ -104,22 +104,18 @@ class UEGSystem(System):
         self.T = T
         self.L = L
         self.basis = UEGBasis(L, Emax, norb=norb)
-        if na is None:
-            assert nb is None
-            assert mu is not None
-            self.mu = mu
-            beta = 1.0 / self.T if self.T > 0.0 else 1.0e20
-            N = ft_utils.ff(beta, self.g_energies_tot(), self.mu).sum()
-            self.Na = N / 2.0
-            self.Nb = self.Na
-        else:
-            self.Na, self.Nb = na, nb
-            assert na > 0 and nb > 0
-            mua = self.basis.Es[self.Na - 1] + 0.00001
-            mub = self.basis.Es[self.Nb - 1] + 0.00001
-            assert mua == mub
-            self.mu = mua
-            assert self.T == 0.0
+        if na is not None or nb is not None or not T > 0.0:
+            # fixed occupations at T = 0 (kelvin/ueg_system.py:46-55) belong
+            # to zero-temperature CCSD, which the GPU path does not build
+            raise NotImplementedError(
+                "UEGSystem needs T > 0 and mu (zero-temperature occupations "
+                "are not part of the GPU path)")
+        if mu is None:
+            raise ValueError("UEGSystem needs the chemical potential mu")
+        self.mu = mu
+        N = ft_utils.ff(1.0 / self.T, self.g_energies_tot(), self.mu).sum()
+        self.Na = N / 2.0
+        self.Nb = self.Na
         self.N = self.Na + self.Nb
         self.den = self.N / (L * L * L)
         self.rs = (3 / (4.0 * numpy.pi * self.den)) ** (1.0 / 3.0)

@@ -256,3 +256,48 @@ def test_autograd_grand_potential():
     gm = (f(b0, m0 + h) - f(b0, m0 - h)) / (2 * h)
     assert abs(float(beta.grad) - gb) < 1e-7 * max(1.0, abs(gb))
     assert abs(float(mu.grad) - gm) < 1e-7
+
+
+def test_transposed_residual_spin_blocks_equal_dense():
+    """On a spin-orbital UEG (19 k-points) the transposed residual runs
+    over the spin blocks its operands' rules allow; it equals the dense
+    evaluation (FTCC_SPIN=0) to rounding, for amplitudes after a step and
+    the integrated lambda guess."""
+    from kelvin_amd import quadrature
+    from kelvin_amd.engine import FTEngine
+    from kelvin_amd.ueg_system import ueg_from_config
+    dev()
+    sysg = ueg_from_config(19, 2.0, N=14, t_over_tf=0.3)
+    en = sysg.g_energies_tot()
+    n, ng = en.shape[0], 4
+    beta = 1.0 / sysg.T
+    ti, g, G = quadrature.ft_quad(ng, beta, 'lin')
+    out = []
+    for spin in ("1", "0"):
+        os.environ["FTCC_SPIN"] = spin
+        try:
+            eng = FTEngine(n, n, ng)
+            fock, eri, _ = sysg.g_device_integrals(eng)
+            eng.set_system(en, fock, eri, beta, sysg.mu)
+        finally:
+            del os.environ["FTCC_SPIN"]
+        del fock, eri
+        eng.set_grid(ti, G)
+        assert (eng.spin_mode() != 0) == (spin == "1")
+        T1, T2 = eng.empty_amplitudes()
+        eng.driving_terms(T1, T2)
+        eng.integrate(T1, T2)
+        B1, B2 = eng.empty_amplitudes()
+        eng.step(T1, T2, B1, B2, g, beta, 0.3)
+        L1, L2 = eng.empty_amplitudes()
+        eng.lambda_guess(T1, g, beta, L1, L2)
+        Lt1, Lt2 = eng.empty_amplitudes()
+        eng.integrate_adjoint(L1, L2, Lt1, Lt2)
+        eng.adjoint_begin(T1, T2)
+        G1, G2 = eng.empty_amplitudes()
+        eng.adjoint_apply(Lt1, Lt2, G1, G2)
+        eng.adjoint_end()
+        out.append((to_np(G1), to_np(G2)))
+        eng.close()
+    assert rel(out[0][0], out[1][0]) < 1e-12
+    assert rel(out[0][1], out[1][1]) < 1e-12
