@@ -10,19 +10,24 @@
 //
 // Tiling: a 512-thread workgroup (8 wave64s, two per SIMD) owns a 256x256
 // tile of C, each wave a 128x64 quadrant (4x2 MFMA 32x32 blocks, 128 FP32
-// accumulators per lane); per 16-deep K step a wave reads its A and B
-// fragments (hi and lo) from LDS and issues the three MFMAs of each block in
-// turn.  K advances in slabs of 32 through double-buffered LDS; each LDS row
-// holds the 32 hi values, then the 32 lo values, then 8 bf16 of padding
-// (144 B rows: 16 lanes reading one row each land on distinct banks).  Slabs
-// are loaded into registers (16-byte loads) two slabs ahead of their use,
-// alternating two register sets, and stored to the other LDS stage one slab
-// ahead, so each load has a slab of MFMA work besides its own to land
-// (5.6% shorter launches than prefetching one slab ahead at n = 114).  Workgroup ids are remapped so that each XCD
-// works on a compact group of tiles (shared A/B panels stay in that XCD's
-// L2).  Measured alternatives -- 4 waves of 128x128 with accumulators in
-// AGPRs, with and without a software pipeline that double-buffers the
-// fragments, and product-major MFMA order -- were no faster on gfx950.
+// accumulators per lane).  K advances in slabs of 32 through two LDS stages
+// of 64 KB; each LDS row holds the row's 32 hi values and 32 lo values as
+// eight 16-byte chunks, chunk c stored at slot c ^ ((row >> 1) & 7) so that
+// the MFMA fragment reads (32 rows at one chunk per half-wave) hit distinct
+// banks.  Slabs go from HBM straight into LDS (global_load_lds_dwordx4: each
+// lane fetches the global chunk that belongs at its LDS slot, no register
+// staging), two slabs ahead of their use, into the stage the previous slab
+// has left.  Per slab a wave reads the fragments of the next 16-deep K step
+// while the MFMAs of the current one run (fragments double-buffered in
+// registers), so one barrier per slab is the only synchronisation; the loop
+// body has no branches (the loads past the last slab re-read the last one
+// into a stage that is never read).  Rows past M (N) read the last row:
+// their products land in accumulators that are never stored.  Workgroup
+// ids are remapped so that each XCD works on a compact group of tiles
+// (shared A/B panels stay in that XCD's L2).  Against staging the slabs
+// through registers with padded LDS rows (the earlier form of this
+// kernel), 8-13% shorter launches at n = 114 shapes with bit-identical
+// results; one wave per SIMD with 128x128 wave tiles was slower.
 #include "ftcc_internal.h"
 
 #include <algorithm>
@@ -34,9 +39,13 @@ typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 typedef float f16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 256, BN = 256, BK = 32, LDR = 72;
+constexpr int BM = 256, BN = 256, BK = 32;
 constexpr int GROUP_M = 8;
-constexpr size_t LDS_BYTES = 2 * 2 * (size_t)BM * LDR * sizeof(__bf16);
+constexpr int ROWB = 128;              // bytes per LDS row: 32 hi, 32 lo
+constexpr int OPB = BM * ROWB;         // one operand of one stage: 32 KB
+constexpr int STAGEB = 2 * OPB;        // A then B
+constexpr size_t LDS_BYTES = 2 * (size_t)STAGEB;
+typedef __attribute__((address_space(3))) void lds_void;
 
 // One product of a grouped launch (the spin blocks of one O(n^6) product
 // share C, alpha, beta and the batch; up to MAXP problems per launch).
@@ -81,130 +90,126 @@ __device__ inline SbProb tile_of(const SbDev &p, int &bt, int &tm, int &tn) {
   return pr;
 }
 
-// 2048 16-byte chunks per 256-row operand slab: (array, row, quarter) each
-template <int NT>
-__device__ inline void load_operand(const __bf16 *hi, const __bf16 *lo,
-                                    int64_t ld, int row0, int rows, int k0,
-                                    u4_t reg[2048 / NT]) {
-  const int t = threadIdx.x;
+// LDS slot of chunk c (0-3 hi, 4-7 lo, eight bf16 each) of row r
+__device__ inline int lds_slot(int row, int c) { return c ^ ((row >> 1) & 7); }
+
+// One wave's share of a slab of one operand: rows [32w, 32w + 32) of the
+// tile, four loads of eight rows x eight chunks; the lane that lands at
+// slot (lane & 7) of row r fetches the chunk stored there.
+__device__ inline void load_slab(const __bf16 *hi, const __bf16 *lo,
+                                 int64_t ld, int row0, int rows, int k0,
+                                 char *lds_op, int wave, int lane) {
+  const int r8 = lane >> 3, slot = lane & 7;
 #pragma unroll
-  for (int r = 0; r < 2048 / NT; ++r) {
-    const int c = t + NT * r;
-    const int arr = c >> 10, row = (c >> 2) & 255, kq = c & 3;
-    const __bf16 *src = (arr ? lo : hi) + (int64_t)(row0 + row) * ld + k0 +
-                        8 * kq;
-    if (row0 + row < rows)
-      reg[r] = *(const u4_t *)src;
-    else
-      reg[r] = u4_t{0u, 0u, 0u, 0u};
+  for (int j = 0; j < 4; ++j) {
+    const int rowl = wave * 32 + j * 8 + r8;
+    const int c = lds_slot(rowl, slot);   // the slot map is an involution
+    int g = row0 + rowl;
+    g = g < rows ? g : rows - 1;
+    const __bf16 *src = (c < 4 ? hi : lo) + (int64_t)g * ld + k0 + 8 * (c & 3);
+    __builtin_amdgcn_global_load_lds(
+        (const void *)src, (lds_void *)(lds_op + (wave * 32 + j * 8) * ROWB),
+        16, 0, 0);
   }
 }
 
-template <int NT>
-__device__ inline void store_operand(__bf16 *lds, const u4_t reg[2048 / NT]) {
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < 2048 / NT; ++r) {
-    const int c = t + NT * r;
-    const int arr = c >> 10, row = (c >> 2) & 255, kq = c & 3;
-    *(u4_t *)(lds + row * LDR + arr * BK + 8 * kq) = reg[r];
-  }
+__device__ inline bf8_t lds_frag(const char *lds_op, int row, int c) {
+  return *(const bf8_t *)(lds_op + row * ROWB + (lds_slot(row, c) << 4));
 }
 
-__global__ __launch_bounds__(512, 2) void sbgemm_kernel(SbDev p) {
-  constexpr int WN = 4, NT = 512, RT = 4, CT = 2;
-  extern __shared__ __bf16 lds[];   // [2 stages][A|B][BM*LDR]
+// the A (4 row blocks) and B (2 column blocks) fragments, hi and lo, of one
+// 16-deep K step (chunk c: 0 or 2, plus the lane's half)
+struct Frag {
+  bf8_t ah[4], al[4], bh[2], bl[2];
+};
+__device__ inline void read_frag(Frag &f, const char *As, const char *Bs,
+                                 int wm, int wn, int li, int c) {
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    const int row = wm * 128 + rt * 32 + li;
+    f.ah[rt] = lds_frag(As, row, c);
+    f.al[rt] = lds_frag(As, row, c + 4);
+  }
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct) {
+    const int row = wn * 64 + ct * 32 + li;
+    f.bh[ct] = lds_frag(Bs, row, c);
+    f.bl[ct] = lds_frag(Bs, row, c + 4);
+  }
+}
+// lo*hi + hi*lo + hi*hi, in that order for every accumulator
+__device__ inline void mfma_step(f16_t (&acc)[4][2], const Frag &f) {
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+          f.al[rt], f.bh[ct], acc[rt][ct], 0, 0, 0);
+      acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+          f.ah[rt], f.bl[ct], acc[rt][ct], 0, 0, 0);
+      acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+          f.ah[rt], f.bh[ct], acc[rt][ct], 0, 0, 0);
+    }
+}
+
+__global__ __launch_bounds__(512, 1) void sbgemm_kernel(SbDev p) {
+  extern __shared__ char lds[];   // [2 stages][A | B][256 rows][128 B]
   int bt, tm, tn;
   const SbProb pr = tile_of(p, bt, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const __bf16 *Ahi = pr.Ahi + bt * pr.sAb, *Alo = pr.Alo + bt * pr.sAb;
   const __bf16 *Bhi = pr.Bhi + bt * pr.sBb, *Blo = pr.Blo + bt * pr.sBb;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave / WN, wn = wave % WN;
+  const int wm = wave >> 2, wn = wave & 3;
   const int li = lane & 31, lg = lane >> 5;
   const int nslab = pr.Kp / BK;
 
-  f16_t acc[RT][CT];
+  f16_t acc[4][2];
 #pragma unroll
-  for (int i = 0; i < RT; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < CT; ++j)
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
 
-  // Slab k is loaded into register set k % 2 two slabs ahead of its use and
-  // stored to LDS stage k % 2 one slab ahead, so each load has a whole slab
-  // of MFMA work, besides its own, to land (the loop is unrolled by two so
-  // both register sets are addressed statically).
-  u4_t ra0[2048 / NT], rb0[2048 / NT], ra1[2048 / NT], rb1[2048 / NT];
-  constexpr int STAGE = 2 * BM * LDR;   // bf16 per stage (A then B)
   // an empty K (a spin class without pairs) only writes beta * C
+  Frag f0, f1;
   if (nslab > 0) {
-    load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, 0, ra0);
-    load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, 0, rb0);
-    store_operand<NT>(lds, ra0);
-    store_operand<NT>(lds + BM * LDR, rb0);
-  }
-  if (nslab > 1) {
-    load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, BK, ra1);
-    load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, BK, rb1);
-  }
-  __syncthreads();
-  auto step = [&](int s, u4_t(&lda_)[2048 / NT], u4_t(&ldb_)[2048 / NT],
-                  const u4_t(&sta)[2048 / NT], const u4_t(&stb)[2048 / NT]) {
-    const __bf16 *As = lds + (s & 1) * STAGE;
-    const __bf16 *Bs = As + BM * LDR;
-    if (s + 2 < nslab) {
-      load_operand<NT>(Ahi, Alo, pr.lda, m0, pr.M, (s + 2) * BK, lda_);
-      load_operand<NT>(Bhi, Blo, pr.ldb, n0, pr.N, (s + 2) * BK, ldb_);
-    }
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      bf8_t ah[RT], al[RT], bh[CT], bl[CT];
-      const int koff = ks * 16 + 8 * lg;
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt) {
-        const __bf16 *a = As + (wm * 128 + rt * 32 + li) * LDR + koff;
-        ah[rt] = *(const bf8_t *)a;
-        al[rt] = *(const bf8_t *)(a + BK);
-      }
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const __bf16 *b = Bs + (wn * (32 * CT) + ct * 32 + li) * LDR + koff;
-        bh[ct] = *(const bf8_t *)b;
-        bl[ct] = *(const bf8_t *)(b + BK);
-      }
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-          acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-              al[rt], bh[ct], acc[rt][ct], 0, 0, 0);
-          acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-              ah[rt], bl[ct], acc[rt][ct], 0, 0, 0);
-          acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-              ah[rt], bh[ct], acc[rt][ct], 0, 0, 0);
-        }
-    }
-    if (s + 1 < nslab) {
-      __bf16 *nxt = lds + ((s + 1) & 1) * STAGE;
-      store_operand<NT>(nxt, sta);
-      store_operand<NT>(nxt + BM * LDR, stb);
-    }
+    load_slab(Ahi, Alo, pr.lda, m0, pr.M, 0, lds, wave, lane);
+    load_slab(Bhi, Blo, pr.ldb, n0, pr.N, 0, lds + OPB, wave, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-  };
-  for (int s = 0; s < nslab; s += 2) {
-    step(s, ra0, rb0, ra1, rb1);
-    if (s + 1 < nslab) step(s + 1, ra1, rb1, ra0, rb0);
+    const int k1 = (nslab > 1 ? 1 : 0) * BK;
+    load_slab(Ahi, Alo, pr.lda, m0, pr.M, k1, lds + STAGEB, wave, lane);
+    load_slab(Bhi, Blo, pr.ldb, n0, pr.N, k1, lds + STAGEB + OPB, wave,
+              lane);
+    read_frag(f0, lds, lds + OPB, wm, wn, li, lg);
   }
+  for (int s = 0; s < nslab; ++s) {
+    char *cur = lds + (s & 1) * STAGEB, *nxt = lds + ((s + 1) & 1) * STAGEB;
+    // second K step of slab s from LDS while the first one's MFMAs run
+    read_frag(f1, cur, cur + OPB, wm, wn, li, 2 + lg);
+    mfma_step(acc, f0);
+    // slab s + 1 has landed (every wave's loads) and every wave is done
+    // with stage s: slab s + 2 goes there, slab s + 1's first K step is read
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    const int k2 = min(s + 2, nslab - 1) * BK;
+    load_slab(Ahi, Alo, pr.lda, m0, pr.M, k2, cur, wave, lane);
+    load_slab(Bhi, Blo, pr.ldb, n0, pr.N, k2, cur + OPB, wave, lane);
+    read_frag(f0, nxt, nxt + OPB, wm, wn, li, lg);
+    mfma_step(acc, f1);
+  }
+  // no load may still be writing LDS when the workgroup ends
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 
   // lane holds column li, rows 8*(v/4) + 4*lg + v%4 of each 32x32 block
   double *C = p.C + bt * p.sCb;
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt)
+  for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-      const int col = n0 + wn * (32 * CT) + ct * 32 + li;
+    for (int ct = 0; ct < 2; ++ct) {
+      const int col = n0 + wn * 64 + ct * 32 + li;
       if (col >= pr.N) continue;
       const int cc = pr.cmap ? pr.cmap[col] : col;
       if (cc < 0) continue;

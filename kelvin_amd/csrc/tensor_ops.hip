@@ -402,10 +402,10 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
   const bool wimage = fimage && b < zh;   // both alpha
   // a mixed pair (a alpha, b beta) and (b - zh, a + zh) are images of each
   // other: the one with a < b - zh writes both.  A pair with a == b - zh is
-  // its own image, element by element: R2[a,b,i,j] = R2[a,b,fl(j),fl(i)],
-  // and of each such couple of elements the one with the smaller (i, j)
-  // writes both (computed separately, the two would differ by rounding in
-  // the low-precision modes)
+  // its own image: R2[a,b,i,j] = R2[a,b,fl(j),fl(i)] = -R2[a,b,j,i]
+  // = -R2[a,b,fl(i),fl(j)], so the four elements are written from one of
+  // them, the one with i alpha and the smaller i (computed separately, they
+  // would differ by rounding in the low-precision modes)
   bool mimage = false, simage = false;
   if (fimage && b >= zh) {
     if (a > b - zh) return;
@@ -494,9 +494,15 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
       out[nat(a + zh, b - zh, flo(i), flo(j))] = -v;
       out[nat(b - zh, a + zh, flo(i), flo(j))] = v;
     }
-    if (simage && i != flo(j)) {
-      out[nat(a, b, flo(j), flo(i))] = -v;
-      out[nat(b, a, flo(j), flo(i))] = v;
+    if (simage) {
+      out[nat(a, b, j, i)] = v;
+      out[nat(b, a, j, i)] = -v;
+      if (i != flo(j)) {
+        out[nat(a, b, flo(j), flo(i))] = -v;
+        out[nat(b, a, flo(j), flo(i))] = v;
+        out[nat(a, b, flo(i), flo(j))] = v;
+        out[nat(b, a, flo(i), flo(j))] = -v;
+      }
     }
   };
   if (zero_forbidden) {
@@ -525,8 +531,9 @@ __global__ __launch_bounds__(256) void r2_packed_kernel(
         if (wimage) out[nat(a + zh, b + zh, flo(i), flo(j))] = 0.0;
         continue;
       }
-      // a self-image pair: (fl(j), fl(i)) before (i, j) writes this one
-      if (simage && (flo(j) < i || (flo(j) == i && flo(i) < j))) continue;
+      // a self-image pair: written by its element with i alpha and the
+      // smaller i (i beta: from (j, i); fl(j) < i: from (fl(j), fl(i)))
+      if (simage && (lo[i] != 0 || flo(j) < i)) continue;
       element(il, i, j);
     }
   }
