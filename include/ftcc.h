@@ -166,6 +166,18 @@ int ftcc_energy(ftcc_ctx *ctx, const double *T1_dev, const double *T2_dev,
                 const double *g_host, double beta, int qterm,
                 double *energy);
 
+/* ftcc_energy without allocating, copying from the host or synchronising:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The weights
+ * g_dev[ng] and the result energy_dev[1] are device memory; the partial
+ * sums live in a fixed device array per GPU (196,656 bytes: 8192 partial
+ * triples, the block cap of the grid-stride reductions, so every problem
+ * size fits; a reduction needing more blocks would get FTCC_EINVAL before
+ * any launch).  Calls that overlap on one GPU share that array and race. */
+int ftcc_energy_noalloc(ftcc_ctx *ctx, const double *T1_dev,
+                        const double *T2_dev, const double *g_dev,
+                        double beta, int qterm, double *energy_dev);
+
 /* sum of squares of n doubles (numpy.linalg.norm(x)**2 of
  * kelvin/cc_utils.py:139-140,145-151). */
 int ftcc_sumsq(ftcc_ctx *ctx, const double *x_dev, int64_t n, double *out);

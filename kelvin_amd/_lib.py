@@ -41,6 +41,9 @@ SIGNATURES = {
     "ftcc_energy": (ctypes.c_int, [_vp, _vp, _vp, _c_double_p,
                                    ctypes.c_double, ctypes.c_int,
                                    _c_double_p]),
+    "ftcc_energy_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp,
+                                           ctypes.c_double, ctypes.c_int,
+                                           _vp]),
     "ftcc_sumsq": (ctypes.c_int, [_vp, _vp, _i64, _c_double_p]),
     "ftcc_step": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _c_double_p,
                                  ctypes.c_double, ctypes.c_double,

@@ -12,8 +12,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libftcc.so")
 ARCH = os.environ.get("FTCC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["ftcc_api.hip", "adjoint.hip", "gemm.hip", "sbgemm.hip",
-           "tensor_ops.hip", "update.hip", "ueg.hip"]
+SOURCES = ["ftcc_api.hip", "adjoint.hip", "noalloc.hip", "gemm.hip",
+           "sbgemm.hip", "tensor_ops.hip", "update.hip", "ueg.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1935,6 +1935,10 @@ int ftcc_create(ftcc_ctx **out, int no, int nv, int ng, int precision,
     c->red = dalloc(c, 16);
     FTCC_CHECK_HIP(hipHostMalloc((void **)&c->host_red, 16 * sizeof(double),
                                  hipHostMallocDefault));
+    // the allocation-free entry point's scratch on this device
+    int device = 0;
+    FTCC_CHECK_HIP(hipGetDevice(&device));
+    noalloc_scratch(device);
   });
   if (rc != FTCC_OK) {
     fprintf(stderr, "ftcc_create: %s\n", c->err.c_str());

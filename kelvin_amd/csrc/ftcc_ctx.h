@@ -207,5 +207,8 @@ bool try_spin(ftcc_ctx *c, const std::vector<int> &lo,
 UpdateArgs upd(ftcc_ctx *c);
 // release the adjoint workspace (adjoint.hip)
 void adjoint_release(ftcc_ctx *c);
+// device address of the allocation-free entry point's scratch array on
+// `device` (noalloc.hip; looked up once per device)
+double *noalloc_scratch(int device);
 
 }  // namespace ftcc_core
