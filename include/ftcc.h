@@ -387,6 +387,15 @@ int ftcc_workspace_bytes(const ftcc_ctx *ctx, int64_t *bytes);
 int ftcc_profile_enable(ftcc_ctx *ctx, int on);
 int ftcc_profile_read(ftcc_ctx *ctx, int64_t *launches, double *ms,
                       double *flops);
+/* Launch plan of the generic GEMMs (the O(n^5) terms, and everything in the
+ * FP64/FP32 modes): tile_policy 0 picks the tile by the shape (128x64 or
+ * 64x128 when a dimension or K is at most 64, else 128x128), 1 keeps
+ * 128x128 unless a dimension is at most 64, 2 always narrow; K is split
+ * while a launch has fewer than split_target workgroups (default 1024).
+ * Results are the same up to the summation order of split K.  No
+ * counterpart in the reference (the tile autotuner of the batch sets it). */
+int ftcc_set_gemm_plan(ftcc_ctx *ctx, int tile_policy, int split_target);
+
 /* The same for every kernel class while profiling is on: 0 the O(n^6)
  * products (as ftcc_profile_read), 1 the other GEMMs (O(n^5) terms), 2 the
  * product operand gathers, 3 the doubles assembly, 4 the integration and

@@ -106,6 +106,7 @@ SIGNATURES = {
     "ftcc_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ftcc_profile_read": (ctypes.c_int, [_vp, ctypes.POINTER(_i64),
                                          _c_double_p, _c_double_p]),
+    "ftcc_set_gemm_plan": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ftcc_profile_class": (ctypes.c_int, [_vp, ctypes.c_int,
                                           ctypes.POINTER(_i64), _c_double_p,
                                           _c_double_p, _c_double_p]),

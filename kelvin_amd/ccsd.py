@@ -24,8 +24,10 @@ Extensions over the reference (all optional):
               entry) the solver's own iterates are trusted to conserve
               spin and the per-step check of the amplitudes is skipped.
   chunk       grid points per residual chunk (default: chosen from the
-              workspace budget; ``kelvin_amd.batch.autotune_chunk`` times
-              the choices).
+              workspace budget; ``kelvin_amd.batch.autotune`` times the
+              choices).
+  gemm_plan   (tile_policy, split_target) of the generic GEMMs (default
+              the library's; ``kelvin_amd.batch.autotune`` times them).
 """
 import copy
 import logging
@@ -48,7 +50,7 @@ class ccsd(object):
                  tconv=None, max_iter=40, damp=0.0, ngrid=10, realtime=False,
                  athresh=0.0, quad='lin', rt_iter="all", precision="fp64",
                  refine=True, refine_max_iter=None, graph=False,
-                 chunk=None):
+                 chunk=None, gemm_plan=None):
         self.T = T
         self.mu = mu
         self.finite_T = False if T == 0 else True
@@ -68,6 +70,7 @@ class ccsd(object):
         self.refine_max_iter = refine_max_iter
         self.graph = graph
         self.chunk = chunk
+        self.gemm_plan = gemm_plan
         if self.finite_T:
             self.realtime = True
         if not sys.verify(self.T, self.mu):
@@ -265,6 +268,8 @@ class ccsd(object):
         self.engine = eng
         if self.chunk is not None:
             eng.set_chunk(self.chunk)
+        if self.gemm_plan is not None:
+            eng.set_gemm_plan(*self.gemm_plan)
         if self.graph:
             eng.use_graph = True
         # iterates started from the driving terms conserve spin: the
@@ -344,7 +349,7 @@ class ccsd(object):
             Eccn, T1, T2 = cc_utils.ft_cc_iter(
                 "CCSD", T1old, T2old, F, I, D1, D2, g, G, self.beta_max, ng,
                 ti, self.iprint, conv, history=self.history)
-        torch.cuda.synchronize(eng.device)
+        torch.cuda.current_stream(eng.device).synchronize()
         # the solve in the chosen precision, before any continuation in FP64
         self.Gcc_unrefined = Eccn
         self.niter_unrefined = len(self.history)
@@ -361,7 +366,7 @@ class ccsd(object):
             Eccn, T1, T2 = cc_utils.ft_cc_iter(
                 "CCSD", T1, T2, F, I, D1, D2, g, G, self.beta_max, ng, ti,
                 self.iprint, conv_r, history=self.history)
-            torch.cuda.synchronize(eng.device)
+            torch.cuda.current_stream(eng.device).synchronize()
             self.niter_refine = len(self.history) - self.niter_unrefined
             self.time_refine = time.time() - t_ref
         self.T1 = T1

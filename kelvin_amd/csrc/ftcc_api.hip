@@ -180,6 +180,17 @@ void flush_profile(ftcc_ctx *c) {
     c->prof_flops[p.cls] += p.flops;
     c->prof_bytes[p.cls] += p.bytes;
     c->prof_launches[p.cls] += 1;
+    if (c->prof_sites) {
+      ftcc_ctx::SiteStat &st = c->site_stats[{p.cls, p.line}];
+      st.launches += 1;
+      st.ms += ms;
+      st.flops += p.flops;
+      st.bytes += p.bytes;
+      st.M = p.M;
+      st.N = p.N;
+      st.K = p.K;
+      st.nb = p.nb;
+    }
     c->ev_pool.push_back(p.e0);
     c->ev_pool.push_back(p.e1);
   }
@@ -197,6 +208,11 @@ void timed_class(ftcc_ctx *c, int cls, double flops, double bytes, Fn fn) {
     p.cls = cls;
     p.flops = flops;
     p.bytes = bytes;
+    p.line = c->cur_line;
+    p.M = c->cur_M;
+    p.N = c->cur_N;
+    p.K = c->cur_K;
+    p.nb = c->cur_nb;
     FTCC_CHECK_HIP(hipEventRecord(p.e0, c->stream));
     fn();
     FTCC_CHECK_HIP(hipEventRecord(p.e1, c->stream));
@@ -250,11 +266,20 @@ Gemm fold_n(Gemm g) {
   return g;
 }
 
-void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false) {
+void run(ftcc_ctx *c, const Gemm &g0, int prec, bool big, int line) {
+  Gemm g = g0;
+  g.tile_policy = c->tile_policy;
+  g.split_target = c->split_target;
   ensure_partial(c, g);
+  c->cur_line = c->site_override ? c->site_override : line;
+  c->cur_M = g.M;
+  c->cur_N = g.N;
+  c->cur_K = g.K;
+  c->cur_nb = g.nb1 * g.nb2;
   timed_class(c, big ? ftcc_ctx::PROF_PRODUCTS : ftcc_ctx::PROF_N5,
               gemm_flops(g), big ? 0.0 : gemm_bytes(g),
               [&] { gemm_run(g, prec, c->scr, c->stream); });
+  c->cur_line = 0;
 }
 
 
@@ -288,7 +313,13 @@ bool rule_holds(const char *rule, const std::map<char, int> &sp) {
 void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
               const std::vector<SIdx> &Mi, const std::vector<SIdx> &Ni,
               const std::vector<SIdx> &Ki, const char *ruleA,
-              const char *ruleB, const char *ruleC) {
+              const char *ruleB, const char *ruleC, int line) {
+  struct Site {
+    ftcc_ctx *c;
+    int saved;
+    ~Site() { c->site_override = saved; }
+  } site{c, c->site_override};
+  if (!c->site_override) c->site_override = line;
   std::vector<char> letters;
   int batch = 0;
   int unit = 0;
@@ -525,8 +556,9 @@ void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
 void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
               const std::vector<SIdx> &Mi, const std::vector<SIdx> &Ni,
               const SIdx &Ki, const char *ruleA, const char *ruleB,
-              const char *ruleC) {
-  run_spin(c, base, prec, Mi, Ni, std::vector<SIdx>{Ki}, ruleA, ruleB, ruleC);
+              const char *ruleC, int line = __builtin_LINE()) {
+  run_spin(c, base, prec, Mi, Ni, std::vector<SIdx>{Ki}, ruleA, ruleB, ruleC,
+           line);
 }
 
 // One of the five O(n^6) products: split-bf16 operands in bf16x3 mode
@@ -1841,6 +1873,7 @@ bool residual_all(ftcc_ctx *c, const double *T1, const double *T2,
     c->spin = 0;
   // the flip symmetry of the amplitudes is part of the trusted promise
   c->flip_now = c->flip && c->spin && c->trusted;
+  c->last_flip_step = c->flip_now && c->spin5 && c->pack && c->tbar_scratch;
   struct Restore {
     ftcc_ctx *c;
     int v;
@@ -2660,6 +2693,10 @@ void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
       a.lv = c->d_lab_v;
       a.lo = c->d_lab_o;
     }
+    // flip-symmetric step: the alpha-lead half, results written twice
+    a.flip_half = (a.rec && c->last_flip_step && c->v_count[0] == c->v_count[1] &&
+                   c->o_count[0] == c->o_count[1] && c->v_start[0] == 0 &&
+                   c->o_start[0] == 0) ? 1 : 0;
   } else if (spin_used) {   // spin-forbidden doubles are zero: skipped
     a.lv = c->d_lab_v;
     a.lo = c->d_lab_o;
@@ -2667,7 +2704,12 @@ void step_device(ftcc_ctx *c, double *T1old, double *T2old, double *T1b,
   const double elems2 = a.nallowed > 0 && a.rec
                             ? (double)a.nallowed
                             : (double)c->nv * c->nv * c->no * c->no;
-  timed_class(c, ftcc_ctx::PROF_UPDATE, 0.0, per_elem * elems2,
+  // flip half: Tbar, Told (and the correction) read for half the elements,
+  // Told written for all
+  const double bytes2 =
+      a.flip_half ? (0.5 * (per_elem - 8.0 * c->ng) + 8.0 * c->ng) * elems2
+                  : per_elem * elems2;
+  timed_class(c, ftcc_ctx::PROF_UPDATE, 0.0, bytes2,
               [&] { nb = update_t2(a, 1, s); });
   reduce_partials(c->partials, nb, c->red + 3, s);
 }
@@ -2795,8 +2837,45 @@ int ftcc_workspace_bytes(const ftcc_ctx *c, int64_t *bytes) {
   return FTCC_OK;
 }
 
+int ftcc_set_gemm_plan(ftcc_ctx *c, int tile_policy, int split_target) {
+  return guarded(c, [&] {
+    require(tile_policy >= 0 && tile_policy <= 2, "tile policy is 0, 1 or 2");
+    require(split_target >= 1 && split_target <= (1 << 20),
+            "split target out of range");
+    if (tile_policy == c->tile_policy && split_target == c->split_target)
+      return;
+    drop_graph(c);   // a captured step holds the old launches
+    c->tile_policy = tile_policy;
+    c->split_target = split_target;
+  });
+}
+
 int ftcc_profile_enable(ftcc_ctx *c, int on) {
-  return guarded(c, [&] { c->prof_on = (on != 0); });
+  return guarded(c, [&] {
+    if (on && !c->prof_on) {
+      const char *e = getenv("FTCC_PROFILE_SITES");
+      c->prof_sites = e && atoi(e) != 0;
+    }
+    if (!on && c->prof_on && c->prof_sites) {
+      FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+      flush_profile(c);
+      std::vector<std::pair<std::pair<int, int>, ftcc_ctx::SiteStat>> v(
+          c->site_stats.begin(), c->site_stats.end());
+      std::sort(v.begin(), v.end(), [](const auto &a, const auto &b) {
+        return a.second.ms > b.second.ms;
+      });
+      fprintf(stderr, "ftcc sites: class line launches ms GB GFLOP "
+                      "M N K slices\n");
+      for (const auto &x : v)
+        fprintf(stderr, "ftcc site %d %d %lld %.3f %.3f %.3f %d %d %d %d\n",
+                x.first.first, x.first.second,
+                (long long)x.second.launches, x.second.ms,
+                x.second.bytes / 1e9, x.second.flops / 1e9, x.second.M,
+                x.second.N, x.second.K, x.second.nb);
+      c->site_stats.clear();
+    }
+    c->prof_on = (on != 0);
+  });
 }
 
 int ftcc_profile_read(ftcc_ctx *c, int64_t *launches, double *ms,

@@ -83,7 +83,20 @@ struct ftcc_ctx {
     hipEvent_t e0, e1;
     int cls;
     double flops, bytes;
+    int line, M, N, K, nb;   // call site and shape (per-site table)
   };
+  // per call site (source line) totals when FTCC_PROFILE_SITES is set,
+  // printed to stderr when profiling is switched off
+  struct SiteStat {
+    int64_t launches = 0;
+    double ms = 0.0, flops = 0.0, bytes = 0.0;
+    int M = 0, N = 0, K = 0, nb = 0;
+  };
+  bool prof_sites = false;
+  std::map<std::pair<int, int>, SiteStat> site_stats;
+  // shape and line of the GEMM being launched (0: none)
+  int cur_line = 0, cur_M = 0, cur_N = 0, cur_K = 0, cur_nb = 0;
+  int site_override = 0;   // the line of an enclosing run_spin call
   // split-bf16 operands of the five O(n^6) products (bf16x3 mode)
   SplitMat sp_tauA, sp_tauB, sp_t2rA, sp_aringB, sp_wmnijB, sp_wabefA, sp_wrB;
   SplitMat sp_oovvA, sp_ringA;
@@ -130,6 +143,9 @@ struct ftcc_ctx {
   // ftcc_step under the flip symmetry (for the chunk being evaluated): the
   // ladder operands' beta-beta blocks are read by no product
   bool flip_step_now = false;
+  // the last residual_all ran the flip-symmetric step form (its Tbar, like
+  // the amplitudes, is exactly symmetric under the spin flip)
+  bool last_flip_step = false;
   // set by ftcc_step around its residual: its update reads the spin-allowed
   // doubles of Tbar only, so the assembly leaves the forbidden ones unwritten
   bool tbar_scratch = false;
@@ -142,6 +158,9 @@ struct ftcc_ctx {
   double *b64[10] = {};            // FP64 blocked operands (fp64/fp32 modes)
   int b64_layout = -1;
   bool b64_fixed_valid = false;
+
+  // launch plan of the generic GEMMs (ftcc_set_gemm_plan)
+  int tile_policy = 0, split_target = 1024;
 
   bool prof_on = false;
   std::vector<Pending> prof_pending;
@@ -177,7 +196,8 @@ int guarded(ftcc_ctx *c, Fn fn) {
   }
 }
 // one GEMM of the engine (split-K scratch grown as needed)
-void run(ftcc_ctx *c, const Gemm &g, int prec, bool big);
+void run(ftcc_ctx *c, const Gemm &g, int prec, bool big = false,
+         int line = __builtin_LINE());
 // One index of a spin-restricted contraction: its letter, its orbital role
 // ('o' or 'v'; 'y' the slices of the base, '1' extent one) and its stride
 // in A, B and C (C's strides also address D).
@@ -191,7 +211,8 @@ struct SIdx {
 void run_spin(ftcc_ctx *c, const Gemm &base, int prec,
               const std::vector<SIdx> &Mi, const std::vector<SIdx> &Ni,
               const std::vector<SIdx> &Ki, const char *ruleA,
-              const char *ruleB, const char *ruleC);
+              const char *ruleB, const char *ruleC,
+              int line = __builtin_LINE());
 void check_ready(ftcc_ctx *c);
 void check_grid_D(ftcc_ctx *c);
 void drop_graph(ftcc_ctx *c);

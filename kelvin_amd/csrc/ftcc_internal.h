@@ -84,6 +84,12 @@ struct Gemm {
   const int *mmap = nullptr, *nmap = nullptr;
   int mlvl = -1, nlvl = -1;
   int64_t sAk0 = 0, sAk1 = 0, sBk0 = 0, sBk1 = 0;
+  // launch plan: tile shape policy (0 by the shape: narrow tiles when one
+  // dimension is at most 64 or K is; 1 wide: 128x128 unless one dimension is
+  // at most 64; 2 narrow: 128x64 or 64x128 by the aspect) and the workgroup
+  // count below which K is split
+  int tile_policy = 0;
+  int split_target = 1024;
 };
 
 // Scratch the GEMM engine may use for split-K partial sums and low-precision
@@ -335,6 +341,12 @@ struct UpdateArgs {
   // the number of spin-allowed doubles (> 0: only those are enumerated)
   int vrun[4] = {0, 0, 0, 0}, orun[4] = {0, 0, 0, 0};
   int64_t nallowed = 0;
+  // spin-flip symmetric Tbar and Told with label runs (mode 1 of the
+  // recurrence update): only the doubles of an alpha first virtual are
+  // enumerated (half of nallowed); each result is also written at its
+  // image (alpha and beta runs of equal length hv, ho swapped) and the
+  // partial sums count it twice
+  int flip_half = 0;
 };
 // One grid point y of the integral from the ng slices of a.tbar into the
 // single slice a.told: mode 0 stores it, mode 1 damps into it and leaves

@@ -44,10 +44,11 @@ struct Tile {
 // Short K (the O(n^5) contractions over one index): the narrow,
 // single-buffered tiles (three workgroups per CU) stream the output faster
 // than 128x128 (3.5% on those launches at n = 114).
-inline Tile pick_tile(int M, int N, int K = 1 << 30) {
+inline Tile pick_tile(int M, int N, int K, int policy) {
   if (M <= 64 && N > 64) return {64, 128};
   if (N <= 64 && M > 64) return {128, 64};
-  if (K <= 64) return M >= N ? Tile{128, 64} : Tile{64, 128};
+  if (policy == 2 || (policy == 0 && K <= 64))
+    return M >= N ? Tile{128, 64} : Tile{64, 128};
   return {128, 128};
 }
 
@@ -468,9 +469,10 @@ int choose_splits(const Gemm &g, int tiles, int bk) {
   // matrices) stream their operands from HBM once: split K until ~4
   // workgroups per CU keep enough loads in flight; >= 8 slabs per split.
   const int64_t blocks = (int64_t)tiles * g.nb1 * g.nb2;
-  if (blocks >= 1024) return 1;
+  const int64_t target = std::max(1, g.split_target);
+  if (blocks >= target) return 1;
   const int maxs = std::max(1, g.K / (8 * bk));
-  int want = (int)((1024 + blocks - 1) / blocks);
+  int want = (int)((target + blocks - 1) / blocks);
   return std::max(1, std::min(want, std::min(maxs, 2048)));
 }
 
@@ -504,7 +506,7 @@ void launch_layout(Tile tl, bool amf, bool bnf, bool kl, const GemmDev &d,
 }  // namespace
 
 size_t gemm_partial_elems(const Gemm &g) {
-  const Tile tl = pick_tile(g.M, g.N, g.K);
+  const Tile tl = pick_tile(g.M, g.N, g.K, g.tile_policy);
   const int tiles = ((g.M + tl.m - 1) / tl.m) * ((g.N + tl.n - 1) / tl.n);
   const int bk = BK;  // conservative: smallest slab gives the most splits
   const int splits = choose_splits(g, tiles, bk);
@@ -543,7 +545,7 @@ void gemm_run(const Gemm &g, int precision, Scratch &scr, hipStream_t s) {
   d.sCn = g.sCn;
   d.sC1 = g.sC1;
   d.sC2 = g.sC2;
-  const Tile tl = pick_tile(g.M, g.N, g.K);
+  const Tile tl = pick_tile(g.M, g.N, g.K, g.tile_policy);
   d.tilesM = (g.M + tl.m - 1) / tl.m;
   d.tilesN = (g.N + tl.n - 1) / tl.n;
   if (g.K <= 0) {

@@ -61,6 +61,7 @@ struct SbProb {
 };
 struct SbDev {
   int nprob, nb;
+  int group_m;               // tile rows per band of the launch order
   SbProb pr[MAXP];
   double *C;
   int64_t sCm, sCn, sCb;
@@ -82,9 +83,9 @@ __device__ inline SbProb tile_of(const SbDev &p, int &bt, int &tm, int &tn) {
   const int per = pr.tilesM * pr.tilesN;
   bt = local / per;
   const int t = local % per;
-  const int span = GROUP_M * pr.tilesN;
-  const int first_m = (t / span) * GROUP_M;
-  const int gsz = min(pr.tilesM - first_m, GROUP_M);
+  const int span = p.group_m * pr.tilesN;
+  const int first_m = (t / span) * p.group_m;
+  const int gsz = min(pr.tilesM - first_m, p.group_m);
   tm = first_m + (t % span) % gsz;
   tn = (t % span) / gsz;
   return pr;
@@ -512,8 +513,15 @@ void sbgemm_grouped(int nprob, const SbProblem *prob, int nb, double alpha,
   require(nprob >= 1 && nprob <= MAXP, "1 to 3 problems per grouped launch");
   static std::once_flag attr;
   set_max_lds_once(attr, (const void *)sbgemm_kernel, (int)LDS_BYTES);
+  // band height of the launch order (FTCC_SB_GROUP_M tunes it)
+  static const int group_m = [] {
+    const char *e = getenv("FTCC_SB_GROUP_M");
+    const int v = e ? atoi(e) : GROUP_M;
+    return v >= 1 && v <= 64 ? v : GROUP_M;
+  }();
   SbDev p;
   p.nb = nb;
+  p.group_m = group_m;
   p.C = C;
   p.sCm = sCm;
   p.sCn = sCn;

@@ -174,29 +174,49 @@ __global__ __launch_bounds__(UT) void update_kernel(UpdateArgs p, int mode) {
 // flops per grid point, and the kernel is bound by its HBM streams alone:
 // Tbar in (and T out in mode 0), Told in and out in mode 1.  Grid points are
 // taken in batches of RCH so that every load of a batch is in flight
-// together.  In-place integration is safe: slice y is read before it is
-// written and earlier slices are carried in registers.
+// together (batches of 16 or 24 measured slower at n = 114, ng = 24).
+// In-place integration is safe: slice y is read before it is written and
+// earlier slices are carried in registers.
 constexpr int RCH = 8;
 
 // Allowed doubles of the spin layout when the labels are runs (alpha then
 // beta in each list): the six (sa, sb, si, sj) boxes with sa + sb = si + sj,
 // each [a range] x [b range] x [i range] x [j range], enumerated with j
 // innermost.  Returns false past the last element.
+// x / d and x % d for 0 <= x < 2^50 and 0 < d < 2^20 through one FP64
+// multiply by 1/d and an exact integer correction (no 64-bit integer
+// division, which the GPU expands into a long instruction sequence)
+__device__ inline int64_t divmod(int64_t x, int d, int &rem) {
+  int64_t q = (int64_t)((double)x * (1.0 / (double)d));
+  int64_t r = x - q * d;
+  if (r < 0) {
+    --q;
+    r += d;
+  } else if (r >= d) {
+    ++q;
+    r -= d;
+  }
+  rem = (int)r;
+  return q;
+}
 __device__ inline bool run_elem(const UpdateArgs &p, int64_t k, Elem &q) {
-  for (int box = 0; box < 16; ++box) {
+  const int nbox = p.flip_half ? 8 : 16;   // flip_half: alpha a only
+  for (int box = 0; box < nbox; ++box) {
     const int sa = box >> 3, sb = (box >> 2) & 1, si = (box >> 1) & 1,
               sj = box & 1;
     if (sa + sb != si + sj) continue;
-    const int64_t na = p.vrun[2 * sa + 1], nb = p.vrun[2 * sb + 1],
-                  ni = p.orun[2 * si + 1], nj = p.orun[2 * sj + 1];
-    const int64_t size = na * nb * ni * nj;
+    const int na = p.vrun[2 * sa + 1], nb = p.vrun[2 * sb + 1],
+              ni = p.orun[2 * si + 1], nj = p.orun[2 * sj + 1];
+    const int64_t size = (int64_t)na * nb * ni * nj;
     if (k < size) {
-      q.j = p.orun[2 * sj] + (int)(k % nj);
-      k /= nj;
-      q.i = p.orun[2 * si] + (int)(k % ni);
-      k /= ni;
-      q.b = p.vrun[2 * sb] + (int)(k % nb);
-      q.a = p.vrun[2 * sa] + (int)(k / nb);
+      int r;
+      k = divmod(k, nj, r);
+      q.j = p.orun[2 * sj] + r;
+      k = divmod(k, ni, r);
+      q.i = p.orun[2 * si] + r;
+      k = divmod(k, nb, r);
+      q.b = p.vrun[2 * sb] + r;
+      q.a = p.vrun[2 * sa] + (int)k;
       return true;
     }
     k -= size;
@@ -220,14 +240,22 @@ __global__ __launch_bounds__(UT) void update_rec_kernel(UpdateArgs p,
   double dsum = 0.0, nsum = 0.0, esum = 0.0;
   // with label runs only the allowed doubles are enumerated
   const bool runs = DOUBLES && p.nallowed > 0;
-  const int64_t count = runs ? p.nallowed : per;
+  const bool half = runs && p.flip_half && mode == 1;
+  const int64_t count = runs ? (half ? p.nallowed / 2 : p.nallowed) : per;
+  const int hv = p.vrun[1], ho = p.orun[1];
   for (int64_t k = blockIdx.x * (int64_t)UT + threadIdx.x; k < count;
        k += (int64_t)gridDim.x * UT) {
     Elem q;
-    int64_t e = k;
+    int64_t e = k, eimg = 0;
     if (runs) {
       if (!run_elem(p, k, q)) break;
       e = (((int64_t)q.a * nv + q.b) * no + q.i) * no + q.j;
+      if (half) {
+        auto fv = [&](int x) { return x < hv ? x + hv : x - hv; };
+        auto fo = [&](int x) { return x < ho ? x + ho : x - ho; };
+        eimg = (((int64_t)fv(q.a) * nv + fv(q.b)) * no + fo(q.i)) * no +
+               fo(q.j);
+      }
     } else {
       q = decode<DOUBLES>(e, nv, no);
     }
@@ -273,14 +301,16 @@ __global__ __launch_bounds__(UT) void update_rec_kernel(UpdateArgs p,
             p.tbar[y * per + e] = tn;
           } else {
             const double df = tn - to[k];
-            dsum += df * df;
             const double tnew = p.damp * to[k] + (1.0 - p.damp) * tn;
             p.told[y * per + e] = tnew;
-            nsum += tnew * tnew;
+            if (half) p.told[y * per + eimg] = tnew;
+            const double w = half ? 2.0 : 1.0;
+            dsum += w * (df * df);
+            nsum += w * (tnew * tnew);
             if (DOUBLES) {
               const double *t1 = p.t1new + y * per1;
               const double qq = t1[q.a * no + q.i] * t1[q.b * no + q.j];
-              eacc += cg[y] * (0.25 * tnew + 0.5 * qq);
+              eacc += w * (cg[y] * (0.25 * tnew + 0.5 * qq));
             } else {
               eacc += cg[y] * tnew;
             }
@@ -302,6 +332,11 @@ int launch_update(const UpdateArgs &a, int mode, hipStream_t s) {
   const int blocks =
       (int)std::max<int64_t>(1, std::min<int64_t>((per + UT - 1) / UT, MAX_BLOCKS));
   if (a.rec) {
+    require(!a.flip_half ||
+                (a.vrun[1] == a.vrun[3] && a.orun[1] == a.orun[3] &&
+                 a.vrun[0] == 0 && a.vrun[2] == a.vrun[1] &&
+                 a.orun[0] == 0 && a.orun[2] == a.orun[1]),
+            "flip-half update needs equal alpha and beta runs");
     hipLaunchKernelGGL((update_rec_kernel<DOUBLES>), dim3(blocks), dim3(UT),
                        sizeof(double) * 4 * a.ng, s, a, mode);
     FTCC_CHECK_HIP(hipGetLastError());

@@ -529,6 +529,11 @@ class FTEngine:
                    ctypes.byref(fl))
         return n.value, ms.value, fl.value
 
+    def set_gemm_plan(self, tile_policy=0, split_target=1024):
+        """Launch plan of the generic GEMMs (see ``ftcc_set_gemm_plan``):
+        the knobs the tile autotuner of the batch searches."""
+        self._call("ftcc_set_gemm_plan", int(tile_policy), int(split_target))
+
     PROFILE_CLASSES = ("products", "n5_gemms", "gathers", "assembly",
                        "update")
 

@@ -285,3 +285,57 @@ def test_c5_derivatives():
     assert abs(N - Nf) < 1e-6
     assert abs(E - Ef) < 1e-6
     assert abs(S - Sf) < 1e-5
+
+
+# ------------------------------------------------------------------- C4
+C4_MIX = [dict(nk=19, ng=8, t_over_tf=0.3, rs=3.0, N=14),
+          dict(nk=27, ng=12, t_over_tf=0.6, rs=2.0, N=14),
+          dict(nk=33, ng=16, t_over_tf=0.9, rs=1.5, N=14),
+          dict(nk=19, ng=24, t_over_tf=0.5, rs=4.5, N=14),
+          dict(nk=33, ng=8, t_over_tf=0.2, rs=2.5, N=14),
+          dict(nk=27, ng=24, t_over_tf=0.8, rs=1.2, N=14)]
+
+
+def test_c4_reduced_batch():
+    """A reduced serving mix (n in {38, 54, 66}, ng in {8, ..., 24}): the
+    concurrent graph solves on four streams, with launch shapes from the
+    autotuner, give the plain loop's grand potentials and iteration counts;
+    the smallest instance's MP2 energy and first three iterations equal the
+    oracle's.  Throughput and p50/p99 latency are reported."""
+    from kelvin_amd import batch
+    from kelvin_amd.ueg_system import ueg_from_config
+    dev()
+    kw = dict(damp=0.5, max_iter=60, econv=1e-8)
+    chunks, plans = {}, {}
+    for key in sorted({(x["nk"], x["ng"]) for x in C4_MIX}):
+        chunks[key], plans[key], _ = batch.autotune(key[0], key[1], reps=2)
+    loop, t_loop = batch.solve_loop(C4_MIX, **kw)
+    bat, t_bat = batch.solve_batch(C4_MIX, workers=4, chunks=chunks,
+                                   plans=plans, **kw)
+    for a, b in zip(loop, bat):
+        assert abs(a["Omega"] - b["Omega"]) < 1e-10
+        assert a["iterations"] == b["iterations"]
+    # the smallest instance against the oracle
+    x = C4_MIX[0]
+    sysg = ueg_from_config(x["nk"], x["rs"], N=x["N"],
+                           t_over_tf=x["t_over_tf"])
+    osys = oracle.UEGSystem(sysg.T, sysg.L, sysg.basis.cutoff, sysg.mu,
+                            norb=x["nk"])
+    hist = []
+    okw = dict(kw, ngrid=x["ng"], max_iter=3)
+    r = oracle.ft_ccsd(osys, sysg.T, sysg.mu, history=hist, **okw)
+    assert abs(bat[0]["mp2"] - r["mp2"]) < 1e-11
+    for (E, res), (Eo, reso) in zip(bat[0]["history"][:3], hist):
+        assert abs(E - Eo) < 1e-11
+        assert abs(res - reso) < 1e-9 * max(1.0, reso)
+    lat_l = [v["latency"] for v in loop]
+    lat_b = [v["latency"] for v in bat]
+    report("c4_reduced", instances=len(C4_MIX), loop_s=t_loop,
+           batched_s=t_bat, speedup=t_loop / t_bat,
+           latency_p50_loop=float(numpy.percentile(lat_l, 50)),
+           latency_p99_loop=float(numpy.percentile(lat_l, 99)),
+           latency_p50_batched=float(numpy.percentile(lat_b, 50)),
+           latency_p99_batched=float(numpy.percentile(lat_b, 99)),
+           chunks={"%d,%d" % k: v for k, v in chunks.items()},
+           plans={"%d,%d" % k: list(v) for k, v in plans.items()},
+           iterations=[v["iterations"] for v in bat])
