@@ -142,6 +142,17 @@ def _solve_one(inst, precision, graph, chunk, plan, ccsd_kw, t_batch=None):
     return out
 
 
+def _trim_cache(limit_gb=24.0):
+    """Hand torch's cached, unused device blocks back once they exceed
+    ``limit_gb``: the library allocates with hipMalloc beside torch, and
+    instances of other sizes do not reuse each other's blocks.  (Releasing
+    synchronises the device, so it is done rarely rather than after every
+    instance.)"""
+    idle = torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
+    if idle > limit_gb * 2 ** 30:
+        torch.cuda.empty_cache()
+
+
 def solve_loop(instances, precision="fp64", progress=None, **ccsd_kw):
     """The plain loop: one instance after another on the default stream, in
     the given order; each result's ``latency`` is its completion time from
@@ -150,6 +161,7 @@ def solve_loop(instances, precision="fp64", progress=None, **ccsd_kw):
     res = []
     for k, x in enumerate(instances):
         res.append(_solve_one(x, precision, False, None, None, ccsd_kw, t))
+        _trim_cache()
         if progress:
             progress(k + 1, len(instances))
     return res, time.perf_counter() - t
@@ -161,8 +173,9 @@ def solve_batch(instances, precision="fp64", workers=4, chunks=None,
     stream, every step a HIP graph, instances started shortest first;
     ``chunks`` and ``plans`` map (nk, ng) to a chunk size and a GEMM plan
     (see ``autotune``); the chunk is capped by each instance's share of HBM
-    (half the device's memory over ``workers`` for the residual workspace,
-    through FTCC_WORKSPACE_GB).  Returns (results in input order, wall
+    (0.35 of the device's memory over ``workers`` for the residual
+    workspace, through FTCC_WORKSPACE_GB; amplitudes, integrals and torch's
+    cache take the rest).  Returns (results in input order, wall
     seconds); each result's ``seconds`` is that instance's own solve time
     and ``latency`` its completion time from the start of the batch."""
     if not 1 <= workers <= 4:
@@ -172,7 +185,8 @@ def solve_batch(instances, precision="fp64", workers=4, chunks=None,
     plans = plans or {}
     total = torch.cuda.get_device_properties(0).total_memory
     saved = os.environ.get("FTCC_WORKSPACE_GB")
-    os.environ["FTCC_WORKSPACE_GB"] = "%.3f" % (0.5 * total / workers / 2 ** 30)
+    os.environ["FTCC_WORKSPACE_GB"] = "%.3f" % (0.35 * total / workers /
+                                                2 ** 30)
     order = sorted(range(len(instances)), key=lambda k: cost(instances[k]))
     t = time.perf_counter()
 
@@ -184,6 +198,7 @@ def solve_batch(instances, precision="fp64", workers=4, chunks=None,
             r = _solve_one(inst, precision, True, chunks.get(key),
                            plans.get(key), ccsd_kw, t)
         stream.synchronize()
+        _trim_cache()
         return k, r
 
     res = [None] * len(instances)

@@ -28,6 +28,9 @@ Extensions over the reference (all optional):
               choices).
   gemm_plan   (tile_policy, split_target) of the generic GEMMs (default
               the library's; ``kelvin_amd.batch.autotune`` times them).
+  profile     time every kernel class of the solve with HIP events;
+              ``kernel_classes`` then holds {class: (launches, ms, flops,
+              bytes)} of the whole solve (default False).
 """
 import copy
 import logging
@@ -50,7 +53,7 @@ class ccsd(object):
                  tconv=None, max_iter=40, damp=0.0, ngrid=10, realtime=False,
                  athresh=0.0, quad='lin', rt_iter="all", precision="fp64",
                  refine=True, refine_max_iter=None, graph=False,
-                 chunk=None, gemm_plan=None):
+                 chunk=None, gemm_plan=None, profile=False):
         self.T = T
         self.mu = mu
         self.finite_T = False if T == 0 else True
@@ -71,6 +74,8 @@ class ccsd(object):
         self.graph = graph
         self.chunk = chunk
         self.gemm_plan = gemm_plan
+        self.profile = profile
+        self.kernel_classes = None
         if self.finite_T:
             self.realtime = True
         if not sys.verify(self.T, self.mu):
@@ -272,6 +277,9 @@ class ccsd(object):
             eng.set_gemm_plan(*self.gemm_plan)
         if self.graph:
             eng.use_graph = True
+        if self.profile:
+            eng.profile(True)
+            eng.profile_classes()
         # iterates started from the driving terms conserve spin: the
         # per-step check of the amplitudes is needed for a guess only
         eng.set_trusted(T1in is None and T2in is None)
@@ -369,6 +377,9 @@ class ccsd(object):
             torch.cuda.current_stream(eng.device).synchronize()
             self.niter_refine = len(self.history) - self.niter_unrefined
             self.time_refine = time.time() - t_ref
+        if self.profile:
+            self.kernel_classes = eng.profile_classes()
+            eng.profile(False)
         self.T1 = T1
         self.T2 = T2
         self.G0 = E0

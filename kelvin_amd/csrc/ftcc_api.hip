@@ -1683,8 +1683,11 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
   }
   // forbidden blocks (spin layout): written as zero without reading
   const bool zflip = flip && c->pack;
-  // algorithmic bytes: one read of each of the six inputs and one write per
-  // spin-allowed double
+  // algorithmic bytes per spin-allowed double: one read of each of the six
+  // inputs and one write (56); with packed pairs the four packed inputs
+  // hold one of each antisymmetric couple (40); the flip-symmetric step
+  // reads them for the pairs it computes, half of them, and writes every
+  // double (24, the measured traffic of that form)
   double allowed = (double)S2;
   if (c->spin5) {
     allowed = 0.0;
@@ -1696,7 +1699,10 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
                    c->o_count[si] * c->o_count[sj];
     }
   }
-  timed_class(c, ftcc_ctx::PROF_ASSEMBLY, 0.0, 56.0 * allowed * nb, [&] {
+  const double per_double =
+      flip_step ? 24.0 : (c->spin && c->pack ? 40.0 : 56.0);
+  timed_class(c, ftcc_ctx::PROF_ASSEMBLY, 0.0, per_double * allowed * nb,
+              [&] {
     r2_final(t2bar, c->wr, c->aring, c->s1, c->s2, c->z, V, O, nb, s,
              c->spin ? c->d_lab_v : nullptr, c->spin ? c->d_lab_o : nullptr,
              vvoo, c->spin && c->pack, zflip ? hv : 0, zflip ? ho : 0,

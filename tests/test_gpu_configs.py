@@ -43,6 +43,25 @@ def report(name, **kw):
     print("CONFIG " + json.dumps(dict(test=name, **kw)), flush=True)
 
 
+def class_rates(classes):
+    """Achieved rates of the unrefined bf16x3 solve from the library's
+    HIP-event timing of every launch: split-bf16 products in TFLOP/s of
+    algorithmic product flops (and the matrix-core rate, three MFMA
+    products each, as a fraction of the 2.5 PF bf16 peak); the HBM-bound
+    classes (update, doubles assembly, gathers, O(n^5) GEMMs) in GB/s of
+    algorithmic bytes."""
+    out = {}
+    n, ms, fl, by = classes["products"]
+    if ms > 0:
+        out["products_tflops"] = fl / (ms * 1e-3) / 1e12
+        out["mfma_rate_frac"] = 3.0 * out["products_tflops"] / 2500.0
+    for k in ("update", "assembly", "gathers", "n5_gemms"):
+        n, ms, fl, by = classes[k]
+        if ms > 0:
+            out[k + "_gbs"] = by / (ms * 1e-3) / 1e9
+    return out
+
+
 def solve(sysg, precision="fp64", refine=True, **kw):
     """One ccsd solve; returns a summary and releases the engine."""
     from kelvin_amd.ccsd import ccsd
@@ -54,7 +73,7 @@ def solve(sysg, precision="fp64", refine=True, **kw):
                unrefined=cc.Gcc_unrefined, phases=cc.phase_steps,
                time=cc.time_unrefined + cc.time_refine,
                wall=time.time() - t0, history=list(cc.history),
-               mp2=cc.mp2)
+               mp2=cc.mp2, classes=cc.kernel_classes)
     cc.engine = None
     del cc
     gc.collect()
@@ -139,13 +158,14 @@ def test_c3_sweep(rs, t_over_tf):
     assert numpy.isfinite(ref["Omega"])
     assert ref["history"][-1][1] < kw["tconv"]
     fin = solve(sysg, "bf16x3", refine=True, **kw)
-    plain = solve(sysg, "bf16x3", refine=False, **kw)
+    plain = solve(sysg, "bf16x3", refine=False, profile=True, **kw)
     report("c3", rs=rs, t_over_tf=t_over_tf, Omega=ref["Omega"],
            niter_fp64=ref["niter"], time_fp64=ref["time"],
            d_omega_refined=fin["Omega_cc"] - ref["Omega_cc"],
            niter_refined=fin["niter"], phases=fin["phases"],
            time_refined=fin["time"],
-           d_omega_unrefined=plain["Omega_cc"] - ref["Omega_cc"])
+           d_omega_unrefined=plain["Omega_cc"] - ref["Omega_cc"],
+           **class_rates(plain["classes"]))
     assert abs(fin["Omega"] - ref["Omega"]) < 1e-9
     assert abs(plain["Omega"] - ref["Omega"]) < 1e-5
 
