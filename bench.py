@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--chunk", type=int, default=0,
                     help="grid points per residual chunk (0: the library's "
                          "choice from its workspace budget)")
+    ap.add_argument("--gemm-plan", default=None,
+                    help="tile_policy/split_target of the generic GEMMs "
+                         "(default: the library's)")
     ap.add_argument("--nk", type=int, default=57)
     ap.add_argument("--ng", type=int, default=24)
     ap.add_argument("--rs", type=float, default=2.0)
@@ -227,6 +230,8 @@ def main():
     eng = FTEngine(n, n, ng, precision=args.precision)
     if args.chunk:
         eng.set_chunk(args.chunk)
+    if args.gemm_plan:
+        eng.set_gemm_plan(*[int(x) for x in args.gemm_plan.split("/")])
     fock, eri, e1 = sysg.g_device_integrals(eng)     # built in HBM
     log("UEG nk=%d n=%d beta=%.4f mu=%.6f, integrals built on the device "
         "in %.1f s" % (args.nk, n, beta, mu, time.time() - t0))

@@ -620,6 +620,103 @@ int64_t elems(const Ten &t, const Extents &ext) {
   return s;
 }
 
+// Operand element (r, k) of a GEMM whose rows (columns) and K run over up to
+// three index levels: index x = (x0 * n1 + x1) * n2 + x2 sits at
+// x0 * s0 + x1 * s1 + x2 * s2.
+struct PackIdx {
+  int n1, n2;
+  int64_t s0, s1, s2;
+};
+__device__ inline int64_t pack_off(int x, const PackIdx &l) {
+  const int x2 = x % l.n2, q = x / l.n2;
+  return (int64_t)(q / l.n1) * l.s0 + (int64_t)(q % l.n1) * l.s1 +
+         (int64_t)x2 * l.s2;
+}
+// dst[r][k] (row length K) = src(r, k): rows along the thread index when the
+// source is contiguous along rows (through a 32x32 LDS tile), else along k.
+__global__ __launch_bounds__(256) void pack_kernel(const double *src,
+                                                   PackIdx rl, PackIdx kl,
+                                                   int R, int K, double *dst,
+                                                   int along_r) {
+  __shared__ double tile[32][33];
+  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (along_r) {
+    for (int i = ty; i < 32; i += 8) {
+      const int r = r0 + tx, k = k0 + i;
+      tile[i][tx] =
+          (r < R && k < K) ? src[pack_off(r, rl) + pack_off(k, kl)] : 0.0;
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+      const int r = r0 + i, k = k0 + tx;
+      if (r < R && k < K) dst[(int64_t)r * K + k] = tile[tx][i];
+    }
+  } else {
+    for (int i = ty; i < 32; i += 8) {
+      const int r = r0 + i, k = k0 + tx;
+      if (r < R && k < K)
+        dst[(int64_t)r * K + k] = src[pack_off(r, rl) + pack_off(k, kl)];
+    }
+  }
+}
+
+PackIdx pack_levels(int n1, int n2, int64_t s0, int64_t s1, int64_t s) {
+  // Gemm convention: n2 == 0 means one level with stride s
+  if (n2 == 0) return PackIdx{1, 1 << 30, 0, 0, s};
+  return PackIdx{n1, n2, s0, s1, s};
+}
+
+// A large block with strided operands runs on contiguous copies ([M][K] and
+// [N][K], K fastest): the engine's strided loads of such operands reach a
+// third of its rate on the contiguous form.  Returns false when the block
+// is left as it is (small, already contiguous, batched or level-mapped).
+bool pack_block(ftcc_ctx *c, Gemm &g) {
+  const double flops = 2.0 * g.M * (double)g.N * g.K;
+  if (flops < 1e9 || g.nb1 != 1 || g.nb2 != 1 || g.mmap || g.nmap)
+    return false;
+  if (g.sAk == 1 && g.sBk == 1 && g.K2 == 0) return false;
+  const int64_t need = ((int64_t)g.M + g.N) * g.K;
+  if (need > c->pack_elems) {
+    FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+    dfree(c, c->pack_buf);
+    c->pack_buf = nullptr;
+    c->pack_elems = 0;
+    c->pack_buf = dalloc(c, need);
+    c->pack_elems = need;
+  }
+  double *pa = c->pack_buf, *pb = c->pack_buf + (int64_t)g.M * g.K;
+  const PackIdx kA = pack_levels(g.K1, g.K2, g.sAk0, g.sAk1, g.sAk);
+  const PackIdx kB = pack_levels(g.K1, g.K2, g.sBk0, g.sBk1, g.sBk);
+  const PackIdx rA = pack_levels(g.M1, g.M2, g.sAm0, g.sAm1, g.sAm);
+  const PackIdx rB = pack_levels(g.N1, g.N2, g.sBn0, g.sBn1, g.sBn);
+  require(g.M < (1 << 30) && g.N < (1 << 30) && g.K < (1 << 30),
+          "pack: operand too large");
+  const dim3 ga((g.K + 31) / 32, (g.M + 31) / 32), gb((g.K + 31) / 32,
+                                                     (g.N + 31) / 32);
+  require(ga.y < 65536 && gb.y < 65536, "pack: grid too large");
+  hipLaunchKernelGGL(pack_kernel, ga, dim3(256), 0, c->stream, g.A, rA, kA,
+                     g.M, g.K, pa, g.sAm == 1 ? 1 : 0);
+  hipLaunchKernelGGL(pack_kernel, gb, dim3(256), 0, c->stream, g.B, rB, kB,
+                     g.N, g.K, pb, g.sBn == 1 ? 1 : 0);
+  FTCC_CHECK_HIP(hipGetLastError());
+  // the packed rows keep the levels of C: row m at m * K
+  const int64_t K = g.K;
+  g.A = pa;
+  g.sAm = K;
+  g.sAm1 = g.M2 > 0 ? (int64_t)g.M2 * K : 0;
+  g.sAm0 = g.M2 > 0 ? (int64_t)g.M1 * g.M2 * K : 0;
+  g.B = pb;
+  g.sBn = K;
+  g.sBn1 = g.N2 > 0 ? (int64_t)g.N2 * K : 0;
+  g.sBn0 = g.N2 > 0 ? (int64_t)g.N1 * g.N2 * K : 0;
+  g.K1 = 1;
+  g.K2 = 0;
+  g.sAk = g.sBk = 1;
+  g.sAk0 = g.sAk1 = g.sBk0 = g.sBk1 = 0;
+  return true;
+}
+
 // C[...] = alpha sum_K A[...] B[...] + beta C[...] (beta 0 or 1).  When the
 // orbital labels form one alpha run and one beta run in each list, only the
 // spin blocks allowed by the three operands' rules are multiplied (a
@@ -627,8 +724,21 @@ int64_t elems(const Ten &t, const Extents &ext) {
 // when a block has no layout the engine takes, the whole product runs
 // dense.
 void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
-         const Ten &C) {
+         const Ten &C, int line = __builtin_LINE()) {
   require(beta == 0.0 || beta == 1.0, "contraction: beta must be 0 or 1");
+  // per-call-site timing: the GEMMs below belong to the caller's line
+  struct Site {
+    ftcc_ctx *c;
+    int saved, policy;
+    ~Site() {
+      c->site_override = saved;
+      c->tile_policy = policy;
+    }
+  } site{c, c->site_override, c->tile_policy};
+  if (!c->site_override) c->site_override = line;
+  // the adjoint's blocks run on narrow tiles: 41 against 24 TFLOP/s on its
+  // 3249^3 blocks at n = 114 (three workgroups per CU instead of two)
+  c->tile_policy = 2;
   const Groups G = groups_of(A0, B0, C);
   Extents full;
   std::vector<char> letters;
@@ -678,7 +788,11 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
     if (ok) {
       if (beta == 0.0) fill(const_cast<double *>(C.p), elems(C, full), 0.0,
                             c->stream);
-      for (const Block &blk : blocks) run(c, blk.g, FTCC_PREC_FP64, false);
+      for (const Block &blk : blocks) {
+        Gemm g = blk.g;
+        pack_block(c, g);
+        run(c, g, FTCC_PREC_FP64, false);
+      }
       return;
     }
   }
@@ -686,6 +800,7 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
   require(plan(full, alpha, A0, B0, beta, C, G, g),
           "contraction " + A0.l + "," + B0.l + "->" + C.l +
               ": no layout the GEMM engine takes");
+  pack_block(c, g);
   run(c, g, FTCC_PREC_FP64, false);
 }
 

@@ -181,7 +181,8 @@ void flush_profile(ftcc_ctx *c) {
     c->prof_bytes[p.cls] += p.bytes;
     c->prof_launches[p.cls] += 1;
     if (c->prof_sites) {
-      ftcc_ctx::SiteStat &st = c->site_stats[{p.cls, p.line}];
+      ftcc_ctx::SiteStat &st = c->site_stats[std::array<int64_t, 6>{
+          p.cls, p.line, p.M, p.N, p.K, p.nb}];
       st.launches += 1;
       st.ms += ms;
       st.flops += p.flops;
@@ -2865,7 +2866,7 @@ int ftcc_profile_enable(ftcc_ctx *c, int on) {
     if (!on && c->prof_on && c->prof_sites) {
       FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
       flush_profile(c);
-      std::vector<std::pair<std::pair<int, int>, ftcc_ctx::SiteStat>> v(
+      std::vector<std::pair<std::array<int64_t, 6>, ftcc_ctx::SiteStat>> v(
           c->site_stats.begin(), c->site_stats.end());
       std::sort(v.begin(), v.end(), [](const auto &a, const auto &b) {
         return a.second.ms > b.second.ms;
@@ -2874,7 +2875,7 @@ int ftcc_profile_enable(ftcc_ctx *c, int on) {
                       "M N K slices\n");
       for (const auto &x : v)
         fprintf(stderr, "ftcc site %d %d %lld %.3f %.3f %.3f %d %d %d %d\n",
-                x.first.first, x.first.second,
+                (int)x.first[0], (int)x.first[1],
                 (long long)x.second.launches, x.second.ms,
                 x.second.bytes / 1e9, x.second.flops / 1e9, x.second.M,
                 x.second.N, x.second.K, x.second.nb);

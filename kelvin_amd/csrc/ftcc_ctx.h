@@ -4,6 +4,7 @@
 #pragma once
 #include "ftcc_internal.h"
 
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -93,7 +94,8 @@ struct ftcc_ctx {
     int M = 0, N = 0, K = 0, nb = 0;
   };
   bool prof_sites = false;
-  std::map<std::pair<int, int>, SiteStat> site_stats;
+  // keyed by (class, line, M, N, K, slices): one entry per shape of a site
+  std::map<std::array<int64_t, 6>, SiteStat> site_stats;
   // shape and line of the GEMM being launched (0: none)
   int cur_line = 0, cur_M = 0, cur_N = 0, cur_K = 0, cur_nb = 0;
   int site_override = 0;   // the line of an enclosing run_spin call
@@ -159,6 +161,9 @@ struct ftcc_ctx {
   int b64_layout = -1;
   bool b64_fixed_valid = false;
 
+  // contiguous copies of large strided GEMM operands (adjoint contractions)
+  double *pack_buf = nullptr;
+  int64_t pack_elems = 0;
   // launch plan of the generic GEMMs (ftcc_set_gemm_plan)
   int tile_policy = 0, split_target = 1024;
 
