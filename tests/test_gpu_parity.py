@@ -62,7 +62,11 @@ def random_T(ng, nv, no, rng, scale=0.2):
 # ---------------------------------------------------------------- GEMM engine
 @pytest.mark.parametrize("M,N,K,batch", [(16, 16, 4, 1), (129, 70, 33, 1),
                                          (200, 130, 300, 3), (7, 5, 20000, 1),
-                                         (256, 256, 256, 2)])
+                                         (256, 256, 256, 2),
+                                         # short K with a small side: the
+                                         # streaming kernel
+                                         (57, 3000, 57, 2), (3000, 57, 40, 1),
+                                         (64, 64, 64, 1), (5, 1000, 3, 3)])
 @pytest.mark.parametrize("layout", ["nn", "tn", "nt", "tt"])
 def test_gemm_fp64_matches_torch(M, N, K, batch, layout):
     from kelvin_amd.engine import FTEngine
