@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--rs", type=float, default=2.0)
     ap.add_argument("--t-over-tf", type=float, default=0.1)
     ap.add_argument("--nelec", type=int, default=14)
-    ap.add_argument("--damp", type=float, default=0.0)
+    ap.add_argument("--damp", type=float, default=0.3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=90.0,
                     help="predicted CPU sample time above which the sample "
