@@ -303,10 +303,18 @@ int ftcc_adjoint_begin(ftcc_ctx *ctx, const double *T1_dev,
 int ftcc_adjoint_cached(ftcc_ctx *ctx, int *nslices);
 /* G = J^T Lt at every grid point: the transpose of the Jacobian of the
  * residual (cqcpy _Lambda_opt's operator), Lt2 antisymmetric in (a,b) and
- * in (i,j); G2 is projected onto such tensors.  FP64. */
+ * in (i,j); G2 is projected onto such tensors.  FP64, or with its large
+ * O(n^6) blocks as split-bf16 products after ftcc_adjoint_precision. */
 int ftcc_adjoint_apply(ftcc_ctx *ctx, const double *Lt1_dev,
                        const double *Lt2_dev, double *G1_dev, double *G2_dev);
 int ftcc_adjoint_end(ftcc_ctx *ctx);
+/* Contraction precision of the large blocks of ftcc_adjoint_apply:
+ * FTCC_PREC_FP64 (default) or FTCC_PREC_BF16X3 (the low-precision phase of
+ * a refined lambda iteration).  Unbatched blocks of at least
+ * FTCC_ADJOINT_LOWP_MIN_GFLOP GFLOP (environment, read by this call; default
+ * 10) run split-bf16.  No counterpart in the reference (its lambda equations
+ * are FP64 numpy). */
+int ftcc_adjoint_precision(ftcc_ctx *ctx, int precision);
 /* out[x] = sum_y G[y,x] exp((t_x - t_y) D) L[y] (x < y; weight 1 for
  * x >= y): the transpose of ftcc_integrate over the grid points
  * (kelvin/quadrature.py:320-345 int_L1/int_L2 without their g weights). */

@@ -62,6 +62,7 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_int)]),
     "ftcc_adjoint_apply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ftcc_adjoint_end": (ctypes.c_int, [_vp]),
+    "ftcc_adjoint_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ftcc_integrate_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ftcc_lambda_update": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp,
                                           _c_double_p, ctypes.c_double,

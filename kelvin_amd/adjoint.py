@@ -41,7 +41,8 @@ def _norm_scaled(eng, L, w):
     return math.sqrt(sum(wy * wy * eng.sumsq(L[y]) for y, wy in enumerate(w)))
 
 
-def solve_lambda(eng, T1, T2, g, beta, conv, L1=None, L2=None, history=None):
+def solve_lambda(eng, T1, T2, g, beta, conv, L1=None, L2=None, history=None,
+                 low=None, stats=None):
     """Adjoint amplitudes of the converged T (kelvin/cc_utils.py:414-480
     ft_lambda_iter with the guess of kelvin/ft_cc_equations.py:502-512, or
     kelvin's L1 / L2 as the start): L <- damp L + (1 - damp) L_new with
@@ -49,7 +50,18 @@ def solve_lambda(eng, T1, T2, g, beta, conv, L1=None, L2=None, history=None):
     the test res1 + res2 < tconv, all in kelvin's lambda scale.  The
     amplitude-only intermediates of J^T are built once (kept for as many
     grid points as FTCC_ADJOINT_CACHE_GB allows, rebuilt from T for the
-    others).  Returns (Lam1, Lam2, converged, iterations)."""
+    others).
+
+    ``low`` ("bf16x3") runs the refined form.  J^T is linear in Lt, so
+    (1) iterations with the large blocks of J^T as split-bf16 products run
+    until the residual is below max(100 tconv, 1e-6) or stalls below 1e-3;
+    (2) one FP64 application at that iterate, J^T_64(Lt0), becomes the
+    anchor of the following steps, J^T(Lt) = J^T_64(Lt0) + J^T_low(Lt - Lt0),
+    whose rounding error scales with |Lt - Lt0| rather than |Lt|; (3) once
+    kelvin's test holds, an FP64 step decides it: converged when the test
+    holds for that step too, otherwise that step is the new anchor.
+    ``stats`` receives the steps of each kind (1 low, 2 anchored, 3 FP64).
+    Returns (Lam1, Lam2, converged, iterations)."""
     g = numpy.asarray(g, dtype=numpy.float64)
     w1 = beta / g
     w2 = 4.0 * beta / g
@@ -72,10 +84,40 @@ def solve_lambda(eng, T1, T2, g, beta, conv, L1=None, L2=None, history=None):
         Lt1, Lt2 = eng.empty_amplitudes()
         G1, G2 = eng.empty_amplitudes()
         i, converged = 0, False
+        phase = 1 if low else 3
+        steps = {1: 0, 2: 0, 3: 0}
+        anchor = None     # (Lt1, Lt2, G1, G2) of the last FP64 application
+        anchors = 0
+        switch = max(100.0 * thresh, 1e-6)
+        best, flat = float("inf"), 0
+        if low:
+            eng.adjoint_precision(low)
         tbeg = time.time()
         while i < max_iter and not converged:
             eng.integrate_adjoint(Lam1, Lam2, Lt1, Lt2)
-            eng.adjoint_apply(Lt1, Lt2, G1, G2)
+            if phase == 3 or (phase == 2 and anchor is None):
+                kind = 3
+                if low:
+                    eng.adjoint_precision("fp64")
+                eng.adjoint_apply(Lt1, Lt2, G1, G2)
+                if low:
+                    eng.adjoint_precision(low)
+                    if anchor is None:
+                        anchor = tuple(x.clone() for x in (Lt1, Lt2, G1, G2))
+                    else:
+                        for dst, src in zip(anchor, (Lt1, Lt2, G1, G2)):
+                            dst.copy_(src)
+                    anchors += 1
+            elif phase == 2:
+                kind = 2
+                Lt1.sub_(anchor[0])
+                Lt2.sub_(anchor[1])
+                eng.adjoint_apply(Lt1, Lt2, G1, G2)
+                G1.add_(anchor[2])
+                G2.add_(anchor[3])
+            else:
+                kind = 1
+                eng.adjoint_apply(Lt1, Lt2, G1, G2)
             d1, d2, n1, n2 = eng.lambda_update(Lam1, Lam2, G1, G2, T1, g,
                                                beta, damp)
             res = math.sqrt(d1) / nl1 + math.sqrt(d2) / nl2
@@ -85,11 +127,29 @@ def solve_lambda(eng, T1, T2, g, beta, conv, L1=None, L2=None, history=None):
             if history is not None:
                 history.append(res)
             i += 1
-            converged = res < thresh
+            steps[kind] += 1
+            if res < best:
+                best, flat = res, 0
+            else:
+                flat += 1
+            if kind == 3:
+                converged = res < thresh
+                # an FP64 step that does not converge anchors the next ones
+                # (after four anchors the iteration stays FP64)
+                phase = 2 if (low and not converged and anchors < 4) else 3
+            elif phase == 1 and (res < switch or (flat >= 3 and res < 1e-3)):
+                phase = 2
+                logging.info("Lambda: FP64 anchor at iteration %d" % (i + 1))
+            elif phase == 2 and res < thresh:
+                phase = 3
+        if stats is not None:
+            stats["steps"] = steps
         if not converged:
             logging.warning("CCSD Lambda-equations did not converge!")
         logging.info("Total CCSD Lambda time: %f s" % (time.time() - tbeg))
     finally:
+        if low:
+            eng.adjoint_precision("fp64")
         eng.adjoint_end()
     return Lam1, Lam2, converged, i
 
@@ -147,9 +207,15 @@ def compute_esn(cc, L1=None, L2=None, gderiv=True):
         eng.residual(T1, T2, Tb1, Tb2, skip)
     conv = {"tconv": cc.tconv, "max_iter": cc.max_iter, "damp": cc.damp}
     cc.lambda_history = []
+    # a refined bf16x3 forward solve takes the refined lambda iteration
+    low = ("bf16x3" if cc.precision == "bf16x3" and cc.refine is True
+           else None)
+    lstats = {}
     Lam1, Lam2, lconv, lit = solve_lambda(eng, T1, T2, g, beta, conv, L1=L1,
-                                          L2=L2, history=cc.lambda_history)
+                                          L2=L2, history=cc.lambda_history,
+                                          low=low, stats=lstats)
     cc.lambda_converged, cc.lambda_iter = lconv, lit
+    cc.lambda_phases = lstats.get("steps")
     cc.adjoint_slices_kept = eng.adjoint_slices_kept
     t_lambda = time.time() - t0
     Omega_cc = cc.Gcc

@@ -717,6 +717,117 @@ bool pack_block(ftcc_ctx *c, Gemm &g) {
   return true;
 }
 
+// dst_hi/dst_lo[r][k] (row length ld) = the split of src(r, k) for k < K,
+// zero for K <= k < ld (the split-bf16 GEMM reads whole 32-deep slabs)
+__global__ __launch_bounds__(256) void split_pack_kernel(
+    const double *src, PackIdx rl, PackIdx kl, int R, int K, int ld,
+    __bf16 *hi, __bf16 *lo, int along_r) {
+  __shared__ double tile[32][33];
+  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  auto put = [&](int r, int k, double v) {
+    const float f = (float)v;
+    const __bf16 h = (__bf16)f;
+    hi[(int64_t)r * ld + k] = h;
+    lo[(int64_t)r * ld + k] = (__bf16)(f - (float)h);
+  };
+  if (along_r) {
+    for (int i = ty; i < 32; i += 8) {
+      const int r = r0 + tx, k = k0 + i;
+      tile[i][tx] =
+          (r < R && k < K) ? src[pack_off(r, rl) + pack_off(k, kl)] : 0.0;
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+      const int r = r0 + i, k = k0 + tx;
+      if (r < R && k < ld) put(r, k, tile[tx][i]);
+    }
+  } else {
+    for (int i = ty; i < 32; i += 8) {
+      const int r = r0 + i, k = k0 + tx;
+      if (r < R && k < ld)
+        put(r, k, k < K ? src[pack_off(r, rl) + pack_off(k, kl)] : 0.0);
+    }
+  }
+}
+
+// out[x] = offset of index x over the levels (int: the tensor has fewer
+// than 2^31 elements)
+__global__ void level_offsets_kernel(PackIdx l, int n, int *out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) out[x] = (int)pack_off(x, l);
+}
+
+// In the low-precision mode of the lambda iteration, a large block runs as a
+// split-bf16 product (lo*hi + hi*lo + hi*hi, FP32 accumulation): both
+// operands are split into contiguous K-fast bf16 copies and C is addressed
+// through per-row and per-column offset tables.  Returns false when the
+// block stays FP64 (the mode is off, the block is small, batched, mapped or
+// has an addend).
+bool lowp_block(ftcc_ctx *c, const Gemm &g) {
+  if (c->adj_prec != FTCC_PREC_BF16X3) return false;
+  const double flops = 2.0 * g.M * (double)g.N * g.K;
+  if (flops < c->adj_lowp_min_flops || g.nb1 != 1 || g.nb2 != 1 || g.mmap || g.nmap ||
+      g.D || g.rmap || g.cmap || g.K < 1)
+    return false;
+  // C offsets must fit an int
+  const PackIdx rc = pack_levels(g.M1, g.M2, g.sCm0, g.sCm1, g.sCm);
+  const PackIdx cc = pack_levels(g.N1, g.N2, g.sCn0, g.sCn1, g.sCn);
+  auto extent = [](const PackIdx &l, int n) {
+    if (l.n2 >= (1 << 30)) return (int64_t)(n - 1) * l.s2;
+    const int n0 = n / (l.n1 * l.n2);
+    return (int64_t)(n0 - 1) * l.s0 + (int64_t)(l.n1 - 1) * l.s1 +
+           (int64_t)(l.n2 - 1) * l.s2;
+  };
+  if (extent(rc, g.M) + extent(cc, g.N) >= (1LL << 31)) return false;
+  const int64_t ld = split_ld(g.K);
+  const int64_t bf = 2 * ((int64_t)g.M + g.N) * ld;   // bf16 elements
+  const int64_t need = (bf + 3) / 4 + ((int64_t)g.M + g.N + 1) / 2 + 8;
+  if (need > c->lp_doubles) {
+    FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+    dfree(c, c->lp_buf);
+    c->lp_buf = nullptr;
+    c->lp_doubles = 0;
+    c->lp_buf = dalloc(c, need);
+    c->lp_doubles = need;
+  }
+  __bf16 *base = (__bf16 *)c->lp_buf;
+  SplitMat A, B;
+  A.hi = base;
+  A.lo = A.hi + (int64_t)g.M * ld;
+  B.hi = A.lo + (int64_t)g.M * ld;
+  B.lo = B.hi + (int64_t)g.N * ld;
+  A.ld = B.ld = ld;
+  A.bstride = B.bstride = 0;
+  int *rmap = (int *)(c->lp_buf + (bf + 3) / 4);
+  int *cmap = rmap + g.M;
+  const PackIdx kA = pack_levels(g.K1, g.K2, g.sAk0, g.sAk1, g.sAk);
+  const PackIdx kB = pack_levels(g.K1, g.K2, g.sBk0, g.sBk1, g.sBk);
+  const PackIdx rA = pack_levels(g.M1, g.M2, g.sAm0, g.sAm1, g.sAm);
+  const PackIdx rB = pack_levels(g.N1, g.N2, g.sBn0, g.sBn1, g.sBn);
+  const dim3 ga((unsigned)((ld + 31) / 32), (g.M + 31) / 32),
+      gb((unsigned)((ld + 31) / 32), (g.N + 31) / 32);
+  require(ga.y < 65536 && gb.y < 65536, "split pack: grid too large");
+  hipLaunchKernelGGL(split_pack_kernel, ga, dim3(256), 0, c->stream, g.A, rA,
+                     kA, g.M, g.K, (int)ld, A.hi, A.lo, g.sAm == 1 ? 1 : 0);
+  hipLaunchKernelGGL(split_pack_kernel, gb, dim3(256), 0, c->stream, g.B, rB,
+                     kB, g.N, g.K, (int)ld, B.hi, B.lo, g.sBn == 1 ? 1 : 0);
+  hipLaunchKernelGGL(level_offsets_kernel, dim3((g.M + 255) / 256), dim3(256),
+                     0, c->stream, rc, g.M, rmap);
+  hipLaunchKernelGGL(level_offsets_kernel, dim3((g.N + 255) / 256), dim3(256),
+                     0, c->stream, cc, g.N, cmap);
+  FTCC_CHECK_HIP(hipGetLastError());
+  c->cur_M = g.M;
+  c->cur_N = g.N;
+  c->cur_K = g.K;
+  c->cur_nb = 1;
+  const int tok = prof_begin(c, ftcc_ctx::PROF_PRODUCTS, flops, 0.0);
+  sbgemm(g.M, g.N, g.K, 1, g.alpha, A, B, g.beta, g.C, 1, 1, 0, c->stream,
+         rmap, cmap);
+  prof_end(c, tok);
+  return true;
+}
+
 // C[...] = alpha sum_K A[...] B[...] + beta C[...] (beta 0 or 1).  When the
 // orbital labels form one alpha run and one beta run in each list, only the
 // spin blocks allowed by the three operands' rules are multiplied (a
@@ -790,6 +901,7 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
                             c->stream);
       for (const Block &blk : blocks) {
         Gemm g = blk.g;
+        if (lowp_block(c, g)) continue;
         pack_block(c, g);
         run(c, g, FTCC_PREC_FP64, false);
       }
@@ -1205,6 +1317,19 @@ int ftcc_adjoint_apply(ftcc_ctx *c, const double *Lt1, const double *Lt2,
       apply_slice(c, A, t1, t2, *s, Lt1 + y * VO, Lt2 + y * S2, G1 + y * VO,
                   G2 + y * S2);
     }
+  });
+}
+
+int ftcc_adjoint_precision(ftcc_ctx *c, int precision) {
+  return guarded(c, [&] {
+    require(precision == FTCC_PREC_FP64 || precision == FTCC_PREC_BF16X3,
+            "the adjoint's large contractions run in FP64 or bf16x3");
+    c->adj_prec = precision;
+    // blocks of at least this many flops take the split-bf16 product
+    // (FTCC_ADJOINT_LOWP_MIN_GFLOP, default 10)
+    c->adj_lowp_min_flops = 1e10;
+    if (const char *env = getenv("FTCC_ADJOINT_LOWP_MIN_GFLOP"))
+      c->adj_lowp_min_flops = atof(env) * 1e9;
   });
 }
 

@@ -223,6 +223,30 @@ void timed_class(ftcc_ctx *c, int cls, double flops, double bytes, Fn fn) {
   }
 }
 
+// the same for code outside this file: prof_begin returns a token for
+// prof_end (-1: profiling is off)
+int prof_begin(ftcc_ctx *c, int cls, double flops, double bytes) {
+  if (cls < 0 || !c->prof_on) return -1;
+  ftcc_ctx::Pending p;
+  p.e0 = take_event(c);
+  p.e1 = take_event(c);
+  p.cls = cls;
+  p.flops = flops;
+  p.bytes = bytes;
+  p.line = c->site_override ? c->site_override : c->cur_line;
+  p.M = c->cur_M;
+  p.N = c->cur_N;
+  p.K = c->cur_K;
+  p.nb = c->cur_nb;
+  FTCC_CHECK_HIP(hipEventRecord(p.e0, c->stream));
+  c->prof_pending.push_back(p);
+  return (int)c->prof_pending.size() - 1;
+}
+void prof_end(ftcc_ctx *c, int token) {
+  if (token < 0) return;
+  FTCC_CHECK_HIP(hipEventRecord(c->prof_pending[token].e1, c->stream));
+}
+
 // the O(n^6) products (big) are class PROF_PRODUCTS
 template <typename Fn>
 void timed(ftcc_ctx *c, bool big, double flops, Fn fn) {

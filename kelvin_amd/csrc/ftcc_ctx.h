@@ -164,6 +164,12 @@ struct ftcc_ctx {
   // contiguous copies of large strided GEMM operands (adjoint contractions)
   double *pack_buf = nullptr;
   int64_t pack_elems = 0;
+  // precision of the adjoint's large contractions (ftcc_adjoint_precision)
+  // and their split-bf16 operand copies and C offset tables
+  int adj_prec = 0;   // FTCC_PREC_FP64
+  double adj_lowp_min_flops = 1e10;
+  double *lp_buf = nullptr;
+  int64_t lp_doubles = 0;
   // launch plan of the generic GEMMs (ftcc_set_gemm_plan)
   int tile_policy = 0, split_target = 1024;
 
@@ -231,6 +237,10 @@ void build_derived(ftcc_ctx *c);
 bool try_spin(ftcc_ctx *c, const std::vector<int> &lo,
               const std::vector<int> &lv);
 UpdateArgs upd(ftcc_ctx *c);
+// HIP-event bracket of one launch of kernel class cls while profiling is on
+// (token -1 when off)
+int prof_begin(ftcc_ctx *c, int cls, double flops, double bytes);
+void prof_end(ftcc_ctx *c, int token);
 // release the adjoint workspace (adjoint.hip)
 void adjoint_release(ftcc_ctx *c);
 // device address of the allocation-free entry point's scratch array on

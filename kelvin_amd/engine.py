@@ -359,6 +359,11 @@ class FTEngine:
         self._call("ftcc_adjoint_end")
         self._adj_T = None
 
+    def adjoint_precision(self, precision):
+        """Precision of the large blocks of adjoint_apply: "fp64" or
+        "bf16x3"."""
+        self._call("ftcc_adjoint_precision", PREC[precision])
+
     def integrate_adjoint(self, L1, L2, out1, out2):
         """out[x] = sum_y G[y,x] exp(-(t_y - t_x) D) L[y] (x < y; weight 1
         otherwise): the transpose of ``integrate``."""

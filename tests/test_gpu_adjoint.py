@@ -162,6 +162,62 @@ def test_esn_goldens(key, precision):
     assert abs(cc.E0 + cc.E1 + cc.Ecc - E) < 1e-12
 
 
+def test_refined_low_precision_lambda(monkeypatch):
+    """The refined lambda iteration of a bf16x3 solve with every unbatched
+    block of J^T as a split-bf16 product (threshold 0): one application
+    differs from FP64 at the split-bf16 level, and the three phases end at
+    the FP64 fixed point -- E, S, N within 1e-9 of the FP64 lambda solve
+    from the same T."""
+    from kelvin_amd.ccsd import ccsd
+    from kelvin_amd.adjoint import solve_lambda
+    dev()
+    monkeypatch.setenv("FTCC_ADJOINT_LOWP_MIN_GFLOP", "0")
+    g = GOLD["G4"]
+    sysg, s = _ueg_gold("G4")
+    kw = dict(g["ccsd"])
+    kw["tconv"] = 1e-10
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], precision="bf16x3", **kw)
+    cc.run()
+    eng = cc.engine
+    E, S, N = cc.compute_ESN()
+    assert cc.lambda_phases[1] > 0 and cc.lambda_phases[3] > 0
+    assert cc.lambda_converged
+    L1, L2 = cc.L1.clone(), cc.L2.clone()
+    # the same lambda solve in FP64 throughout
+    from kelvin_amd import adjoint as adj
+    orig = adj.solve_lambda
+
+    def fp64_only(*a, **k):
+        k["low"] = None
+        return orig(*a, **k)
+    monkeypatch.setattr(adj, "solve_lambda", fp64_only)
+    E64, S64, N64 = cc.compute_ESN()
+    print("refined lambda: phases %s, dE %.2e dS %.2e dN %.2e"
+          % (cc.lambda_phases, E - E64, S - S64, N - N64))
+    assert abs(E - E64) < 1e-9
+    assert abs(S - S64) < 1e-9
+    assert abs(N - N64) < 1e-9
+    # one application at the converged lambda: low against FP64
+    T1, T2 = cc.T1, cc.T2
+    eng.adjoint_begin(T1, T2)
+    try:
+        Lt1, Lt2 = eng.empty_amplitudes()
+        eng.integrate_adjoint(L1, L2, Lt1, Lt2)
+        out = []
+        for p in ("fp64", "bf16x3"):
+            eng.adjoint_precision(p)
+            G1, G2 = eng.empty_amplitudes()
+            eng.adjoint_apply(Lt1, Lt2, G1, G2)
+            out.append((to_np(G1), to_np(G2)))
+        eng.adjoint_precision("fp64")
+    finally:
+        eng.adjoint_end()
+    d2 = rel(out[1][1], out[0][1])
+    print("one application, bf16x3 against FP64: rel %.2e" % d2)
+    assert 1e-12 < d2 < 1e-4
+    assert rel(out[1][0], out[0][0]) < 1e-4
+
+
 def test_esn_and_lambda_match_oracle_G4():
     """The G4 system: E, S, N and the lambda amplitudes in kelvin's layout
     equal the oracle's adjoint (kelvin's lambda iteration restated), the
