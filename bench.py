@@ -293,6 +293,11 @@ def main():
         one untimed step (its workspace and operand buffers are built on
         first use), then ``steps`` timed ones."""
         eng.set_precision(prec)
+        # a solve in this precision alone plans its own chunk: release the
+        # workspace (a chunk-setting round trip) before the untimed step
+        cur = args.chunk or 0
+        eng.set_chunk(cur + 1)
+        eng.set_chunk(cur)
         T1.copy_(save1)
         T2.copy_(save2)
         eng.step(T1, T2, T1b, T2b, g, beta, args.damp, skip)

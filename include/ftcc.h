@@ -51,8 +51,10 @@ int ftcc_create(ftcc_ctx **ctx, int no, int nv, int ng, int precision,
 int ftcc_destroy(ftcc_ctx *ctx);
 const char *ftcc_last_error(const ftcc_ctx *ctx);
 /* Contraction precision of the O(n^6) products (FTCC_PREC_*).  A change
- * releases the residual workspace (its product operands and chunk size
- * depend on the precision); the next residual rebuilds it. */
+ * keeps the residual workspace: the next residual carves this precision's
+ * product operands from the same arena and runs the largest chunk that the
+ * allocated slots and arena hold (ftcc_set_chunk releases the workspace, so
+ * that the next residual plans it for the current precision). */
 int ftcc_set_precision(ftcc_ctx *ctx, int precision);
 
 /* Fermi-weighted integrals (kelvin/cc_utils.py:569-602 ft_integrals and

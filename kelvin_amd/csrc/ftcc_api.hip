@@ -680,42 +680,91 @@ int64_t split_elems(const ftcc_ctx *c, int id) {
          (OP_FIXED[id] ? 1 : c->nb);
 }
 
-// Split-bf16 operand buffers for the current layout (allocated on first use
-// of bf16x3, reallocated when the spin layout changes).
-void ensure_split_buffers(ftcc_ctx *c) {
-  if (c->split_layout == op_layout(c)) return;
+// doubles of one carved buffer, rounded to 256-byte boundaries
+int64_t carve_len(int64_t doubles) { return (doubles + 31) / 32 * 32; }
+
+// arena doubles of the operand set of precision `prec` at chunk nb (0: the
+// dense FP64/FP32 layout reads its operands in place)
+int64_t operand_doubles(const ftcc_ctx *c, int prec, int nb) {
+  int64_t tot = 0;
   for (int id = 0; id < NOPS; ++id) {
-    SplitMat &m = split_of(c, id);
-    dfree(c, m.hi);
-    dfree(c, m.lo);
-    m = SplitMat();
+    const int64_t rows = space_len(c, op_rows(c, id));
+    const int64_t slots = OP_FIXED[id] ? 1 : nb;
+    if (prec == FTCC_PREC_BF16X3) {
+      const int64_t elems = rows * split_ld(space_len(c, op_k(c, id))) * slots;
+      tot += 2 * carve_len((elems + 3) / 4 + 1);   // four bf16 per double
+    } else if (c->spin) {
+      tot += carve_len(rows * space_len(c, op_k(c, id)) * slots);
+    }
   }
+  return tot;
+}
+
+// The operand arena, at least `need` doubles.  Its first allocation also
+// holds the other precision's set at half the chunk, so that the switches
+// of a mixed-precision solve find room without allocating.
+double *operand_arena(ftcc_ctx *c, int64_t need) {
+  if (need <= c->arena_doubles) return c->arena;
+  if (!c->arena) {
+    const int other = c->prec == FTCC_PREC_BF16X3 ? FTCC_PREC_FP64
+                                                  : FTCC_PREC_BF16X3;
+    need = std::max(need,
+                    operand_doubles(c, other, std::max(1, c->nb / 2)));
+  }
+  FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
+  dfree(c, c->arena);
+  c->arena = nullptr;
+  c->arena_doubles = 0;
+  c->arena_owner = 0;
+  c->arena = dalloc(c, need);
+  c->arena_doubles = need;
+  return c->arena;
+}
+
+// Split-bf16 operand buffers for the current layout and chunk, carved from
+// the arena (the FP64 copies there are given up).
+void ensure_split_buffers(ftcc_ctx *c) {
+  if (c->arena_owner == 1 && c->split_layout == op_layout(c) &&
+      c->split_nb == c->nb)
+    return;
+  double *p = operand_arena(c, operand_doubles(c, FTCC_PREC_BF16X3, c->nb));
+  for (int id = 0; id < NOPS; ++id) c->b64[id] = nullptr;
+  c->b64_layout = -1;
   for (int id = 0; id < NOPS; ++id) {
     SplitMat &m = split_of(c, id);
+    m = SplitMat();
     m.ld = split_ld(space_len(c, op_k(c, id)));
     const int64_t rows = space_len(c, op_rows(c, id));
     m.bstride = OP_FIXED[id] ? 0 : rows * m.ld;
-    const int64_t elems = split_elems(c, id);
-    // dalloc counts doubles: four bf16 per double
-    m.hi = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
-    m.lo = (__bf16 *)dalloc(c, (elems + 3) / 4 + 1);
+    const int64_t len = carve_len((split_elems(c, id) + 3) / 4 + 1);
+    m.hi = (__bf16 *)p;
+    p += len;
+    m.lo = (__bf16 *)p;
+    p += len;
   }
+  c->arena_owner = 1;
   c->split_layout = op_layout(c);
+  c->split_nb = c->nb;
   c->split_fixed_valid = false;
 }
 
-// FP64 spin-blocked operand buffers (FP64 and FP32 modes)
+// FP64 spin-blocked operand buffers (FP64 and FP32 modes), carved from the
+// arena (the split copies there are given up).
 void ensure_b64_buffers(ftcc_ctx *c) {
-  if (c->b64_layout == op_layout(c) && c->b64[0]) return;
+  if (c->arena_owner == 2 && c->b64_layout == op_layout(c) &&
+      c->b64_nb == c->nb)
+    return;
+  double *p = operand_arena(c, operand_doubles(c, FTCC_PREC_FP64, c->nb));
+  for (int id = 0; id < NOPS; ++id) split_of(c, id) = SplitMat();
+  c->split_layout = -1;
   for (int id = 0; id < NOPS; ++id) {
-    dfree(c, c->b64[id]);
-    c->b64[id] = nullptr;
+    c->b64[id] = p;
+    p += carve_len((int64_t)space_len(c, op_rows(c, id)) *
+                   space_len(c, op_k(c, id)) * (OP_FIXED[id] ? 1 : c->nb));
   }
-  for (int id = 0; id < NOPS; ++id)
-    c->b64[id] = dalloc(c, (int64_t)space_len(c, op_rows(c, id)) *
-                               space_len(c, op_k(c, id)) *
-                               (OP_FIXED[id] ? 1 : c->nb));
+  c->arena_owner = 2;
   c->b64_layout = op_layout(c);
+  c->b64_nb = c->nb;
   c->b64_fixed_valid = false;
 }
 
@@ -1179,8 +1228,8 @@ void ensure_integrals(ftcc_ctx *c) {
 // bf16x3, six in FP64, the whole grid at n <= 66) -- larger chunks
 // measured faster at every size tried (fewer, fuller launches; integrals
 // read once per chunk).
-void ensure_workspace(ftcc_ctx *c) {
-  if (c->aring) return;
+// grid points per chunk for the current precision within the budget
+int chunk_for_budget(const ftcc_ctx *c) {
   const int O = c->no, V = c->nv, ng = c->ng;
   const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
   const int64_t S2 = VV * OO;
@@ -1188,8 +1237,7 @@ void ensure_workspace(ftcc_ctx *c) {
   // five n^4 intermediates (seven in the dense layout, which also forms tau
   // and t2r), Wmnij/Wabef/U, and the seven per-slot product operands of the
   // current precision: split into bf16 pairs (bf16x3) or as FP64
-  // spin-blocked copies (FP64/FP32 spin layout).  A precision switch
-  // releases the workspace, so only one of the two ever exists.
+  // spin-blocked copies (FP64/FP32 spin layout), one set at a time
   const int64_t operands =
       c->prec == FTCC_PREC_BF16X3
           ? 4 * 7 *
@@ -1205,7 +1253,31 @@ void ensure_workspace(ftcc_ctx *c) {
   int nb = (int)std::min<int64_t>(ng, cap);
   if (c->nb_fixed > 0)
     nb = (int)std::min<int64_t>(std::min<int64_t>(ng, c->nb_fixed), cap);
+  return nb;
+}
+
+void ensure_workspace(ftcc_ctx *c) {
+  if (c->aring) {
+    if (c->ws_prec == c->prec) return;
+    // a precision switch keeps every buffer: the chunk is what the
+    // allocated slots and the operand arena hold (a mixed-precision solve
+    // runs its FP64 steps at about half the low-precision chunk)
+    int nb = std::min(chunk_for_budget(c), c->nb_alloc);
+    if (c->arena)
+      while (nb > 1 && operand_doubles(c, c->prec, nb) > c->arena_doubles)
+        --nb;
+    c->nb = nb;
+    c->ws_prec = c->prec;
+    return;
+  }
+  const int O = c->no, V = c->nv;
+  const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V, OO = (int64_t)O * O;
+  const int64_t S2 = VV * OO;
+  const int64_t sm = std::max<int64_t>(VV, std::max<int64_t>(VO, OO));
+  const int nb = chunk_for_budget(c);
   c->nb = nb;
+  c->nb_alloc = nb;
+  c->ws_prec = c->prec;
   c->aring = dalloc(c, S2 * nb);
   c->wr = dalloc(c, S2 * nb);
   c->z = dalloc(c, S2 * nb);
@@ -1236,15 +1308,17 @@ void free_workspace(ftcc_ctx *c) {
     *p = nullptr;
   }
   for (int id = 0; id < NOPS; ++id) {
-    SplitMat &m = split_of(c, id);
-    dfree(c, m.hi);
-    dfree(c, m.lo);
-    m = SplitMat();
-    dfree(c, c->b64[id]);
+    split_of(c, id) = SplitMat();
     c->b64[id] = nullptr;
   }
+  dfree(c, c->arena);
+  c->arena = nullptr;
+  c->arena_doubles = 0;
+  c->arena_owner = 0;
   c->split_layout = -1;
   c->b64_layout = -1;
+  c->nb_alloc = 0;
+  c->ws_prec = -1;
 }
 
 // Forget the captured step (its buffers or kernels may change).
@@ -1318,8 +1392,8 @@ void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
     // dense layout (also a spin-blocked system's rare dense fallback):
     // tau and t2r are formed whole
     if (!c->tau) {
-      c->tau = dalloc(c, S2 * nb);
-      c->t2r = dalloc(c, S2 * nb);
+      c->tau = dalloc(c, S2 * c->nb_alloc);
+      c->t2r = dalloc(c, S2 * c->nb_alloc);
     }
     PrepOut po{c->tau, c->t2r, c->aring};
     prep_amplitudes(t1, t2, V, O, nb, po, s);
@@ -2043,9 +2117,8 @@ int ftcc_set_precision(ftcc_ctx *c, int precision) {
     drop_graph(c);   // buffers or kernels of the captured step change
     require(precision >= 0 && precision <= 2, "unknown precision");
     if (precision == c->prec) return;
-    // the product operands and the chunk size depend on the precision:
-    // release the workspace (rebuilt by the next residual)
-    free_workspace(c);
+    // the workspace stays: the next residual carves this precision's
+    // operands from the same arena and picks the chunk that fits
     c->prec = precision;
   });
 }
@@ -2328,21 +2401,13 @@ int ftcc_defect(ftcc_ctx *c, const double *T1, const double *T2,
     struct Back {
       ftcc_ctx *c;
       int p;
-      ~Back() {
-        if (c->prec != p) {
-          free_workspace(c);
-          c->prec = p;
-        }
-      }
+      ~Back() { c->prec = p; }
     } back{c, keep};
     // the residual of ftcc_step (spin-forbidden doubles left unwritten,
     // the step's flip images), so that the correction cancels the step's
     // own low-precision residual at T exactly
     auto with = [&](int prec, double *o1, double *o2) {
-      if (c->prec != prec) {
-        free_workspace(c);
-        c->prec = prec;
-      }
+      c->prec = prec;
       c->tbar_scratch = true;
       try {
         residual_all(c, T1, T2, o1, o2, skip);

@@ -104,6 +104,17 @@ struct ftcc_ctx {
   SplitMat sp_oovvA, sp_ringA;
   bool split_fixed_valid = false;
   int split_layout = -1;        // spin state the split buffers are sized for
+  int split_nb = 0;             // and the chunk
+  // the product operands of either precision (split-bf16 pairs or FP64
+  // blocked copies, one set at a time) are carved from one allocation, so a
+  // precision switch allocates nothing: owner 0 none, 1 split, 2 FP64
+  double *arena = nullptr;
+  int64_t arena_doubles = 0;
+  int arena_owner = 0;
+  // slots of the per-chunk workspace as allocated, and the precision the
+  // current chunk was chosen for (-1: none)
+  int nb_alloc = 0;
+  int ws_prec = -1;
 
   // Spin blocking of the O(n^6) products: when every integral block
   // conserves spin under the orbital labels, the pair indices of the ten
@@ -159,6 +170,7 @@ struct ftcc_ctx {
   PairSpace ps_vvp, ps_oop;
   double *b64[10] = {};            // FP64 blocked operands (fp64/fp32 modes)
   int b64_layout = -1;
+  int b64_nb = 0;
   bool b64_fixed_valid = false;
 
   // contiguous copies of large strided GEMM operands (adjoint contractions)
