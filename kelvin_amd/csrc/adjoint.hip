@@ -751,38 +751,68 @@ __global__ __launch_bounds__(256) void split_pack_kernel(
   }
 }
 
-// out[x] = offset of index x over the levels (int: the tensor has fewer
-// than 2^31 elements)
-__global__ void level_offsets_kernel(PackIdx l, int n, int *out) {
+// out[x] = add + offset of index x over the levels (int: the output spans
+// fewer than 2^31 elements)
+__global__ void level_offsets_kernel(PackIdx l, int n, int64_t add,
+                                     int *out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x < n) out[x] = (int)pack_off(x, l);
+  if (x < n) out[x] = (int)(add + pack_off(x, l));
+}
+
+PackIdx c_rows(const Gemm &g) {
+  return pack_levels(g.M1, g.M2, g.sCm0, g.sCm1, g.sCm);
+}
+PackIdx c_cols(const Gemm &g) {
+  return pack_levels(g.N1, g.N2, g.sCn0, g.sCn1, g.sCn);
+}
+// largest offset of an index of n values over the levels
+int64_t level_extent(const PackIdx &l, int n) {
+  if (l.n2 >= (1 << 30)) return (int64_t)(n - 1) * l.s2;
+  const int n0 = n / (l.n1 * l.n2);
+  return (int64_t)(n0 - 1) * l.s0 + (int64_t)(l.n1 - 1) * l.s1 +
+         (int64_t)(l.n2 - 1) * l.s2;
 }
 
 // In the low-precision mode of the lambda iteration, a large block runs as a
-// split-bf16 product (lo*hi + hi*lo + hi*hi, FP32 accumulation): both
-// operands are split into contiguous K-fast bf16 copies and C is addressed
-// through per-row and per-column offset tables.  Returns false when the
-// block stays FP64 (the mode is off, the block is small, batched, mapped or
-// has an addend).
-bool lowp_block(ftcc_ctx *c, const Gemm &g) {
+// split-bf16 product (lo*hi + hi*lo + hi*hi, FP32 accumulation).  False when
+// the block stays FP64 (the mode is off, the block is small, batched, mapped
+// or has an addend).
+bool lowp_ok(const ftcc_ctx *c, const Gemm &g) {
   if (c->adj_prec != FTCC_PREC_BF16X3) return false;
   const double flops = 2.0 * g.M * (double)g.N * g.K;
-  if (flops < c->adj_lowp_min_flops || g.nb1 != 1 || g.nb2 != 1 || g.mmap || g.nmap ||
-      g.D || g.rmap || g.cmap || g.K < 1)
-    return false;
-  // C offsets must fit an int
-  const PackIdx rc = pack_levels(g.M1, g.M2, g.sCm0, g.sCm1, g.sCm);
-  const PackIdx cc = pack_levels(g.N1, g.N2, g.sCn0, g.sCn1, g.sCn);
-  auto extent = [](const PackIdx &l, int n) {
-    if (l.n2 >= (1 << 30)) return (int64_t)(n - 1) * l.s2;
-    const int n0 = n / (l.n1 * l.n2);
-    return (int64_t)(n0 - 1) * l.s0 + (int64_t)(l.n1 - 1) * l.s1 +
-           (int64_t)(l.n2 - 1) * l.s2;
-  };
-  if (extent(rc, g.M) + extent(cc, g.N) >= (1LL << 31)) return false;
-  const int64_t ld = split_ld(g.K);
-  const int64_t bf = 2 * ((int64_t)g.M + g.N) * ld;   // bf16 elements
-  const int64_t need = (bf + 3) / 4 + ((int64_t)g.M + g.N + 1) / 2 + 8;
+  return flops >= c->adj_lowp_min_flops && g.nb1 == 1 && g.nb2 == 1 &&
+         !g.mmap && !g.nmap && !g.D && !g.rmap && !g.cmap && g.K >= 1;
+}
+
+// doubles of a block's split operands and of its offset tables in the
+// low-precision buffer, each rounded to 256 bytes
+int64_t lowp_split_doubles(const Gemm &g) {
+  const int64_t bf = 2 * ((int64_t)g.M + g.N) * split_ld(g.K);
+  return ((bf + 3) / 4 + 31) / 32 * 32;
+}
+int64_t lowp_table_doubles(const Gemm &g) {
+  return (((int64_t)g.M + g.N + 1) / 2 + 31) / 32 * 32;
+}
+
+// Up to three such blocks of one contraction (equal alpha and beta, disjoint
+// outputs) as one grouped launch, so that a single grid point's 3249^3
+// blocks (169 tiles each) fill the device together: both operands of each
+// are split into contiguous K-fast bf16 copies, and C is addressed from the
+// lowest block's C through per-row and per-column offset tables.  False
+// (nothing launched) when the offsets do not fit an int.
+bool lowp_group(ftcc_ctx *c, const Gemm *const *gs, int n) {
+  const Gemm &g0 = *gs[0];
+  double *base = g0.C;
+  for (int k = 1; k < n; ++k) base = std::min(base, gs[k]->C);
+  int64_t need = 8;
+  for (int k = 0; k < n; ++k) {
+    const Gemm &g = *gs[k];
+    if ((int64_t)(g.C - base) + level_extent(c_rows(g), g.M) +
+            level_extent(c_cols(g), g.N) >=
+        (1LL << 31))
+      return false;
+    need += lowp_split_doubles(g) + lowp_table_doubles(g);
+  }
   if (need > c->lp_doubles) {
     FTCC_CHECK_HIP(hipStreamSynchronize(c->stream));
     dfree(c, c->lp_buf);
@@ -791,39 +821,57 @@ bool lowp_block(ftcc_ctx *c, const Gemm &g) {
     c->lp_buf = dalloc(c, need);
     c->lp_doubles = need;
   }
-  __bf16 *base = (__bf16 *)c->lp_buf;
-  SplitMat A, B;
-  A.hi = base;
-  A.lo = A.hi + (int64_t)g.M * ld;
-  B.hi = A.lo + (int64_t)g.M * ld;
-  B.lo = B.hi + (int64_t)g.N * ld;
-  A.ld = B.ld = ld;
-  A.bstride = B.bstride = 0;
-  int *rmap = (int *)(c->lp_buf + (bf + 3) / 4);
-  int *cmap = rmap + g.M;
-  const PackIdx kA = pack_levels(g.K1, g.K2, g.sAk0, g.sAk1, g.sAk);
-  const PackIdx kB = pack_levels(g.K1, g.K2, g.sBk0, g.sBk1, g.sBk);
-  const PackIdx rA = pack_levels(g.M1, g.M2, g.sAm0, g.sAm1, g.sAm);
-  const PackIdx rB = pack_levels(g.N1, g.N2, g.sBn0, g.sBn1, g.sBn);
-  const dim3 ga((unsigned)((ld + 31) / 32), (g.M + 31) / 32),
-      gb((unsigned)((ld + 31) / 32), (g.N + 31) / 32);
-  require(ga.y < 65536 && gb.y < 65536, "split pack: grid too large");
-  hipLaunchKernelGGL(split_pack_kernel, ga, dim3(256), 0, c->stream, g.A, rA,
-                     kA, g.M, g.K, (int)ld, A.hi, A.lo, g.sAm == 1 ? 1 : 0);
-  hipLaunchKernelGGL(split_pack_kernel, gb, dim3(256), 0, c->stream, g.B, rB,
-                     kB, g.N, g.K, (int)ld, B.hi, B.lo, g.sBn == 1 ? 1 : 0);
-  hipLaunchKernelGGL(level_offsets_kernel, dim3((g.M + 255) / 256), dim3(256),
-                     0, c->stream, rc, g.M, rmap);
-  hipLaunchKernelGGL(level_offsets_kernel, dim3((g.N + 255) / 256), dim3(256),
-                     0, c->stream, cc, g.N, cmap);
-  FTCC_CHECK_HIP(hipGetLastError());
-  c->cur_M = g.M;
-  c->cur_N = g.N;
-  c->cur_K = g.K;
-  c->cur_nb = 1;
+  double *p = c->lp_buf;
+  SbProblem prob[3];
+  double flops = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const Gemm &g = *gs[k];
+    const int64_t ld = split_ld(g.K);
+    SplitMat A, B;
+    A.ld = B.ld = ld;
+    A.bstride = B.bstride = 0;
+    A.hi = (__bf16 *)p;
+    A.lo = A.hi + (int64_t)g.M * ld;
+    B.hi = A.lo + (int64_t)g.M * ld;
+    B.lo = B.hi + (int64_t)g.N * ld;
+    p += lowp_split_doubles(g);
+    int *rmap = (int *)p, *cmap = rmap + g.M;
+    p += lowp_table_doubles(g);
+    const PackIdx kA = pack_levels(g.K1, g.K2, g.sAk0, g.sAk1, g.sAk);
+    const PackIdx kB = pack_levels(g.K1, g.K2, g.sBk0, g.sBk1, g.sBk);
+    const PackIdx rA = pack_levels(g.M1, g.M2, g.sAm0, g.sAm1, g.sAm);
+    const PackIdx rB = pack_levels(g.N1, g.N2, g.sBn0, g.sBn1, g.sBn);
+    const dim3 ga((unsigned)((ld + 31) / 32), (g.M + 31) / 32),
+        gb((unsigned)((ld + 31) / 32), (g.N + 31) / 32);
+    require(ga.y < 65536 && gb.y < 65536, "split pack: grid too large");
+    hipLaunchKernelGGL(split_pack_kernel, ga, dim3(256), 0, c->stream, g.A,
+                       rA, kA, g.M, g.K, (int)ld, A.hi, A.lo,
+                       g.sAm == 1 ? 1 : 0);
+    hipLaunchKernelGGL(split_pack_kernel, gb, dim3(256), 0, c->stream, g.B,
+                       rB, kB, g.N, g.K, (int)ld, B.hi, B.lo,
+                       g.sBn == 1 ? 1 : 0);
+    hipLaunchKernelGGL(level_offsets_kernel, dim3((g.M + 255) / 256),
+                       dim3(256), 0, c->stream, c_rows(g), g.M,
+                       (int64_t)(g.C - base), rmap);
+    hipLaunchKernelGGL(level_offsets_kernel, dim3((g.N + 255) / 256),
+                       dim3(256), 0, c->stream, c_cols(g), g.N, (int64_t)0,
+                       cmap);
+    FTCC_CHECK_HIP(hipGetLastError());
+    prob[k].M = g.M;
+    prob[k].N = g.N;
+    prob[k].K = g.K;
+    prob[k].A = A;
+    prob[k].B = B;
+    prob[k].rmap = rmap;
+    prob[k].cmap = cmap;
+    flops += 2.0 * g.M * (double)g.N * g.K;
+  }
+  c->cur_M = g0.M;
+  c->cur_N = g0.N;
+  c->cur_K = g0.K;
+  c->cur_nb = n;
   const int tok = prof_begin(c, ftcc_ctx::PROF_PRODUCTS, flops, 0.0);
-  sbgemm(g.M, g.N, g.K, 1, g.alpha, A, B, g.beta, g.C, 1, 1, 0, c->stream,
-         rmap, cmap);
+  sbgemm_grouped(n, prob, 1, g0.alpha, g0.beta, base, 1, 1, 0, c->stream);
   prof_end(c, tok);
   return true;
 }
@@ -862,6 +910,7 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
   if (c->spin && c->spin5 && letters.size() <= 10) {
     struct Block {
       Gemm g;
+      int ckey;   // spins of C's letters: blocks of equal key share outputs
     };
     std::vector<Block> blocks;
     bool ok = true;
@@ -894,17 +943,43 @@ void con(ftcc_ctx *c, double alpha, const Ten &A0, const Ten &B0, double beta,
       cc.p += off[2];
       Block blk;
       if (!plan(ext, alpha, a, b, 1.0, cc, G, blk.g)) ok = false;
+      blk.ckey = 0;
+      for (int k = 0; k < L; ++k)
+        if (C.stride(letters[k]) >= 0 && sp[(unsigned char)letters[k]])
+          blk.ckey |= 1 << k;
       blocks.push_back(blk);
     }
     if (ok) {
       if (beta == 0.0) fill(const_cast<double *>(C.p), elems(C, full), 0.0,
                             c->stream);
+      // split-bf16 blocks go out in groups of up to three with disjoint
+      // outputs (a group's launch must not add twice into one element)
+      std::vector<const Block *> group;
+      auto flush = [&] {
+        if (group.empty()) return;
+        const Gemm *gs[3];
+        for (size_t k = 0; k < group.size(); ++k) gs[k] = &group[k]->g;
+        if (!lowp_group(c, gs, (int)group.size()))
+          for (const Block *b : group) {   // offsets too large: FP64
+            Gemm g = b->g;
+            pack_block(c, g);
+            run(c, g, FTCC_PREC_FP64, false);
+          }
+        group.clear();
+      };
       for (const Block &blk : blocks) {
+        if (lowp_ok(c, blk.g)) {
+          bool clash = group.size() == 3;
+          for (const Block *b : group) clash = clash || b->ckey == blk.ckey;
+          if (clash) flush();
+          group.push_back(&blk);
+          continue;
+        }
         Gemm g = blk.g;
-        if (lowp_block(c, g)) continue;
         pack_block(c, g);
         run(c, g, FTCC_PREC_FP64, false);
       }
+      flush();
       return;
     }
   }
