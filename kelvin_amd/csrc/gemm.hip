@@ -190,7 +190,7 @@ template <typename T, bool A_MFAST, bool B_NFAST, int TM, int TN, bool KL>
 __device__ inline void load_slab(const GemmDev &p, const double *A,
                                  const double *B, const int64_t *aoff,
                                  const int64_t *boff, int k0, int kend,
-                                 T ra[TM / 16], T rb[TN / 16]) {
+                                 double ra[TM / 16], double rb[TN / 16]) {
   static_assert(!KL || (!A_MFAST && !B_NFAST), "K levels need K-fast loads");
   const int t = threadIdx.x;
   int64_t ka = 0, kb = 0;
@@ -212,7 +212,7 @@ __device__ inline void load_slab(const GemmDev &p, const double *A,
       const int k = A_MFAST ? k0 + t / TM + (NT / TM) * r : k0 + (t & 15);
       if (off >= 0 && k < kend) v = A[off + (int64_t)k * p.sAk];
     }
-    ra[r] = (T)v;
+    ra[r] = v;
   }
 #pragma unroll
   for (int r = 0; r < TN / 16; ++r) {
@@ -224,28 +224,30 @@ __device__ inline void load_slab(const GemmDev &p, const double *A,
       const int k = B_NFAST ? k0 + t / TN + (NT / TN) * r : k0 + (t & 15);
       if (off >= 0 && k < kend) v = B[(int64_t)k * p.sBk + off];
     }
-    rb[r] = (T)v;
+    rb[r] = v;
   }
 }
 
 template <typename T, bool A_MFAST, bool B_NFAST, int TM, int TN>
-__device__ inline void store_slab(T *As, T *Bs, const T ra[TM / 16],
-                                  const T rb[TN / 16]) {
+// (the prefetched values stay FP64 until here, so that the FP32 form's
+// rounding does not wait for the loads right where they are issued)
+__device__ inline void store_slab(T *As, T *Bs, const double ra[TM / 16],
+                                  const double rb[TN / 16]) {
   constexpr int LDK = Mma<T>::LDK;
   const int t = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < TM / 16; ++r) {
     if (A_MFAST)
-      As[(t % TM) * LDK + t / TM + (NT / TM) * r] = ra[r];
+      As[(t % TM) * LDK + t / TM + (NT / TM) * r] = (T)ra[r];
     else
-      As[((t >> 4) + 16 * r) * LDK + (t & 15)] = ra[r];
+      As[((t >> 4) + 16 * r) * LDK + (t & 15)] = (T)ra[r];
   }
 #pragma unroll
   for (int r = 0; r < TN / 16; ++r) {
     if (B_NFAST)
-      Bs[(t % TN) * LDK + t / TN + (NT / TN) * r] = rb[r];
+      Bs[(t % TN) * LDK + t / TN + (NT / TN) * r] = (T)rb[r];
     else
-      Bs[((t >> 4) + 16 * r) * LDK + (t & 15)] = rb[r];
+      Bs[((t >> 4) + 16 * r) * LDK + (t & 15)] = (T)rb[r];
   }
 }
 
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
 #pragma unroll
     for (int j = 0; j < CT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  T ra[TM / 16], rb[TN / 16];
+  double ra[TM / 16], rb[TN / 16];
   tile_offsets<TM, TN>(p, m0, n0, aoff, boff);
   __syncthreads();
   if (nslab > 0) {
