@@ -275,6 +275,28 @@ __device__ inline int src_index(const int *map, int i, int n) {
   return map ? map[i] : (i < n ? i : -1);
 }
 
+// The source columns of columns c0, c0 + step, ... (NQ of them): -1 past W
+// or where the map drops them.  The map test is hoisted (it is uniform) and
+// the map reads are unconditional, so they are in flight together.
+template <int NQ>
+__device__ inline void map_cols(const int *map, int n, int W, int c0,
+                                int step, int out[NQ]) {
+  if (map) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int c = c0 + q * step;
+      const int m = map[c < W ? c : 0];
+      out[q] = c < W ? m : -1;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int c = c0 + q * step;
+      out[q] = (c < W && c < n) ? c : -1;
+    }
+  }
+}
+
 // Sources of the gather kernels: value of the dense matrix at (slice b,
 // row pair rr, column pair cc).  Plain: a strided FP64 tensor.
 struct PlainSrc {
@@ -312,10 +334,12 @@ struct FlipRingSrc {
 //   AMP_TAU_IJ: rows (i,j), columns (a,b): the same tau
 //   AMP_T2R:    rows (a,i), columns (e,m): t2[a,e,i,m]
 //   AMP_ARING:  rows (b,j), columns (f,n): -1/2 t2[f,b,n,j] + t1[f,j] t1[b,n]
-// (pair indices x * ny + y of the pair maps).
+// (pair indices x * ny + y of the pair maps), one kind per instantiation (no
+// branch on the kind inside the gathers' loads).
+template <int KIND>
 struct AmpSrc {
   const double *t1, *t2;
-  int nv, no, kind;
+  int nv, no;
   __device__ double operator()(int64_t b, int rr, int cc) const {
     const int64_t VO = (int64_t)nv * no;
     const double *x1 = t1 + b * VO, *x2 = t2 + b * VO * VO;
@@ -323,22 +347,17 @@ struct AmpSrc {
       return x2[(((int64_t)a * nv + e) * no + i) * no + m];
     };
     auto T1 = [&](int a, int i) { return x1[(int64_t)a * no + i]; };
-    switch (kind) {
-      case AMP_TAU_AB:
-      case AMP_TAU_IJ: {
-        const int ab = kind == AMP_TAU_AB ? rr : cc;
-        const int ij = kind == AMP_TAU_AB ? cc : rr;
-        const int a = ab / nv, bb = ab % nv, i = ij / no, j = ij % no;
-        return T2(a, bb, i, j) + T1(a, i) * T1(bb, j) - T1(bb, i) * T1(a, j);
-      }
-      case AMP_T2R: {
-        const int a = rr / no, i = rr % no, e = cc / no, m = cc % no;
-        return T2(a, e, i, m);
-      }
-      default: {   // AMP_ARING
-        const int bb = rr / no, j = rr % no, f = cc / no, n = cc % no;
-        return -0.5 * T2(f, bb, n, j) + T1(f, j) * T1(bb, n);
-      }
+    if (KIND == AMP_TAU_AB || KIND == AMP_TAU_IJ) {
+      const int ab = KIND == AMP_TAU_AB ? rr : cc;
+      const int ij = KIND == AMP_TAU_AB ? cc : rr;
+      const int a = ab / nv, bb = ab % nv, i = ij / no, j = ij % no;
+      return T2(a, bb, i, j) + T1(a, i) * T1(bb, j) - T1(bb, i) * T1(a, j);
+    } else if (KIND == AMP_T2R) {
+      const int a = rr / no, i = rr % no, e = cc / no, m = cc % no;
+      return T2(a, e, i, m);
+    } else {   // AMP_ARING
+      const int bb = rr / no, j = rr % no, f = cc / no, n = cc % no;
+      return -0.5 * T2(f, bb, n, j) + T1(f, j) * T1(bb, n);
     }
   }
 };
@@ -360,11 +379,14 @@ __global__ void gather_rows_kernel(Src src, const int *rmap, int R,
     const int r = (int)(e / W4);
     const int c = 4 * (int)(e % W4);
     const int rr = src_index(rmap, r, R);
+    int cc[4];
+    map_cols<4>(cmap, C, W, c, 1, cc);
     double v[4];
+    // branch-free loads (out of range reads element (0, 0), then zero)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int cc = src_index(cmap, c + k, C);
-      v[k] = (rr >= 0 && cc >= 0) ? src(b, rr, cc) : 0.0;
+      const double x = src(b, rr < 0 ? 0 : rr, cc[k] < 0 ? 0 : cc[k]);
+      v[k] = (rr >= 0 && cc[k] >= 0) ? x : 0.0;
     }
     out.store4(b * db + (int64_t)r * os + c, v);
   }
@@ -383,12 +405,22 @@ __global__ void gather_transpose_kernel(Src src, const int *rmap, int R,
   const int b = blockIdx.z;
   const int r0 = blockIdx.y * GT, c0 = blockIdx.x * GT;
   const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;   // 256 threads
-  for (int i = ty; i < GT; i += 256 / GT) {
-    const int r = r0 + tx, c = c0 + i;
-    const int rr = r < R ? src_index(rmap, r, R) : -1;
-    const int cc = c < W ? src_index(cmap, c, C) : -1;
-    tile[i][tx] = (rr >= 0 && cc >= 0) ? src(b, rr, cc) : 0.0;
+  // a fixed trip count with branch-free loads (an out-of-range element
+  // reads element (0, 0) and is replaced by zero): every row's map and
+  // source loads are in flight together instead of one round trip each
+  constexpr int NQ = GT / (256 / GT);
+  const int r = r0 + tx;
+  const int rr = r < R ? src_index(rmap, r, R) : -1;
+  int cc[NQ];
+  map_cols<NQ>(cmap, C, W, c0 + ty, 256 / GT, cc);
+  double v[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const double x = src(b, rr < 0 ? 0 : rr, cc[q] < 0 ? 0 : cc[q]);
+    v[q] = (rr >= 0 && cc[q] >= 0) ? x : 0.0;
   }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) tile[ty + q * (256 / GT)][tx] = v[q];
   __syncthreads();
   const int cq = threadIdx.x % (GT / 4), rq = threadIdx.x / (GT / 4);
   for (int i = rq; i < GT; i += 1024 / GT) {
@@ -431,11 +463,30 @@ PlainSrc plain(const double *src, int64_t sr, int64_t sc, int64_t sb,
   along_rows = (sr == 1 && sc != 1);
   return PlainSrc{src, sr, sc, sb};
 }
-// tau by (i,j) rows and aring by (b,j) rows are contiguous along rows
-AmpSrc amp(int kind, const double *t1, const double *t2, int nv, int no,
-           bool &along_rows) {
-  along_rows = (kind == AMP_TAU_IJ || kind == AMP_ARING);
-  return AmpSrc{t1, t2, nv, no, kind};
+// gather_launch over the amplitude source of `kind` (tau by (i,j) rows and
+// aring by (b,j) rows are contiguous along rows)
+template <typename Out>
+void amp_launch(int kind, const double *t1, const double *t2, int nv, int no,
+                const int *rmap, int R, const int *cmap, int W, int64_t os,
+                int64_t db, Out out, int nb, hipStream_t s) {
+  const bool ar = (kind == AMP_TAU_IJ || kind == AMP_ARING);
+  switch (kind) {
+    case AMP_TAU_AB:
+      gather_launch(AmpSrc<AMP_TAU_AB>{t1, t2, nv, no}, ar, rmap, R, cmap, W,
+                    W, os, db, out, nb, s);
+      break;
+    case AMP_TAU_IJ:
+      gather_launch(AmpSrc<AMP_TAU_IJ>{t1, t2, nv, no}, ar, rmap, R, cmap, W,
+                    W, os, db, out, nb, s);
+      break;
+    case AMP_T2R:
+      gather_launch(AmpSrc<AMP_T2R>{t1, t2, nv, no}, ar, rmap, R, cmap, W, W,
+                    os, db, out, nb, s);
+      break;
+    default:
+      gather_launch(AmpSrc<AMP_ARING>{t1, t2, nv, no}, ar, rmap, R, cmap, W,
+                    W, os, db, out, nb, s);
+  }
 }
 
 }  // namespace
@@ -467,10 +518,8 @@ void amp_split_gather(int kind, const double *t1, const double *t2, int nv,
                       const SplitMat &d, int nb, hipStream_t s) {
   require(d.ld % 32 == 0 && W <= d.ld, "split operand leading dimension");
   require(rmap && cmap, "amplitude operands are gathered through pair maps");
-  bool ar;
-  const AmpSrc src_ = amp(kind, t1, t2, nv, no, ar);
-  gather_launch(src_, ar, rmap, R, cmap, W, W, d.ld, d.bstride,
-                SplitOut{d.hi, d.lo}, nb, s);
+  amp_launch(kind, t1, t2, nv, no, rmap, R, cmap, W, d.ld, d.bstride,
+             SplitOut{d.hi, d.lo}, nb, s);
 }
 
 void amp_gather64(int kind, const double *t1, const double *t2, int nv,
@@ -478,9 +527,8 @@ void amp_gather64(int kind, const double *t1, const double *t2, int nv,
                   int64_t os, double *dst, int64_t db, int nb,
                   hipStream_t s) {
   require(rmap && cmap, "amplitude operands are gathered through pair maps");
-  bool ar;
-  const AmpSrc src_ = amp(kind, t1, t2, nv, no, ar);
-  gather_launch(src_, ar, rmap, R, cmap, W, W, os, db, F64Out{dst}, nb, s);
+  amp_launch(kind, t1, t2, nv, no, rmap, R, cmap, W, os, db, F64Out{dst}, nb,
+             s);
 }
 
 void flip_ring_gather(const double *src, int nv, int no, int hv, int ho,
