@@ -369,33 +369,53 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(GemmDev p) {
   // are issued before any store (a store may alias them as far as the
   // compiler knows, which would otherwise serialise every element on a
   // round trip to memory).
+  // The reads of D and C are unconditional (a dropped element reads
+  // element 0 and is not stored) and every value is formed before the first
+  // store: with reads or stores under divergent branches the compiler can
+  // no longer count which memory operations are outstanding and waits for
+  // all of them before each store, serialising the stores one round trip
+  // at a time.
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) {
     if (direct) {
-      double dv[CT][4], cv[CT][4];
+      int64_t oc[CT][4];
+      double res[CT][4];
 #pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int lc = wn * (TN / 2) + ct * 16 + li;
+      for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
+          const int lc = wn * (TN / 2) + ct * 16 + li;
           const int lr = wm * (TM / 2) + rt * 16 + Mma<T>::crow(lg, v);
           const bool ok = crow[lr] >= 0 && ccol[lc] >= 0;
-          dv[ct][v] = (D && ok) ? D[drow[lr] + dcol[lc]] : 0.0;
-          cv[ct][v] = (p.beta != 0.0 && ok) ? C[crow[lr] + ccol[lc]] : 0.0;
+          oc[ct][v] = ok ? crow[lr] + ccol[lc] : -1;
+          res[ct][v] = p.alpha * (double)acc[rt][ct][v];
         }
+      if (D) {
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int lc = wn * (TN / 2) + ct * 16 + li;
+            const int lr = wm * (TM / 2) + rt * 16 + Mma<T>::crow(lg, v);
+            const double dv =
+                D[oc[ct][v] >= 0 ? drow[lr] + dcol[lc] : (int64_t)0];
+            res[ct][v] += p.gamma * dv;
+          }
+      }
+      if (p.beta != 0.0) {
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const double cv = C[oc[ct][v] >= 0 ? oc[ct][v] : (int64_t)0];
+            res[ct][v] += p.beta * cv;
+          }
       }
 #pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int lc = wn * (TN / 2) + ct * 16 + li;
+      for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int lr = wm * (TM / 2) + rt * 16 + Mma<T>::crow(lg, v);
-          const int64_t oc = crow[lr], occ = ccol[lc];
-          if (oc >= 0 && occ >= 0)
-            C[oc + occ] = p.alpha * (double)acc[rt][ct][v] +
-                          p.gamma * dv[ct][v] + p.beta * cv[ct][v];
-        }
-      }
+        for (int v = 0; v < 4; ++v)
+          if (oc[ct][v] >= 0) C[oc[ct][v]] = res[ct][v];
     } else {
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
