@@ -155,9 +155,22 @@ __device__ inline void mfma_step(f16_t (&acc)[4][2], const Frag &f) {
 
 __global__ __launch_bounds__(512, 1) void sbgemm_kernel(SbDev p) {
   extern __shared__ char lds[];   // [2 stages][A | B][256 rows][128 B]
+  // the tile's output rows and columns after the scatter maps (-1: past
+  // the end or dropped), read once here, alongside the first slab's loads
+  __shared__ int emap[BM + BN];
   int bt, tm, tn;
   const SbProb pr = tile_of(p, bt, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
+  {
+    const int t = threadIdx.x;   // 512 threads: 256 rows, then 256 columns
+    if (t < BM) {
+      const int row = m0 + t;
+      emap[t] = row < pr.M ? (pr.rmap ? pr.rmap[row] : row) : -1;
+    } else {
+      const int col = n0 + t - BM;
+      emap[t] = col < pr.N ? (pr.cmap ? pr.cmap[col] : col) : -1;
+    }
+  }
   const __bf16 *Ahi = pr.Ahi + bt * pr.sAb, *Alo = pr.Alo + bt * pr.sAb;
   const __bf16 *Bhi = pr.Bhi + bt * pr.sBb, *Blo = pr.Blo + bt * pr.sBb;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -204,27 +217,36 @@ __global__ __launch_bounds__(512, 1) void sbgemm_kernel(SbDev p) {
   // no load may still be writing LDS when the workgroup ends
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 
-  // lane holds column li, rows 8*(v/4) + 4*lg + v%4 of each 32x32 block
+  // lane holds column li, rows 8*(v/4) + 4*lg + v%4 of each 32x32 block.
+  // Per block, the reads of C (beta != 0) are unconditional (a dropped
+  // element reads element 0) and every value is formed before the first
+  // store: reads or stores under divergent branches make the compiler wait
+  // for all outstanding memory operations before each store.
   double *C = p.C + bt * p.sCb;
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
     for (int ct = 0; ct < 2; ++ct) {
-      const int col = n0 + wn * 64 + ct * 32 + li;
-      if (col >= pr.N) continue;
-      const int cc = pr.cmap ? pr.cmap[col] : col;
-      if (cc < 0) continue;
+      const int cc = emap[BM + wn * 64 + ct * 32 + li];
+      int64_t off[16];
+      double res[16];
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int row = m0 + wm * 128 + rt * 32 + 8 * (v >> 2) + 4 * lg + (v & 3);
-        if (row < pr.M) {
-          const int rr = pr.rmap ? pr.rmap[row] : row;
-          if (rr < 0) continue;
-          double *c = C + (int64_t)rr * p.sCm + (int64_t)cc * p.sCn;
-          const double x = (double)acc[rt][ct][v];
-          *c = (p.beta == 0.0) ? p.alpha * x : p.alpha * x + p.beta * *c;
-        }
+        const int rr =
+            emap[wm * 128 + rt * 32 + 8 * (v >> 2) + 4 * lg + (v & 3)];
+        off[v] = (rr >= 0 && cc >= 0)
+                     ? (int64_t)rr * p.sCm + (int64_t)cc * p.sCn
+                     : (int64_t)-1;
+        res[v] = p.alpha * (double)acc[rt][ct][v];
       }
+      if (p.beta != 0.0) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+          res[v] += p.beta * C[off[v] >= 0 ? off[v] : (int64_t)0];
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v)
+        if (off[v] >= 0) C[off[v]] = res[v];
     }
 }
 
