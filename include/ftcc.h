@@ -310,6 +310,17 @@ int ftcc_adjoint_cached(ftcc_ctx *ctx, int *nslices);
 int ftcc_adjoint_apply(ftcc_ctx *ctx, const double *Lt1_dev,
                        const double *Lt2_dev, double *G1_dev, double *G2_dev);
 int ftcc_adjoint_end(ftcc_ctx *ctx);
+/* The unrelaxed one-particle response densities of the converged lambda
+ * amplitudes (kelvin/ft_cc_equations.py:704-722 ccsd_1rdm, as used by
+ * kelvin/ccsd.py:1333-1387 _g_ft_1rdm): from T1/T2 and the integrated
+ * adjoint amplitudes Lt1/Lt2 = ftcc_integrate_adjoint(Lam) (engine layout),
+ * the host weights g and beta, writes pia [no][nv], pba [nv][nv],
+ * pji [no][no] and pai [nv][no] (device), each the derivative of kelvin's
+ * Lagrangian with respect to one Fock block times beta. */
+int ftcc_density1(ftcc_ctx *ctx, const double *T1_dev, const double *T2_dev,
+                  const double *Lt1_dev, const double *Lt2_dev,
+                  const double *g_host, double beta, double *pia_dev,
+                  double *pba_dev, double *pji_dev, double *pai_dev);
 /* Contraction precision of the large blocks of ftcc_adjoint_apply:
  * FTCC_PREC_FP64 (default) or FTCC_PREC_BF16X3 (the low-precision phase of
  * a refined lambda iteration).  Unbatched blocks of at least

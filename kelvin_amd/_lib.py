@@ -63,6 +63,8 @@ SIGNATURES = {
     "ftcc_adjoint_apply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ftcc_adjoint_end": (ctypes.c_int, [_vp]),
     "ftcc_adjoint_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ftcc_density1": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _c_double_p,
+                                     ctypes.c_double, _vp, _vp, _vp, _vp]),
     "ftcc_integrate_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ftcc_lambda_update": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp,
                                           _c_double_p, ctypes.c_double,

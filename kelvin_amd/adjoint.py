@@ -172,6 +172,46 @@ def _response_fock(cc, eng, eri, fock_dev, en, fo, fv, dx):
     return dfock, float(numpy.dot(ron1 / cc.beta, dx))
 
 
+def ft_1rdm(cc):
+    """The unrelaxed one-particle density of a converged ``ccsd``
+    (kelvin/ccsd.py:1333-1387 _g_ft_1rdm): solves the lambda equations if
+    they have not been (kelvin calls _ft_ccsd_lambda there), forms the
+    response densities on the device (``FTEngine.density1``: kelvin's
+    ft_cc_equations.ccsd_1rdm) and sets dia, dba, dji, dai, their
+    normal-ordered ndia, ndba, ndji, ndai (scaled by the square roots of the
+    occupation numbers) and n1rdm = (ndia + ndba + ndji + ndai) / beta, as
+    numpy arrays."""
+    if cc.beta_max != cc.beta:
+        raise Exception("the 1-RDM needs beta_max == beta")
+    if cc.athresh > 0.0:
+        raise NotImplementedError("the 1-RDM of an active space (athresh > 0) "
+                                  "is not built")
+    if cc.L2 is None:
+        cc.compute_ESN()
+    eng = cc.engine
+    if eng is None:
+        raise Exception("the 1-RDM needs the converged solve's engine")
+    beta, mu, g = cc.beta, cc.mu, cc.g
+    eng.set_grid(cc.ti, cc.G)
+    Lam1, Lam2 = eng.lambda_from_kelvin(cc.L1, cc.L2, g, beta)
+    Lt1, Lt2 = eng.empty_amplitudes()
+    eng.integrate_adjoint(Lam1, Lam2, Lt1, Lt2)
+    del Lam1, Lam2
+    pia, pba, pji, pai = (x.cpu().numpy() for x in
+                          eng.density1(cc.T1, cc.T2, Lt1, Lt2, g, beta))
+    del Lt1, Lt2
+    en = numpy.asarray(cc.sys.g_energies_tot(), dtype=numpy.float64)
+    sfo = numpy.sqrt(ft_utils.ff(beta, en, mu))
+    sfv = numpy.sqrt(ft_utils.ffv(beta, en, mu))
+    cc.dia, cc.dba, cc.dji, cc.dai = pia, pba, pji, pai
+    cc.ndia = pia * sfo[:, None] * sfv[None, :]
+    cc.ndba = pba * sfv[:, None] * sfv[None, :]
+    cc.ndji = pji * sfo[:, None] * sfo[None, :]
+    cc.ndai = pai * sfv[:, None] * sfo[None, :]
+    cc.n1rdm = (cc.ndia + cc.ndba + cc.ndji + cc.ndai) / beta
+    return cc.n1rdm
+
+
 def compute_esn(cc, L1=None, L2=None, gderiv=True):
     """E, S and N of a converged ``ccsd`` (generalised path of
     kelvin/ccsd.py:128-214); sets the attributes the reference sets (E, S,

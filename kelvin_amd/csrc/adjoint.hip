@@ -1335,6 +1335,40 @@ void adjoint_release(ftcc_ctx *c) {
 
 using namespace ftcc_core;
 
+namespace {
+
+// pia[i][a] = s_vo[a][i]; pba[b][a] = s_vv[a][b]; pji[j][i] = s_oo[i][j];
+// pai[a][i] = s_ov[i][a] + t1sum[a][i]
+__global__ void density1_out_kernel(const double *s_vo, const double *s_vv,
+                                    const double *s_oo, const double *s_ov,
+                                    const double *t1sum, int V, int O,
+                                    double *pia, double *pba, double *pji,
+                                    double *pai) {
+  const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V,
+                OO = (int64_t)O * O;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+       k < 2 * VO + VV + OO; k += (int64_t)gridDim.x * blockDim.x) {
+    if (k < VO) {                       // pia[i][a]
+      const int i = (int)(k / V), a = (int)(k % V);
+      pia[k] = s_vo[(int64_t)a * O + i];
+    } else if (k < VO + VV) {           // pba[b][a]
+      const int64_t q = k - VO;
+      const int b = (int)(q / V), a = (int)(q % V);
+      pba[q] = s_vv[(int64_t)a * V + b];
+    } else if (k < VO + VV + OO) {      // pji[j][i]
+      const int64_t q = k - VO - VV;
+      const int j = (int)(q / O), i = (int)(q % O);
+      pji[q] = s_oo[(int64_t)i * O + j];
+    } else {                            // pai[a][i]
+      const int64_t q = k - VO - VV - OO;
+      const int a = (int)(q / O), i = (int)(q % O);
+      pai[q] = s_ov[(int64_t)i * V + a] + t1sum[q];
+    }
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 int ftcc_adjoint_begin(ftcc_ctx *c, const double *T1, const double *T2) {
@@ -1431,6 +1465,99 @@ int ftcc_integrate_adjoint(ftcc_ctx *c, const double *L1, const double *L2,
                        dim3(AT), lds, c->stream, L2, out2, ng, V, O, g.d_ti,
                        g.d_G, c->ev, c->eo, c->have_D ? c->D2 : nullptr,
                        g.uniform ? 1 : 0, g.delta);
+    FTCC_CHECK_HIP(hipGetLastError());
+  });
+}
+
+// The unrelaxed one-particle response densities of kelvin's
+// ft_cc_equations.ccsd_1rdm (kelvin/ft_cc_equations.py:704-722): with
+// l = int_L(L) (the reverse-integrated lambda amplitudes) at every grid
+// point x and weights g_x,
+//   pia = sum_x g_x l1,
+//   pba[b,a] = sum_x g_x (sum_i l1[i,a] t1[b,i]
+//                         + 1/2 sum_cij l2[i,j,c,a] t2[c,b,i,j]),
+//   pji[j,i] = sum_x g_x (-sum_a l1[j,a] t1[a,i]
+//                         - 1/2 sum_kab l2[k,j,a,b] t2[a,b,k,i]),
+//   pai[a,i] = sum_x g_x (t1[a,i] - sum_jb l1[j,b] t1[b,i] t1[a,j]
+//                + sum_jb l1[j,b] t2[b,a,j,i]
+//                - 1/2 sum l2[j,k,b,c] t2[b,a,j,k] t1[c,i]
+//                - 1/2 sum l2[j,k,b,c] t2[b,c,j,i] t1[a,k]),
+// the derivatives of the Lagrangian E - (1/beta) sum_x g_x <L_x, T_new,x>
+// with respect to the Fock blocks (the definition kelvin's tests check,
+// kelvin/tests/test_ft_ccsd_rdm.py:12-64).  The engine's integrated adjoint
+// amplitudes u = integrate_adjoint(Lam) carry the weights: g_x l1_x =
+// -beta u1_x and g_x l2_x = -4 beta u2_x (transposed), so the terms are
+// accumulated from u with those factors.  T and u in the engine layout
+// ([ng][V][O], [ng][V][V][O][O]); outputs in kelvin's (pia [O][V], pba
+// [V][V], pji [O][O], pai [V][O]), device memory.
+int ftcc_density1(ftcc_ctx *c, const double *T1, const double *T2,
+                  const double *Lt1, const double *Lt2, const double *g_host,
+                  double beta, double *pia, double *pba, double *pji,
+                  double *pai) {
+  return guarded(c, [&] {
+    require(c->have_ints, "integrals not set", FTCC_ESTATE);
+    require(T1 && T2 && Lt1 && Lt2 && g_host && pia && pba && pji && pai,
+            "null pointer");
+    require(beta > 0.0, "beta must be positive");
+    const int O = c->no, V = c->nv, ng = c->ng;
+    const int64_t VO = (int64_t)V * O, VV = (int64_t)V * V,
+                  OO = (int64_t)O * O, S2 = VV * OO;
+    hipStream_t s = c->stream;
+    // accumulators and per-point intermediates
+    const int64_t tot = VO + VV + OO + VO + VO + OO + VV + OO;
+    double *buf = dalloc(c, tot);
+    struct Free {
+      ftcc_ctx *c;
+      double *p;
+      ~Free() {
+        (void)hipStreamSynchronize(c->stream);
+        try {
+          dfree(c, p);
+        } catch (...) {
+        }
+      }
+    } fr{c, buf};
+    double *s_vo = buf, *s_vv = s_vo + VO, *s_oo = s_vv + VV,
+           *s_ov = s_oo + OO, *t1sum = s_ov + VO, *x1 = t1sum + VO,
+           *y2 = x1 + OO, *z2 = y2 + VV;
+    fill(buf, VO + VV + OO + VO + VO, 0.0, s);
+    for (int x = 0; x < ng; ++x) {
+      const double *t1 = T1 + x * VO, *t2 = T2 + x * S2;
+      const double *u1 = Lt1 + x * VO, *u2 = Lt2 + x * S2;
+      axpby(s_vo, u1, VO, -beta, 1.0, s);          // pia
+      axpby(t1sum, t1, VO, g_host[x], 1.0, s);     // energy term of pai
+      // y2[b,e] = sum u2[a,b,i,j] t2[a,e,i,j];  z2[m,j] = sum u2[a,b,i,j]
+      // t2[a,b,i,m];  x1[i,m] = sum_a u1[a,i] t1[a,m]
+      con(c, 1.0, ten(c, u2, "abij"), ten(c, t2, "aeij"), 0.0,
+          ten(c, y2, "be"));
+      con(c, 1.0, ten(c, u2, "abij"), ten(c, t2, "abim"), 0.0,
+          ten(c, z2, "mj"));
+      con(c, 1.0, ten(c, u1, "ai"), ten(c, t1, "am"), 0.0, ten(c, x1, "im"));
+      // pba: s_vv[a,e] += -beta sum_i u1[a,i] t1[e,i] - 2 beta y2[a,e]
+      con(c, -beta, ten(c, u1, "ai"), ten(c, t1, "ei"), 1.0,
+          ten(c, s_vv, "ae"));
+      axpby(s_vv, y2, VV, -2.0 * beta, 1.0, s);
+      // pji: s_oo[m,j] += beta sum_a u1[a,j] t1[a,m] + 2 beta z2[m,j]
+      con(c, beta, ten(c, u1, "aj"), ten(c, t1, "am"), 1.0,
+          ten(c, s_oo, "mj"));
+      axpby(s_oo, z2, OO, 2.0 * beta, 1.0, s);
+      // pai: s_ov[m,e] += beta sum_i t1[e,i] x1[i,m]
+      //   - beta sum u1[a,i] t2[a,e,i,m] + 2 beta sum_b y2[b,e] t1[b,m]
+      //   + 2 beta sum_j z2[m,j] t1[e,j]
+      con(c, beta, ten(c, t1, "ei"), ten(c, x1, "im"), 1.0,
+          ten(c, s_ov, "me"));
+      con(c, -beta, ten(c, u1, "ai"), ten(c, t2, "aeim"), 1.0,
+          ten(c, s_ov, "me"));
+      con(c, 2.0 * beta, ten(c, y2, "be"), ten(c, t1, "bm"), 1.0,
+          ten(c, s_ov, "me"));
+      con(c, 2.0 * beta, ten(c, z2, "mj"), ten(c, t1, "ej"), 1.0,
+          ten(c, s_ov, "me"));
+    }
+    const int64_t n = 2 * VO + VV + OO;
+    hipLaunchKernelGGL(density1_out_kernel,
+                       dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)),
+                       dim3(256), 0, s, s_vo, s_vv, s_oo, s_ov, t1sum, V, O,
+                       pia, pba, pji, pai);
     FTCC_CHECK_HIP(hipGetLastError());
   });
 }

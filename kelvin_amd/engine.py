@@ -371,6 +371,23 @@ class FTEngine:
         self._call("ftcc_integrate_adjoint", _vp(L1), _vp(L2), _vp(out1),
                    _vp(out2))
 
+    def density1(self, T1, T2, Lt1, Lt2, g, beta):
+        """The unrelaxed one-particle response densities (pia [no,nv],
+        pba [nv,nv], pji [no,no], pai [nv,no], kelvin's ccsd_1rdm) from the
+        amplitudes and the integrated adjoint amplitudes Lt =
+        integrate_adjoint(Lam); device tensors."""
+        self._check_amps(T1, T2, Lt1, Lt2)
+        no, nv = self.no, self.nv
+        dv = dict(dtype=torch.float64, device=self.device)
+        pia = torch.empty((no, nv), **dv)
+        pba = torch.empty((nv, nv), **dv)
+        pji = torch.empty((no, no), **dv)
+        pai = torch.empty((nv, no), **dv)
+        g_a, g_p = _dbuf(g)
+        self._call("ftcc_density1", _vp(T1), _vp(T2), _vp(Lt1), _vp(Lt2),
+                   g_p, float(beta), _vp(pia), _vp(pba), _vp(pji), _vp(pai))
+        return pia, pba, pji, pai
+
     def lambda_update(self, L1, L2, G1, G2, T1, g, beta, damp):
         """L <- damp L + (1 - damp) (dE/dT - G); returns the squared norms
         (|dL1|^2, |dL2|^2, |L1|^2, |L2|^2) in kelvin's lambda scale."""

@@ -24,6 +24,10 @@ How each piece is pinned (tests/test_oracle_*.py):
   * ``stanton_rhs``: brute-force Fock-space projection
     <Phi_mu| e^{-T} H e^{T} |Phi_0> on random non-Hermitian Hamiltonians
     (oracle/fock_space.py), then end to end.
+  * ``ccsd_1rdm``: every element against the derivative of
+    ``ft_lagrangian`` with respect to its Fock element, evaluated exactly by
+    unit perturbations (the Lagrangian is linear in the Fock blocks) -- the
+    identity kelvin/tests/test_ft_ccsd_rdm.py:12-170 checks as a trace.
   * end to end: the reference's golden UEG grand potentials
     (kelvin/tests/test_ft_ccsd.py:23 with :119-133, Omega_cc=-0.010750238811
     at 1e-9) and the per-iteration energies printed in
@@ -753,6 +757,57 @@ def ft_ccsd_esn(sys, T, mu, ngrid=10, damp=0.0, max_iter=40, econv=1e-8,
     return dict(E=E, S=S, N=N, E0=E0, E1=E1, Ecc=Ecc, N0=N0, N1=N1,
                 Ncc=Ncc, Omega=Om, Omega_cc=r["Omega_cc"], L1=L1, L2=L2,
                 lambda_converged=lconv, lambda_iter=lit, T1=T1, T2=T2)
+
+
+# ---------------------------------------------------------------------------
+# One-particle response densities: kelvin/ft_cc_equations.py:704-722
+# (ccsd_1rdm).  The per-point densities come from cqcpy's ccsd_1rdm_*_opt
+# (absent); they are restated here as the derivatives of kelvin's
+# Lagrangian with respect to the Fock blocks, which is what the reference's
+# own tests check them against (kelvin/tests/test_ft_ccsd_rdm.py:12-170:
+# (1/beta) sum p F = the Lagrangian with only that Fock block non-zero).
+# ---------------------------------------------------------------------------
+def ft_lagrangian(F, I, T1, T2, L1, L2, D1, D2, ti, ng, g, G, beta):
+    """kelvin/tests/test_ft_ccsd_rdm.py:12-22 compute_ref (with the Stanton
+    form of the residual in place of ccsd_simple, the same map)."""
+    T1t, T2t = ccsd_stanton(F, I, T1, T2, D1, D2, ti, ng, G)
+    E = ft_cc_energy(T1, T2, F.ov, I.oovv, g, beta)
+    A1 = einsum('via,vai->v', L1, T1t) / beta
+    A2 = 0.25 * einsum('vijab,vabij->v', L2, T2t) / beta
+    return E - numpy.dot(A1, g) - numpy.dot(A2, g)
+
+
+def ccsd_1rdm(T1, T2, L1, L2, D1, D2, ti, ng, g, G):
+    """(pia, pba, pji, pai) of kelvin/ft_cc_equations.py:704-722: with the
+    reverse-integrated l = int_L(L), pia = sum_x g_x l1 and the rest the
+    derivatives of the per-point Lagrangian with respect to F.vv, F.oo and
+    F.ov (the last with the energy's t1 term)."""
+    l1 = int_L1(ng, L1, ti, D1, g, G)
+    l2 = int_L2(ng, L2, ti, D2, g, G)
+    pia = einsum('x,xia->ia', g, l1)
+    dvv = (einsum('xia,xei->xae', l1, T1)
+           + 0.5 * einsum('xijca,xceij->xae', l2, T2))
+    pba = einsum('x,xab->ba', g, dvv)
+    doo = (-einsum('xja,xam->xmj', l1, T1)
+           - 0.5 * einsum('xkjab,xabkm->xmj', l2, T2))
+    pji = einsum('x,xij->ji', g, doo)
+    dov = (einsum('xem->xme', T1)
+           - einsum('xia,xam,xei->xme', l1, T1, T1)
+           + einsum('xia,xaeim->xme', l1, T2)
+           - 0.5 * einsum('xijab,xaeij,xbm->xme', l2, T2, T1)
+           - 0.5 * einsum('xijab,xabim,xej->xme', l2, T2, T1))
+    pai = einsum('x,xia->ai', g, dov)
+    return pia, pba, pji, pai
+
+
+def normal_1rdm(pia, pba, pji, pai, fo, fv, beta):
+    """kelvin/ccsd.py:1373-1387 (no active space): the occupation-scaled
+    blocks over beta, and kelvin/ccsd.py:1995-2006: plus diag(fo)."""
+    sfo, sfv = numpy.sqrt(fo), numpy.sqrt(fv)
+    n1 = (pia * sfo[:, None] * sfv[None, :] + pba * sfv[:, None] * sfv[None, :]
+          + pji * sfo[:, None] * sfo[None, :]
+          + pai * sfv[:, None] * sfo[None, :]) / beta
+    return n1 + numpy.diag(fo)
 
 
 # ---------------------------------------------------------------------------

@@ -357,3 +357,86 @@ def test_transposed_residual_spin_blocks_equal_dense():
         eng.close()
     assert rel(out[0][0], out[1][0]) < 1e-12
     assert rel(out[0][1], out[1][1]) < 1e-12
+
+
+@pytest.mark.parametrize("spin", [False, True])
+def test_density1_matches_oracle(spin):
+    """ftcc_density1 (kelvin's ft_cc_equations.ccsd_1rdm on the device)
+    against the oracle's restatement on random amplitudes and lambda
+    amplitudes, with orbital-energy denominators on a Simpson grid; dense,
+    and spin-blocked on a spin-conserving system (amplitudes projected onto
+    the allowed blocks)."""
+    from kelvin_amd import quadrature
+    from kelvin_amd.engine import FTEngine
+    d = dev()
+    rng = numpy.random.default_rng(43)
+    ng, n, beta = 7, 6, 3.0
+    ti, g, G = quadrature.ft_quad(ng, beta, 'lin')
+    en = numpy.sort(rng.uniform(-1, 1, n))
+    eng = FTEngine(n, n, ng)
+    eri = fs.random_antisym(n, rng, hermitian=True) * 0.1
+    lab = numpy.arange(n) % 2
+    if spin:
+        # zero every spin-forbidden integral so that blocking applies
+        ok = ((lab[:, None, None, None] + lab[None, :, None, None])
+              == (lab[None, None, :, None] + lab[None, None, None, :]))
+        eri = eri * ok
+    eng.set_system(en, numpy.diag(en), eri, beta, 0.0)
+    if spin:
+        eng.set_spin(lab, lab)
+    eng.set_grid(ti, G)
+    t1 = rng.normal(size=(ng, n, n)) * 0.3
+    t2 = antisym(rng.normal(size=(ng, n, n, n, n))) * 0.2
+    l1 = rng.normal(size=(ng, n, n)) * 0.3
+    l2 = antisym(rng.normal(size=(ng, n, n, n, n))) * 0.2
+    if spin:
+        m1 = lab[:, None] == lab[None, :]
+        m2 = ((lab[:, None, None, None] + lab[None, :, None, None])
+              == (lab[None, None, :, None] + lab[None, None, None, :]))
+        t1, l1 = t1 * m1, l1 * m1
+        t2, l2 = t2 * m2, l2 * m2
+    D1, D2 = oracle.D1(en, en), oracle.D2(en, en)
+    ref = oracle.ccsd_1rdm(t1, t2, l1, l2, D1, D2, ti, ng, g, G)
+    T1, T2 = (torch.as_tensor(x, device=d) for x in (t1, t2))
+    Lam1, Lam2 = eng.lambda_from_kelvin(l1, l2, g, beta)
+    Lt1, Lt2 = eng.empty_amplitudes()
+    eng.integrate_adjoint(Lam1, Lam2, Lt1, Lt2)
+    got = eng.density1(T1, T2, Lt1, Lt2, g, beta)
+    for name, a, b in zip(("pia", "pba", "pji", "pai"), got, ref):
+        print("density1 %s %s: rel %.2e" % (spin, name, rel(to_np(a), b)))
+        assert rel(to_np(a), b) < 1e-12
+    eng.close()
+
+
+def test_full_1rdm_matches_oracle_G4():
+    """ccsd.full_1rdm on the G4 system: the unrelaxed density of the GPU
+    lambda solve plus the Fermi occupations, against the oracle's (its own
+    converged T and L, kelvin/ccsd.py:1333-1387 and 1961-2006)."""
+    from kelvin_amd.ccsd import ccsd
+    dev()
+    g = GOLD["G4"]
+    sysg, s = _ueg_gold("G4")
+    cc = ccsd(sysg, T=s["T"], mu=s["mu"], **g["ccsd"])
+    cc.run()
+    rdm = cc.full_1rdm()
+    osys = oracle.UEGSystem(s["T"], 2 * numpy.pi, s["Emax"], s["mu"],
+                            norb=s["norb"])
+    r = oracle.ft_ccsd_esn(osys, s["T"], s["mu"], **g["ccsd"])
+    beta = 1.0 / s["T"]
+    en = osys.g_energies_tot()
+    ng = g["ccsd"]["ngrid"]
+    ti, gw, G = oracle.ft_quad(ng, beta, 'lin')
+    D1, D2 = oracle.D1(en, en), oracle.D2(en, en)
+    blocks = oracle.ccsd_1rdm(r["T1"], r["T2"], r["L1"], r["L2"], D1, D2, ti,
+                              ng, gw, G)
+    fo, fv = oracle.ff(beta, en, s["mu"]), oracle.ffv(beta, en, s["mu"])
+    ref = oracle.normal_1rdm(*blocks, fo, fv, beta)
+    # the correlation part alone (the occupations on the diagonal are the
+    # same numbers on both sides)
+    cor, cref = rdm - numpy.diag(fo), ref - numpy.diag(fo)
+    print("full_1rdm G4: n=%d, |correlation part| %.3e, rel %.2e, blocks "
+          "%s, trace %.10f" % (len(en), numpy.abs(cref).max(), rel(cor, cref),
+                               ["%.2e" % numpy.abs(b).max() for b in blocks],
+                               numpy.trace(rdm)))
+    assert numpy.abs(cref).max() > 1e-6
+    assert rel(cor, cref) < 1e-7

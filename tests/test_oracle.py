@@ -558,3 +558,49 @@ def test_esn_matches_finite_differences(kind):
     assert abs(r["N"] - N) < 1e-7
     assert abs(r["S"] - S) < 1e-7
     assert abs(r["E"] - E) < 1e-7
+
+
+def test_ccsd_1rdm_is_the_lagrangian_derivative():
+    """The oracle's FT 1-RDM blocks (kelvin/ft_cc_equations.py:704-722)
+    against the derivative of kelvin's Lagrangian with respect to each Fock
+    element (kelvin/tests/test_ft_ccsd_rdm.py:12-170 checks the trace of
+    the same identity), evaluated exactly by unit perturbations; random
+    amplitudes, lambdas and denominators, ng = 5."""
+    rng = numpy.random.default_rng(3)
+    n, ng, beta = 4, 5, 2.0
+
+    def antisym(x):
+        x = x - numpy.swapaxes(x, -4, -3)
+        return x - numpy.swapaxes(x, -2, -1)
+    T1 = rng.normal(size=(ng, n, n)) * 0.3
+    T2 = antisym(rng.normal(size=(ng, n, n, n, n)) * 0.2)
+    L1 = rng.normal(size=(ng, n, n)) * 0.3
+    L2 = antisym(rng.normal(size=(ng, n, n, n, n)) * 0.2)
+    D1 = rng.normal(size=(n, n))
+    D2 = rng.normal(size=(n, n, n, n))
+    D2 = 0.5 * (D2 + D2.transpose(1, 0, 3, 2))
+    ti, g, G = ftcc.ft_quad(ng, beta, 'lin')
+    Z = numpy.zeros
+    keys = ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo', 'oovv', 'vooo', 'ooov',
+            'oooo')
+    I = ftcc.Blocks({k: Z((n, n, n, n)) for k in keys})
+
+    def brute(block):
+        P = Z((n, n))
+        for p in range(n):
+            for q in range(n):
+                F = ftcc.Blocks(oo=Z((n, n)), ov=Z((n, n)), vo=Z((n, n)),
+                                vv=Z((n, n)))
+                F[block][p, q] = 1.0
+                P[q, p] = beta * ftcc.ft_lagrangian(F, I, T1, T2, L1, L2, D1,
+                                                    D2, ti, ng, g, G, beta)
+        return P
+    got = ftcc.ccsd_1rdm(T1, T2, L1, L2, D1, D2, ti, ng, g, G)
+    for block, p in zip(('vo', 'vv', 'oo', 'ov'), got):
+        ref = brute(block)
+        assert numpy.abs(p - ref).max() < 1e-12 * max(1.0, numpy.abs(ref).max())
+    # a planted mistake (the l2 term of pba with the wrong sign) is caught
+    bad = got[1] - 2 * 0.5 * numpy.einsum(
+        'x,xab->ba', g, numpy.einsum('xijca,xceij->xae',
+                                     ftcc.int_L2(ng, L2, ti, D2, g, G), T2))
+    assert numpy.abs(bad - brute('vv')).max() > 1e-3
