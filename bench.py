@@ -287,6 +287,8 @@ def main():
     eng.profile(False)
     launches, gemm_ms, gemm_flops, _ = classes["products"]
     value = args.steps / t_mixed
+    # the chunk of the timed steps (the other forms below plan their own)
+    chunk = eng.chunk()
 
     def other_form(prec, steps):
         """The same step in another precision from the same amplitudes:
@@ -336,7 +338,7 @@ def main():
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
         "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
         "traffic": measured_traffic(args.precision, n, ng,
-                                    "%s-c%d" % (layout, eng.chunk())),
+                                    "%s-c%d" % (layout, chunk)),
         "kernel": "%s (the five O(n^6) products of the residual)"
                   % ("sbgemm_kernel" if args.precision == "bf16x3"
                      else "gemm_mfma_kernel<%s>" % ("double" if args.precision == "fp64" else "float")),
@@ -396,7 +398,7 @@ def main():
                                   args.nelec, ng),
                    "n": n, "ng": ng, "precision": args.precision,
                    "damp": args.damp, "spin_blocked": spin_blocked,
-                   "layout": layout, "chunk": eng.chunk()},
+                   "layout": layout, "chunk": chunk},
         "fp64": fp64,
         "fp32": fp32,
         "omega_step_delta_vs_fp64": d_omega,
