@@ -143,21 +143,27 @@ class ccsd(object):
         return compute_esn(self, L1=L1, L2=L2, gderiv=gderiv)
 
     def full_1rdm(self, relax=False):
-        """The one-particle density matrix (kelvin/ccsd.py:1961-2006, 'g'
-        orbitals): the unrelaxed response density n1rdm
-        (``kelvin_amd.adjoint.ft_1rdm``, solving the lambda equations first
-        if needed) plus the Fermi occupations on the diagonal.  The relaxed
-        form (relax=True: orbital-occupation relaxation, kelvin's
-        _grel_ft_1rdm) and the unrestricted layout are not built."""
+        """The one-particle density matrix (kelvin/ccsd.py:1961-2006): the
+        unrelaxed response density n1rdm (``kelvin_amd.adjoint.ft_1rdm``,
+        solving the lambda equations first if needed) plus the Fermi
+        occupations on the diagonal; for 'u' systems the [alpha, beta]
+        blocks of the same spin-orbital density (alpha orbitals first, the
+        order the solve uses).  The relaxed form (relax=True: the
+        orbital-occupation relaxation of kelvin's _grel_ft_1rdm) is not
+        built."""
         if relax:
             raise NotImplementedError("the relaxed 1-RDM is not built")
-        if self.sys.orbtype != 'g':
-            raise NotImplementedError(
-                "the 1-RDM is built for generalised ('g') orbitals")
+        if self.sys.orbtype not in ('g', 'u'):
+            raise NotImplementedError("the 1-RDM is built for 'g' and 'u' "
+                                      "orbitals")
         from .adjoint import ft_1rdm
         n1 = ft_1rdm(self)
         en = self.sys.g_energies_tot()
-        return n1 + numpy.diag(ft_utils.ff(self.beta, en, self.mu))
+        rdm = n1 + numpy.diag(ft_utils.ff(self.beta, en, self.mu))
+        if self.sys.orbtype == 'g':
+            return rdm
+        na = len(self.sys.u_energies_tot()[0])
+        return [rdm[:na, :na].copy(), rdm[na:, na:].copy()]
 
     def compute_ESN_fd(self, delta=1e-4, richardson=False, fd_econv=1e-12):
         """E, S and N by central differences of full GPU solves.
