@@ -216,6 +216,8 @@ __global__ __launch_bounds__(512, 1) void sbgemm_kernel(SbDev p) {
   }
   // no load may still be writing LDS when the workgroup ends
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  // an empty K passed no barrier since the scatter maps went to LDS
+  if (nslab == 0) __syncthreads();
 
   // lane holds column li, rows 8*(v/4) + 4*lg + v%4 of each 32x32 block.
   // Per block, the reads of C (beta != 0) are unconditional (a dropped
