@@ -1,10 +1,10 @@
 """Integral-provider protocol (kelvin/system.py:1-167).
 
-The finite-temperature CCSD driver uses, for a generalised-orbital system:
+The finite-temperature CCSD solver uses, for a generalised-orbital system:
 ``verify(T, mu)``, ``has_u()``, ``const_energy()``, ``get_mp1()``,
 ``g_energies_tot()`` (n orbital energies), ``g_fock_tot()`` (n x n Fock
 matrix with its diagonal) and ``g_aint_tot()`` (antisymmetrised
-<pq||rs>, n^4).  Arrays may be numpy arrays or torch tensors; the driver
+<pq||rs>, n^4).  Arrays may be numpy arrays or torch tensors; the solver
 moves them to the GPU once per solve.
 """
 
