@@ -180,7 +180,10 @@ def ft_1rdm(cc):
     ft_cc_equations.ccsd_1rdm) and sets dia, dba, dji, dai, their
     normal-ordered ndia, ndba, ndji, ndai (scaled by the square roots of the
     occupation numbers) and n1rdm = (ndia + ndba + ndji + ndai) / beta, as
-    numpy arrays."""
+    numpy arrays.  For 'u' orbitals (solved in their spin-orbital form,
+    alpha orbitals first) each of these is the pair of its alpha and beta
+    blocks, as kelvin/ccsd.py:1579-1665 _u_ft_1rdm sets them (the blocks
+    that mix the spins vanish: the density conserves spin)."""
     if cc.beta_max != cc.beta:
         raise Exception("the 1-RDM needs beta_max == beta")
     if cc.athresh > 0.0:
@@ -203,12 +206,21 @@ def ft_1rdm(cc):
     en = numpy.asarray(cc.sys.g_energies_tot(), dtype=numpy.float64)
     sfo = numpy.sqrt(ft_utils.ff(beta, en, mu))
     sfv = numpy.sqrt(ft_utils.ffv(beta, en, mu))
-    cc.dia, cc.dba, cc.dji, cc.dai = pia, pba, pji, pai
-    cc.ndia = pia * sfo[:, None] * sfv[None, :]
-    cc.ndba = pba * sfv[:, None] * sfv[None, :]
-    cc.ndji = pji * sfo[:, None] * sfo[None, :]
-    cc.ndai = pai * sfv[:, None] * sfo[None, :]
-    cc.n1rdm = (cc.ndia + cc.ndba + cc.ndji + cc.ndai) / beta
+    d = (pia, pba, pji, pai)
+    nd = (pia * sfo[:, None] * sfv[None, :], pba * sfv[:, None] * sfv[None, :],
+          pji * sfo[:, None] * sfo[None, :], pai * sfv[:, None] * sfo[None, :])
+    n1 = sum(nd) / beta
+    if cc.sys.orbtype == 'u':
+        na = len(cc.sys.u_energies_tot()[0])
+
+        def spins(x):
+            return (x[:na, :na].copy(), x[na:, na:].copy())
+        d = tuple(spins(x) for x in d)
+        nd = tuple(spins(x) for x in nd)
+        n1 = list(spins(n1))
+    cc.dia, cc.dba, cc.dji, cc.dai = d
+    cc.ndia, cc.ndba, cc.ndji, cc.ndai = nd
+    cc.n1rdm = n1
     return cc.n1rdm
 
 

@@ -98,6 +98,11 @@ class ccsd(object):
         self.T2 = None
         self.L1 = None
         self.L2 = None
+        # pieces of the unrelaxed 1-RDM and the 1-RDM itself
+        # (kelvin/ccsd.py:84-103)
+        self.dia = self.dba = self.dji = self.dai = None
+        self.ndia = self.ndba = self.ndji = self.ndai = None
+        self.n1rdm = None
         self.engine = None
         self.history = []
 
@@ -145,25 +150,28 @@ class ccsd(object):
     def full_1rdm(self, relax=False):
         """The one-particle density matrix (kelvin/ccsd.py:1961-2006): the
         unrelaxed response density n1rdm (``kelvin_amd.adjoint.ft_1rdm``,
-        solving the lambda equations first if needed) plus the Fermi
-        occupations on the diagonal; for 'u' systems the [alpha, beta]
-        blocks of the same spin-orbital density (alpha orbitals first, the
-        order the solve uses).  The relaxed form (relax=True: the
-        orbital-occupation relaxation of kelvin's _grel_ft_1rdm) is not
-        built."""
+        formed on the first call, solving the lambda equations first if
+        needed) plus the Fermi occupations on the diagonal; for 'u' systems
+        the [alpha, beta] pair.  The relaxed form (relax=True: the
+        occupation and orbital-energy relaxation of kelvin's _grel_ft_1rdm
+        and _urel_ft_1rdm) is not built."""
+        orbtype = self.sys.orbtype
+        if orbtype not in ('g', 'u'):
+            raise Exception("orbital type " + orbtype +
+                            " is not implemented for 1rdm")
         if relax:
             raise NotImplementedError("the relaxed 1-RDM is not built")
-        if self.sys.orbtype not in ('g', 'u'):
-            raise NotImplementedError("the 1-RDM is built for 'g' and 'u' "
-                                      "orbitals")
-        from .adjoint import ft_1rdm
-        n1 = ft_1rdm(self)
+        if self.n1rdm is None:
+            from .adjoint import ft_1rdm
+            ft_1rdm(self)
+        if orbtype == 'u':
+            ea, eb = self.sys.u_energies_tot()
+            return [self.n1rdm[0] + numpy.diag(ft_utils.ff(self.beta, ea,
+                                                           self.mu)),
+                    self.n1rdm[1] + numpy.diag(ft_utils.ff(self.beta, eb,
+                                                           self.mu))]
         en = self.sys.g_energies_tot()
-        rdm = n1 + numpy.diag(ft_utils.ff(self.beta, en, self.mu))
-        if self.sys.orbtype == 'g':
-            return rdm
-        na = len(self.sys.u_energies_tot()[0])
-        return [rdm[:na, :na].copy(), rdm[na:, na:].copy()]
+        return self.n1rdm + numpy.diag(ft_utils.ff(self.beta, en, self.mu))
 
     def compute_ESN_fd(self, delta=1e-4, richardson=False, fd_econv=1e-12):
         """E, S and N by central differences of full GPU solves.
