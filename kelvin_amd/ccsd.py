@@ -46,6 +46,20 @@ from .engine import FTEngine
 from .ov_blocks import EngineDenominators, engine_eri, engine_fock
 
 
+def system_at(sys, T, mu):
+    """The system at another (T, mu): a shallow copy whose temperature and
+    chemical potential are replaced (the systems of this package derive
+    their (T, mu)-dependent quantities from these attributes)."""
+    s = copy.copy(sys)
+    s.T = T
+    s.mu = mu
+    if hasattr(s, "beta"):
+        s.beta = 1.0 / T
+    if not s.verify(T, mu):
+        raise Exception("system cannot be moved to T=%r, mu=%r" % (T, mu))
+    return s
+
+
 class ccsd(object):
     """Coupled cluster singles and doubles (CCSD) driver at finite T."""
 
@@ -250,16 +264,7 @@ class ccsd(object):
         return self.E, self.S, self.N
 
     def _system_at(self, T, mu):
-        """The system at another (T, mu): a shallow copy whose temperature
-        and chemical potential are replaced."""
-        s = copy.copy(self.sys)
-        s.T = T
-        s.mu = mu
-        if hasattr(s, "beta"):
-            s.beta = 1.0 / T
-        if not s.verify(T, mu):
-            raise Exception("system cannot be moved to T=%r, mu=%r" % (T, mu))
-        return s
+        return system_at(self.sys, T, mu)
 
     def _warm_start(self, other):
         """Converged amplitudes as the starting point of a neighbouring

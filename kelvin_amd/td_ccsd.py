@@ -11,7 +11,9 @@ Runge-Kutta stage; ``ftcc_td_implicit_step``: the damped inner iteration of
 the implicit schemes, one fused update-and-norm kernel per pass); the
 propagators are kelvin/propagation.py's explicit rk1, rk2, rk4 (:5-52) and
 implicit be, cn, am2 (:55-128, with the ``max_iter``, ``damp`` and
-``thresh`` entries of ``prop``).
+``thresh`` entries of ``prop``).  ``compute_ESN()`` gives E, S and N
+(kelvin/td_ccsd.py:160-235) as derivatives of the propagated grand
+potential.
 
 It is an independent check of the fixed-point solver: both converge to the
 same Omega_cc as the grid is refined (kelvin/tests/test_td_ccsd.py).
@@ -59,6 +61,8 @@ class TDCCSD(object):
         self.T1 = [None] * ngrid
         self.T2 = [None] * ngrid
         self.G0 = self.G1 = self.Gcc = self.Gtot = None
+        self.E = self.S = self.N = None
+        self.engine = None
 
     def run(self, response=None):
         """Run the propagation; returns (Omega_total, Omega_cc)."""
@@ -71,6 +75,66 @@ class TDCCSD(object):
         logging.info('Running CCSD at an electronic temperature of %f K'
                      % ft_utils.HtoK(self.T))
         return self._ccsd(tprop)
+
+    def compute_ESN(self, delta=1e-4):
+        """E, S and N with kelvin's attributes (E0/E1/Ecc, S0/S1/Scc,
+        N0/N1/Ncc and their sums; kelvin/td_ccsd.py:160-235): the zero
+        order parts in closed form (kelvin/td_ccsd.py:200-205), the first
+        order and CC parts as N = -dOmega/dmu and S = -dOmega/dT by central
+        differences of propagations at (T, mu +- delta) and (T +- delta, mu)
+        on grids scaled with beta, and E = Omega + T S + mu N for every
+        part.  A propagation has no convergence test, so the differences
+        carry only the O(delta^2) truncation: the result is the derivative
+        of the propagated grand potential itself.  Kelvin obtains the same
+        derivatives by propagating the lambda amplitudes backwards and
+        contracting response densities, with an approximate grid
+        derivative; the two agree with the fixed-point solver's E, S and N
+        to the tolerance of kelvin/tests/test_td_ccsd_ESN.py:42-75.  The
+        system must derive its (T, mu)-dependent quantities from its ``T``
+        and ``mu`` attributes (true of the systems in this package)."""
+        if not self.finite_T:
+            raise NotImplementedError(
+                "zero-temperature TD-CCSD is not part of the GPU path")
+        from .ccsd import system_at
+        if self.Gtot is None:
+            self.run()
+        T, mu, beta = self.T, self.mu, self.beta
+        en = numpy.asarray(self.sys.g_energies_tot(), dtype=numpy.float64)
+        fo = ft_utils.ff(beta, en, mu)
+        N0 = fo.sum()
+        E0 = beta * ft_utils.dGP0(beta, en, mu).sum() + mu * N0 + self.G0
+
+        def parts(Ts, mus):
+            """(Omega_1, Omega_cc) of the propagation at (Ts, mus)."""
+            td = TDCCSD(system_at(self.sys, Ts, mus), self.prop, T=Ts,
+                        mu=mus, iprint=self.iprint, singles=self.singles,
+                        ngrid=self.ngrid, athresh=self.athresh,
+                        fthresh=self.fthresh, quad=self.quad,
+                        precision=self.precision)
+            td.run()
+            return td.G1, td.Gcc
+
+        # each neighbouring propagation holds an engine of its own
+        self.engine = None
+        e1p, ccp = parts(T, mu + delta)
+        e1m, ccm = parts(T, mu - delta)
+        e1f, ccf = parts(T + delta, mu)
+        e1b, ccb = parts(T - delta, mu)
+        N1 = -(e1p - e1m) / (2 * delta)
+        Ncc = -(ccp - ccm) / (2 * delta)
+        S1 = -(e1f - e1b) / (2 * delta)
+        Scc = -(ccf - ccb) / (2 * delta)
+        self.N0, self.N1, self.Ncc = N0, N1, Ncc
+        self.N = N0 + N1 + Ncc
+        self.E0 = E0
+        self.E1 = self.G1 + T * S1 + mu * N1
+        self.Ecc = self.Gcc + T * Scc + mu * Ncc
+        self.E = self.E0 + self.E1 + self.Ecc
+        self.S = -beta * (self.Gtot - self.E + mu * self.N)
+        self.S0 = -beta * (self.G0 - self.E0 + mu * self.N0)
+        self.S1 = -beta * (self.G1 - self.E1 + mu * self.N1)
+        self.Scc = self.S - self.S0 - self.S1
+        return self.E, self.S, self.N
 
     def _active(self, en):
         """kelvin/td_ccsd.py:17-72 ('g' branch): orbitals with occupation

@@ -233,3 +233,48 @@ def test_full_1rdm_host_assembly(orbtype):
     cc.sys.orbtype = 'r'
     with pytest.raises(Exception, match="orbital type r is not implemented"):
         cc.full_1rdm()
+
+
+def test_tdccsd_esn_assembly(monkeypatch):
+    """TDCCSD.compute_ESN's host part (kelvin/td_ccsd.py:160-235
+    attributes) with the propagation replaced by closed forms of (T, mu):
+    the zero order parts against the free-fermion energy sum(e f) and
+    entropy -sum(f ln f + (1-f) ln(1-f)), the first order and CC parts
+    against the analytic derivatives of the stand-in grand potentials, and
+    E = Omega + T S + mu N."""
+    from kelvin_amd import ft_utils
+    from kelvin_amd.td_ccsd import TDCCSD
+    from kelvin_amd.ueg_system import UEGSystem
+
+    def g1(T, mu):
+        return -0.3 * mu * T + 0.05 * T * T
+
+    def gcc(T, mu):
+        return -0.012 + 0.02 * T * mu - 0.3 * T ** 3 + 0.1 * mu * mu * T
+
+    def fake(self, tprop):
+        en = numpy.asarray(self.sys.g_energies_tot(), dtype=numpy.float64)
+        self.G0 = ft_utils.GP0(self.beta, en, self.mu).sum()
+        self.G1 = g1(self.T, self.mu)
+        self.Gcc = gcc(self.T, self.mu)
+        self.Gtot = self.G0 + self.G1 + self.Gcc
+        return self.Gtot, self.Gcc
+    monkeypatch.setattr(TDCCSD, "_ccsd", fake)
+    T, mu = 0.1, 0.1
+    sysx = UEGSystem(T, 2 * numpy.pi, 1.2, mu=mu, norb=7, orbtype='g')
+    td = TDCCSD(sysx, {"tprop": "rk4"}, T=T, mu=mu, ngrid=40)
+    td.run()
+    E, S, N = td.compute_ESN()
+    en = numpy.asarray(sysx.g_energies_tot(), dtype=numpy.float64)
+    f = oracle.ff(1.0 / T, en, mu)
+    assert abs(td.N0 - f.sum()) < 1e-13
+    assert abs(td.E0 - (en * f).sum()) < 1e-12
+    s0 = -(f * numpy.log(f) + (1 - f) * numpy.log(1 - f)).sum()
+    assert abs(td.S0 - s0) < 1e-10
+    assert abs(td.N1 - 0.3 * T) < 1e-9
+    assert abs(td.Ncc + (0.02 * T + 0.2 * mu * T)) < 1e-9
+    assert abs(td.S1 + (-0.3 * mu + 0.1 * T)) < 1e-8
+    assert abs(td.Scc + (0.02 * mu - 0.9 * T * T + 0.1 * mu * mu)) < 1e-8
+    assert abs(N - (td.N0 + td.N1 + td.Ncc)) < 1e-14
+    assert abs(E - (td.Gtot + T * S + mu * N)) < 1e-12
+    assert abs(S - (td.S0 + td.S1 + td.Scc)) < 1e-10
