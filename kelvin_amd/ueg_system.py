@@ -226,6 +226,87 @@ class UEGSystem(System):
         return numpy.einsum('piqi,i->pqi', self.g_aint_tot(), fo * fv)
 
 
+class PUEGSystem(System):
+    """The spin-polarised uniform electron gas (kelvin/pueg_system.py:10-203):
+    one spin, so the generalised orbitals are the plane waves themselves,
+    with energies k^2/2 (:114-115), <pq||rs> = <pq|rs> - <pq|sr> of the
+    spatial integrals (:197-203), the Fock matrix diag(e) + sum_r <pr||qr>
+    fo_r (:141-168), E1 = 1/2 sum_ij <ij||ij> fo_i fo_j (:71-80) and the
+    derivative densities of E1 and of the Fock matrix (:95-103, :178-195).
+    Finite temperature only, like UEGSystem."""
+
+    def __init__(self, T, L, Emax, mu=None, n=None, norb=None):
+        self.T = T
+        self.L = L
+        self.basis = UEGBasis(L, Emax, norb=norb)
+        if n is not None or not T > 0.0:
+            # a fixed electron count sets the zero-temperature occupations
+            # (kelvin/pueg_system.py:37-41), which the GPU path does not build
+            raise NotImplementedError(
+                "PUEGSystem needs T > 0 and mu (zero-temperature occupations "
+                "are not part of the GPU path)")
+        if mu is None:
+            raise ValueError("PUEGSystem needs the chemical potential mu")
+        self.mu = mu
+        self.N = ft_utils.ff(1.0 / self.T, self.g_energies_tot(),
+                             self.mu).sum()
+        self.den = self.N / (L * L * L)
+        self.rs = (3 / (4.0 * numpy.pi * self.den)) ** (1.0 / 3.0)
+        self.Ef = 0.5 * (3.0 * numpy.pi * numpy.pi * self.den) ** (2.0 / 3.0)
+        self.Tf = self.T / self.Ef
+        self.orbtype = 'g'
+        self._eri = None
+
+    def has_g(self):
+        return True
+
+    def has_u(self):
+        return False
+
+    def has_r(self):
+        return False
+
+    def verify(self, T, mu):
+        if T > 0.0:
+            return T == self.T and mu == self.mu
+        return T == self.T
+
+    def const_energy(self):
+        return 0.0
+
+    def g_energies_tot(self):
+        return numpy.asarray(self.basis.Es)
+
+    def g_aint_tot(self):
+        if self._eri is None:
+            V = self.basis.build_r2e_matrix()
+            self._eri = V - V.transpose(0, 1, 3, 2)
+        return self._eri
+
+    def _fo(self):
+        return ft_utils.ff(1.0 / self.T, self.g_energies_tot(), self.mu)
+
+    def g_fock_tot(self):
+        d = self.g_energies_tot()
+        JK = numpy.einsum('prqr,r->pq', self.g_aint_tot(), self._fo())
+        return numpy.diag(d) + JK
+
+    def get_mp1(self):
+        fo = self._fo()
+        return 0.5 * numpy.einsum('ijij,i,j->', self.g_aint_tot(), fo, fo)
+
+    def g_mp1_den(self):
+        fo = self._fo()
+        fv = ft_utils.ffv(1.0 / self.T, self.g_energies_tot(), self.mu)
+        return -(1.0 / self.T) * numpy.einsum(
+            'ijij,i,j->i', self.g_aint_tot(), fo * fv, fo)
+
+    def g_fock_d_den(self):
+        fo = self._fo()
+        fv = ft_utils.ffv(1.0 / self.T, self.g_energies_tot(), self.mu)
+        return numpy.einsum('piqi,i->pqi', self.g_aint_tot(), fo * fv)
+
+
 def ueg_parameters(rs, N, t_over_tf):
     """(L, T, E_F) of an N-electron gas at Wigner-Seitz radius rs and
     reduced temperature T/T_F (L as kelvin/tests/test_ueg.py:8-10, E_F as

@@ -1080,6 +1080,26 @@ class UEGSystem:
         return einsum('piqi,i->pqi', self.g_aint_tot(), fo * fv)
 
 
+class PUEGSystem(UEGSystem):
+    """kelvin/pueg_system.py:10-203 at finite temperature: the spin-polarised
+    gas, one spin, so the orbitals are the plane waves themselves (energies
+    :114-115 over kelvin/ueg_utils.py:73-74) and <pq||rs> = <pq|rs> - <pq|sr>
+    of the spatial integrals (:197-203, the same-spin block of the
+    unsymmetrised spin-orbital matrix); the Fock matrix (:141-168), E1
+    (:71-80) and derivative densities (:95-103, :178-195) take UEGSystem's
+    forms over these."""
+
+    def g_energies_tot(self):
+        return numpy.asarray(self.basis.Es)
+
+    def g_aint_tot(self):
+        if self._eri is None:
+            nb = self.basis.get_nbsf()
+            V = self.basis.build_g2e_matrix(anti=False)[:nb, :nb, :nb, :nb]
+            self._eri = V - V.transpose(0, 1, 3, 2)
+        return self._eri
+
+
 class DFModelSystem:
     """Molecule-like model with density-fitted ERIs.
 

@@ -75,6 +75,33 @@ def test_ueg_provider_matches_oracle(norb, L, Emax):
     assert abs(a.N - b.N) < 1e-14
 
 
+@pytest.mark.parametrize("norb,L,Emax", [(7, 2 * numpy.pi, 1.2),
+                                         (19, 1.942, 30.0)])
+def test_pueg_provider_matches_oracle(norb, L, Emax):
+    """The spin-polarised gas (kelvin/pueg_system.py:10-203): vectorised
+    spatial integrals against the oracle's literal loops, and every
+    quantity the solver and the derivatives read; the interface's
+    errors."""
+    from kelvin_amd.ueg_system import PUEGSystem
+    T, mu = 0.1, 0.1
+    a = PUEGSystem(T, L, Emax, mu=mu, norb=norb)
+    b = oracle.PUEGSystem(T, L, Emax, mu, norb=norb)
+    assert a.orbtype == 'g' and not a.has_u() and a.const_energy() == 0.0
+    assert numpy.array_equal(a.g_energies_tot(), b.g_energies_tot())
+    assert a.g_energies_tot().shape == (norb,)
+    assert numpy.abs(a.g_aint_tot() - b.g_aint_tot()).max() == 0.0
+    assert numpy.abs(a.g_fock_tot() - b.g_fock_tot()).max() < 1e-15
+    assert abs(a.get_mp1() - b.get_mp1()) < 1e-15
+    assert numpy.abs(a.g_mp1_den() - b.g_mp1_den()).max() < 1e-14
+    assert numpy.abs(a.g_fock_d_den() - b.g_fock_d_den()).max() < 1e-15
+    assert abs(a.N - b.N) < 1e-14
+    assert a.verify(T, mu) and not a.verify(T, mu + 0.1)
+    with pytest.raises(NotImplementedError):
+        PUEGSystem(0.0, L, Emax, n=2, norb=norb)
+    with pytest.raises(ValueError):
+        PUEGSystem(T, L, Emax, norb=norb)
+
+
 def test_ueg_config_shells_and_mu():
     from kelvin_amd.ueg_system import ueg_from_config, ueg_parameters
     L, T, Ef = ueg_parameters(4.0, 14, 0.1)

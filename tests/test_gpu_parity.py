@@ -271,6 +271,34 @@ def test_golden_G1_ueg_omega_cc():
         assert abs(Ecc - g["Omega_cc"]) < g["tol"]
 
 
+def test_golden_G2_pueg_omega_cc():
+    """kelvin/tests/test_ft_ccsd.py:24,151-165: the spin-polarised gas (one
+    spin, 7 plane waves; an odd orbital count, so the dense layout) against
+    the reference's Omega_cc at its 1e-8.  The golden value is the energy of
+    iteration 10, the first whose change is below econv, printed by a run
+    that tested the energy alone; that iterate is checked at 1e-11, and the
+    whole history against the oracle's."""
+    from kelvin_amd.ccsd import ccsd
+    from kelvin_amd.ueg_system import PUEGSystem
+    dev()
+    g = GOLD["G2"]
+    s = g["system"]
+    sysp = PUEGSystem(s["T"], 2 * numpy.pi, s["Emax"], mu=s["mu"],
+                      norb=s["norb"])
+    cc = ccsd(sysp, T=s["T"], mu=s["mu"], iprint=0, **g["ccsd"])
+    Etot, Ecc = cc.run()
+    assert abs(Ecc - g["Omega_cc"]) < g["tol"]
+    assert abs(cc.history[9][0] - g["Omega_cc"]) < 1e-11
+    osys = oracle.PUEGSystem(s["T"], 2 * numpy.pi, s["Emax"], s["mu"],
+                             norb=s["norb"])
+    hist = []
+    r = oracle.ft_ccsd(osys, s["T"], s["mu"], history=hist, **g["ccsd"])
+    assert len(cc.history) == len(hist)
+    for (E, _), (Eo, _) in zip(cc.history, hist):
+        assert abs(E - Eo) < 1e-12
+    assert abs(Etot - r["Omega"]) < 1e-11
+
+
 def test_golden_G4_iterations_and_oracle():
     from kelvin_amd.ccsd import ccsd
     dev()

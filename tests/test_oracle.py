@@ -86,6 +86,28 @@ def test_golden_G1_omega_cc():
     assert abs(r["Omega_cc"] - g["Omega_cc"]) < g["tol"]
 
 
+def test_golden_G2_pueg_omega_cc():
+    """kelvin/tests/test_ft_ccsd.py:24,151-165: the spin-polarised gas
+    (one spin, 7 plane waves) through the oracle's solver.  The golden
+    value is the energy of iteration 10, the first whose change is below
+    econv, to all 12 printed digits: it was printed by a run that tested
+    the energy alone.  The loop as it stands (kelvin/cc_utils.py:158, energy
+    and amplitude tests) stops at iteration 12, 3.3e-9 away, inside the
+    test's 1e-8; the iteration-10 energy pins the integrals and the
+    residual to 1e-12."""
+    g = GOLD["G2"]
+    s = g["system"]
+    pueg = ftcc.PUEGSystem(s["T"], 2 * numpy.pi, s["Emax"], s["mu"],
+                           norb=s["norb"])
+    assert pueg.g_energies_tot().shape == (s["norb"],)
+    hist = []
+    r = ftcc.ft_ccsd(pueg, s["T"], s["mu"], history=hist, **g["ccsd"])
+    assert abs(r["Omega_cc"] - g["Omega_cc"]) < g["tol"]
+    assert abs(hist[9][0] - hist[8][0]) < 1e-8
+    assert abs(hist[9][0] - g["Omega_cc"]) < 1e-12
+    assert r["niter"] == 12
+
+
 def test_golden_G4_iterations():
     g = GOLD["G4"]
     ueg, s = _ueg(g)
