@@ -626,3 +626,74 @@ def test_ccsd_1rdm_is_the_lagrangian_derivative():
         'x,xab->ba', g, numpy.einsum('xijca,xceij->xae',
                                      ftcc.int_L2(ng, L2, ti, D2, g, G), T2))
     assert numpy.abs(bad - brute('vv')).max() > 1e-3
+
+
+def test_lambda_makes_the_lagrangian_stationary():
+    """kelvin/tests/test_ft_lambda.py:16-73 (fd_test_L1; :75-133 for the
+    doubles): at the converged amplitudes and the lambdas of ft_lambda_iter,
+    L(T) = E(T) - (1/beta) sum_y g_y [L1.(Phi1(T) - T1) + 1/4 L2.(Phi2(T) - T2)]
+    has zero central-difference derivative (d = 1e-4) along every singles
+    amplitude and along random antisymmetric doubles directions, on a
+    generic system whose Fock matrix has a vo block (T1 is not zero); E
+    alone does not, and lambdas off by 1% do not."""
+    rng = numpy.random.default_rng(5)
+    n, T, mu, ng = 6, 0.25, -0.02, 8
+    h = rng.normal(size=(n, n)) * 0.2
+    h = 0.5 * (h + h.T) + numpy.diag(numpy.linspace(-0.6, 0.6, n))
+    eri = fs.random_antisym(n, rng, hermitian=True) * 0.08
+    en = numpy.diag(h).copy() + 0.03
+    sysx = ftcc.GenericSystem(T, mu, h, eri, en)
+    r = ftcc.ft_ccsd(sysx, T, mu, ngrid=ng, damp=0.2, max_iter=300,
+                     econv=1e-13, tconv=1e-11)
+    assert r["converged"]
+    beta = 1.0 / T
+    ti, g, G = ftcc.ft_quad(ng, beta, 'lin')
+    F, I = ftcc.ft_integrals(sysx.g_fock_tot(), sysx.g_aint_tot(), en, beta,
+                             mu)
+    D1, D2 = ftcc.D1(en, en), ftcc.D2(en, en)
+    T1, T2 = r["T1"], r["T2"]
+    assert numpy.abs(T1).max() > 1e-4
+    L1, L2 = ftcc.ccsd_lambda_guess(F, I, T1, beta, ng)
+    L1, L2, conv, _ = ftcc.ft_lambda_iter(
+        L1, L2, T1, T2, F, I, D1, D2, g, G, beta, ng, ti,
+        {"tconv": 1e-12, "max_iter": 300, "damp": 0.2})
+    assert conv
+
+    def lag(t1, t2, l1, l2):
+        c = (numpy.einsum('yia,yai->y', l1, t1)
+             + 0.25 * numpy.einsum('yijab,yabij->y', l2, t2)) / beta
+        return (ftcc.ft_lagrangian(F, I, t1, t2, l1, l2, D1, D2, ti, ng, g,
+                                   G, beta) + numpy.dot(c, g))
+
+    def energy(t1, t2):
+        return ftcc.ft_cc_energy(t1, t2, F.ov, I.oovv, g, beta)
+
+    d = 1e-4
+
+    def slope(f, x1, x2):
+        return (f(T1 + d * x1, T2 + d * x2)
+                - f(T1 - d * x1, T2 - d * x2)) / (2 * d)
+    worst, e_scale = 0.0, 0.0
+    for y in range(ng):
+        for a in range(n):
+            for i in range(n):
+                x1 = numpy.zeros_like(T1)
+                x1[y, a, i] = 1.0
+                x2 = numpy.zeros_like(T2)
+                worst = max(worst, abs(slope(
+                    lambda t1, t2: lag(t1, t2, L1, L2), x1, x2)))
+                e_scale = max(e_scale, abs(slope(energy, x1, x2)))
+    for _ in range(3):
+        x2 = ftcc.antisymmetrize(rng.normal(size=T2.shape))
+        x1 = numpy.zeros_like(T1)
+        worst = max(worst, abs(slope(lambda t1, t2: lag(t1, t2, L1, L2),
+                                     x1, x2)))
+        e_scale = max(e_scale, abs(slope(energy, x1, x2)))
+    assert e_scale > 1e-3
+    assert worst < 1e-7 * e_scale
+    # lambdas off by 1%: the derivative no longer vanishes
+    x1 = numpy.zeros_like(T1)
+    x1[0, 0, 0] = 1.0
+    off = abs(slope(lambda t1, t2: lag(t1, t2, 1.01 * L1, 1.01 * L2), x1,
+                    numpy.zeros_like(T2)))
+    assert off > 1e-5 * e_scale
