@@ -244,20 +244,26 @@ def test_esn_and_lambda_match_oracle_G4():
     assert abs(cc.E - E) < 1e-9
 
 
-@pytest.mark.parametrize("kind", ["df", "generic"])
+@pytest.mark.parametrize("kind", ["df", "generic", "pueg"])
 def test_esn_matches_finite_differences(kind):
     """The adjoint against central differences of full solves (h = 1e-4)
-    on the density-fitted model and on a system whose Fock matrix depends
-    on the occupations (kelvin's g_fock_d_den term), and against the
-    oracle's adjoint (1e-10); gderiv=False against the oracle too."""
+    on the density-fitted model, on a system whose Fock matrix depends
+    on the occupations (kelvin's g_fock_d_den term) and on the
+    spin-polarised gas (kelvin/tests/test_ft_deriv.py:387-441), and against
+    the oracle's adjoint (1e-10); gderiv=False against the oracle too."""
     from kelvin_amd.ccsd import ccsd
     from kelvin_amd.model_systems import ArraySystem, DFModelSystem
+    from kelvin_amd.ueg_system import PUEGSystem
     dev()
     rng = numpy.random.default_rng(5)
     if kind == "df":
         T, mu = 0.3, 0.05
         sysg = DFModelSystem(T, mu, norb=3, seed=1, sigma=0.2)
         osys = oracle.DFModelSystem(T, mu, norb=3, seed=1, sigma=0.2)
+    elif kind == "pueg":
+        T, mu = 0.1, 0.1
+        sysg = PUEGSystem(T, 2 * numpy.pi, 1.2, mu=mu, norb=7)
+        osys = oracle.PUEGSystem(T, 2 * numpy.pi, 1.2, mu, norb=7)
     else:
         n = 6
         h = rng.normal(size=(n, n)) * 0.2

@@ -549,7 +549,7 @@ def _omega(make, T, mu, kw):
     return ftcc.ft_ccsd(make(T, mu), T, mu, **kw)["Omega"]
 
 
-@pytest.mark.parametrize("kind", ["df", "generic"])
+@pytest.mark.parametrize("kind", ["df", "generic", "pueg"])
 def test_esn_matches_finite_differences(kind):
     """N = -dOmega/dmu, S = -dOmega/dT and E = Omega + T S + mu N from the
     adjoint equal central differences (h = 1e-4) of fully converged solves
@@ -561,6 +561,11 @@ def test_esn_matches_finite_differences(kind):
         def make(T, mu):
             return ftcc.DFModelSystem(T, mu, norb=3, seed=1, sigma=0.2)
         T, mu = 0.3, 0.05
+    elif kind == "pueg":
+        # kelvin/tests/test_ft_deriv.py:387-441 (the spin-polarised gas)
+        def make(T, mu):
+            return ftcc.PUEGSystem(T, 2 * numpy.pi, 1.2, mu, norb=7)
+        T, mu = 0.1, 0.1
     else:
         n = 6
         h = rng.normal(size=(n, n)) * 0.2
