@@ -108,6 +108,14 @@ def _mapped_rule(ng, quad):
     return delta, numpy.asarray([float(i) * delta for i in range(ng)])
 
 
+def _power(quad):
+    """The power n of the polynomial map t = beta (s^n + s) / 2: a rule
+    name, or n itself (simpsons_p)."""
+    if isinstance(quad, str):
+        return {'quad': 2, 'cub': 3, 'quar': 4}[quad]
+    return int(quad)
+
+
 def _weigh(quad, x, si, scale):
     """x * scale * dt/ds in the reference's operation order."""
     if quad == 'ln':
@@ -116,7 +124,7 @@ def _weigh(quad, x, si, scale):
         return scale * x * numpy.cos(si) / 2.0
     if quad == 'exp':
         return scale * x * numpy.exp(si)
-    p = {'quad': 2, 'cub': 3, 'quar': 4}[quad]
+    p = _power(quad)
     return scale * x * (numpy.power(si, p - 1) * float(p) + 1.0) / 2.0
 
 
@@ -127,7 +135,7 @@ def _times(quad, si, beta):
         return beta * (numpy.sin(si) + 1.0) / 2.0
     if quad == 'exp':
         return beta * (numpy.exp(si) - 1.0)
-    p = {'quad': 2, 'cub': 3, 'quar': 4}[quad]
+    p = _power(quad)
     return beta * (numpy.power(si, p) + si) / 2.0
 
 
@@ -143,6 +151,42 @@ def _mapped(ng, beta, quad, with_t):
 
 
 MAPPED = ('ln', 'sin', 'exp', 'quad', 'cub', 'quar')
+
+
+# The named mapped rules of kelvin/quadrature.py:118-212: Simpson's rule in
+# s, mapped to t by t = beta ln s (ln), beta (sin s + 1)/2 (sin),
+# beta (e^s - 1) (exp) and beta (s^n + s)/2 (p); (ti, g, G), and the beta
+# derivatives (g, G) of the d_ forms.
+def simpsons_ln(ng, beta):
+    return _mapped(ng, beta, 'ln', True)
+
+
+def d_simpsons_ln(ng, beta):
+    return _mapped(ng, 1.0, 'ln', False)
+
+
+def simpsons_sin(ng, beta):
+    return _mapped(ng, beta, 'sin', True)
+
+
+def d_simpsons_sin(ng, beta):
+    return _mapped(ng, 1.0, 'sin', False)
+
+
+def simpsons_exp(ng, beta):
+    return _mapped(ng, beta, 'exp', True)
+
+
+def d_simpsons_exp(ng, beta):
+    return _mapped(ng, 1.0, 'exp', False)
+
+
+def simpsons_p(ng, beta, n=2):
+    return _mapped(ng, beta, n, True)
+
+
+def d_simpsons_p(ng, beta, n=2):
+    return _mapped(ng, 1.0, n, False)
 
 
 def ft_quad(ng, beta, quad):

@@ -59,6 +59,34 @@ def test_host_grids_match_oracle(quad):
         quadrature.d_ft_quad(5, 1.0, 'mid')
 
 
+def test_named_mapped_rules():
+    """kelvin's named rules (kelvin/quadrature.py:118-212) equal ft_quad /
+    d_ft_quad of the same name bit for bit; simpsons_p at a power with no
+    name (n = 5) maps [0, 1] onto [0, beta], is linear in beta (the d_ form
+    times beta), and its weights integrate dt to beta as the grid refines
+    (Simpson's error in s)."""
+    from kelvin_amd import quadrature as q
+    beta, ng = 3.1, 9
+    named = [(q.simpsons_ln, q.d_simpsons_ln, 'ln'),
+             (q.simpsons_sin, q.d_simpsons_sin, 'sin'),
+             (q.simpsons_exp, q.d_simpsons_exp, 'exp')]
+    for n, name in ((2, 'quad'), (3, 'cub'), (4, 'quar')):
+        named.append((lambda ng, b, n=n: q.simpsons_p(ng, b, n),
+                      lambda ng, b, n=n: q.d_simpsons_p(ng, b, n), name))
+    for rule, drule, name in named:
+        for x, y in zip(rule(ng, beta), q.ft_quad(ng, beta, name)):
+            assert numpy.array_equal(x, y)
+        for x, y in zip(drule(ng, beta), q.d_ft_quad(ng, beta, name)):
+            assert numpy.array_equal(x, y)
+    ti, g, G = q.simpsons_p(ng, beta, 5)
+    dg, dG = q.d_simpsons_p(ng, beta, 5)
+    assert ti[0] == 0.0 and abs(ti[-1] - beta) < 1e-15
+    assert numpy.abs(g - beta * dg).max() < 1e-14
+    assert numpy.abs(G - beta * dG).max() < 1e-14
+    err = [abs(q.simpsons_p(m, beta, 5)[1].sum() - beta) for m in (11, 21)]
+    assert err[1] < err[0] / 10 and err[1] < 1e-4 * beta
+
+
 @pytest.mark.parametrize("norb,L,Emax", [(7, 2 * numpy.pi, 1.2),
                                          (19, 2 * numpy.pi, 1.2),
                                          (19, 1.942, 30.0)])
