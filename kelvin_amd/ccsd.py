@@ -187,6 +187,14 @@ class ccsd(object):
         en = self.sys.g_energies_tot()
         return self.n1rdm + numpy.diag(ft_utils.ff(self.beta, en, self.mu))
 
+    def full_2rdm(self, relax=False):
+        """kelvin/ccsd.py:2008-2053, the two-particle density: not built (E,
+        S, N and the one-particle density do not need it); the reference's
+        error for relax=True, NotImplementedError otherwise."""
+        if relax:
+            raise Exception("Rexalex 2-RDM is not implemented")
+        raise NotImplementedError("the two-particle density is not built")
+
     def compute_ESN_fd(self, delta=1e-4, richardson=False, fd_econv=1e-12):
         """E, S and N by central differences of full GPU solves.
 

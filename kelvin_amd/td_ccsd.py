@@ -136,6 +136,20 @@ class TDCCSD(object):
         self.Scc = self.S - self.S0 - self.S1
         return self.E, self.S, self.N
 
+    def full_1rdm(self, relax=False):
+        """kelvin/td_ccsd.py:1996-2031: the densities of the backward lambda
+        propagation, which is not built (ccsd.full_1rdm gives the
+        fixed-point density of the same system)."""
+        raise NotImplementedError(
+            "TD-CCSD densities (backward lambda propagation) are not built")
+
+    def full_2rdm(self, relax=False):
+        """kelvin/td_ccsd.py:2032-2062: not built (see full_1rdm)."""
+        if relax:
+            raise Exception("Rexalex 2-RDM is not implemented")
+        raise NotImplementedError(
+            "TD-CCSD densities (backward lambda propagation) are not built")
+
     def _active(self, en):
         """kelvin/td_ccsd.py:17-72 ('g' branch): orbitals with occupation
         above athresh and vacancy above fthresh, and vice versa."""

@@ -333,3 +333,22 @@ def test_tdccsd_esn_assembly(monkeypatch):
     assert abs(N - (td.N0 + td.N1 + td.Ncc)) < 1e-14
     assert abs(E - (td.Gtot + T * S + mu * N)) < 1e-12
     assert abs(S - (td.S0 + td.S1 + td.Scc)) < 1e-10
+
+
+def test_unbuilt_densities_raise():
+    """The densities outside the path raise instead of being missing:
+    kelvin's own error for a relaxed 2-RDM, NotImplementedError for the rest
+    (no GPU needed: nothing is solved)."""
+    from kelvin_amd.ccsd import ccsd
+    from kelvin_amd.td_ccsd import TDCCSD
+    from kelvin_amd.ueg_system import UEGSystem
+    sysg = UEGSystem(0.1, 2 * numpy.pi, 1.2, mu=0.1, norb=7, orbtype='g')
+    cc = ccsd(sysg, T=0.1, mu=0.1, ngrid=4)
+    td = TDCCSD(sysg, {"tprop": "rk4"}, T=0.1, mu=0.1, ngrid=4)
+    for obj in (cc, td):
+        with pytest.raises(NotImplementedError):
+            obj.full_2rdm()
+        with pytest.raises(Exception, match="Rexalex 2-RDM"):
+            obj.full_2rdm(relax=True)
+    with pytest.raises(NotImplementedError):
+        td.full_1rdm()
