@@ -352,3 +352,28 @@ def test_unbuilt_densities_raise():
             obj.full_2rdm(relax=True)
     with pytest.raises(NotImplementedError):
         td.full_1rdm()
+
+
+def test_batch_host_logic():
+    """The serving mix's host side: the instance list is reproducible and
+    drawn from the stated classes and ranges, the cost model orders by
+    n^6 ng, more than four worker streams are refused before anything
+    touches a device, and the latency statistics are the percentiles."""
+    from kelvin_amd import batch
+    a, b = batch.c4_instances(64, seed=4), batch.c4_instances(64, seed=4)
+    assert a == b and len(a) == 64
+    assert batch.c4_instances(64, seed=5) != a
+    for inst in a:
+        assert inst["nk"] in (19, 27, 33) and inst["ng"] in (8, 12, 16, 24)
+        assert 0.05 <= inst["t_over_tf"] <= 1.0 and 1.0 <= inst["rs"] <= 5.0
+        assert inst["N"] == 14
+    assert len({(x["nk"], x["ng"]) for x in a}) > 6
+    small = dict(nk=19, ng=8)
+    assert batch.cost(small) == float(38) ** 6 * 8
+    assert batch.cost(dict(nk=33, ng=8)) > batch.cost(dict(nk=27, ng=24))
+    for w in (0, 5):
+        with pytest.raises(ValueError):
+            batch.solve_batch(a[:1], workers=w)
+    res = [{"latency": float(x)} for x in range(1, 101)]
+    st = batch._stats(res, "latency")
+    assert st["p50_s"] == 50.5 and abs(st["p99_s"] - 99.01) < 1e-12
