@@ -28,6 +28,20 @@ How each piece is pinned (tests/test_oracle_*.py):
     ``ft_lagrangian`` with respect to its Fock element, evaluated exactly by
     unit perturbations (the Lagrangian is linear in the Fock blocks) -- the
     identity kelvin/tests/test_ft_ccsd_rdm.py:12-170 checks as a trace.
+  * adjoint (``stanton_rhs_transpose``, ``int_adjoint``, ``int_L1/2``,
+    ``ft_lambda_iter``, ``response_blocks``, ``grid_derivative``,
+    ``ft_ccsd_esn``): the dot-product identity <Lam, J X> = <J^T Lam, X>
+    (exact: the residual is quadratic), the reverse integration as the
+    transpose of int_tbar1/2, the stationarity of the Lagrangian at the
+    converged lambdas (kelvin/tests/test_ft_lambda.py:16-133), kelvin's
+    E, S, N of the G4 example, and central differences of full solves.
+  * systems: ``UEGSystem`` by its symmetries and the goldens;
+    ``PUEGSystem`` by the reference's polarised-gas grand potential
+    (kelvin/tests/test_ft_ccsd.py:24, whose printed value is this oracle's
+    iteration-10 energy to 1e-12); ``DFModelSystem`` by the positive
+    semidefinite, rank-limited (pr|qs) block and the spin zero pattern.
+  * TD-CCSD (oracle/td_ccsd.py): the order of each propagator on an exactly
+    solvable problem, and agreement with the fixed point.
   * end to end: the reference's golden UEG grand potentials
     (kelvin/tests/test_ft_ccsd.py:23 with :119-133, Omega_cc=-0.010750238811
     at 1e-9) and the per-iteration energies printed in
