@@ -11,7 +11,7 @@ def __getattr__(name):
     if name == "ccsd":
         from .ccsd import ccsd
         return ccsd
-    if name in ("UEGSystem", "UEGBasis", "ueg_from_config"):
+    if name in ("UEGSystem", "PUEGSystem", "UEGBasis", "ueg_from_config"):
         from . import ueg_system
         return getattr(ueg_system, name)
     if name in ("DFModelSystem", "ArraySystem"):

@@ -16,7 +16,7 @@ import torch
 
 from . import ft_cc_equations
 from .engine import FTEngine
-from .ov_blocks import EngineBlocks, EngineDenominators, engine_eri, engine_fock
+from .ov_blocks import engine_eri, engine_fock
 
 
 def _engine_for_system(sys, en, beta, mu, iocc, ivir, ng, precision):
