@@ -1335,7 +1335,6 @@ void drop_graph(ftcc_ctx *c) {
 void residual_chunk(ftcc_ctx *c, const double *t1, const double *t2,
                     double *t1bar, double *t2bar, int nb) {
   const int O = c->no, V = c->nv;
-  const int P = c->prec;           // contraction precision of n^5/n^6 terms
   const int P64 = FTCC_PREC_FP64;  // small terms
   // O(n^5) contractions run in FP64 whatever the precision of the O(n^6)
   // products: they stream n^4 operands once, and on these skinny shapes

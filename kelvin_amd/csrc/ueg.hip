@@ -90,7 +90,7 @@ __global__ __launch_bounds__(UB) void mp1_kernel(int n, const double *eri,
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int w = UB / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {

@@ -428,7 +428,7 @@ __global__ void reduce_partials_kernel(const double *partials, int nblocks,
   red[2][threadIdx.x] = s2;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
+    if ((int)threadIdx.x < w) {
       red[0][threadIdx.x] += red[0][threadIdx.x + w];
       red[1][threadIdx.x] += red[1][threadIdx.x + w];
       red[2][threadIdx.x] += red[2][threadIdx.x + w];
