@@ -1,4 +1,4 @@
-"""Finite-temperature CCSD driver with the reference's user API.
+"""Finite-temperature CCSD solver with the reference's user API.
 
 ``ccsd(sys, T=, mu=, ngrid=, ...)`` / ``run()`` mirror kelvin/ccsd.py:43-126
 and the generalised-orbital solver ``_ft_ccsd`` (kelvin/ccsd.py:595-712):
@@ -61,7 +61,7 @@ def system_at(sys, T, mu):
 
 
 class ccsd(object):
-    """Coupled cluster singles and doubles (CCSD) driver at finite T."""
+    """Coupled cluster singles and doubles (CCSD) solver at finite T."""
 
     def __init__(self, sys, T=0.0, mu=0.0, iprint=0, singles=True, econv=1e-8,
                  tconv=None, max_iter=40, damp=0.0, ngrid=10, realtime=False,

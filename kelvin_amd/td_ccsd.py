@@ -31,7 +31,7 @@ _IMPLICIT = ("be", "cn", "am2")
 
 
 class TDCCSD(object):
-    """Time-dependent (imaginary-time propagated) CCSD driver."""
+    """Time-dependent (imaginary-time propagated) CCSD solver."""
 
     def __init__(self, sys, prop, T=0.0, mu=0.0, iprint=0, singles=True,
                  ngrid=10, athresh=0.0, fthresh=0.0, quad='lin',

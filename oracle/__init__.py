@@ -11,7 +11,7 @@ Modules
 ``ftcc``        numpy restatement of the reference's hot path (quadrature,
                 Fermi factors, UEG integrals, Fermi-weighted blocks, Stanton
                 CCSD residual, exp(-tau D) integrators, grand-potential, the
-                damped fixed-point driver).
+                damped fixed-point solver).
 ``fock_space``  brute-force Fock-space checks used to pin the residual:
                 <Phi_mu| exp(-T) H exp(T) |Phi_0> by explicit matrices, and
                 the finite-temperature full-CI grand potential.
