@@ -27,7 +27,10 @@ How each piece is pinned (tests/test_oracle_*.py):
   * ``ccsd_1rdm``: every element against the derivative of
     ``ft_lagrangian`` with respect to its Fock element, evaluated exactly by
     unit perturbations (the Lagrangian is linear in the Fock blocks) -- the
-    identity kelvin/tests/test_ft_ccsd_rdm.py:12-170 checks as a trace.
+    identity kelvin/tests/test_ft_ccsd_rdm.py:12-170 checks as a trace;
+    ``normal_1rdm`` (less its occupations) against the derivative of the
+    Lagrangian with respect to the orbital Fock matrix through
+    ``ft_integrals``' square-root weights, also exact by unit perturbations.
   * adjoint (``stanton_rhs_transpose``, ``int_adjoint``, ``int_L1/2``,
     ``ft_lambda_iter``, ``response_blocks``, ``grid_derivative``,
     ``ft_ccsd_esn``): the dot-product identity <Lam, J X> = <J^T Lam, X>

@@ -702,3 +702,53 @@ def test_lambda_makes_the_lagrangian_stationary():
     off = abs(slope(lambda t1, t2: lag(t1, t2, 1.01 * L1, 1.01 * L2), x1,
                     numpy.zeros_like(T2)))
     assert off > 1e-5 * e_scale
+
+
+def test_normal_1rdm_is_the_orbital_fock_derivative():
+    """normal_1rdm (kelvin/ccsd.py:1373-1387) without its occupations is
+    the derivative of the Lagrangian with respect to the orbital Fock
+    matrix: with F[p,q] = s_p f[p,q] s_q (ft_integrals, kelvin/
+    cc_utils.py:569-602, s the square roots of the occupations or
+    vacancies), d L / d f[p,q] = n1[q,p], evaluated exactly by unit
+    perturbations of f through ft_integrals; random amplitudes, multipliers
+    and integrals, non-uniform occupations.  The occupation factors swapped
+    in one block are detected."""
+    rng = numpy.random.default_rng(8)
+    n, ng, beta, mu = 4, 5, 2.0, 0.1
+    en = numpy.sort(rng.uniform(-1.0, 1.0, n))
+
+    def antisym(x):
+        x = x - numpy.swapaxes(x, -4, -3)
+        return x - numpy.swapaxes(x, -2, -1)
+    T1 = rng.normal(size=(ng, n, n)) * 0.3
+    T2 = antisym(rng.normal(size=(ng, n, n, n, n)) * 0.2)
+    L1 = rng.normal(size=(ng, n, n)) * 0.3
+    L2 = antisym(rng.normal(size=(ng, n, n, n, n)) * 0.2)
+    eri = fs.random_antisym(n, rng, hermitian=True) * 0.1
+    f0 = rng.normal(size=(n, n)) * 0.2 + numpy.diag(en)
+    ti, g, G = ftcc.ft_quad(ng, beta, 'lin')
+    D1, D2 = ftcc.D1(en, en), ftcc.D2(en, en)
+
+    def lag(fock):
+        F, I = ftcc.ft_integrals(fock, eri, en, beta, mu)
+        return ftcc.ft_lagrangian(F, I, T1, T2, L1, L2, D1, D2, ti, ng, g,
+                                  G, beta)
+    base = lag(f0)
+    deriv = numpy.zeros((n, n))
+    for p in range(n):
+        for q in range(n):
+            f = f0.copy()
+            f[p, q] += 1.0
+            deriv[p, q] = lag(f) - base     # exact: L is linear in f
+    fo, fv = ftcc.ff(beta, en, mu), ftcc.ffv(beta, en, mu)
+    blocks = ftcc.ccsd_1rdm(T1, T2, L1, L2, D1, D2, ti, ng, g, G)
+    n1 = ftcc.normal_1rdm(*blocks, fo, fv, beta) - numpy.diag(fo)
+    scale = numpy.abs(deriv).max()
+    assert scale > 1e-3
+    assert numpy.abs(n1.T - deriv).max() < 1e-12 * max(1.0, scale)
+    # the vo block scaled by sfv x sfo instead of sfo x sfv is caught
+    pia, pba, pji, pai = blocks
+    sfo, sfv = numpy.sqrt(fo), numpy.sqrt(fv)
+    bad = n1 + (pia * sfv[:, None] * sfo[None, :]
+                - pia * sfo[:, None] * sfv[None, :]) / beta
+    assert numpy.abs(bad.T - deriv).max() > 1e-4 * scale
