@@ -38,6 +38,9 @@ How each piece is pinned (tests/test_oracle_*.py):
     transpose of int_tbar1/2, the stationarity of the Lagrangian at the
     converged lambdas (kelvin/tests/test_ft_lambda.py:16-133), kelvin's
     E, S, N of the G4 example, and central differences of full solves.
+  * ``ft_active_integrals``: every block equals ``ft_integrals``' block
+    restricted to the kept lists; keeping every orbital reproduces the
+    full-space solve.
   * systems: ``UEGSystem`` by its symmetries and the goldens;
     ``PUEGSystem`` by the reference's polarised-gas grand potential
     (kelvin/tests/test_ft_ccsd.py:24, whose printed value is this oracle's

@@ -752,3 +752,44 @@ def test_normal_1rdm_is_the_orbital_fock_derivative():
     bad = n1 + (pia * sfv[:, None] * sfo[None, :]
                 - pia * sfo[:, None] * sfv[None, :]) / beta
     assert numpy.abs(bad.T - deriv).max() > 1e-4 * scale
+
+
+def test_active_integrals_are_the_full_blocks_restricted():
+    """ft_active_integrals (kelvin/cc_utils.py:825-858) with the occupied
+    list of orbitals whose occupation exceeds athresh and the virtual list
+    of those whose vacancy does equals ft_integrals' blocks restricted to
+    those lists, every one of the 13 blocks; and a solve whose threshold
+    keeps every orbital (the active branch of kelvin/ccsd.py:642-660) equals
+    the full-space solve."""
+    rng = numpy.random.default_rng(12)
+    n, T, mu = 6, 0.25, -0.02
+    h = rng.normal(size=(n, n)) * 0.2
+    h = 0.5 * (h + h.T) + numpy.diag(numpy.linspace(-0.6, 0.6, n))
+    eri = fs.random_antisym(n, rng, hermitian=True) * 0.08
+    en = numpy.diag(h).copy() + 0.03
+    sysx = ftcc.GenericSystem(T, mu, h, eri, en)
+    beta = 1.0 / T
+    fo, fv = ftcc.ff(beta, en, mu), ftcc.ffv(beta, en, mu)
+    thr = 0.1
+    iocc = [i for i in range(n) if fo[i] > thr]
+    ivir = [i for i in range(n) if fv[i] > thr]
+    assert 0 < len(iocc) < n and 0 < len(ivir) < n and iocc != ivir
+    fock, eri_t = sysx.g_fock_tot(), sysx.g_aint_tot()
+    Fa, Ia = ftcc.ft_active_integrals(fock, eri_t, en, fo[iocc], fv[ivir],
+                                      iocc, ivir)
+    Ff, If = ftcc.ft_integrals(fock, eri_t, en, beta, mu)
+    idx = {'o': iocc, 'v': ivir}
+    for name in ('oo', 'ov', 'vo', 'vv'):
+        ref = Ff[name][numpy.ix_(idx[name[0]], idx[name[1]])]
+        assert numpy.array_equal(Fa[name], ref), name
+    for name in ('vvvv', 'vvvo', 'vovv', 'vvoo', 'vovo', 'oovv', 'vooo',
+                 'ooov', 'oooo'):
+        ref = If[name][numpy.ix_(*(idx[x] for x in name))]
+        assert numpy.abs(Ia[name] - ref).max() < 1e-15, name
+    kw = dict(ngrid=6, damp=0.2, max_iter=200, econv=1e-12, tconv=1e-10)
+    full = ftcc.ft_ccsd(sysx, T, mu, **kw)
+    keep_all = ftcc.ft_ccsd(sysx, T, mu, athresh=1e-300, **kw)
+    assert abs(full["Omega_cc"] - keep_all["Omega_cc"]) < 1e-14
+    cut = ftcc.ft_ccsd(sysx, T, mu, athresh=thr, **kw)
+    assert cut["T1"].shape[1:] == (len(ivir), len(iocc))
+    assert abs(cut["Omega_cc"] - full["Omega_cc"]) > 1e-8
