@@ -793,3 +793,25 @@ def test_active_integrals_are_the_full_blocks_restricted():
     cut = ftcc.ft_ccsd(sysx, T, mu, athresh=thr, **kw)
     assert cut["T1"].shape[1:] == (len(ivir), len(iocc))
     assert abs(cut["Omega_cc"] - full["Omega_cc"]) > 1e-8
+
+
+def test_ueg_params_and_mu():
+    """The workload helpers: rs is the radius of the sphere that holds one
+    electron (4/3 pi rs^3 N = L^3), E_F = k_F^2 / 2 with k_F^3 = 3 pi^2 N / L^3
+    (the free-electron sphere holding N electrons of both spins), T = x E_F;
+    solve_mu returns the mu at which the Fermi occupations sum to N."""
+    N, rs, x = 14, 4.0, 0.05
+    L, T, Ef = ftcc.ueg_params(19, rs, N, x)
+    assert abs(4.0 / 3.0 * numpy.pi * rs ** 3 * N - L ** 3) < 1e-12 * L ** 3
+    kF = (3.0 * numpy.pi ** 2 * N / L ** 3) ** (1.0 / 3.0)
+    assert abs(Ef - 0.5 * kF * kF) < 1e-15 and abs(T - x * Ef) < 1e-17
+    # kF from the electron count of the filled sphere: 2 (4/3 pi kF^3) /
+    # (2 pi / L)^3 = N
+    assert abs(2 * 4.0 / 3.0 * numpy.pi * kF ** 3 / (2 * numpy.pi / L) ** 3
+               - N) < 1e-10
+    s0 = ftcc.UEGSystem(T, L, 0.5 * (2 * numpy.pi / L) ** 2 * 8.0, 0.0,
+                        norb=19)
+    en = s0.g_energies_tot()
+    mu = ftcc.solve_mu(en, 1.0 / T, N)
+    assert abs(ftcc.ff(1.0 / T, en, mu).sum() - N) < 1e-10
+    assert abs(ftcc.ff(1.0 / T, en, mu + 1e-3).sum() - N) > 1e-6
