@@ -815,3 +815,48 @@ def test_ueg_params_and_mu():
     mu = ftcc.solve_mu(en, 1.0 / T, N)
     assert abs(ftcc.ff(1.0 / T, en, mu).sum() - N) < 1e-10
     assert abs(ftcc.ff(1.0 / T, en, mu + 1e-3).sum() - N) > 1e-6
+
+
+def test_fock_space_builders_closed_forms():
+    """The brute-force checker's own pieces (oracle/fock_space.py), by what
+    the mathematics fixes: the Jordan-Wigner matrices obey the canonical
+    anticommutation relations; without interaction the grand potential is
+    the free-fermion sum of GP0 over the eigenvalues of h, and with one
+    pair interaction V = <01||01> the doubly occupied two-mode state has
+    energy h00 + h11 + V; at T = 0 the projection gives the driving terms
+    f[a,i] and <ab||ij>."""
+    nso = 4
+    ops = fs.jw_annihilators(nso)
+    eye = numpy.eye(1 << nso)
+    for p in range(nso):
+        for q in range(nso):
+            acr = ops[p] @ ops[q].T + ops[q].T @ ops[p]
+            aa = ops[p] @ ops[q] + ops[q] @ ops[p]
+            assert numpy.array_equal(acr, eye * (p == q))
+            assert not aa.any()
+    rng = numpy.random.default_rng(13)
+    h = rng.normal(size=(nso, nso))
+    h = 0.5 * (h + h.T)
+    beta, mu = 1.7, 0.2
+    eps = numpy.linalg.eigvalsh(h)
+    om = fs.ft_fci_omega(h, numpy.zeros((nso,) * 4), beta, mu)
+    assert abs(om - ftcc.GP0(beta, eps, mu).sum()) < 1e-12
+    # two modes, one pair interaction
+    h2 = numpy.diag([0.3, -0.2])
+    V = 0.7
+    eri = numpy.zeros((2, 2, 2, 2))
+    eri[0, 1, 0, 1] = eri[1, 0, 1, 0] = V
+    eri[0, 1, 1, 0] = eri[1, 0, 0, 1] = -V
+    H = fs.hamiltonian(h2, eri, fs.jw_annihilators(2))
+    assert abs(H[3, 3] - (0.3 - 0.2 + V)) < 1e-15
+    assert abs(H[1, 1] - 0.3) < 1e-15 and abs(H[2, 2] + 0.2) < 1e-15
+    # T = 0: <Phi_i^a|H|Phi_0> = f_ai, <Phi_ij^ab|H|Phi_0> = <ab||ij>
+    no, nv = 2, 2
+    n = no + nv
+    f = rng.normal(size=(n, n))
+    f = 0.5 * (f + f.T)
+    g = fs.random_antisym(n, rng, hermitian=True)
+    r1, r2 = fs.cc_projection(f, g, numpy.zeros((nv, no)),
+                              numpy.zeros((nv, nv, no, no)), no)
+    assert numpy.abs(r1 - f[no:, :no]).max() < 1e-12
+    assert numpy.abs(r2 - g[no:, no:, :no, :no]).max() < 1e-12
