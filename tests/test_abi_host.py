@@ -377,3 +377,31 @@ def test_batch_host_logic():
     res = [{"latency": float(x)} for x in range(1, 101)]
     st = batch._stats(res, "latency")
     assert st["p50_s"] == 50.5 and abs(st["p99_s"] - 99.01) < 1e-12
+
+
+def test_host_fermi_helpers():
+    """kelvin_amd.ft_utils (the host Fermi helpers of the providers and the
+    derivative assembly): equal to the oracle's closed forms, ff + ffv = 1,
+    finite at extreme arguments, dGP0 the beta derivative of GP0 (central
+    differences), and the free-fermion identity E0 = beta sum dGP0 +
+    mu N0 + Omega0 = sum e ff."""
+    from kelvin_amd import ft_utils as fu
+    rng = numpy.random.default_rng(14)
+    en = numpy.concatenate([rng.uniform(-2, 2, 50), [-1e3, 1e3, 0.1]])
+    beta, mu = 3.0, 0.1
+    for name in ("ff", "ffv", "GP0", "dGP0"):
+        a = getattr(fu, name)(beta, en, mu)
+        b = getattr(oracle, name)(beta, en, mu)
+        assert numpy.all(numpy.isfinite(a))
+        assert numpy.abs(a - b).max() < 1e-13 * max(1.0, numpy.abs(b).max())
+    assert numpy.abs(fu.ff(beta, en, mu) + fu.ffv(beta, en, mu) - 1).max() \
+        < 1e-15
+    e = en[:50]
+    d = 1e-5
+    fd = (fu.GP0(beta + d, e, mu) - fu.GP0(beta - d, e, mu)) / (2 * d)
+    assert numpy.abs(fd - fu.dGP0(beta, e, mu)).max() < 1e-8
+    f = fu.ff(beta, e, mu)
+    E0 = beta * fu.dGP0(beta, e, mu).sum() + mu * f.sum() + \
+        fu.GP0(beta, e, mu).sum()
+    assert abs(E0 - (e * f).sum()) < 1e-12
+    assert abs(fu.HtoK(1.0) - 315775.12891) < 1e-9
