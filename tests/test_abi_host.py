@@ -405,3 +405,42 @@ def test_host_fermi_helpers():
         fu.GP0(beta, e, mu).sum()
     assert abs(E0 - (e * f).sum()) < 1e-12
     assert abs(fu.HtoK(1.0) - 315775.12891) < 1e-9
+
+
+def test_array_system_matches_oracle_and_its_derivatives():
+    """ArraySystem (explicit h, <pq||rs>, energies) equals the oracle's
+    GenericSystem in every quantity the solver and the derivatives read;
+    its derivative densities are the derivatives of E1 and of the Fock
+    matrix with respect to x_k = beta (e_k - mu) through the occupations
+    (beta dE1/dx_k and -dF/dx_k, the conventions of kelvin's g_mp1_den and
+    g_fock_d_den), by central differences."""
+    from kelvin_amd import ft_utils as fu
+    from kelvin_amd.model_systems import ArraySystem
+    from oracle import fock_space as fs
+    rng = numpy.random.default_rng(15)
+    n, T, mu = 6, 0.3, 0.05
+    h = rng.normal(size=(n, n)) * 0.2
+    h = 0.5 * (h + h.T) + numpy.diag(numpy.linspace(-0.6, 0.6, n))
+    eri = fs.random_antisym(n, rng, hermitian=True) * 0.1
+    en = numpy.diag(h).copy() + 0.03
+    a = ArraySystem(T, mu, h, eri, en)
+    b = oracle.GenericSystem(T, mu, h, eri, en)
+    assert numpy.array_equal(a.g_energies_tot(), b.g_energies_tot())
+    assert numpy.abs(a.g_fock_tot() - b.g_fock_tot()).max() < 1e-15
+    assert abs(a.get_mp1() - b.get_mp1()) < 1e-15
+    assert numpy.abs(a.g_mp1_den() - b.g_mp1_den()).max() < 1e-14
+    assert numpy.abs(a.g_fock_d_den() - b.g_fock_d_den()).max() < 1e-15
+    beta, d = 1.0 / T, 1e-6
+    den, fden = a.g_mp1_den(), a.g_fock_d_den()
+    for k in range(n):
+        out = []
+        for s in (d, -d):
+            x = beta * (en - mu)
+            x[k] += s
+            sysx = ArraySystem(T, mu, h, eri, en)
+            fo = fu.ff(1.0, x, 0.0)
+            sysx._fo = lambda fo=fo: fo
+            out.append((sysx.get_mp1(), sysx.g_fock_tot()))
+        (e1p, fp), (e1m, fm) = out
+        assert abs(beta * (e1p - e1m) / (2 * d) - den[k]) < 1e-7
+        assert numpy.abs(-(fp - fm) / (2 * d) - fden[:, :, k]).max() < 1e-8
