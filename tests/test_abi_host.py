@@ -444,3 +444,38 @@ def test_array_system_matches_oracle_and_its_derivatives():
         (e1p, fp), (e1m, fm) = out
         assert abs(beta * (e1p - e1m) / (2 * d) - den[k]) < 1e-7
         assert numpy.abs(-(fp - fm) / (2 * d) - fden[:, :, k]).max() < 1e-8
+
+
+@pytest.mark.parametrize("kind", ["ueg", "pueg"])
+def test_ueg_derivative_densities(kind):
+    """g_mp1_den and g_fock_d_den of the UEG providers are beta dE1/dx_k
+    and -dF/dx_k with x_k = beta (e_k - mu) taken through the occupations
+    (kelvin/ueg_system.py:204-215,498-510; kelvin/pueg_system.py:95-103,
+    178-195), by central differences."""
+    from kelvin_amd import ft_utils as fu
+    from kelvin_amd.ueg_system import PUEGSystem, UEGSystem
+    T, mu = 0.1, 0.1
+    if kind == "ueg":
+        def make():
+            return UEGSystem(T, 2 * numpy.pi, 1.2, mu=mu, norb=7,
+                             orbtype='g')
+    else:
+        def make():
+            return PUEGSystem(T, 2 * numpy.pi, 1.2, mu=mu, norb=7)
+    a = make()
+    en = a.g_energies_tot()
+    beta, d = 1.0 / T, 1e-6
+    den, fden = a.g_mp1_den(), a.g_fock_d_den()
+    assert numpy.abs(den).max() > 1e-4
+    for k in range(en.size):
+        out = []
+        for s in (d, -d):
+            x = beta * (en - mu)
+            x[k] += s
+            sysx = make()
+            fo = fu.ff(1.0, x, 0.0)
+            sysx._fo = lambda fo=fo: fo
+            out.append((sysx.get_mp1(), sysx.g_fock_tot()))
+        (e1p, fp), (e1m, fm) = out
+        assert abs(beta * (e1p - e1m) / (2 * d) - den[k]) < 1e-7
+        assert numpy.abs(-(fp - fm) / (2 * d) - fden[:, :, k]).max() < 1e-8
